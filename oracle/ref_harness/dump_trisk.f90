@@ -1,0 +1,376 @@
+! Golden-vector harness for the WAVETRISK reference (test infrastructure only).
+!
+! Drives the reference's own modules (compiled from /root/reference/src by
+! oracle/Makefile.ref) through the initialisation sequence of main.f90
+! (init_basic main.f90:23-34, init_structures main.f90:482-524) without the
+! driver's checkpoint/restart, sets a uniform single-level grid with every
+! node and edge active, applies an analytic initial state, and dumps
+!   * every per-domain array (nodes incl. ghost cells) with coordinates,
+!   * geometry (len, pedlen, areas, triarea, coriolis, topography),
+!   * the state after the first ghost fill and trend_ml's output
+!     (multi_level.f90:5-47) with its work arrays,
+!   * the state after each of nsteps RK4 steps (time_integr.f90:78-103).
+!
+! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out.
+! Output: one stream file of tagged records (tag*16, kind int32 (1=i32,
+! 2=f64), count int64, data), read by tests/golden/make_golden.py.
+module harness_hooks
+  use domain_mod
+  implicit none
+contains
+  function no_scalar_flux (q, dom, id, idE, idNE, idN, v, zlev, type)
+    real(8), dimension(1:EDGE)                           :: no_scalar_flux
+    type(Float_Field), dimension(1:N_VARIABLE,1:zlevels) :: q
+    type(domain)                                         :: dom
+    integer                                              :: d, id, idE, idNE, idN, v, zlev
+    logical, optional                                    :: type
+    no_scalar_flux = 0d0
+  end function no_scalar_flux
+
+  function no_velo_source (dom, i, j, zlev, offs, dims)
+    real(8), dimension(1:EDGE)     :: no_velo_source
+    type(domain)                   :: dom
+    integer                        :: i, j, zlev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    no_velo_source = 0d0
+  end function no_velo_source
+end module harness_hooks
+
+module harness_ic
+  use domain_mod
+  use geom_mod
+  implicit none
+  character(16) :: ic_case = 'tc5'
+  real(8)       :: u0, h0, b0, rmount, lonc, latc
+contains
+  ! Williamson TC5-like zonal flow: u = u0 cos(lat)
+  subroutine zonal_vel (lon, lat, u, v)
+    real(8) :: lon, lat, u, v
+    u = u0 * cos (lat)
+    if (ic_case == 'ident') then
+       ! non-symmetric field: every edge and node gets a distinct value
+       u = u0 * (sin (7d0*lon + 3d0*lat) + 0.37d0*cos (11d0*lat - 2d0*lon))
+       v = u0 * (cos (5d0*lon - 9d0*lat) + 0.29d0*sin (13d0*lon))
+       return
+    end if
+    if (ic_case == 'ocean') u = u0 * cos (lat) * (1d0 + 0.3d0*sin (3d0*lon)*cos (lat))
+    v = 0d0
+    if (ic_case == 'ocean') v = 0.2d0 * u0 * sin (2d0*lon) * cos (lat)**2
+  end subroutine zonal_vel
+
+  real(8) function bottom (p)
+    type(Coord) :: p
+    real(8)     :: lon, lat, r
+    call cart2sph (p, lon, lat)
+    if (ic_case == 'tc5') then
+       r = min (rmount, sqrt ((lon - lonc)**2 + (lat - latc)**2))
+       bottom = b0 * (1d0 - r/rmount)
+    elseif (ic_case == 'ident') then
+       bottom = 100d0 * sin (3d0*lon + 5d0*lat) + 37d0 * cos (7d0*lon - 4d0*lat)
+    else
+       bottom = -4000d0 + 500d0 * sin (2d0*lon) * cos (lat)**2 + 300d0 * sin (3d0*lat)
+    end if
+  end function bottom
+
+  real(8) function surface (p)
+    type(Coord) :: p
+    real(8)     :: lon, lat
+    call cart2sph (p, lon, lat)
+    if (ic_case == 'tc5') then
+       surface = h0 - (radius*omega*u0 + 0.5d0*u0**2) * sin (lat)**2 / grav_accel
+    elseif (ic_case == 'ident') then
+       surface = h0 + 50d0 * sin (9d0*lon - 2d0*lat) + 20d0 * cos (4d0*lon + 6d0*lat)
+    else
+       surface = 0.5d0 * exp (-((lon - 0.5d0)**2 + (lat - 0.3d0)**2) / 0.04d0) &
+            - (radius*omega*u0 + 0.5d0*u0**2) * sin (lat)**2 / grav_accel
+    end if
+  end function surface
+
+  ! layer density (k=1 bottom), stably stratified for the ocean case
+  real(8) function layer_density (k)
+    integer :: k
+    if (zlevels == 1) then
+       layer_density = ref_density
+    else
+       layer_density = 1028d0 - 4d0 * dble(k-1) / dble(zlevels-1)
+    end if
+  end function layer_density
+
+  subroutine init_sol (dom, i, j, zlev, offs, dims)
+    type(Domain)                   :: dom
+    integer                        :: i, j, zlev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+
+    integer :: d, id, idN, idE, idNE, k
+    real(8) :: b, s, dz, rho
+
+    d    = dom%id + 1
+    id   = idx (i,   j,   offs, dims)
+    idN  = idx (i,   j+1, offs, dims)
+    idE  = idx (i+1, j,   offs, dims)
+    idNE = idx (i+1, j+1, offs, dims)
+
+    b = bottom (dom%node%elts(id+1))
+    s = surface (dom%node%elts(id+1))
+    topography%data(d)%elts(id+1) = b
+    do k = 1, zlevels
+       dz  = (s - b) / dble(zlevels) * (1d0 + 0.1d0*dble(k-1)/dble(max(zlevels-1,1)))
+       rho = layer_density (k)
+       sol(S_MASS,k)%data(d)%elts(id+1) = ref_density * dz
+       sol(S_TEMP,k)%data(d)%elts(id+1) = ref_density * dz * (1d0 - rho/ref_density)
+       sol_mean(S_MASS,k)%data(d)%elts(id+1) = 0d0
+       sol_mean(S_TEMP,k)%data(d)%elts(id+1) = 0d0
+       sol(S_VELO,k)%data(d)%elts(EDGE*id+RT+1) = &
+            proj_vel (zonal_vel, dom%node%elts(id+1),   dom%node%elts(idE+1)) * (1d0 - 0.05d0*dble(k-1))
+       sol(S_VELO,k)%data(d)%elts(EDGE*id+DG+1) = &
+            proj_vel (zonal_vel, dom%node%elts(idNE+1), dom%node%elts(id+1))  * (1d0 - 0.05d0*dble(k-1))
+       sol(S_VELO,k)%data(d)%elts(EDGE*id+UP+1) = &
+            proj_vel (zonal_vel, dom%node%elts(id+1),   dom%node%elts(idN+1)) * (1d0 - 0.05d0*dble(k-1))
+       sol_mean(S_VELO,k)%data(d)%elts(EDGE*id+RT+1:EDGE*id+UP+1) = 0d0
+    end do
+  end subroutine init_sol
+end module harness_ic
+
+program dump_trisk
+  use time_integr_mod
+  use harness_hooks
+  use harness_ic
+  implicit none
+
+  character(16)  :: case
+  character(255) :: out
+  integer        :: zlev, nsteps, d, k, l, n, u, v
+  real(8)        :: step
+  logical        :: nanfill
+  namelist /run/ case, zlev, nsteps, step, out, nanfill
+
+  case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.
+  read (5, nml=run)
+  ic_case = case
+
+  ! --- parameters (shared.f90:259-442 defaults, overridden as a test case would) ---
+  call init_shared_mod
+  zlevels       = zlev
+  max_level     = MIN_LEVEL
+  level_start   = MIN_LEVEL
+  level_end     = MIN_LEVEL
+  compressible  = .false.
+  mode_split    = .false.
+  optimize_grid = NO_OPTIM
+  Laplace_sclr  = 0; Laplace_divu = 0; Laplace_rotu = 0
+  radius        = 6.371d6
+  omega         = 7.292d-5
+  grav_accel    = 9.80616d0
+  ref_density   = 1028d0
+  tol           = 0d0
+  u0 = 20d0; h0 = 5960d0; b0 = 2000d0; rmount = MATH_PI/9d0
+  lonc = -MATH_PI/2d0; latc = MATH_PI/6d0
+  if (ic_case == 'ocean') u0 = 0.5d0
+
+  allocate (a_vert(0:zlevels), b_vert(0:zlevels), a_vert_mass(1:zlevels), b_vert_mass(1:zlevels))
+  do k = 0, zlevels
+     a_vert(k) = 0d0; b_vert(k) = dble(k)/dble(zlevels)
+  end do
+  do k = 1, zlevels
+     a_vert_mass(k) = 0d0; b_vert_mass(k) = b_vert(k) - b_vert(k-1)
+  end do
+
+  physics_scalar_flux => no_scalar_flux
+  physics_velo_source => no_velo_source
+
+  ! --- init_basic (main.f90:23-34), minus io ---
+  call init_comm_mod
+  call init_init_mod
+  call init_refine_patch_mod
+  call init_time_integr_mod
+  call init_wavelet_mod
+  call init_mask_mod
+  call init_adapt_mod
+
+  ! --- init_structures (main.f90:482-524) ---
+  level_start = min_level
+  level_end   = level_start
+  call distribute_grid (cp_idx, 'harness')
+  call init_grid
+  call init_comm_mpi
+  call init_geometry
+  call comm_nodes3_mpi (get_coord, set_coord, NONE)
+  call precompute_geometry
+  call init_wavelets
+  call init_masks
+  call add_second_level
+  call apply_onescale2 (set_level, level_start, z_null, -BDRY_THICKNESS, +BDRY_THICKNESS)
+  call apply_interscale (mask_adj_child, level_start-1, z_null, 0, 1)
+  call init_RK_mem
+
+  ! uniform non-adaptive grid: every node and edge active
+  do d = 1, size(grid)
+     grid(d)%mask_n%elts = TOLRNZ
+     grid(d)%mask_e%elts = TOLRNZ
+     do k = 1, zmax
+        penal_node(k)%data(d)%elts = 0d0
+        penal_edge(k)%data(d)%elts = 0d0
+     end do
+  end do
+
+  do l = level_start, level_end
+     call apply_onescale (init_sol, l, z_null, -1, 1)
+  end do
+
+  ! optional: poison every cell that is neither inside a patch of level_start
+  ! nor a pole, so that ghost values the ghost fill does not provide show up
+  ! as NaN wherever a stencil reads them
+  if (nanfill) call poison_ghosts
+
+  open (newunit=u, file=trim(out), access='stream', form='unformatted', status='replace')
+  call wr_i ('min_level', (/ MIN_LEVEL, DOMAIN_LEVEL, PATCH_SIZE, zlevels, size(grid) /))
+  call wr_f ('params', (/ radius, omega, grav_accel, ref_density, step /))
+  call dump_grid
+  call update_bdry (sol(:,1:zlevels), NONE, 1)
+  call dump_state ('s0_')
+  call trend_ml (sol(:,1:zlevels), trend(:,1:zlevels))
+  call dump_trend ('t0_')
+  call dump_work
+  do n = 1, nsteps
+     call RK4 (sol(:,1:zlevels), wav_coeff(:,1:zlevels), trend_ml, step)
+     call update_bdry (sol(:,1:zlevels), NONE, 1)
+     call dump_state ('s'//achar(48+n/10)//achar(48+mod(n,10))//'_')
+  end do
+  close (u)
+contains
+  subroutine wr_i (tag, a)
+    character(*)          :: tag
+    integer, dimension(:) :: a
+    character(16)         :: t
+    t = tag
+    write (u) t, 1_4, int(size(a),8), a
+  end subroutine wr_i
+
+  subroutine wr_f (tag, a)
+    character(*)          :: tag
+    real(8), dimension(:) :: a
+    character(16)         :: t
+    t = tag
+    write (u) t, 2_4, int(size(a),8), a
+  end subroutine wr_f
+
+  subroutine dump_grid
+    integer :: dd, j, p
+    integer, allocatable :: starts(:)
+    call wr_i ('nelts', (/ (grid(dd)%node%length, dd = 1, size(grid)) /))
+    do dd = 1, size(grid)
+       allocate (starts(grid(dd)%lev(level_start)%length))
+       do j = 1, grid(dd)%lev(level_start)%length
+          p = grid(dd)%lev(level_start)%elts(j)
+          starts(j) = grid(dd)%patch%elts(p+1)%elts_start
+       end do
+       call wr_i ('patch_starts', starts)
+       deallocate (starts)
+       call dump_patch_tables (dd)
+       call wr_f ('node_x', grid(dd)%node%elts(1:grid(dd)%node%length)%x)
+       call wr_f ('node_y', grid(dd)%node%elts(1:grid(dd)%node%length)%y)
+       call wr_f ('node_z', grid(dd)%node%elts(1:grid(dd)%node%length)%z)
+       call wr_f ('len', grid(dd)%len%elts(1:EDGE*grid(dd)%node%length))
+       call wr_f ('pedlen', grid(dd)%pedlen%elts(1:EDGE*grid(dd)%node%length))
+       do j = 1, 6
+          call wr_f ('area_part', grid(dd)%areas%elts(1:grid(dd)%node%length)%part(j))
+       end do
+       call wr_f ('hex_inv', grid(dd)%areas%elts(1:grid(dd)%node%length)%hex_inv)
+       call wr_f ('triarea', grid(dd)%triarea%elts(1:TRIAG*grid(dd)%node%length))
+       call wr_f ('coriolis', grid(dd)%coriolis%elts(1:TRIAG*grid(dd)%node%length))
+       call wr_f ('topo', topography%data(dd)%elts(1:grid(dd)%node%length))
+       call wr_i ('pole_master', (/ merge(1,0,grid(dd)%pole_master(1)), merge(1,0,grid(dd)%pole_master(2)) /))
+    end do
+  end subroutine dump_grid
+
+  subroutine poison_ghosts
+    integer :: dd, j, p, e, kk, s0
+    logical, allocatable :: own(:)
+    real(8) :: nan
+    nan = 1d20
+    do dd = 1, size(grid)
+       allocate (own(grid(dd)%node%length)); own = .false.
+       do j = 1, grid(dd)%lev(level_start)%length
+          p = grid(dd)%lev(level_start)%elts(j)
+          s0 = grid(dd)%patch%elts(p+1)%elts_start
+          own(s0+1:s0+PATCH_SIZE**2) = .true.
+       end do
+       do e = 1, grid(dd)%node%length
+          if (abs(abs(grid(dd)%node%elts(e)%z) - radius) < 1d-6*radius) own(e) = .true.
+       end do
+       do kk = 1, zlevels
+          do e = 1, grid(dd)%node%length
+             if (own(e)) cycle
+             sol(S_MASS,kk)%data(dd)%elts(e) = nan
+             sol(S_TEMP,kk)%data(dd)%elts(e) = nan
+             sol(S_VELO,kk)%data(dd)%elts(EDGE*(e-1)+1:EDGE*e) = nan
+          end do
+       end do
+       deallocate (own)
+    end do
+  end subroutine poison_ghosts
+
+  subroutine dump_patch_tables (dd)
+    ! For each child c of the root patch (main.f90 add_second_level): the
+    ! child index and idx() over (i,j) in [-BDRY_THICKNESS, PATCH_SIZE+BDRY_THICKNESS)
+    integer :: dd, c, p, i, j, m
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    integer, allocatable :: tab(:)
+    m = PATCH_SIZE + 2*BDRY_THICKNESS
+    allocate (tab(m*m + 2))
+    do c = 1, N_CHDRN
+       p = grid(dd)%patch%elts(2)%children(c)
+       call get_offs_Domain (grid(dd), p, offs, dims)
+       tab(1) = c; tab(2) = p
+       do j = -BDRY_THICKNESS, PATCH_SIZE+BDRY_THICKNESS-1
+          do i = -BDRY_THICKNESS, PATCH_SIZE+BDRY_THICKNESS-1
+             tab(3 + (j+BDRY_THICKNESS)*m + (i+BDRY_THICKNESS)) = idx (i, j, offs, dims)
+          end do
+       end do
+       call wr_i ('patch_idx', tab)
+    end do
+    deallocate (tab)
+  end subroutine dump_patch_tables
+
+  subroutine dump_state (pre)
+    character(*) :: pre
+    integer :: dd, kk
+    do kk = 1, zlevels
+       do dd = 1, size(grid)
+          call wr_f (pre//'mass', sol(S_MASS,kk)%data(dd)%elts)
+          call wr_f (pre//'temp', sol(S_TEMP,kk)%data(dd)%elts)
+          call wr_f (pre//'velo', sol(S_VELO,kk)%data(dd)%elts)
+       end do
+    end do
+  end subroutine dump_state
+
+  subroutine dump_trend (pre)
+    character(*) :: pre
+    integer :: dd, kk
+    do kk = 1, zlevels
+       do dd = 1, size(grid)
+          call wr_f (pre//'dmass', trend(S_MASS,kk)%data(dd)%elts)
+          call wr_f (pre//'dtemp', trend(S_TEMP,kk)%data(dd)%elts)
+          call wr_f (pre//'dvelo', trend(S_VELO,kk)%data(dd)%elts)
+       end do
+    end do
+  end subroutine dump_trend
+
+  subroutine dump_work
+    ! work arrays of the last layer evaluated by trend_ml
+    integer :: dd
+    do dd = 1, size(grid)
+       call wr_f ('w_qe', grid(dd)%qe%elts)
+       call wr_f ('w_ke', grid(dd)%ke%elts)
+       call wr_f ('w_bern', grid(dd)%bernoulli%elts)
+       call wr_f ('w_exner', exner_fun(zlevels)%data(dd)%elts)
+       call wr_f ('w_flux_m', horiz_flux(S_MASS)%data(dd)%elts)
+       call wr_f ('w_flux_t', horiz_flux(S_TEMP)%data(dd)%elts)
+       call wr_f ('w_press', grid(dd)%press%elts)
+       call wr_f ('w_geopot', grid(dd)%geopot%elts)
+    end do
+  end subroutine dump_work
+end program dump_trisk

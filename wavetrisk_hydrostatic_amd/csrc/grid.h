@@ -1,0 +1,44 @@
+// Lozenge-tile grid of one uniform icosahedral level (see grid.cpp).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace wtg {
+
+constexpr int HALO = 2;      // halo cells on every side of a lozenge tile
+constexpr int NLOZ = 10;     // lozenges of the icosahedron
+
+struct Level {
+  int level = 0, n = 0, S = 0;
+  double radius = 0, omega = 0;
+  size_t south_pole = 0, north_pole = 0;  // master cells of the poles
+
+  size_t ncells() const { return (size_t)NLOZ * S * S; }
+  size_t cell(int lz, int i, int j) const {
+    return (size_t)lz * S * S + (size_t)(j + HALO) * S + (size_t)(i + HALO);
+  }
+
+  // node_src[c]: owned cell whose value halo cell c copies (c itself for owned
+  // cells and pole masters, -1 for cells no stencil reads)
+  std::vector<int64_t> node_src;
+  // edge ghost map, per edge slot 3*c+e (e = RT, DG, UP): source slot and sign
+  std::vector<int64_t> edge_src;
+  std::vector<int8_t> edge_sign;
+
+  std::vector<double> X;         // coordinates, planes x | y | z, [3*ncells]
+  std::vector<double> len;       // primal edge length, planes RT | DG | UP
+  std::vector<double> pedlen;    // dual edge length, planes RT | DG | UP
+  std::vector<double> area;      // hexagon part areas 1..6 (6 planes)
+  std::vector<double> hex_inv;   // 1 / hexagon area
+  std::vector<double> triarea;   // triangle area, planes LORT | UPLT
+  std::vector<double> coriolis;  // f * dual area, planes LORT | UPLT
+};
+
+// min_level, domain_level: the reference's compile-time MIN_LEVEL and
+// DOMAIN_LEVEL (src/param_J*.f90).  They fix how node coordinates are rounded
+// (sub-domain corners first) and, through PATCH_SIZE = 2^(MIN_LEVEL -
+// DOMAIN_LEVEL - 1), one ghost-list convention beside the south pole.
+Level build_level(int level, int min_level, int domain_level, double radius, double omega);
+
+}  // namespace wtg
