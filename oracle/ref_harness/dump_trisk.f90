@@ -50,8 +50,10 @@ contains
     u = u0 * cos (lat)
     if (ic_case == 'ident') then
        ! non-symmetric field: every edge and node gets a distinct value
-       u = u0 * (sin (7d0*lon + 3d0*lat) + 0.37d0*cos (11d0*lat - 2d0*lon))
-       v = u0 * (cos (5d0*lon - 9d0*lat) + 0.29d0*sin (13d0*lon))
+       ! single-valued at the poles (cos(lat) factor): the reference keeps one
+       ! copy of each pole per lozenge and never reconciles them
+       u = u0 * cos (lat) * (sin (7d0*lon + 3d0*lat) + 0.37d0*cos (11d0*lat - 2d0*lon))
+       v = u0 * cos (lat) * (cos (5d0*lon - 9d0*lat) + 0.29d0*sin (13d0*lon))
        return
     end if
     if (ic_case == 'ocean') u = u0 * cos (lat) * (1d0 + 0.3d0*sin (3d0*lon)*cos (lat))
@@ -67,7 +69,7 @@ contains
        r = min (rmount, sqrt ((lon - lonc)**2 + (lat - latc)**2))
        bottom = b0 * (1d0 - r/rmount)
     elseif (ic_case == 'ident') then
-       bottom = 100d0 * sin (3d0*lon + 5d0*lat) + 37d0 * cos (7d0*lon - 4d0*lat)
+       bottom = cos (lat) * (100d0 * sin (3d0*lon + 5d0*lat) + 37d0 * cos (7d0*lon - 4d0*lat))
     else
        bottom = -4000d0 + 500d0 * sin (2d0*lon) * cos (lat)**2 + 300d0 * sin (3d0*lat)
     end if
@@ -80,7 +82,7 @@ contains
     if (ic_case == 'tc5') then
        surface = h0 - (radius*omega*u0 + 0.5d0*u0**2) * sin (lat)**2 / grav_accel
     elseif (ic_case == 'ident') then
-       surface = h0 + 50d0 * sin (9d0*lon - 2d0*lat) + 20d0 * cos (4d0*lon + 6d0*lat)
+       surface = h0 + cos (lat) * (50d0 * sin (9d0*lon - 2d0*lat) + 20d0 * cos (4d0*lon + 6d0*lat))
     else
        surface = 0.5d0 * exp (-((lon - 0.5d0)**2 + (lat - 0.3d0)**2) / 0.04d0) &
             - (radius*omega*u0 + 0.5d0*u0**2) * sin (lat)**2 / grav_accel

@@ -403,6 +403,7 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
   for (int lz = 0; lz < 10; ++lz)
     for (int j = -HALO; j < n + HALO; ++j)
       for (int i = -HALO; i < n + HALO; ++i) g.node_src[cell(lz, i, j)] = esrc[ecell(lz, i, j)];
+  const std::vector<int64_t> geo_src = esrc;  // coordinates of every cell, frozen or not
 
   // --- 5. edge ghost map -------------------------------------------------------
   // A halo edge joins two halo/owned nodes; its source is the owned edge with
@@ -498,6 +499,46 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
       g.edge_sign[slot] = 1;
     }
 
+  // Halo cells and edges the reference's ghost lists never fill (found by
+  // poisoning every ghost value before update_bdry).  They keep whatever
+  // value they were initialised with for the whole run: the ghost fill skips
+  // them and a time step leaves them unchanged (their trend is zero), so they
+  // are "frozen" here too (source -1).  Some are read with non-zero weight
+  // beside the icosahedron vertices, so callers must initialise them exactly
+  // as the reference's initial state does.
+  {
+    const int RT = 0, DG = 1, UP = 2, ALL = 3;
+    struct Z { int i, j, e; };
+    const Z south[] = {{n, n + 1, DG}, {n, n + 1, UP}, {n, -1, ALL}, {n + 1, n + 1, UP}, {n + 1, -1, ALL},
+                       {n + 1, 0, DG}, {n + 1, 0, RT}, {n + 1, 1, DG}, {n + 1, 1, RT}, {-1, n, ALL},
+                       {-1, n + 1, ALL}, {-1, -1, DG}, {-1, -1, UP}};
+    const Z north[] = {{n, -1, ALL}, {n + 1, n, ALL}, {n + 1, n + 1, RT}, {n + 1, -1, ALL}, {-1, n, ALL},
+                       {-1, n + 1, ALL}, {-1, -1, DG}, {0, n + 1, DG}, {0, n + 1, UP}, {1, n + 1, DG},
+                       {1, n + 1, UP}};
+    const int south_nodes[][2] = {{-1, n + 1}, {-1, -1}};
+    const int north_nodes[][2] = {{n + 1, -1}, {-1, -1}};
+    for (int lz = 0; lz < 10; ++lz) {
+      auto drop = [&](int i, int j, int e) {
+        for (int k = 0; k < 3; ++k) {
+          if (e != ALL && e != k) continue;
+          size_t slot = 3 * cell(lz, i, j) + k;
+          g.edge_src[slot] = -1;
+          g.edge_sign[slot] = 0;
+        }
+      };
+      if (lz < 5) {
+        for (auto& t : south) drop(t.i, t.j, t.e);
+        for (auto& t : south_nodes) esrc[ecell(lz, t[0], t[1])] = -1;
+      } else {
+        for (auto& t : north) drop(t.i, t.j, t.e);
+        for (auto& t : north_nodes) esrc[ecell(lz, t[0], t[1])] = -1;
+      }
+    }
+    for (int lz = 0; lz < 10; ++lz)
+      for (int j = -HALO; j < n + HALO; ++j)
+        for (int i = -HALO; i < n + HALO; ++i) g.node_src[cell(lz, i, j)] = esrc[ecell(lz, i, j)];
+  }
+
   // --- 6. geometry on the virtual frame, in each lozenge's own frame ---------
   // Coordinates of halo cells are the owners' (comm_nodes3 after refinement,
   // refine_patch.f90:154-175), except the two artificial corner cells, which
@@ -509,6 +550,7 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
     for (int j = -EH; j < n + EH; ++j)
       for (int i = -EH; i < n + EH; ++i) {
         int64_t s0 = esrc[ecell(lz, i, j)];
+        if (s0 < 0) s0 = geo_src[ecell(lz, i, j)];
         if (s0 >= 0) XE[ecell(lz, i, j)] = X[s0];
       }
   for (int lz = 0; lz < 10; ++lz) {
@@ -612,7 +654,6 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
     for (int j = -1; j <= n + 1; ++j)
       for (int i = -1; i <= n + 1; ++i) {
         if (i >= 0 && i < n && j >= 0 && j < n) continue;
-        if ((i == -1 && j == -1) || (i == n + 1 && j == n + 1)) continue;
         size_t c = ecell(lz, i, j);
         int64_t s0 = esrc[c];
         if (s0 < 0 || s0 == (int64_t)cell(lz, i, j)) continue;
