@@ -1,0 +1,144 @@
+"""Benchmark of the TRiSK hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--level 8] [--layers 30]
+
+Workload (default): a uniform J8 icosahedral grid (655 362 nodes, 1 966 080
+edges), 30 Boussinesq layers, synthetic stratified ocean state, FP64; one
+step = one low-storage RK4 step of trend_ml (4 fused trend+update kernel
+launches + 4 ghost fills).  DOF per step = layers * (2 * nodes + edges).
+Inputs are resident in HBM before the timed region; W untimed warm-up steps,
+then exactly K timed steps between two stream synchronisations.
+
+Prints ONE JSON line.  `roofline` is the dominant kernel (the fused stage
+kernel): algorithmic HBM bytes per launch / its mean launch time, both
+measured here with HIP events on the kernel's stream.  `cpu_baseline` times
+the CPU oracle (a one-core C port of the same algorithm) on a bounded sample
+of the same workload (fewer layers, one step) on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "DOF updates/s J8 L30; free-surface solve ms @1e-9 rel. resid, learned vs MG"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_launch(layout, L: int) -> float:
+    """Minimum HBM bytes of one fused stage launch, averaged over the four
+    RK4 stages: per node-layer, read the stage state y (5 words), read the
+    base state q (5 words; stage 1 has y = q), write the new state (5 words);
+    per node, read the geometry once (len 3, pedlen 3, flux pedlen 3, area 6,
+    hex_inv 1, coriolis 2, topography 1 = 19 words)."""
+    nodes = 10 * layout.n * layout.n
+    words_state = (10 + 15 + 15 + 15) / 4.0
+    return 8.0 * (nodes * L * words_state + nodes * 19)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--layers", type=int, default=30)
+    ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--cpu-layers", type=int, default=30, help="layers of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding of the lozenge domains is a design note, not a code path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+
+    J, L = args.level, args.layers
+    print(f"bench: building J{J} grid", file=sys.stderr, flush=True)
+    grid = W.Grid(level=J, min_level=J, domain_level=2)
+    lay = grid.layout
+    topo, mass, temp, velo = cases.ocean(grid, L, seed=2026)
+    ctx = W.Context(grid, nlayers=L)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, mass, temp, velo)
+    print(f"bench: warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    if args.warmup:
+        ctx.rk4(0, args.dt, args.warmup)
+    ctx.synchronize()
+    total_ms, kernel_ms = ctx.rk4_timed(0, args.dt, args.steps)
+    ctx.synchronize()
+    m_out, _, u_out = ctx.download_state(0)
+    own = lay.owned_cells()
+    if not (np.isfinite(m_out[:, own]).all() and np.isfinite(u_out[:, :, lay.owned_edge_cells()]).all()):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    dof_per_step = L * lay.n_dof_per_layer
+    ms_per_step = total_ms / args.steps
+    value = dof_per_step * args.steps / (total_ms / 1e3)
+    alg_bytes = algorithmic_bytes_per_launch(lay, L)
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        print(f"bench: CPU baseline (oracle, J{J} L{args.cpu_layers}, 1 RK4 step)", file=sys.stderr, flush=True)
+        from oracle.oracle import Oracle, geometry_of
+
+        Lc = args.cpu_layers
+        t_c, m_c, tt_c, u_c = cases.ocean(grid, Lc, seed=2026)
+        o = Oracle(lay, geometry_of(grid), t_c)
+        m_c, tt_c, u_c = m_c.copy(), tt_c.copy(), u_c.copy()
+        o.ghost_fill(m_c, tt_c, u_c)
+        t0 = time.perf_counter()
+        o.rk4(m_c, tt_c, u_c, args.dt, 1)
+        secs = time.perf_counter() - t0
+        cpu = {"value": Lc * lay.n_dof_per_layer / secs, "unit": "DOF updates/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (C, -O2, 1 thread), 1 RK4 step of the same J{J} grid with {Lc} layers "
+                         f"({secs:.1f} s)"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded stratified ocean, analytic fields on the grid)",
+        "config": {
+            "workload": f"J{J} uniform icosahedral grid, L{L} Boussinesq layers, explicit RK4 of trend_ml "
+                        f"(TRiSK), FP64, dt={args.dt}s; no free-surface solve in this path",
+            "level": J, "layers": L, "nodes": 10 * lay.n * lay.n + 2, "edges": 30 * lay.n * lay.n,
+            "dof_per_step": dof_per_step,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "trisk_stage_kernel",
+            "kernel_ms": kernel_ms,
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

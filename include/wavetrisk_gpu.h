@@ -1,0 +1,133 @@
+/* wavetrisk_gpu.h -- C ABI of the MI355X dynamical-core hot path.
+ *
+ * Drop-in boundary for the procedures WAVETRISK's time integrator calls:
+ *   trend_fun => trend_ml        (time_integr.f90:7-13, multi_level.f90:5-47)
+ *   dt_step   => RK4             (time_integr.f90:15-31, 78-103; main.f90:411)
+ *   update_bdry                  (comm_ser.f90:113-152 -> comm.f90:785-808)
+ * for a uniform level of the icosahedral grid.  A Fortran caller binds these
+ * with ISO_C_BINDING; all arguments are plain pointers,
+ * sizes and scalars.  Every function returns 0 on success and a negative
+ * WTG_E_* code on failure (the Fortran shim calls abort on non-zero, as the
+ * reference's `call abort` does); wtg_last_error() gives the message.
+ *
+ * Data layout (host arrays passed in and out).  The sphere is 10 lozenges
+ * of n x n nodes, n = 2^level, each stored as a tile of S x S cells,
+ * S = n + 2*halo, halo = 2; cell (lz, i, j) = lz*S*S + (j+halo)*S + (i+halo)
+ * for i, j in [-halo, n+halo).  Owned nodes have i, j in [0, n); the south
+ * pole is cell (0, n, 0), the north pole cell (5, 0, n).  A node owns edges
+ * RT (i,j)->(i+1,j), DG (i+1,j+1)->(i,j), UP (i,j)->(i,j+1).
+ *   scalar fields (mass = rho*dz, temp = rho*dz*theta): [nlayers][ncells]
+ *   velocity (edge-normal, m/s):                        [nlayers][3][ncells]
+ * Layer 0 is the bottom layer (k = 1 in the reference).
+ */
+#ifndef WAVETRISK_GPU_H
+#define WAVETRISK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WTG_OK = 0,
+  WTG_E_ARG = -1,      /* bad argument */
+  WTG_E_DEVICE = -2,   /* HIP runtime error (no GPU, out of memory, launch failure) */
+  WTG_E_STATE = -3,    /* call out of order (e.g. topography not set) */
+  WTG_E_INTERNAL = -4
+};
+
+/* Grid: the reference's compile-time parameters (src/param_J*.f90) plus
+ * the level.  min_level and domain_level fix how node coordinates round and
+ * one ghost-list convention beside the south pole, so they must match the
+ * reference build being replaced. */
+typedef struct {
+  int level;         /* J: n = 2^J nodes per lozenge side */
+  int min_level;     /* MIN_LEVEL of the reference build, min_level-1 <= level */
+  int domain_level;  /* DOMAIN_LEVEL of the reference build */
+  double radius;     /* m */
+  double omega;      /* s^-1 */
+} wtg_grid_desc;
+
+typedef struct {
+  int n, S, halo;
+  int64_t ncells;      /* 10*S*S */
+  int64_t south_pole;  /* cell index of the south pole */
+  int64_t north_pole;  /* cell index of the north pole */
+} wtg_grid_info;
+
+/* Dynamics parameters of the Boussinesq (incompressible) layer model. */
+typedef struct {
+  int nlayers;         /* zlevels */
+  double grav;         /* grav_accel, m s^-2 */
+  double ref_density;  /* ref_density, kg m^-3 */
+  double p_top;        /* pressure at the top of the top layer */
+} wtg_params;
+
+/* ---- host-side grid (no GPU needed) ------------------------------------ */
+typedef struct wtg_grid wtg_grid;
+
+/* Builds coordinates, ghost maps and TRiSK geometry of one level; replaces
+ * init_grid / precompute_geometry / add_second_level (init.f90:142-524,
+ * refine_patch.f90:102-186) for a uniform level. */
+int wtg_grid_create(const wtg_grid_desc* desc, wtg_grid** out);
+int wtg_grid_info_get(const wtg_grid* grid, wtg_grid_info* info);
+/* Copies one host array of the grid.  name: "x" [3][ncells] coordinates,
+ * "len", "pedlen" [3][ncells], "area" [6][ncells], "hex_inv" [ncells],
+ * "triarea", "coriolis" [2][ncells] (double); "node_src" [ncells],
+ * "edge_src" [3*ncells] (int64; -1 = frozen halo slot, see below);
+ * "edge_sign" [3*ncells] (int8).  bytes must equal the array's size. */
+int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes);
+void wtg_grid_destroy(wtg_grid* grid);
+
+/* ---- device context ------------------------------------------------------ */
+typedef struct wtg_ctx wtg_ctx;
+
+/* Uploads the grid to the current HIP device and allocates nslots user
+ * state slots (>= 1) plus two internal stage buffers.  One HIP stream per
+ * context; calls are stream ordered and the host functions below
+ * synchronise before returning. */
+int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_ctx** out);
+void wtg_destroy(wtg_ctx* ctx);
+
+/* Bottom height z_s per cell [ncells] (topography, m); halo cells may be
+ * anything, they are filled from their owners. */
+int wtg_set_topography(wtg_ctx* ctx, const double* topo);
+
+/* Host <-> device copies of a state slot (layouts above).  Halo slots whose
+ * ghost map is -1 ("frozen": the reference's ghost lists never refresh them
+ * and a time step never changes them) keep the uploaded values; every other
+ * halo slot is refilled from its owner on upload. */
+int wtg_upload_state(wtg_ctx* ctx, int slot, const double* mass, const double* temp, const double* velo);
+int wtg_download_state(wtg_ctx* ctx, int slot, double* mass, double* temp, double* velo);
+
+/* update_bdry on one slot: halo := owner values (edges with sign). */
+int wtg_ghost_fill(wtg_ctx* ctx, int slot);
+
+/* trend_ml: dq_slot := trend of q_slot (q_slot must be ghost filled).
+ * The trend is written on owned cells/edges and the pole masters. */
+int wtg_trend(wtg_ctx* ctx, int q_slot, int dq_slot);
+
+/* dt_step => RK4: nsteps low-storage RK4 steps of size dt on slot `slot`
+ * (q1 = q + dt/4 f(q); q1 = q + dt/3 f(q1); q1 = q + dt/2 f(q1);
+ * q = q + dt f(q1), each stage followed by a ghost fill). */
+int wtg_rk4(wtg_ctx* ctx, int slot, double dt, int nsteps);
+
+/* Same as wtg_rk4, timed with HIP events on the context's stream:
+ * total_ms for all steps, stage_kernel_ms = mean duration of one launch of
+ * the fused trend+update kernel (the dominant kernel). */
+int wtg_rk4_timed(wtg_ctx* ctx, int slot, double dt, int nsteps, double* total_ms, double* stage_kernel_ms);
+
+/* Device addresses of a slot's arrays (for zero-copy interop). */
+int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
+
+int wtg_synchronize(wtg_ctx* ctx);
+
+/* Last error message of ctx (or of the last failed create when ctx is NULL). */
+int wtg_last_error(const wtg_ctx* ctx, char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVETRISK_GPU_H */

@@ -1,0 +1,95 @@
+"""ctypes binding of the C oracle (oracle/trisk_oracle.c) -- test infrastructure.
+
+The oracle restates trend_ml / RK4 / update_bdry of the reference on the
+lozenge-tile layout; its geometry and ghost maps are inputs (arrays of a
+wavetrisk_hydrostatic_amd.Grid or of a golden fixture).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "libtrisk_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.wto_ghost_fill.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.wto_trend.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        _lib.wto_rk4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    return _lib
+
+
+class _G(C.Structure):
+    _fields_ = [("n", C.c_int), ("S", C.c_int), ("ncells", C.c_int64), ("south_pole", C.c_int64),
+                ("north_pole", C.c_int64), ("len", C.c_void_p), ("pedlen", C.c_void_p), ("area", C.c_void_p),
+                ("hex_inv", C.c_void_p), ("triarea", C.c_void_p), ("coriolis", C.c_void_p), ("topo", C.c_void_p),
+                ("node_src", C.c_void_p), ("edge_src", C.c_void_p), ("edge_sign", C.c_void_p),
+                ("grav", C.c_double), ("rho0", C.c_double), ("p_top", C.c_double)]
+
+
+class Oracle:
+    """Holds the geometry arrays alive and exposes ghost_fill / trend / rk4."""
+
+    def __init__(self, layout, geom: dict, topo: np.ndarray, grav=9.80616, rho0=1028.0, p_top=0.0):
+        self.layout = layout
+        f = lambda a, dt=np.float64: np.ascontiguousarray(a, dtype=dt).ravel()
+        self._keep = {k: f(geom[k]) for k in ("len", "pedlen", "area", "hex_inv", "triarea", "coriolis")}
+        self._keep["node_src"] = f(geom["node_src"], np.int64)
+        self._keep["edge_src"] = f(geom["edge_src"], np.int64)
+        self._keep["edge_sign"] = f(geom["edge_sign"], np.int8)
+        # halo topography = owner's
+        topo = f(topo).copy()
+        src = self._keep["node_src"]
+        m = src >= 0
+        topo[m] = topo[src[m]]
+        self._keep["topo"] = topo
+        P = lambda k: C.c_void_p(self._keep[k].ctypes.data)
+        self.g = _G(layout.n, layout.S, layout.ncells, layout.south_pole, layout.north_pole, P("len"), P("pedlen"),
+                    P("area"), P("hex_inv"), P("triarea"), P("coriolis"), P("topo"), P("node_src"), P("edge_src"),
+                    P("edge_sign"), grav, rho0, p_top)
+        self.lib = load()
+
+    @staticmethod
+    def _state(mass, temp, velo):
+        return (np.ascontiguousarray(mass, dtype=np.float64), np.ascontiguousarray(temp, dtype=np.float64),
+                np.ascontiguousarray(velo, dtype=np.float64))
+
+    def ghost_fill(self, mass, temp, velo):
+        """In place on float64 C-contiguous arrays [L, nc], [L, nc], [L, 3, nc]."""
+        L = mass.shape[0]
+        self.lib.wto_ghost_fill(C.byref(self.g), L, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
+
+    def trend(self, mass, temp, velo):
+        mass, temp, velo = self._state(mass, temp, velo)
+        L = mass.shape[0]
+        dm, dt, du = np.zeros_like(mass), np.zeros_like(temp), np.zeros_like(velo)
+        self.lib.wto_trend(C.byref(self.g), L, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data,
+                           dm.ctypes.data, dt.ctypes.data, du.ctypes.data)
+        return dm, dt, du
+
+    def rk4(self, mass, temp, velo, dt: float, nsteps: int):
+        mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))
+        L = mass.shape[0]
+        self.lib.wto_rk4(C.byref(self.g), L, dt, nsteps, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
+        return mass, temp, velo
+
+
+def geometry_of(grid) -> dict:
+    """Geometry and ghost maps of a wavetrisk_hydrostatic_amd.Grid as oracle inputs."""
+    return {k: grid.array(k) for k in ("len", "pedlen", "area", "hex_inv", "triarea", "coriolis", "node_src",
+                                        "edge_src", "edge_sign")}

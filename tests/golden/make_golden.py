@@ -1,0 +1,270 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Runs the serial WAVETRISK reference built by oracle/Makefile.ref (Fortran
+sources under /root/reference/src, amdflang) through the harness
+oracle/ref_harness/dump_trisk.f90, and converts its per-domain arrays to the
+lozenge-tile layout:
+
+  grid_J<J>.npz      reference geometry on every cell a stencil reads, and the
+                     reference's ghost maps: for each halo cell/edge the owned
+                     cell/edge it copies (found by exact value matching on a
+                     state with distinct values everywhere) or -1 when the
+                     reference never fills it (found by poisoning every ghost
+                     value before its update_bdry)
+  trisk_<case>.npz   initial state (owned cells, pole masters, frozen halo
+                     slots), trend_ml of it, and the state after each RK4 step
+
+Only this container has /root/reference; the fixtures are committed and the
+GPU box reads them.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+OUT = os.path.dirname(os.path.abspath(__file__))
+H = 2
+
+# (param file, MIN_LEVEL, DOMAIN_LEVEL)
+PARAMS = {4: ("param_J4", 4, 1), 5: ("param_J5", 5, 2)}
+CASES = [
+    # name, J, case, layers, steps, dt
+    ("tc5_J4", 4, "tc5", 1, 2, 1200.0),
+    ("ocean_J4", 4, "ocean", 3, 2, 1200.0),
+    ("ident_J4", 4, "ident", 2, 2, 600.0),
+    ("tc5_J5", 5, "tc5", 1, 1, 600.0),
+]
+
+
+def read_records(path):
+    raw = open(path, "rb").read()
+    off, recs = 0, []
+    while off < len(raw):
+        tag = raw[off:off + 16].decode().strip()
+        off += 16
+        kind = int(np.frombuffer(raw, np.int32, 1, off)[0])
+        off += 4
+        n = int(np.frombuffer(raw, np.int64, 1, off)[0])
+        off += 8
+        dt = np.int32 if kind == 1 else np.float64
+        a = np.frombuffer(raw, dt, n, off).copy()
+        off += n * a.itemsize
+        recs.append((tag, a))
+    return recs
+
+
+def load_dump(path):
+    out = {"dom": []}
+    d = None
+    for t, a in read_records(path):
+        if t in ("min_level", "params", "nelts"):
+            out[t] = a
+        elif t == "patch_starts":
+            d = {"patch_idx": [], "area_part": []}
+            out["dom"].append(d)
+        elif t in ("patch_idx", "area_part"):
+            d[t].append(a)
+        elif d is not None and t in ("node_x", "node_y", "node_z", "len", "pedlen", "hex_inv", "triarea",
+                                     "coriolis", "topo", "pole_master"):
+            d[t] = a
+        else:
+            out.setdefault(t, []).append(a)
+    return out
+
+
+def lozenge_cover(o):
+    """For every lozenge cell (lz, i, j), i, j in [-H, n+H): the (domain,
+    element) copies of it held by the reference, nearest patch first."""
+    J, DL, PS, L, ND = (int(v) for v in o["min_level"])
+    n = 2 ** J
+    nspd = 2 ** DL
+    nsub = nspd * nspd
+    N = n // nspd
+    S = n + 2 * H
+    chd = {1: (PS, PS), 2: (PS, 0), 3: (0, 0), 4: (0, PS)}  # child offsets (domain.f90:122)
+    cover = {}
+    m = PS + 2 * H
+    for dd, d in enumerate(o["dom"]):
+        loz, sub = dd // nsub, dd % nsub
+        si, sj = sub % nspd, sub // nspd
+        for tab in d["patch_idx"]:
+            c = int(tab[0])
+            t = tab[2:].reshape(m, m)
+            ci, cj = chd[c]
+            for j in range(-H, PS + H):
+                for i in range(-H, PS + H):
+                    I, Jj = si * N + ci + i, sj * N + cj + j
+                    if -H <= I < n + H and -H <= Jj < n + H:
+                        e = int(t[j + H][i + H])
+                        if e < 0:
+                            continue
+                        dist = max(0, -i, i - (PS - 1)) + max(0, -j, j - (PS - 1))
+                        cover.setdefault((loz, I, Jj), []).append((dist, dd, e))
+    for k in cover:
+        cover[k].sort()
+    return n, S, ND, L, cover
+
+
+def run_ref(param, namelist, path):
+    exe = os.path.join(ROOT, "oracle", "_ref", param, "dump_trisk")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile.ref"), f"PARAM={param}"],
+                       cwd=ROOT, check=True)
+    subprocess.run([exe], input=namelist.replace("OUT", path), text=True, cwd=ROOT, check=True)
+
+
+def cell_index(S, lz, i, j):
+    return lz * S * S + (j + H) * S + (i + H)
+
+
+def grid_fixture(J):
+    param, ml, dl = PARAMS[J]
+    with tempfile.TemporaryDirectory() as tmp:
+        p_id, p_nan = os.path.join(tmp, "id.bin"), os.path.join(tmp, "nan.bin")
+        run_ref(param, "&run case='ident', zlev=1, nsteps=0, step=60.0, out='OUT' /\n", p_id)
+        run_ref(param, "&run case='ident', zlev=1, nsteps=0, step=60.0, nanfill=.true., out='OUT' /\n", p_nan)
+        o, q = load_dump(p_id), load_dump(p_nan)
+    n, S, ND, L, cover = lozenge_cover(o)
+    NC = 10 * S * S
+    mass, velo = o["s0_mass"], o["s0_velo"]
+    pm, pu = q["s0_mass"], q["s0_velo"]
+    first = lambda key: cover[key][0][1:] if key in cover else None
+    # owned values -> owned cell / edge slot
+    nval, evals = {}, {}
+    for lz in range(10):
+        for j in range(n):
+            for i in range(n):
+                dd, e = [x[1:] for x in cover[(lz, i, j)] if x[0] == 0][0]
+                c = cell_index(S, lz, i, j)
+                nval.setdefault(mass[dd][e], []).append(c)
+                for k in range(3):
+                    evals.setdefault(abs(velo[dd][3 * e + k]), []).append((3 * c + k, np.sign(velo[dd][3 * e + k])))
+    node_src = np.full(NC, -2, np.int64)   # -2: not determined
+    edge_src = np.full(3 * NC, -2, np.int64)
+    edge_sign = np.zeros(3 * NC, np.int8)
+    geo = {k: np.full((m_, NC), np.nan) for k, m_ in (("len", 3), ("pedlen", 3), ("area", 6), ("hex_inv", 1),
+                                                       ("triarea", 2), ("coriolis", 2))}
+    for lz in range(10):
+        for j in range(-1, n + 2):
+            for i in range(-1, n + 2):
+                key = (lz, i, j)
+                if key not in cover:
+                    continue
+                dd, e = first(key)
+                d = o["dom"][dd]
+                c = cell_index(S, lz, i, j)
+                # geometry on the cells the TRiSK stencils read
+                inner = -1 <= i <= n and -1 <= j <= n
+                if inner:
+                    geo["pedlen"][:, c] = d["pedlen"][3 * e:3 * e + 3]
+                    geo["triarea"][:, c] = d["triarea"][2 * e:2 * e + 2]
+                    geo["coriolis"][:, c] = d["coriolis"][2 * e:2 * e + 2]
+                    geo["len"][:, c] = d["len"][3 * e:3 * e + 3]
+                    geo["area"][:, c] = [d["area_part"][p][e] for p in range(6)]
+                elif j == n + 1 and i <= n:
+                    geo["len"][0, c] = d["len"][3 * e]
+                    geo["area"][3:6, c] = [d["area_part"][p][e] for p in (3, 4, 5)]
+                elif i == n + 1 and j <= n:
+                    geo["len"][2, c] = d["len"][3 * e + 2]
+                    geo["area"][2:5, c] = [d["area_part"][p][e] for p in (2, 3, 4)]
+                if 0 <= i <= n and 0 <= j <= n:
+                    geo["hex_inv"][0, c] = d["hex_inv"][e]
+                # ghost maps of halo cells
+                if 0 <= i < n and 0 <= j < n:
+                    continue
+                qd, qe = (cover[key][0][1:] if key in cover else (dd, e))
+                if abs(pm[qd][qe]) >= 1e19:
+                    node_src[c] = -1
+                else:
+                    cand = nval.get(mass[dd][e], [])
+                    if len(cand) == 1:
+                        node_src[c] = cand[0]
+                for k in range(3):
+                    if abs(pu[qd][3 * qe + k]) >= 1e19:
+                        edge_src[3 * c + k] = -1
+                        continue
+                    v = velo[dd][3 * e + k]
+                    cand = evals.get(abs(v), [])
+                    if v != 0 and len(cand) == 1:
+                        edge_src[3 * c + k] = cand[0][0]
+                        edge_sign[3 * c + k] = int(np.sign(v) * cand[0][1])
+    np.savez_compressed(os.path.join(OUT, f"grid_J{J}.npz"), level=J, min_level=ml, domain_level=dl,
+                        radius=o["params"][0], omega=o["params"][1], node_src=node_src, edge_src=edge_src,
+                        edge_sign=edge_sign, **{"geo_" + k: v for k, v in geo.items()})
+    print(f"grid_J{J}.npz written")
+
+
+def trisk_fixture(name, J, case, L, nsteps, dt):
+    param, ml, dl = PARAMS[J]
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "d.bin")
+        run_ref(param, f"&run case='{case}', zlev={L}, nsteps={nsteps}, step={dt}, out='OUT' /\n", p)
+        o = load_dump(p)
+    n, S, ND, L_, cover = lozenge_cover(o)
+    assert L_ == L
+    NC = 10 * S * S
+
+    def owned_value(arr, k, lz, i, j, mult, q):
+        hits = [x for x in cover[(lz, i, j)] if x[0] == 0]
+        dd, e = (hits[0] if hits else cover[(lz, i, j)][0])[1:]
+        return arr[k * ND + dd][mult * e + q]
+
+    poles = [(0, n, 0), (5, 0, n)]
+    cells = [(lz, i, j) for lz in range(10) for j in range(n) for i in range(n)]
+
+    def field(tag, mult, with_halo):
+        out = np.full((L, mult, NC), np.nan)
+        arr = o[tag]
+        for k in range(L):
+            for (lz, i, j) in cells + (poles if mult == 1 else []):
+                c = cell_index(S, lz, i, j)
+                for q in range(mult):
+                    out[k, q, c] = owned_value(arr, k, lz, i, j, mult, q)
+            if with_halo:  # every halo copy of the initial state (frozen slots need these)
+                for lz in range(10):
+                    for j in range(-1, n + 2):
+                        for i in range(-1, n + 2):
+                            if (0 <= i < n and 0 <= j < n) or (lz, i, j) in poles or (lz, i, j) not in cover:
+                                continue
+                            dd, e = cover[(lz, i, j)][0][1:]
+                            c = cell_index(S, lz, i, j)
+                            for q in range(mult):
+                                out[k, q, c] = arr[k * ND + dd][mult * e + q]
+        return out[:, 0] if mult == 1 else out
+
+    topo = np.full(NC, np.nan)
+    for (lz, i, j) in cells + poles:
+        hits = [x for x in cover[(lz, i, j)] if x[0] == 0] or cover[(lz, i, j)]
+        dd, e = hits[0][1:]
+        topo[cell_index(S, lz, i, j)] = o["dom"][dd]["topo"][e]
+    data = dict(level=J, min_level=ml, domain_level=dl, nlayers=L, dt=dt, nsteps=nsteps, radius=o["params"][0],
+                omega=o["params"][1], grav=o["params"][2], ref_density=o["params"][3], topo=topo,
+                mass0=field("s0_mass", 1, True), temp0=field("s0_temp", 1, True),
+                velo0=field("s0_velo", 3, True), dmass=field("t0_dmass", 1, False),
+                dtemp=field("t0_dtemp", 1, False), dvelo=field("t0_dvelo", 3, False))
+    for s in range(1, nsteps + 1):
+        data[f"mass{s}"] = field(f"s{s:02d}_mass", 1, False)
+        data[f"temp{s}"] = field(f"s{s:02d}_temp", 1, False)
+        data[f"velo{s}"] = field(f"s{s:02d}_velo", 3, False)
+    np.savez_compressed(os.path.join(OUT, f"trisk_{name}.npz"), **data)
+    print(f"trisk_{name}.npz written")
+
+
+def main():
+    if not os.path.isdir("/root/reference/src"):
+        sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
+    for J in sorted(PARAMS):
+        grid_fixture(J)
+    for c in CASES:
+        trisk_fixture(*c)
+
+
+if __name__ == "__main__":
+    main()

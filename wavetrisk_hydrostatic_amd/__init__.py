@@ -1,0 +1,212 @@
+"""MI355X-native TRiSK dynamical-core hot path (host-side Python mirror).
+
+Thin ctypes binding of the C ABI in include/wavetrisk_gpu.h.  The compute
+path is the in-tree HIP library libwavetrisk_gpu.so; there is no CPU
+fallback: every entry point raises if the library is missing or fails.
+
+    grid = Grid(level=5, min_level=5, domain_level=2)       # host-side grid
+    ctx = Context(grid, nlayers=1)                           # needs a GPU
+    ctx.set_topography(topo); ctx.upload_state(0, mass, temp, velo)
+    ctx.rk4(0, dt, nsteps)                                   # dt_step => RK4
+    mass, temp, velo = ctx.download_state(0)
+
+Array layouts follow the header: scalars [nlayers, ncells], velocity
+[nlayers, 3, ncells] (RT, DG, UP), float64, C-contiguous.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .layout import Layout
+
+__all__ = ["Grid", "Context", "Layout", "WtgError", "library_path", "load_library"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libwavetrisk_gpu.so"
+_lib = None
+
+
+class WtgError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+class _GridDesc(C.Structure):
+    _fields_ = [("level", C.c_int), ("min_level", C.c_int), ("domain_level", C.c_int),
+                ("radius", C.c_double), ("omega", C.c_double)]
+
+
+class _GridInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("S", C.c_int), ("halo", C.c_int), ("ncells", C.c_int64),
+                ("south_pole", C.c_int64), ("north_pole", C.c_int64)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("nlayers", C.c_int), ("grav", C.c_double), ("ref_density", C.c_double), ("p_top", C.c_double)]
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every function of include/wavetrisk_gpu.h
+SIGNATURES = {
+    "wtg_grid_create": (C.c_int, [C.POINTER(_GridDesc), C.POINTER(_P)]),
+    "wtg_grid_info_get": (C.c_int, [_P, C.POINTER(_GridInfo)]),
+    "wtg_grid_get": (C.c_int, [_P, C.c_char_p, _P, C.c_size_t]),
+    "wtg_grid_destroy": (None, [_P]),
+    "wtg_create": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.POINTER(_P)]),
+    "wtg_destroy": (None, [_P]),
+    "wtg_set_topography": (C.c_int, [_P, _P]),
+    "wtg_upload_state": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "wtg_download_state": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "wtg_ghost_fill": (C.c_int, [_P, C.c_int]),
+    "wtg_trend": (C.c_int, [_P, C.c_int, C.c_int]),
+    "wtg_rk4": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
+    "wtg_rk4_timed": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _D, _D]),
+    "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "wtg_synchronize": (C.c_int, [_P]),
+    "wtg_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+}
+
+
+def load_library():
+    """Loads the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise WtgError(f"{path} is missing: build it with `python -m wavetrisk_hydrostatic_amd.build`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _err(ctx_handle) -> str:
+    buf = C.create_string_buffer(1024)
+    load_library().wtg_last_error(ctx_handle, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc: int, ctx_handle=None, what: str = ""):
+    if rc != 0:
+        raise WtgError(f"{what} failed ({rc}): {_err(ctx_handle)}")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Grid:
+    """Host-side icosahedral grid of one uniform level (no GPU needed)."""
+
+    def __init__(self, level: int, min_level: int | None = None, domain_level: int = 2,
+                 radius: float = 6.371e6, omega: float = 7.292e-5):
+        lib = load_library()
+        if min_level is None:
+            min_level = level
+        self.desc = _GridDesc(level, min_level, domain_level, radius, omega)
+        h = _P()
+        _check(lib.wtg_grid_create(C.byref(self.desc), C.byref(h)), None, "wtg_grid_create")
+        self._h = h
+        info = _GridInfo()
+        _check(lib.wtg_grid_info_get(h, C.byref(info)), None, "wtg_grid_info_get")
+        self.layout = Layout(level=level, n=info.n, S=info.S, halo=info.halo)
+        assert self.layout.ncells == info.ncells
+        assert self.layout.south_pole == info.south_pole and self.layout.north_pole == info.north_pole
+        self.radius, self.omega = radius, omega
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.wtg_grid_destroy(self._h)
+            self._h = None
+
+    _SHAPES = {"x": (3, np.float64), "len": (3, np.float64), "pedlen": (3, np.float64), "area": (6, np.float64),
+               "hex_inv": (1, np.float64), "triarea": (2, np.float64), "coriolis": (2, np.float64),
+               "node_src": (1, np.int64), "edge_src": (3, np.int64), "edge_sign": (3, np.int8)}
+
+    def array(self, name: str) -> np.ndarray:
+        comp, dt = self._SHAPES[name]
+        nc = self.layout.ncells
+        out = np.empty(comp * nc, dtype=dt)
+        _check(load_library().wtg_grid_get(self._h, name.encode(), _ptr(out), out.nbytes), None, f"wtg_grid_get({name})")
+        if name in ("edge_src", "edge_sign"):
+            return out  # slot = 3*cell + e
+        return out.reshape(comp, nc) if comp > 1 else out
+
+    @property
+    def handle(self):
+        return self._h
+
+
+class Context:
+    """Device state of one grid level on the current HIP device."""
+
+    def __init__(self, grid: Grid, nlayers: int, grav: float = 9.80616, ref_density: float = 1028.0,
+                 p_top: float = 0.0, nslots: int = 2):
+        lib = load_library()
+        self.grid = grid
+        self.layout = grid.layout
+        self.nlayers = nlayers
+        self.params = _Params(nlayers, grav, ref_density, p_top)
+        h = _P()
+        _check(lib.wtg_create(grid.handle, C.byref(self.params), nslots, C.byref(h)), None, "wtg_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.wtg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _shape(self):
+        L, nc = self.nlayers, self.layout.ncells
+        return (L, nc), (L, 3, nc)
+
+    def set_topography(self, topo: np.ndarray):
+        topo = np.ascontiguousarray(topo, dtype=np.float64)
+        assert topo.shape == (self.layout.ncells,)
+        _check(_lib.wtg_set_topography(self._h, _ptr(topo)), self._h, "wtg_set_topography")
+
+    def upload_state(self, slot: int, mass, temp, velo):
+        s_shape, e_shape = self._shape()
+        mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(s_shape)
+        temp = np.ascontiguousarray(temp, dtype=np.float64).reshape(s_shape)
+        velo = np.ascontiguousarray(velo, dtype=np.float64).reshape(e_shape)
+        _check(_lib.wtg_upload_state(self._h, slot, _ptr(mass), _ptr(temp), _ptr(velo)), self._h, "wtg_upload_state")
+
+    def download_state(self, slot: int):
+        s_shape, e_shape = self._shape()
+        mass, temp, velo = np.empty(s_shape), np.empty(s_shape), np.empty(e_shape)
+        _check(_lib.wtg_download_state(self._h, slot, _ptr(mass), _ptr(temp), _ptr(velo)), self._h,
+               "wtg_download_state")
+        return mass, temp, velo
+
+    def ghost_fill(self, slot: int):
+        _check(_lib.wtg_ghost_fill(self._h, slot), self._h, "wtg_ghost_fill")
+
+    def trend(self, q_slot: int, dq_slot: int):
+        _check(_lib.wtg_trend(self._h, q_slot, dq_slot), self._h, "wtg_trend")
+
+    def rk4(self, slot: int, dt: float, nsteps: int = 1):
+        _check(_lib.wtg_rk4(self._h, slot, dt, nsteps), self._h, "wtg_rk4")
+
+    def rk4_timed(self, slot: int, dt: float, nsteps: int):
+        tot, ker = C.c_double(), C.c_double()
+        _check(_lib.wtg_rk4_timed(self._h, slot, dt, nsteps, C.byref(tot), C.byref(ker)), self._h, "wtg_rk4_timed")
+        return tot.value, ker.value
+
+    def synchronize(self):
+        _check(_lib.wtg_synchronize(self._h), self._h, "wtg_synchronize")

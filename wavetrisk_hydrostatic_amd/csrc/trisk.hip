@@ -1,0 +1,848 @@
+// MI355X kernels and C ABI of the TRiSK hot path (see include/wavetrisk_gpu.h).
+//
+// One RK stage is one launch of trisk_stage_kernel followed by one launch of
+// ghost_fill_kernel.  The stage kernel evaluates trend_ml (multi_level.f90:5-47:
+// cal_surf_press, integrate_pressure_up, step1 itype 0, post_step1,
+// scalar_trend, du_source/Qperp, du_grad; ops.f90) on a 16 x 16 tile of owned
+// nodes of one lozenge, for every layer bottom to top, and fuses the RK
+// sub-step (time_integr.f90:206-225) into its stores:
+//   out = q + h * trend(y)      (or out = trend(y) for wtg_trend)
+// Per layer the block stages the tile's state plus a halo of one cell
+// (low side) and two cells (high side) in LDS, computes potential vorticity
+// on triangles, qe and mass fluxes on edges, kinetic energy / Bernoulli /
+// Exner on nodes, and finally the trends of the owned nodes and edges.  The
+// hydrostatic column scan (surface pressure, then pressure and geopotential
+// carried upward through the layers) lives in registers across the layer
+// loop, so a node's column is read once per stage.
+//
+// Arithmetic is FP64 in the reference's evaluation order and compiled
+// without FMA contraction, so results match the CPU oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/wavetrisk_gpu.h"
+#include "grid.h"
+
+namespace {
+
+constexpr int H = wtg::HALO;  // 2
+constexpr int T = 16;         // owned nodes per tile side
+constexpr int R = T + 3;      // staged region: tile cells -1 .. T+1
+constexpr int RR = R * R;     // 361
+constexpr int NP = T + 1;     // pressure / Bernoulli cells per side: 0 .. T
+constexpr int NT = T + 2;     // triangle / edge cells per side: -1 .. T
+constexpr int THREADS = 256;
+
+struct Geo {
+  const double* len;    // [3][NC]
+  const double* ped;    // [3][NC] local dual lengths (kinetic energy)
+  const double* pedf;   // [3][NC] dual lengths of the flux (owner's for halo edges)
+  const double* area;   // [6][NC]
+  const double* hinv;   // [NC]
+  const double* tri;    // [2][NC]
+  const double* cor;    // [2][NC]
+  const double* topo;   // [NC]
+};
+
+struct StageArgs {
+  int n, S, L, ntile;
+  int64_t NC;
+  int64_t spole, npole;
+  const double *ym, *yt, *yu;  // state the trend is evaluated on (ghost filled)
+  const double *qm, *qt, *qu;  // base state of the update
+  double *om, *ot, *ou;        // output state (or trend)
+  double h;                    // update coefficient
+  int trend_only;              // 1: out = trend
+  Geo g;
+  double grav, rho0, p_top;
+};
+
+__device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
+
+// LDS working set of one block (15 planes of the 19 x 19 region)
+struct Smem {
+  double m[RR], t[RR], u[3][RR];
+  double pvL[RR], pvU[RR];
+  double qe[3][RR], hm[3][RR];
+  double bern[RR], exn[RR];
+};
+
+// get_weights (ops.f90:1006-1021)
+__device__ __forceinline__ void weights(const Geo& g, int64_t NC, int64_t c, int offs, double w[5]) {
+  double acc0 = g.area[(int64_t)offs * NC + c];
+  double acc1 = acc0 + g.area[(int64_t)((1 + offs) % 6) * NC + c];
+  double acc2 = acc1 + g.area[(int64_t)((2 + offs) % 6) * NC + c];
+  double acc3 = acc2 + g.area[(int64_t)((3 + offs) % 6) * NC + c];
+  double acc4 = acc3 + g.area[(int64_t)((4 + offs) % 6) * NC + c];
+  double hi = g.hinv[c];
+  w[0] = 0.5 - acc0 * hi;
+  w[1] = -(0.5 - acc1 * hi);
+  w[2] = 0.5 - acc2 * hi;
+  w[3] = -(0.5 - acc3 * hi);
+  w[4] = 0.5 - acc4 * hi;
+}
+
+__global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a) {
+  __shared__ Smem sm;
+  const int tid = threadIdx.x;
+  const int64_t NC = a.NC;
+  const int S = a.S;
+  const Geo& g = a.g;
+
+  // XCD-aware tile order: block b runs on XCD b % 8, so give every XCD a
+  // contiguous band of tiles (its halos then hit that XCD's L2)
+  const int nb = gridDim.x;
+  int b = blockIdx.x;
+  int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int tpl = a.ntile * a.ntile;
+  const int lz = tile / tpl;
+  const int rem = tile % tpl;
+  const int i0 = (rem % a.ntile) * T, j0 = (rem / a.ntile) * T;
+  // global cell of region cell (ri, rj) = lozenge (i0-1+ri, j0-1+rj)
+  const int64_t base = (int64_t)lz * S * S + (int64_t)(j0 - 1 + H) * S + (i0 - 1 + H);
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+  auto r = [](int ri, int rj) { return rj * R + ri; };
+
+  // --- column state: surface pressure and geopotential of the P cells ------
+  // (cal_surf_press, set_surf_geopot; ops.f90:1023-1092)
+  double pl[2], gp[2], pr[2], gpl[2];
+  int pcell[2];
+  for (int q = 0; q < 2; ++q) {
+    int c = tid + q * THREADS;
+    pcell[q] = c < NP * NP ? c : -1;
+    pl[q] = gp[q] = pr[q] = gpl[q] = 0.0;
+    if (pcell[q] < 0) continue;
+    int ri = 1 + c % NP, rj = 1 + c / NP;
+    int64_t gc = gcell(ri, rj);
+    double s = 0.0;
+    for (int k = 0; k < a.L; ++k) s = s + (a.ym[(int64_t)k * NC + gc] - a.yt[(int64_t)k * NC + gc]);
+    pl[q] = a.grav * s + a.p_top;
+    gp[q] = a.grav * g.topo[gc];
+  }
+
+  const bool cSW = (i0 == 0 && j0 == 0), cSE = (i0 + T == a.n && j0 == 0);
+  const bool cNW = (i0 == 0 && j0 + T == a.n), cNE = (i0 + T == a.n && j0 + T == a.n);
+  const bool corner = cSW || cSE || cNW || cNE;
+
+  for (int k = 0; k < a.L; ++k) {
+    const double* ym = a.ym + (int64_t)k * NC;
+    const double* yt = a.yt + (int64_t)k * NC;
+    const double* yu = a.yu + (int64_t)k * 3 * NC;
+    // --- 1. stage the region; integrate pressure upward (ops.f90:1094-1131)
+    for (int c = tid; c < RR; c += THREADS) {
+      int64_t gc = gcell(c % R, c / R);
+      sm.m[c] = ym[gc];
+      sm.t[c] = yt[gc];
+      sm.u[0][c] = yu[gc];
+      sm.u[1][c] = yu[NC + gc];
+      sm.u[2][c] = yu[2 * NC + gc];
+    }
+    for (int q = 0; q < 2; ++q) {
+      if (pcell[q] < 0) continue;
+      int64_t gc = gcell(1 + pcell[q] % NP, 1 + pcell[q] / NP);
+      double rho_dz = ym[gc], rho_dz_theta = yt[gc];
+      gpl[q] = gp[q];
+      double p_upper = pl[q] - a.grav * (rho_dz - rho_dz_theta);
+      pr[q] = interp(pl[q], p_upper);
+      gp[q] = gpl[q] + a.grav * rho_dz / (a.rho0 * 1.0);
+      pl[q] = p_upper;
+    }
+    __syncthreads();
+
+    // --- 2. PV on triangles and mass fluxes on edges, cells -1 .. T ---------
+    for (int c = tid; c < NT * NT; c += THREADS) {
+      int ri = c % NT, rj = c / NT;
+      int x = r(ri, rj), xE = r(ri + 1, rj), xN = r(ri, rj + 1), xNE = r(ri + 1, rj + 1);
+      int64_t gc = gcell(ri, rj), gE = gcell(ri + 1, rj), gN = gcell(ri, rj + 1), gNE = gcell(ri + 1, rj + 1);
+      double upRT = sm.u[0][x] * g.len[gc], upDG = sm.u[1][x] * g.len[NC + gc], upUP = sm.u[2][x] * g.len[2 * NC + gc];
+      double circL = upRT + sm.u[2][xE] * g.len[2 * NC + gE] + upDG;
+      double circU = -(upDG + upUP + sm.u[0][xN] * g.len[gN]);
+      double massL = sm.m[x] * g.area[gc] + sm.m[xE] * g.area[2 * NC + gE] + sm.m[xNE] * g.area[4 * NC + gNE];
+      double massU = sm.m[x] * g.area[NC + gc] + sm.m[xNE] * g.area[3 * NC + gNE] + sm.m[xN] * g.area[5 * NC + gN];
+      sm.pvL[x] = (g.cor[gc] + circL) / massL;
+      sm.pvU[x] = (g.cor[NC + gc] + circU) / massU;
+      sm.hm[0][x] = sm.u[0][x] * g.pedf[gc] * interp(sm.m[x], sm.m[xE]) + 0.0;
+      sm.hm[1][x] = sm.u[1][x] * g.pedf[NC + gc] * interp(sm.m[x], sm.m[xNE]) + 0.0;
+      sm.hm[2][x] = sm.u[2][x] * g.pedf[2 * NC + gc] * interp(sm.m[x], sm.m[xN]) + 0.0;
+    }
+    __syncthreads();
+
+    // --- 3. qe on edges; kinetic energy, Bernoulli, Exner on nodes 0 .. T ---
+    for (int c = tid; c < NT * NT; c += THREADS) {
+      int ri = c % NT, rj = c / NT, x = r(ri, rj);
+      if (rj >= 1) sm.qe[0][x] = interp(sm.pvU[r(ri, rj - 1)], sm.pvL[x]);
+      sm.qe[1][x] = interp(sm.pvU[x], sm.pvL[x]);
+      if (ri >= 1) sm.qe[2][x] = interp(sm.pvU[x], sm.pvL[r(ri - 1, rj)]);
+    }
+    for (int q = 0; q < 2; ++q) {
+      if (pcell[q] < 0) continue;
+      int ri = 1 + pcell[q] % NP, rj = 1 + pcell[q] / NP;
+      int x = r(ri, rj), xW = r(ri - 1, rj), xS = r(ri, rj - 1), xSW = r(ri - 1, rj - 1);
+      int64_t gc = gcell(ri, rj), gW = gcell(ri - 1, rj), gS = gcell(ri, rj - 1), gSW = gcell(ri - 1, rj - 1);
+      double ke = ((sm.u[2][x] * g.len[2 * NC + gc]) * (sm.u[2][x] * g.ped[2 * NC + gc]) +
+                   (sm.u[1][x] * g.len[NC + gc]) * (sm.u[1][x] * g.ped[NC + gc]) +
+                   (sm.u[0][x] * g.len[gc]) * (sm.u[0][x] * g.ped[gc]) +
+                   (sm.u[2][xS] * g.len[2 * NC + gS]) * (sm.u[2][xS] * g.ped[2 * NC + gS]) +
+                   (sm.u[1][xSW] * g.len[NC + gSW]) * (sm.u[1][xSW] * g.ped[NC + gSW]) +
+                   (sm.u[0][xW] * g.len[gW]) * (sm.u[0][xW] * g.ped[gW])) *
+                  g.hinv[gc] / 4.0;
+      double phi_k = interp(gp[q], gpl[q]);
+      sm.bern[x] = ke + phi_k + pr[q] / (a.rho0 * 1.0);
+      sm.exn[x] = -phi_k;
+    }
+    __syncthreads();
+
+    // --- 3b. pentagon corners (post_step1, ops.f90:526-737) -----------------
+    if (corner) {
+      if (tid < 4) {
+        auto UPL = [&](int e, int ri, int rj) { return sm.u[e][r(ri, rj)] * g.len[(int64_t)e * NC + gcell(ri, rj)]; };
+        auto M = [&](int ri, int rj) { return sm.m[r(ri, rj)]; };
+        auto AR = [&](int p, int ri, int rj) { return g.area[(int64_t)(p - 1) * NC + gcell(ri, rj)]; };
+        auto TRI = [&](int t, int ri, int rj) { return g.tri[(int64_t)t * NC + gcell(ri, rj)]; };
+        auto COR = [&](int t, int ri, int rj) { return g.cor[(int64_t)t * NC + gcell(ri, rj)]; };
+        if (tid == 0 && cSW) {  // IJMINUS at lozenge (0,0) = region (1,1)
+          const int i = 1, j = 1;
+          double cSW_L = UPL(2, i, j - 1) - UPL(0, i - 1, j) + UPL(0, i - 1, j - 1);
+          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+          double cS_U = (-(UPL(0, i, j) + UPL(1, i, j - 1) + UPL(2, i, j - 1)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+          double m1 = M(i - 1, j) * AR(6, i - 1, j) + M(i, j) * (AR(4, i, j) + AR(5, i, j)) + M(i, j - 1) * AR(3, i, j - 1);
+          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+          double pvWL = (COR(0, i - 1, j) + cW_L) / m1;
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m2;
+          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+          sm.qe[0][r(i - 1, j)] = interp(pvWL, pvSWL);
+          sm.qe[2][r(i, j - 1)] = interp(pvSU, pvSWL);
+        }
+        if (tid == 1 && cSE) {  // IPLUSJMINUS at lozenge (n,0) = region (T+1,1)
+          const int i = T + 1, j = 1;
+          double cSW_L = -UPL(0, i, j) + UPL(0, i - 1, j - 1) + UPL(1, i - 1, j - 1);
+          double cL = ((UPL(0, i, j) + UPL(2, i + 1, j) + UPL(1, i, j)) / TRI(0, i, j)) * TRI(0, i, j);
+          double cSW_U = (-(UPL(0, i - 1, j) + UPL(1, i - 1, j - 1) + UPL(2, i - 1, j - 1)) / TRI(1, i - 1, j - 1)) *
+                         TRI(1, i - 1, j - 1);
+          double m1 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) +
+                      M(i, j) * (AR(5, i, j) + AR(6, i, j));
+          double m2 = M(i, j) * AR(1, i, j) + M(i + 1, j) * AR(3, i + 1, j) + M(i + 1, j + 1) * AR(5, i + 1, j + 1);
+          double m3 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * AR(4, i, j) + M(i - 1, j) * AR(6, i - 1, j);
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m1;
+          double pvL = (COR(0, i, j) + cL) / m2;
+          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m3;
+          sm.qe[0][r(i, j)] = interp(pvSWL, pvL);
+          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+        }
+        if (tid == 2 && cNW) {  // IMINUSJPLUS at lozenge (0,n) = region (1,T+1)
+          const int i = 1, j = T + 1;
+          double cSW_U = UPL(2, i, j) - UPL(1, i - 1, j - 1) - UPL(2, i - 1, j - 1);
+          double cU = (-(UPL(1, i, j) + UPL(2, i, j) + UPL(0, i, j + 1)) / TRI(1, i, j)) * TRI(1, i, j);
+          double cSW_L = ((UPL(0, i - 1, j - 1) + UPL(2, i, j - 1) + UPL(1, i - 1, j - 1)) / TRI(0, i - 1, j - 1)) *
+                         TRI(0, i - 1, j - 1);
+          double m1 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * (AR(3, i, j) + AR(4, i, j)) +
+                      M(i - 1, j) * AR(6, i - 1, j);
+          double m2 = M(i, j) * AR(2, i, j) + M(i + 1, j + 1) * AR(4, i + 1, j + 1) + M(i, j + 1) * AR(6, i, j + 1);
+          double m3 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) + M(i, j) * AR(5, i, j);
+          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m1;
+          double pvU = (COR(1, i, j) + cU) / m2;
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m3;
+          sm.qe[2][r(i, j)] = interp(pvSWU, pvU);
+          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+        }
+        if (tid == 3 && cNE) {  // IJPLUS at lozenge (n,n) = region (T+1,T+1)
+          const int i = T + 1, j = T + 1;
+          double cL = UPL(0, i, j) - UPL(0, i, j + 1) - UPL(2, i, j);
+          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+          double cS_U = (-(UPL(1, i, j - 1) + UPL(2, i, j - 1) + UPL(0, i, j)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+          double m1 = M(i + 1, j) * AR(3, i + 1, j) + M(i, j) * (AR(1, i, j) + AR(2, i, j)) + M(i, j + 1) * AR(6, i, j + 1);
+          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+          double pvL = (COR(0, i, j) + cL) / m1;
+          double pvWL = (COR(0, i - 1, j) + cW_L) / m2;
+          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+          sm.qe[0][r(i, j)] = interp(pvL, pvSU);
+          sm.qe[2][r(i, j)] = interp(pvL, pvWL);
+        }
+      }
+      __syncthreads();
+    }
+
+    // --- 4. trends of owned nodes and edges; fused RK update -----------------
+    {
+      const int ri = 1 + tid % T, rj = 1 + tid / T;
+      const int x = r(ri, rj), xW = r(ri - 1, rj), xS = r(ri, rj - 1), xSW = r(ri - 1, rj - 1);
+      const int xE = r(ri + 1, rj), xN = r(ri, rj + 1), xNE = r(ri + 1, rj + 1), xNW = r(ri - 1, rj + 1);
+      const int xSE = r(ri + 1, rj - 1);
+      const int64_t gc = gcell(ri, rj);
+      const double hi = g.hinv[gc];
+      // scalar trends (scalar_trend / div, ops.f90:739-755, 1325-1343)
+      double divm = (sm.hm[0][x] - sm.hm[0][xW] + sm.hm[1][xSW] - sm.hm[1][x] + sm.hm[2][x] - sm.hm[2][xS]) * hi;
+      auto ft = [&](int e, int xa, int xb, int64_t ga) {
+        return sm.u[e][xa] * g.pedf[(int64_t)e * NC + ga] * interp(sm.t[xa], sm.t[xb]) + 0.0;
+      };
+      const int64_t gW = gcell(ri - 1, rj), gS = gcell(ri, rj - 1), gSW = gcell(ri - 1, rj - 1);
+      double divt = (ft(0, x, xE, gc) - ft(0, xW, x, gW) + ft(1, xSW, x, gSW) - ft(1, x, xNE, gc) + ft(2, x, xN, gc) -
+                     ft(2, xS, x, gS)) *
+                    hi;
+      double dmv = -divm, dtv = -divt;
+      // velocity trend: -Qperp (ops.f90:834-901), then gradients (ops.f90:787-832)
+      const int64_t gE = gcell(ri + 1, rj), gNE = gcell(ri + 1, rj + 1), gN = gcell(ri, rj + 1);
+      double w1[5], w2[5], qp[3];
+#define HM(e, xx) sm.hm[e][xx]
+#define QE(e, xx) sm.qe[e][xx]
+      weights(g, NC, gc, 0, w1);
+      weights(g, NC, gE, 3, w2);
+      double qr = QE(0, x);
+      qp[0] = HM(1, x) * interp(QE(1, x), qr) * w1[0] + HM(2, x) * interp(QE(2, x), qr) * w1[1] +
+              HM(0, xW) * interp(QE(0, xW), qr) * w1[2] + HM(1, xSW) * interp(QE(1, xSW), qr) * w1[3] +
+              HM(2, xS) * interp(QE(2, xS), qr) * w1[4] + HM(1, xS) * interp(QE(1, xS), qr) * w2[0] +
+              HM(2, xSE) * interp(QE(2, xSE), qr) * w2[1] + HM(0, xE) * interp(QE(0, xE), qr) * w2[2] +
+              HM(1, xE) * interp(QE(1, xE), qr) * w2[3] + HM(2, xE) * interp(QE(2, xE), qr) * w2[4];
+      weights(g, NC, gc, 1, w1);
+      weights(g, NC, gNE, 4, w2);
+      double qd = QE(1, x);
+      qp[1] = HM(2, x) * interp(QE(2, x), qd) * w1[0] + HM(0, xW) * interp(QE(0, xW), qd) * w1[1] +
+              HM(1, xSW) * interp(QE(1, xSW), qd) * w1[2] + HM(2, xS) * interp(QE(2, xS), qd) * w1[3] +
+              HM(0, x) * interp(QE(0, x), qd) * w1[4] + HM(2, xE) * interp(QE(2, xE), qd) * w2[0] +
+              HM(0, xNE) * interp(QE(0, xNE), qd) * w2[1] + HM(1, xNE) * interp(QE(1, xNE), qd) * w2[2] +
+              HM(2, xNE) * interp(QE(2, xNE), qd) * w2[3] + HM(0, xN) * interp(QE(0, xN), qd) * w2[4];
+      weights(g, NC, gc, 2, w1);
+      weights(g, NC, gN, 5, w2);
+      double qu = QE(2, x);
+      qp[2] = HM(0, xW) * interp(QE(0, xW), qu) * w1[0] + HM(1, xSW) * interp(QE(1, xSW), qu) * w1[1] +
+              HM(2, xS) * interp(QE(2, xS), qu) * w1[2] + HM(0, x) * interp(QE(0, x), qu) * w1[3] +
+              HM(1, x) * interp(QE(1, x), qu) * w1[4] + HM(0, xN) * interp(QE(0, xN), qu) * w2[0] +
+              HM(1, xN) * interp(QE(1, xN), qu) * w2[1] + HM(2, xN) * interp(QE(2, xN), qu) * w2[2] +
+              HM(0, xNW) * interp(QE(0, xNW), qu) * w2[3] + HM(1, xW) * interp(QE(1, xW), qu) * w2[4];
+#undef HM
+#undef QE
+      double th0 = sm.t[x] / sm.m[x], thE = sm.t[xE] / sm.m[xE], thNE = sm.t[xNE] / sm.m[xNE],
+             thN = sm.t[xN] / sm.m[xN];
+      double the[3] = {interp(th0, thE), interp(th0, thNE), interp(th0, thN)};
+      double l0 = g.len[gc], l1 = g.len[NC + gc], l2 = g.len[2 * NC + gc];
+      double gB[3] = {(sm.bern[xE] - sm.bern[x]) / l0, (sm.bern[x] - sm.bern[xNE]) / l1, (sm.bern[xN] - sm.bern[x]) / l2};
+      double gX[3] = {(sm.exn[xE] - sm.exn[x]) / l0, (sm.exn[x] - sm.exn[xNE]) / l1, (sm.exn[xN] - sm.exn[x]) / l2};
+      double ll[3] = {l0, l1, l2};
+      const int64_t ks = (int64_t)k * NC, ke3 = (int64_t)k * 3 * NC;
+      if (a.trend_only) {
+        a.om[ks + gc] = dmv;
+        a.ot[ks + gc] = dtv;
+      } else {
+        a.om[ks + gc] = a.qm[ks + gc] + a.h * dmv;
+        a.ot[ks + gc] = a.qt[ks + gc] + a.h * dtv;
+      }
+      for (int e = 0; e < 3; ++e) {
+        double src = -qp[e] + 0.0 * ll[e];
+        double du = src / ll[e] - gB[e] - the[e] * gX[e];
+        int64_t ge = ke3 + (int64_t)e * NC + gc;
+        a.ou[ge] = a.trend_only ? du : a.qu[ge] + a.h * du;
+      }
+      // pole master cell (south: lozenge 0 cell (n,0); north: lozenge 5 (0,n))
+      if (tid == 0 && ((lz == 0 && cSE) || (lz == 5 && cNW))) {
+        const int pi = (lz == 0) ? T + 1 : 1, pj = (lz == 0) ? 1 : T + 1;
+        const int y = r(pi, pj), yW = r(pi - 1, pj), yS = r(pi, pj - 1), ySW = r(pi - 1, pj - 1);
+        const int yE = r(pi + 1, pj), yN = r(pi, pj + 1), yNE = r(pi + 1, pj + 1);
+        const int64_t pc = gcell(pi, pj);
+        const double phi = g.hinv[pc];
+        double pdm = -((sm.hm[0][y] - sm.hm[0][yW] + sm.hm[1][ySW] - sm.hm[1][y] + sm.hm[2][y] - sm.hm[2][yS]) * phi);
+        const int64_t pW = gcell(pi - 1, pj), pS = gcell(pi, pj - 1), pSW = gcell(pi - 1, pj - 1);
+        double pdt = -((ft(0, y, yE, pc) - ft(0, yW, y, pW) + ft(1, ySW, y, pSW) - ft(1, y, yNE, pc) +
+                        ft(2, y, yN, pc) - ft(2, yS, y, pS)) *
+                       phi);
+        if (a.trend_only) {
+          a.om[ks + pc] = pdm;
+          a.ot[ks + pc] = pdt;
+        } else {
+          a.om[ks + pc] = a.qm[ks + pc] + a.h * pdm;
+          a.ot[ks + pc] = a.qt[ks + pc] + a.h * pdt;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// halo := owner (nodes: mass and temp; edges: velocity with sign), all layers
+__global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__ nd, int nn,
+                                  const int64_t* __restrict__ ed, const int8_t* __restrict__ es, int ne, double* m,
+                                  double* t, double* u) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)(nn + ne) * L;
+  if (idx >= total) return;
+  int k = (int)(idx % L);
+  int64_t e = idx / L;
+  if (e < nn) {
+    int64_t dst = nd[2 * e], src = nd[2 * e + 1];
+    m[(int64_t)k * NC + dst] = m[(int64_t)k * NC + src];
+    t[(int64_t)k * NC + dst] = t[(int64_t)k * NC + src];
+  } else {
+    e -= nn;
+    int64_t dst = ed[2 * e], src = ed[2 * e + 1];  // plane indices e*NC + cell
+    u[(int64_t)k * 3 * NC + dst] = (double)es[e] * u[(int64_t)k * 3 * NC + src];
+  }
+}
+
+// copy frozen halo slots (never ghost filled) from one state to another
+__global__ void copy_frozen_kernel(int L, int64_t NC, const int64_t* __restrict__ fn, int nn,
+                                   const int64_t* __restrict__ fe, int ne, const double* sm_, const double* st_,
+                                   const double* su_, double* dm_, double* dt_, double* du_) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)(nn + ne) * L;
+  if (idx >= total) return;
+  int k = (int)(idx % L);
+  int64_t e = idx / L;
+  if (e < nn) {
+    int64_t c = fn[e];
+    dm_[(int64_t)k * NC + c] = sm_[(int64_t)k * NC + c];
+    dt_[(int64_t)k * NC + c] = st_[(int64_t)k * NC + c];
+  } else {
+    int64_t p = fe[e - nn];
+    du_[(int64_t)k * 3 * NC + p] = su_[(int64_t)k * 3 * NC + p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host runtime
+// ---------------------------------------------------------------------------
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct wtg_grid {
+  wtg::Level lv;
+};
+
+struct wtg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  wtg_grid_info info{};
+  wtg_params params{};
+  int nslots = 0;
+  bool topo_set = false;
+  std::string err;
+  // device geometry
+  double *d_len = nullptr, *d_ped = nullptr, *d_pedf = nullptr, *d_area = nullptr, *d_hinv = nullptr;
+  double *d_tri = nullptr, *d_cor = nullptr, *d_topo = nullptr;
+  int64_t *d_nd = nullptr, *d_ed = nullptr, *d_fn = nullptr, *d_fe = nullptr;
+  int8_t* d_es = nullptr;
+  int nn = 0, ne = 0, nfn = 0, nfe = 0;
+  std::vector<int64_t> node_src;
+  // state slots: user slots then two stage buffers
+  struct Slot {
+    double *m = nullptr, *t = nullptr, *u = nullptr;
+  };
+  std::vector<Slot> slots;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(wtg_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                        \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail(ctx, WTG_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class Tp>
+int dalloc(wtg_ctx* c, Tp** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(Tp) > 0 ? count * sizeof(Tp) : 1);
+  if (e != hipSuccess) return fail(c, WTG_E_DEVICE, "hipMalloc(%zu bytes): %s", count * sizeof(Tp), hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = (Tp*)q;
+  return 0;
+}
+
+int check_slot(wtg_ctx* c, int slot) {
+  if (!c) return WTG_E_ARG;
+  if (slot < 0 || slot >= c->nslots) return fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
+  return 0;
+}
+
+int launch_ghost_fill(wtg_ctx* c, const wtg_ctx::Slot& s) {
+  const int L = c->params.nlayers;
+  int64_t total = (int64_t)(c->nn + c->ne) * L;
+  if (total == 0) return 0;
+  int threads = 256;
+  int64_t blocks = (total + threads - 1) / threads;
+  hipLaunchKernelGGL(ghost_fill_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, L, c->info.ncells,
+                     c->d_nd, c->nn, c->d_ed, c->d_es, c->ne, s.m, s.t, s.u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int launch_copy_frozen(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to) {
+  const int L = c->params.nlayers;
+  int64_t total = (int64_t)(c->nfn + c->nfe) * L;
+  if (total == 0) return 0;
+  int threads = 256;
+  int64_t blocks = (total + threads - 1) / threads;
+  hipLaunchKernelGGL(copy_frozen_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, L, c->info.ncells,
+                     c->d_fn, c->nfn, c->d_fe, c->nfe, from.m, from.t, from.u, to.m, to.t, to.u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, const wtg_ctx::Slot& out, double h,
+                 int trend_only) {
+  StageArgs a{};
+  a.n = c->info.n;
+  a.S = c->info.S;
+  a.L = c->params.nlayers;
+  a.ntile = a.n / T;
+  a.NC = c->info.ncells;
+  a.spole = c->info.south_pole;
+  a.npole = c->info.north_pole;
+  a.ym = y.m;
+  a.yt = y.t;
+  a.yu = y.u;
+  a.qm = q.m;
+  a.qt = q.t;
+  a.qu = q.u;
+  a.om = out.m;
+  a.ot = out.t;
+  a.ou = out.u;
+  a.h = h;
+  a.trend_only = trend_only;
+  a.g = Geo{c->d_len, c->d_ped, c->d_pedf, c->d_area, c->d_hinv, c->d_tri, c->d_cor, c->d_topo};
+  a.grav = c->params.grav;
+  a.rho0 = c->params.ref_density;
+  a.p_top = c->params.p_top;
+  unsigned blocks = (unsigned)(10 * a.ntile * a.ntile);
+  hipLaunchKernelGGL(trisk_stage_kernel, dim3(blocks), dim3(THREADS), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// nsteps RK4 steps; when ev is given, ev[2i] / ev[2i+1] are recorded around
+// the i-th stage-kernel launch (no synchronisation between launches)
+int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
+  if (nsteps < 0) return fail(c, WTG_E_ARG, "nsteps < 0");
+  const auto& sol = c->slots[slot];
+  const auto& w1 = c->slots[c->nslots];
+  const auto& w2 = c->slots[c->nslots + 1];
+  if (int rc = launch_copy_frozen(c, sol, w1)) return rc;
+  if (int rc = launch_copy_frozen(c, sol, w2)) return rc;
+  const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
+  int launch = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    const wtg_ctx::Slot* ys[4] = {&sol, &w1, &w2, &w1};
+    const wtg_ctx::Slot* os[4] = {&w1, &w2, &w1, &sol};
+    for (int st = 0; st < 4; ++st, ++launch) {
+      if (ev) HIPCHK(c, hipEventRecord(ev[2 * launch], c->stream));
+      if (int rc = launch_stage(c, *ys[st], sol, *os[st], coef[st], 0)) return rc;
+      if (ev) HIPCHK(c, hipEventRecord(ev[2 * launch + 1], c->stream));
+      if (int rc = launch_ghost_fill(c, *os[st])) return rc;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int wtg_grid_create(const wtg_grid_desc* desc, wtg_grid** out) {
+  if (!desc || !out) return fail(nullptr, WTG_E_ARG, "null argument");
+  *out = nullptr;
+  try {
+    auto* g = new wtg_grid;
+    g->lv = wtg::build_level(desc->level, desc->min_level, desc->domain_level, desc->radius, desc->omega);
+    if (g->lv.n % T != 0) {
+      delete g;
+      return fail(nullptr, WTG_E_ARG, "level %d: lozenge side %d is not a multiple of the tile %d", desc->level,
+                  1 << desc->level, T);
+    }
+    *out = g;
+  } catch (const std::exception& e) {
+    return fail(nullptr, WTG_E_ARG, "wtg_grid_create: %s", e.what());
+  }
+  return 0;
+}
+
+int wtg_grid_info_get(const wtg_grid* grid, wtg_grid_info* info) {
+  if (!grid || !info) return fail(nullptr, WTG_E_ARG, "null argument");
+  info->n = grid->lv.n;
+  info->S = grid->lv.S;
+  info->halo = H;
+  info->ncells = (int64_t)grid->lv.ncells();
+  info->south_pole = (int64_t)grid->lv.south_pole;
+  info->north_pole = (int64_t)grid->lv.north_pole;
+  return 0;
+}
+
+int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes) {
+  if (!grid || !name || !out) return fail(nullptr, WTG_E_ARG, "null argument");
+  const auto& g = grid->lv;
+  const void* src = nullptr;
+  size_t n = 0;
+  std::string s(name);
+  if (s == "x") { src = g.X.data(); n = g.X.size() * 8; }
+  else if (s == "len") { src = g.len.data(); n = g.len.size() * 8; }
+  else if (s == "pedlen") { src = g.pedlen.data(); n = g.pedlen.size() * 8; }
+  else if (s == "area") { src = g.area.data(); n = g.area.size() * 8; }
+  else if (s == "hex_inv") { src = g.hex_inv.data(); n = g.hex_inv.size() * 8; }
+  else if (s == "triarea") { src = g.triarea.data(); n = g.triarea.size() * 8; }
+  else if (s == "coriolis") { src = g.coriolis.data(); n = g.coriolis.size() * 8; }
+  else if (s == "node_src") { src = g.node_src.data(); n = g.node_src.size() * 8; }
+  else if (s == "edge_src") { src = g.edge_src.data(); n = g.edge_src.size() * 8; }
+  else if (s == "edge_sign") { src = g.edge_sign.data(); n = g.edge_sign.size(); }
+  else return fail(nullptr, WTG_E_ARG, "unknown grid array '%s'", name);
+  if (bytes != n) return fail(nullptr, WTG_E_ARG, "grid array '%s' has %zu bytes, %zu given", name, n, bytes);
+  std::memcpy(out, src, n);
+  return 0;
+}
+
+void wtg_grid_destroy(wtg_grid* grid) { delete grid; }
+
+int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_ctx** out) {
+  if (!grid || !params || !out) return fail(nullptr, WTG_E_ARG, "null argument");
+  *out = nullptr;
+  if (nslots < 1 || nslots > 64) return fail(nullptr, WTG_E_ARG, "nslots %d out of range [1,64]", nslots);
+  if (params->nlayers < 1) return fail(nullptr, WTG_E_ARG, "nlayers %d < 1", params->nlayers);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, WTG_E_DEVICE, "no HIP device");
+  std::unique_ptr<wtg_ctx> c(new wtg_ctx);
+  if (hipGetDevice(&c->device) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipGetDevice failed");
+  if (hipStreamCreate(&c->stream) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipStreamCreate failed");
+  wtg_grid_info_get(grid, &c->info);
+  c->params = *params;
+  c->nslots = nslots;
+  const auto& g = grid->lv;
+  const int64_t NC = c->info.ncells;
+  // flux dual lengths: a halo edge that is ghost filled carries its owner's flux
+  std::vector<double> pedf(3 * NC);
+  for (int64_t cc = 0; cc < NC; ++cc)
+    for (int e = 0; e < 3; ++e) {
+      int64_t slot = 3 * cc + e, s = g.edge_src[slot];
+      pedf[e * NC + cc] = (s >= 0 && s != slot) ? g.pedlen[(s % 3) * NC + s / 3] : g.pedlen[e * NC + cc];
+    }
+  // ghost lists and frozen lists
+  std::vector<int64_t> nd, ed, fn, fe;
+  std::vector<int8_t> es;
+  const int n = c->info.n, S = c->info.S;
+  for (int64_t cc = 0; cc < NC; ++cc) {
+    int64_t rr = cc % ((int64_t)S * S);
+    int j = (int)(rr / S) - H, i = (int)(rr % S) - H;
+    bool owned = (i >= 0 && i < n && j >= 0 && j < n);
+    if (owned) continue;
+    int64_t s = g.node_src[cc];
+    if (s >= 0 && s != cc) { nd.push_back(cc); nd.push_back(s); }
+    if (s < 0) fn.push_back(cc);
+    for (int e = 0; e < 3; ++e) {
+      int64_t slot = 3 * cc + e, se = g.edge_src[slot];
+      if (se >= 0 && se != slot) {
+        ed.push_back(e * NC + cc);
+        ed.push_back((se % 3) * NC + se / 3);
+        es.push_back(g.edge_sign[slot]);
+      }
+      if (se < 0) fe.push_back(e * NC + cc);
+    }
+  }
+  c->nn = (int)(nd.size() / 2);
+  c->ne = (int)(ed.size() / 2);
+  c->nfn = (int)fn.size();
+  c->nfe = (int)fe.size();
+  c->node_src = g.node_src;
+  wtg_ctx* cp = c.get();
+  int rc = 0;
+  auto up = [&](auto** dptr, const auto* host, size_t count) -> int {
+    using Tp = std::remove_const_t<std::remove_reference_t<decltype(*host)>>;
+    if (int r2 = dalloc<Tp>(cp, (Tp**)dptr, count)) return r2;
+    if (count) HIPCHK(cp, hipMemcpy(*dptr, host, count * sizeof(Tp), hipMemcpyHostToDevice));
+    return 0;
+  };
+  if ((rc = up(&c->d_len, g.len.data(), g.len.size())) || (rc = up(&c->d_ped, g.pedlen.data(), g.pedlen.size())) ||
+      (rc = up(&c->d_pedf, pedf.data(), pedf.size())) || (rc = up(&c->d_area, g.area.data(), g.area.size())) ||
+      (rc = up(&c->d_hinv, g.hex_inv.data(), g.hex_inv.size())) ||
+      (rc = up(&c->d_tri, g.triarea.data(), g.triarea.size())) ||
+      (rc = up(&c->d_cor, g.coriolis.data(), g.coriolis.size())) || (rc = up(&c->d_nd, nd.data(), nd.size())) ||
+      (rc = up(&c->d_ed, ed.data(), ed.size())) || (rc = up(&c->d_es, es.data(), es.size())) ||
+      (rc = up(&c->d_fn, fn.data(), fn.size())) || (rc = up(&c->d_fe, fe.data(), fe.size()))) {
+    g_create_error = c->err;
+    wtg_destroy(c.release());
+    return rc;
+  }
+  if ((rc = dalloc<double>(cp, &c->d_topo, NC))) {
+    g_create_error = c->err;
+    wtg_destroy(c.release());
+    return rc;
+  }
+  const int L = params->nlayers;
+  c->slots.resize(nslots + 2);
+  for (auto& s : c->slots) {
+    if ((rc = dalloc<double>(cp, &s.m, (size_t)L * NC)) || (rc = dalloc<double>(cp, &s.t, (size_t)L * NC)) ||
+        (rc = dalloc<double>(cp, &s.u, (size_t)3 * L * NC))) {
+      g_create_error = c->err;
+      wtg_destroy(c.release());
+      return rc;
+    }
+    hipMemset(s.m, 0, sizeof(double) * L * NC);
+    hipMemset(s.t, 0, sizeof(double) * L * NC);
+    hipMemset(s.u, 0, sizeof(double) * 3 * L * NC);
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    g_create_error = "device synchronisation after allocation failed";
+    wtg_destroy(c.release());
+    return WTG_E_DEVICE;
+  }
+  *out = c.release();
+  return 0;
+}
+
+void wtg_destroy(wtg_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int wtg_set_topography(wtg_ctx* c, const double* topo) {
+  if (!c || !topo) return WTG_E_ARG;
+  const int64_t NC = c->info.ncells;
+  std::vector<double> t(topo, topo + NC);
+  for (int64_t cc = 0; cc < NC; ++cc) {
+    int64_t s = c->node_src[cc];
+    if (s >= 0) t[cc] = topo[s];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_topo, t.data(), NC * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->topo_set = true;
+  return 0;
+}
+
+int wtg_upload_state(wtg_ctx* c, int slot, const double* mass, const double* temp, const double* velo) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (!mass || !temp || !velo) return fail(c, WTG_E_ARG, "null state array");
+  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const auto& s = c->slots[slot];
+  HIPCHK(c, hipMemcpyAsync(s.m, mass, ns * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.t, temp, ns * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.u, velo, 3 * ns * 8, hipMemcpyHostToDevice, c->stream));
+  if (int rc = launch_ghost_fill(c, s)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_download_state(wtg_ctx* c, int slot, double* mass, double* temp, double* velo) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (!mass || !temp || !velo) return fail(c, WTG_E_ARG, "null state array");
+  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const auto& s = c->slots[slot];
+  HIPCHK(c, hipMemcpyAsync(mass, s.m, ns * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(temp, s.t, ns * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(velo, s.u, 3 * ns * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_ghost_fill(wtg_ctx* c, int slot) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (int rc = launch_ghost_fill(c, c->slots[slot])) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_trend(wtg_ctx* c, int q_slot, int dq_slot) {
+  if (int rc = check_slot(c, q_slot)) return rc;
+  if (int rc = check_slot(c, dq_slot)) return rc;
+  if (q_slot == dq_slot) return fail(c, WTG_E_ARG, "q_slot and dq_slot must differ");
+  if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before wtg_trend");
+  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const auto& dq = c->slots[dq_slot];
+  HIPCHK(c, hipMemsetAsync(dq.m, 0, ns * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(dq.t, 0, ns * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(dq.u, 0, 3 * ns * 8, c->stream));
+  const auto& q = c->slots[q_slot];
+  if (int rc = launch_stage(c, q, q, dq, 0.0, 1)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_rk4(wtg_ctx* c, int slot, double dt, int nsteps) {
+  if (int rc = rk4_impl(c, slot, dt, nsteps, nullptr)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_rk4_timed(wtg_ctx* c, int slot, double dt, int nsteps, double* total_ms, double* stage_kernel_ms) {
+  if (!c || !total_ms || !stage_kernel_ms) return WTG_E_ARG;
+  if (nsteps < 1 || nsteps > 100000) return fail(c, WTG_E_ARG, "nsteps %d out of range [1,100000]", nsteps);
+  const int nl = 4 * nsteps;
+  std::vector<hipEvent_t> ev(2 * nl + 2, nullptr);
+  int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { rc = fail(c, WTG_E_DEVICE, "hipEventCreate failed"); break; }
+  if (!rc) {
+    // whole job between ev[2nl] and ev[2nl+1]; every stage launch between its own pair
+    if (hipStreamSynchronize(c->stream) != hipSuccess || hipEventRecord(ev[2 * nl], c->stream) != hipSuccess)
+      rc = fail(c, WTG_E_DEVICE, "event record failed");
+    if (!rc) rc = rk4_impl(c, slot, dt, nsteps, ev.data());
+    if (!rc && (hipEventRecord(ev[2 * nl + 1], c->stream) != hipSuccess ||
+                hipEventSynchronize(ev[2 * nl + 1]) != hipSuccess))
+      rc = fail(c, WTG_E_DEVICE, "event synchronise failed");
+    if (!rc) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[2 * nl], ev[2 * nl + 1]);
+      *total_ms = ms;
+      double sum = 0.0;
+      for (int i = 0; i < nl; ++i) {
+        float k = 0.f;
+        hipEventElapsedTime(&k, ev[2 * i], ev[2 * i + 1]);
+        sum += k;
+      }
+      *stage_kernel_ms = sum / nl;
+    }
+  }
+  for (auto& e : ev)
+    if (e) hipEventDestroy(e);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_state_device_ptrs(wtg_ctx* c, int slot, double** mass, double** temp, double** velo) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (mass) *mass = c->slots[slot].m;
+  if (temp) *temp = c->slots[slot].t;
+  if (velo) *velo = c->slots[slot].u;
+  return 0;
+}
+
+int wtg_synchronize(wtg_ctx* c) {
+  if (!c) return WTG_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_last_error(const wtg_ctx* c, char* buf, size_t n) {
+  if (!buf || n == 0) return WTG_E_ARG;
+  const std::string& s = c ? c->err : g_create_error;
+  std::snprintf(buf, n, "%s", s.c_str());
+  return 0;
+}
+
+}  // extern "C"
