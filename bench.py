@@ -49,7 +49,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--layers", type=int, default=30)
-    ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--cfl", type=float, default=0.5, help="dt = cfl * min edge length / sqrt(g * max depth)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--cpu-layers", type=int, default=30, help="layers of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -63,11 +65,24 @@ def main():
     import wavetrisk_hydrostatic_amd as W
     from wavetrisk_hydrostatic_amd import cases
 
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copy calibration {args.copy_calibration} MiB read + written 3x", file=sys.stderr, flush=True)
+
     J, L = args.level, args.layers
     print(f"bench: building J{J} grid", file=sys.stderr, flush=True)
     grid = W.Grid(level=J, min_level=J, domain_level=2)
     lay = grid.layout
     topo, mass, temp, velo = cases.ocean(grid, L, seed=2026)
+    own_e = lay.owned_edge_cells()
+    args.dt = args.cfl * float(grid.array("len")[:, own_e].min()) / float(np.sqrt(9.80616 * np.max(-topo)))
     ctx = W.Context(grid, nlayers=L)
     ctx.set_topography(topo)
     ctx.upload_state(0, mass, temp, velo)
@@ -79,7 +94,7 @@ def main():
     ctx.synchronize()
     m_out, _, u_out = ctx.download_state(0)
     own = lay.owned_cells()
-    if not (np.isfinite(m_out[:, own]).all() and np.isfinite(u_out[:, :, lay.owned_edge_cells()]).all()):
+    if not (np.isfinite(m_out[:, own]).all() and np.isfinite(u_out[:, :, own_e]).all()):
         raise RuntimeError("non-finite state after the timed steps")
 
     dof_per_step = L * lay.n_dof_per_layer
@@ -120,7 +135,7 @@ def main():
         "data": "synthetic (seeded stratified ocean, analytic fields on the grid)",
         "config": {
             "workload": f"J{J} uniform icosahedral grid, L{L} Boussinesq layers, explicit RK4 of trend_ml "
-                        f"(TRiSK), FP64, dt={args.dt}s; no free-surface solve in this path",
+                        f"(TRiSK), FP64, dt={args.dt:.1f}s (CFL {args.cfl}); no free-surface solve in this path",
             "level": J, "layers": L, "nodes": 10 * lay.n * lay.n + 2, "edges": 30 * lay.n * lay.n,
             "dof_per_step": dof_per_step,
         },

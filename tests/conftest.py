@@ -23,9 +23,11 @@ def golden(name):
 
 
 def maxrel(a, b):
-    """max |a-b| / max |b| over finite entries of b (normwise relative error)."""
+    """max |a-b| / max |b| (normwise relative error); NaN entries of b are
+    "not compared", and a must be finite wherever b is."""
     a, b = np.asarray(a), np.asarray(b)
     m = np.isfinite(b)
+    assert np.isfinite(a[m]).all(), "non-finite values where the reference is finite"
     scale = np.max(np.abs(b[m])) if m.any() else 0.0
     err = np.max(np.abs(a[m] - b[m])) if m.any() else 0.0
     return err / scale if scale > 0 else err
