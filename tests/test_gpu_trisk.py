@@ -126,3 +126,28 @@ def test_errors_are_raised():
         ctx.rk4(0, 1.0, 1)
     with pytest.raises(W.WtgError, match="slot"):
         ctx.trend(0, 7)
+
+
+@pytest.mark.parametrize("version", ["1", "2", "3"])
+def test_stage_kernel_variants_agree(version, monkeypatch):
+    """Every selectable stage kernel (WTG_STAGE_KERNEL) gives the oracle's
+    RK4 step to the last bit, including the pole and pentagon-corner tiles."""
+    monkeypatch.setenv("WTG_STAGE_KERNEL", version)
+    print(f"stage kernel {version}", flush=True)
+    g = W.Grid(level=6, min_level=6, domain_level=2)
+    topo, m, t, u = cases.ocean(g, 4, seed=11)
+    ctx = W.Context(g, nlayers=4)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.rk4(0, 600.0, 2)
+    got = ctx.download_state(0)
+    ctx.trend(0, 1)
+    got_trend = ctx.download_state(1)
+    o = oracle_of(g, topo)
+    m, t, u = m.copy(), t.copy(), u.copy()
+    o.ghost_fill(m, t, u)
+    want = o.rk4(m, t, u, 600.0, 2)
+    _cmp_state(got, want, g.layout, 0.0, f"kernel {version} rk4 vs oracle")
+    mw, tw, uw = (x.copy() for x in want)
+    o.ghost_fill(mw, tw, uw)
+    _cmp_state(got_trend, o.trend(mw, tw, uw), g.layout, 0.0, f"kernel {version} trend vs oracle")

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -366,6 +367,333 @@ __global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// trisk_stage_kernel2: same arithmetic as trisk_stage_kernel, organised for
+// throughput.  One thread per cell of the 19 x 19 staged region (384 threads,
+// 361 active), fixed for the whole launch, so every layer-invariant
+// geometric coefficient a cell needs -- its own lengths, areas and Coriolis
+// terms and the neighbour-derived ones (the Qperp weights of E, NE and N,
+// the area parts of the triangle vertices, the lengths of the W/SW/S edges
+// of its hexagon) -- is loaded once into registers before the layer loop.
+// Per layer only the state is read from HBM; scalar fluxes of both tracers
+// go through LDS so the owned cells read them instead of recomputing.
+// ---------------------------------------------------------------------------
+constexpr int THREADS2 = 384;
+
+struct Smem2 {
+  double m[RR], t[RR], u[3][RR];
+  double pvL[RR], pvU[RR];
+  double qe[3][RR], hm[3][RR], ht[3][RR];
+  double bern[RR], exn[RR];
+};
+
+__device__ __forceinline__ void weights_from(double a1, double a2, double a3, double a4, double a5, double a6,
+                                             double hi, int offs, double w[5]) {
+  const double a[6] = {a1, a2, a3, a4, a5, a6};
+  double acc0 = a[offs];
+  double acc1 = acc0 + a[(1 + offs) % 6];
+  double acc2 = acc1 + a[(2 + offs) % 6];
+  double acc3 = acc2 + a[(3 + offs) % 6];
+  double acc4 = acc3 + a[(4 + offs) % 6];
+  w[0] = 0.5 - acc0 * hi;
+  w[1] = -(0.5 - acc1 * hi);
+  w[2] = 0.5 - acc2 * hi;
+  w[3] = -(0.5 - acc3 * hi);
+  w[4] = 0.5 - acc4 * hi;
+}
+
+template <int MINW>
+__global__ __launch_bounds__(THREADS2, MINW) void trisk_stage_kernel2(StageArgs a) {
+  __shared__ Smem2 sm;
+  const int tid = threadIdx.x;
+  const int64_t NC = a.NC;
+  const int S = a.S;
+  const Geo& g = a.g;
+
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int tpl = a.ntile * a.ntile;
+  const int lz = tile / tpl;
+  const int rem = tile % tpl;
+  const int i0 = (rem % a.ntile) * T, j0 = (rem / a.ntile) * T;
+  const int64_t base = (int64_t)lz * S * S + (int64_t)(j0 - 1 + H) * S + (i0 - 1 + H);
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+  auto r = [](int ri, int rj) { return rj * R + ri; };
+
+  const bool cSW = (i0 == 0 && j0 == 0), cSE = (i0 + T == a.n && j0 == 0);
+  const bool cNW = (i0 == 0 && j0 + T == a.n), cNE = (i0 + T == a.n && j0 + T == a.n);
+  const bool corner = cSW || cSE || cNW || cNE;
+
+  // this thread's cell and roles
+  const bool active = tid < RR;
+  const int ri = tid % R, rj = tid / R;
+  const int x = tid;
+  const int64_t gc = active ? gcell(ri, rj) : base;
+  const bool tri_cell = active && ri <= T + 1 && rj <= T + 1;                   // PV, fluxes, qe
+  const bool p_cell = active && ri >= 1 && rj >= 1 && ri <= T + 1 && rj <= T + 1;  // pressure, KE, Bernoulli
+  const bool own = active && ri >= 1 && rj >= 1 && ri <= T && rj <= T;
+  const bool pole = (lz == 0 && cSE && ri == T + 1 && rj == 1) || (lz == 5 && cNW && ri == 1 && rj == T + 1);
+
+  // --- layer-invariant geometry, once per launch ------------------------------
+  double len0 = 0, len1 = 0, len2 = 0, ped0 = 0, ped1 = 0, ped2 = 0, pf0 = 0, pf1 = 0, pf2 = 0;
+  double A1 = 0, A2 = 0, A3 = 0, A4 = 0, A5 = 0, A6 = 0, hi = 0, corL = 0, corU = 0;
+  double lenUPE = 0, lenRTN = 0, A3E = 0, A5NE = 0, A4NE = 0, A6N = 0;
+  double lenRTW = 0, pedRTW = 0, lenDGSW = 0, pedDGSW = 0, lenUPS = 0, pedUPS = 0;
+  double w2E[5] = {0, 0, 0, 0, 0}, w2NE[5] = {0, 0, 0, 0, 0}, w2N[5] = {0, 0, 0, 0, 0};
+  if (tri_cell) {
+    len0 = g.len[gc]; len1 = g.len[NC + gc]; len2 = g.len[2 * NC + gc];
+    pf0 = g.pedf[gc]; pf1 = g.pedf[NC + gc]; pf2 = g.pedf[2 * NC + gc];
+    A1 = g.area[gc]; A2 = g.area[NC + gc];
+    corL = g.cor[gc]; corU = g.cor[NC + gc];
+    const int64_t gE = gcell(ri + 1, rj), gN = gcell(ri, rj + 1), gNE = gcell(ri + 1, rj + 1);
+    lenUPE = g.len[2 * NC + gE]; lenRTN = g.len[gN];
+    A3E = g.area[2 * NC + gE]; A5NE = g.area[4 * NC + gNE]; A4NE = g.area[3 * NC + gNE]; A6N = g.area[5 * NC + gN];
+  }
+  if (p_cell) {
+    ped0 = g.ped[gc]; ped1 = g.ped[NC + gc]; ped2 = g.ped[2 * NC + gc];
+    A3 = g.area[2 * NC + gc]; A4 = g.area[3 * NC + gc]; A5 = g.area[4 * NC + gc]; A6 = g.area[5 * NC + gc];
+    hi = g.hinv[gc];
+    const int64_t gW = gcell(ri - 1, rj), gS = gcell(ri, rj - 1), gSW = gcell(ri - 1, rj - 1);
+    lenRTW = g.len[gW]; pedRTW = g.ped[gW];
+    lenDGSW = g.len[NC + gSW]; pedDGSW = g.ped[NC + gSW];
+    lenUPS = g.len[2 * NC + gS]; pedUPS = g.ped[2 * NC + gS];
+  }
+  if (own) {
+    weights(g, NC, gcell(ri + 1, rj), 3, w2E);
+    weights(g, NC, gcell(ri + 1, rj + 1), 4, w2NE);
+    weights(g, NC, gcell(ri, rj + 1), 5, w2N);
+  }
+
+  // column state (cal_surf_press, set_surf_geopot)
+  double pl = 0, gp = 0, pr = 0, gpl = 0;
+  if (p_cell) {
+    double s = 0.0;
+    for (int k = 0; k < a.L; ++k) s = s + (a.ym[(int64_t)k * NC + gc] - a.yt[(int64_t)k * NC + gc]);
+    pl = a.grav * s + a.p_top;
+    gp = a.grav * g.topo[gc];
+  }
+
+  // state of layer 0; later layers are prefetched one layer ahead
+  double m = 0, t = 0, u0 = 0, u1 = 0, u2 = 0;
+  if (active) {
+    m = a.ym[gc];
+    t = a.yt[gc];
+    u0 = a.yu[gc];
+    u1 = a.yu[NC + gc];
+    u2 = a.yu[2 * NC + gc];
+  }
+
+  for (int k = 0; k < a.L; ++k) {
+    const int64_t ks = (int64_t)k * NC, ke3 = (int64_t)k * 3 * NC;
+    // --- 1. stage own cell; integrate pressure upward ------------------------
+    if (active) {
+      sm.m[x] = m;
+      sm.t[x] = t;
+      sm.u[0][x] = u0;
+      sm.u[1][x] = u1;
+      sm.u[2][x] = u2;
+    }
+    double nm = 0, nt = 0, nu0 = 0, nu1 = 0, nu2 = 0;
+    if (active && k + 1 < a.L) {
+      const int64_t ks1 = ks + NC, ke31 = ke3 + 3 * NC;
+      nm = a.ym[ks1 + gc];
+      nt = a.yt[ks1 + gc];
+      nu0 = a.yu[ke31 + gc];
+      nu1 = a.yu[ke31 + NC + gc];
+      nu2 = a.yu[ke31 + 2 * NC + gc];
+    }
+    // base state of the update, consumed in phase 4
+    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
+    if (!a.trend_only) {
+      if (own || pole) {
+        bm = a.qm[ks + gc];
+        bt = a.qt[ks + gc];
+      }
+      if (own) {
+        bu0 = a.qu[ke3 + gc];
+        bu1 = a.qu[ke3 + NC + gc];
+        bu2 = a.qu[ke3 + 2 * NC + gc];
+      }
+    }
+    if (p_cell) {
+      gpl = gp;
+      double p_upper = pl - a.grav * (m - t);
+      pr = interp(pl, p_upper);
+      gp = gpl + a.grav * m / (a.rho0 * 1.0);
+      pl = p_upper;
+    }
+    __syncthreads();
+
+    // --- 2. PV on the cell's two triangles; mass and tracer fluxes ------------
+    if (tri_cell) {
+      const int xE = x + 1, xN = x + R, xNE = x + R + 1;
+      double upRT = u0 * len0, upDG = u1 * len1, upUP = u2 * len2;
+      double circL = upRT + sm.u[2][xE] * lenUPE + upDG;
+      double circU = -(upDG + upUP + sm.u[0][xN] * lenRTN);
+      double mE = sm.m[xE], mNE = sm.m[xNE], mN = sm.m[xN];
+      double massL = m * A1 + mE * A3E + mNE * A5NE;
+      double massU = m * A2 + mNE * A4NE + mN * A6N;
+      sm.pvL[x] = (corL + circL) / massL;
+      sm.pvU[x] = (corU + circU) / massU;
+      sm.hm[0][x] = u0 * pf0 * interp(m, mE) + 0.0;
+      sm.hm[1][x] = u1 * pf1 * interp(m, mNE) + 0.0;
+      sm.hm[2][x] = u2 * pf2 * interp(m, mN) + 0.0;
+      sm.ht[0][x] = u0 * pf0 * interp(t, sm.t[xE]) + 0.0;
+      sm.ht[1][x] = u1 * pf1 * interp(t, sm.t[xNE]) + 0.0;
+      sm.ht[2][x] = u2 * pf2 * interp(t, sm.t[xN]) + 0.0;
+    }
+    __syncthreads();
+
+    // --- 3. qe; kinetic energy, Bernoulli, Exner ------------------------------
+    if (tri_cell) {
+      if (rj >= 1) sm.qe[0][x] = interp(sm.pvU[x - R], sm.pvL[x]);
+      sm.qe[1][x] = interp(sm.pvU[x], sm.pvL[x]);
+      if (ri >= 1) sm.qe[2][x] = interp(sm.pvU[x], sm.pvL[x - 1]);
+    }
+    if (p_cell) {
+      double uS = sm.u[2][x - R], uSW = sm.u[1][x - R - 1], uW = sm.u[0][x - 1];
+      double ke = ((u2 * len2) * (u2 * ped2) + (u1 * len1) * (u1 * ped1) + (u0 * len0) * (u0 * ped0) +
+                   (uS * lenUPS) * (uS * pedUPS) + (uSW * lenDGSW) * (uSW * pedDGSW) + (uW * lenRTW) * (uW * pedRTW)) *
+                  hi / 4.0;
+      double phi_k = interp(gp, gpl);
+      sm.bern[x] = ke + phi_k + pr / (a.rho0 * 1.0);
+      sm.exn[x] = -phi_k;
+    }
+    __syncthreads();
+
+    // --- 3b. pentagon corners (post_step1) -------------------------------------
+    if (corner) {
+      if (tid < 4) {
+        auto UPL = [&](int e, int i, int j) { return sm.u[e][r(i, j)] * g.len[(int64_t)e * NC + gcell(i, j)]; };
+        auto M = [&](int i, int j) { return sm.m[r(i, j)]; };
+        auto AR = [&](int p, int i, int j) { return g.area[(int64_t)(p - 1) * NC + gcell(i, j)]; };
+        auto TRI = [&](int tt, int i, int j) { return g.tri[(int64_t)tt * NC + gcell(i, j)]; };
+        auto COR = [&](int tt, int i, int j) { return g.cor[(int64_t)tt * NC + gcell(i, j)]; };
+        if (tid == 0 && cSW) {
+          const int i = 1, j = 1;
+          double cSW_L = UPL(2, i, j - 1) - UPL(0, i - 1, j) + UPL(0, i - 1, j - 1);
+          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+          double cS_U = (-(UPL(0, i, j) + UPL(1, i, j - 1) + UPL(2, i, j - 1)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+          double m1 = M(i - 1, j) * AR(6, i - 1, j) + M(i, j) * (AR(4, i, j) + AR(5, i, j)) + M(i, j - 1) * AR(3, i, j - 1);
+          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+          double pvWL = (COR(0, i - 1, j) + cW_L) / m1;
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m2;
+          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+          sm.qe[0][r(i - 1, j)] = interp(pvWL, pvSWL);
+          sm.qe[2][r(i, j - 1)] = interp(pvSU, pvSWL);
+        }
+        if (tid == 1 && cSE) {
+          const int i = T + 1, j = 1;
+          double cSW_L = -UPL(0, i, j) + UPL(0, i - 1, j - 1) + UPL(1, i - 1, j - 1);
+          double cL = ((UPL(0, i, j) + UPL(2, i + 1, j) + UPL(1, i, j)) / TRI(0, i, j)) * TRI(0, i, j);
+          double cSW_U = (-(UPL(0, i - 1, j) + UPL(1, i - 1, j - 1) + UPL(2, i - 1, j - 1)) / TRI(1, i - 1, j - 1)) *
+                         TRI(1, i - 1, j - 1);
+          double m1 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) +
+                      M(i, j) * (AR(5, i, j) + AR(6, i, j));
+          double m2 = M(i, j) * AR(1, i, j) + M(i + 1, j) * AR(3, i + 1, j) + M(i + 1, j + 1) * AR(5, i + 1, j + 1);
+          double m3 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * AR(4, i, j) + M(i - 1, j) * AR(6, i - 1, j);
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m1;
+          double pvL = (COR(0, i, j) + cL) / m2;
+          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m3;
+          sm.qe[0][r(i, j)] = interp(pvSWL, pvL);
+          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+        }
+        if (tid == 2 && cNW) {
+          const int i = 1, j = T + 1;
+          double cSW_U = UPL(2, i, j) - UPL(1, i - 1, j - 1) - UPL(2, i - 1, j - 1);
+          double cU = (-(UPL(1, i, j) + UPL(2, i, j) + UPL(0, i, j + 1)) / TRI(1, i, j)) * TRI(1, i, j);
+          double cSW_L = ((UPL(0, i - 1, j - 1) + UPL(2, i, j - 1) + UPL(1, i - 1, j - 1)) / TRI(0, i - 1, j - 1)) *
+                         TRI(0, i - 1, j - 1);
+          double m1 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * (AR(3, i, j) + AR(4, i, j)) +
+                      M(i - 1, j) * AR(6, i - 1, j);
+          double m2 = M(i, j) * AR(2, i, j) + M(i + 1, j + 1) * AR(4, i + 1, j + 1) + M(i, j + 1) * AR(6, i, j + 1);
+          double m3 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) + M(i, j) * AR(5, i, j);
+          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m1;
+          double pvU = (COR(1, i, j) + cU) / m2;
+          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m3;
+          sm.qe[2][r(i, j)] = interp(pvSWU, pvU);
+          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+        }
+        if (tid == 3 && cNE) {
+          const int i = T + 1, j = T + 1;
+          double cL = UPL(0, i, j) - UPL(0, i, j + 1) - UPL(2, i, j);
+          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+          double cS_U = (-(UPL(1, i, j - 1) + UPL(2, i, j - 1) + UPL(0, i, j)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+          double m1 = M(i + 1, j) * AR(3, i + 1, j) + M(i, j) * (AR(1, i, j) + AR(2, i, j)) + M(i, j + 1) * AR(6, i, j + 1);
+          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+          double pvL = (COR(0, i, j) + cL) / m1;
+          double pvWL = (COR(0, i - 1, j) + cW_L) / m2;
+          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+          sm.qe[0][r(i, j)] = interp(pvL, pvSU);
+          sm.qe[2][r(i, j)] = interp(pvL, pvWL);
+        }
+      }
+      __syncthreads();
+    }
+
+    // --- 4. trends of owned nodes/edges and the pole master; RK update --------
+    if (own || pole) {
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+      double divm = (sm.hm[0][x] - sm.hm[0][xW] + sm.hm[1][xSW] - sm.hm[1][x] + sm.hm[2][x] - sm.hm[2][xS]) * hi;
+      double divt = (sm.ht[0][x] - sm.ht[0][xW] + sm.ht[1][xSW] - sm.ht[1][x] + sm.ht[2][x] - sm.ht[2][xS]) * hi;
+      double dmv = -divm, dtv = -divt;
+      a.om[ks + gc] = a.trend_only ? dmv : bm + a.h * dmv;
+      a.ot[ks + gc] = a.trend_only ? dtv : bt + a.h * dtv;
+    }
+    if (own) {
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1, xE = x + 1, xN = x + R, xNE = x + R + 1;
+      const int xNW = x + R - 1, xSE = x - R + 1;
+      double w1[5], qp[3];
+#define HM(e, xx) sm.hm[e][xx]
+#define QE(e, xx) sm.qe[e][xx]
+      weights_from(A1, A2, A3, A4, A5, A6, hi, 0, w1);
+      double qr = QE(0, x);
+      qp[0] = HM(1, x) * interp(QE(1, x), qr) * w1[0] + HM(2, x) * interp(QE(2, x), qr) * w1[1] +
+              HM(0, xW) * interp(QE(0, xW), qr) * w1[2] + HM(1, xSW) * interp(QE(1, xSW), qr) * w1[3] +
+              HM(2, xS) * interp(QE(2, xS), qr) * w1[4] + HM(1, xS) * interp(QE(1, xS), qr) * w2E[0] +
+              HM(2, xSE) * interp(QE(2, xSE), qr) * w2E[1] + HM(0, xE) * interp(QE(0, xE), qr) * w2E[2] +
+              HM(1, xE) * interp(QE(1, xE), qr) * w2E[3] + HM(2, xE) * interp(QE(2, xE), qr) * w2E[4];
+      weights_from(A1, A2, A3, A4, A5, A6, hi, 1, w1);
+      double qd = QE(1, x);
+      qp[1] = HM(2, x) * interp(QE(2, x), qd) * w1[0] + HM(0, xW) * interp(QE(0, xW), qd) * w1[1] +
+              HM(1, xSW) * interp(QE(1, xSW), qd) * w1[2] + HM(2, xS) * interp(QE(2, xS), qd) * w1[3] +
+              HM(0, x) * interp(QE(0, x), qd) * w1[4] + HM(2, xE) * interp(QE(2, xE), qd) * w2NE[0] +
+              HM(0, xNE) * interp(QE(0, xNE), qd) * w2NE[1] + HM(1, xNE) * interp(QE(1, xNE), qd) * w2NE[2] +
+              HM(2, xNE) * interp(QE(2, xNE), qd) * w2NE[3] + HM(0, xN) * interp(QE(0, xN), qd) * w2NE[4];
+      weights_from(A1, A2, A3, A4, A5, A6, hi, 2, w1);
+      double qu = QE(2, x);
+      qp[2] = HM(0, xW) * interp(QE(0, xW), qu) * w1[0] + HM(1, xSW) * interp(QE(1, xSW), qu) * w1[1] +
+              HM(2, xS) * interp(QE(2, xS), qu) * w1[2] + HM(0, x) * interp(QE(0, x), qu) * w1[3] +
+              HM(1, x) * interp(QE(1, x), qu) * w1[4] + HM(0, xN) * interp(QE(0, xN), qu) * w2N[0] +
+              HM(1, xN) * interp(QE(1, xN), qu) * w2N[1] + HM(2, xN) * interp(QE(2, xN), qu) * w2N[2] +
+              HM(0, xNW) * interp(QE(0, xNW), qu) * w2N[3] + HM(1, xW) * interp(QE(1, xW), qu) * w2N[4];
+#undef HM
+#undef QE
+      double th0 = t / m, thE = sm.t[xE] / sm.m[xE], thNE = sm.t[xNE] / sm.m[xNE], thN = sm.t[xN] / sm.m[xN];
+      double the[3] = {interp(th0, thE), interp(th0, thNE), interp(th0, thN)};
+      double b0 = sm.bern[x], x0 = sm.exn[x];
+      double gB[3] = {(sm.bern[xE] - b0) / len0, (b0 - sm.bern[xNE]) / len1, (sm.bern[xN] - b0) / len2};
+      double gX[3] = {(sm.exn[xE] - x0) / len0, (x0 - sm.exn[xNE]) / len1, (sm.exn[xN] - x0) / len2};
+      double ll[3] = {len0, len1, len2};
+      double bu[3] = {bu0, bu1, bu2};
+      for (int e = 0; e < 3; ++e) {
+        double src = -qp[e] + 0.0 * ll[e];
+        double du = src / ll[e] - gB[e] - the[e] * gX[e];
+        a.ou[ke3 + (int64_t)e * NC + gc] = a.trend_only ? du : bu[e] + a.h * du;
+      }
+    }
+    m = nm;
+    t = nt;
+    u0 = nu0;
+    u1 = nu1;
+    u2 = nu2;
+    __syncthreads();
+  }
+}
+
 // halo := owner (nodes: mass and temp; edges: velocity with sign), all layers
 __global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__ nd, int nn,
                                   const int64_t* __restrict__ ed, const int8_t* __restrict__ es, int ne, double* m,
@@ -422,6 +750,7 @@ struct wtg_ctx {
   wtg_grid_info info{};
   wtg_params params{};
   int nslots = 0;
+  int stage_version = 2;  // WTG_STAGE_KERNEL: 1 = first (LDS-loop) version, 2/3 = per-cell version at 2/3 waves per SIMD
   bool topo_set = false;
   std::string err;
   // device geometry
@@ -526,7 +855,12 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
   a.rho0 = c->params.ref_density;
   a.p_top = c->params.p_top;
   unsigned blocks = (unsigned)(10 * a.ntile * a.ntile);
-  hipLaunchKernelGGL(trisk_stage_kernel, dim3(blocks), dim3(THREADS), 0, c->stream, a);
+  if (c->stage_version == 1)
+    hipLaunchKernelGGL(trisk_stage_kernel, dim3(blocks), dim3(THREADS), 0, c->stream, a);
+  else if (c->stage_version == 3)
+    hipLaunchKernelGGL(trisk_stage_kernel2<3>, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(trisk_stage_kernel2<2>, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -630,6 +964,10 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
   wtg_grid_info_get(grid, &c->info);
   c->params = *params;
   c->nslots = nslots;
+  if (const char* v = std::getenv("WTG_STAGE_KERNEL")) {
+    int sv = std::atoi(v);
+    c->stage_version = (sv == 1 || sv == 3) ? sv : 2;
+  }
   const auto& g = grid->lv;
   const int64_t NC = c->info.ncells;
   // flux dual lengths: a halo edge that is ghost filled carries its owner's flux
