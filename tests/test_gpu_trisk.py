@@ -130,13 +130,15 @@ def test_errors_are_raised():
 
 @pytest.mark.parametrize("version", ["1", "2", "3"])
 def test_stage_kernel_variants_agree(version, monkeypatch):
-    """Every selectable stage kernel (WTG_STAGE_KERNEL) gives the oracle's
-    RK4 step to the last bit, including the pole and pentagon-corner tiles."""
+    """Every selectable stage kernel (WTG_STAGE_KERNEL: 1 tile loop, 2 one
+    thread per region cell, 3 layer pairs) gives the oracle's RK4 step to the
+    last bit, including the pole and pentagon-corner tiles; 5 layers make the
+    last layer pair of kernel 3 a single layer."""
     monkeypatch.setenv("WTG_STAGE_KERNEL", version)
     print(f"stage kernel {version}", flush=True)
     g = W.Grid(level=6, min_level=6, domain_level=2)
-    topo, m, t, u = cases.ocean(g, 4, seed=11)
-    ctx = W.Context(g, nlayers=4)
+    topo, m, t, u = cases.ocean(g, 5, seed=11)
+    ctx = W.Context(g, nlayers=5)
     ctx.set_topography(topo)
     ctx.upload_state(0, m, t, u)
     ctx.rk4(0, 600.0, 2)

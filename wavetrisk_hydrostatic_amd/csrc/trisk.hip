@@ -367,6 +367,81 @@ __global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a) {
   }
 }
 
+// post_step1 (ops.f90:526-737): at the four lozenge corners the PV of the
+// triangles next to the pentagon is recomputed with the pentagon's
+// circulation and mass, and two qe values per corner are overwritten.
+// `which` = 0..3 for the SW, SE, NW, NE corner; region cell (1,1), (T+1,1),
+// (1,T+1), (T+1,T+1).  u*, m, qe* are LDS planes of the 19 x 19 region.
+__device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const double* __restrict__ u1p,
+                               const double* __restrict__ u2p, const double* __restrict__ mp, double* qe0,
+                               double* qe1, double* qe2, const Geo& g, int64_t NC, int64_t base, int S) {
+  auto r = [](int ri, int rj) { return rj * R + ri; };
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+  const double* up[3] = {u0p, u1p, u2p};
+  auto UPL = [&](int e, int i, int j) { return up[e][r(i, j)] * g.len[(int64_t)e * NC + gcell(i, j)]; };
+  auto M = [&](int i, int j) { return mp[r(i, j)]; };
+  auto AR = [&](int p, int i, int j) { return g.area[(int64_t)(p - 1) * NC + gcell(i, j)]; };
+  auto TRI = [&](int tt, int i, int j) { return g.tri[(int64_t)tt * NC + gcell(i, j)]; };
+  auto COR = [&](int tt, int i, int j) { return g.cor[(int64_t)tt * NC + gcell(i, j)]; };
+  if (which == 0) {  // IJMINUS
+    const int i = 1, j = 1;
+    double cSW_L = UPL(2, i, j - 1) - UPL(0, i - 1, j) + UPL(0, i - 1, j - 1);
+    double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+    double cS_U = (-(UPL(0, i, j) + UPL(1, i, j - 1) + UPL(2, i, j - 1)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+    double m1 = M(i - 1, j) * AR(6, i - 1, j) + M(i, j) * (AR(4, i, j) + AR(5, i, j)) + M(i, j - 1) * AR(3, i, j - 1);
+    double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+    double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+    double pvWL = (COR(0, i - 1, j) + cW_L) / m1;
+    double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m2;
+    double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+    qe0[r(i - 1, j)] = interp(pvWL, pvSWL);
+    qe2[r(i, j - 1)] = interp(pvSU, pvSWL);
+  } else if (which == 1) {  // IPLUSJMINUS
+    const int i = T + 1, j = 1;
+    double cSW_L = -UPL(0, i, j) + UPL(0, i - 1, j - 1) + UPL(1, i - 1, j - 1);
+    double cL = ((UPL(0, i, j) + UPL(2, i + 1, j) + UPL(1, i, j)) / TRI(0, i, j)) * TRI(0, i, j);
+    double cSW_U = (-(UPL(0, i - 1, j) + UPL(1, i - 1, j - 1) + UPL(2, i - 1, j - 1)) / TRI(1, i - 1, j - 1)) *
+                   TRI(1, i - 1, j - 1);
+    double m1 =
+        M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) + M(i, j) * (AR(5, i, j) + AR(6, i, j));
+    double m2 = M(i, j) * AR(1, i, j) + M(i + 1, j) * AR(3, i + 1, j) + M(i + 1, j + 1) * AR(5, i + 1, j + 1);
+    double m3 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * AR(4, i, j) + M(i - 1, j) * AR(6, i - 1, j);
+    double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m1;
+    double pvL = (COR(0, i, j) + cL) / m2;
+    double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m3;
+    qe0[r(i, j)] = interp(pvSWL, pvL);
+    qe1[r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+  } else if (which == 2) {  // IMINUSJPLUS
+    const int i = 1, j = T + 1;
+    double cSW_U = UPL(2, i, j) - UPL(1, i - 1, j - 1) - UPL(2, i - 1, j - 1);
+    double cU = (-(UPL(1, i, j) + UPL(2, i, j) + UPL(0, i, j + 1)) / TRI(1, i, j)) * TRI(1, i, j);
+    double cSW_L = ((UPL(0, i - 1, j - 1) + UPL(2, i, j - 1) + UPL(1, i - 1, j - 1)) / TRI(0, i - 1, j - 1)) *
+                   TRI(0, i - 1, j - 1);
+    double m1 =
+        M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * (AR(3, i, j) + AR(4, i, j)) + M(i - 1, j) * AR(6, i - 1, j);
+    double m2 = M(i, j) * AR(2, i, j) + M(i + 1, j + 1) * AR(4, i + 1, j + 1) + M(i, j + 1) * AR(6, i, j + 1);
+    double m3 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) + M(i, j) * AR(5, i, j);
+    double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m1;
+    double pvU = (COR(1, i, j) + cU) / m2;
+    double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m3;
+    qe2[r(i, j)] = interp(pvSWU, pvU);
+    qe1[r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+  } else {  // IJPLUS
+    const int i = T + 1, j = T + 1;
+    double cL = UPL(0, i, j) - UPL(0, i, j + 1) - UPL(2, i, j);
+    double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
+    double cS_U = (-(UPL(1, i, j - 1) + UPL(2, i, j - 1) + UPL(0, i, j)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
+    double m1 = M(i + 1, j) * AR(3, i + 1, j) + M(i, j) * (AR(1, i, j) + AR(2, i, j)) + M(i, j + 1) * AR(6, i, j + 1);
+    double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
+    double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
+    double pvL = (COR(0, i, j) + cL) / m1;
+    double pvWL = (COR(0, i - 1, j) + cW_L) / m2;
+    double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
+    qe0[r(i, j)] = interp(pvL, pvSU);
+    qe2[r(i, j)] = interp(pvL, pvWL);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // trisk_stage_kernel2: same arithmetic as trisk_stage_kernel, organised for
 // throughput.  One thread per cell of the 19 x 19 staged region (384 threads,
@@ -402,8 +477,7 @@ __device__ __forceinline__ void weights_from(double a1, double a2, double a3, do
   w[4] = 0.5 - acc4 * hi;
 }
 
-template <int MINW>
-__global__ __launch_bounds__(THREADS2, MINW) void trisk_stage_kernel2(StageArgs a) {
+__global__ __launch_bounds__(THREADS2, 2) void trisk_stage_kernel2(StageArgs a) {
   __shared__ Smem2 sm;
   const int tid = threadIdx.x;
   const int64_t NC = a.NC;
@@ -564,73 +638,9 @@ __global__ __launch_bounds__(THREADS2, MINW) void trisk_stage_kernel2(StageArgs 
 
     // --- 3b. pentagon corners (post_step1) -------------------------------------
     if (corner) {
-      if (tid < 4) {
-        auto UPL = [&](int e, int i, int j) { return sm.u[e][r(i, j)] * g.len[(int64_t)e * NC + gcell(i, j)]; };
-        auto M = [&](int i, int j) { return sm.m[r(i, j)]; };
-        auto AR = [&](int p, int i, int j) { return g.area[(int64_t)(p - 1) * NC + gcell(i, j)]; };
-        auto TRI = [&](int tt, int i, int j) { return g.tri[(int64_t)tt * NC + gcell(i, j)]; };
-        auto COR = [&](int tt, int i, int j) { return g.cor[(int64_t)tt * NC + gcell(i, j)]; };
-        if (tid == 0 && cSW) {
-          const int i = 1, j = 1;
-          double cSW_L = UPL(2, i, j - 1) - UPL(0, i - 1, j) + UPL(0, i - 1, j - 1);
-          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
-          double cS_U = (-(UPL(0, i, j) + UPL(1, i, j - 1) + UPL(2, i, j - 1)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
-          double m1 = M(i - 1, j) * AR(6, i - 1, j) + M(i, j) * (AR(4, i, j) + AR(5, i, j)) + M(i, j - 1) * AR(3, i, j - 1);
-          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
-          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
-          double pvWL = (COR(0, i - 1, j) + cW_L) / m1;
-          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m2;
-          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
-          sm.qe[0][r(i - 1, j)] = interp(pvWL, pvSWL);
-          sm.qe[2][r(i, j - 1)] = interp(pvSU, pvSWL);
-        }
-        if (tid == 1 && cSE) {
-          const int i = T + 1, j = 1;
-          double cSW_L = -UPL(0, i, j) + UPL(0, i - 1, j - 1) + UPL(1, i - 1, j - 1);
-          double cL = ((UPL(0, i, j) + UPL(2, i + 1, j) + UPL(1, i, j)) / TRI(0, i, j)) * TRI(0, i, j);
-          double cSW_U = (-(UPL(0, i - 1, j) + UPL(1, i - 1, j - 1) + UPL(2, i - 1, j - 1)) / TRI(1, i - 1, j - 1)) *
-                         TRI(1, i - 1, j - 1);
-          double m1 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) +
-                      M(i, j) * (AR(5, i, j) + AR(6, i, j));
-          double m2 = M(i, j) * AR(1, i, j) + M(i + 1, j) * AR(3, i + 1, j) + M(i + 1, j + 1) * AR(5, i + 1, j + 1);
-          double m3 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * AR(4, i, j) + M(i - 1, j) * AR(6, i - 1, j);
-          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m1;
-          double pvL = (COR(0, i, j) + cL) / m2;
-          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m3;
-          sm.qe[0][r(i, j)] = interp(pvSWL, pvL);
-          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
-        }
-        if (tid == 2 && cNW) {
-          const int i = 1, j = T + 1;
-          double cSW_U = UPL(2, i, j) - UPL(1, i - 1, j - 1) - UPL(2, i - 1, j - 1);
-          double cU = (-(UPL(1, i, j) + UPL(2, i, j) + UPL(0, i, j + 1)) / TRI(1, i, j)) * TRI(1, i, j);
-          double cSW_L = ((UPL(0, i - 1, j - 1) + UPL(2, i, j - 1) + UPL(1, i - 1, j - 1)) / TRI(0, i - 1, j - 1)) *
-                         TRI(0, i - 1, j - 1);
-          double m1 = M(i - 1, j - 1) * AR(2, i - 1, j - 1) + M(i, j) * (AR(3, i, j) + AR(4, i, j)) +
-                      M(i - 1, j) * AR(6, i - 1, j);
-          double m2 = M(i, j) * AR(2, i, j) + M(i + 1, j + 1) * AR(4, i + 1, j + 1) + M(i, j + 1) * AR(6, i, j + 1);
-          double m3 = M(i - 1, j - 1) * AR(1, i - 1, j - 1) + M(i, j - 1) * AR(3, i, j - 1) + M(i, j) * AR(5, i, j);
-          double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m1;
-          double pvU = (COR(1, i, j) + cU) / m2;
-          double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m3;
-          sm.qe[2][r(i, j)] = interp(pvSWU, pvU);
-          sm.qe[1][r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
-        }
-        if (tid == 3 && cNE) {
-          const int i = T + 1, j = T + 1;
-          double cL = UPL(0, i, j) - UPL(0, i, j + 1) - UPL(2, i, j);
-          double cW_L = ((UPL(0, i - 1, j) + UPL(2, i, j) + UPL(1, i - 1, j)) / TRI(0, i - 1, j)) * TRI(0, i - 1, j);
-          double cS_U = (-(UPL(1, i, j - 1) + UPL(2, i, j - 1) + UPL(0, i, j)) / TRI(1, i, j - 1)) * TRI(1, i, j - 1);
-          double m1 = M(i + 1, j) * AR(3, i + 1, j) + M(i, j) * (AR(1, i, j) + AR(2, i, j)) + M(i, j + 1) * AR(6, i, j + 1);
-          double m2 = M(i - 1, j) * AR(1, i - 1, j) + M(i, j) * AR(3, i, j) + M(i, j + 1) * AR(5, i, j + 1);
-          double m3 = M(i, j - 1) * AR(2, i, j - 1) + M(i + 1, j) * AR(4, i + 1, j) + M(i, j) * AR(6, i, j);
-          double pvL = (COR(0, i, j) + cL) / m1;
-          double pvWL = (COR(0, i - 1, j) + cW_L) / m2;
-          double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
-          sm.qe[0][r(i, j)] = interp(pvL, pvSU);
-          sm.qe[2][r(i, j)] = interp(pvL, pvWL);
-        }
-      }
+      const bool mine[4] = {cSW, cSE, cNW, cNE};
+      if (tid < 4 && mine[tid])
+        pentagon_fixup(tid, sm.u[0], sm.u[1], sm.u[2], sm.m, sm.qe[0], sm.qe[1], sm.qe[2], g, NC, base, S);
       __syncthreads();
     }
 
@@ -684,6 +694,315 @@ __global__ __launch_bounds__(THREADS2, MINW) void trisk_stage_kernel2(StageArgs 
         double du = src / ll[e] - gB[e] - the[e] * gX[e];
         a.ou[ke3 + (int64_t)e * NC + gc] = a.trend_only ? du : bu[e] + a.h * du;
       }
+    }
+    m = nm;
+    t = nt;
+    u0 = nu0;
+    u1 = nu1;
+    u2 = nu2;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// trisk_stage_pair_kernel: two layer groups per block.  768 threads = two
+// groups of 384; group 0 evaluates the even layers, group 1 the odd ones, in
+// lock step, so each barrier serves two layers and a compute unit holds 12
+// waves.  The launch-invariant geometry of the 19 x 19 region is staged once
+// in LDS and shared by both groups; the Qperp weights of the owned cell (its
+// own offsets 0-2 and those of E, NE, N for offsets 3-5) live in registers.
+// Each thread has two roles: region cell lt (lt < 361) for staging, PV,
+// fluxes and Bernoulli, and owned cell lt (lt < 256) for the trends, so only
+// four waves per group run the Qperp / gradient phase.
+// The hydrostatic column scan stays sequential per column: both groups
+// advance the bottom pressure and geopotential through every layer, taking
+// the other group's layer from its LDS planes, with the same operations in
+// the same order as a single sweep.
+// ---------------------------------------------------------------------------
+constexpr int GROUP = 384;
+constexpr int THREADS3 = 2 * GROUP;
+
+struct GeoS {  // launch-invariant geometry of the staged region
+  double len[3][RR], ped[3][RR], pedf[3][RR], area[6][RR], hinv[RR], cor[2][RR];
+};
+struct LayerS {  // one layer of one group; t holds t/m from phase 3 on
+  double m[RR], t[RR], u[3][RR];
+  double pvL[RR], pvU[RR];
+  double qe[3][RR], hm[3][RR], ht[3][RR];
+  double bern[RR], exn[RR];
+};
+
+__device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, double w[5]) {
+  double acc0 = gs.area[offs][y];
+  double acc1 = acc0 + gs.area[(1 + offs) % 6][y];
+  double acc2 = acc1 + gs.area[(2 + offs) % 6][y];
+  double acc3 = acc2 + gs.area[(3 + offs) % 6][y];
+  double acc4 = acc3 + gs.area[(4 + offs) % 6][y];
+  double hi = gs.hinv[y];
+  w[0] = 0.5 - acc0 * hi;
+  w[1] = -(0.5 - acc1 * hi);
+  w[2] = 0.5 - acc2 * hi;
+  w[3] = -(0.5 - acc3 * hi);
+  w[4] = 0.5 - acc4 * hi;
+}
+
+// Loads and stores at a uniform base pointer plus a 32-bit byte offset, so the
+// per-thread part of an address is one VGPR (global_* with saddr).
+__device__ __forceinline__ double ldo(const double* p, uint32_t off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off);
+}
+__device__ __forceinline__ void sto(double* p, uint32_t off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
+}
+
+__global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a) {
+  __shared__ GeoS gs;
+  __shared__ LayerS ls[2];
+  const int tid = threadIdx.x;
+  const int grp = tid / GROUP, lt = tid % GROUP;
+  const int64_t NC = a.NC;
+  const int S = a.S;
+  const Geo& g = a.g;
+
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int tpl = a.ntile * a.ntile;
+  const int lz = tile / tpl;
+  const int rem = tile % tpl;
+  const int i0 = (rem % a.ntile) * T, j0 = (rem / a.ntile) * T;
+  const int64_t base = (int64_t)lz * S * S + (int64_t)(j0 - 1 + H) * S + (i0 - 1 + H);
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+
+  const bool cSW = (i0 == 0 && j0 == 0), cSE = (i0 + T == a.n && j0 == 0);
+  const bool cNW = (i0 == 0 && j0 + T == a.n), cNE = (i0 + T == a.n && j0 + T == a.n);
+  const bool corner = cSW || cSE || cNW || cNE;
+
+  // region role
+  const bool active = lt < RR;
+  const int ri = lt % R, rj = lt / R;
+  const int x = lt;
+  const uint32_t xb = (uint32_t)(active ? gcell(ri, rj) : base) * 8u;  // byte offset of the cell in a plane
+  const bool tri_cell = active && ri <= T + 1 && rj <= T + 1;
+  const bool p_cell = active && ri >= 1 && rj >= 1 && ri <= T + 1 && rj <= T + 1;
+  const bool pole = (lz == 0 && cSE && ri == T + 1 && rj == 1) || (lz == 5 && cNW && ri == 1 && rj == T + 1);
+  // owned role
+  const bool own = lt < T * T;
+  const int oi = 1 + lt % T, oj = 1 + lt / T;
+  const int y = oj * R + oi;
+  const uint32_t ob = (uint32_t)gcell(oi, oj) * 8u;
+
+  // --- launch-invariant geometry into LDS (each group stages half) ----------
+  if (active) {
+    if (grp == 0) {
+      for (int e = 0; e < 3; ++e) {
+        gs.len[e][x] = ldo(g.len + e * NC, xb);
+        gs.ped[e][x] = ldo(g.ped + e * NC, xb);
+        gs.pedf[e][x] = ldo(g.pedf + e * NC, xb);
+      }
+    } else {
+      for (int p = 0; p < 6; ++p) gs.area[p][x] = ldo(g.area + p * NC, xb);
+      gs.hinv[x] = ldo(g.hinv, xb);
+      gs.cor[0][x] = ldo(g.cor, xb);
+      gs.cor[1][x] = ldo(g.cor + NC, xb);
+    }
+  }
+
+  // column state (cal_surf_press, set_surf_geopot): P = pressure at the
+  // bottom of the next layer to scan, G = geopotential there
+  double P = 0, G = 0;
+  if (p_cell) {
+    double s = 0.0;
+    for (int k = 0; k < a.L; ++k) s = s + (ldo(a.ym + k * NC, xb) - ldo(a.yt + k * NC, xb));
+    P = a.grav * s + a.p_top;
+    G = a.grav * ldo(g.topo, xb);
+  }
+
+  double m = 0, t = 0, u0 = 0, u1 = 0, u2 = 0;
+  if (active && grp < a.L) {
+    const int64_t ks = (int64_t)grp * NC, ke3 = 3 * ks;
+    m = ldo(a.ym + ks, xb);
+    t = ldo(a.yt + ks, xb);
+    u0 = ldo(a.yu + ke3, xb);
+    u1 = ldo(a.yu + ke3 + NC, xb);
+    u2 = ldo(a.yu + ke3 + 2 * NC, xb);
+  }
+  __syncthreads();
+
+  // Qperp weights of the neighbours' offsets 3-5 stay in registers; the
+  // cell's own offsets 0-2 are rebuilt from LDS in each layer
+  double w[6][5];
+  if (own) {
+    weights_lds(gs, y + 1, 3, w[3]);
+    weights_lds(gs, y + R + 1, 4, w[4]);
+    weights_lds(gs, y + R, 5, w[5]);
+  }
+
+  LayerS& sl = ls[grp];
+  const LayerS& so = ls[1 - grp];
+  const int niter = (a.L + 1) / 2;
+  for (int it = 0; it < niter; ++it) {
+    const int k = 2 * it + grp;
+    const bool lay = k < a.L;                 // uniform over the group
+    const bool pair = 2 * it + 1 < a.L;       // both layers of the pair exist
+    const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
+    // --- 1. stage own cell; prefetch the group's next layer -------------------
+    if (active && lay) {
+      sl.m[x] = m;
+      sl.t[x] = t;
+      sl.u[0][x] = u0;
+      sl.u[1][x] = u1;
+      sl.u[2][x] = u2;
+    }
+    double nm = 0, nt = 0, nu0 = 0, nu1 = 0, nu2 = 0;
+    if (active && k + 2 < a.L) {
+      const int64_t ks2 = ks + 2 * NC, ke32 = ke3 + 6 * NC;
+      nm = ldo(a.ym + ks2, xb);
+      nt = ldo(a.yt + ks2, xb);
+      nu0 = ldo(a.yu + ke32, xb);
+      nu1 = ldo(a.yu + ke32 + NC, xb);
+      nu2 = ldo(a.yu + ke32 + 2 * NC, xb);
+    }
+    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
+    if (lay && !a.trend_only) {
+      if (own) {
+        bm = ldo(a.qm + ks, ob);
+        bt = ldo(a.qt + ks, ob);
+        bu0 = ldo(a.qu + ke3, ob);
+        bu1 = ldo(a.qu + ke3 + NC, ob);
+        bu2 = ldo(a.qu + ke3 + 2 * NC, ob);
+      }
+    }
+    __syncthreads();
+
+    // --- 2. column scan; PV, fluxes; kinetic energy, Bernoulli, Exner ---------
+    if (lay) {
+      if (p_cell) {
+        // layer 2it is group 0's; group 1 scans it from group 0's planes first
+        double pr, phi_k;
+        if (grp == 1) {
+          const double mo = so.m[x], to = so.t[x];
+          P = P - a.grav * (mo - to);
+          G = G + a.grav * mo / (a.rho0 * 1.0);
+        }
+        {
+          double p_upper = P - a.grav * (m - t);
+          pr = interp(P, p_upper);
+          double gpl = G;
+          G = gpl + a.grav * m / (a.rho0 * 1.0);
+          phi_k = interp(G, gpl);
+          P = p_upper;
+        }
+        if (grp == 0 && pair) {
+          const double mo = so.m[x], to = so.t[x];
+          P = P - a.grav * (mo - to);
+          G = G + a.grav * mo / (a.rho0 * 1.0);
+        }
+        const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+        double uS = sl.u[2][xS], uSW = sl.u[1][xSW], uW = sl.u[0][xW];
+        double ke = ((u2 * gs.len[2][x]) * (u2 * gs.ped[2][x]) + (u1 * gs.len[1][x]) * (u1 * gs.ped[1][x]) +
+                     (u0 * gs.len[0][x]) * (u0 * gs.ped[0][x]) + (uS * gs.len[2][xS]) * (uS * gs.ped[2][xS]) +
+                     (uSW * gs.len[1][xSW]) * (uSW * gs.ped[1][xSW]) + (uW * gs.len[0][xW]) * (uW * gs.ped[0][xW])) *
+                    gs.hinv[x] / 4.0;
+        sl.bern[x] = ke + phi_k + pr / (a.rho0 * 1.0);
+        sl.exn[x] = -phi_k;
+      }
+      if (tri_cell) {
+        const int xE = x + 1, xN = x + R, xNE = x + R + 1;
+        double upRT = u0 * gs.len[0][x], upDG = u1 * gs.len[1][x], upUP = u2 * gs.len[2][x];
+        double circL = upRT + sl.u[2][xE] * gs.len[2][xE] + upDG;
+        double circU = -(upDG + upUP + sl.u[0][xN] * gs.len[0][xN]);
+        double mE = sl.m[xE], mNE = sl.m[xNE], mN = sl.m[xN];
+        double massL = m * gs.area[0][x] + mE * gs.area[2][xE] + mNE * gs.area[4][xNE];
+        double massU = m * gs.area[1][x] + mNE * gs.area[3][xNE] + mN * gs.area[5][xN];
+        sl.pvL[x] = (gs.cor[0][x] + circL) / massL;
+        sl.pvU[x] = (gs.cor[1][x] + circU) / massU;
+        const double f0 = gs.pedf[0][x], f1 = gs.pedf[1][x], f2 = gs.pedf[2][x];
+        sl.hm[0][x] = u0 * f0 * interp(m, mE) + 0.0;
+        sl.hm[1][x] = u1 * f1 * interp(m, mNE) + 0.0;
+        sl.hm[2][x] = u2 * f2 * interp(m, mN) + 0.0;
+        sl.ht[0][x] = u0 * f0 * interp(t, sl.t[xE]) + 0.0;
+        sl.ht[1][x] = u1 * f1 * interp(t, sl.t[xNE]) + 0.0;
+        sl.ht[2][x] = u2 * f2 * interp(t, sl.t[xN]) + 0.0;
+      }
+    }
+    __syncthreads();
+
+    // --- 3. qe on edges; potential temperature t/m in place -------------------
+    if (lay && active) {
+      if (tri_cell) {
+        if (rj >= 1) sl.qe[0][x] = interp(sl.pvU[x - R], sl.pvL[x]);
+        sl.qe[1][x] = interp(sl.pvU[x], sl.pvL[x]);
+        if (ri >= 1) sl.qe[2][x] = interp(sl.pvU[x], sl.pvL[x - 1]);
+      }
+      sl.t[x] = t / m;
+    }
+    __syncthreads();
+
+    if (corner) {
+      const bool mine[4] = {cSW, cSE, cNW, cNE};
+      if (lay && lt < 4 && mine[lt])
+        pentagon_fixup(lt, sl.u[0], sl.u[1], sl.u[2], sl.m, sl.qe[0], sl.qe[1], sl.qe[2], g, NC, base, S);
+      __syncthreads();
+    }
+
+    // --- 4. trends of owned nodes and edges, pole master; RK update -----------
+    if (lay && own) {
+      const int xW = y - 1, xS = y - R, xSW = y - R - 1, xE = y + 1, xN = y + R, xNE = y + R + 1;
+      const int xNW = y + R - 1, xSE = y - R + 1;
+      const double hi = gs.hinv[y];
+      double divm = (sl.hm[0][y] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][y] + sl.hm[2][y] - sl.hm[2][xS]) * hi;
+      double divt = (sl.ht[0][y] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][y] + sl.ht[2][y] - sl.ht[2][xS]) * hi;
+      double dmv = -divm, dtv = -divt;
+      sto(a.om + ks, ob, a.trend_only ? dmv : bm + a.h * dmv);
+      sto(a.ot + ks, ob, a.trend_only ? dtv : bt + a.h * dtv);
+      double qp[3];
+      weights_lds(gs, y, 0, w[0]);
+      weights_lds(gs, y, 1, w[1]);
+      weights_lds(gs, y, 2, w[2]);
+#define HM(e, xx) sl.hm[e][xx]
+#define QE(e, xx) sl.qe[e][xx]
+      double qr = QE(0, y);
+      qp[0] = HM(1, y) * interp(QE(1, y), qr) * w[0][0] + HM(2, y) * interp(QE(2, y), qr) * w[0][1] +
+              HM(0, xW) * interp(QE(0, xW), qr) * w[0][2] + HM(1, xSW) * interp(QE(1, xSW), qr) * w[0][3] +
+              HM(2, xS) * interp(QE(2, xS), qr) * w[0][4] + HM(1, xS) * interp(QE(1, xS), qr) * w[3][0] +
+              HM(2, xSE) * interp(QE(2, xSE), qr) * w[3][1] + HM(0, xE) * interp(QE(0, xE), qr) * w[3][2] +
+              HM(1, xE) * interp(QE(1, xE), qr) * w[3][3] + HM(2, xE) * interp(QE(2, xE), qr) * w[3][4];
+      double qd = QE(1, y);
+      qp[1] = HM(2, y) * interp(QE(2, y), qd) * w[1][0] + HM(0, xW) * interp(QE(0, xW), qd) * w[1][1] +
+              HM(1, xSW) * interp(QE(1, xSW), qd) * w[1][2] + HM(2, xS) * interp(QE(2, xS), qd) * w[1][3] +
+              HM(0, y) * interp(QE(0, y), qd) * w[1][4] + HM(2, xE) * interp(QE(2, xE), qd) * w[4][0] +
+              HM(0, xNE) * interp(QE(0, xNE), qd) * w[4][1] + HM(1, xNE) * interp(QE(1, xNE), qd) * w[4][2] +
+              HM(2, xNE) * interp(QE(2, xNE), qd) * w[4][3] + HM(0, xN) * interp(QE(0, xN), qd) * w[4][4];
+      double qu = QE(2, y);
+      qp[2] = HM(0, xW) * interp(QE(0, xW), qu) * w[2][0] + HM(1, xSW) * interp(QE(1, xSW), qu) * w[2][1] +
+              HM(2, xS) * interp(QE(2, xS), qu) * w[2][2] + HM(0, y) * interp(QE(0, y), qu) * w[2][3] +
+              HM(1, y) * interp(QE(1, y), qu) * w[2][4] + HM(0, xN) * interp(QE(0, xN), qu) * w[5][0] +
+              HM(1, xN) * interp(QE(1, xN), qu) * w[5][1] + HM(2, xN) * interp(QE(2, xN), qu) * w[5][2] +
+              HM(0, xNW) * interp(QE(0, xNW), qu) * w[5][3] + HM(1, xW) * interp(QE(1, xW), qu) * w[5][4];
+#undef HM
+#undef QE
+      const double th0 = sl.t[y];
+      double the[3] = {interp(th0, sl.t[xE]), interp(th0, sl.t[xNE]), interp(th0, sl.t[xN])};
+      const double b0 = sl.bern[y], x0 = sl.exn[y];
+      const double ll[3] = {gs.len[0][y], gs.len[1][y], gs.len[2][y]};
+      double gB[3] = {(sl.bern[xE] - b0) / ll[0], (b0 - sl.bern[xNE]) / ll[1], (sl.bern[xN] - b0) / ll[2]};
+      double gX[3] = {(sl.exn[xE] - x0) / ll[0], (x0 - sl.exn[xNE]) / ll[1], (sl.exn[xN] - x0) / ll[2]};
+      const double bu[3] = {bu0, bu1, bu2};
+      for (int e = 0; e < 3; ++e) {
+        double src = -qp[e] + 0.0 * ll[e];
+        double du = src / ll[e] - gB[e] - the[e] * gX[e];
+        sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
+      }
+    }
+    if (lay && pole) {
+      const double hi = gs.hinv[x];
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+      double divm = (sl.hm[0][x] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][x] + sl.hm[2][x] - sl.hm[2][xS]) * hi;
+      double divt = (sl.ht[0][x] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][x] + sl.ht[2][x] - sl.ht[2][xS]) * hi;
+      double dmv = -divm, dtv = -divt;
+      sto(a.om + ks, xb, a.trend_only ? dmv : ldo(a.qm + ks, xb) + a.h * dmv);
+      sto(a.ot + ks, xb, a.trend_only ? dtv : ldo(a.qt + ks, xb) + a.h * dtv);
     }
     m = nm;
     t = nt;
@@ -750,7 +1069,7 @@ struct wtg_ctx {
   wtg_grid_info info{};
   wtg_params params{};
   int nslots = 0;
-  int stage_version = 2;  // WTG_STAGE_KERNEL: 1 = first (LDS-loop) version, 2/3 = per-cell version at 2/3 waves per SIMD
+  int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
   bool topo_set = false;
   std::string err;
   // device geometry
@@ -857,10 +1176,10 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
   unsigned blocks = (unsigned)(10 * a.ntile * a.ntile);
   if (c->stage_version == 1)
     hipLaunchKernelGGL(trisk_stage_kernel, dim3(blocks), dim3(THREADS), 0, c->stream, a);
-  else if (c->stage_version == 3)
-    hipLaunchKernelGGL(trisk_stage_kernel2<3>, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
+  else if (c->stage_version == 2)
+    hipLaunchKernelGGL(trisk_stage_kernel2, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(trisk_stage_kernel2<2>, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
+    hipLaunchKernelGGL(trisk_stage_pair_kernel, dim3(blocks), dim3(THREADS3), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -966,7 +1285,7 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
   c->nslots = nslots;
   if (const char* v = std::getenv("WTG_STAGE_KERNEL")) {
     int sv = std::atoi(v);
-    c->stage_version = (sv == 1 || sv == 3) ? sv : 2;
+    c->stage_version = (sv == 1 || sv == 2) ? sv : 3;
   }
   const auto& g = grid->lv;
   const int64_t NC = c->info.ncells;
