@@ -442,6 +442,37 @@ __device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const 
   }
 }
 
+// Loads and stores at a uniform base pointer plus a 32-bit byte offset, so the
+// per-thread part of an address is one VGPR (global_* with saddr).
+__device__ __forceinline__ double ldo(const double* p, uint32_t off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off);
+}
+__device__ __forceinline__ void sto(double* p, uint32_t off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
+}
+
+// sum over layers of (mass - temp) of one column, in layer order (the first
+// step of cal_surf_press, ops.f90:1023-1049); the loads of a chunk of
+// layers are issued together so the column costs a few memory round trips
+__device__ __forceinline__ double column_sum(const double* ym, const double* yt, int L, int64_t NC, uint32_t xb) {
+  constexpr int CH = 10;
+  double s = 0.0;
+  for (int k0 = 0; k0 < L; k0 += CH) {
+    double dm[CH], dt[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int k = k0 + j < L ? k0 + j : L - 1;  // clamped: in bounds, not summed
+      dm[j] = ldo(ym + k * NC, xb);
+      dt[j] = ldo(yt + k * NC, xb);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // all loads of the chunk before the first add
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (k0 + j < L) s = s + (dm[j] - dt[j]);
+  }
+  return s;
+}
+
 // ---------------------------------------------------------------------------
 // trisk_stage_kernel2: same arithmetic as trisk_stage_kernel, organised for
 // throughput.  One thread per cell of the 19 x 19 staged region (384 threads,
@@ -542,8 +573,7 @@ __global__ __launch_bounds__(THREADS2, 2) void trisk_stage_kernel2(StageArgs a) 
   // column state (cal_surf_press, set_surf_geopot)
   double pl = 0, gp = 0, pr = 0, gpl = 0;
   if (p_cell) {
-    double s = 0.0;
-    for (int k = 0; k < a.L; ++k) s = s + (a.ym[(int64_t)k * NC + gc] - a.yt[(int64_t)k * NC + gc]);
+    double s = column_sum(a.ym, a.yt, a.L, NC, (uint32_t)gc * 8u);
     pl = a.grav * s + a.p_top;
     gp = a.grav * g.topo[gc];
   }
@@ -746,20 +776,15 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
   w[4] = 0.5 - acc4 * hi;
 }
 
-// Loads and stores at a uniform base pointer plus a 32-bit byte offset, so the
-// per-thread part of an address is one VGPR (global_* with saddr).
-__device__ __forceinline__ double ldo(const double* p, uint32_t off) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off);
-}
-__device__ __forceinline__ void sto(double* p, uint32_t off, double v) {
-  *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
-}
-
 __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a) {
   __shared__ GeoS gs;
-  __shared__ LayerS ls[2];
+  __shared__ union {
+    LayerS ls[2];
+    double flat[2 * sizeof(LayerS) / sizeof(double)];
+  } lsu;
   const int tid = threadIdx.x;
-  const int grp = tid / GROUP, lt = tid % GROUP;
+  const int grp = __builtin_amdgcn_readfirstlane(tid / GROUP);  // uniform over each wave
+  const int lt = tid % GROUP;
   const int64_t NC = a.NC;
   const int S = a.S;
   const Geo& g = a.g;
@@ -791,6 +816,12 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   const int oi = 1 + lt % T, oj = 1 + lt / T;
   const int y = oj * R + oi;
   const uint32_t ob = (uint32_t)gcell(oi, oj) * 8u;
+  // The loads inside the layer loop are unconditional, at clamped addresses
+  // (the values of idle threads and layers are dropped): with no branch
+  // around them the compiler's wait counts stay exact and a prefetch is not
+  // waited for before its use.
+  const int lo = lt % (T * T);
+  const uint32_t obl = (uint32_t)gcell(1 + lo % T, 1 + lo / T) * 8u;
 
   // --- launch-invariant geometry into LDS (each group stages half) ----------
   if (active) {
@@ -809,25 +840,53 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   }
 
   // column state (cal_surf_press, set_surf_geopot): P = pressure at the
-  // bottom of the next layer to scan, G = geopotential there
+  // bottom of the next layer to scan, G = geopotential there.  The column
+  // sum of (mass - temp) is taken in layer order from LDS, where each group
+  // has stored (m - t) of every other layer (the layer planes are still free)
   double P = 0, G = 0;
-  if (p_cell) {
+  {
+    constexpr int KMAX = (int)(sizeof(lsu.flat) / sizeof(double)) / RR;  // layers per round
     double s = 0.0;
-    for (int k = 0; k < a.L; ++k) s = s + (ldo(a.ym + k * NC, xb) - ldo(a.yt + k * NC, xb));
-    P = a.grav * s + a.p_top;
-    G = a.grav * ldo(g.topo, xb);
+    for (int k0 = 0; k0 < a.L; k0 += KMAX) {
+      const int kn = min(KMAX, a.L - k0);
+      if (active) {
+        for (int kk = grp; kk < kn; kk += 2) {
+          const int64_t ks = (int64_t)(k0 + kk) * NC;
+          lsu.flat[kk * RR + x] = ldo(a.ym + ks, xb) - ldo(a.yt + ks, xb);
+        }
+      }
+      __syncthreads();
+      if (p_cell)
+        for (int kk = 0; kk < kn; ++kk) s = s + lsu.flat[kk * RR + x];
+      __syncthreads();
+    }
+    if (p_cell) {
+      P = a.grav * s + a.p_top;
+      G = a.grav * ldo(g.topo, xb);
+    }
   }
 
-  double m = 0, t = 0, u0 = 0, u1 = 0, u2 = 0;
-  if (active && grp < a.L) {
-    const int64_t ks = (int64_t)grp * NC, ke3 = 3 * ks;
+  LayerS& sl = lsu.ls[grp];
+  const LayerS& so = lsu.ls[1 - grp];
+  // the group's first layer, staged here; later layers are staged at the end
+  // of the previous iteration from registers loaded one iteration ahead
+  double m, t, u0, u1, u2;
+  {
+    const int64_t ks = (int64_t)min(grp, a.L - 1) * NC, ke3 = 3 * ks;
     m = ldo(a.ym + ks, xb);
     t = ldo(a.yt + ks, xb);
     u0 = ldo(a.yu + ke3, xb);
     u1 = ldo(a.yu + ke3 + NC, xb);
     u2 = ldo(a.yu + ke3 + 2 * NC, xb);
   }
-  __syncthreads();
+  if (active && grp < a.L) {
+    sl.m[x] = m;
+    sl.t[x] = t;
+    sl.u[0][x] = u0;
+    sl.u[1][x] = u1;
+    sl.u[2][x] = u2;
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // nothing outstanding enters the layer loop
 
   // Qperp weights of the neighbours' offsets 3-5 stay in registers; the
   // cell's own offsets 0-2 are rebuilt from LDS in each layer
@@ -838,42 +897,23 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     weights_lds(gs, y + R, 5, w[5]);
   }
 
-  LayerS& sl = ls[grp];
-  const LayerS& so = ls[1 - grp];
   const int niter = (a.L + 1) / 2;
   for (int it = 0; it < niter; ++it) {
     const int k = 2 * it + grp;
     const bool lay = k < a.L;                 // uniform over the group
     const bool pair = 2 * it + 1 < a.L;       // both layers of the pair exist
     const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
-    // --- 1. stage own cell; prefetch the group's next layer -------------------
-    if (active && lay) {
-      sl.m[x] = m;
-      sl.t[x] = t;
-      sl.u[0][x] = u0;
-      sl.u[1][x] = u1;
-      sl.u[2][x] = u2;
-    }
-    double nm = 0, nt = 0, nu0 = 0, nu1 = 0, nu2 = 0;
-    if (active && k + 2 < a.L) {
-      const int64_t ks2 = ks + 2 * NC, ke32 = ke3 + 6 * NC;
+    __syncthreads();
+    // --- 1. prefetch the group's next layer (its loads overlap phases 2-4) ----
+    double nm, nt, nu0, nu1, nu2;
+    {
+      const int64_t ks2 = (int64_t)min(k + 2, a.L - 1) * NC, ke32 = 3 * ks2;
       nm = ldo(a.ym + ks2, xb);
       nt = ldo(a.yt + ks2, xb);
       nu0 = ldo(a.yu + ke32, xb);
       nu1 = ldo(a.yu + ke32 + NC, xb);
       nu2 = ldo(a.yu + ke32 + 2 * NC, xb);
     }
-    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
-    if (lay && !a.trend_only) {
-      if (own) {
-        bm = ldo(a.qm + ks, ob);
-        bt = ldo(a.qt + ks, ob);
-        bu0 = ldo(a.qu + ke3, ob);
-        bu1 = ldo(a.qu + ke3 + NC, ob);
-        bu2 = ldo(a.qu + ke3 + 2 * NC, ob);
-      }
-    }
-    __syncthreads();
 
     // --- 2. column scan; PV, fluxes; kinetic energy, Bernoulli, Exner ---------
     if (lay) {
@@ -928,7 +968,17 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     }
     __syncthreads();
 
-    // --- 3. qe on edges; potential temperature t/m in place -------------------
+    // --- 3. base state of the update (q = y for a trend), consumed in phase 4;
+    //        qe on edges; potential temperature t/m in place ------------------
+    double bm, bt, bu0, bu1, bu2;
+    {
+      const int64_t ksb = (int64_t)min(k, a.L - 1) * NC, ke3b = 3 * ksb;
+      bm = ldo(a.qm + ksb, obl);
+      bt = ldo(a.qt + ksb, obl);
+      bu0 = ldo(a.qu + ke3b, obl);
+      bu1 = ldo(a.qu + ke3b + NC, obl);
+      bu2 = ldo(a.qu + ke3b + 2 * NC, obl);
+    }
     if (lay && active) {
       if (tri_cell) {
         if (rj >= 1) sl.qe[0][x] = interp(sl.pvU[x - R], sl.pvL[x]);
@@ -995,6 +1045,11 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
         sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
       }
     }
+    if (!(lay && own)) {
+      // threads that store nothing still wait for their base-state loads
+      // here, so no later phase inherits an unknown wait on them
+      asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
+    }
     if (lay && pole) {
       const double hi = gs.hinv[x];
       const int xW = x - 1, xS = x - R, xSW = x - R - 1;
@@ -1004,12 +1059,20 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
       sto(a.om + ks, xb, a.trend_only ? dmv : ldo(a.qm + ks, xb) + a.h * dmv);
       sto(a.ot + ks, xb, a.trend_only ? dtv : ldo(a.qt + ks, xb) + a.h * dtv);
     }
+    __syncthreads();
+    // --- 5. stage the group's next layer --------------------------------------
+    if (active && k + 2 < a.L) {
+      sl.m[x] = nm;
+      sl.t[x] = nt;
+      sl.u[0][x] = nu0;
+      sl.u[1][x] = nu1;
+      sl.u[2][x] = nu2;
+    }
     m = nm;
     t = nt;
     u0 = nu0;
     u1 = nu1;
     u2 = nu2;
-    __syncthreads();
   }
 }
 
