@@ -127,6 +127,59 @@ int wtg_synchronize(wtg_ctx* ctx);
 /* Last error message of ctx (or of the last failed create when ctx is NULL). */
 int wtg_last_error(const wtg_ctx* ctx, char* buf, size_t n);
 
+/* ---------------------------------------------------------------------------
+ * Free-surface (barotropic) elliptic problem of the mode-split step, one
+ * uniform level (barotropic_2d.f90, lin_solve.f90).  Node fields of the
+ * context, each [ncells] like a one-layer scalar:
+ *   WTG_FS_ETA      free-surface perturbation   (sol(S_MASS, zlevels+1))
+ *   WTG_FS_RHS      right-hand side             (sol(S_TEMP, zlevels+1))
+ *   WTG_FS_ETA_OLD  free surface in the operator (Laplacian_scalar(S_TEMP))
+ *   WTG_FS_DIVF     flux divergence of the layer-integrated velocity
+ *                   (trend(S_MASS, zlevels+1))
+ *   WTG_FS_DIVF_OLD the same at the previous step (trend(S_TEMP, zlevels+1))
+ * Porosity is 1 and every node is active.  Fields are ghost filled by every
+ * call that writes them.
+ * ------------------------------------------------------------------------- */
+enum { WTG_FS_ETA = 0, WTG_FS_RHS = 1, WTG_FS_ETA_OLD = 2, WTG_FS_DIVF = 3, WTG_FS_DIVF_OLD = 4 };
+enum { WTG_SOLVE_BICGSTAB = 0, WTG_SOLVE_JACOBI = 1 };
+
+typedef struct {
+  double dt;     /* time step (dt of shared.f90) */
+  double theta1; /* implicitness of the external pressure gradient (theta1, shared.f90:437) */
+  double theta2; /* implicitness of the barotropic divergence (theta2, shared.f90:438) */
+} wtg_fs_params;
+
+int wtg_fs_set_params(wtg_ctx* ctx, const wtg_fs_params* p);
+/* host <-> device copy of one free-surface field ([ncells] doubles; halo
+ * cells are filled from their owners on upload) */
+int wtg_fs_upload(wtg_ctx* ctx, int field, const double* host);
+int wtg_fs_download(wtg_ctx* ctx, int field, double* host);
+
+/* out = elliptic_lo(in) (barotropic_2d.f90:187-253):
+ *   theta1 theta2 dt^2 div(g (|topo| + eta_old) grad in) - in */
+int wtg_elliptic_apply(wtg_ctx* ctx, int in_field, int out_field);
+/* out = elliptic_lo_diag (barotropic_2d.f90:255-324, the averaged-depth branch) */
+int wtg_elliptic_diag(wtg_ctx* ctx, int out_field);
+/* nsweeps of Jacobi_iteration (lin_solve.f90:271-340): u += w (f - L u) / diag */
+int wtg_jacobi(wtg_ctx* ctx, int u_field, int f_field, double w, int nsweeps);
+/* dp (lin_solve.f90:477-503): sum over the reference's (0,1) node sweep;
+ * l2 (lin_solve.f90:433-461) is sqrt(dot(a, a)) */
+int wtg_fs_dot(wtg_ctx* ctx, int a_field, int b_field, double* out);
+/* WTG_FS_DIVF := flux_divergence of state slot `slot` (barotropic_2d.f90:326-366) */
+int wtg_flux_divergence(wtg_ctx* ctx, int slot);
+/* WTG_FS_RHS := -eta + dt (theta2 divF + (1 - theta2) divF_old) / ref_density
+ * (rhs_elliptic, barotropic_2d.f90:151-185) */
+int wtg_rhs_elliptic(wtg_ctx* ctx);
+/* Solves elliptic_lo(eta) = rhs for WTG_FS_ETA in place, from its current
+ * value, until ||rhs - L eta|| / ||rhs|| <= rel_tol (norms as l2 above) or
+ * max_iter iterations.  WTG_SOLVE_BICGSTAB is bicgstab (lin_solve.f90:361-431),
+ * the reference's solver on the coarsest level (and on the only level of a
+ * uniform grid, FMG / SRJ lin_solve.f90:33-231); WTG_SOLVE_JACOBI is
+ * Jacobi_iteration with weight 1.  *iters as the reference's iter_out
+ * (max_iter + 1 if rel_tol was not reached), *rel_resid the last relative
+ * residual. */
+int wtg_elliptic_solve(wtg_ctx* ctx, int method, double rel_tol, int max_iter, int* iters, double* rel_resid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,22 @@ def load():
         _lib.wto_ghost_fill.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.wto_trend.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         _lib.wto_rk4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.wto_node_ghost.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.wto_elliptic_apply.argtypes = [C.c_void_p] * 5
+        _lib.wto_elliptic_diag.argtypes = [C.c_void_p] * 4
+        _lib.wto_jacobi.argtypes = [C.c_void_p] * 5 + [C.c_double]
+        _lib.wto_wdot.argtypes = [C.c_void_p] * 4
+        _lib.wto_wdot.restype = C.c_double
+        _lib.wto_bicgstab.argtypes = [C.c_void_p] * 6 + [C.c_double, C.c_double, C.c_int, C.c_void_p]
+        _lib.wto_bicgstab.restype = C.c_int
+        _lib.wto_flux_divergence.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.wto_rhs_elliptic.argtypes = [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 4
     return _lib
+
+
+class FsParams(C.Structure):
+    """Parameters of the free-surface operator (grav, dt, theta1, theta2)."""
+    _fields_ = [("grav", C.c_double), ("dt", C.c_double), ("theta1", C.c_double), ("theta2", C.c_double)]
 
 
 class _G(C.Structure):
@@ -92,4 +107,74 @@ class Oracle:
 def geometry_of(grid) -> dict:
     """Geometry and ghost maps of a wavetrisk_hydrostatic_amd.Grid as oracle inputs."""
     return {k: grid.array(k) for k in ("len", "pedlen", "area", "hex_inv", "triarea", "coriolis", "node_src",
-                                        "edge_src", "edge_sign")}
+                                        "edge_src", "edge_sign", "red_weight")}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class EllipticOracle(Oracle):
+    """The free-surface elliptic routines (oracle/elliptic_oracle.c) on the
+    same grid arrays; node fields are [ncells] float64, ghost filled."""
+
+    def __init__(self, layout, geom: dict, topo, grav=9.80616, rho0=1028.0, dt=600.0, theta1=1.0, theta2=1.0):
+        super().__init__(layout, geom, topo, grav=grav, rho0=rho0)
+        self.fs = FsParams(grav, dt, theta1, theta2)
+        self.rho0 = rho0
+        self.weight = _f64(geom["red_weight"]) if "red_weight" in geom else None
+
+    def node_ghost(self, f):
+        f = _f64(f).copy()
+        self.lib.wto_node_ghost(C.byref(self.g), f.ctypes.data)
+        return f
+
+    def apply(self, eta_old, q):
+        eta_old, q = _f64(eta_old), _f64(q)
+        out = np.zeros_like(q)
+        self.lib.wto_elliptic_apply(C.byref(self.g), C.byref(self.fs), eta_old.ctypes.data, q.ctypes.data,
+                                    out.ctypes.data)
+        return out
+
+    def diag(self, eta_old):
+        eta_old = _f64(eta_old)
+        out = np.zeros_like(eta_old)
+        self.lib.wto_elliptic_diag(C.byref(self.g), C.byref(self.fs), eta_old.ctypes.data, out.ctypes.data)
+        return out
+
+    def jacobi(self, eta_old, u, f, w):
+        eta_old, f = _f64(eta_old), _f64(f)
+        u = _f64(u).copy()
+        self.lib.wto_jacobi(C.byref(self.g), C.byref(self.fs), eta_old.ctypes.data, u.ctypes.data, f.ctypes.data, w)
+        return u
+
+    def wdot(self, a, b):
+        a, b = _f64(a), _f64(b)
+        return self.lib.wto_wdot(C.byref(self.g), self.weight.ctypes.data, a.ctypes.data, b.ctypes.data)
+
+    def l2(self, a):
+        return float(np.sqrt(self.wdot(a, a)))
+
+    def bicgstab(self, eta_old, u, f, tol, maxit, nrm_f=None):
+        eta_old, f = _f64(eta_old), _f64(f)
+        u = _f64(u).copy()
+        if nrm_f is None:
+            nrm_f = self.l2(f) or 1.0
+        err = C.c_double(0)
+        it = self.lib.wto_bicgstab(C.byref(self.g), C.byref(self.fs), eta_old.ctypes.data, self.weight.ctypes.data,
+                                   u.ctypes.data, f.ctypes.data, nrm_f, tol, maxit, C.byref(err))
+        return u, it, err.value
+
+    def flux_divergence(self, mass, velo):
+        mass, velo = _f64(mass), _f64(velo)
+        out = np.zeros(self.layout.ncells)
+        self.lib.wto_flux_divergence(C.byref(self.g), mass.shape[0], mass.ctypes.data, velo.ctypes.data,
+                                     out.ctypes.data)
+        return out
+
+    def rhs(self, eta, divF, divF_old):
+        eta, divF, divF_old = _f64(eta), _f64(divF), _f64(divF_old)
+        out = np.zeros_like(eta)
+        self.lib.wto_rhs_elliptic(C.byref(self.g), C.byref(self.fs), self.rho0, eta.ctypes.data, divF.ctypes.data,
+                                  divF_old.ctypes.data, out.ctypes.data)
+        return out

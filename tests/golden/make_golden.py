@@ -13,6 +13,11 @@ lozenge-tile layout:
                      value before its update_bdry)
   trisk_<case>.npz   initial state (owned cells, pole masters, frozen halo
                      slots), trend_ml of it, and the state after each RK4 step
+  elliptic_<case>.npz  free-surface elliptic path (harness dump_elliptic.f90):
+                     operand, old free surface, layer state, the operator
+                     and its diagonal, the (0,1)-sweep visit counts, l2/dp,
+                     flux divergence, RHS, one Jacobi sweep and a BiCGSTAB
+                     solve with its iteration count
 
 Only this container has /root/reference; the fixtures are committed and the
 GPU box reads them.
@@ -112,8 +117,8 @@ def lozenge_cover(o):
     return n, S, ND, L, cover
 
 
-def run_ref(param, namelist, path):
-    exe = os.path.join(ROOT, "oracle", "_ref", param, "dump_trisk")
+def run_ref(param, namelist, path, prog="dump_trisk"):
+    exe = os.path.join(ROOT, "oracle", "_ref", param, prog)
     if not os.path.exists(exe):
         subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile.ref"), f"PARAM={param}"],
                        cwd=ROOT, check=True)
@@ -257,13 +262,85 @@ def trisk_fixture(name, J, case, L, nsteps, dt):
     print(f"trisk_{name}.npz written")
 
 
+ELLIPTIC_CASES = [
+    # name, J, layers, dt, theta1, theta2, Jacobi weight, BiCGSTAB tolerance
+    ("J4", 4, 2, 600.0, 1.0, 1.0, 0.7, 1e-9),
+    ("J5", 5, 3, 3000.0, 1.0, 1.0, 0.7, 1e-9),
+]
+
+
+def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol):
+    param, ml, dl = PARAMS[J]
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "e.bin")
+        run_ref(param, f"&run zlev={L}, step={dt}, theta_1={th1}, theta_2={th2}, wjac={wjac}, tol_solve={tol}, "
+                       f"maxit=400, out='OUT' /\n", p, prog="dump_elliptic")
+        o = load_dump(p)
+    n, S, ND, L_, cover = lozenge_cover(o)
+    assert L_ == L
+    NC = 10 * S * S
+    masters = [(lz, i, j) for lz in range(10) for j in range(n) for i in range(n)] + [(0, n, 0), (5, 0, n)]
+
+    def slot(key):
+        hits = [x for x in cover[key] if x[0] == 0] or cover[key]
+        return hits[0][1:]
+
+    def node(arrs, k=0):
+        out = np.full(NC, np.nan)
+        for key in masters:
+            dd, e = slot(key)
+            out[cell_index(S, *key)] = arrs[k * ND + dd][e]
+        return out
+
+    def layers(tag, mult):
+        out = np.full((L, mult, NC), np.nan)
+        for k in range(L):
+            for key in masters:
+                dd, e = slot(key)
+                out[k, :, cell_index(S, *key)] = o[tag][k * ND + dd][mult * e:mult * e + mult]
+        return out
+
+    topo = np.full(NC, np.nan)
+    for key in masters:
+        dd, e = slot(key)
+        topo[cell_index(S, *key)] = o["dom"][dd]["topo"][e]
+    # visit counts of the (0,1) sweep, per (domain, element), folded onto master cells later by the test
+    wsum = np.zeros(NC)
+    slot2cell = {}
+    for key, lst in cover.items():
+        for dist, dd, e in lst:
+            slot2cell.setdefault((dd, e), key)
+    visits = []  # (lozenge cell of the visited copy, count)
+    for dd in range(ND):
+        v = o["e_wsum"][dd]
+        for e in np.nonzero(v)[0]:
+            visits.append((cell_index(S, *slot2cell[(dd, int(e))]), int(v[e])))
+    visits = np.array(visits, dtype=np.int64)
+    prm = o["params"]
+    np.savez_compressed(os.path.join(OUT, f"elliptic_{name}.npz"), level=J, min_level=ml, domain_level=dl,
+                        nlayers=L, radius=prm[0], omega=prm[1], grav=prm[2], ref_density=prm[3], dt=prm[4],
+                        theta1=prm[5], theta2=prm[6], wjac=prm[7], tol=prm[8], topo=topo,
+                        q=node(o["e_q"]), eta_old=node(o["e_etan"]), mass=layers("s0_mass", 1)[:, 0],
+                        velo=layers("s0_velo", 3), Lq=node(o["e_Lq"]), diag=node(o["e_diag"]),
+                        visits=visits, red=o["e_red"][0], divF=node(o["e_divF"]), rhs=node(o["e_rhs"]),
+                        jac=node(o["e_jac"]), sol=node(o["e_sol"]), iters=int(o["e_iters"][0][0]),
+                        err=o["e_err"][0])
+    print(f"elliptic_{name}.npz written")
+
+
 def main():
     if not os.path.isdir("/root/reference/src"):
         sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
-    for J in sorted(PARAMS):
-        grid_fixture(J)
-    for c in CASES:
-        trisk_fixture(*c)
+    what = sys.argv[1:] or ["grid", "trisk", "elliptic"]
+    if "grid" in what:
+        for J in sorted(PARAMS):
+            grid_fixture(J)
+    if "trisk" in what:
+        for c in CASES:
+            trisk_fixture(*c)
+    if "elliptic" in what:
+        for c in ELLIPTIC_CASES:
+            elliptic_fixture(*c)
 
 
 if __name__ == "__main__":

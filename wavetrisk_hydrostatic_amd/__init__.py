@@ -22,7 +22,8 @@ import numpy as np
 
 from .layout import Layout
 
-__all__ = ["Grid", "Context", "Layout", "WtgError", "library_path", "load_library"]
+__all__ = ["Grid", "Context", "Layout", "WtgError", "library_path", "load_library", "FS_ETA", "FS_RHS",
+           "FS_ETA_OLD", "FS_DIVF", "FS_DIVF_OLD", "SOLVE_BICGSTAB", "SOLVE_JACOBI"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libwavetrisk_gpu.so"
@@ -72,7 +73,25 @@ SIGNATURES = {
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
     "wtg_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "wtg_fs_set_params": (C.c_int, [_P, _P]),
+    "wtg_fs_upload": (C.c_int, [_P, C.c_int, _P]),
+    "wtg_fs_download": (C.c_int, [_P, C.c_int, _P]),
+    "wtg_elliptic_apply": (C.c_int, [_P, C.c_int, C.c_int]),
+    "wtg_elliptic_diag": (C.c_int, [_P, C.c_int]),
+    "wtg_jacobi": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int]),
+    "wtg_fs_dot": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "wtg_flux_divergence": (C.c_int, [_P, C.c_int]),
+    "wtg_rhs_elliptic": (C.c_int, [_P]),
+    "wtg_elliptic_solve": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _D]),
 }
+
+# free-surface fields and solver methods (include/wavetrisk_gpu.h)
+FS_ETA, FS_RHS, FS_ETA_OLD, FS_DIVF, FS_DIVF_OLD = 0, 1, 2, 3, 4
+SOLVE_BICGSTAB, SOLVE_JACOBI = 0, 1
+
+
+class _FsParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("theta1", C.c_double), ("theta2", C.c_double)]
 
 
 def load_library():
@@ -133,7 +152,8 @@ class Grid:
 
     _SHAPES = {"x": (3, np.float64), "len": (3, np.float64), "pedlen": (3, np.float64), "area": (6, np.float64),
                "hex_inv": (1, np.float64), "triarea": (2, np.float64), "coriolis": (2, np.float64),
-               "node_src": (1, np.int64), "edge_src": (3, np.int64), "edge_sign": (3, np.int8)}
+               "node_src": (1, np.int64), "edge_src": (3, np.int64), "edge_sign": (3, np.int8),
+               "red_weight": (1, np.float64)}
 
     def array(self, name: str) -> np.ndarray:
         comp, dt = self._SHAPES[name]
@@ -210,3 +230,44 @@ class Context:
 
     def synchronize(self):
         _check(_lib.wtg_synchronize(self._h), self._h, "wtg_synchronize")
+
+    # --- free-surface elliptic problem (node fields [ncells]) ---------------
+    def fs_set_params(self, dt: float, theta1: float = 1.0, theta2: float = 1.0):
+        p = _FsParams(dt, theta1, theta2)
+        _check(_lib.wtg_fs_set_params(self._h, C.byref(p)), self._h, "wtg_fs_set_params")
+
+    def fs_upload(self, field: int, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        assert a.shape == (self.layout.ncells,)
+        _check(_lib.wtg_fs_upload(self._h, field, _ptr(a)), self._h, "wtg_fs_upload")
+
+    def fs_download(self, field: int) -> np.ndarray:
+        a = np.empty(self.layout.ncells)
+        _check(_lib.wtg_fs_download(self._h, field, _ptr(a)), self._h, "wtg_fs_download")
+        return a
+
+    def elliptic_apply(self, in_field: int, out_field: int):
+        _check(_lib.wtg_elliptic_apply(self._h, in_field, out_field), self._h, "wtg_elliptic_apply")
+
+    def elliptic_diag(self, out_field: int):
+        _check(_lib.wtg_elliptic_diag(self._h, out_field), self._h, "wtg_elliptic_diag")
+
+    def jacobi(self, u_field: int, f_field: int, w: float, nsweeps: int = 1):
+        _check(_lib.wtg_jacobi(self._h, u_field, f_field, w, nsweeps), self._h, "wtg_jacobi")
+
+    def fs_dot(self, a_field: int, b_field: int) -> float:
+        out = C.c_double()
+        _check(_lib.wtg_fs_dot(self._h, a_field, b_field, C.byref(out)), self._h, "wtg_fs_dot")
+        return out.value
+
+    def flux_divergence(self, slot: int):
+        _check(_lib.wtg_flux_divergence(self._h, slot), self._h, "wtg_flux_divergence")
+
+    def rhs_elliptic(self):
+        _check(_lib.wtg_rhs_elliptic(self._h), self._h, "wtg_rhs_elliptic")
+
+    def elliptic_solve(self, method: int = SOLVE_BICGSTAB, rel_tol: float = 1e-9, max_iter: int = 200):
+        it, res = C.c_int(), C.c_double()
+        _check(_lib.wtg_elliptic_solve(self._h, method, rel_tol, max_iter, C.byref(it), C.byref(res)), self._h,
+               "wtg_elliptic_solve")
+        return it.value, res.value

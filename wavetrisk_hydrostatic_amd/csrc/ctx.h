@@ -1,0 +1,85 @@
+// Internal: the device context shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wavetrisk_gpu.h"
+#include "grid.h"
+
+struct wtg_grid {
+  wtg::Level lv;
+};
+
+// node fields of the free-surface (2-D) problem, one level
+enum FsField {
+  FS_ETA = 0,      // free-surface perturbation (sol(S_MASS, zlevels+1))
+  FS_RHS,          // right-hand side of the elliptic equation (sol(S_TEMP, zlevels+1))
+  FS_ETA_OLD,      // free surface of the previous step, in the operator (Laplacian_scalar(S_TEMP))
+  FS_DIVF,         // flux divergence of the layer-integrated velocity (trend(S_MASS, zlevels+1))
+  FS_DIVF_OLD,     // the same at the previous step (trend(S_TEMP, zlevels+1))
+  FS_NUSER,        // fields above are reachable through the C ABI
+  FS_W0 = FS_NUSER,  // solver work fields
+  FS_NFIELDS = FS_W0 + 9
+};
+
+struct wtg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  wtg_grid_info info{};
+  wtg_params params{};
+  int nslots = 0;
+  int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
+  bool topo_set = false;
+  std::string err;
+  // device geometry
+  double *d_len = nullptr, *d_ped = nullptr, *d_pedf = nullptr, *d_area = nullptr, *d_hinv = nullptr;
+  double *d_tri = nullptr, *d_cor = nullptr, *d_topo = nullptr;
+  int64_t *d_nd = nullptr, *d_ed = nullptr, *d_fn = nullptr, *d_fe = nullptr;
+  int8_t* d_es = nullptr;
+  int nn = 0, ne = 0, nfn = 0, nfe = 0;
+  std::vector<int64_t> node_src;
+  // state slots: user slots then two stage buffers
+  struct Slot {
+    double *m = nullptr, *t = nullptr, *u = nullptr;
+  };
+  std::vector<Slot> slots;
+  // free-surface problem
+  double* d_lenf = nullptr;      // primal lengths of the flux (owner's for halo edges)
+  int64_t* d_ncl = nullptr;      // node compute list: (dst, src) pairs, masters then ghost copies
+  int nncl = 0, nmaster = 0;
+  int64_t* d_rcell = nullptr;    // cells with a reduction weight
+  double* d_rw = nullptr;        // their weights (visits of the reference's (0,1) sweep)
+  int nred = 0;
+  double fs_dt = 0, fs_theta1 = 1, fs_theta2 = 1;
+  bool fs_params_set = false;
+  double* fs[FS_NFIELDS] = {};   // allocated on first use
+  double* d_part = nullptr;      // reduction partials
+  double* d_scal = nullptr;      // device scalars of the solvers
+  double* h_scal = nullptr;      // pinned host copy (hipHostMalloc)
+  int part_blocks = 0;
+  std::vector<void*> allocs;
+};
+
+extern thread_local std::string g_create_error;
+int wtg_fail(wtg_ctx* c, int code, const char* fmt, ...);
+
+#define WTG_HIPCHK(ctx, call)                                                                         \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess) return wtg_fail(ctx, WTG_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class Tp>
+inline int wtg_dalloc(wtg_ctx* c, Tp** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(Tp) > 0 ? count * sizeof(Tp) : 1);
+  if (e != hipSuccess)
+    return wtg_fail(c, WTG_E_DEVICE, "hipMalloc(%zu bytes): %s", count * sizeof(Tp), hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = (Tp*)q;
+  return 0;
+}

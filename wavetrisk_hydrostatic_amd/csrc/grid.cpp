@@ -688,6 +688,30 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
           g.coriolis[k * NC + c] = CO[2 * x + k];
         }
       }
+  // --- 7. reduction weights ---------------------------------------------------
+  // The reference's norms and dot products (l2, dp in lin_solve.f90:433-518)
+  // visit, per domain, the domain's (N+1) x (N+1) nodes including its E and N
+  // boundary rows (apply_onescale_to_patch with (st,en) = (0,1),
+  // domain.f90:358-383), so nodes on domain boundaries count more than once.
+  // red_weight[c] is that visit count folded onto the owner of each copy.
+  {
+    const int N = n >> domain_level, nsd = 1 << domain_level;
+    g.red_weight.assign(NC, 0.0);
+    for (int lz = 0; lz < 10; ++lz)
+      for (int sj = 0; sj < nsd; ++sj)
+        for (int si = 0; si < nsd; ++si)
+          for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+              const int I = si * N + i, J = sj * N + j;
+              size_t c = cell(lz, I, J);
+              if ((I >= n || J >= n) && c != g.south_pole && c != g.north_pole) {
+                const int64_t o = g.node_src[c];
+                if (o < 0) throw std::runtime_error("reduction sweep reaches a halo cell without owner");
+                c = (size_t)o;
+              }
+              g.red_weight[c] += 1.0;
+            }
+  }
   (void)S;
   return g;
 }

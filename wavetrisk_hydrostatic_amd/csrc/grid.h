@@ -33,6 +33,7 @@ struct Level {
   std::vector<double> hex_inv;   // 1 / hexagon area
   std::vector<double> triarea;   // triangle area, planes LORT | UPLT
   std::vector<double> coriolis;  // f * dual area, planes LORT | UPLT
+  std::vector<double> red_weight;  // visits of the reference's reduction sweep, folded onto owners
 };
 
 // min_level, domain_level: the reference's compile-time MIN_LEVEL and

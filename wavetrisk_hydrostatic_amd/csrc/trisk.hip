@@ -28,8 +28,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/wavetrisk_gpu.h"
-#include "grid.h"
+#include "ctx.h"
 
 namespace {
 
@@ -776,6 +775,41 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
   w[4] = 0.5 - acc4 * hi;
 }
 
+// 16-byte loads of region rows.  A region row (ri = 0..18) is read as ten
+// aligned pairs of cells starting one cell to its left (ri = -1..18): the
+// cell left of a tile region has an even index, so every pair is 16-byte
+// aligned.  Lane slot r < 190 of a plane covers row r / 10, pair r % 10; a
+// plane takes three waves (192 slots, the last two clamped).
+constexpr int PAIRS = 10, PSLOTS = R * PAIRS;  // 190
+
+__device__ __forceinline__ double2 ldo2(const double* p, uint32_t off) {
+  return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(p) + off);
+}
+
+struct PairSlot {
+  uint32_t off;  // byte offset of the pair in a plane
+  int li;        // LDS index of the pair's first cell (ri may be -1)
+  bool valid;    // slot < 190
+};
+
+__device__ __forceinline__ PairSlot pair_slot(int r, int64_t base, int S) {
+  PairSlot ps;
+  ps.valid = r < PSLOTS;
+  const int rc = r < PSLOTS ? r : PSLOTS - 1;
+  const int prj = rc / PAIRS, pq = rc - prj * PAIRS;
+  ps.off = (uint32_t)(base - 1 + (int64_t)prj * S + 2 * pq) * 8u;
+  ps.li = prj * R + 2 * pq - 1;
+  return ps;
+}
+
+// writes a loaded pair into an LDS plane (the cell at ri = -1 is dropped)
+__device__ __forceinline__ void put_pair(double* plane, const PairSlot& ps, double2 v) {
+  if (ps.valid) {
+    if (ps.li % R != R - 1) plane[ps.li] = v.x;  // li = rj * R - 1 is ri = -1 of row rj
+    plane[ps.li + 1] = v.y;
+  }
+}
+
 __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a) {
   __shared__ GeoS gs;
   __shared__ union {
@@ -822,20 +856,23 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   // waited for before its use.
   const int lo = lt % (T * T);
   const uint32_t obl = (uint32_t)gcell(1 + lo % T, 1 + lo / T) * 8u;
+  const int wv = __builtin_amdgcn_readfirstlane(lt / 64);  // wave in the group
+  const int lane = lt % 64;
 
-  // --- launch-invariant geometry into LDS (each group stages half) ----------
-  if (active) {
-    if (grp == 0) {
-      for (int e = 0; e < 3; ++e) {
-        gs.len[e][x] = ldo(g.len + e * NC, xb);
-        gs.ped[e][x] = ldo(g.ped + e * NC, xb);
-        gs.pedf[e][x] = ldo(g.pedf + e * NC, xb);
-      }
-    } else {
-      for (int p = 0; p < 6; ++p) gs.area[p][x] = ldo(g.area + p * NC, xb);
-      gs.hinv[x] = ldo(g.hinv, xb);
-      gs.cor[0][x] = ldo(g.cor, xb);
-      gs.cor[1][x] = ldo(g.cor + NC, xb);
+  // --- launch-invariant geometry into LDS: 18 planes x 3 slot waves, spread
+  // over the block's 12 waves -------------------------------------------------
+  {
+    const int W = grp * 6 + wv;
+    double* gplane0 = &gs.len[0][0];
+    for (int sw = W; sw < 18 * 3; sw += 12) {
+      const int pl = sw / 3;
+      const PairSlot ps = pair_slot((sw % 3) * 64 + lane, base, S);
+      const double* src = pl < 3 ? g.len + pl * NC
+                          : pl < 6 ? g.ped + (pl - 3) * NC
+                          : pl < 9 ? g.pedf + (pl - 6) * NC
+                          : pl < 15 ? g.area + (pl - 9) * NC
+                          : pl == 15 ? g.hinv : g.cor + (pl - 16) * NC;
+      put_pair(gplane0 + pl * RR, ps, ldo2(src, ps.off));
     }
   }
 
@@ -846,14 +883,16 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   double P = 0, G = 0;
   {
     constexpr int KMAX = (int)(sizeof(lsu.flat) / sizeof(double)) / RR;  // layers per round
+    const PairSlot ps = pair_slot((wv % 3) * 64 + lane, base, S);
     double s = 0.0;
     for (int k0 = 0; k0 < a.L; k0 += KMAX) {
       const int kn = min(KMAX, a.L - k0);
-      if (active) {
-        for (int kk = grp; kk < kn; kk += 2) {
-          const int64_t ks = (int64_t)(k0 + kk) * NC;
-          lsu.flat[kk * RR + x] = ldo(a.ym + ks, xb) - ldo(a.yt + ks, xb);
-        }
+      // the group's layers kk = grp, grp + 2, ...; waves 0-2 and 3-5 alternate
+#pragma unroll 4
+      for (int kk = grp + 2 * (wv / 3); kk < kn; kk += 4) {
+        const int64_t ks = (int64_t)(k0 + kk) * NC;
+        const double2 vm = ldo2(a.ym + ks, ps.off), vt = ldo2(a.yt + ks, ps.off);
+        put_pair(lsu.flat + kk * RR, ps, make_double2(vm.x - vt.x, vm.y - vt.y));
       }
       __syncthreads();
       if (p_cell)
@@ -868,23 +907,32 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
 
   LayerS& sl = lsu.ls[grp];
   const LayerS& so = lsu.ls[1 - grp];
+  // layer staging slots: slot wave sw in [0, 15) covers plane sw / 3 (m, t,
+  // u0, u1, u2) part sw % 3; wave wv takes sw = wv, wv + 6 and, for wv < 3,
+  // wv + 12.  Plane pl of layer k:
+  auto lplane = [&](int pl, int k) -> const double* {
+    return pl == 0 ? a.ym + (int64_t)k * NC : pl == 1 ? a.yt + (int64_t)k * NC : a.yu + (int64_t)(3 * k + pl - 2) * NC;
+  };
+  const int sw0 = wv, sw1 = wv + 6, sw2 = wv + 12;
+  const bool has2 = wv < 3;
+  // slots are recomputed where used (a few integer operations) rather than
+  // held in registers through the layer loop
+  auto lslot = [&](int sw) { return pair_slot((sw % 3) * 64 + lane, base, S); };
   // the group's first layer, staged here; later layers are staged at the end
   // of the previous iteration from registers loaded one iteration ahead
-  double m, t, u0, u1, u2;
   {
-    const int64_t ks = (int64_t)min(grp, a.L - 1) * NC, ke3 = 3 * ks;
-    m = ldo(a.ym + ks, xb);
-    t = ldo(a.yt + ks, xb);
-    u0 = ldo(a.yu + ke3, xb);
-    u1 = ldo(a.yu + ke3 + NC, xb);
-    u2 = ldo(a.yu + ke3 + 2 * NC, xb);
-  }
-  if (active && grp < a.L) {
-    sl.m[x] = m;
-    sl.t[x] = t;
-    sl.u[0][x] = u0;
-    sl.u[1][x] = u1;
-    sl.u[2][x] = u2;
+    const int k = min(grp, a.L - 1);
+    const PairSlot ls0 = lslot(sw0), ls1 = lslot(sw1), ls2 = lslot(sw2);
+    const double2 v0 = ldo2(lplane(sw0 / 3, k), ls0.off);
+    const double2 v1 = ldo2(lplane(sw1 / 3, k), ls1.off);
+    double2 v2 = make_double2(0.0, 0.0);
+    if (has2) v2 = ldo2(lplane(sw2 / 3, k), ls2.off);
+    if (grp < a.L) {
+      double* lp0 = &sl.m[0];
+      put_pair(lp0 + (sw0 / 3) * RR, ls0, v0);
+      put_pair(lp0 + (sw1 / 3) * RR, ls1, v1);
+      if (has2) put_pair(lp0 + (sw2 / 3) * RR, ls2, v2);
+    }
   }
   __builtin_amdgcn_s_waitcnt(0);  // nothing outstanding enters the layer loop
 
@@ -904,15 +952,22 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     const bool pair = 2 * it + 1 < a.L;       // both layers of the pair exist
     const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
     __syncthreads();
-    // --- 1. prefetch the group's next layer (its loads overlap phases 2-4) ----
-    double nm, nt, nu0, nu1, nu2;
+    // --- 1. prefetch the group's next layer (its loads overlap phases 2-4);
+    //        the cell's own state from LDS ---------------------------------------
+    double2 pf0, pf1, pf2 = make_double2(0.0, 0.0);
     {
-      const int64_t ks2 = (int64_t)min(k + 2, a.L - 1) * NC, ke32 = 3 * ks2;
-      nm = ldo(a.ym + ks2, xb);
-      nt = ldo(a.yt + ks2, xb);
-      nu0 = ldo(a.yu + ke32, xb);
-      nu1 = ldo(a.yu + ke32 + NC, xb);
-      nu2 = ldo(a.yu + ke32 + 2 * NC, xb);
+      const int kp = min(k + 2, a.L - 1);
+      pf0 = ldo2(lplane(sw0 / 3, kp), lslot(sw0).off);
+      pf1 = ldo2(lplane(sw1 / 3, kp), lslot(sw1).off);
+      if (has2) pf2 = ldo2(lplane(sw2 / 3, kp), lslot(sw2).off);
+    }
+    double m = 0, t = 0, u0 = 0, u1 = 0, u2 = 0;
+    if (active) {
+      m = sl.m[x];
+      t = sl.t[x];
+      u0 = sl.u[0][x];
+      u1 = sl.u[1][x];
+      u2 = sl.u[2][x];
     }
 
     // --- 2. column scan; PV, fluxes; kinetic energy, Bernoulli, Exner ---------
@@ -970,8 +1025,8 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
 
     // --- 3. base state of the update (q = y for a trend), consumed in phase 4;
     //        qe on edges; potential temperature t/m in place ------------------
-    double bm, bt, bu0, bu1, bu2;
-    {
+    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
+    if (wv < 4) {  // the owned waves
       const int64_t ksb = (int64_t)min(k, a.L - 1) * NC, ke3b = 3 * ksb;
       bm = ldo(a.qm + ksb, obl);
       bt = ldo(a.qt + ksb, obl);
@@ -1045,7 +1100,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
         sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
       }
     }
-    if (!(lay && own)) {
+    if (wv < 4 && !(lay && own)) {
       // threads that store nothing still wait for their base-state loads
       // here, so no later phase inherits an unknown wait on them
       asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
@@ -1061,18 +1116,12 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     }
     __syncthreads();
     // --- 5. stage the group's next layer --------------------------------------
-    if (active && k + 2 < a.L) {
-      sl.m[x] = nm;
-      sl.t[x] = nt;
-      sl.u[0][x] = nu0;
-      sl.u[1][x] = nu1;
-      sl.u[2][x] = nu2;
+    if (k + 2 < a.L) {
+      double* lp0 = &sl.m[0];
+      put_pair(lp0 + (sw0 / 3) * RR, lslot(sw0), pf0);
+      put_pair(lp0 + (sw1 / 3) * RR, lslot(sw1), pf1);
+      if (has2) put_pair(lp0 + (sw2 / 3) * RR, lslot(sw2), pf2);
     }
-    m = nm;
-    t = nt;
-    u0 = nu0;
-    u1 = nu1;
-    u2 = nu2;
   }
 }
 
@@ -1115,44 +1164,14 @@ __global__ void copy_frozen_kernel(int L, int64_t NC, const int64_t* __restrict_
   }
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // host runtime
 // ---------------------------------------------------------------------------
 thread_local std::string g_create_error;
 
-}  // namespace
-
-struct wtg_grid {
-  wtg::Level lv;
-};
-
-struct wtg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  wtg_grid_info info{};
-  wtg_params params{};
-  int nslots = 0;
-  int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
-  bool topo_set = false;
-  std::string err;
-  // device geometry
-  double *d_len = nullptr, *d_ped = nullptr, *d_pedf = nullptr, *d_area = nullptr, *d_hinv = nullptr;
-  double *d_tri = nullptr, *d_cor = nullptr, *d_topo = nullptr;
-  int64_t *d_nd = nullptr, *d_ed = nullptr, *d_fn = nullptr, *d_fe = nullptr;
-  int8_t* d_es = nullptr;
-  int nn = 0, ne = 0, nfn = 0, nfe = 0;
-  std::vector<int64_t> node_src;
-  // state slots: user slots then two stage buffers
-  struct Slot {
-    double *m = nullptr, *t = nullptr, *u = nullptr;
-  };
-  std::vector<Slot> slots;
-  std::vector<void*> allocs;
-};
-
-namespace {
-
-int fail(wtg_ctx* c, int code, const char* fmt, ...) {
+int wtg_fail(wtg_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -1165,20 +1184,17 @@ int fail(wtg_ctx* c, int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(ctx, call)                                                                        \
-  do {                                                                                           \
-    hipError_t e_ = (call);                                                                      \
-    if (e_ != hipSuccess) return fail(ctx, WTG_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
 
+int fail(wtg_ctx* c, int code, const char* msg) { return wtg_fail(c, code, "%s", msg); }
+template <class... A>
+int fail(wtg_ctx* c, int code, const char* fmt, A... args) {
+  return wtg_fail(c, code, fmt, args...);
+}
+#define HIPCHK WTG_HIPCHK
 template <class Tp>
 int dalloc(wtg_ctx* c, Tp** p, size_t count) {
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, count * sizeof(Tp) > 0 ? count * sizeof(Tp) : 1);
-  if (e != hipSuccess) return fail(c, WTG_E_DEVICE, "hipMalloc(%zu bytes): %s", count * sizeof(Tp), hipGetErrorString(e));
-  c->allocs.push_back(q);
-  *p = (Tp*)q;
-  return 0;
+  return wtg_dalloc(c, p, count);
 }
 
 int check_slot(wtg_ctx* c, int slot) {
@@ -1325,6 +1341,7 @@ int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes
   else if (s == "node_src") { src = g.node_src.data(); n = g.node_src.size() * 8; }
   else if (s == "edge_src") { src = g.edge_src.data(); n = g.edge_src.size() * 8; }
   else if (s == "edge_sign") { src = g.edge_sign.data(); n = g.edge_sign.size(); }
+  else if (s == "red_weight") { src = g.red_weight.data(); n = g.red_weight.size() * 8; }
   else return fail(nullptr, WTG_E_ARG, "unknown grid array '%s'", name);
   if (bytes != n) return fail(nullptr, WTG_E_ARG, "grid array '%s' has %zu bytes, %zu given", name, n, bytes);
   std::memcpy(out, src, n);
@@ -1363,6 +1380,7 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
   std::vector<int64_t> nd, ed, fn, fe;
   std::vector<int8_t> es;
   const int n = c->info.n, S = c->info.S;
+  const int n_ = n, S_ = S;
   for (int64_t cc = 0; cc < NC; ++cc) {
     int64_t rr = cc % ((int64_t)S * S);
     int j = (int)(rr / S) - H, i = (int)(rr % S) - H;
@@ -1381,6 +1399,34 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
       if (se < 0) fe.push_back(e * NC + cc);
     }
   }
+  // free-surface problem: flux lengths, node compute list, reduction weights
+  std::vector<double> lenf(3 * NC);
+  for (int64_t cc = 0; cc < NC; ++cc)
+    for (int e = 0; e < 3; ++e) {
+      int64_t slot = 3 * cc + e, s = g.edge_src[slot];
+      lenf[e * NC + cc] = (s >= 0 && s != slot) ? g.len[(s % 3) * NC + s / 3] : g.len[e * NC + cc];
+    }
+  std::vector<int64_t> ncl, rcell;
+  std::vector<double> rw;
+  for (int64_t cc = 0; cc < NC; ++cc) {
+    int64_t rr = cc % ((int64_t)S_ * S_);
+    int j = (int)(rr / S_) - H, i = (int)(rr % S_) - H;
+    if ((i >= 0 && i < n_ && j >= 0 && j < n_) || cc == (int64_t)g.south_pole || cc == (int64_t)g.north_pole) {
+      ncl.push_back(cc);
+      ncl.push_back(cc);
+    }
+    if (g.red_weight[cc] > 0) {
+      rcell.push_back(cc);
+      rw.push_back(g.red_weight[cc]);
+    }
+  }
+  c->nmaster = (int)(ncl.size() / 2);
+  for (size_t q = 0; q < nd.size(); q += 2) {
+    ncl.push_back(nd[q]);
+    ncl.push_back(nd[q + 1]);
+  }
+  c->nncl = (int)(ncl.size() / 2);
+  c->nred = (int)rcell.size();
   c->nn = (int)(nd.size() / 2);
   c->ne = (int)(ed.size() / 2);
   c->nfn = (int)fn.size();
@@ -1400,7 +1446,9 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
       (rc = up(&c->d_tri, g.triarea.data(), g.triarea.size())) ||
       (rc = up(&c->d_cor, g.coriolis.data(), g.coriolis.size())) || (rc = up(&c->d_nd, nd.data(), nd.size())) ||
       (rc = up(&c->d_ed, ed.data(), ed.size())) || (rc = up(&c->d_es, es.data(), es.size())) ||
-      (rc = up(&c->d_fn, fn.data(), fn.size())) || (rc = up(&c->d_fe, fe.data(), fe.size()))) {
+      (rc = up(&c->d_fn, fn.data(), fn.size())) || (rc = up(&c->d_fe, fe.data(), fe.size())) ||
+      (rc = up(&c->d_lenf, lenf.data(), lenf.size())) || (rc = up(&c->d_ncl, ncl.data(), ncl.size())) ||
+      (rc = up(&c->d_rcell, rcell.data(), rcell.size())) || (rc = up(&c->d_rw, rw.data(), rw.size()))) {
     g_create_error = c->err;
     wtg_destroy(c.release());
     return rc;
@@ -1436,6 +1484,7 @@ void wtg_destroy(wtg_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) hipFree(p);
+  if (c->h_scal) hipHostFree(c->h_scal);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
