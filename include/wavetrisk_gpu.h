@@ -79,6 +79,11 @@ int wtg_grid_info_get(const wtg_grid* grid, wtg_grid_info* info);
  * "edge_src" [3*ncells] (int64; -1 = frozen halo slot, see below);
  * "edge_sign" [3*ncells] (int8).  bytes must equal the array's size. */
 int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes);
+/* Overlap areas of the scalar wavelet transform between a grid and the next
+ * coarser one (get_overl_areas / set_RF_wgts, wavelet.f90:1408-1539):
+ * [4][fine ncells] doubles, a(1..4) at fine nodes that are coarse edge
+ * midpoints, 0 elsewhere.  Both grids need the same min_level/domain_level. */
+int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double* out, size_t bytes);
 void wtg_grid_destroy(wtg_grid* grid);
 
 /* ---- device context ------------------------------------------------------ */

@@ -41,6 +41,10 @@ def load():
         _lib.wto_bicgstab.restype = C.c_int
         _lib.wto_flux_divergence.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.wto_rhs_elliptic.argtypes = [C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 4
+        _lib.wto_scalar_wavelets.argtypes = [C.c_void_p] * 4
+        _lib.wto_restrict.argtypes = [C.c_void_p] * 6
+        _lib.wto_prolong.argtypes = [C.c_void_p] * 6
+        _lib.wto_fmg.argtypes = [C.c_void_p] * 5
     return _lib
 
 
@@ -178,3 +182,75 @@ class EllipticOracle(Oracle):
         self.lib.wto_rhs_elliptic(C.byref(self.g), C.byref(self.fs), self.rho0, eta.ctypes.data, divF.ctypes.data,
                                   divF_old.ctypes.data, out.ctypes.data)
         return out
+
+
+class _Mg(C.Structure):
+    _fields_ = [("nlevels", C.c_int), ("grid", C.c_void_p * 32), ("ovl", C.c_void_p * 32),
+                ("eta_old", C.c_void_p * 32), ("weight", C.c_void_p * 32), ("p", FsParams),
+                ("coarse_tol", C.c_double), ("fine_tol", C.c_double), ("coarse_iter", C.c_int),
+                ("max_vcycle", C.c_int), ("pre_iter", C.c_int), ("down_iter", C.c_int), ("up_iter", C.c_int),
+                ("post_iter", C.c_int)]
+
+
+class MgOracle:
+    """Scalar wavelet transforms and FMG (oracle/mg_oracle.c) on a hierarchy
+    of grids (coarse to fine) with topography and eta_old per level."""
+
+    def __init__(self, grids, topos, eta_olds=None, grav=9.80616, rho0=1028.0, dt=600.0, theta1=1.0, theta2=1.0,
+                 coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=50, pre_iter=2, down_iter=2, up_iter=2,
+                 post_iter=4):
+        self.grids = grids
+        self.levels = [EllipticOracle(g.layout, geometry_of(g), t, grav=grav, rho0=rho0, dt=dt, theta1=theta1,
+                                      theta2=theta2) for g, t in zip(grids, topos)]
+        self.ovl = [None] + [_f64(grids[l].overlap_areas(grids[l - 1])).ravel() for l in range(1, len(grids))]
+        self.eta_old = [None] * len(grids)
+        self.h = _Mg()
+        self.h.nlevels = len(grids)
+        self.h.p = FsParams(grav, dt, theta1, theta2)
+        self.h.coarse_tol, self.h.fine_tol = coarse_tol, fine_tol
+        self.h.coarse_iter, self.h.max_vcycle = coarse_iter, max_vcycle
+        self.h.pre_iter, self.h.down_iter, self.h.up_iter, self.h.post_iter = pre_iter, down_iter, up_iter, post_iter
+        for l, lv in enumerate(self.levels):
+            self.h.grid[l] = C.addressof(lv.g)
+            self.h.weight[l] = lv.weight.ctypes.data
+            if l:
+                self.h.ovl[l] = self.ovl[l].ctypes.data
+        if eta_olds is not None:
+            self.set_eta_old(eta_olds)
+        self.lib = load()
+
+    def set_eta_old(self, eta_olds):
+        self.eta_old = [lv.node_ghost(e) for lv, e in zip(self.levels, eta_olds)]
+        for l, e in enumerate(self.eta_old):
+            self.h.eta_old[l] = e.ctypes.data
+
+    def wavelets(self, l, s_fine):
+        s = self.levels[l].node_ghost(s_fine)
+        wc = np.zeros_like(s)
+        self.lib.wto_scalar_wavelets(C.byref(self.levels[l].g), self.ovl[l].ctypes.data, s.ctypes.data, wc.ctypes.data)
+        return wc
+
+    def restrict(self, l, s_fine, wc):
+        s, wc = _f64(s_fine), _f64(wc)
+        out = np.zeros(self.grids[l - 1].layout.ncells)
+        self.lib.wto_restrict(C.byref(self.levels[l].g), C.byref(self.levels[l - 1].g), self.ovl[l].ctypes.data,
+                              s.ctypes.data, wc.ctypes.data, out.ctypes.data)
+        return out
+
+    def prolong(self, l, s_coarse, wc, s_fine=None):
+        sc, wc = _f64(s_coarse), _f64(wc)
+        out = np.zeros(self.grids[l].layout.ncells) if s_fine is None else _f64(s_fine).copy()
+        self.lib.wto_prolong(C.byref(self.levels[l].g), C.byref(self.levels[l - 1].g), self.ovl[l].ctypes.data,
+                             sc.ctypes.data, wc.ctypes.data, out.ctypes.data)
+        return out
+
+    def fmg(self, u, f):
+        nl = len(self.grids)
+        u = [self.levels[l].node_ghost(u[l]) for l in range(nl)]
+        f = [self.levels[l].node_ghost(f[l]) for l in range(nl)]
+        U = (C.c_void_p * nl)(*[a.ctypes.data for a in u])
+        F = (C.c_void_p * nl)(*[a.ctypes.data for a in f])
+        iters = np.zeros(nl, dtype=np.int32)
+        res = np.zeros(nl)
+        self.lib.wto_fmg(C.byref(self.h), U, F, iters.ctypes.data, res.ctypes.data)
+        return u, iters, res

@@ -61,6 +61,7 @@ SIGNATURES = {
     "wtg_grid_info_get": (C.c_int, [_P, C.POINTER(_GridInfo)]),
     "wtg_grid_get": (C.c_int, [_P, C.c_char_p, _P, C.c_size_t]),
     "wtg_grid_destroy": (None, [_P]),
+    "wtg_grid_overlap_areas": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "wtg_create": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.POINTER(_P)]),
     "wtg_destroy": (None, [_P]),
     "wtg_set_topography": (C.c_int, [_P, _P]),
@@ -154,6 +155,14 @@ class Grid:
                "hex_inv": (1, np.float64), "triarea": (2, np.float64), "coriolis": (2, np.float64),
                "node_src": (1, np.int64), "edge_src": (3, np.int64), "edge_sign": (3, np.int8),
                "red_weight": (1, np.float64)}
+
+    def overlap_areas(self, coarse: "Grid") -> np.ndarray:
+        """a(1..4) of the scalar wavelet transform between this grid and the
+        next coarser one, [4, ncells]."""
+        out = np.empty(4 * self.layout.ncells)
+        _check(load_library().wtg_grid_overlap_areas(self._h, coarse._h, _ptr(out), out.nbytes), None,
+               "wtg_grid_overlap_areas")
+        return out.reshape(4, self.layout.ncells)
 
     def array(self, name: str) -> np.ndarray:
         comp, dt = self._SHAPES[name]

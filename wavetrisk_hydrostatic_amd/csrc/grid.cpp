@@ -565,9 +565,9 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
   for (int lz = 0; lz < 10; ++lz) {
     auto e = [&](int i, int j) { return ecell(lz, i, j); };
     auto P = [&](int i, int j) { return XE[e(i, j)]; };
-    // circumcentres (init.f90:707-723), cells [-2, n+1]
-    for (int j = -2; j <= n + 1; ++j)
-      for (int i = -2; i <= n + 1; ++i) {
+    // circumcentres (init.f90:707-723), cells [-3, n+1]
+    for (int j = -EH; j <= n + 1; ++j)
+      for (int i = -EH; i <= n + 1; ++i) {
         CL[e(i, j)] = circumcentre(P(i, j), P(i + 1, j + 1), P(i + 1, j), R);
         CU[e(i, j)] = circumcentre(P(i, j), P(i, j + 1), P(i + 1, j + 1), R);
       }
@@ -662,6 +662,17 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
         HI[c] = HI[ecell(l2, i2, j2)];
       }
 
+  g.EH = EH;
+  g.ES = ES;
+  g.xe.resize(3 * ENC);
+  g.cle.resize(3 * ENC);
+  g.cue.resize(3 * ENC);
+  for (size_t x = 0; x < ENC; ++x) {
+    g.xe[3 * x] = XE[x].x; g.xe[3 * x + 1] = XE[x].y; g.xe[3 * x + 2] = XE[x].z;
+    g.cle[3 * x] = CL[x].x; g.cle[3 * x + 1] = CL[x].y; g.cle[3 * x + 2] = CL[x].z;
+    g.cue[3 * x] = CU[x].x; g.cue[3 * x + 1] = CU[x].y; g.cue[3 * x + 2] = CU[x].z;
+  }
+
   // --- 7. copy the tile window out of the virtual frame ------------------------
   g.X.assign(3 * NC, 0.0);
   g.len.assign(3 * NC, 0.0);
@@ -714,6 +725,120 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
   }
   (void)S;
   return g;
+}
+
+
+// ---------------------------------------------------------------------------
+// Overlap areas between two levels
+// ---------------------------------------------------------------------------
+namespace {
+
+inline double inner(V3 u, V3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+
+// sphere.f90:102-140 arc_inters: intersection of the great-circle arcs
+// (a1, a2) and (b1, b2); the point is b2 when a2 and b2 coincide
+bool arc_inters(V3 a1, V3 a2, V3 b1, V3 b2, V3& pt, double R) {
+  pt = b2;
+  const double eps = R * 1e-13;
+  if (nrm(sub(b2, a2)) < eps) return true;
+  V3 n1 = cross(a1, a2);
+  double inpr = inner(n1, b1) * inner(n1, b2);
+  if (inpr > 0.0) return false;
+  V3 n2 = cross(b1, b2);
+  inpr = inner(n2, a1) * inner(n2, a2);
+  if (inpr > 0.0) return false;
+  pt = project(cross(n1, n2), R);
+  V3 neg{-pt.x, -pt.y, -pt.z};
+  if (nrm(sub(a1, neg)) < nrm(sub(a1, pt))) pt = neg;
+  return true;
+}
+
+struct Tri {
+  int di, dj, t;  // triangle of node (i+di, j+dj): t = 0 LORT, 1 UPLT
+};
+// shared.f90:308-327 (no_adj_tri, adj_tri, bfly_tri), hex_s_offs = (2, 0, 4)
+constexpr Tri NO_ADJ[10] = {{0, 0, 0},   {0, -1, 1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, 0},
+                            {0, 0, 1},   {0, 0, 0},  {0, -1, 1},  {-1, -1, 0}, {-1, -1, 1}};
+constexpr Tri ADJ[3][2] = {{{0, -1, 1}, {0, 0, 0}}, {{0, 0, 1}, {0, 0, 0}}, {{0, 0, 1}, {-1, 0, 0}}};
+constexpr Tri BFLY[3][4] = {{{-1, -1, 0}, {0, -1, 0}, {1, 0, 1}, {0, 0, 1}},
+                            {{0, 1, 0}, {-1, 0, 0}, {0, -1, 1}, {1, 0, 1}},
+                            {{0, 0, 0}, {0, 1, 0}, {-1, 0, 1}, {-1, -1, 1}}};
+constexpr int HEX_S_OFFS[3] = {2, 0, 4};
+
+}  // namespace
+
+std::vector<double> overlap_areas(const Level& fine, const Level& coarse) {
+  if (fine.n != 2 * coarse.n) throw std::runtime_error("overlap_areas: levels are not consecutive");
+  const size_t NC = fine.ncells();
+  const double R = fine.radius;
+  std::vector<double> a(4 * NC, 0.0);
+  auto V = [](const std::vector<double>& v, size_t x) { return V3{v[3 * x], v[3 * x + 1], v[3 * x + 2]}; };
+  const int nf = fine.n, nc = coarse.n;
+  for (int lz = 0; lz < 10; ++lz) {
+    auto fcc = [&](int i, int j, const Tri& t) {
+      size_t x = fine.ecell(lz, i + t.di, j + t.dj);
+      return t.t == 0 ? V(fine.cle, x) : V(fine.cue, x);
+    };
+    auto ccc = [&](int i, int j, const Tri& t) {
+      size_t x = coarse.ecell(lz, i + t.di, j + t.dj);
+      return t.t == 0 ? V(coarse.cle, x) : V(coarse.cue, x);
+    };
+    // parents whose children fall in the fine tile window with a complete stencil
+    for (int jp = -1; jp <= nc; ++jp)
+      for (int ip = -1; ip <= nc; ++ip)
+        for (int e = 0; e < 3; ++e) {
+          const int ic = 2 * ip + (e == 0 || e == 1 ? 1 : 0), jc = 2 * jp + (e == 1 || e == 2 ? 1 : 0);
+          if (ic < -HALO || jc < -HALO || ic >= nf + HALO || jc >= nf + HALO) continue;
+          // hexagon corners: fine circumcentres around the child (needs fine cells >= -EH)
+          if (ic - 1 < -fine.EH || jc - 1 < -fine.EH || ic > nf + 1 || jc > nf + 1) continue;
+          if (ip - 1 < -coarse.EH || jp - 1 < -coarse.EH || ip + 1 > nc + 1 || jp + 1 > nc + 1) continue;
+          V3 hex[6];
+          const int so = HEX_S_OFFS[2 - e];
+          for (int k = 0; k < 6; ++k) hex[k] = fcc(ic, jc, NO_ADJ[k + so]);
+          // coarse circumcentres around the parent edge: tri(row, col)
+          V3 tri[3][2];
+          tri[0][0] = ccc(ip, jp, BFLY[e][3]);
+          tri[1][0] = ccc(ip, jp, ADJ[e][1]);
+          tri[2][0] = ccc(ip, jp, BFLY[e][2]);
+          tri[0][1] = ccc(ip, jp, BFLY[e][1]);
+          tri[1][1] = ccc(ip, jp, ADJ[e][0]);
+          tri[2][1] = ccc(ip, jp, BFLY[e][0]);
+          const V3 pt = V(fine.xe, fine.ecell(lz, ic, jc));
+          double area[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          area[0] = triarea(hex[5], hex[0], pt, R);
+          area[1] = triarea(hex[2], hex[3], pt, R);
+          for (int i = 1; i <= 2; ++i) {
+            const int c = i - 1;                     // tri column
+            const V3 h1 = hex[3 * i - 3], h2 = hex[3 * i - 2], h3 = hex[3 * i - 1];  // hex(3i-2), hex(3i-1), hex(3i)
+            V3 p0, p1;
+            bool d0 = arc_inters(tri[0][c], tri[1][c], h1, h2, p0, R);
+            bool d1 = arc_inters(tri[2][c], tri[1][c], h3, h2, p1, R);
+            double& ai = area[i - 1];
+            double& aj = area[2 - i];   // area(-i+3)
+            if (d0 && d1) {
+              area[i + 3] = triarea(p0, tri[1][c], h2, R);
+              area[i + 5] = triarea(tri[1][c], h2, p1, R);
+              area[i + 1] = area[i + 3] + area[i + 5];
+              ai = ai + triarea(h1, p0, pt, R) + triarea(p0, pt, tri[1][c], R);
+              aj = aj + triarea(p1, h3, pt, R) + triarea(tri[1][c], pt, p1, R);
+            } else if (!d0 && !d1) {
+              area[i + 1] = 0.0;
+              d0 = arc_inters(tri[1][0], tri[1][1], h1, h2, p0, R);
+              d1 = arc_inters(tri[1][1], tri[1][0], h2, h3, p1, R);
+              if (!d0 && d1) {
+                ai = ai + triarea(h1, h2, pt, R) + triarea(h2, p1, pt, R);
+                aj = aj + triarea(p1, h3, pt, R);
+              } else if (d0 && !d1) {
+                ai = ai + triarea(h1, p0, pt, R);
+                aj = aj + triarea(h2, h3, pt, R) + triarea(p0, h2, pt, R);
+              }
+            }
+          }
+          const size_t cf = fine.cell(lz, ic, jc);
+          for (int k = 0; k < 4; ++k) a[k * NC + cf] = area[k];
+        }
+  }
+  return a;
 }
 
 }  // namespace wtg

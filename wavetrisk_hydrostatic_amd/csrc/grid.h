@@ -34,7 +34,24 @@ struct Level {
   std::vector<double> triarea;   // triangle area, planes LORT | UPLT
   std::vector<double> coriolis;  // f * dual area, planes LORT | UPLT
   std::vector<double> red_weight;  // visits of the reference's reduction sweep, folded onto owners
+
+  // each lozenge's extended frame (cells [-EH, n+EH), EH = HALO + 1): node
+  // coordinates and triangle circumcentres (LORT, UPLT), xyz interleaved;
+  // kept for the inter-level geometry of the wavelet transforms
+  int EH = 0, ES = 0;
+  std::vector<double> xe, cle, cue;  // [10*ES*ES][3]
+  size_t ecell(int lz, int i, int j) const {
+    return (size_t)lz * ES * ES + (size_t)(j + EH) * ES + (size_t)(i + EH);
+  }
 };
+
+// Overlap areas of the scalar wavelet transform between a level and the next
+// finer one (get_overl_areas, wavelet.f90:1408-1475, stored by set_RF_wgts
+// 1512-1539): for every fine node that is the midpoint of a coarse edge,
+// a(1..4) = areas of its dual hexagon shared with the dual cells of the
+// edge's two end points and two opposite nodes.  Returns [4][fine ncells];
+// zero at fine nodes that coincide with coarse nodes.
+std::vector<double> overlap_areas(const Level& fine, const Level& coarse);
 
 // min_level, domain_level: the reference's compile-time MIN_LEVEL and
 // DOMAIN_LEVEL (src/param_J*.f90).  They fix how node coordinates are rounded

@@ -788,8 +788,9 @@ __device__ __forceinline__ double2 ldo2(const double* p, uint32_t off) {
 
 struct PairSlot {
   uint32_t off;  // byte offset of the pair in a plane
-  int li;        // LDS index of the pair's first cell (ri may be -1)
+  int li;        // LDS index of the pair's second cell (ri = 2 * pair)
   bool valid;    // slot < 190
+  bool first;    // the pair's first cell (ri = 2 * pair - 1) is in the region (pair > 0)
 };
 
 __device__ __forceinline__ PairSlot pair_slot(int r, int64_t base, int S) {
@@ -798,15 +799,16 @@ __device__ __forceinline__ PairSlot pair_slot(int r, int64_t base, int S) {
   const int rc = r < PSLOTS ? r : PSLOTS - 1;
   const int prj = rc / PAIRS, pq = rc - prj * PAIRS;
   ps.off = (uint32_t)(base - 1 + (int64_t)prj * S + 2 * pq) * 8u;
-  ps.li = prj * R + 2 * pq - 1;
+  ps.li = prj * R + 2 * pq;
+  ps.first = pq > 0;
   return ps;
 }
 
 // writes a loaded pair into an LDS plane (the cell at ri = -1 is dropped)
 __device__ __forceinline__ void put_pair(double* plane, const PairSlot& ps, double2 v) {
   if (ps.valid) {
-    if (ps.li % R != R - 1) plane[ps.li] = v.x;  // li = rj * R - 1 is ri = -1 of row rj
-    plane[ps.li + 1] = v.y;
+    if (ps.first) plane[ps.li - 1] = v.x;
+    plane[ps.li] = v.y;
   }
 }
 
@@ -1345,6 +1347,20 @@ int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes
   else return fail(nullptr, WTG_E_ARG, "unknown grid array '%s'", name);
   if (bytes != n) return fail(nullptr, WTG_E_ARG, "grid array '%s' has %zu bytes, %zu given", name, n, bytes);
   std::memcpy(out, src, n);
+  return 0;
+}
+
+int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double* out, size_t bytes) {
+  if (!fine || !coarse || !out) return fail(nullptr, WTG_E_ARG, "null argument");
+  if (fine->lv.level != coarse->lv.level + 1) return fail(nullptr, WTG_E_ARG, "grids are not consecutive levels");
+  const size_t n = 4 * fine->lv.ncells() * sizeof(double);
+  if (bytes != n) return fail(nullptr, WTG_E_ARG, "overlap areas have %zu bytes, %zu given", n, bytes);
+  try {
+    std::vector<double> a = wtg::overlap_areas(fine->lv, coarse->lv);
+    std::memcpy(out, a.data(), n);
+  } catch (const std::exception& e) {
+    return fail(nullptr, WTG_E_ARG, "%s", e.what());
+  }
   return 0;
 }
 
