@@ -91,6 +91,36 @@ static double lifting(const wto_grid* gf, const double* ovl, const double* wc, i
          wc[S2W] * a4[S2W] + wc[SW] * a1[SW] + wc[So] * a2[So] + wc[SW2] * a3[SW2] + wc[SE] * a4[SE];
 }
 
+/* Lifting at a pole (fine cell c is the pole master): the five edge
+ * midpoints next to the pole, with the area they share with the pole's dual
+ * cell (a2: the pole is their second end point), and the five DG midpoints of
+ * the second ring, with the area on the pole's side (a3 at the south pole,
+ * a4 at the north pole).  The reference's own pole stencil is its lozenge
+ * stencil evaluated through the patch halo (wavelet.f90:542-567), where two
+ * of the twelve neighbours fall on halo slots that no communication fills;
+ * at the south pole the remaining terms are exactly these ten.  The inverse
+ * (Prolong_scalar) drops the id2NE term, which at the poles is the DG
+ * midpoint of lozenge 1 (south) / lozenge 7 (north). */
+static double pole_lifting(const wto_grid* gf, const double* ovl, const double* wc, int64_t c, int prolong) {
+  const int64_t S = gf->S, NC = gf->ncells, T = S * S;
+  const int n = gf->n;
+  double sum = 0.0;
+  if (c == gf->south_pole) {
+    for (int k = 0; k < 5; ++k) {
+      const int64_t m = k * T + 2 * S + (n - 1 + 2), d = k * T + 3 * S + (n - 1 + 2);
+      sum += wc[m] * ovl[NC + m];
+      if (!(prolong && k == 1)) sum += wc[d] * ovl[2 * NC + d];
+    }
+  } else {
+    for (int k = 5; k < 10; ++k) {
+      const int64_t m = k * T + (n - 1 + 2) * S + 2, d = k * T + (n - 1 + 2) * S + 3;
+      sum += wc[m] * ovl[NC + m];
+      if (!(prolong && k == 7)) sum += wc[d] * ovl[3 * NC + d];
+    }
+  }
+  return sum;
+}
+
 static int64_t child(const wto_grid* gf, const wto_grid* gc, int64_t p) {
   int i, j;
   ij_of(gc, p, &i, &j);
@@ -103,7 +133,8 @@ void wto_restrict(const wto_grid* gf, const wto_grid* gc, const double* ovl, con
   for (int64_t p = 0; p < gc->ncells; ++p) {
     if (!is_master(gc, p)) continue;
     const int64_t c = child(gf, gc, p);
-    s_c[p] = s_f[c] + lifting(gf, ovl, wc, c, 1) * gc->hex_inv[p];
+    const int pole = p == gc->south_pole || p == gc->north_pole;
+    s_c[p] = s_f[c] + (pole ? pole_lifting(gf, ovl, wc, c, 0) : lifting(gf, ovl, wc, c, 1)) * gc->hex_inv[p];
   }
   ghost(gc, s_c);
 }
@@ -114,7 +145,8 @@ void wto_prolong(const wto_grid* gf, const wto_grid* gc, const double* ovl, cons
   for (int64_t p = 0; p < gc->ncells; ++p) {
     if (!is_master(gc, p)) continue;
     const int64_t c = child(gf, gc, p);
-    s_f[c] = s_c[p] - lifting(gf, ovl, wc, c, 0) * gc->hex_inv[p];
+    const int pole = p == gc->south_pole || p == gc->north_pole;
+    s_f[c] = s_c[p] - (pole ? pole_lifting(gf, ovl, wc, c, 1) : lifting(gf, ovl, wc, c, 0)) * gc->hex_inv[p];
   }
   ghost(gf, s_f);
   /* midpoints: interpolate and add the wavelet */

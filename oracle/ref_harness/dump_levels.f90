@@ -22,6 +22,7 @@ module lev_ic
   use geom_mod
   implicit none
   type(Float_Field), target :: xf, etan, ff
+  integer :: hu, cur_dom, cur_l
 contains
   real(8) function bottom (p)
     type(Coord) :: p
@@ -83,6 +84,52 @@ contains
     etan%data(d)%elts(id) = eta_old (dom%node%elts(id))
     ff%data(d)%elts(id)   = f_field (dom%node%elts(id))
   end subroutine init_test_fields
+
+  subroutine pole_stencil (dom, i_par, j_par, i_chd, j_chd, zlev, offs_par, dims_par, offs_chd, dims_chd)
+    ! records, for a parent that is a pole, the child and the twelve lifting
+    ! neighbours in the order of Restrict_scalar (wavelet.f90:542-553)
+    type(Domain)                   :: dom
+    integer                        :: i_par, j_par, i_chd, j_chd, zlev
+    integer, dimension(N_BDRY+1)   :: offs_par, offs_chd
+    integer, dimension(2,N_BDRY+1) :: dims_par, dims_chd
+    integer :: id_par, k
+    integer, dimension(13) :: ids
+    real(8), dimension(13) :: buf
+    id_par = idx (i_par, j_par, offs_par, dims_par)
+    if (abs (dom%node%elts(id_par+1)%z) < radius * (1d0 - 1d-12)) return
+    ids = (/ idx (i_chd, j_chd, offs_chd, dims_chd), &
+         idx (i_chd+1, j_chd,   offs_chd, dims_chd), idx (i_chd+1, j_chd+1, offs_chd, dims_chd), &
+         idx (i_chd+2, j_chd+1, offs_chd, dims_chd), idx (i_chd+1, j_chd+2, offs_chd, dims_chd), &
+         idx (i_chd,   j_chd+1, offs_chd, dims_chd), idx (i_chd-1, j_chd,   offs_chd, dims_chd), &
+         idx (i_chd-1, j_chd+1, offs_chd, dims_chd), idx (i_chd-2, j_chd-1, offs_chd, dims_chd), &
+         idx (i_chd-1, j_chd-1, offs_chd, dims_chd), idx (i_chd,   j_chd-1, offs_chd, dims_chd), &
+         idx (i_chd-1, j_chd-2, offs_chd, dims_chd), idx (i_chd+1, j_chd-1, offs_chd, dims_chd) /) + 1
+    call mwr_i ('pst_ids', (/ cur_dom, cur_l, id_par + 1, i_par, j_par, i_chd, j_chd, &
+         dom%mask_n%elts(ids(1)), ids /))
+    do k = 1, 4
+       buf = dom%overl_areas%elts(ids)%a(k); call mwr_f ('pst_ovl', buf)
+    end do
+    buf = dom%node%elts(ids)%x; call mwr_f ('pst_x', buf)
+    buf = dom%node%elts(ids)%y; call mwr_f ('pst_y', buf)
+    buf = dom%node%elts(ids)%z; call mwr_f ('pst_z', buf)
+    buf(1) = dom%areas%elts(id_par+1)%hex_inv; call mwr_f ('pst_hinv', buf(1:1))
+  end subroutine pole_stencil
+
+  subroutine mwr_i (tag, a)
+    character(*)          :: tag
+    integer, dimension(:) :: a
+    character(16)         :: t
+    t = tag
+    write (hu) t, 1_4, int(size(a),8), a
+  end subroutine mwr_i
+
+  subroutine mwr_f (tag, a)
+    character(*)          :: tag
+    real(8), dimension(:) :: a
+    character(16)         :: t
+    t = tag
+    write (hu) t, 2_4, int(size(a),8), a
+  end subroutine mwr_f
 
   subroutine no_update
     ! the test case's "update" hook (init.f90:120), called by adapt: nothing to re-evaluate
@@ -190,6 +237,13 @@ program dump_levels
   call wr_f ('params', (/ radius, omega, grav_accel, ref_density, dt, theta1, theta2, tol_solve /))
   do l = level_start, level_end
      call dump_level (l)
+  end do
+  ! the fine elements that Restrict_scalar reads for each pole parent
+  do l = level_start, level_end - 1
+     do d = 1, size(grid)
+        cur_dom = d; cur_l = l; hu = u
+        call apply_interscale_d (pole_stencil, grid(d), l, z_null, 0, 1)
+     end do
   end do
   call dump_field ('x', xf)
   call dump_field ('etan', etan)
@@ -310,6 +364,29 @@ contains
           buf = grid(dd)%overl_areas%elts(ids)%a(i); call wr_f ('lev_ovl', buf)
        end do
        deallocate (ids, buf)
+       ! nodes (-2..1, -2..1) around a pentagon at the south-west corner of a
+       ! level-l patch: element ids, overlap areas, coordinates
+       do j = 1, grid(dd)%lev(l)%length
+          p = grid(dd)%lev(l)%elts(j)
+          if (.not. is_penta (grid(dd), p, IJMINUS - 1)) cycle
+          call get_offs_Domain (grid(dd), p, offs, dims)
+          allocate (ids(16), buf(16))
+          m = 0
+          do jj = -2, 1
+             do i = -2, 1
+                m = m + 1
+                ids(m) = idx (i, jj, offs, dims) + 1
+             end do
+          end do
+          call wr_i ('pen_ids', (/ l, ids /))
+          do i = 1, 4
+             buf = grid(dd)%overl_areas%elts(ids)%a(i); call wr_f ('pen_ovl', buf)
+          end do
+          buf = grid(dd)%node%elts(ids)%x; call wr_f ('pen_x', buf)
+          buf = grid(dd)%node%elts(ids)%y; call wr_f ('pen_y', buf)
+          buf = grid(dd)%node%elts(ids)%z; call wr_f ('pen_z', buf)
+          deallocate (ids, buf)
+       end do
     end do
   end subroutine dump_level
 end program dump_levels

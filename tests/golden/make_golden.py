@@ -18,6 +18,8 @@ lozenge-tile layout:
                      and its diagonal, the (0,1)-sweep visit counts, l2/dp,
                      flux divergence, RHS, one Jacobi sweep and a BiCGSTAB
                      solve with its iteration count
+  levels_J4.npz      hierarchy J4..J6 (harness dump_levels.f90): forward and
+                     inverse scalar wavelet transforms and an FMG solve
 
 Only this container has /root/reference; the fixtures are committed and the
 GPU box reads them.
@@ -328,10 +330,86 @@ def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol):
     print(f"elliptic_{name}.npz written")
 
 
+def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
+    """Hierarchy J..maxlev of uniform levels (harness dump_levels.f90): per
+    level, on the owned nodes and the two pole masters, the scalar x, the
+    forward transform's restricted scalars (fst) and wavelets (fsw), the
+    inverse transform of (fsw, fst) (ist), eta^n, f, topography and the FMG
+    solution; the FMG iteration counts and final residuals per level from the
+    reference's log; overlap areas a(1..4) on the second level.
+
+    The reference keeps one pole copy per lozenge and its transforms let the
+    copies drift apart; the fixture takes the south pole from lozenge 0 and
+    the north pole from lozenge 5, the frames of this layout's pole masters."""
+    from scipy.spatial import cKDTree
+    sys.path.insert(0, ROOT)
+    import wavetrisk_hydrostatic_amd as W
+    param, ml, dl = PARAMS[J]
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "l.bin")
+        exe = os.path.join(ROOT, "oracle", "_ref", param, "dump_levels")
+        if not os.path.exists(exe):
+            subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile.ref"), f"PARAM={param}"],
+                           cwd=ROOT, check=True)
+        r = subprocess.run([exe], input=f"&run maxlev={maxlev}, zlev={L}, step={dt}d0, tol_solve={tol}, out='{p}' /\n",
+                           text=True, cwd=tmp, check=True, capture_output=True)
+        recs = read_records(p)
+    log = [ln.split() for ln in r.stdout.splitlines()]
+    fmg_log = {int(t[0]): (float(t[2]), int(t[3])) for t in log if len(t) == 4 and t[0].isdigit()}
+    meta = dict((t, a) for t, a in recs if t in ("min_level", "params"))
+    _, DL, PS, L_, ND, ls, le = (int(v) for v in meta["min_level"])
+    nsub = 4 ** DL
+    levels, fields, cur = {}, {}, None
+    for t, a in recs:
+        if t == "lev_ids":
+            cur = {"ids": a[1:] - 1, "ovl": []}
+            levels.setdefault(int(a[0]), []).append(cur)
+        elif t in ("lev_x", "lev_y", "lev_z", "lev_poles"):
+            cur[t] = a
+        elif t == "lev_ovl":
+            cur["ovl"].append(a)
+        elif t in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg"):
+            fields.setdefault(t, []).append(a)
+    out = dict(level_start=ls, level_end=le, min_level=ml, domain_level=dl, radius=meta["params"][0],
+               grav=meta["params"][2], ref_density=meta["params"][3], dt=meta["params"][4],
+               theta1=meta["params"][5], theta2=meta["params"][6], tol=meta["params"][7],
+               fmg_iters=np.array([fmg_log[l][1] for l in range(ls, le + 1)]),
+               fmg_res=np.array([fmg_log[l][0] for l in range(ls, le + 1)]))
+    for l in range(ls, le + 1):
+        g = W.Grid(level=l, min_level=ml, domain_level=dl)
+        X = g.array("x")
+        own = g.layout.owned_cells(with_poles=False)
+        tree = cKDTree(X[:, own].T)
+        NC = g.layout.ncells
+        maps = []
+        for dd, dom in enumerate(levels[l]):
+            P = np.stack([dom["lev_x"], dom["lev_y"], dom["lev_z"]], 1)
+            dist, idx = tree.query(P)
+            assert dist.max() < 1e-6
+            maps.append((dd, dom, own[idx]))
+        for tag in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg"):
+            a = np.full(NC, np.nan)
+            for dd, dom, cells in maps:
+                a[cells] = fields[tag][dd][dom["ids"]]
+                sp, npl = dom["lev_poles"]
+                if sp > 0 and dd // nsub == 0:
+                    a[g.layout.south_pole] = fields[tag][dd][sp - 1]
+                if npl > 0 and dd // nsub == 5:
+                    a[g.layout.north_pole] = fields[tag][dd][npl - 1]
+            out[f"{tag}_{l}"] = a
+        if l == ls + 1:
+            a = np.full((4, NC), np.nan)
+            for dd, dom, cells in maps:
+                a[:, cells] = np.stack(dom["ovl"])
+            out[f"ovl_{l}"] = a
+    np.savez_compressed(os.path.join(OUT, f"levels_J{J}.npz"), **out)
+    print(f"levels_J{J}.npz written: FMG iterations {out['fmg_iters']}")
+
+
 def main():
     if not os.path.isdir("/root/reference/src"):
         sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
-    what = sys.argv[1:] or ["grid", "trisk", "elliptic"]
+    what = sys.argv[1:] or ["grid", "trisk", "elliptic", "levels"]
     if "grid" in what:
         for J in sorted(PARAMS):
             grid_fixture(J)
@@ -341,6 +419,8 @@ def main():
     if "elliptic" in what:
         for c in ELLIPTIC_CASES:
             elliptic_fixture(*c)
+    if "levels" in what:
+        levels_fixture()
 
 
 if __name__ == "__main__":

@@ -838,6 +838,32 @@ std::vector<double> overlap_areas(const Level& fine, const Level& coarse) {
           for (int k = 0; k < 4; ++k) a[k * NC + cf] = area[k];
         }
   }
+  // Halo nodes hold their owner's areas, as the reference's node
+  // communication leaves them (get_areas / set_areas, refine_patch.f90:
+  // 436-467): across a rotated lozenge join the midpoint changes type (RT,
+  // DG, UP) and its two edge end points trade places, so a(1) <-> a(2) and
+  // a(3) <-> a(4).  Nodes that no communication reaches keep zero areas, and
+  // so do the nodes (n, -1) and (-1, n) beside the acute lozenge corners
+  // (area_post_comm, refine_patch.f90:469-486).
+  auto kind = [](int i, int j) { return (i & 1) | ((j & 1) << 1); };
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = -HALO; j < nf + HALO; ++j)
+      for (int i = -HALO; i < nf + HALO; ++i) {
+        if (i >= 0 && i < nf && j >= 0 && j < nf) continue;
+        const size_t c = fine.cell(lz, i, j);
+        if (c == fine.south_pole || c == fine.north_pole) continue;
+        const int64_t src = fine.node_src[c];
+        const bool corner = (i == nf && j == -1) || (i == -1 && j == nf);
+        if (src < 0 || corner || (i & 1) + (j & 1) == 0) {
+          for (int k = 0; k < 4; ++k) a[k * NC + c] = 0.0;
+          continue;
+        }
+        const int r = (int)(src % ((int64_t)fine.S * fine.S));
+        const int si = r % fine.S - HALO, sj = r / fine.S - HALO;
+        static const int same[4] = {0, 1, 2, 3}, swapped[4] = {1, 0, 3, 2};
+        const int* perm = kind(i, j) == kind(si, sj) ? same : swapped;
+        for (int k = 0; k < 4; ++k) a[k * NC + c] = a[perm[k] * NC + (size_t)src];
+      }
   return a;
 }
 
