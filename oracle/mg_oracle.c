@@ -15,6 +15,7 @@
  *   forward_scalar_transform (one level)   wavelet.f90:104-150
  *   inverse_scalar_transform (one level)   wavelet.f90:298-346
  *   FMG, v_cycle, Jacobi, prol_fun         lin_solve.f90:33-140, 233-299
+ *   SRJ, SRJ_iter                          lin_solve.f90:142-231
  * Child cells of coarse cell (i, j) on the next finer level are (2i, 2j)
  * (coincident node) and the midpoints (2i+1, 2j) RT, (2i+1, 2j+1) DG,
  * (2i, 2j+1) UP of its edges, in the same lozenge frame.  Build with
@@ -269,3 +270,46 @@ void wto_fmg(const wto_mg* h, double** u, double* const* f, int* iters, double* 
     free(res[l]);
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* SRJ (lin_solve.f90:142-231): BiCGSTAB at the coarsest level, then on each  */
+/* finer level a zero-wavelet prolongation followed by scheduled-relaxation   */
+/* Jacobi sweeps, weights w(n) = 2 / (k1 + k2 - (k2 - k1) cos(pi (2n-1)/2m)). */
+/* ------------------------------------------------------------------------- */
+void wto_srj(const wto_mg* h, int max_srj_iter, double** u, double* const* f, int* iters, double* res_out) {
+  enum { M = 8 };
+  const double k1 = 3e-2, k2 = 1.8, pi = acos(-1.0);
+  const int nl = h->nlevels;
+  double w[M], nrm_f[32];
+  if (max_srj_iter < M) {
+    for (int i = 0; i < M; ++i) w[i] = 1.0;
+  } else {
+    max_srj_iter = M * (max_srj_iter / M);
+    for (int i = 1; i <= M; ++i) w[i - 1] = 2.0 / (k1 + k2 - (k2 - k1) * cos(pi * (2.0 * (double)i - 1.0) / (2.0 * (double)M)));
+  }
+  for (int l = 0; l < nl; ++l) {
+    nrm_f[l] = l2(h, l, f[l]);
+    if (nrm_f[l] == 0.0) nrm_f[l] = 1.0;
+    iters[l] = 0;
+    res_out[l] = 0.0;
+  }
+  double err;
+  iters[0] = wto_bicgstab(h->grid[0], &h->p, h->eta_old[0], h->weight[0], u[0], f[0], nrm_f[0], h->coarse_tol,
+                          h->coarse_iter, &err);
+  res_out[0] = err;
+  for (int l = 1; l < nl; ++l) {
+    prolong0(h, l, u[l - 1], u[l]);
+    int ii = 0;
+    for (int it = 1; it <= max_srj_iter; ++it) {
+      if (++ii > M) {
+        if (res_out[l] < h->fine_tol) break;
+        ii = 1;
+      }
+      wto_jacobi(h->grid[l], &h->p, h->eta_old[l], u[l], f[l], w[ii - 1]);
+      res_out[l] = res_err(h, l, f[l], u[l], nrm_f[l]);
+      iters[l] = it;
+      if (res_out[l] <= h->fine_tol) break;
+    }
+  }
+}
+

@@ -33,6 +33,8 @@ void wto_prolong(const wto_grid* gf, const wto_grid* gc, const double* ovl, cons
  * filled; iters[l] = BiCGSTAB iterations (l = 0) or V-cycles, res[l] the
  * final relative residual per level */
 void wto_fmg(const wto_mg* h, double** u, double* const* f, int* iters, double* res);
+/* SRJ multiscale solve (same arguments; iters[l>0] = Jacobi sweeps) */
+void wto_srj(const wto_mg* h, int max_srj_iter, double** u, double* const* f, int* iters, double* res);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,7 @@ def load():
         _lib.wto_restrict.argtypes = [C.c_void_p] * 6
         _lib.wto_prolong.argtypes = [C.c_void_p] * 6
         _lib.wto_fmg.argtypes = [C.c_void_p] * 5
+        _lib.wto_srj.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
     return _lib
 
 
@@ -253,4 +254,15 @@ class MgOracle:
         iters = np.zeros(nl, dtype=np.int32)
         res = np.zeros(nl)
         self.lib.wto_fmg(C.byref(self.h), U, F, iters.ctypes.data, res.ctypes.data)
+        return u, iters, res
+
+    def srj(self, u, f, max_srj_iter=200):
+        nl = len(self.grids)
+        u = [self.levels[l].node_ghost(u[l]) for l in range(nl)]
+        f = [self.levels[l].node_ghost(f[l]) for l in range(nl)]
+        U = (C.c_void_p * nl)(*[a.ctypes.data for a in u])
+        F = (C.c_void_p * nl)(*[a.ctypes.data for a in f])
+        iters = np.zeros(nl, dtype=np.int32)
+        res = np.zeros(nl)
+        self.lib.wto_srj(C.byref(self.h), int(max_srj_iter), U, F, iters.ctypes.data, res.ctypes.data)
         return u, iters, res

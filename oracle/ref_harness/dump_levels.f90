@@ -15,6 +15,9 @@
 !   ist_*   inverse_scalar_transform (wavelet.f90:298-346) of (fsw, fst)
 !   fmg_*   FMG (lin_solve.f90:33-140) of elliptic_lo u = f from u = 0,
 !           with eta^n and f given at every level
+!   srj_*   SRJ (lin_solve.f90:142-231) of the same problem from u = 0
+! The solvers' logs (residuals, iterations per level) go to stdout, SRJ's
+! after a line 'SRJ'.
 !
 ! Namelist &run (stdin): maxlev, zlev, step, tol_solve, out.
 module lev_ic
@@ -270,6 +273,15 @@ program dump_levels
   call FMG (uu, ff, elliptic_lo, elliptic_lo_diag)
   call update_bdry (uu, NONE, 1)
   call dump_field ('fmg', uu)
+
+  ! SRJ (lin_solve.f90:142-231, the driver's default elliptic solver) from u = 0
+  write (6,'(a)') 'SRJ'
+  do d = 1, size(grid)
+     uu%data(d)%elts = 0d0
+  end do
+  call SRJ (uu, ff, elliptic_lo, elliptic_lo_diag)
+  call update_bdry (uu, NONE, 1)
+  call dump_field ('srj', uu)
   close (u)
 contains
   subroutine set_all

@@ -335,8 +335,8 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
     level, on the owned nodes and the two pole masters, the scalar x, the
     forward transform's restricted scalars (fst) and wavelets (fsw), the
     inverse transform of (fsw, fst) (ist), eta^n, f, topography and the FMG
-    solution; the FMG iteration counts and final residuals per level from the
-    reference's log; overlap areas a(1..4) on the second level.
+    and SRJ solutions; both solvers' iteration counts and final residuals per
+    level from the reference's log; overlap areas a(1..4) on the second level.
 
     The reference keeps one pole copy per lozenge and its transforms let the
     copies drift apart; the fixture takes the south pole from lozenge 0 and
@@ -354,8 +354,10 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
         r = subprocess.run([exe], input=f"&run maxlev={maxlev}, zlev={L}, step={dt}d0, tol_solve={tol}, out='{p}' /\n",
                            text=True, cwd=tmp, check=True, capture_output=True)
         recs = read_records(p)
-    log = [ln.split() for ln in r.stdout.splitlines()]
-    fmg_log = {int(t[0]): (float(t[2]), int(t[3])) for t in log if len(t) == 4 and t[0].isdigit()}
+    fmg_txt, srj_txt = r.stdout.split("\nSRJ\n")
+    parse = lambda txt: {int(t[0]): (float(t[2]), int(t[3])) for t in (ln.split() for ln in txt.splitlines())
+                         if len(t) == 4 and t[0].isdigit()}
+    fmg_log, srj_log = parse(fmg_txt), parse(srj_txt)
     meta = dict((t, a) for t, a in recs if t in ("min_level", "params"))
     _, DL, PS, L_, ND, ls, le = (int(v) for v in meta["min_level"])
     nsub = 4 ** DL
@@ -368,13 +370,15 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
             cur[t] = a
         elif t == "lev_ovl":
             cur["ovl"].append(a)
-        elif t in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg"):
+        elif t in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg", "srj"):
             fields.setdefault(t, []).append(a)
     out = dict(level_start=ls, level_end=le, min_level=ml, domain_level=dl, radius=meta["params"][0],
                grav=meta["params"][2], ref_density=meta["params"][3], dt=meta["params"][4],
                theta1=meta["params"][5], theta2=meta["params"][6], tol=meta["params"][7],
                fmg_iters=np.array([fmg_log[l][1] for l in range(ls, le + 1)]),
-               fmg_res=np.array([fmg_log[l][0] for l in range(ls, le + 1)]))
+               fmg_res=np.array([fmg_log[l][0] for l in range(ls, le + 1)]),
+               srj_iters=np.array([srj_log[l][1] for l in range(ls, le + 1)]),
+               srj_res=np.array([srj_log[l][0] for l in range(ls, le + 1)]))
     for l in range(ls, le + 1):
         g = W.Grid(level=l, min_level=ml, domain_level=dl)
         X = g.array("x")
@@ -387,7 +391,7 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
             dist, idx = tree.query(P)
             assert dist.max() < 1e-6
             maps.append((dd, dom, own[idx]))
-        for tag in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg"):
+        for tag in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg", "srj"):
             a = np.full(NC, np.nan)
             for dd, dom, cells in maps:
                 a[cells] = fields[tag][dd][dom["ids"]]

@@ -3,16 +3,17 @@ reference.
 
 Fixture tests/golden/levels_J4.npz comes from the reference's own routines on
 a uniform hierarchy J4..J6 (oracle/ref_harness/dump_levels.f90):
-forward_scalar_transform, inverse_scalar_transform and FMG on elliptic_lo.
+forward_scalar_transform, inverse_scalar_transform, and FMG and SRJ on
+elliptic_lo.
 
 Bars (fields scaled by their max-norm over the level):
   overlap areas a(1..4)            within 1e-13 of the reference
   restricted scalars               within 1e-14, south pole included
   wavelet coefficients             within 1e-13 of the scalar's scale
   inverse transform                bit-exact away from the poles
-  FMG                              the reference's V-cycle count per level,
-                                   its final residuals (3 printed digits) and
-                                   a solution within 1e-12
+  FMG, SRJ                         the reference's V-cycle / sweep count per
+                                   level, its final residuals (3 printed
+                                   digits) and a solution within 1e-12
 The reference keeps one pole copy per lozenge, fills each from that lozenge's
 halo and lets the copies drift apart (its north-pole lifting reads halo slots
 that no communication fills).  This layout has one pole master per pole:
@@ -39,7 +40,8 @@ def lev():
     ls, le = int(f["level_start"]), int(f["level_end"])
     grids = [W.Grid(level=l, min_level=int(f["min_level"]), domain_level=int(f["domain_level"]))
              for l in range(ls, le + 1)]
-    F = {k: [f[f"{k}_{l}"] for l in range(ls, le + 1)] for k in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg")}
+    F = {k: [f[f"{k}_{l}"] for l in range(ls, le + 1)]
+         for k in ("x", "etan", "f", "topo", "fst", "fsw", "ist", "fmg", "srj")}
     mg = MgOracle(grids, [np.nan_to_num(t) for t in F["topo"]], eta_olds=[np.nan_to_num(e) for e in F["etan"]],
                   grav=float(f["grav"]), rho0=float(f["ref_density"]), dt=float(f["dt"]), theta1=float(f["theta1"]),
                   theta2=float(f["theta2"]), coarse_tol=float(f["tol"]), fine_tol=float(f["tol"]))
@@ -116,6 +118,17 @@ def test_fmg_matches_reference(lev):
     for k, g in enumerate(grids):
         o = g.layout.owned_cells()
         ref = F["fmg"][k]
+        assert np.max(np.abs(u[k][o] - ref[o])) <= 1e-12 * np.max(np.abs(ref[o]))
+
+
+def test_srj_matches_reference(lev):
+    f, grids, F, mg = lev
+    u, iters, res = mg.srj([np.zeros(g.layout.ncells) for g in grids], [np.nan_to_num(x) for x in F["f"]], 200)
+    assert list(iters) == list(f["srj_iters"])
+    assert np.all(np.abs(res / f["srj_res"] - 1) <= 6e-3)
+    for k, g in enumerate(grids):
+        o = g.layout.owned_cells()
+        ref = F["srj"][k]
         assert np.max(np.abs(u[k][o] - ref[o])) <= 1e-12 * np.max(np.abs(ref[o]))
 
 
