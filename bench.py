@@ -54,6 +54,8 @@ def main():
                     help="first run a plain device copy of MIB MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--cpu-layers", type=int, default=30, help="layers of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-free-surface", action="store_true")
+    ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -120,6 +122,8 @@ def main():
                "sample": f"CPU oracle (C, -O2, 1 thread), 1 RK4 step of the same J{J} grid with {Lc} layers "
                          f"({secs:.1f} s)"}
 
+    fs = None if args.no_free_surface else free_surface(args.fs_level)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -151,8 +155,59 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        "free_surface": fs,
     }
     print(json.dumps(out))
+
+
+def free_surface(fine: int):
+    """The free-surface solve to 1e-9 relative residual on a uniform
+    hierarchy J(fine-2)..J(fine) of a seeded ocean draw (dt = 1800 s): plain
+    V-cycles from zero, the same cycles with the learned correction (the
+    shipped network if present, else one trained here on harvested pairs),
+    and the reference's FMG; cycles, final FP64 residual and wall-clock ms
+    (median of 3, host-driven including its residual read-backs)."""
+    import numpy as np
+
+    from wavetrisk_hydrostatic_amd import SOLVE_FMG, SOLVE_LEARNED, SOLVE_VCYCLE, MgParams
+    from wavetrisk_hydrostatic_amd import learned as LC
+
+    print(f"bench: free surface, hierarchy J{fine - 2}..J{fine}", file=sys.stderr, flush=True)
+    grids = LC.hierarchy_grids(fine)
+    prob = LC.Problem(grids, seed=4242)
+    npatch = prob.h.npatches
+    mlp = LC.load_network(npatch, os.path.join(os.path.dirname(LC.WEIGHTS), f"learned_J{fine}.npz"))
+    source = "shipped weights"
+    if mlp is None:
+        print("bench: no shipped network, training one (6 draws, 1500 steps)", file=sys.stderr, flush=True)
+        X, T = [], []
+        for s in range(6):
+            pr = LC.Problem(grids, seed=100 + s)
+            x, t = LC.harvest(pr, ncycles=5)
+            pr.close()
+            X.append(x)
+            T.append(t)
+        mlp, _ = LC.train_network(np.concatenate(X), np.concatenate(T), steps=1500, batch=1024, max_batch=npatch)
+        source = "trained in this run on 6 draws"
+
+    def timed(method, p, learned=False):
+        prob.h.set_learned(mlp if learned else None)
+        runs = [prob.solve(method, p) for _ in range(4)][1:]
+        prob.h.set_learned(None)
+        ms = sorted(r[2] for r in runs)[1]
+        it, res, _ = runs[-1]
+        return {"iterations": [int(v) for v in it], "rel_resid": float(res[-1]), "ms": ms}
+
+    pv = LC.vcycle_params(1e-9, 40)
+    out = {"workload": f"elliptic_lo on J{fine - 2}..J{fine} (finest {10 * 4 ** fine + 2} nodes), dt 1800 s, "
+                       "seeded ocean draw 4242, tolerance 1e-9",
+           "vcycle": timed(SOLVE_VCYCLE, pv),
+           "learned": timed(SOLVE_LEARNED, pv, learned=True),
+           "fmg": timed(SOLVE_FMG, MgParams.defaults(coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)),
+           "network": source}
+    prob.close()
+    mlp.close()
+    return out
 
 
 if __name__ == "__main__":

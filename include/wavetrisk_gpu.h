@@ -185,6 +185,102 @@ int wtg_rhs_elliptic(wtg_ctx* ctx);
  * residual. */
 int wtg_elliptic_solve(wtg_ctx* ctx, int method, double rel_tol, int max_iter, int* iters, double* rel_resid);
 
+/* ---- multi-level free surface: scalar wavelet transforms, FMG, SRJ -------
+ * A hierarchy is a list of free-surface contexts on consecutive uniform
+ * levels, coarse (index 0) to fine, each with its topography, dt / theta and
+ * eta_old (WTG_FS_ETA_OLD); the reference solves on the levels
+ * level_start..level_end of its grid with the same dt and theta on each.
+ * While a hierarchy exists its contexts run on the finest context's stream;
+ * destroy it before its contexts.  Errors of the hierarchy calls are
+ * reported through wtg_last_error(finest context) (NULL for create).
+ * ------------------------------------------------------------------------- */
+typedef struct wtg_mg wtg_mg;
+enum { WTG_SOLVE_FMG = 2, WTG_SOLVE_SRJ = 3, WTG_SOLVE_VCYCLE = 4, WTG_SOLVE_LEARNED = 5 };
+
+/* lin_solve.f90:11-26 */
+typedef struct {
+  double coarse_tol;  /* BiCGSTAB tolerance on the coarsest level */
+  double fine_tol;    /* relative residual that ends the work on a finer level */
+  int coarse_iter;    /* BiCGSTAB iteration limit */
+  int max_vcycle;     /* FMG: V-cycles per level */
+  int pre_iter, down_iter, up_iter, post_iter;  /* FMG: Jacobi sweeps */
+  int max_srj_iter;   /* SRJ: sweeps per level (rounded down to a multiple of 8) */
+} wtg_mg_params;
+
+/* the reference's module defaults (coarse_tol = fine_tol = 1e-3, ...) */
+void wtg_mg_default_params(wtg_mg_params* p);
+/* grids[l] the grid levels[l] was created from; overlap areas between
+ * consecutive levels are computed here (wavelet.f90:1408-1539). */
+int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlevels, wtg_mg** out);
+void wtg_mg_destroy(wtg_mg* mg);
+/* elliptic_solver (init.f90:127) => FMG (lin_solve.f90:33-140) or SRJ
+ * (lin_solve.f90:142-231): solves elliptic_lo(u) = f on every level, u in
+ * each level's WTG_FS_ETA (initial guess in, solution out; a finer level's
+ * guess is replaced by the prolongation of the coarser solution, as in the
+ * reference), f in each level's WTG_FS_RHS.  iters[l]: BiCGSTAB iterations
+ * on level 0, V-cycles (FMG) or Jacobi sweeps (SRJ) above; rel_resid[l] the
+ * final ||f - L u|| / ||f|| per level ([nlevels] each). */
+int wtg_mg_solve(wtg_mg* mg, int method, const wtg_mg_params* p, int* iters, double* rel_resid);
+/* WTG_SOLVE_VCYCLE: plain V-cycles (v_cycle, lin_solve.f90:102-139) on the
+ * finest level from its current u until fine_tol or max_vcycle cycles.
+ * WTG_SOLVE_LEARNED (no counterpart in the reference): the same cycles, each
+ * preceded by the attached network's correction e of the residual r, taken
+ * with the step a = <r, L e> / <L e, L e> when a > 0; the V-cycle is the
+ * fallback and the FP64 residual ends the iteration.  For both, iters and
+ * rel_resid report the finest level (coarser entries 0). */
+/* One level of forward_scalar_transform (wavelet.f90:104-150): on level
+ * `fine`, wc_field := Compute_scalar_wavelets of s_field; on level fine - 1,
+ * coarse_field := Restrict_scalar of (s_field, wc_field).  s_field must be
+ * ghost filled (every field of this API is). */
+int wtg_mg_forward(wtg_mg* mg, int fine, int s_field, int wc_field, int coarse_field);
+/* One level of inverse_scalar_transform (wavelet.f90:298-346): on level
+ * `fine`, fine_field := Prolong_scalar / Reconstruct_scalar of level
+ * fine - 1's coarse_field and level fine's wc_field. */
+int wtg_mg_inverse(wtg_mg* mg, int fine, int coarse_field, int wc_field, int fine_field);
+
+/* The finest level's 8x8 patches of owned nodes (n a multiple of 8): 10
+ * (n/8)^2 of them, lozenge-major, then patch row, then patch column; nodes
+ * row-major within a patch. */
+int wtg_mg_npatches(const wtg_mg* mg);
+/* Attaches (NULL detaches) the network of WTG_SOLVE_LEARNED; it must take a
+ * batch of one row per patch.  The hierarchy does not own it. */
+typedef struct wtg_mlp wtg_mlp;
+int wtg_mg_set_learned(wtg_mg* mg, wtg_mlp* mlp);
+/* The network's input rows for the finest level's field `field`:
+ * feats [npatch][192] = [r / sigma | (|topo| + eta_old) / 4000 m |
+ * sin(latitude)] over the patch's 64 nodes, sigma [npatch] the RMS of r over
+ * the patch; cells [npatch][64] (may be NULL) the cell of each patch node. */
+int wtg_mg_features(wtg_mg* mg, int field, float* feats, double* sigma, int64_t* cells);
+
+/* ---- learned correction: a 3-layer MLP on 8x8 node patches ---------------
+ * No counterpart in the reference (its solvers are FMG and SRJ above).
+ * y = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3 per row, x [WTG_MLP_IN],
+ * y [WTG_MLP_OUT]; weights and activations BF16, products accumulated in
+ * FP32 on the matrix cores, FP32 master parameters.  Parameters flattened as
+ * W1 [IN][HIDDEN], b1, W2 [HIDDEN][HIDDEN], b2, W3 [HIDDEN][OUT], b3
+ * (WTG_MLP_NPARAMS floats).  One HIP stream per MLP; calls synchronise.
+ * ------------------------------------------------------------------------- */
+enum { WTG_MLP_IN = 192, WTG_MLP_HIDDEN = 128, WTG_MLP_OUT = 64, WTG_MLP_NPARAMS = 49472 };
+
+/* He-normal weights from `seed`, zero biases; activations for up to
+ * max_batch rows */
+int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out);
+void wtg_mlp_destroy(wtg_mlp* mlp);
+int wtg_mlp_last_error(const wtg_mlp* mlp, char* buf, size_t n);
+int wtg_mlp_get_params(wtg_mlp* mlp, float* host, size_t n);
+int wtg_mlp_set_params(wtg_mlp* mlp, const float* host, size_t n);  /* also resets the Adam state */
+/* y [batch][OUT] = MLP(x [batch][IN]) (host arrays) */
+int wtg_mlp_forward(wtg_mlp* mlp, const float* x, int batch, float* y);
+/* loss = sum (y - y_true)^2 / (2 batch) and its gradient with respect to the
+ * parameters (WTG_MLP_NPARAMS floats), by the backward kernels */
+int wtg_mlp_gradients(wtg_mlp* mlp, const float* x, const float* y_true, int batch, float* loss, float* grads);
+/* training pairs resident on the device (n rows of x and y_true) */
+int wtg_mlp_set_dataset(wtg_mlp* mlp, const float* x, const float* y_true, int64_t n);
+/* nsteps Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on minibatches drawn
+ * without replacement from shuffled epochs (seed); losses[nsteps] (may be
+ * NULL) the minibatch losses before each step */
+int wtg_mlp_train(wtg_mlp* mlp, int nsteps, int batch, float lr, uint64_t seed, float* losses);
+
 #ifdef __cplusplus
 }
 #endif

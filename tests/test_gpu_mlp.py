@@ -1,0 +1,82 @@
+"""HIP MLP of the learned correction against the CPU oracle (needs a GPU).
+
+Bars: the forward pass within 2e-3 (normwise, relative) of the oracle with
+the same bf16 roundings -- the two differ in FP32 vs FP64 accumulation and in
+an occasional one-ulp bf16 rounding of an activation; the loss within 1e-4
+and every parameter block's gradient within 2e-2 (normwise) of the oracle's;
+training lowers the loss of a learnable map by two orders of magnitude.
+"""
+import numpy as np
+import pytest
+
+import wavetrisk_hydrostatic_amd as W
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def test_forward_matches_oracle():
+    print("mlp forward", flush=True)
+    net = W.MLP(max_batch=1024, seed=7)
+    p = net.params.astype(np.float64)
+    rng = np.random.default_rng(0)
+    for B in (1, 64, 100, 1000):
+        x = rng.standard_normal((B, M.NI)).astype(np.float32)
+        y = net.forward(x)
+        yo, _ = M.forward(p, x, emulate=True)
+        assert y.shape == (B, M.NO)
+        assert rel(y, yo) <= 2e-3, B
+    net.close()
+
+
+def test_gradients_match_oracle():
+    print("mlp gradients", flush=True)
+    net = W.MLP(max_batch=512, seed=8)
+    p = net.params.astype(np.float64)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((300, M.NI)).astype(np.float32)
+    t = (0.5 * rng.standard_normal((300, M.NO))).astype(np.float32)
+    loss, g = net.gradients(x, t)
+    lo, go = M.loss_and_grad(p, x, t, emulate=True)
+    assert abs(loss - lo) <= 1e-4 * lo
+    o = 0
+    for name, shape in M.SHAPES:
+        n = int(np.prod(shape))
+        assert rel(g[o:o + n], go[o:o + n]) <= 2e-2, name
+        o += n
+    net.close()
+
+
+def test_set_params_round_trip_and_errors():
+    net = W.MLP(max_batch=64, seed=9)
+    p = np.random.default_rng(2).standard_normal(M.NPARAMS).astype(np.float32) * 0.05
+    net.params = p
+    assert np.array_equal(net.params, p)
+    with pytest.raises(W.WtgError, match="maximum"):
+        net.forward(np.zeros((65, M.NI), np.float32))
+    with pytest.raises(W.WtgError, match="set_dataset"):
+        net.train(1, 8)
+    net.close()
+
+
+def test_training_learns_a_map():
+    print("mlp training", flush=True)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((M.NI, M.NO)) / np.sqrt(M.NI)
+    x = rng.standard_normal((8192, M.NI)).astype(np.float32)
+    t = (np.tanh(x @ A)).astype(np.float32)
+    net = W.MLP(max_batch=256, seed=10)
+    net.set_dataset(x, t)
+    losses = net.train(600, 256, lr=2e-3, seed=4)
+    print(f"loss {losses[0]:.4f} -> {losses[-20:].mean():.5f}", flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() <= 1e-2 * losses[:5].mean()
+    # the trained parameters evaluate like the oracle
+    y = net.forward(x[:256])
+    yo, _ = M.forward(net.params.astype(np.float64), x[:256], emulate=True)
+    assert rel(y, yo) <= 2e-3
+    net.close()

@@ -22,8 +22,9 @@ import numpy as np
 
 from .layout import Layout
 
-__all__ = ["Grid", "Context", "Layout", "WtgError", "library_path", "load_library", "FS_ETA", "FS_RHS",
-           "FS_ETA_OLD", "FS_DIVF", "FS_DIVF_OLD", "SOLVE_BICGSTAB", "SOLVE_JACOBI"]
+__all__ = ["Grid", "Context", "Hierarchy", "MgParams", "MLP", "Layout", "WtgError", "library_path", "load_library",
+           "FS_ETA", "FS_RHS", "FS_ETA_OLD", "FS_DIVF", "FS_DIVF_OLD", "SOLVE_BICGSTAB", "SOLVE_JACOBI", "SOLVE_FMG",
+           "SOLVE_SRJ", "SOLVE_VCYCLE", "SOLVE_LEARNED"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libwavetrisk_gpu.so"
@@ -84,15 +85,49 @@ SIGNATURES = {
     "wtg_flux_divergence": (C.c_int, [_P, C.c_int]),
     "wtg_rhs_elliptic": (C.c_int, [_P]),
     "wtg_elliptic_solve": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _D]),
+    "wtg_mg_default_params": (None, [_P]),
+    "wtg_mg_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "wtg_mg_destroy": (None, [_P]),
+    "wtg_mg_solve": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "wtg_mg_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "wtg_mg_inverse": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "wtg_mg_npatches": (C.c_int, [_P]),
+    "wtg_mg_set_learned": (C.c_int, [_P, _P]),
+    "wtg_mg_features": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "wtg_mlp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "wtg_mlp_destroy": (None, [_P]),
+    "wtg_mlp_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "wtg_mlp_get_params": (C.c_int, [_P, _P, C.c_size_t]),
+    "wtg_mlp_set_params": (C.c_int, [_P, _P, C.c_size_t]),
+    "wtg_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P]),
+    "wtg_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_float), _P]),
+    "wtg_mlp_set_dataset": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "wtg_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
 }
 
 # free-surface fields and solver methods (include/wavetrisk_gpu.h)
 FS_ETA, FS_RHS, FS_ETA_OLD, FS_DIVF, FS_DIVF_OLD = 0, 1, 2, 3, 4
-SOLVE_BICGSTAB, SOLVE_JACOBI = 0, 1
+SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED = 0, 1, 2, 3, 4, 5
 
 
 class _FsParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta1", C.c_double), ("theta2", C.c_double)]
+
+
+class MgParams(C.Structure):
+    """wtg_mg_params: tolerances and iteration counts of FMG / SRJ
+    (lin_solve.f90:11-26); MgParams.defaults() gives the reference's."""
+    _fields_ = [("coarse_tol", C.c_double), ("fine_tol", C.c_double), ("coarse_iter", C.c_int),
+                ("max_vcycle", C.c_int), ("pre_iter", C.c_int), ("down_iter", C.c_int), ("up_iter", C.c_int),
+                ("post_iter", C.c_int), ("max_srj_iter", C.c_int)]
+
+    @classmethod
+    def defaults(cls, **overrides):
+        p = cls()
+        load_library().wtg_mg_default_params(C.byref(p))
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        return p
 
 
 def load_library():
@@ -280,3 +315,131 @@ class Context:
         _check(_lib.wtg_elliptic_solve(self._h, method, rel_tol, max_iter, C.byref(it), C.byref(res)), self._h,
                "wtg_elliptic_solve")
         return it.value, res.value
+
+
+class Hierarchy:
+    """Free-surface contexts on consecutive uniform levels, coarse to fine
+    (wtg_mg_*): one level of the scalar wavelet transform each way, and the
+    FMG / SRJ elliptic solvers.  While it exists the contexts run on the
+    finest one's stream; close it before the contexts."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        n = len(self.contexts)
+        grids = (_P * n)(*[c.grid.handle.value for c in self.contexts])
+        ctxs = (_P * n)(*[c._h.value for c in self.contexts])
+        h = _P()
+        _check(load_library().wtg_mg_create(grids, ctxs, n, C.byref(h)), None, "wtg_mg_create")
+        self._h = h
+
+    @property
+    def _top(self):
+        return self.contexts[-1]._h
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.wtg_mg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve(self, method: int = SOLVE_FMG, params: MgParams | None = None):
+        """u on every level's FS_ETA; returns (iterations, relative residuals) per level"""
+        p = params if params is not None else MgParams.defaults()
+        n = len(self.contexts)
+        it = np.zeros(n, dtype=np.int32)
+        res = np.zeros(n)
+        _check(_lib.wtg_mg_solve(self._h, method, C.byref(p), _ptr(it), _ptr(res)), self._top, "wtg_mg_solve")
+        return it, res
+
+    @property
+    def npatches(self) -> int:
+        return _lib.wtg_mg_npatches(self._h)
+
+    def set_learned(self, mlp: "MLP | None"):
+        self._mlp = mlp
+        _check(_lib.wtg_mg_set_learned(self._h, mlp._h if mlp is not None else None), self._top, "wtg_mg_set_learned")
+
+    def features(self, field: int):
+        """network input rows [npatch, 192], patch RMS sigma [npatch] and patch
+        node cells [npatch, 64] of the finest level's field"""
+        n = self.npatches
+        f = np.empty((n, MLP_IN), dtype=np.float32)
+        sg = np.empty(n)
+        cells = np.empty((n, 64), dtype=np.int64)
+        _check(_lib.wtg_mg_features(self._h, field, _ptr(f), _ptr(sg), _ptr(cells)), self._top, "wtg_mg_features")
+        return f, sg, cells
+
+    def forward(self, fine: int, s_field: int, wc_field: int, coarse_field: int):
+        _check(_lib.wtg_mg_forward(self._h, fine, s_field, wc_field, coarse_field), self._top, "wtg_mg_forward")
+
+    def inverse(self, fine: int, coarse_field: int, wc_field: int, fine_field: int):
+        _check(_lib.wtg_mg_inverse(self._h, fine, coarse_field, wc_field, fine_field), self._top, "wtg_mg_inverse")
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_NPARAMS = 192, 128, 64, 49472
+
+
+class MLP:
+    """The learned correction's network on the current HIP device
+    (wtg_mlp_*): BF16 weights and activations, FP32 accumulation on the
+    matrix cores, Adam training on device-resident pairs."""
+
+    def __init__(self, max_batch: int = 4096, seed: int = 0):
+        h = _P()
+        _check(load_library().wtg_mlp_create(max_batch, seed, C.byref(h)), None, "wtg_mlp_create")
+        self._h = h
+        self.max_batch = max_batch
+
+    def _check(self, rc, what):
+        if rc != 0:
+            buf = C.create_string_buffer(1024)
+            _lib.wtg_mlp_last_error(self._h, buf, len(buf))
+            raise WtgError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.wtg_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_NPARAMS, dtype=np.float32)
+        self._check(_lib.wtg_mlp_get_params(self._h, _ptr(out), out.size), "wtg_mlp_get_params")
+        return out
+
+    @params.setter
+    def params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(_lib.wtg_mlp_set_params(self._h, _ptr(a), a.size), "wtg_mlp_set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], MLP_OUT), dtype=np.float32)
+        self._check(_lib.wtg_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)), "wtg_mlp_forward")
+        return y
+
+    def gradients(self, x, y_true):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(y_true, dtype=np.float32)
+        g = np.empty(MLP_NPARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(_lib.wtg_mlp_gradients(self._h, _ptr(x), _ptr(t), x.shape[0], C.byref(loss), _ptr(g)),
+                    "wtg_mlp_gradients")
+        return loss.value, g
+
+    def set_dataset(self, x, y_true):
+        self._x = np.ascontiguousarray(x, dtype=np.float32)
+        self._t = np.ascontiguousarray(y_true, dtype=np.float32)
+        self._check(_lib.wtg_mlp_set_dataset(self._h, _ptr(self._x), _ptr(self._t), self._x.shape[0]),
+                    "wtg_mlp_set_dataset")
+
+    def train(self, nsteps: int, batch: int, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.empty(nsteps, dtype=np.float32)
+        self._check(_lib.wtg_mlp_train(self._h, nsteps, batch, lr, seed, _ptr(losses)), "wtg_mlp_train")
+        return losses
+

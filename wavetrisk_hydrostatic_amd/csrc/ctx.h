@@ -22,13 +22,21 @@ enum FsField {
   FS_DIVF,         // flux divergence of the layer-integrated velocity (trend(S_MASS, zlevels+1))
   FS_DIVF_OLD,     // the same at the previous step (trend(S_TEMP, zlevels+1))
   FS_NUSER,        // fields above are reachable through the C ABI
-  FS_W0 = FS_NUSER,  // solver work fields
-  FS_NFIELDS = FS_W0 + 9
+  FS_W0 = FS_NUSER,  // solver work fields: W0..W0+7 BiCGSTAB, then the multigrid's
+  MG_U2 = FS_W0 + 8,  // second buffer of the solution (Jacobi ping-pong)
+  MG_RES,             // residual, two buffers
+  MG_RES2,
+  MG_CORR,            // V-cycle correction, two buffers
+  MG_CORR2,
+  MG_WC,              // scalar wavelet coefficients
+  MG_PF,              // prolongation of the coarser correction
+  FS_NFIELDS
 };
 
 struct wtg_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream of every launch (a hierarchy points it at its finest level's)
+  hipStream_t own_stream = nullptr;  // the stream this context created
   wtg_grid_info info{};
   wtg_params params{};
   int nslots = 0;

@@ -16,9 +16,11 @@
 // run); they differ from a sequential sum in the last bits.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <utility>
 
 #include "ctx.h"
+#include "mlp.h"
 
 namespace {
 
@@ -238,6 +240,200 @@ __global__ void fs_scalar_kernel(int op, double* __restrict__ s) {
   }
 }
 
+// ===========================================================================
+// Multi-level scalar wavelet transform (one level) and the multigrid solvers
+// ===========================================================================
+// Child cells of coarse cell (i, j) are, on the next finer level and in the
+// same lozenge frame, the coincident node (2i, 2j) and the edge midpoints
+// (2i+1, 2j) RT, (2i+1, 2j+1) DG, (2i, 2j+1) UP.  Overlap areas a(1..4) live
+// at the fine midpoints (four planes); halo copies carry their owner's areas.
+
+struct MgGeo {
+  const double* ovl;     // [4][NCf]
+  const double* hinv_f;  // fine 1/area
+  const double* hinv_c;  // coarse 1/area
+  int64_t NCf;
+  int Sf, nf, Sc;
+  int64_t sp_f, np_f, sp_c, np_c;  // pole masters
+};
+
+__device__ __forceinline__ void mg_ij(int64_t c, int S, int& lz, int& i, int& j) {
+  const int64_t T = (int64_t)S * S;
+  lz = (int)(c / T);
+  const int r = (int)(c - (int64_t)lz * T);
+  i = r % S - 2;
+  j = r / S - 2;
+}
+
+// Interp_node (wavelet.f90:1487-1498) at fine midpoint x from its four
+// coincident neighbours: the edge's end points and the two opposite nodes
+__device__ __forceinline__ double mg_interp(const MgGeo& g, const double* __restrict__ s, int64_t x, int i, int j) {
+  const int64_t S = g.Sf, NC = g.NCf;
+  int64_t id1, id2, id3, id4;
+  if ((i & 1) && !(j & 1)) {  // RT
+    id1 = x - 1; id2 = x + 1; id3 = x + 1 + 2 * S; id4 = x - 1 - 2 * S;
+  } else if ((i & 1) && (j & 1)) {  // DG
+    id1 = x + 1 + S; id2 = x - 1 - S; id3 = x + 1 - S; id4 = x - 1 + S;
+  } else {  // UP
+    id1 = x - S; id2 = x + S; id3 = x - 2 - S; id4 = x + 2 + S;
+  }
+  return (g.ovl[x] * s[id1] + g.ovl[NC + x] * s[id2] + g.ovl[2 * NC + x] * s[id3] + g.ovl[3 * NC + x] * s[id4]) *
+         g.hinv_f[x];
+}
+
+// lifting around fine coincident cell c (Restrict_scalar wavelet.f90:542-567,
+// twelve terms; Prolong_scalar 586-611 omits id2NE)
+__device__ __forceinline__ double mg_lift(const MgGeo& g, const double* __restrict__ wc, int64_t c, bool with_2ne) {
+  const int64_t S = g.Sf, NC = g.NCf;
+  const double *a1 = g.ovl, *a2 = g.ovl + NC, *a3 = g.ovl + 2 * NC, *a4 = g.ovl + 3 * NC;
+  const int64_t E = c + 1, NE = c + 1 + S, N2E = c + 2 + S, N2NE = c + 1 + 2 * S, N = c + S, W = c - 1,
+                NW = c - 1 + S, S2W = c - 2 - S, SW = c - 1 - S, So = c - S, SW2 = c - 1 - 2 * S, SE = c + 1 - S;
+  if (with_2ne)
+    return wc[E] * a1[E] + wc[NE] * a2[NE] + wc[N2E] * a3[N2E] + wc[N2NE] * a4[N2NE] + wc[N] * a1[N] +
+           wc[W] * a2[W] + wc[NW] * a3[NW] + wc[S2W] * a4[S2W] + wc[SW] * a1[SW] + wc[So] * a2[So] +
+           wc[SW2] * a3[SW2] + wc[SE] * a4[SE];
+  return wc[E] * a1[E] + wc[NE] * a2[NE] + wc[N2E] * a3[N2E] + wc[N] * a1[N] + wc[W] * a2[W] + wc[NW] * a3[NW] +
+         wc[S2W] * a4[S2W] + wc[SW] * a1[SW] + wc[So] * a2[So] + wc[SW2] * a3[SW2] + wc[SE] * a4[SE];
+}
+
+// lifting at a pole: its five edge midpoints (area a2, the pole being their
+// second end point) and the five DG midpoints of the second ring (a3 south,
+// a4 north); the prolongation drops the second-ring term of lozenge 1 (south)
+// / 7 (north), the pole's id2NE
+__device__ __forceinline__ double mg_pole_lift(const MgGeo& g, const double* __restrict__ wc, bool south,
+                                               bool prolong) {
+  const int64_t S = g.Sf, NC = g.NCf, T = S * S;
+  const int n = g.nf;
+  double sum = 0.0;
+  if (south) {
+    for (int k = 0; k < 5; ++k) {
+      const int64_t m = k * T + 2 * S + (n - 1 + 2), d = k * T + 3 * S + (n - 1 + 2);
+      sum += wc[m] * g.ovl[NC + m];
+      if (!(prolong && k == 1)) sum += wc[d] * g.ovl[2 * NC + d];
+    }
+  } else {
+    for (int k = 5; k < 10; ++k) {
+      const int64_t m = k * T + (int64_t)(n - 1 + 2) * S + 2, d = k * T + (int64_t)(n - 1 + 2) * S + 3;
+      sum += wc[m] * g.ovl[NC + m];
+      if (!(prolong && k == 7)) sum += wc[d] * g.ovl[3 * NC + d];
+    }
+  }
+  return sum;
+}
+
+// Compute_scalar_wavelets (wavelet.f90:648-670) over the fine compute list:
+// wc = s - Interp_node at midpoints, 0 at coincident nodes
+__global__ void mg_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ s,
+                                  double* __restrict__ wc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t x = ncl[2 * t + 1];
+  int lz, i, j;
+  mg_ij(x, g.Sf, lz, i, j);
+  wc[ncl[2 * t]] = ((i | j) & 1) ? s[x] - mg_interp(g, s, x, i, j) : 0.0;
+}
+
+// Restrict_scalar over the coarse compute list: s_c = s_f(child) + lift * hex_inv_c
+__global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ sf,
+                                   const double* __restrict__ wc, double* __restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t p = ncl[2 * t + 1];
+  int lz, i, j;
+  mg_ij(p, g.Sc, lz, i, j);
+  const int64_t c = (int64_t)lz * g.Sf * g.Sf + (int64_t)(2 * j + 2) * g.Sf + (2 * i + 2);
+  const bool pole = p == g.sp_c || p == g.np_c;
+  const double lift = pole ? mg_pole_lift(g, wc, p == g.sp_c, false) : mg_lift(g, wc, c, true);
+  sc[ncl[2 * t]] = sf[c] + lift * g.hinv_c[p];
+}
+
+// Prolong_scalar over the fine compute list, coincident nodes:
+// s_f = s_c(parent) - lift * hex_inv_c
+__global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ sc,
+                                       const double* __restrict__ wc, double* __restrict__ sf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t x = ncl[2 * t + 1];
+  int lz, i, j;
+  mg_ij(x, g.Sf, lz, i, j);
+  if ((i | j) & 1) return;
+  const int64_t p = (int64_t)lz * g.Sc * g.Sc + (int64_t)(j / 2 + 2) * g.Sc + (i / 2 + 2);
+  const bool pole = x == g.sp_f || x == g.np_f;
+  const double lift = pole ? mg_pole_lift(g, wc, x == g.sp_f, true) : mg_lift(g, wc, x, false);
+  sf[ncl[2 * t]] = sc[p] - lift * g.hinv_c[p];
+}
+
+// Reconstruct_scalar (wavelet.f90:613-646) over the fine compute list,
+// midpoints: s_f = Interp_node(coincident s_f) + wc.  Reads only coincident
+// cells and writes only midpoints, so it runs in place after the even pass.
+__global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ wc,
+                                      double* __restrict__ sf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t x = ncl[2 * t + 1];
+  int lz, i, j;
+  mg_ij(x, g.Sf, lz, i, j);
+  if (!((i | j) & 1)) return;
+  sf[ncl[2 * t]] = mg_interp(g, sf, x, i, j) + wc[x];
+}
+
+// out := v on every cell of the array (work fields; not a memset, so the
+// sequence stays capturable in a graph)
+__global__ void mg_fill_kernel(int64_t n, double v, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = v;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned correction: features of 8x8 node patches and the scattered output
+// ---------------------------------------------------------------------------
+constexpr double LC_DEPTH_SCALE = 4000.0;  // m
+
+// one wave per patch, one lane per node: row p of the features is
+// [r / sigma | (|topo| + eta_old) / 4000 m | sin(latitude)], sigma the RMS of
+// r over the patch (0 for a zero patch, then the first block is 0)
+template <bool BF16>
+__global__ void lc_features_kernel(int npatch, const int64_t* __restrict__ cells, const double* __restrict__ r,
+                                   const double* __restrict__ topo, const double* __restrict__ eo,
+                                   const double* __restrict__ fnode, double* __restrict__ sigma,
+                                   float* __restrict__ outf, uint16_t* __restrict__ outb) {
+  const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (p >= npatch) return;
+  const int64_t c = cells[(int64_t)p * 64 + lane];
+  const double v = r[c];
+  double ss = v * v;
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+  const double sg = sqrt(ss / 64.0);
+  if (lane == 0) sigma[p] = sg;
+  const float f0 = sg > 0.0 ? (float)(v / sg) : 0.f;
+  const float f1 = (float)((fabs(topo[c]) + eo[c]) / LC_DEPTH_SCALE), f2 = (float)fnode[c];
+  const int64_t row = (int64_t)p * 192;
+  if (BF16) {
+    auto bf = [](float f) {
+      uint32_t u = __float_as_uint(f);
+      u += 0x7fffu + ((u >> 16) & 1u);
+      return (uint16_t)(u >> 16);
+    };
+    outb[row + lane] = bf(f0);
+    outb[row + 64 + lane] = bf(f1);
+    outb[row + 128 + lane] = bf(f2);
+  } else {
+    outf[row + lane] = f0;
+    outf[row + 64 + lane] = f1;
+    outf[row + 128 + lane] = f2;
+  }
+}
+
+// e(patch node) = sigma * y (other cells keep their value)
+__global__ void lc_scatter_kernel(int npatch, const int64_t* __restrict__ cells, const float* __restrict__ y,
+                                  const double* __restrict__ sigma, double* __restrict__ e) {
+  const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (p >= npatch) return;
+  e[cells[(int64_t)p * 64 + lane]] = (double)y[(int64_t)p * 64 + lane] * sigma[p];
+}
+
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -413,6 +609,307 @@ int jacobi_solve(wtg_ctx* c, double* u, const double* f, double tol, int maxit, 
 
 }  // namespace
 
+// hierarchy of free-surface contexts, coarse (0) to fine (nl - 1)
+struct wtg_mg {
+  int nl = 0;
+  std::vector<wtg_ctx*> lv;
+  std::vector<double*> ovl;  // ovl[l], l >= 1: overlap areas between levels l and l - 1
+  std::vector<wtg_grid_info> info;
+  // learned correction on the finest level: 8x8 patches of owned nodes
+  wtg_mlp* mlp = nullptr;
+  int npatch = 0;
+  int64_t* d_cells = nullptr;  // [npatch][64] cell of each patch node
+  double* d_fnode = nullptr;   // sin(latitude) per cell
+  double* d_sigma = nullptr;   // [npatch]
+  float* d_feat = nullptr;     // [npatch][192] (feature extraction)
+};
+
+namespace {
+
+constexpr int MG_MAX_LEVELS = 16;
+
+// errors of a level are reported through the finest level's context
+int mg_err(wtg_mg* m, int rc, int l) {
+  if (rc && l != m->nl - 1) m->lv[m->nl - 1]->err = m->lv[l]->err;
+  return rc;
+}
+
+MgGeo mg_geo(const wtg_mg* m, int l) {
+  const wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
+  return MgGeo{m->ovl[l], f->d_hinv, c->d_hinv, f->info.ncells, f->info.S, f->info.n, c->info.S,
+               f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole};
+}
+
+int mg_fill(wtg_ctx* c, double* a, double v) {
+  const int64_t n = c->info.ncells;
+  hipLaunchKernelGGL(mg_fill_kernel, dim3((unsigned)((n + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream, n, v, a);
+  WTG_HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// wc := wavelets of s on level l, s_c := restriction of (s, wc) to level l - 1
+int mg_forward(wtg_mg* m, int l, const double* s, double* wc, double* s_c) {
+  wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
+  const MgGeo g = mg_geo(m, l);
+  hipLaunchKernelGGL(mg_wavelet_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s, wc);
+  WTG_HIPCHK(f, hipGetLastError());
+  hipLaunchKernelGGL(mg_restrict_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, f->stream, c->nncl, c->d_ncl, g, s, wc,
+                     s_c);
+  WTG_HIPCHK(f, hipGetLastError());
+  return 0;
+}
+
+// s_f := inverse transform of (s_c, wc) from level l - 1 to level l
+int mg_inverse(wtg_mg* m, int l, const double* s_c, const double* wc, double* s_f) {
+  wtg_ctx* f = m->lv[l];
+  const MgGeo g = mg_geo(m, l);
+  hipLaunchKernelGGL(mg_prolong_even_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
+                     wc, s_f);
+  WTG_HIPCHK(f, hipGetLastError());
+  hipLaunchKernelGGL(mg_prolong_odd_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
+                     s_f);
+  WTG_HIPCHK(f, hipGetLastError());
+  return 0;
+}
+
+// a level's fields during a solve; every operation writes a fresh buffer
+// (kernels over the compute list must not write what their halo threads
+// read), so the current buffers are tracked as pointers
+struct MgLv {
+  wtg_ctx* c;
+  double *u, *u_alt, *f, *res, *res_alt, *corr, *corr_alt;
+};
+
+int lv_jacobi(MgLv& v, double*& u, double*& alt, const double* f, double w, int n) {
+  for (int i = 0; i < n; ++i) {
+    RC(launch_jacobi(v.c, u, f, w, alt));
+    std::swap(u, alt);
+  }
+  return 0;
+}
+
+// ||f - L u|| / nrm_f, read back
+int lv_res_err(MgLv& v, const double* f, const double* u, double nrm_f, double* out) {
+  double* r = v.c->fs[FS_W0];
+  RC(launch_residual(v.c, u, f, r));
+  RC(launch_dots(v.c, 1, r, r, SC_T0));
+  double d;
+  RC(read_scalar(v.c, SC_T0, &d));
+  *out = std::sqrt(d) / nrm_f;
+  return 0;
+}
+
+int lv_l2(MgLv& v, const double* a, double* out) {
+  RC(launch_dots(v.c, 1, a, a, SC_T0));
+  double d;
+  RC(read_scalar(v.c, SC_T0, &d));
+  *out = std::sqrt(d);
+  return 0;
+}
+
+// zero-wavelet prolongation (prolong / prol_fun, lin_solve.f90:244-269) of
+// level l - 1's coarse field into level l
+int lv_prolong0(wtg_mg* m, std::vector<MgLv>& V, int l, const double* coarse, double* fine) {
+  double* wc = V[l].c->fs[MG_WC];
+  RC(mg_fill(V[l].c, wc, 0.0));
+  return mg_inverse(m, l, coarse, wc, fine);
+}
+
+std::vector<MgLv> mg_levels(wtg_mg* m) {
+  std::vector<MgLv> V(m->nl);
+  for (int l = 0; l < m->nl; ++l) {
+    wtg_ctx* c = m->lv[l];
+    V[l] = MgLv{c, c->fs[FS_ETA], c->fs[MG_U2], c->fs[FS_RHS], c->fs[MG_RES], c->fs[MG_RES2], c->fs[MG_CORR],
+                c->fs[MG_CORR2]};
+  }
+  return V;
+}
+
+// results back into each level's WTG_FS_ETA
+int mg_finish(std::vector<MgLv>& V) {
+  for (auto& v : V)
+    if (v.u != v.c->fs[FS_ETA]) RC(launch_copy(v.c, v.u, v.c->fs[FS_ETA]));
+  WTG_HIPCHK(V.back().c, hipStreamSynchronize(V.back().c->stream));
+  return 0;
+}
+
+// v_cycle (lin_solve.f90:102-139) on level l: u := u + V-cycle correction,
+// Jacobi post-smoothing, *err = ||f - L u|| / nrm_f
+int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
+  for (int j = 0; j <= l; ++j) RC(mg_fill(V[j].c, V[j].corr, 0.0));
+  RC(launch_residual(V[l].c, V[l].u, V[l].f, V[l].res));
+  for (int j = l; j >= 1; --j) {
+    RC(lv_jacobi(V[j], V[j].corr, V[j].corr_alt, V[j].res, 1.0, P.down_iter));
+    RC(launch_residual(V[j].c, V[j].corr, V[j].res, V[j].res_alt));  // res := res - L corr
+    std::swap(V[j].res, V[j].res_alt);
+    RC(mg_forward(m, j, V[j].res, V[j].c->fs[MG_WC], V[j - 1].res));
+  }
+  int cit;
+  double cerr;
+  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr));
+  for (int j = 1; j <= l; ++j) {
+    double* pf = V[j].c->fs[MG_PF];
+    RC(lv_prolong0(m, V, j, V[j - 1].corr, pf));
+    RC(launch_lc(V[j].c, 1.0, -1, V[j].corr, 1.0, -1, pf, V[j].corr_alt));
+    std::swap(V[j].corr, V[j].corr_alt);
+    RC(lv_jacobi(V[j], V[j].corr, V[j].corr_alt, V[j].res, 1.0, P.up_iter));
+  }
+  RC(launch_lc(V[l].c, 1.0, -1, V[l].u, 1.0, -1, V[l].corr, V[l].u_alt));
+  std::swap(V[l].u, V[l].u_alt);
+  RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, 1.0, P.post_iter));
+  return lv_res_err(V[l], V[l].f, V[l].u, nrm_f, err);
+}
+
+// FMG with V-cycles (lin_solve.f90:33-140)
+int mg_fmg(wtg_mg* m, const wtg_mg_params& P, int* iters, double* res_out) {
+  std::vector<MgLv> V = mg_levels(m);
+  const int nl = m->nl;
+  std::vector<double> nrm_f(nl), nrm1(nl);
+  for (int l = 0; l < nl; ++l) {
+    RC(lv_l2(V[l], V[l].f, &nrm_f[l]));
+    if (nrm_f[l] == 0.0) nrm_f[l] = 1.0;
+    RC(lv_res_err(V[l], V[l].f, V[l].u, nrm_f[l], &nrm1[l]));
+    iters[l] = 0;
+    res_out[l] = 0.0;
+  }
+  RC(bicgstab(V[0].c, V[0].u, V[0].f, P.coarse_tol, P.coarse_iter, &iters[0], &res_out[0]));
+  for (int l = 1; l < nl; ++l) {
+    if (nrm1[l] < P.fine_tol) continue;
+    RC(lv_prolong0(m, V, l, V[l - 1].u, V[l].u));
+    RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, 1.0, P.pre_iter));
+    for (int it = 1; it <= P.max_vcycle; ++it) {
+      RC(vcycle(m, V, l, P, nrm_f[l], &res_out[l]));
+      iters[l] = it;
+      if (res_out[l] <= P.fine_tol) break;
+    }
+  }
+  return mg_finish(V);
+}
+
+// SRJ (lin_solve.f90:142-231)
+int mg_srj(wtg_mg* m, const wtg_mg_params& P, int* iters, double* res_out) {
+  constexpr int M = 8;
+  const double k1 = 3e-2, k2 = 1.8, pi = std::acos(-1.0);
+  std::vector<MgLv> V = mg_levels(m);
+  const int nl = m->nl;
+  double w[M];
+  int max_it = P.max_srj_iter;
+  if (max_it < M) {
+    for (double& x : w) x = 1.0;
+  } else {
+    max_it = M * (max_it / M);
+    for (int i = 1; i <= M; ++i)
+      w[i - 1] = 2.0 / (k1 + k2 - (k2 - k1) * std::cos(pi * (2.0 * (double)i - 1.0) / (2.0 * (double)M)));
+  }
+  std::vector<double> nrm_f(nl);
+  for (int l = 0; l < nl; ++l) {
+    RC(lv_l2(V[l], V[l].f, &nrm_f[l]));
+    if (nrm_f[l] == 0.0) nrm_f[l] = 1.0;
+    iters[l] = 0;
+    res_out[l] = 0.0;
+  }
+  RC(bicgstab(V[0].c, V[0].u, V[0].f, P.coarse_tol, P.coarse_iter, &iters[0], &res_out[0]));
+  for (int l = 1; l < nl; ++l) {
+    RC(lv_prolong0(m, V, l, V[l - 1].u, V[l].u));
+    int ii = 0;
+    for (int it = 1; it <= max_it; ++it) {
+      if (++ii > M) {
+        if (res_out[l] < P.fine_tol) break;
+        ii = 1;
+      }
+      RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, w[ii - 1], 1));
+      RC(lv_res_err(V[l], V[l].f, V[l].u, nrm_f[l], &res_out[l]));
+      iters[l] = it;
+      if (res_out[l] <= P.fine_tol) break;
+    }
+  }
+  return mg_finish(V);
+}
+
+// features of field r on the finest level (BF16 into the MLP's input rows,
+// or FP32 into d_feat)
+int lc_features(wtg_mg* m, const double* r, bool to_mlp) {
+  wtg_ctx* c = m->lv[m->nl - 1];
+  uint16_t* x0 = nullptr;
+  float* y = nullptr;
+  int maxb = 0;
+  if (to_mlp) wtg_mlp_buffers(m->mlp, &x0, &y, &maxb);
+  const unsigned nb = (unsigned)((m->npatch + 3) / 4);
+  if (to_mlp)
+    hipLaunchKernelGGL(lc_features_kernel<true>, dim3(nb), dim3(256), 0, c->stream, m->npatch, m->d_cells, r, c->d_topo,
+                       c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, (float*)nullptr, x0);
+  else
+    hipLaunchKernelGGL(lc_features_kernel<false>, dim3(nb), dim3(256), 0, c->stream, m->npatch, m->d_cells, r,
+                       c->d_topo, c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, m->d_feat, (uint16_t*)nullptr);
+  WTG_HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// e := ghost-filled learned correction of residual r on the finest level
+int lc_correction(wtg_mg* m, const double* r, double* tmp, double* e) {
+  wtg_ctx* c = m->lv[m->nl - 1];
+  RC(lc_features(m, r, true));
+  if (int rc = wtg_mlp_forward_on(m->mlp, c->stream, m->npatch))
+    return wtg_fail(c, rc, "learned correction: %s", wtg_mlp_error_text(m->mlp));
+  uint16_t* x0;
+  float* y;
+  int maxb;
+  wtg_mlp_buffers(m->mlp, &x0, &y, &maxb);
+  RC(mg_fill(c, tmp, 0.0));
+  hipLaunchKernelGGL(lc_scatter_kernel, dim3((unsigned)((m->npatch + 3) / 4)), dim3(256), 0, c->stream, m->npatch,
+                     m->d_cells, y, m->d_sigma, tmp);
+  WTG_HIPCHK(c, hipGetLastError());
+  return launch_copy(c, tmp, e);  // halo copies
+}
+
+// Cycles on the finest level from its current u: a plain V-cycle each, or
+// (learned) first the network's correction e of the residual r, taken with
+// the step a = <r, L e> / <L e, L e> that minimises ||r - a L e|| when a > 0
+// (skipped otherwise, the V-cycle alone remains), then the V-cycle.
+// iters[top] = cycles, res_out[top] the final relative residual; coarser
+// levels serve the V-cycles and report 0.
+int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, double* res_out) {
+  std::vector<MgLv> V = mg_levels(m);
+  const int l = m->nl - 1;
+  for (int j = 0; j < m->nl; ++j) iters[j] = 0, res_out[j] = 0.0;
+  double nrm_f;
+  RC(lv_l2(V[l], V[l].f, &nrm_f));
+  if (nrm_f == 0.0) nrm_f = 1.0;
+  wtg_ctx* c = V[l].c;
+  double err;
+  RC(lv_res_err(V[l], V[l].f, V[l].u, nrm_f, &err));
+  res_out[l] = err;
+  for (int it = 1; it <= P.max_vcycle && err > P.fine_tol; ++it) {
+    if (learned) {
+      double *r = c->fs[FS_W0 + 1], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3], *tmp = c->fs[FS_W0 + 4];
+      RC(launch_residual(c, V[l].u, V[l].f, r));
+      RC(lc_correction(m, r, tmp, e));
+      RC(launch_apply(c, e, w));
+      RC(launch_dots(c, 2, r, w, SC_T0, w, w, SC_T1));
+      double rw, ww;
+      RC(read_scalar(c, SC_T0, &rw));
+      RC(read_scalar(c, SC_T1, &ww));
+      const double a = ww > 0.0 ? rw / ww : 0.0;
+      if (a > 0.0 && std::isfinite(a)) {
+        RC(launch_lc(c, 1.0, -1, V[l].u, a, -1, e, V[l].u_alt));
+        std::swap(V[l].u, V[l].u_alt);
+      }
+    }
+    RC(vcycle(m, V, l, P, nrm_f, &err));
+    iters[l] = it;
+    res_out[l] = err;
+  }
+  return mg_finish(V);
+}
+
+int mg_check_level(wtg_mg* m, int fine) {
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (fine < 1 || fine >= m->nl) return wtg_fail(top, WTG_E_ARG, "fine level %d out of range [1,%d)", fine, m->nl);
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int wtg_fs_set_params(wtg_ctx* c, const wtg_fs_params* p) {
@@ -531,6 +1028,191 @@ int wtg_elliptic_solve(wtg_ctx* c, int method, double rel_tol, int max_iter, int
   if (method == WTG_SOLVE_BICGSTAB) return bicgstab(c, c->fs[FS_ETA], c->fs[FS_RHS], rel_tol, max_iter, iters, rel_resid);
   if (method == WTG_SOLVE_JACOBI) return jacobi_solve(c, c->fs[FS_ETA], c->fs[FS_RHS], rel_tol, max_iter, iters, rel_resid);
   return wtg_fail(c, WTG_E_ARG, "unknown solver method %d", method);
+}
+
+
+void wtg_mg_default_params(wtg_mg_params* p) {
+  if (!p) return;
+  p->coarse_tol = 1e-3;
+  p->fine_tol = 1e-3;
+  p->coarse_iter = 50;
+  p->max_vcycle = 3;
+  p->pre_iter = 2;
+  p->down_iter = 2;
+  p->up_iter = 2;
+  p->post_iter = 4;
+  p->max_srj_iter = 200;
+}
+
+int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlevels, wtg_mg** out) {
+  if (!grids || !levels || !out) return wtg_fail(nullptr, WTG_E_ARG, "null argument");
+  *out = nullptr;
+  if (nlevels < 1 || nlevels > MG_MAX_LEVELS)
+    return wtg_fail(nullptr, WTG_E_ARG, "nlevels %d out of range [1,%d]", nlevels, MG_MAX_LEVELS);
+  for (int l = 0; l < nlevels; ++l) {
+    if (!grids[l] || !levels[l]) return wtg_fail(nullptr, WTG_E_ARG, "level %d: null grid or context", l);
+    wtg_ctx* c = levels[l];
+    if (c->info.n != grids[l]->lv.n) return wtg_fail(nullptr, WTG_E_ARG, "level %d: context and grid differ", l);
+    if (l && grids[l]->lv.level != grids[l - 1]->lv.level + 1)
+      return wtg_fail(nullptr, WTG_E_ARG, "levels %d and %d are not consecutive", l - 1, l);
+    if (c->device != levels[0]->device) return wtg_fail(nullptr, WTG_E_ARG, "level %d is on another device", l);
+    if (!c->topo_set || !c->fs_params_set)
+      return wtg_fail(nullptr, WTG_E_STATE, "level %d: topography and free-surface parameters must be set", l);
+    for (int k = 0; k < l; ++k)
+      if (levels[k] == c) return wtg_fail(nullptr, WTG_E_ARG, "level %d repeats a context", l);
+  }
+  auto m = std::make_unique<wtg_mg>();
+  m->nl = nlevels;
+  m->lv.assign(levels, levels + nlevels);
+  m->ovl.assign(nlevels, nullptr);
+  wtg_ctx* top = levels[nlevels - 1];
+  for (int l = 0; l < nlevels; ++l) {
+    if (int rc = fs_ready(levels[l])) {
+      g_create_error = levels[l]->err;
+      return rc;
+    }
+    if (hipStreamSynchronize(levels[l]->stream) != hipSuccess)
+      return wtg_fail(nullptr, WTG_E_DEVICE, "level %d: stream synchronisation failed", l);
+  }
+  for (int l = 1; l < nlevels; ++l) {
+    std::vector<double> a;
+    try {
+      a = wtg::overlap_areas(grids[l]->lv, grids[l - 1]->lv);
+    } catch (const std::exception& e) {
+      wtg_mg_destroy(m.release());
+      return wtg_fail(nullptr, WTG_E_ARG, "%s", e.what());
+    }
+    if (hipMalloc((void**)&m->ovl[l], a.size() * sizeof(double)) != hipSuccess) {
+      m->ovl[l] = nullptr;
+      wtg_mg_destroy(m.release());
+      return wtg_fail(nullptr, WTG_E_DEVICE, "hipMalloc of %zu overlap areas failed", a.size());
+    }
+    if (hipMemcpy(m->ovl[l], a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      wtg_mg_destroy(m.release());
+      return wtg_fail(nullptr, WTG_E_DEVICE, "overlap area upload failed");
+    }
+  }
+  // 8x8 patches of the finest level's owned nodes (row-major within a patch)
+  const wtg::Level& fl = grids[nlevels - 1]->lv;
+  if (fl.n % 8 == 0) {
+    const int np1 = fl.n / 8;
+    m->npatch = 10 * np1 * np1;
+    std::vector<int64_t> cells((size_t)m->npatch * 64);
+    for (int lz = 0; lz < 10; ++lz)
+      for (int pj = 0; pj < np1; ++pj)
+        for (int pi = 0; pi < np1; ++pi) {
+          const int64_t p = ((int64_t)lz * np1 + pj) * np1 + pi;
+          for (int b = 0; b < 8; ++b)
+            for (int a = 0; a < 8; ++a) cells[p * 64 + b * 8 + a] = (int64_t)fl.cell(lz, 8 * pi + a, 8 * pj + b);
+        }
+    const size_t NC = fl.ncells();
+    std::vector<double> fn(NC);
+    for (size_t c = 0; c < NC; ++c) fn[c] = fl.X[2 * NC + c] / fl.radius;
+    bool ok = hipMalloc((void**)&m->d_cells, cells.size() * sizeof(int64_t)) == hipSuccess &&
+              hipMalloc((void**)&m->d_fnode, NC * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&m->d_sigma, (size_t)m->npatch * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&m->d_feat, (size_t)m->npatch * 192 * sizeof(float)) == hipSuccess;
+    ok = ok && hipMemcpy(m->d_cells, cells.data(), cells.size() * sizeof(int64_t), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(m->d_fnode, fn.data(), NC * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+      wtg_mg_destroy(m.release());
+      return wtg_fail(nullptr, WTG_E_DEVICE, "patch table upload failed");
+    }
+  }
+  for (auto* c : m->lv) c->stream = top->own_stream;
+  *out = m.release();
+  return 0;
+}
+
+void wtg_mg_destroy(wtg_mg* m) {
+  if (!m) return;
+  if (!m->lv.empty()) hipStreamSynchronize(m->lv.back()->stream);
+  for (auto* c : m->lv) c->stream = c->own_stream;
+  for (double* a : m->ovl)
+    if (a) hipFree(a);
+  for (void* a : {(void*)m->d_cells, (void*)m->d_fnode, (void*)m->d_sigma, (void*)m->d_feat})
+    if (a) hipFree(a);
+  delete m;
+}
+
+int wtg_mg_solve(wtg_mg* m, int method, const wtg_mg_params* p, int* iters, double* rel_resid) {
+  if (!m) return WTG_E_ARG;
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (!p || !iters || !rel_resid) return wtg_fail(top, WTG_E_ARG, "null argument");
+  if (p->coarse_iter < 0 || p->max_vcycle < 0 || p->pre_iter < 0 || p->down_iter < 0 || p->up_iter < 0 ||
+      p->post_iter < 0 || p->max_srj_iter < 0)
+    return wtg_fail(top, WTG_E_ARG, "negative iteration count");
+  if (method == WTG_SOLVE_FMG) return mg_fmg(m, *p, iters, rel_resid);
+  if (method == WTG_SOLVE_SRJ) return mg_srj(m, *p, iters, rel_resid);
+  if (method == WTG_SOLVE_VCYCLE) return mg_cycles(m, *p, false, iters, rel_resid);
+  if (method == WTG_SOLVE_LEARNED) {
+    if (!m->mlp) return wtg_fail(top, WTG_E_STATE, "no network attached (wtg_mg_set_learned)");
+    return mg_cycles(m, *p, true, iters, rel_resid);
+  }
+  return wtg_fail(top, WTG_E_ARG, "unknown multigrid method %d", method);
+}
+
+int wtg_mg_forward(wtg_mg* m, int fine, int s_field, int wc_field, int coarse_field) {
+  if (!m) return WTG_E_ARG;
+  RC(mg_check_level(m, fine));
+  wtg_ctx *f = m->lv[fine], *c = m->lv[fine - 1];
+  RC(mg_err(m, fs_field(f, s_field), fine));
+  RC(mg_err(m, fs_field(f, wc_field), fine));
+  RC(mg_err(m, fs_field(c, coarse_field), fine - 1));
+  if (s_field == wc_field) return wtg_fail(m->lv[m->nl - 1], WTG_E_ARG, "wavelets must not overwrite the scalar");
+  RC(mg_err(m, mg_forward(m, fine, f->fs[s_field], f->fs[wc_field], c->fs[coarse_field]), fine));
+  WTG_HIPCHK(f, hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+int wtg_mg_inverse(wtg_mg* m, int fine, int coarse_field, int wc_field, int fine_field) {
+  if (!m) return WTG_E_ARG;
+  RC(mg_check_level(m, fine));
+  wtg_ctx *f = m->lv[fine], *c = m->lv[fine - 1];
+  RC(mg_err(m, fs_field(c, coarse_field), fine - 1));
+  RC(mg_err(m, fs_field(f, wc_field), fine));
+  RC(mg_err(m, fs_field(f, fine_field), fine));
+  if (wc_field == fine_field) return wtg_fail(m->lv[m->nl - 1], WTG_E_ARG, "the scalar must not overwrite the wavelets");
+  RC(mg_err(m, mg_inverse(m, fine, c->fs[coarse_field], f->fs[wc_field], f->fs[fine_field]), fine));
+  WTG_HIPCHK(f, hipStreamSynchronize(f->stream));
+  return 0;
+}
+
+
+int wtg_mg_npatches(const wtg_mg* m) { return m ? m->npatch : WTG_E_ARG; }
+
+int wtg_mg_set_learned(wtg_mg* m, wtg_mlp* mlp) {
+  if (!m) return WTG_E_ARG;
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (mlp) {
+    if (!m->npatch) return wtg_fail(top, WTG_E_ARG, "the finest level's n is not a multiple of 8");
+    uint16_t* x0;
+    float* y;
+    int maxb;
+    wtg_mlp_buffers(mlp, &x0, &y, &maxb);
+    if (maxb < m->npatch)
+      return wtg_fail(top, WTG_E_ARG, "the network takes %d rows, the finest level has %d patches", maxb, m->npatch);
+  }
+  m->mlp = mlp;
+  return 0;
+}
+
+int wtg_mg_features(wtg_mg* m, int field, float* feats, double* sigma, int64_t* cells) {
+  if (!m) return WTG_E_ARG;
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (!m->npatch) return wtg_fail(top, WTG_E_ARG, "the finest level's n is not a multiple of 8");
+  RC(fs_field(top, field));
+  if (!feats || !sigma) return wtg_fail(top, WTG_E_ARG, "null output");
+  RC(lc_features(m, top->fs[field], false));
+  WTG_HIPCHK(top, hipMemcpyAsync(feats, m->d_feat, (size_t)m->npatch * 192 * sizeof(float), hipMemcpyDeviceToHost,
+                                 top->stream));
+  WTG_HIPCHK(top, hipMemcpyAsync(sigma, m->d_sigma, (size_t)m->npatch * sizeof(double), hipMemcpyDeviceToHost,
+                                 top->stream));
+  if (cells)
+    WTG_HIPCHK(top, hipMemcpyAsync(cells, m->d_cells, (size_t)m->npatch * 64 * sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, top->stream));
+  WTG_HIPCHK(top, hipStreamSynchronize(top->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,13 @@
+// Internal: what the free-surface solver uses of the learned correction.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/wavetrisk_gpu.h"
+
+// bf16 input rows [max_batch][WTG_MLP_IN] and fp32 output rows [max_batch][WTG_MLP_OUT]
+void wtg_mlp_buffers(wtg_mlp* m, uint16_t** x0, float** y, int* max_batch);
+// forward pass of the first B rows of the input buffer, on `stream`
+int wtg_mlp_forward_on(wtg_mlp* m, hipStream_t stream, int B);
+const char* wtg_mlp_error_text(const wtg_mlp* m);

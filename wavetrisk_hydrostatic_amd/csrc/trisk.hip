@@ -1376,6 +1376,7 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
   std::unique_ptr<wtg_ctx> c(new wtg_ctx);
   if (hipGetDevice(&c->device) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipGetDevice failed");
   if (hipStreamCreate(&c->stream) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipStreamCreate failed");
+  c->own_stream = c->stream;
   wtg_grid_info_get(grid, &c->info);
   c->params = *params;
   c->nslots = nslots;
