@@ -185,6 +185,18 @@ int wtg_rhs_elliptic(wtg_ctx* ctx);
  * residual. */
 int wtg_elliptic_solve(wtg_ctx* ctx, int method, double rel_tol, int max_iter, int* iters, double* rel_resid);
 
+/* dt_step_split => RK4_split (time_integr.f90:360-382, main.f90:209): nsteps
+ * split-explicit steps of size dt (wtg_fs_set_params) on state slot `slot`
+ * with the free surface in WTG_FS_ETA (sol(S_MASS, zlevels+1)).  Per step:
+ * flux divergence of the old state into WTG_FS_DIVF_OLD; four RK_split stages
+ * (444-462: barotropic_correction of the stage input, trend_ml, scalar_star
+ * and u_star with the theta1-weighted external pressure gradient removed);
+ * free_surface_update (464-478: rhs_elliptic, eta_old := eta, the elliptic
+ * solve of WTG_SOLVE_BICGSTAB to rel_tol -- the reference's SRJ on a single
+ * level --, barotropic_correction, u_update).  Porosity 1, mean state 0.
+ * *iters = BiCGSTAB iterations of the last step. */
+int wtg_rk4_split(wtg_ctx* ctx, int slot, int nsteps, double rel_tol, int max_iter, int* iters);
+
 /* ---- multi-level free surface: scalar wavelet transforms, FMG, SRJ -------
  * A hierarchy is a list of free-surface contexts on consecutive uniform
  * levels, coarse (index 0) to fine, each with its topography, dt / theta and
@@ -262,8 +274,8 @@ int wtg_mg_features(wtg_mg* mg, int field, float* feats, double* sigma, int64_t*
  * ------------------------------------------------------------------------- */
 enum { WTG_MLP_IN = 192, WTG_MLP_HIDDEN = 128, WTG_MLP_OUT = 64, WTG_MLP_NPARAMS = 49472 };
 
-/* He-normal weights from `seed`, zero biases; activations for up to
- * max_batch rows */
+/* He-normal weights from `seed` (the output layer's scaled by 0.1), zero
+ * biases; activations for up to max_batch rows */
 int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out);
 void wtg_mlp_destroy(wtg_mlp* mlp);
 int wtg_mlp_last_error(const wtg_mlp* mlp, char* buf, size_t n);

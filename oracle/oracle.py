@@ -46,6 +46,10 @@ def load():
         _lib.wto_prolong.argtypes = [C.c_void_p] * 6
         _lib.wto_fmg.argtypes = [C.c_void_p] * 5
         _lib.wto_srj.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        _lib.wto_barotropic_correction.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        _lib.wto_grad_eta.argtypes = [C.c_void_p] * 3
+        _lib.wto_rk4_split.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
+        _lib.wto_rk4_split.restype = C.c_int
     return _lib
 
 
@@ -176,6 +180,27 @@ class EllipticOracle(Oracle):
         self.lib.wto_flux_divergence(C.byref(self.g), mass.shape[0], mass.ctypes.data, velo.ctypes.data,
                                      out.ctypes.data)
         return out
+
+    def barotropic_correction(self, eta, mass, temp):
+        eta = _f64(eta)
+        mass, temp = _f64(mass).copy(), _f64(temp).copy()
+        self.lib.wto_barotropic_correction(C.byref(self.g), mass.shape[0], eta.ctypes.data, mass.ctypes.data,
+                                           temp.ctypes.data)
+        return mass, temp
+
+    def grad_eta(self, eta):
+        eta = _f64(eta)
+        out = np.zeros((3, self.layout.ncells))
+        self.lib.wto_grad_eta(C.byref(self.g), eta.ctypes.data, out.ctypes.data)
+        return out
+
+    def rk4_split(self, mass, temp, velo, eta, tol=1e-13, maxit=400):
+        """one RK4_split step of size dt (the FsParams dt); returns the new
+        (mass, temp, velo, eta) and the BiCGSTAB iteration count"""
+        mass, temp, velo, eta = (_f64(a).copy() for a in (mass, temp, velo, eta))
+        it = self.lib.wto_rk4_split(C.byref(self.g), C.byref(self.fs), self.weight.ctypes.data, mass.shape[0], tol,
+                                    maxit, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data, eta.ctypes.data)
+        return mass, temp, velo, eta, it
 
     def rhs(self, eta, divF, divF_old):
         eta, divF, divF_old = _f64(eta), _f64(divF), _f64(divF_old)

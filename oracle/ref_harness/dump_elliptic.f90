@@ -15,8 +15,12 @@
 !   e_jac               one Jacobi_iteration of u = eta^n with f, weight wjac (lin_solve.f90:271-340)
 !   e_divF, e_rhs       flux_divergence of sol and the RHS of rhs_elliptic (barotropic_2d.f90:151-185, 326-366)
 !   e_sol, e_iters      bicgstab (u = eta^n, f = e_rhs) to tol_solve (lin_solve.f90:361-431)
+! and with split = 1, each from the initial state again:
+!   bc_mass, bc_temp, bc_velo   barotropic_correction of the state
+!   ge_flux                     grad_eta: g grad(eta) on the edges (horiz_flux(S_TEMP))
+!   sp_mass, sp_temp, sp_velo, sp_eta   the state after one RK4_split step
 !
-! Namelist &run (stdin): zlev, step (dt), theta_1, theta_2, wjac, tol_solve, maxit, out.
+! Namelist &run (stdin): zlev, step (dt), theta_1, theta_2, wjac, tol_solve, maxit, split, out.
 module ell_hooks
   use domain_mod
   implicit none
@@ -135,12 +139,12 @@ program dump_elliptic
   implicit none
 
   character(255) :: out
-  integer        :: zlev, maxit, d, k, l, u, iters, j, nmax
+  integer        :: zlev, maxit, d, k, l, u, iters, j, nmax, split
   real(8)        :: step, theta_1, theta_2, wjac, tol_solve, err, nrm
   type(Float_Field) :: Lq, dgl, uu
-  namelist /run/ zlev, step, theta_1, theta_2, wjac, tol_solve, maxit, out
+  namelist /run/ zlev, step, theta_1, theta_2, wjac, tol_solve, maxit, split, out
 
-  zlev = 2; step = 600d0; theta_1 = 1d0; theta_2 = 1d0; wjac = 0.7d0; tol_solve = 1d-9; maxit = 200
+  zlev = 2; step = 600d0; theta_1 = 1d0; theta_2 = 1d0; wjac = 0.7d0; tol_solve = 1d-9; maxit = 200; split = 0
   out = 'ell.bin'
   read (5, nml=run)
 
@@ -277,8 +281,54 @@ program dump_elliptic
   call dump_field ('e_sol', uu)
   call wr_i ('e_iters', (/ iters /))
   call wr_f ('e_err', (/ err, nrm /))
+
+  if (split > 0) then
+     ! barotropic_correction of the initial state (barotropic_2d.f90:52-116)
+     call reset_state
+     call barotropic_correction (sol(:,1:zmax))
+     call dump_layers ('bc')
+     ! external pressure gradient of the initial free surface (grad_eta, step1 itype 3)
+     call reset_state
+     call grad_eta
+     do d = 1, size(grid)
+        call wr_f ('ge_flux', horiz_flux(S_TEMP)%data(d)%elts(1:EDGE*grid(d)%node%length))
+     end do
+     ! one RK4_split step (time_integr.f90:360-382) with the SRJ solver
+     ! (main.f90:50), which on one level is BiCGSTAB to coarse_tol
+     call reset_state
+     elliptic_solver => SRJ
+     coarse_tol = tol_solve
+     fine_tol   = tol_solve
+     coarse_iter = maxit
+     log_iter = .false.
+     call RK4_split (dt)
+     call update_bdry (sol(:,1:zmax), NONE, 1)
+     call dump_layers ('sp')
+     call dump_field ('sp_eta', sol(S_MASS,zlevels+1))
+  end if
   close (u)
 contains
+  subroutine reset_state
+    call apply_onescale (init_sol, level_start, z_null, -1, 1)
+    call update_bdry (sol(:,1:zmax), NONE, 1)
+    do d = 1, size(grid)
+       trend(S_MASS,zlevels+1)%data(d)%elts = 0d0
+       trend(S_TEMP,zlevels+1)%data(d)%elts = 0d0
+    end do
+  end subroutine reset_state
+
+  subroutine dump_layers (pre)
+    character(*) :: pre
+    integer :: kk, dd
+    do kk = 1, zlevels
+       do dd = 1, size(grid)
+          call wr_f (pre // '_mass', sol(S_MASS,kk)%data(dd)%elts)
+          call wr_f (pre // '_temp', sol(S_TEMP,kk)%data(dd)%elts)
+          call wr_f (pre // '_velo', sol(S_VELO,kk)%data(dd)%elts)
+       end do
+    end do
+  end subroutine dump_layers
+
   subroutine wr_i (tag, a)
     character(*)          :: tag
     integer, dimension(:) :: a

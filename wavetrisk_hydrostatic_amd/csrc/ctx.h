@@ -91,3 +91,6 @@ inline int wtg_dalloc(wtg_ctx* c, Tp** p, size_t count) {
   *p = (Tp*)q;
   return 0;
 }
+
+// allocates the free-surface fields on first use (elliptic.hip)
+int wtg_fs_ready(wtg_ctx* c);

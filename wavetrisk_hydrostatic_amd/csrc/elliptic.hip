@@ -609,6 +609,8 @@ int jacobi_solve(wtg_ctx* c, double* u, const double* f, double tol, int maxit, 
 
 }  // namespace
 
+int wtg_fs_ready(wtg_ctx* c) { return fs_ready(c); }
+
 // hierarchy of free-surface contexts, coarse (0) to fine (nl - 1)
 struct wtg_mg {
   int nl = 0;

@@ -438,17 +438,18 @@ int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out) {
     wtg_mlp_destroy(m.release());
     return rc;
   }
-  // He-normal weights (fan in), zero biases, from the seed
+  // He-normal weights (fan in), the output layer scaled by 0.1 so that an
+  // untrained network proposes small corrections; zero biases
   std::vector<float> p(NPAR, 0.f);
   std::mt19937_64 gen(seed);
   std::normal_distribution<double> nd(0.0, 1.0);
-  auto fill = [&](int off, int fin, int fout) {
-    const double s = std::sqrt(2.0 / fin);
+  auto fill = [&](int off, int fin, int fout, double scale) {
+    const double s = scale * std::sqrt(2.0 / fin);
     for (int i = 0; i < fin * fout; ++i) p[off + i] = (float)(s * nd(gen));
   };
-  fill(OW1, NI, NH);
-  fill(OW2, NH, NH);
-  fill(OW3, NH, NO);
+  fill(OW1, NI, NH, 1.0);
+  fill(OW2, NH, NH, 1.0);
+  fill(OW3, NH, NO, 0.1);
   std::vector<int> wmap(NPAR, -1);
   for (int i = 0; i < NI * NH; ++i) wmap[OW1 + i] = BW1 + i;
   for (int i = 0; i < NH * NH; ++i) wmap[OW2 + i] = BW2 + i;

@@ -775,43 +775,6 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
   w[4] = 0.5 - acc4 * hi;
 }
 
-// 16-byte loads of region rows.  A region row (ri = 0..18) is read as ten
-// aligned pairs of cells starting one cell to its left (ri = -1..18): the
-// cell left of a tile region has an even index, so every pair is 16-byte
-// aligned.  Lane slot r < 190 of a plane covers row r / 10, pair r % 10; a
-// plane takes three waves (192 slots, the last two clamped).
-constexpr int PAIRS = 10, PSLOTS = R * PAIRS;  // 190
-
-__device__ __forceinline__ double2 ldo2(const double* p, uint32_t off) {
-  return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(p) + off);
-}
-
-struct PairSlot {
-  uint32_t off;  // byte offset of the pair in a plane
-  int li;        // LDS index of the pair's second cell (ri = 2 * pair)
-  bool valid;    // slot < 190
-  bool first;    // the pair's first cell (ri = 2 * pair - 1) is in the region (pair > 0)
-};
-
-__device__ __forceinline__ PairSlot pair_slot(int r, int64_t base, int S) {
-  PairSlot ps;
-  ps.valid = r < PSLOTS;
-  const int rc = r < PSLOTS ? r : PSLOTS - 1;
-  const int prj = rc / PAIRS, pq = rc - prj * PAIRS;
-  ps.off = (uint32_t)(base - 1 + (int64_t)prj * S + 2 * pq) * 8u;
-  ps.li = prj * R + 2 * pq;
-  ps.first = pq > 0;
-  return ps;
-}
-
-// writes a loaded pair into an LDS plane (the cell at ri = -1 is dropped)
-__device__ __forceinline__ void put_pair(double* plane, const PairSlot& ps, double2 v) {
-  if (ps.valid) {
-    if (ps.first) plane[ps.li - 1] = v.x;
-    plane[ps.li] = v.y;
-  }
-}
-
 __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a) {
   __shared__ GeoS gs;
   __shared__ union {
@@ -858,23 +821,20 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   // waited for before its use.
   const int lo = lt % (T * T);
   const uint32_t obl = (uint32_t)gcell(1 + lo % T, 1 + lo / T) * 8u;
-  const int wv = __builtin_amdgcn_readfirstlane(lt / 64);  // wave in the group
-  const int lane = lt % 64;
 
-  // --- launch-invariant geometry into LDS: 18 planes x 3 slot waves, spread
-  // over the block's 12 waves -------------------------------------------------
-  {
-    const int W = grp * 6 + wv;
-    double* gplane0 = &gs.len[0][0];
-    for (int sw = W; sw < 18 * 3; sw += 12) {
-      const int pl = sw / 3;
-      const PairSlot ps = pair_slot((sw % 3) * 64 + lane, base, S);
-      const double* src = pl < 3 ? g.len + pl * NC
-                          : pl < 6 ? g.ped + (pl - 3) * NC
-                          : pl < 9 ? g.pedf + (pl - 6) * NC
-                          : pl < 15 ? g.area + (pl - 9) * NC
-                          : pl == 15 ? g.hinv : g.cor + (pl - 16) * NC;
-      put_pair(gplane0 + pl * RR, ps, ldo2(src, ps.off));
+  // --- launch-invariant geometry into LDS (each group stages half) ----------
+  if (active) {
+    if (grp == 0) {
+      for (int e = 0; e < 3; ++e) {
+        gs.len[e][x] = ldo(g.len + e * NC, xb);
+        gs.ped[e][x] = ldo(g.ped + e * NC, xb);
+        gs.pedf[e][x] = ldo(g.pedf + e * NC, xb);
+      }
+    } else {
+      for (int p = 0; p < 6; ++p) gs.area[p][x] = ldo(g.area + p * NC, xb);
+      gs.hinv[x] = ldo(g.hinv, xb);
+      gs.cor[0][x] = ldo(g.cor, xb);
+      gs.cor[1][x] = ldo(g.cor + NC, xb);
     }
   }
 
@@ -885,16 +845,14 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
   double P = 0, G = 0;
   {
     constexpr int KMAX = (int)(sizeof(lsu.flat) / sizeof(double)) / RR;  // layers per round
-    const PairSlot ps = pair_slot((wv % 3) * 64 + lane, base, S);
     double s = 0.0;
     for (int k0 = 0; k0 < a.L; k0 += KMAX) {
       const int kn = min(KMAX, a.L - k0);
-      // the group's layers kk = grp, grp + 2, ...; waves 0-2 and 3-5 alternate
-#pragma unroll 4
-      for (int kk = grp + 2 * (wv / 3); kk < kn; kk += 4) {
-        const int64_t ks = (int64_t)(k0 + kk) * NC;
-        const double2 vm = ldo2(a.ym + ks, ps.off), vt = ldo2(a.yt + ks, ps.off);
-        put_pair(lsu.flat + kk * RR, ps, make_double2(vm.x - vt.x, vm.y - vt.y));
+      if (active) {
+        for (int kk = grp; kk < kn; kk += 2) {
+          const int64_t ks = (int64_t)(k0 + kk) * NC;
+          lsu.flat[kk * RR + x] = ldo(a.ym + ks, xb) - ldo(a.yt + ks, xb);
+        }
       }
       __syncthreads();
       if (p_cell)
@@ -909,32 +867,23 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
 
   LayerS& sl = lsu.ls[grp];
   const LayerS& so = lsu.ls[1 - grp];
-  // layer staging slots: slot wave sw in [0, 15) covers plane sw / 3 (m, t,
-  // u0, u1, u2) part sw % 3; wave wv takes sw = wv, wv + 6 and, for wv < 3,
-  // wv + 12.  Plane pl of layer k:
-  auto lplane = [&](int pl, int k) -> const double* {
-    return pl == 0 ? a.ym + (int64_t)k * NC : pl == 1 ? a.yt + (int64_t)k * NC : a.yu + (int64_t)(3 * k + pl - 2) * NC;
-  };
-  const int sw0 = wv, sw1 = wv + 6, sw2 = wv + 12;
-  const bool has2 = wv < 3;
-  // slots are recomputed where used (a few integer operations) rather than
-  // held in registers through the layer loop
-  auto lslot = [&](int sw) { return pair_slot((sw % 3) * 64 + lane, base, S); };
   // the group's first layer, staged here; later layers are staged at the end
   // of the previous iteration from registers loaded one iteration ahead
+  double m, t, u0, u1, u2;
   {
-    const int k = min(grp, a.L - 1);
-    const PairSlot ls0 = lslot(sw0), ls1 = lslot(sw1), ls2 = lslot(sw2);
-    const double2 v0 = ldo2(lplane(sw0 / 3, k), ls0.off);
-    const double2 v1 = ldo2(lplane(sw1 / 3, k), ls1.off);
-    double2 v2 = make_double2(0.0, 0.0);
-    if (has2) v2 = ldo2(lplane(sw2 / 3, k), ls2.off);
-    if (grp < a.L) {
-      double* lp0 = &sl.m[0];
-      put_pair(lp0 + (sw0 / 3) * RR, ls0, v0);
-      put_pair(lp0 + (sw1 / 3) * RR, ls1, v1);
-      if (has2) put_pair(lp0 + (sw2 / 3) * RR, ls2, v2);
-    }
+    const int64_t ks = (int64_t)min(grp, a.L - 1) * NC, ke3 = 3 * ks;
+    m = ldo(a.ym + ks, xb);
+    t = ldo(a.yt + ks, xb);
+    u0 = ldo(a.yu + ke3, xb);
+    u1 = ldo(a.yu + ke3 + NC, xb);
+    u2 = ldo(a.yu + ke3 + 2 * NC, xb);
+  }
+  if (active && grp < a.L) {
+    sl.m[x] = m;
+    sl.t[x] = t;
+    sl.u[0][x] = u0;
+    sl.u[1][x] = u1;
+    sl.u[2][x] = u2;
   }
   __builtin_amdgcn_s_waitcnt(0);  // nothing outstanding enters the layer loop
 
@@ -954,22 +903,15 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     const bool pair = 2 * it + 1 < a.L;       // both layers of the pair exist
     const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
     __syncthreads();
-    // --- 1. prefetch the group's next layer (its loads overlap phases 2-4);
-    //        the cell's own state from LDS ---------------------------------------
-    double2 pf0, pf1, pf2 = make_double2(0.0, 0.0);
+    // --- 1. prefetch the group's next layer (its loads overlap phases 2-4) ----
+    double nm, nt, nu0, nu1, nu2;
     {
-      const int kp = min(k + 2, a.L - 1);
-      pf0 = ldo2(lplane(sw0 / 3, kp), lslot(sw0).off);
-      pf1 = ldo2(lplane(sw1 / 3, kp), lslot(sw1).off);
-      if (has2) pf2 = ldo2(lplane(sw2 / 3, kp), lslot(sw2).off);
-    }
-    double m = 0, t = 0, u0 = 0, u1 = 0, u2 = 0;
-    if (active) {
-      m = sl.m[x];
-      t = sl.t[x];
-      u0 = sl.u[0][x];
-      u1 = sl.u[1][x];
-      u2 = sl.u[2][x];
+      const int64_t ks2 = (int64_t)min(k + 2, a.L - 1) * NC, ke32 = 3 * ks2;
+      nm = ldo(a.ym + ks2, xb);
+      nt = ldo(a.yt + ks2, xb);
+      nu0 = ldo(a.yu + ke32, xb);
+      nu1 = ldo(a.yu + ke32 + NC, xb);
+      nu2 = ldo(a.yu + ke32 + 2 * NC, xb);
     }
 
     // --- 2. column scan; PV, fluxes; kinetic energy, Bernoulli, Exner ---------
@@ -1027,8 +969,8 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
 
     // --- 3. base state of the update (q = y for a trend), consumed in phase 4;
     //        qe on edges; potential temperature t/m in place ------------------
-    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
-    if (wv < 4) {  // the owned waves
+    double bm, bt, bu0, bu1, bu2;
+    {
       const int64_t ksb = (int64_t)min(k, a.L - 1) * NC, ke3b = 3 * ksb;
       bm = ldo(a.qm + ksb, obl);
       bt = ldo(a.qt + ksb, obl);
@@ -1102,7 +1044,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
         sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
       }
     }
-    if (wv < 4 && !(lay && own)) {
+    if (!(lay && own)) {
       // threads that store nothing still wait for their base-state loads
       // here, so no later phase inherits an unknown wait on them
       asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
@@ -1118,12 +1060,18 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a)
     }
     __syncthreads();
     // --- 5. stage the group's next layer --------------------------------------
-    if (k + 2 < a.L) {
-      double* lp0 = &sl.m[0];
-      put_pair(lp0 + (sw0 / 3) * RR, lslot(sw0), pf0);
-      put_pair(lp0 + (sw1 / 3) * RR, lslot(sw1), pf1);
-      if (has2) put_pair(lp0 + (sw2 / 3) * RR, lslot(sw2), pf2);
+    if (active && k + 2 < a.L) {
+      sl.m[x] = nm;
+      sl.t[x] = nt;
+      sl.u[0][x] = nu0;
+      sl.u[1][x] = nu1;
+      sl.u[2][x] = nu2;
     }
+    m = nm;
+    t = nt;
+    u0 = nu0;
+    u1 = nu1;
+    u2 = nu2;
   }
 }
 
@@ -1163,6 +1111,74 @@ __global__ void copy_frozen_kernel(int L, int64_t NC, const int64_t* __restrict_
   } else {
     int64_t p = fe[e - nn];
     du_[(int64_t)k * 3 * NC + p] = su_[(int64_t)k * 3 * NC + p];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// split-explicit step with implicit free surface (mode split)
+// ---------------------------------------------------------------------------
+// barotropic_correction (barotropic_2d.f90:52-116) in place over the masters
+// (the first nmaster entries of the node compute list, dst == src): layer
+// dilation of mass / temp to the free surface eta; porosity 1, mean state 0
+__global__ void baro_correction_kernel(int nmaster, const int64_t* __restrict__ ncl, int L, int64_t NC, double rho0,
+                                       const double* __restrict__ eta, const double* __restrict__ topo, double* m,
+                                       double* t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmaster) return;
+  const int64_t c = ncl[2 * i + 1];
+  double h = 0.0;  // total_height (cal_height: rho_dz / porous_density)
+  for (int k = 0; k < L; ++k) h = h + (0.0 + m[(int64_t)k * NC + c]) / (rho0 * 1.0);
+  const double e = eta[c] / 1.0, b = topo[c];
+  for (int k = 0; k < L; ++k) {
+    const int64_t x = (int64_t)k * NC + c;
+    double rho_dz = 0.0 + m[x];
+    const double theta = (0.0 + t[x]) / rho_dz;
+    const double mm = (e - b) / h * rho_dz - 0.0;
+    m[x] = mm;
+    rho_dz = 0.0 + mm;
+    t[x] = rho_dz * theta - 0.0;
+  }
+}
+
+// g grad(eta) on edge e of cell c (grad_eta, step1 itype 3, ops.f90:217-220)
+__device__ __forceinline__ double grad_eta_e(const double* __restrict__ eta, const double* __restrict__ len,
+                                             int64_t NC, int S, int64_t c, int e, double grav) {
+  if (e == 0) return grav * (eta[c + 1] / 1.0 - eta[c] / 1.0) / len[c];
+  if (e == 1) return grav * (eta[c] / 1.0 - eta[c + S + 1] / 1.0) / len[NC + c];
+  return grav * (eta[c + S] / 1.0 - eta[c] / 1.0) / len[2 * NC + c];
+}
+
+// scalar_star and u_star (barotropic_2d.f90:9-50) over the masters:
+// out = base + h tr (scalars), out = base + h (tr + theta1 g grad eta) (edges)
+__global__ void split_axpy_kernel(int nmaster, const int64_t* __restrict__ ncl, int L, int64_t NC, int S, double h,
+                                  double th1, double grav, const double* __restrict__ eta,
+                                  const double* __restrict__ len, const double* bm, const double* bt,
+                                  const double* bu, const double* tm, const double* tt, const double* tu, double* om,
+                                  double* ot, double* ou) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nmaster * L) return;
+  const int k = (int)(idx / nmaster), i = (int)(idx % nmaster);
+  const int64_t c = ncl[2 * i + 1], x = (int64_t)k * NC + c;
+  om[x] = bm[x] + h * tm[x];
+  ot[x] = bt[x] + h * tt[x];
+  for (int e = 0; e < 3; ++e) {
+    const int64_t y = ((int64_t)k * 3 + e) * NC + c;
+    ou[y] = bu[y] + h * (tu[y] + th1 * grad_eta_e(eta, len, NC, S, c, e, grav));
+  }
+}
+
+// u_update (barotropic_2d.f90:131-149) in place over the masters: u -= theta1 dt g grad eta
+__global__ void u_update_kernel(int nmaster, const int64_t* __restrict__ ncl, int L, int64_t NC, int S, double th1dt,
+                                double grav, const double* __restrict__ eta, const double* __restrict__ len,
+                                double* u) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nmaster * L) return;
+  const int k = (int)(idx / nmaster), i = (int)(idx % nmaster);
+  const int64_t c = ncl[2 * i + 1];
+  for (int e = 0; e < 3; ++e) {
+    const int64_t y = ((int64_t)k * 3 + e) * NC + c;
+    u[y] = u[y] - th1dt * grad_eta_e(eta, len, NC, S, c, e, grav);
   }
 }
 
@@ -1287,6 +1303,70 @@ int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
       if (ev) HIPCHK(c, hipEventRecord(ev[2 * launch + 1], c->stream));
       if (int rc = launch_ghost_fill(c, *os[st])) return rc;
     }
+  }
+  return 0;
+}
+
+
+// RK4_split (time_integr.f90:360-382): four RK_split stages (444-462) and
+// free_surface_update (464-478); the free surface solved by BiCGSTAB, the
+// reference's SRJ on a single level
+int baro_correction(wtg_ctx* c, const wtg_ctx::Slot& s) {
+  const int n = c->nmaster;
+  hipLaunchKernelGGL(baro_correction_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, c->d_ncl,
+                     c->params.nlayers, c->info.ncells, c->params.ref_density, c->fs[FS_ETA], c->d_topo, s.m, s.t);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int split_impl(wtg_ctx* c, int slot, int nsteps, double tol, int maxit, int* iters) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
+  if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
+  if (nsteps < 0 || maxit < 0) return fail(c, WTG_E_ARG, "negative step or iteration count");
+  if (int rc = wtg_fs_ready(c)) return rc;
+  const auto& sol = c->slots[slot];
+  const auto& w1 = c->slots[c->nslots];
+  const auto& w2 = c->slots[c->nslots + 1];
+  const int L = c->params.nlayers;
+  const int64_t NC = c->info.ncells;
+  const double dt = c->fs_dt, th1 = c->fs_theta1, grav = c->params.grav;
+  if (int rc = launch_copy_frozen(c, sol, w1)) return rc;
+  if (int rc = launch_copy_frozen(c, sol, w2)) return rc;
+  const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
+  const wtg_ctx::Slot* ins[4] = {&sol, &w1, &w2, &w1};
+  const wtg_ctx::Slot* trs[4] = {&w1, &w2, &w1, &w2};
+  const wtg_ctx::Slot* outs[4] = {&w1, &w2, &w1, &sol};
+  const unsigned nb = (unsigned)(((int64_t)c->nmaster * L + 255) / 256);
+  for (int s = 0; s < nsteps; ++s) {
+    if (int rc = launch_ghost_fill(c, sol)) return rc;
+    // flux divergence of the layer-integrated velocity at the previous step
+    if (int rc = wtg_flux_divergence(c, slot)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->fs[FS_DIVF_OLD], c->fs[FS_DIVF], NC * sizeof(double), hipMemcpyDeviceToDevice,
+                             c->stream));
+    for (int st = 0; st < 4; ++st) {
+      const auto &in = *ins[st], &tr = *trs[st], &out = *outs[st];
+      if (int rc = baro_correction(c, in)) return rc;
+      if (int rc = launch_ghost_fill(c, in)) return rc;
+      if (int rc = launch_stage(c, in, in, tr, 0.0, 1)) return rc;
+      hipLaunchKernelGGL(split_axpy_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC,
+                         c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, tr.m, tr.t,
+                         tr.u, out.m, out.t, out.u);
+      HIPCHK(c, hipGetLastError());
+      if (int rc = launch_ghost_fill(c, out)) return rc;
+    }
+    // free_surface_update: eta_update, barotropic_correction, u_update
+    if (int rc = wtg_flux_divergence(c, slot)) return rc;
+    if (int rc = wtg_rhs_elliptic(c)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NC * sizeof(double), hipMemcpyDeviceToDevice,
+                             c->stream));
+    double res = 0.0;
+    if (int rc = wtg_elliptic_solve(c, WTG_SOLVE_BICGSTAB, tol, maxit, iters, &res)) return rc;
+    if (int rc = baro_correction(c, sol)) return rc;
+    hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC, c->info.S,
+                       th1 * dt, grav, c->fs[FS_ETA], c->d_len, sol.u);
+    HIPCHK(c, hipGetLastError());
+    if (int rc = launch_ghost_fill(c, sol)) return rc;
   }
   return 0;
 }
@@ -1564,6 +1644,14 @@ int wtg_trend(wtg_ctx* c, int q_slot, int dq_slot) {
   HIPCHK(c, hipMemsetAsync(dq.u, 0, 3 * ns * 8, c->stream));
   const auto& q = c->slots[q_slot];
   if (int rc = launch_stage(c, q, q, dq, 0.0, 1)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter, int* iters) {
+  if (!c || !iters) return WTG_E_ARG;
+  *iters = 0;
+  if (int rc = split_impl(c, slot, nsteps, rel_tol, max_iter, iters)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
