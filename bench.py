@@ -122,6 +122,7 @@ def main():
                "sample": f"CPU oracle (C, -O2, 1 thread), 1 RK4 step of the same J{J} grid with {Lc} layers "
                          f"({secs:.1f} s)"}
 
+    split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
     fs = None if args.no_free_surface else free_surface(args.fs_level)
 
     out = {
@@ -149,15 +150,52 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(J, L),
             "kernel": "trisk_stage_kernel",
             "kernel_ms": kernel_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        "split_step": split,
         "free_surface": fs,
     }
     print(json.dumps(out))
+
+
+def measured_traffic(J: int, L: int):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by a plain copy's readings), or None if it is absent."""
+    path = os.path.join(ROOT, "profiles", f"trisk_stage_J{J}_L{L}_hbm.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
+    """The ocean's split-explicit step (RK4_split: four trend stages at the
+    baroclinic time step, then the implicit free surface solved by BiCGSTAB
+    to 1e-9) on the same J8 L30 state: ms per step and DOF updates/s."""
+    import numpy as np
+
+    from wavetrisk_hydrostatic_amd import FS_ETA
+
+    print(f"bench: split-explicit steps, dt {dt} s", file=sys.stderr, flush=True)
+    ctx.fs_set_params(dt, 0.6, 0.8)
+    ctx.upload_state(1, mass, temp, velo)
+    ctx.fs_upload(FS_ETA, mass.sum(0) / 1028.0 + topo)
+    ctx.rk4_split(1, 1, 1e-9, 400)  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    its = [ctx.rk4_split(1, 1, 1e-9, 400) for _ in range(nsteps)]
+    ms = (time.perf_counter() - t0) * 1e3 / nsteps
+    dof = L * grid.layout.n_dof_per_layer
+    return {"dt": dt, "theta1": 0.6, "theta2": 0.8, "rel_tol": 1e-9, "ms_per_step": ms,
+            "dof_updates_per_s": dof / (ms / 1e3), "bicgstab_iterations": [int(i) for i in its],
+            "note": "host wall clock over 5 steps; the solver reads one 8-byte residual back per iteration"}
 
 
 def free_surface(fine: int):
