@@ -240,6 +240,12 @@ int wtg_mg_solve(wtg_mg* mg, int method, const wtg_mg_params* p, int* iters, dou
  * with the step a = <r, L e> / <L e, L e> when a > 0; the V-cycle is the
  * fallback and the FP64 residual ends the iteration.  For both, iters and
  * rel_resid report the finest level (coarser entries 0). */
+/* wtg_rk4_split on the finest level's context with the free surface solved
+ * on the hierarchy: eta^n restricted to every coarser level as its
+ * operator's eta_old, then WTG_SOLVE_VCYCLE or WTG_SOLVE_LEARNED cycles from
+ * eta^n to p->fine_tol (at most p->max_vcycle).  *iters = cycles of the
+ * last step. */
+int wtg_mg_rk4_split(wtg_mg* mg, int slot, int nsteps, int method, const wtg_mg_params* p, int* iters);
 /* One level of forward_scalar_transform (wavelet.f90:104-150): on level
  * `fine`, wc_field := Compute_scalar_wavelets of s_field; on level fine - 1,
  * coarse_field := Restrict_scalar of (s_field, wc_field).  s_field must be

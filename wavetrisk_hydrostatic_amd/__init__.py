@@ -92,6 +92,7 @@ SIGNATURES = {
     "wtg_mg_solve": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "wtg_mg_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "wtg_mg_inverse": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "wtg_mg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "wtg_mg_npatches": (C.c_int, [_P]),
     "wtg_mg_set_learned": (C.c_int, [_P, _P]),
     "wtg_mg_features": (C.c_int, [_P, C.c_int, _P, _P, _P]),
@@ -360,6 +361,15 @@ class Hierarchy:
         res = np.zeros(n)
         _check(_lib.wtg_mg_solve(self._h, method, C.byref(p), _ptr(it), _ptr(res)), self._top, "wtg_mg_solve")
         return it, res
+
+    def rk4_split(self, slot: int, nsteps: int = 1, method: int = SOLVE_VCYCLE, params: MgParams | None = None) -> int:
+        """split-explicit steps on the finest context with the free surface
+        solved on the hierarchy; returns the last step's cycle count"""
+        p = params if params is not None else MgParams.defaults()
+        it = C.c_int()
+        _check(_lib.wtg_mg_rk4_split(self._h, slot, nsteps, method, C.byref(p), C.byref(it)), self._top,
+               "wtg_mg_rk4_split")
+        return it.value
 
     @property
     def npatches(self) -> int:

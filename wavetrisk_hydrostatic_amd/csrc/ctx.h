@@ -94,3 +94,9 @@ inline int wtg_dalloc(wtg_ctx* c, Tp** p, size_t count) {
 
 // allocates the free-surface fields on first use (elliptic.hip)
 int wtg_fs_ready(wtg_ctx* c);
+
+// split-explicit steps (trisk.hip) with the free-surface solve supplied by
+// the caller: solve(user, c, iters) solves elliptic_lo(WTG_FS_ETA) =
+// WTG_FS_RHS on c with WTG_FS_ETA_OLD set, the old eta as initial guess
+typedef int (*wtg_fs_solver)(void* user, wtg_ctx* c, int* iters);
+int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters);

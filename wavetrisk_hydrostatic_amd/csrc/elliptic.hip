@@ -904,6 +904,29 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
   return mg_finish(V);
 }
 
+// the free-surface solve of a split-explicit step on the hierarchy: eta^n
+// restricted to every coarser level as its operator's eta_old (the
+// reference's coarse levels hold restrictions of sol), then cycles on the
+// finest level from eta^n
+struct MgSplit {
+  wtg_mg* m;
+  wtg_mg_params P;
+  bool learned;
+};
+
+int mg_split_solve(void* user, wtg_ctx* c, int* iters) {
+  auto* s = (MgSplit*)user;
+  wtg_mg* m = s->m;
+  for (int l = m->nl - 1; l >= 1; --l)
+    RC(mg_forward(m, l, m->lv[l]->fs[FS_ETA_OLD], m->lv[l]->fs[MG_WC], m->lv[l - 1]->fs[FS_ETA_OLD]));
+  std::vector<int> it(m->nl);
+  std::vector<double> res(m->nl);
+  RC(mg_cycles(m, s->P, s->learned, it.data(), res.data()));
+  *iters = it[m->nl - 1];
+  (void)c;
+  return 0;
+}
+
 int mg_check_level(wtg_mg* m, int fine) {
   wtg_ctx* top = m->lv[m->nl - 1];
   if (fine < 1 || fine >= m->nl) return wtg_fail(top, WTG_E_ARG, "fine level %d out of range [1,%d)", fine, m->nl);
@@ -1215,6 +1238,21 @@ int wtg_mg_features(wtg_mg* m, int field, float* feats, double* sigma, int64_t* 
                                    hipMemcpyDeviceToHost, top->stream));
   WTG_HIPCHK(top, hipStreamSynchronize(top->stream));
   return 0;
+}
+
+
+int wtg_mg_rk4_split(wtg_mg* m, int slot, int nsteps, int method, const wtg_mg_params* p, int* iters) {
+  if (!m) return WTG_E_ARG;
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (!p || !iters) return wtg_fail(top, WTG_E_ARG, "null argument");
+  if (method != WTG_SOLVE_VCYCLE && method != WTG_SOLVE_LEARNED)
+    return wtg_fail(top, WTG_E_ARG, "split steps on a hierarchy take WTG_SOLVE_VCYCLE or WTG_SOLVE_LEARNED, not %d",
+                    method);
+  if (method == WTG_SOLVE_LEARNED && !m->mlp)
+    return wtg_fail(top, WTG_E_STATE, "no network attached (wtg_mg_set_learned)");
+  MgSplit s{m, *p, method == WTG_SOLVE_LEARNED};
+  *iters = 0;
+  return wtg_split_steps(top, slot, nsteps, mg_split_solve, &s, iters);
 }
 
 }  // extern "C"

@@ -1319,11 +1319,19 @@ int baro_correction(wtg_ctx* c, const wtg_ctx::Slot& s) {
   return 0;
 }
 
-int split_impl(wtg_ctx* c, int slot, int nsteps, double tol, int maxit, int* iters) {
+int bicgstab_solve(void* user, wtg_ctx* c, int* iters) {
+  const double* tm = (const double*)user;  // {tol, maxit}
+  double res = 0.0;
+  return wtg_elliptic_solve(c, WTG_SOLVE_BICGSTAB, tm[0], (int)tm[1], iters, &res);
+}
+
+}  // namespace
+
+int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
-  if (nsteps < 0 || maxit < 0) return fail(c, WTG_E_ARG, "negative step or iteration count");
+  if (nsteps < 0) return fail(c, WTG_E_ARG, "negative step count");
   if (int rc = wtg_fs_ready(c)) return rc;
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
@@ -1360,8 +1368,7 @@ int split_impl(wtg_ctx* c, int slot, int nsteps, double tol, int maxit, int* ite
     if (int rc = wtg_rhs_elliptic(c)) return rc;
     HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NC * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
-    double res = 0.0;
-    if (int rc = wtg_elliptic_solve(c, WTG_SOLVE_BICGSTAB, tol, maxit, iters, &res)) return rc;
+    if (int rc = solve(user, c, iters)) return rc;
     if (int rc = baro_correction(c, sol)) return rc;
     hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC, c->info.S,
                        th1 * dt, grav, c->fs[FS_ETA], c->d_len, sol.u);
@@ -1370,8 +1377,6 @@ int split_impl(wtg_ctx* c, int slot, int nsteps, double tol, int maxit, int* ite
   }
   return 0;
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1651,7 +1656,9 @@ int wtg_trend(wtg_ctx* c, int q_slot, int dq_slot) {
 int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter, int* iters) {
   if (!c || !iters) return WTG_E_ARG;
   *iters = 0;
-  if (int rc = split_impl(c, slot, nsteps, rel_tol, max_iter, iters)) return rc;
+  if (max_iter < 0) return fail(c, WTG_E_ARG, "max_iter < 0");
+  double tm[2] = {rel_tol, (double)max_iter};
+  if (int rc = wtg_split_steps(c, slot, nsteps, bicgstab_solve, tm, iters)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
