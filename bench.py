@@ -195,7 +195,7 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
     dof = L * grid.layout.n_dof_per_layer
     return {"dt": dt, "theta1": 0.6, "theta2": 0.8, "rel_tol": 1e-9, "ms_per_step": ms,
             "dof_updates_per_s": dof / (ms / 1e3), "bicgstab_iterations": [int(i) for i in its],
-            "note": "host wall clock over 5 steps; the solver reads one 8-byte residual back per iteration"}
+            "note": "host wall clock over 5 steps; BiCGSTAB queues 8 gated iterations per host convergence test"}
 
 
 def free_surface(fine: int):
@@ -207,6 +207,7 @@ def free_surface(fine: int):
     (median of 3, host-driven including its residual read-backs)."""
     import numpy as np
 
+    import wavetrisk_hydrostatic_amd as W
     from wavetrisk_hydrostatic_amd import SOLVE_FMG, SOLVE_LEARNED, SOLVE_VCYCLE, MgParams
     from wavetrisk_hydrostatic_amd import learned as LC
 
@@ -236,13 +237,25 @@ def free_surface(fine: int):
         it, res, _ = runs[-1]
         return {"iterations": [int(v) for v in it], "rel_resid": float(res[-1]), "ms": ms}
 
+    # the network at config 4's batch: one row per patch for 64 right-hand sides
+    rows = 64 * npatch
+    big = W.MLP(max_batch=rows, seed=1)
+    big.params = mlp.params
+    big.forward(np.random.default_rng(0).standard_normal((rows, W.MLP_IN)).astype(np.float32))
+    ms = big.forward_timed(rows, 20)
+    flops = 2.0 * (192 * 128 + 128 * 128 + 128 * 64) * rows
+    big.close()
+    mlp_line = {"rows": rows, "ms_per_forward": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15}
+
     pv = LC.vcycle_params(1e-9, 40)
     out = {"workload": f"elliptic_lo on J{fine - 2}..J{fine} (finest {10 * 4 ** fine + 2} nodes), dt 1800 s, "
                        "seeded ocean draw 4242, tolerance 1e-9",
            "vcycle": timed(SOLVE_VCYCLE, pv),
            "learned": timed(SOLVE_LEARNED, pv, learned=True),
            "fmg": timed(SOLVE_FMG, MgParams.defaults(coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)),
-           "network": source}
+           "network": source,
+           "mlp_forward_64_rhs": mlp_line}
     prob.close()
     mlp.close()
     return out

@@ -289,6 +289,9 @@ int wtg_mlp_get_params(wtg_mlp* mlp, float* host, size_t n);
 int wtg_mlp_set_params(wtg_mlp* mlp, const float* host, size_t n);  /* also resets the Adam state */
 /* y [batch][OUT] = MLP(x [batch][IN]) (host arrays) */
 int wtg_mlp_forward(wtg_mlp* mlp, const float* x, int batch, float* y);
+/* nrep forward passes of the first `batch` rows already on the device (the
+ * last wtg_mlp_forward's input), timed with HIP events: mean ms per pass */
+int wtg_mlp_forward_timed(wtg_mlp* mlp, int batch, int nrep, double* ms_per_pass);
 /* loss = sum (y - y_true)^2 / (2 batch) and its gradient with respect to the
  * parameters (WTG_MLP_NPARAMS floats), by the backward kernels */
 int wtg_mlp_gradients(wtg_mlp* mlp, const float* x, const float* y_true, int batch, float* loss, float* grads);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "wtg_mlp_get_params": (C.c_int, [_P, _P, C.c_size_t]),
     "wtg_mlp_set_params": (C.c_int, [_P, _P, C.c_size_t]),
     "wtg_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P]),
+    "wtg_mlp_forward_timed": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "wtg_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_float), _P]),
     "wtg_mlp_set_dataset": (C.c_int, [_P, _P, _P, C.c_int64]),
     "wtg_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
@@ -440,6 +441,12 @@ class MLP:
         y = np.empty((x.shape[0], MLP_OUT), dtype=np.float32)
         self._check(_lib.wtg_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)), "wtg_mlp_forward")
         return y
+
+    def forward_timed(self, batch: int, nrep: int = 20) -> float:
+        """mean ms of a forward pass of the first `batch` rows on the device"""
+        ms = C.c_double()
+        self._check(_lib.wtg_mlp_forward_timed(self._h, batch, nrep, C.byref(ms)), "wtg_mlp_forward_timed")
+        return ms.value
 
     def gradients(self, x, y_true):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -30,8 +30,20 @@ constexpr int RED_MAX = 3;        // dot products fused in one pass
 
 // device scalars of the solvers (d_scal)
 enum Scal {
-  SC_RHO = 0, SC_RHO_OLD, SC_ALPH, SC_OMGA, SC_B, SC_T0, SC_T1, SC_T2, SC_T3, SC_ERR, SC_NRMF, SC_ONE, SC_NSCAL
+  SC_RHO = 0, SC_RHO_OLD, SC_ALPH, SC_OMGA, SC_B, SC_T0, SC_T1, SC_T2, SC_T3, SC_ERR, SC_NRMF, SC_ONE,
+  SC_DONE,  // 1 once an iterative solve has converged: gated kernels then do nothing
+  SC_ITER,  // iterations completed by the gated loop
+  SC_TOL,   // its tolerance
+  SC_NSCAL
 };
+
+// kernels of an iteration loop take the solver scalars as a gate and return
+// at once when the loop has converged, so iterations can be queued ahead of
+// the host's convergence test without changing the iterates
+__device__ __forceinline__ bool gated(const double* __restrict__ gate) { return gate && gate[SC_DONE] != 0.0; }
+
+// scalar steps of the solvers (scalar_op below)
+enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF, OP_LOOP_INIT, OP_ERR_BETA, OP_NONE };
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
 
@@ -72,10 +84,11 @@ __device__ __forceinline__ double ell_diag_at(const FsGeo& g, const double* __re
   return th12 * lap - 1.0;
 }
 
-__global__ void fs_apply_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const double* __restrict__ q,
-                                const double* __restrict__ eo, double grav, double coef, double* __restrict__ out) {
+__global__ void fs_apply_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, FsGeo g,
+                                const double* __restrict__ q, const double* __restrict__ eo, double grav, double coef,
+                                double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || gated(gate)) return;
   out[ncl[2 * t]] = ell_apply_at(g, q, eo, grav, coef, ncl[2 * t + 1]);
 }
 
@@ -109,15 +122,22 @@ __global__ void fs_residual_kernel(int n, const int64_t* __restrict__ ncl, FsGeo
 }
 
 // lc (lin_solve.f90:520-561): out = a1 s1 + a2 s2, a = k (i < 0) or k * scal[i]
-__global__ void fs_lc_kernel(int n, const int64_t* __restrict__ ncl, double k1, int i1, const double* __restrict__ s1,
-                             double k2, int i2, const double* __restrict__ s2, const double* __restrict__ scal,
-                             double* __restrict__ out) {
+__global__ void fs_lc_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, double k1, int i1,
+                             const double* __restrict__ s1, double k2, int i2, const double* __restrict__ s2,
+                             const double* __restrict__ scal, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || gated(gate)) return;
   const double a1 = i1 < 0 ? k1 : k1 * scal[i1];
   const double a2 = i2 < 0 ? k2 : k2 * scal[i2];
   const int64_t c = ncl[2 * t + 1];
   out[ncl[2 * t]] = a1 * s1[c] + a2 * s2[c];
+}
+
+// halo copies of a field from its masters (entries nmaster.. of the list)
+__global__ void fs_ghost_copy_kernel(int nmaster, int n, const int64_t* __restrict__ ncl, double* __restrict__ f) {
+  const int t = nmaster + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  f[ncl[2 * t]] = f[ncl[2 * t + 1]];
 }
 
 // out = src (master and halo cells alike)
@@ -166,12 +186,14 @@ __global__ void fs_rhs_kernel(int n, const int64_t* __restrict__ ncl, const doub
 
 // weighted dot products over the reduction cells: part[b*RED_MAX + k] =
 // sum over this block's cells of w * (a_k * b_k); a fixed-order tree
-__global__ __launch_bounds__(FS_NT) void fs_dot_kernel(int nred, const int64_t* __restrict__ rcell,
+__global__ __launch_bounds__(FS_NT) void fs_dot_kernel(const double* __restrict__ gate, int nred,
+                                                       const int64_t* __restrict__ rcell,
                                                        const double* __restrict__ rw, int nd,
                                                        const double* __restrict__ a0, const double* __restrict__ b0,
                                                        const double* __restrict__ a1, const double* __restrict__ b1,
                                                        const double* __restrict__ a2, const double* __restrict__ b2,
                                                        double* __restrict__ part) {
+  if (gated(gate)) return;
   __shared__ double sh[RED_MAX][FS_NT];
   const double* A[RED_MAX] = {a0, a1, a2};
   const double* B[RED_MAX] = {b0, b1, b2};
@@ -193,8 +215,13 @@ __global__ __launch_bounds__(FS_NT) void fs_dot_kernel(int nred, const int64_t* 
 }
 
 // scal[out_k] = sum of the partials of dot k (one block, fixed order)
-__global__ __launch_bounds__(FS_NT) void fs_dot_final_kernel(int nblocks, int nd, const double* __restrict__ part,
-                                                             int o0, int o1, int o2, double* __restrict__ scal) {
+__device__ void scalar_op(int op, int gated_loop, double arg, double* __restrict__ s);
+
+// ... and then the scalar step post_op of the solver (OP_NONE for none)
+__global__ __launch_bounds__(FS_NT) void fs_dot_final_kernel(int gated_loop, int nblocks, int nd,
+                                                             const double* __restrict__ part, int o0, int o1, int o2,
+                                                             double* __restrict__ scal, int post_op) {
+  if (gated_loop && scal[SC_DONE] != 0.0) return;
   __shared__ double sh[RED_MAX][FS_NT];
   double acc[RED_MAX] = {0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nblocks; b += FS_NT)
@@ -209,23 +236,40 @@ __global__ __launch_bounds__(FS_NT) void fs_dot_final_kernel(int nblocks, int nd
   if (threadIdx.x == 0) {
     const int o[RED_MAX] = {o0, o1, o2};
     for (int k = 0; k < nd; ++k) scal[o[k]] = sh[k][0];
+    if (post_op != OP_NONE) scalar_op(post_op, gated_loop, 0.0, scal);
   }
 }
 
 // scalar steps of bicgstab (lin_solve.f90:395-422), in the reference's order
-enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF };
-__global__ void fs_scalar_kernel(int op, double* __restrict__ s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// gated_loop: the op belongs to an iteration of a gated loop (no-op once it
+// has converged; OP_ERR then also counts the iteration and tests it against
+// SC_TOL).  OP_ERR_BETA is OP_ERR followed, if not converged, by OP_BETA.
+__device__ void scalar_op(int op, int gated_loop, double arg, double* __restrict__ s) {
+  if (gated_loop && s[SC_DONE] != 0.0) return;
   switch (op) {
+    case OP_LOOP_INIT:
+      s[SC_DONE] = 0.0;
+      s[SC_ITER] = 0.0;
+      s[SC_TOL] = arg;
+      break;
     case OP_ALPH:  // alph = rho / dp(Ap, res0)
       s[SC_ALPH] = s[SC_RHO] / s[SC_T0];
       break;
     case OP_OMGA:  // omga = dp(As, s) / dp(As, As)
       s[SC_OMGA] = s[SC_T1] / s[SC_T2];
       break;
-    case OP_ERR:  // err = l2(res) / nrm_f
+    case OP_ERR:
+    case OP_ERR_BETA:  // err = l2(res) / nrm_f
       s[SC_ERR] = sqrt(s[SC_T0]) / s[SC_NRMF];
-      break;
+      if (gated_loop) {
+        s[SC_ITER] = s[SC_ITER] + 1.0;
+        if (s[SC_ERR] <= s[SC_TOL]) {
+          s[SC_DONE] = 1.0;
+          break;
+        }
+      }
+      if (op == OP_ERR) break;
+      [[fallthrough]];
     case OP_BETA: {  // rho_old = rho; rho = dp(res0, res); b = (alph/omga) * (rho/rho_old)
       s[SC_RHO_OLD] = s[SC_RHO];
       s[SC_RHO] = s[SC_T3];
@@ -237,7 +281,38 @@ __global__ void fs_scalar_kernel(int op, double* __restrict__ s) {
       s[SC_NRMF] = nf == 0.0 ? 1.0 : nf;
       break;
     }
+    default:
+      break;
   }
+}
+
+__global__ void fs_scalar_kernel(int op, int gated_loop, double arg, double* __restrict__ s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  scalar_op(op, gated_loop, arg, s);
+}
+
+// u = 1 u + 1 (alph p + omga s) on the masters, in place (bicgstab's
+// corr = alph p + omga s; u = u + corr, fused)
+__global__ void fs_bicg_u_kernel(const double* __restrict__ gate, int nmaster, const int64_t* __restrict__ ncl,
+                                 double* __restrict__ u, const double* __restrict__ p, const double* __restrict__ sv,
+                                 const double* __restrict__ scal) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nmaster || gated(gate)) return;
+  const int64_t c = ncl[2 * t + 1];
+  const double corr = (1.0 * scal[SC_ALPH]) * p[c] + (1.0 * scal[SC_OMGA]) * sv[c];
+  u[c] = 1.0 * u[c] + 1.0 * corr;
+}
+
+// p_new = res + b (p - omga Ap) (bicgstab's corr = p - omga Ap; p = res + b corr, fused), out of place
+__global__ void fs_bicg_p_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl,
+                                 const double* __restrict__ res, const double* __restrict__ p,
+                                 const double* __restrict__ Ap, const double* __restrict__ scal,
+                                 double* __restrict__ pn) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || gated(gate)) return;
+  const int64_t c = ncl[2 * t + 1];
+  const double corr = 1.0 * p[c] + (-1.0 * scal[SC_OMGA]) * Ap[c];
+  pn[ncl[2 * t]] = 1.0 * res[c] + (1.0 * scal[SC_B]) * corr;
 }
 
 // ===========================================================================
@@ -472,9 +547,9 @@ int fs_need_params(wtg_ctx* c) {
 
 double fs_coef(const wtg_ctx* c) { return c->fs_theta1 * c->fs_theta2 * (c->fs_dt * c->fs_dt); }
 
-int launch_apply(wtg_ctx* c, const double* q, double* out) {
-  hipLaunchKernelGGL(fs_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), q,
-                     c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out);
+int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false) {
+  hipLaunchKernelGGL(fs_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
+                     c->nncl, c->d_ncl, fs_geo(c), q, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -494,9 +569,10 @@ int launch_jacobi(wtg_ctx* c, const double* u, const double* f, double w, double
   return 0;
 }
 
-int launch_lc(wtg_ctx* c, double k1, int i1, const double* s1, double k2, int i2, const double* s2, double* out) {
-  hipLaunchKernelGGL(fs_lc_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, k1, i1, s1, k2,
-                     i2, s2, c->d_scal, out);
+int launch_lc(wtg_ctx* c, double k1, int i1, const double* s1, double k2, int i2, const double* s2, double* out,
+              bool gate = false) {
+  hipLaunchKernelGGL(fs_lc_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
+                     c->nncl, c->d_ncl, k1, i1, s1, k2, i2, s2, c->d_scal, out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -508,21 +584,28 @@ int launch_copy(wtg_ctx* c, const double* s, double* out) {
 }
 
 // up to three dot products into scal[o0..]
-int launch_dots(wtg_ctx* c, int nd, const double* a0, const double* b0, int o0, const double* a1 = nullptr,
-                const double* b1 = nullptr, int o1 = 0, const double* a2 = nullptr, const double* b2 = nullptr,
-                int o2 = 0) {
+int launch_dots_g(wtg_ctx* c, bool gate, int post_op, int nd, const double* a0, const double* b0, int o0,
+                  const double* a1 = nullptr, const double* b1 = nullptr, int o1 = 0, const double* a2 = nullptr,
+                  const double* b2 = nullptr, int o2 = 0) {
   const int nb = std::min(RED_BLOCKS, (int)nblk(c->nred));
-  hipLaunchKernelGGL(fs_dot_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, c->nred, c->d_rcell, c->d_rw, nd, a0, b0,
-                     a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0, b2 ? b2 : b0, c->d_part);
+  hipLaunchKernelGGL(fs_dot_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr, c->nred,
+                     c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0, b2 ? b2 : b0,
+                     c->d_part);
   WTG_HIPCHK(c, hipGetLastError());
-  hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1), dim3(FS_NT), 0, c->stream, nb, nd, c->d_part, o0, o1, o2,
-                     c->d_scal);
+  hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1), dim3(FS_NT), 0, c->stream, gate ? 1 : 0, nb, nd, c->d_part, o0, o1,
+                     o2, c->d_scal, post_op);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
-int launch_scalar(wtg_ctx* c, int op) {
-  hipLaunchKernelGGL(fs_scalar_kernel, dim3(1), dim3(64), 0, c->stream, op, c->d_scal);
+int launch_dots(wtg_ctx* c, int nd, const double* a0, const double* b0, int o0, const double* a1 = nullptr,
+                const double* b1 = nullptr, int o1 = 0, const double* a2 = nullptr, const double* b2 = nullptr,
+                int o2 = 0) {
+  return launch_dots_g(c, false, OP_NONE, nd, a0, b0, o0, a1, b1, o1, a2, b2, o2);
+}
+
+int launch_scalar(wtg_ctx* c, int op, bool gate = false, double arg = 0.0) {
+  hipLaunchKernelGGL(fs_scalar_kernel, dim3(1), dim3(64), 0, c->stream, op, gate ? 1 : 0, arg, c->d_scal);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -539,46 +622,63 @@ int read_scalar(wtg_ctx* c, int i, double* out) {
     if (int rc_ = (x)) return rc_; \
   } while (0)
 
-// bicgstab (lin_solve.f90:361-431) on u with right-hand side f
+// bicgstab (lin_solve.f90:361-431) on u with right-hand side f.  The
+// iterations are queued in chunks without host synchronisation: every kernel
+// of an iteration is gated by the device flag that the convergence test sets,
+// so the iterations queued after convergence do nothing and the iterates and
+// the iteration count are those of the sequential loop.  u is updated on its
+// masters in place and its halo copies refreshed after the loop.
+constexpr int BICG_CHUNK = 8;
+
 int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int* iters, double* err_out) {
   double** W = c->fs + FS_W0;
-  double *res = W[0], *res0 = W[1], *p = W[2], *s = W[3], *Ap = W[4], *As = W[5], *corr = W[6], *ualt = W[7];
-  double* ucur = u;
+  double *res = W[0], *res0 = W[1], *p = W[2], *s = W[3], *Ap = W[4], *As = W[5], *p_alt = W[6];
   RC(launch_dots(c, 1, f, f, SC_T0));  // nrm_f = l2(f)
   RC(launch_scalar(c, OP_NRMF));
-  RC(launch_residual(c, ucur, f, res));
+  RC(launch_residual(c, u, f, res));
   RC(launch_copy(c, res, res0));
   RC(launch_copy(c, res0, p));
   RC(launch_copy(c, res0, s));
   RC(launch_apply(c, p, Ap));
   RC(launch_copy(c, Ap, As));
   RC(launch_dots(c, 1, res0, res, SC_RHO));
-  int iter = 1;
-  double err = 0.0;
-  for (; iter <= maxit; ++iter) {
-    RC(launch_dots(c, 1, Ap, res0, SC_T0));
-    RC(launch_scalar(c, OP_ALPH));
-    RC(launch_lc(c, 1.0, -1, res, -1.0, SC_ALPH, Ap, s));  // s = res - alph Ap
-    RC(launch_apply(c, s, As));
-    RC(launch_dots(c, 2, As, s, SC_T1, As, As, SC_T2));
-    RC(launch_scalar(c, OP_OMGA));
-    RC(launch_lc(c, 1.0, SC_ALPH, p, 1.0, SC_OMGA, s, corr));  // corr = alph p + omga s
-    RC(launch_lc(c, 1.0, -1, ucur, 1.0, -1, corr, ualt));      // u = u + corr
-    std::swap(ucur, ualt);
-    RC(launch_lc(c, 1.0, -1, s, -1.0, SC_OMGA, As, res));  // res = s - omga As
-    RC(launch_dots(c, 2, res, res, SC_T0, res0, res, SC_T3));
-    RC(launch_scalar(c, OP_ERR));
-    RC(read_scalar(c, SC_ERR, &err));
-    if (err <= tol) break;
-    RC(launch_scalar(c, OP_BETA));
-    RC(launch_lc(c, 1.0, -1, p, -1.0, SC_OMGA, Ap, corr));  // corr = p - omga Ap
-    RC(launch_lc(c, 1.0, -1, res, 1.0, SC_B, corr, p));     // p = res + b corr
-    RC(launch_apply(c, p, Ap));
+  RC(launch_scalar(c, OP_LOOP_INIT, false, tol));
+  int queued = 0;
+  bool done = false;
+  while (queued < maxit && !done) {
+    const int k = std::min(BICG_CHUNK, maxit - queued);
+    for (int i = 0; i < k; ++i) {
+      RC(launch_dots_g(c, true, OP_ALPH, 1, Ap, res0, SC_T0));
+      RC(launch_lc(c, 1.0, -1, res, -1.0, SC_ALPH, Ap, s, true));  // s = res - alph Ap
+      RC(launch_apply(c, s, As, true));
+      RC(launch_dots_g(c, true, OP_OMGA, 2, As, s, SC_T1, As, As, SC_T2));
+      hipLaunchKernelGGL(fs_bicg_u_kernel, dim3(nblk(c->nmaster)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nmaster,
+                         c->d_ncl, u, p, s, c->d_scal);  // u = u + (alph p + omga s)
+      WTG_HIPCHK(c, hipGetLastError());
+      RC(launch_lc(c, 1.0, -1, s, -1.0, SC_OMGA, As, res, true));  // res = s - omga As
+      // err and the convergence test; unless converged, rho and b
+      RC(launch_dots_g(c, true, OP_ERR_BETA, 2, res, res, SC_T0, res0, res, SC_T3));
+      hipLaunchKernelGGL(fs_bicg_p_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
+                         c->d_ncl, res, p, Ap, c->d_scal, p_alt);  // p = res + b (p - omga Ap)
+      WTG_HIPCHK(c, hipGetLastError());
+      std::swap(p, p_alt);
+      RC(launch_apply(c, p, Ap, true));
+    }
+    queued += k;
+    WTG_HIPCHK(c, hipMemcpyAsync(c->h_scal + SC_ERR, c->d_scal + SC_ERR, (SC_TOL - SC_ERR) * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+    done = c->h_scal[SC_DONE] != 0.0;
   }
-  if (ucur != u) RC(launch_copy(c, ucur, u));
+  const int nghost = c->nncl - c->nmaster;
+  if (nghost > 0) {
+    hipLaunchKernelGGL(fs_ghost_copy_kernel, dim3(nblk(nghost)), dim3(FS_NT), 0, c->stream, c->nmaster, c->nncl,
+                       c->d_ncl, u);
+    WTG_HIPCHK(c, hipGetLastError());
+  }
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
-  *iters = iter;
-  *err_out = err;
+  *iters = done ? (int)c->h_scal[SC_ITER] : maxit + 1;
+  *err_out = queued > 0 ? c->h_scal[SC_ERR] : 0.0;
   return 0;
 }
 

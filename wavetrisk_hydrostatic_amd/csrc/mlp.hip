@@ -68,8 +68,8 @@ struct GemmArgs {
 // op(A)(m, k) = TA ? A[k lda + m] : A[m lda + k];  op(B)(k, n) = TB ? B[n ldb + k] : B[k ldb + n]
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
-  __shared__ uint16_t As[BM][LP];  // As[m][k]
-  __shared__ uint16_t Bs[BN][LP];  // Bs[n][k]
+  __shared__ __align__(16) uint16_t As[BM][LP];  // As[m][k]
+  __shared__ __align__(16) uint16_t Bs[BN][LP];  // Bs[n][k]
   const int t = threadIdx.x, lane = t % 64, wave = t / 64;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
         } else {
           for (int i = 0; i < 8; ++i) v[i] = (gm < a.M && k0 + kk + i < a.K) ? a.A[(int64_t)gm * a.lda + k0 + kk + i] : 0;
         }
-        for (int i = 0; i < 8; ++i) As[m][kk + i] = v[i];
+        std::memcpy(&As[m][kk], v, 16);  // one 16-byte LDS store (row pitch 80 B, kk a multiple of 8)
       } else {  // run of 8 consecutive m for one k
         const int kk = t / 8, m = (t % 8) * 8, gk = k0 + kk;
         if (gk < a.K && m0 + m + 8 <= a.M) {
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
         } else {
           for (int i = 0; i < 8; ++i) v[i] = (gn < a.N && k0 + kk + i < a.K) ? a.B[(int64_t)gn * a.ldb + k0 + kk + i] : 0;
         }
-        for (int i = 0; i < 8; ++i) Bs[n][kk + i] = v[i];
+        std::memcpy(&Bs[n][kk], v, 16);
       } else {  // run of 8 consecutive n for one k
         const int kk = t / 8, n = (t % 8) * 8, gk = k0 + kk;
         if (gk < a.K && n0 + n + 8 <= a.N) {
@@ -129,11 +129,9 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
     for (int ks = 0; ks < BK; ks += 16) {
       const int kq = ks + 4 * (lane / 16);
       short4_t af[2], bfr[2];
-      for (int i = 0; i < 2; ++i) {
-        const uint16_t* pa = &As[wm + i * 16 + lane % 16][kq];
-        const uint16_t* pb = &Bs[wn + i * 16 + lane % 16][kq];
-        af[i] = short4_t{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-        bfr[i] = short4_t{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+      for (int i = 0; i < 2; ++i) {  // one 8-byte LDS read per fragment
+        af[i] = *reinterpret_cast<const short4_t*>(&As[wm + i * 16 + lane % 16][kq]);
+        bfr[i] = *reinterpret_cast<const short4_t*>(&Bs[wn + i * 16 + lane % 16][kq]);
       }
       for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
@@ -518,6 +516,32 @@ int wtg_mlp_forward(wtg_mlp* m, const float* x, int batch, float* y) {
   MCHK(m, hipMemcpyAsync(y, m->y, (size_t)batch * NO * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
   return 0;
+}
+
+int wtg_mlp_forward_timed(wtg_mlp* m, int batch, int nrep, double* ms_per_pass) {
+  if (!m || !ms_per_pass) return WTG_E_ARG;
+  if (batch < 1 || nrep < 1) return mfail(m, WTG_E_ARG, "batch and nrep must be positive");
+  const int Bp = (batch + 63) / 64 * 64;
+  if (Bp > m->max_batch) return mfail(m, WTG_E_ARG, "batch %d exceeds the maximum %d", batch, m->max_batch);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return mfail(m, WTG_E_DEVICE, "hipEventCreate failed");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return mfail(m, WTG_E_DEVICE, "hipEventCreate failed");
+  }
+  int rc = 0;
+  if (hipEventRecord(e0, m->stream) != hipSuccess) rc = mfail(m, WTG_E_DEVICE, "hipEventRecord failed");
+  for (int i = 0; i < nrep && !rc; ++i) rc = forward(m, batch);
+  if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = mfail(m, WTG_E_DEVICE, "event synchronisation failed");
+  if (!rc) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_pass = ms / nrep;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
 }
 
 int wtg_mlp_gradients(wtg_mlp* m, const float* x, const float* y_true, int batch, float* loss, float* grads) {
