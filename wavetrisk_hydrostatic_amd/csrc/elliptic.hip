@@ -34,6 +34,7 @@ enum Scal {
   SC_DONE,  // 1 once an iterative solve has converged: gated kernels then do nothing
   SC_ITER,  // iterations completed by the gated loop
   SC_TOL,   // its tolerance
+  SC_MR,    // minimal-residual step of the learned correction
   SC_NSCAL
 };
 
@@ -43,7 +44,7 @@ enum Scal {
 __device__ __forceinline__ bool gated(const double* __restrict__ gate) { return gate && gate[SC_DONE] != 0.0; }
 
 // scalar steps of the solvers (scalar_op below)
-enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF, OP_LOOP_INIT, OP_ERR_BETA, OP_NONE };
+enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF, OP_LOOP_INIT, OP_ERR_BETA, OP_MRSTEP, OP_NONE };
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
 
@@ -274,6 +275,11 @@ __device__ void scalar_op(int op, int gated_loop, double arg, double* __restrict
       s[SC_RHO_OLD] = s[SC_RHO];
       s[SC_RHO] = s[SC_T3];
       s[SC_B] = (s[SC_ALPH] / s[SC_OMGA]) * (s[SC_RHO] / s[SC_RHO_OLD]);
+      break;
+    }
+    case OP_MRSTEP: {  // a = <r, L e> / <L e, L e> if that is positive and finite, else 0
+      const double a = s[SC_T1] > 0.0 ? s[SC_T0] / s[SC_T1] : 0.0;
+      s[SC_MR] = (a > 0.0 && isfinite(a)) ? a : 0.0;
       break;
     }
     case OP_NRMF: {  // nrm_f = l2(f), 1 if zero (FMG / SRJ, lin_solve.f90:73, 177)
@@ -987,15 +993,11 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
       RC(launch_residual(c, V[l].u, V[l].f, r));
       RC(lc_correction(m, r, tmp, e));
       RC(launch_apply(c, e, w));
+      // the step is computed and applied on the device (a = 0 leaves u unchanged)
       RC(launch_dots(c, 2, r, w, SC_T0, w, w, SC_T1));
-      double rw, ww;
-      RC(read_scalar(c, SC_T0, &rw));
-      RC(read_scalar(c, SC_T1, &ww));
-      const double a = ww > 0.0 ? rw / ww : 0.0;
-      if (a > 0.0 && std::isfinite(a)) {
-        RC(launch_lc(c, 1.0, -1, V[l].u, a, -1, e, V[l].u_alt));
-        std::swap(V[l].u, V[l].u_alt);
-      }
+      RC(launch_scalar(c, OP_MRSTEP));
+      RC(launch_lc(c, 1.0, -1, V[l].u, 1.0, SC_MR, e, V[l].u_alt));
+      std::swap(V[l].u, V[l].u_alt);
     }
     RC(vcycle(m, V, l, P, nrm_f, &err));
     iters[l] = it;
