@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=30, help="layers of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-free-surface", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--members", type=int, default=8, help="ensemble members of the batched-ensemble line")
     ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -122,6 +124,8 @@ def main():
                "sample": f"CPU oracle (C, -O2, 1 thread), 1 RK4 step of the same J{J} grid with {Lc} layers "
                          f"({secs:.1f} s)"}
 
+    ens = None if args.no_ensemble else ensemble(grid, topo, mass, temp, velo, L, args.dt, args.members)
+    ctx.synchronize()
     split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
     fs = None if args.no_free_surface else free_surface(args.fs_level)
 
@@ -156,6 +160,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "split_step": split,
         "free_surface": fs,
     }
@@ -173,6 +178,32 @@ def measured_traffic(J: int, L: int):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10):
+    """The batched ensemble (every layer kernel launched once for all
+    members): members copies of the J8 L30 state, each with its own seeded
+    bathymetry perturbation (+-50 m), FP64 RK4 steps; DOF updates/s of the
+    whole ensemble."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+
+    print(f"bench: ensemble of {members} members", file=sys.stderr, flush=True)
+    ctx = W.Context(grid, nlayers=L, members=members, nslots=1)
+    rng = np.random.default_rng(7)
+    topos = np.stack([topo + 50.0 * np.tanh(rng.standard_normal()) * np.cos(np.arange(topo.size) * 1e-4 * (m + 1))
+                      for m in range(members)])
+    ctx.set_topography(topos)
+    ctx.upload_state(0, np.tile(mass, (members, 1)), np.tile(temp, (members, 1)), np.tile(velo, (members, 1, 1)))
+    ctx.rk4(0, dt, 2)
+    ctx.synchronize()
+    total_ms, kernel_ms = ctx.rk4_timed(0, dt, nsteps)
+    ctx.close()
+    dof = members * L * grid.layout.n_dof_per_layer
+    return {"members": members, "ms_per_step": total_ms / nsteps, "dof_updates_per_s": dof * nsteps / (total_ms / 1e3),
+            "stage_kernel_ms": kernel_ms,
+            "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
 
 
 def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):

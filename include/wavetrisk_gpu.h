@@ -94,10 +94,19 @@ typedef struct wtg_ctx wtg_ctx;
  * context; calls are stream ordered and the host functions below
  * synchronise before returning. */
 int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_ctx** out);
+/* An ensemble of nmembers independent states on the same grid (the batched
+ * ensemble of config 5): every slot holds the members one after another,
+ * scalars [member][layer][cell], velocity [member][layer][edge][cell], and
+ * every layer kernel runs all members in one launch (members on the grid's
+ * second dimension); each member has its own topography.  wtg_trend,
+ * wtg_rk4 and the ghost fill take the whole ensemble; the free-surface calls
+ * need nmembers = 1. */
+int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmembers, int nslots, wtg_ctx** out);
 void wtg_destroy(wtg_ctx* ctx);
 
-/* Bottom height z_s per cell [ncells] (topography, m); halo cells may be
- * anything, they are filled from their owners. */
+/* Bottom height z_s per cell [ncells] (topography, m), [nmembers][ncells]
+ * for an ensemble; halo cells may be anything, they are filled from their
+ * owners. */
 int wtg_set_topography(wtg_ctx* ctx, const double* topo);
 
 /* Host <-> device copies of a state slot (layouts above).  Halo slots whose

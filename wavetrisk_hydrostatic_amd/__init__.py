@@ -64,6 +64,7 @@ SIGNATURES = {
     "wtg_grid_destroy": (None, [_P]),
     "wtg_grid_overlap_areas": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "wtg_create": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.POINTER(_P)]),
+    "wtg_create_ensemble": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.c_int, C.POINTER(_P)]),
     "wtg_destroy": (None, [_P]),
     "wtg_set_topography": (C.c_int, [_P, _P]),
     "wtg_upload_state": (C.c_int, [_P, C.c_int, _P, _P, _P]),
@@ -220,14 +221,18 @@ class Context:
     """Device state of one grid level on the current HIP device."""
 
     def __init__(self, grid: Grid, nlayers: int, grav: float = 9.80616, ref_density: float = 1028.0,
-                 p_top: float = 0.0, nslots: int = 2):
+                 p_top: float = 0.0, nslots: int = 2, members: int = 1):
+        """members > 1: an ensemble of independent states (wtg_create_ensemble);
+        state arrays then have members * nlayers layers, member-major."""
         lib = load_library()
         self.grid = grid
         self.layout = grid.layout
         self.nlayers = nlayers
+        self.members = members
         self.params = _Params(nlayers, grav, ref_density, p_top)
         h = _P()
-        _check(lib.wtg_create(grid.handle, C.byref(self.params), nslots, C.byref(h)), None, "wtg_create")
+        _check(lib.wtg_create_ensemble(grid.handle, C.byref(self.params), members, nslots, C.byref(h)), None,
+               "wtg_create_ensemble")
         self._h = h
 
     def close(self):
@@ -239,12 +244,15 @@ class Context:
         self.close()
 
     def _shape(self):
-        L, nc = self.nlayers, self.layout.ncells
+        L, nc = self.nlayers * self.members, self.layout.ncells
         return (L, nc), (L, 3, nc)
 
     def set_topography(self, topo: np.ndarray):
+        """[ncells], or [members, ncells] for an ensemble (one array is repeated)"""
         topo = np.ascontiguousarray(topo, dtype=np.float64)
-        assert topo.shape == (self.layout.ncells,)
+        if self.members > 1 and topo.shape == (self.layout.ncells,):
+            topo = np.ascontiguousarray(np.broadcast_to(topo, (self.members, self.layout.ncells)))
+        assert topo.shape in ((self.layout.ncells,), (self.members, self.layout.ncells))
         _check(_lib.wtg_set_topography(self._h, _ptr(topo)), self._h, "wtg_set_topography")
 
     def upload_state(self, slot: int, mass, temp, velo):

@@ -40,6 +40,7 @@ struct wtg_ctx {
   wtg_grid_info info{};
   wtg_params params{};
   int nslots = 0;
+  int nmembers = 1;       // ensemble members, each nlayers layers, one after another in every slot
   int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
   bool topo_set = false;
   std::string err;

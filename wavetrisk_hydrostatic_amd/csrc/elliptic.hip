@@ -1127,6 +1127,7 @@ int wtg_fs_dot(wtg_ctx* c, int a_field, int b_field, double* out) {
 
 int wtg_flux_divergence(wtg_ctx* c, int slot) {
   RC(fs_ready(c));
+  if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "the free surface is built for single-member contexts");
   if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
   const auto& s = c->slots[slot];
   hipLaunchKernelGGL(fs_flux_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),

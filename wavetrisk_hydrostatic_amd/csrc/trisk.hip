@@ -66,6 +66,18 @@ struct StageArgs {
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
 
+// ensemble member blockIdx.y: its layers follow the previous member's in every
+// state plane (scalars [member][layer][cell], velocity [member][layer][edge][cell])
+__device__ __forceinline__ StageArgs member_args(const StageArgs& a0) {
+  StageArgs a = a0;
+  const int64_t s1 = (int64_t)blockIdx.y * a0.L * a0.NC, s3 = 3 * s1;
+  a.ym += s1; a.yt += s1; a.yu += s3;
+  a.qm += s1; a.qt += s1; a.qu += s3;
+  a.om += s1; a.ot += s1; a.ou += s3;
+  a.g.topo += (int64_t)blockIdx.y * a0.NC;  // each member has its own bathymetry
+  return a;
+}
+
 // LDS working set of one block (15 planes of the 19 x 19 region)
 struct Smem {
   double m[RR], t[RR], u[3][RR];
@@ -89,7 +101,8 @@ __device__ __forceinline__ void weights(const Geo& g, int64_t NC, int64_t c, int
   w[4] = 0.5 - acc4 * hi;
 }
 
-__global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a_in) {
+  const StageArgs a = member_args(a_in);
   __shared__ Smem sm;
   const int tid = threadIdx.x;
   const int64_t NC = a.NC;
@@ -507,7 +520,8 @@ __device__ __forceinline__ void weights_from(double a1, double a2, double a3, do
   w[4] = 0.5 - acc4 * hi;
 }
 
-__global__ __launch_bounds__(THREADS2, 2) void trisk_stage_kernel2(StageArgs a) {
+__global__ __launch_bounds__(THREADS2, 2) void trisk_stage_kernel2(StageArgs a_in) {
+  const StageArgs a = member_args(a_in);
   __shared__ Smem2 sm;
   const int tid = threadIdx.x;
   const int64_t NC = a.NC;
@@ -775,7 +789,8 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
   w[4] = 0.5 - acc4 * hi;
 }
 
-__global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a) {
+__global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_in) {
+  const StageArgs a = member_args(a_in);
   __shared__ GeoS gs;
   __shared__ union {
     LayerS ls[2];
@@ -1222,7 +1237,7 @@ int check_slot(wtg_ctx* c, int slot) {
 }
 
 int launch_ghost_fill(wtg_ctx* c, const wtg_ctx::Slot& s) {
-  const int L = c->params.nlayers;
+  const int L = c->params.nlayers * c->nmembers;
   int64_t total = (int64_t)(c->nn + c->ne) * L;
   if (total == 0) return 0;
   int threads = 256;
@@ -1234,7 +1249,7 @@ int launch_ghost_fill(wtg_ctx* c, const wtg_ctx::Slot& s) {
 }
 
 int launch_copy_frozen(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to) {
-  const int L = c->params.nlayers;
+  const int L = c->params.nlayers * c->nmembers;
   int64_t total = (int64_t)(c->nfn + c->nfe) * L;
   if (total == 0) return 0;
   int threads = 256;
@@ -1271,12 +1286,13 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
   a.rho0 = c->params.ref_density;
   a.p_top = c->params.p_top;
   unsigned blocks = (unsigned)(10 * a.ntile * a.ntile);
+  const dim3 grid(blocks, (unsigned)c->nmembers);
   if (c->stage_version == 1)
-    hipLaunchKernelGGL(trisk_stage_kernel, dim3(blocks), dim3(THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(trisk_stage_kernel, grid, dim3(THREADS), 0, c->stream, a);
   else if (c->stage_version == 2)
-    hipLaunchKernelGGL(trisk_stage_kernel2, dim3(blocks), dim3(THREADS2), 0, c->stream, a);
+    hipLaunchKernelGGL(trisk_stage_kernel2, grid, dim3(THREADS2), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(trisk_stage_pair_kernel, dim3(blocks), dim3(THREADS3), 0, c->stream, a);
+    hipLaunchKernelGGL(trisk_stage_pair_kernel, grid, dim3(THREADS3), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1332,6 +1348,7 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "negative step count");
+  if (c->nmembers != 1) return fail(c, WTG_E_ARG, "the free surface is built for single-member contexts");
   if (int rc = wtg_fs_ready(c)) return rc;
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
@@ -1452,9 +1469,14 @@ int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double*
 void wtg_grid_destroy(wtg_grid* grid) { delete grid; }
 
 int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_ctx** out) {
+  return wtg_create_ensemble(grid, params, 1, nslots, out);
+}
+
+int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmembers, int nslots, wtg_ctx** out) {
   if (!grid || !params || !out) return fail(nullptr, WTG_E_ARG, "null argument");
   *out = nullptr;
   if (nslots < 1 || nslots > 64) return fail(nullptr, WTG_E_ARG, "nslots %d out of range [1,64]", nslots);
+  if (nmembers < 1 || nmembers > 1024) return fail(nullptr, WTG_E_ARG, "nmembers %d out of range [1,1024]", nmembers);
   if (params->nlayers < 1) return fail(nullptr, WTG_E_ARG, "nlayers %d < 1", params->nlayers);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, WTG_E_DEVICE, "no HIP device");
@@ -1465,6 +1487,7 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
   wtg_grid_info_get(grid, &c->info);
   c->params = *params;
   c->nslots = nslots;
+  c->nmembers = nmembers;
   if (const char* v = std::getenv("WTG_STAGE_KERNEL")) {
     int sv = std::atoi(v);
     c->stage_version = (sv == 1 || sv == 2) ? sv : 3;
@@ -1555,12 +1578,12 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
     wtg_destroy(c.release());
     return rc;
   }
-  if ((rc = dalloc<double>(cp, &c->d_topo, NC))) {
+  if ((rc = dalloc<double>(cp, &c->d_topo, (size_t)nmembers * NC))) {
     g_create_error = c->err;
     wtg_destroy(c.release());
     return rc;
   }
-  const int L = params->nlayers;
+  const int L = params->nlayers * c->nmembers;  // layers of all ensemble members
   c->slots.resize(nslots + 2);
   for (auto& s : c->slots) {
     if ((rc = dalloc<double>(cp, &s.m, (size_t)L * NC)) || (rc = dalloc<double>(cp, &s.t, (size_t)L * NC)) ||
@@ -1593,13 +1616,14 @@ void wtg_destroy(wtg_ctx* c) {
 
 int wtg_set_topography(wtg_ctx* c, const double* topo) {
   if (!c || !topo) return WTG_E_ARG;
-  const int64_t NC = c->info.ncells;
-  std::vector<double> t(topo, topo + NC);
-  for (int64_t cc = 0; cc < NC; ++cc) {
-    int64_t s = c->node_src[cc];
-    if (s >= 0) t[cc] = topo[s];
-  }
-  HIPCHK(c, hipMemcpyAsync(c->d_topo, t.data(), NC * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int64_t NC = c->info.ncells, n = (int64_t)c->nmembers * NC;
+  std::vector<double> t(topo, topo + n);
+  for (int64_t m = 0; m < n; m += NC)
+    for (int64_t cc = 0; cc < NC; ++cc) {
+      int64_t s = c->node_src[cc];
+      if (s >= 0) t[m + cc] = topo[m + s];
+    }
+  HIPCHK(c, hipMemcpyAsync(c->d_topo, t.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->topo_set = true;
   return 0;
@@ -1608,7 +1632,7 @@ int wtg_set_topography(wtg_ctx* c, const double* topo) {
 int wtg_upload_state(wtg_ctx* c, int slot, const double* mass, const double* temp, const double* velo) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!mass || !temp || !velo) return fail(c, WTG_E_ARG, "null state array");
-  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const size_t ns = (size_t)c->params.nlayers * c->nmembers * c->info.ncells;
   const auto& s = c->slots[slot];
   HIPCHK(c, hipMemcpyAsync(s.m, mass, ns * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.t, temp, ns * 8, hipMemcpyHostToDevice, c->stream));
@@ -1621,7 +1645,7 @@ int wtg_upload_state(wtg_ctx* c, int slot, const double* mass, const double* tem
 int wtg_download_state(wtg_ctx* c, int slot, double* mass, double* temp, double* velo) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!mass || !temp || !velo) return fail(c, WTG_E_ARG, "null state array");
-  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const size_t ns = (size_t)c->params.nlayers * c->nmembers * c->info.ncells;
   const auto& s = c->slots[slot];
   HIPCHK(c, hipMemcpyAsync(mass, s.m, ns * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(temp, s.t, ns * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1642,7 +1666,7 @@ int wtg_trend(wtg_ctx* c, int q_slot, int dq_slot) {
   if (int rc = check_slot(c, dq_slot)) return rc;
   if (q_slot == dq_slot) return fail(c, WTG_E_ARG, "q_slot and dq_slot must differ");
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before wtg_trend");
-  const size_t ns = (size_t)c->params.nlayers * c->info.ncells;
+  const size_t ns = (size_t)c->params.nlayers * c->nmembers * c->info.ncells;
   const auto& dq = c->slots[dq_slot];
   HIPCHK(c, hipMemsetAsync(dq.m, 0, ns * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(dq.t, 0, ns * 8, c->stream));
