@@ -124,6 +124,8 @@ def main():
                "sample": f"CPU oracle (C, -O2, 1 thread), 1 RK4 step of the same J{J} grid with {Lc} layers "
                          f"({secs:.1f} s)"}
 
+    bulk = fp32_bulk(grid, topo, mass, temp, velo, L, args.dt, args.warmup, args.steps, m_out, u_out,
+                     dof_per_step, ms_per_step)
     ens = None if args.no_ensemble else ensemble(grid, topo, mass, temp, velo, L, args.dt, args.members)
     ctx.synchronize()
     split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
@@ -160,6 +162,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        "fp32_bulk": bulk,
         "ensemble": ens,
         "split_step": split,
         "free_surface": fs,
@@ -178,6 +181,43 @@ def measured_traffic(J: int, L: int):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def fp32_bulk(grid, topo, mass, temp, velo, L, dt, warmup, steps, m64, u64, dof_per_step, ms64):
+    """The same J8 L30 RK4 steps with the FP32-bulk stencils
+    (PRECISION_FP32_BULK): ms per step, DOF updates/s, mean stage-kernel
+    launch, and the drift of the state from the FP64 run after the same
+    warmup + timed steps (max-norm errors; surface height relative to the
+    column depth)."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+
+    print("bench: FP32-bulk steps", file=sys.stderr, flush=True)
+    ctx = W.Context(grid, nlayers=L, nslots=1)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, mass, temp, velo)
+    ctx.set_precision(W.PRECISION_FP32_BULK)
+    if warmup:
+        ctx.rk4(0, dt, warmup)
+    ctx.synchronize()
+    total_ms, kernel_ms = ctx.rk4_timed(0, dt, steps)
+    m, _, u = ctx.download_state(0)
+    ctx.close()
+    lay = grid.layout
+    own, own_e = lay.owned_cells(), lay.owned_edge_cells()
+    depth = m64[:, own].sum(0) / 1028.0
+    eta_err = np.max(np.abs(m[:, own].sum(0) - m64[:, own].sum(0))) / 1028.0
+    ms = total_ms / steps
+    return {"ms_per_step": ms, "dof_updates_per_s": dof_per_step / (ms / 1e3), "speedup_vs_fp64": ms64 / ms,
+            "stage_kernel_ms": kernel_ms,
+            "drift_vs_fp64": {"steps": warmup + steps,
+                              "surface_height_rel_depth": eta_err / float(np.max(depth)),
+                              "mass_rel": float(np.max(np.abs(m[:, own] - m64[:, own])) / np.max(np.abs(m64[:, own]))),
+                              "velo_rel": float(np.max(np.abs(u[:, :, own_e] - u64[:, :, own_e])) /
+                                                np.max(np.abs(u64[:, :, own_e])))},
+            "note": "PV, fluxes, kinetic energy, divergence, Qperp in FP32; column scan, Bernoulli/Exner "
+                    "gradients and state updates FP64; RK stages as FP32 increments over the FP64 state"}
 
 
 def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10):
@@ -199,10 +239,16 @@ def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10):
     ctx.rk4(0, dt, 2)
     ctx.synchronize()
     total_ms, kernel_ms = ctx.rk4_timed(0, dt, nsteps)
+    ctx.set_precision(W.PRECISION_FP32_BULK)
+    ctx.rk4(0, dt, 2)
+    ctx.synchronize()
+    total_b, kernel_b = ctx.rk4_timed(0, dt, nsteps)
     ctx.close()
     dof = members * L * grid.layout.n_dof_per_layer
     return {"members": members, "ms_per_step": total_ms / nsteps, "dof_updates_per_s": dof * nsteps / (total_ms / 1e3),
             "stage_kernel_ms": kernel_ms,
+            "fp32_bulk": {"ms_per_step": total_b / nsteps, "dof_updates_per_s": dof * nsteps / (total_b / 1e3),
+                          "stage_kernel_ms": kernel_b},
             "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
 
 

@@ -133,6 +133,24 @@ int wtg_rk4(wtg_ctx* ctx, int slot, double dt, int nsteps);
  * the fused trend+update kernel (the dominant kernel). */
 int wtg_rk4_timed(wtg_ctx* ctx, int slot, double dt, int nsteps, double* total_ms, double* stage_kernel_ms);
 
+/* Precision of wtg_rk4 and wtg_rk4_timed on this context (no counterpart in
+ * the reference, which is FP64 throughout):
+ *   WTG_PRECISION_FP64      the FP64 path, bit-exact with trend_ml / RK4
+ *                           (time_integr.f90:78-103, multi_level.f90:5-47);
+ *   WTG_PRECISION_FP32_BULK the same step with potential vorticity, fluxes,
+ *                           kinetic energy, divergence and Qperp in FP32 and
+ *                           the intermediate RK stages kept as FP32
+ *                           increments over the FP64 state; the state, the
+ *                           hydrostatic column scan, Bernoulli / Exner
+ *                           gradients and the update of the state stay FP64.
+ *                           Its error against the FP64 path is bounded by the
+ *                           tests, not bit-exact.
+ * The first switch to FP32_BULK allocates two FP32 increment buffers and FP32
+ * geometry copies.  Other entry points (wtg_trend, the split step) stay FP64. */
+#define WTG_PRECISION_FP64 0
+#define WTG_PRECISION_FP32_BULK 1
+int wtg_set_precision(wtg_ctx* ctx, int precision);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

@@ -1,0 +1,131 @@
+"""FP32-bulk RK4 (WTG_PRECISION_FP32_BULK) against the FP64 oracle (needs a GPU).
+
+The FP32-bulk step is not bit-exact by design: potential vorticity, fluxes,
+kinetic energy, divergence and Qperp run in FP32 and the intermediate RK
+stages are FP32 increments over the FP64 state.  Its bars, as max-norm
+errors against the oracle's FP64 result on the same inputs:
+
+* one RK4 step (J5): mass and temp within 1e-7 relative, velocity within
+  1e-6 relative (measured on MI355X: 5e-9 and 3e-8);
+* 100 RK4 steps (J5 ocean L4 at dt 600 s, J5 shallow water TC5 at dt 300 s):
+  surface height within 1e-6 of the column depth, mass within 1e-6 and
+  velocity within 1e-5 relative (measured: 4e-8, 4e-8, 5e-7);
+* J8 L30, 100 steps against the FP64 GPU path (which is bit-exact with the
+  oracle): the same surface-height bar (measured: 6e-11).
+A dropped or mis-signed term in any FP32 stencil moves these errors to
+1e-3 or more.  Ensembles in FP32-bulk mode equal single-member runs bit for
+bit, and switching back to FP64 gives the bit-exact FP64 path again.
+"""
+import numpy as np
+import pytest
+
+import wavetrisk_hydrostatic_amd as W
+from wavetrisk_hydrostatic_amd import cases
+from oracle.oracle import Oracle, geometry_of
+
+pytestmark = pytest.mark.gpu
+
+RHO0 = 1028.0
+
+
+def _errors(g, got, want, topo):
+    own, own_e = g.layout.owned_cells(), g.layout.owned_edge_cells()
+    m, t, u = got
+    M, T, U = want
+    r = {
+        "mass": np.max(np.abs(m[:, own] - M[:, own])) / np.max(np.abs(M[:, own])),
+        "velo": np.max(np.abs(u[:, :, own_e] - U[:, :, own_e])) / np.max(np.abs(U[:, :, own_e])),
+    }
+    if np.any(T[:, own]):
+        r["temp"] = np.max(np.abs(t[:, own] - T[:, own])) / np.max(np.abs(T[:, own]))
+    eta = m[:, own].sum(0) / RHO0 + topo[own]
+    ETA = M[:, own].sum(0) / RHO0 + topo[own]
+    r["eta/depth"] = np.max(np.abs(eta - ETA)) / np.max(M[:, own].sum(0) / RHO0)
+    return r
+
+
+def _case(name, g):
+    if name == "ocean":
+        topo, m, t, u = cases.ocean(g, 4, seed=1)
+        return 4, 600.0, topo, m, t, u
+    topo, m, t, u = cases.tc5(g, 1)
+    return 1, 300.0, topo, m, t, u
+
+
+def _run(g, L, dt, topo, m, t, u, nsteps, precision, members=1):
+    ctx = W.Context(g, nlayers=L, members=members)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.set_precision(precision)
+    ctx.rk4(0, dt, nsteps)
+    return ctx.download_state(0)
+
+
+@pytest.mark.parametrize("name", ["ocean", "tc5"])
+def test_fp32_bulk_against_oracle(name):
+    print(f"FP32-bulk vs oracle: {name} J5", flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, dt, topo, m, t, u = _case(name, g)
+    o = Oracle(g.layout, geometry_of(g), topo)
+    m0, t0, u0 = m.copy(), t.copy(), u.copy()
+    o.ghost_fill(m0, t0, u0)
+    for nsteps, bars in ((1, {"mass": 1e-7, "temp": 1e-7, "velo": 1e-6, "eta/depth": 1e-7}),
+                         (100, {"mass": 1e-6, "temp": 1e-6, "velo": 1e-5, "eta/depth": 1e-6})):
+        want = o.rk4(m0.copy(), t0.copy(), u0.copy(), dt, nsteps)
+        got = _run(g, L, dt, topo, m, t, u, nsteps, W.PRECISION_FP32_BULK)
+        err = _errors(g, got, want, topo)
+        print(f"  {nsteps} steps: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
+        for k, v in err.items():
+            assert v <= bars[k], f"{name}, {nsteps} steps: {k} error {v:.3e} above {bars[k]:.0e}"
+        # the mode is in effect: FP32 stencils do not reproduce FP64 bits
+        assert err["mass"] > 0 or err["velo"] > 0
+
+
+def test_fp32_bulk_full_size_J8_L30():
+    print("FP32-bulk vs FP64 path: J8 L30, 100 steps", flush=True)
+    g = W.Grid(level=8, min_level=8, domain_level=2)
+    L = 30
+    topo, m, t, u = cases.ocean(g, L, seed=3)
+    ref = _run(g, L, 66.7, topo, m, t, u, 100, W.PRECISION_FP64)
+    got = _run(g, L, 66.7, topo, m, t, u, 100, W.PRECISION_FP32_BULK)
+    err = _errors(g, got, ref, topo)
+    print("  " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
+    assert err["eta/depth"] <= 1e-6
+    assert err["mass"] <= 1e-6 and err["velo"] <= 1e-5
+
+
+def test_fp32_bulk_ensemble_equals_single_runs_and_fp64_switch_back():
+    print("FP32-bulk ensemble J5 L2 x 2", flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, M, dt = 2, 2, 600.0
+    draws = [cases.ocean(g, L, seed=20 + k) for k in range(M)]
+    topo = np.stack([d[0] for d in draws])
+    state = [np.concatenate([d[k] for d in draws]) for k in (1, 2, 3)]
+    em, et, eu = _run(g, L, dt, topo, *state, 3, W.PRECISION_FP32_BULK, members=M)
+    own, own_e = g.layout.owned_cells(), g.layout.owned_edge_cells()
+    for k, (tp, mass, temp, velo) in enumerate(draws):
+        sm, st, su = _run(g, L, dt, tp, mass, temp, velo, 3, W.PRECISION_FP32_BULK)
+        sl = slice(k * L, (k + 1) * L)
+        assert np.array_equal(em[sl][:, own], sm[:, own])
+        assert np.array_equal(et[sl][:, own], st[:, own])
+        assert np.array_equal(eu[sl][:, :, own_e], su[:, :, own_e])
+    # FP32-bulk, then back to FP64 on the same context: the FP64 steps match
+    # the oracle bit for bit from the state the FP32-bulk step left
+    tp, mass, temp, velo = draws[0]
+    ctx = W.Context(g, nlayers=L)
+    ctx.set_topography(tp)
+    ctx.upload_state(0, mass, temp, velo)
+    ctx.set_precision(W.PRECISION_FP32_BULK)
+    ctx.rk4(0, dt, 1)
+    mid = ctx.download_state(0)
+    ctx.set_precision(W.PRECISION_FP64)
+    ctx.rk4(0, dt, 2)
+    got = ctx.download_state(0)
+    o = Oracle(g.layout, geometry_of(g), tp)
+    mm, tt, uu = (a.copy() for a in mid)
+    o.ghost_fill(mm, tt, uu)
+    wm, _, wu = o.rk4(mm, tt, uu, dt, 2)
+    assert np.array_equal(got[0][:, own], wm[:, own])
+    assert np.array_equal(got[2][:, :, own_e], wu[:, :, own_e])
+    with pytest.raises(W.WtgError, match="precision"):
+        ctx.set_precision(7)

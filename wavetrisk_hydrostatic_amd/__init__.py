@@ -24,7 +24,7 @@ from .layout import Layout
 
 __all__ = ["Grid", "Context", "Hierarchy", "MgParams", "MLP", "Layout", "WtgError", "library_path", "load_library",
            "FS_ETA", "FS_RHS", "FS_ETA_OLD", "FS_DIVF", "FS_DIVF_OLD", "SOLVE_BICGSTAB", "SOLVE_JACOBI", "SOLVE_FMG",
-           "SOLVE_SRJ", "SOLVE_VCYCLE", "SOLVE_LEARNED"]
+           "SOLVE_SRJ", "SOLVE_VCYCLE", "SOLVE_LEARNED", "PRECISION_FP64", "PRECISION_FP32_BULK"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libwavetrisk_gpu.so"
@@ -73,6 +73,7 @@ SIGNATURES = {
     "wtg_trend": (C.c_int, [_P, C.c_int, C.c_int]),
     "wtg_rk4": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "wtg_rk4_timed": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _D, _D]),
+    "wtg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
     "wtg_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
@@ -112,6 +113,9 @@ SIGNATURES = {
 # free-surface fields and solver methods (include/wavetrisk_gpu.h)
 FS_ETA, FS_RHS, FS_ETA_OLD, FS_DIVF, FS_DIVF_OLD = 0, 1, 2, 3, 4
 SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED = 0, 1, 2, 3, 4, 5
+# precision of Context.rk4 / rk4_timed: FP64 (bit-exact with the reference's
+# RK4 of trend_ml) or FP32 bulk stencils with FP64 column scan and updates
+PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
 
 
 class _FsParams(C.Structure):
@@ -277,6 +281,10 @@ class Context:
 
     def rk4(self, slot: int, dt: float, nsteps: int = 1):
         _check(_lib.wtg_rk4(self._h, slot, dt, nsteps), self._h, "wtg_rk4")
+
+    def set_precision(self, precision: int):
+        """PRECISION_FP64 or PRECISION_FP32_BULK for rk4 / rk4_timed."""
+        _check(_lib.wtg_set_precision(self._h, precision), self._h, "wtg_set_precision")
 
     def rk4_timed(self, slot: int, dt: float, nsteps: int):
         tot, ker = C.c_double(), C.c_double()

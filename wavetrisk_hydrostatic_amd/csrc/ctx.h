@@ -56,6 +56,16 @@ struct wtg_ctx {
     double *m = nullptr, *t = nullptr, *u = nullptr;
   };
   std::vector<Slot> slots;
+  // precision of wtg_rk4 (WTG_PRECISION_*); the FP32-bulk mode's FP32
+  // geometry copies and two FP32 stage-increment buffers are allocated on first use
+  int precision = 0;
+  bool f32_done = false;
+  float *f_len = nullptr, *f_lp = nullptr, *f_pedf = nullptr, *f_area = nullptr, *f_hinv = nullptr;
+  float* f_cor = nullptr;
+  struct SlotF {
+    float *m = nullptr, *t = nullptr, *u = nullptr;
+  };
+  SlotF fstage[2];
   // free-surface problem
   double* d_lenf = nullptr;      // primal lengths of the flux (owner's for halo edges)
   int64_t* d_ncl = nullptr;      // node compute list: (dst, src) pairs, masters then ghost copies

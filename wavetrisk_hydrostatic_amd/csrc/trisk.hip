@@ -26,6 +26,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ctx.h"
@@ -384,14 +385,16 @@ __global__ __launch_bounds__(THREADS) void trisk_stage_kernel(StageArgs a_in) {
 // circulation and mass, and two qe values per corner are overwritten.
 // `which` = 0..3 for the SW, SE, NW, NE corner; region cell (1,1), (T+1,1),
 // (1,T+1), (T+1,T+1).  u*, m, qe* are LDS planes of the 19 x 19 region.
-__device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const double* __restrict__ u1p,
-                               const double* __restrict__ u2p, const double* __restrict__ mp, double* qe0,
-                               double* qe1, double* qe2, const Geo& g, int64_t NC, int64_t base, int S) {
+// The planes are FP64 or FP32 (F); the corrections are evaluated in FP64.
+template <class F>
+__device__ void pentagon_fixup(int which, const F* __restrict__ u0p, const F* __restrict__ u1p,
+                               const F* __restrict__ u2p, const F* __restrict__ mp, F* qe0, F* qe1, F* qe2,
+                               const Geo& g, int64_t NC, int64_t base, int S) {
   auto r = [](int ri, int rj) { return rj * R + ri; };
   auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
-  const double* up[3] = {u0p, u1p, u2p};
-  auto UPL = [&](int e, int i, int j) { return up[e][r(i, j)] * g.len[(int64_t)e * NC + gcell(i, j)]; };
-  auto M = [&](int i, int j) { return mp[r(i, j)]; };
+  const F* up[3] = {u0p, u1p, u2p};
+  auto UPL = [&](int e, int i, int j) { return (double)up[e][r(i, j)] * g.len[(int64_t)e * NC + gcell(i, j)]; };
+  auto M = [&](int i, int j) { return (double)mp[r(i, j)]; };
   auto AR = [&](int p, int i, int j) { return g.area[(int64_t)(p - 1) * NC + gcell(i, j)]; };
   auto TRI = [&](int tt, int i, int j) { return g.tri[(int64_t)tt * NC + gcell(i, j)]; };
   auto COR = [&](int tt, int i, int j) { return g.cor[(int64_t)tt * NC + gcell(i, j)]; };
@@ -406,8 +409,8 @@ __device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const 
     double pvWL = (COR(0, i - 1, j) + cW_L) / m1;
     double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m2;
     double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
-    qe0[r(i - 1, j)] = interp(pvWL, pvSWL);
-    qe2[r(i, j - 1)] = interp(pvSU, pvSWL);
+    qe0[r(i - 1, j)] = (F)interp(pvWL, pvSWL);
+    qe2[r(i, j - 1)] = (F)interp(pvSU, pvSWL);
   } else if (which == 1) {  // IPLUSJMINUS
     const int i = T + 1, j = 1;
     double cSW_L = -UPL(0, i, j) + UPL(0, i - 1, j - 1) + UPL(1, i - 1, j - 1);
@@ -421,8 +424,8 @@ __device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const 
     double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m1;
     double pvL = (COR(0, i, j) + cL) / m2;
     double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m3;
-    qe0[r(i, j)] = interp(pvSWL, pvL);
-    qe1[r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+    qe0[r(i, j)] = (F)interp(pvSWL, pvL);
+    qe1[r(i - 1, j - 1)] = (F)interp(pvSWL, pvSWU);
   } else if (which == 2) {  // IMINUSJPLUS
     const int i = 1, j = T + 1;
     double cSW_U = UPL(2, i, j) - UPL(1, i - 1, j - 1) - UPL(2, i - 1, j - 1);
@@ -436,8 +439,8 @@ __device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const 
     double pvSWU = (COR(1, i - 1, j - 1) + cSW_U) / m1;
     double pvU = (COR(1, i, j) + cU) / m2;
     double pvSWL = (COR(0, i - 1, j - 1) + cSW_L) / m3;
-    qe2[r(i, j)] = interp(pvSWU, pvU);
-    qe1[r(i - 1, j - 1)] = interp(pvSWL, pvSWU);
+    qe2[r(i, j)] = (F)interp(pvSWU, pvU);
+    qe1[r(i - 1, j - 1)] = (F)interp(pvSWL, pvSWU);
   } else {  // IJPLUS
     const int i = T + 1, j = T + 1;
     double cL = UPL(0, i, j) - UPL(0, i, j + 1) - UPL(2, i, j);
@@ -449,8 +452,8 @@ __device__ void pentagon_fixup(int which, const double* __restrict__ u0p, const 
     double pvL = (COR(0, i, j) + cL) / m1;
     double pvWL = (COR(0, i - 1, j) + cW_L) / m2;
     double pvSU = (COR(1, i, j - 1) + cS_U) / m3;
-    qe0[r(i, j)] = interp(pvL, pvSU);
-    qe2[r(i, j)] = interp(pvL, pvWL);
+    qe0[r(i, j)] = (F)interp(pvL, pvSU);
+    qe2[r(i, j)] = (F)interp(pvL, pvWL);
   }
 }
 
@@ -1090,10 +1093,12 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
   }
 }
 
-// halo := owner (nodes: mass and temp; edges: velocity with sign), all layers
+// halo := owner (nodes: mass and temp; edges: velocity with sign), all
+// layers; F64 state slots and the FP32 stage buffers (F = float)
+template <class F>
 __global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__ nd, int nn,
-                                  const int64_t* __restrict__ ed, const int8_t* __restrict__ es, int ne, double* m,
-                                  double* t, double* u) {
+                                  const int64_t* __restrict__ ed, const int8_t* __restrict__ es, int ne, F* m, F* t,
+                                  F* u) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t total = (int64_t)(nn + ne) * L;
   if (idx >= total) return;
@@ -1106,7 +1111,7 @@ __global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__
   } else {
     e -= nn;
     int64_t dst = ed[2 * e], src = ed[2 * e + 1];  // plane indices e*NC + cell
-    u[(int64_t)k * 3 * NC + dst] = (double)es[e] * u[(int64_t)k * 3 * NC + src];
+    u[(int64_t)k * 3 * NC + dst] = (F)es[e] * u[(int64_t)k * 3 * NC + src];
   }
 }
 
@@ -1129,6 +1134,412 @@ __global__ void copy_frozen_kernel(int L, int64_t NC, const int64_t* __restrict_
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// FP32-bulk stage kernel (precision mode WTG_PRECISION_FP32_BULK)
+// ---------------------------------------------------------------------------
+// The same tile, layer pairing and phases as trisk_stage_pair_kernel, with
+// the bulk of the stencils in FP32: the velocity, mass and temperature planes
+// of the flux stencils, circulation, potential vorticity, qe, mass and
+// temperature fluxes, kinetic energy, divergence and the Qperp sum.  FP64
+// stays where cancellation or accumulation would cost accuracy: the state
+// itself, the hydrostatic column scan (pressure and geopotential), Bernoulli
+// and Exner and their edge differences, and the update of the state.
+// The intermediate RK stages are kept as FP32 increments over the FP64 state
+// q: stage k evaluates the trend on y = q + d_k (d_1 = 0) and writes
+// d_{k+1} = h_{k+1} * trend(y) in FP32, the last stage writes
+// q + h * trend(y) in FP64.  An increment is small beside the state, so its
+// FP32 rounding is far below that of an FP32 copy of the state, and nothing
+// rounded accumulates in the prognostic variables.
+struct GeoF {          // FP32 copies of the stencil geometry
+  const float* len;    // [3][NC]
+  const float* lp;     // [3][NC] len * pedlen (kinetic energy)
+  const float* pedf;   // [3][NC]
+  const float* area;   // [6][NC]
+  const float* hinv;   // [NC]
+  const float* cor;    // [2][NC]
+};
+
+template <class To>
+struct MixedArgs {
+  int n, S, L, ntile;
+  int64_t NC;
+  const double *qm, *qt, *qu;  // FP64 state (ghost filled)
+  const float *dm, *dt, *du;   // FP32 stage increment over it (ghost filled; unused in the first stage)
+  To *om, *ot, *ou;            // increment of the next stage (To = float) or the new state (To = double)
+  double h;
+  GeoF f;
+  Geo g;                       // FP64 geometry: topography, pentagon corrections
+  double grav, inv_rho0, p_top;
+};
+
+template <class To>
+__device__ __forceinline__ MixedArgs<To> member_args(const MixedArgs<To>& a0) {
+  MixedArgs<To> a = a0;
+  const int64_t s1 = (int64_t)blockIdx.y * a0.L * a0.NC, s3 = 3 * s1;
+  a.qm += s1; a.qt += s1; a.qu += s3;
+  a.dm += s1; a.dt += s1; a.du += s3;
+  a.om += s1; a.ot += s1; a.ou += s3;
+  a.g.topo += (int64_t)blockIdx.y * a0.NC;
+  return a;
+}
+
+struct GeoSF {
+  float len[3][RR], lp[3][RR], area[6][RR], hinv[RR];
+};
+struct LayerSF {  // t holds t/m from phase 3 on
+  double bern[RR], exn[RR];
+  double mt[RR], md[RR];  // m - t and m in FP64 for the other group's column scan
+  float m[RR], t[RR], u[3][RR];
+  float pvL[RR], pvU[RR];
+  float qe[3][RR], hm[3][RR], ht[3][RR];
+};
+
+// load / store of element `cell` of a plane (32-bit offset from a uniform base)
+template <class F>
+__device__ __forceinline__ F ldc(const F* p, uint32_t cell) {
+  return *reinterpret_cast<const F*>(reinterpret_cast<const char*>(p) + cell * (uint32_t)sizeof(F));
+}
+template <class F>
+__device__ __forceinline__ void stc(F* p, uint32_t cell, F v) {
+  *reinterpret_cast<F*>(reinterpret_cast<char*>(p) + cell * (uint32_t)sizeof(F)) = v;
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fint(float a, float b) { return 0.5f * (a + b); }
+
+__device__ __forceinline__ void weights_f32(const GeoSF& gs, int y, int offs, float w[5]) {
+  float acc0 = gs.area[offs][y];
+  float acc1 = acc0 + gs.area[(1 + offs) % 6][y];
+  float acc2 = acc1 + gs.area[(2 + offs) % 6][y];
+  float acc3 = acc2 + gs.area[(3 + offs) % 6][y];
+  float acc4 = acc3 + gs.area[(4 + offs) % 6][y];
+  float hi = gs.hinv[y];
+  w[0] = 0.5f - acc0 * hi;
+  w[1] = -(0.5f - acc1 * hi);
+  w[2] = 0.5f - acc2 * hi;
+  w[3] = -(0.5f - acc3 * hi);
+  w[4] = 0.5f - acc4 * hi;
+}
+
+// value of the evaluation state y = q + d at plane offset `ks`, cell `c`
+template <bool HasD>
+__device__ __forceinline__ double yval(const double* q, const float* d, int64_t ks, uint32_t c) {
+  return HasD ? ldc(q + ks, c) + (double)ldc(d + ks, c) : ldc(q + ks, c);
+}
+
+// HasD: the stage has an increment (stages 2-4); To: float = the next
+// stage's increment, double = the new state
+template <bool HasD, class To>
+__global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<To> a_in) {
+#pragma clang fp contract(fast)
+  const MixedArgs<To> a = member_args(a_in);
+  constexpr bool kState = sizeof(To) == 8;
+  __shared__ GeoSF gs;
+  __shared__ union {
+    LayerSF ls[2];
+    double flat[2 * sizeof(LayerSF) / sizeof(double)];
+  } lsu;
+  const int tid = threadIdx.x;
+  const int grp = __builtin_amdgcn_readfirstlane(tid / GROUP);
+  const int lt = tid % GROUP;
+  const int64_t NC = a.NC;
+  const int S = a.S;
+
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int tpl = a.ntile * a.ntile;
+  const int lz = tile / tpl;
+  const int rem = tile % tpl;
+  const int i0 = (rem % a.ntile) * T, j0 = (rem / a.ntile) * T;
+  const int64_t base = (int64_t)lz * S * S + (int64_t)(j0 - 1 + H) * S + (i0 - 1 + H);
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+
+  const bool cSW = (i0 == 0 && j0 == 0), cSE = (i0 + T == a.n && j0 == 0);
+  const bool cNW = (i0 == 0 && j0 + T == a.n), cNE = (i0 + T == a.n && j0 + T == a.n);
+  const bool corner = cSW || cSE || cNW || cNE;
+
+  const bool active = lt < RR;
+  const int ri = lt % R, rj = lt / R;
+  const int x = lt;
+  const uint32_t xc = (uint32_t)(active ? gcell(ri, rj) : base);
+  const bool tri_cell = active && ri <= T + 1 && rj <= T + 1;
+  const bool p_cell = active && ri >= 1 && rj >= 1 && ri <= T + 1 && rj <= T + 1;
+  const bool pole = (lz == 0 && cSE && ri == T + 1 && rj == 1) || (lz == 5 && cNW && ri == 1 && rj == T + 1);
+  const bool own = lt < T * T;
+  const int oi = 1 + lt % T, oj = 1 + lt / T;
+  const int y = oj * R + oi;
+  const uint32_t oc = (uint32_t)gcell(oi, oj);
+  const int lo = lt % (T * T);
+  const uint32_t ocl = (uint32_t)gcell(1 + lo % T, 1 + lo / T);
+
+  // --- geometry: planes in LDS, the flux lengths and Coriolis in registers ---
+  float pf0 = 0.f, pf1 = 0.f, pf2 = 0.f, cr0 = 0.f, cr1 = 0.f;
+  if (active) {
+    if (grp == 0) {
+      for (int e = 0; e < 3; ++e) {
+        gs.len[e][x] = ldc(a.f.len + e * NC, xc);
+        gs.lp[e][x] = ldc(a.f.lp + e * NC, xc);
+      }
+      gs.hinv[x] = ldc(a.f.hinv, xc);
+    } else {
+      for (int p = 0; p < 6; ++p) gs.area[p][x] = ldc(a.f.area + p * NC, xc);
+    }
+    pf0 = ldc(a.f.pedf, xc);
+    pf1 = ldc(a.f.pedf + NC, xc);
+    pf2 = ldc(a.f.pedf + 2 * NC, xc);
+    cr0 = ldc(a.f.cor, xc);
+    cr1 = ldc(a.f.cor + NC, xc);
+  }
+
+  // column state in FP64 (cal_surf_press, set_surf_geopot): P = pressure at
+  // the bottom of the next layer, G = geopotential there
+  double P = 0, G = 0;
+  {
+    constexpr int KMAX = (int)(sizeof(lsu.flat) / sizeof(double)) / RR;
+    double s = 0.0;
+    for (int k0 = 0; k0 < a.L; k0 += KMAX) {
+      const int kn = min(KMAX, a.L - k0);
+      if (active) {
+        for (int kk = grp; kk < kn; kk += 2) {
+          const int64_t ks = (int64_t)(k0 + kk) * NC;
+          lsu.flat[kk * RR + x] = yval<HasD>(a.qm, a.dm, ks, xc) - yval<HasD>(a.qt, a.dt, ks, xc);
+        }
+      }
+      __syncthreads();
+      if (p_cell)
+        for (int kk = 0; kk < kn; ++kk) s = s + lsu.flat[kk * RR + x];
+      __syncthreads();
+    }
+    if (p_cell) {
+      P = a.grav * s + a.p_top;
+      G = a.grav * ldc(a.g.topo, xc);
+    }
+  }
+  const double g_rho = a.grav * a.inv_rho0;
+
+  LayerSF& sl = lsu.ls[grp];
+  const LayerSF& so = lsu.ls[1 - grp];
+  double m, t;  // the group's layer: mass and temp in FP64, velocity in FP32
+  float u0, u1, u2;
+  {
+    const int64_t ks = (int64_t)min(grp, a.L - 1) * NC, ke3 = 3 * ks;
+    m = yval<HasD>(a.qm, a.dm, ks, xc);
+    t = yval<HasD>(a.qt, a.dt, ks, xc);
+    u0 = (float)yval<HasD>(a.qu, a.du, ke3, xc);
+    u1 = (float)yval<HasD>(a.qu, a.du, ke3 + NC, xc);
+    u2 = (float)yval<HasD>(a.qu, a.du, ke3 + 2 * NC, xc);
+  }
+  if (active && grp < a.L) {
+    sl.mt[x] = m - t;
+    sl.md[x] = m;
+    sl.m[x] = (float)m;
+    sl.t[x] = (float)t;
+    sl.u[0][x] = u0;
+    sl.u[1][x] = u1;
+    sl.u[2][x] = u2;
+  }
+  __syncthreads();
+
+  float w[6][5];
+  if (own) {
+    weights_f32(gs, y + 1, 3, w[3]);
+    weights_f32(gs, y + R + 1, 4, w[4]);
+    weights_f32(gs, y + R, 5, w[5]);
+  }
+
+  const int niter = (a.L + 1) / 2;
+  for (int it = 0; it < niter; ++it) {
+    const int k = 2 * it + grp;
+    const bool lay = k < a.L;
+    const bool pair = 2 * it + 1 < a.L;
+    const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
+    __syncthreads();
+    // --- 1. prefetch the group's next layer -----------------------------------
+    double nm, nt, nu0, nu1, nu2;
+    {
+      const int64_t ks2 = (int64_t)min(k + 2, a.L - 1) * NC, ke32 = 3 * ks2;
+      nm = yval<HasD>(a.qm, a.dm, ks2, xc);
+      nt = yval<HasD>(a.qt, a.dt, ks2, xc);
+      nu0 = yval<HasD>(a.qu, a.du, ke32, xc);
+      nu1 = yval<HasD>(a.qu, a.du, ke32 + NC, xc);
+      nu2 = yval<HasD>(a.qu, a.du, ke32 + 2 * NC, xc);
+    }
+
+    // --- 2. column scan (FP64); PV, fluxes, kinetic energy (FP32) -------------
+    const float mf = (float)m, tf = (float)t;
+    if (lay) {
+      if (p_cell) {
+        double pr, phi_k;
+        if (grp == 1) {
+          P = P - a.grav * so.mt[x];
+          G = G + g_rho * so.md[x];
+        }
+        {
+          double p_upper = P - a.grav * (m - t);
+          pr = 0.5 * (P + p_upper);
+          double gpl = G;
+          G = gpl + g_rho * m;
+          phi_k = 0.5 * (G + gpl);
+          P = p_upper;
+        }
+        if (grp == 0 && pair) {
+          P = P - a.grav * so.mt[x];
+          G = G + g_rho * so.md[x];
+        }
+        const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+        const float uS = sl.u[2][xS], uSW = sl.u[1][xSW], uW = sl.u[0][xW];
+        const float ke = (u2 * u2 * gs.lp[2][x] + u1 * u1 * gs.lp[1][x] + u0 * u0 * gs.lp[0][x] +
+                          uS * uS * gs.lp[2][xS] + uSW * uSW * gs.lp[1][xSW] + uW * uW * gs.lp[0][xW]) *
+                         (0.25f * gs.hinv[x]);
+        sl.bern[x] = (double)ke + phi_k + pr * a.inv_rho0;
+        sl.exn[x] = -phi_k;
+      }
+      if (tri_cell) {
+        const int xE = x + 1, xN = x + R, xNE = x + R + 1;
+        const float upRT = u0 * gs.len[0][x], upDG = u1 * gs.len[1][x], upUP = u2 * gs.len[2][x];
+        const float circL = upRT + sl.u[2][xE] * gs.len[2][xE] + upDG;
+        const float circU = -(upDG + upUP + sl.u[0][xN] * gs.len[0][xN]);
+        const float mE = sl.m[xE], mNE = sl.m[xNE], mN = sl.m[xN];
+        const float massL = mf * gs.area[0][x] + mE * gs.area[2][xE] + mNE * gs.area[4][xNE];
+        const float massU = mf * gs.area[1][x] + mNE * gs.area[3][xNE] + mN * gs.area[5][xN];
+        sl.pvL[x] = (cr0 + circL) * frcp(massL);
+        sl.pvU[x] = (cr1 + circU) * frcp(massU);
+        const float f0 = u0 * pf0, f1 = u1 * pf1, f2 = u2 * pf2;
+        sl.hm[0][x] = f0 * fint(mf, mE);
+        sl.hm[1][x] = f1 * fint(mf, mNE);
+        sl.hm[2][x] = f2 * fint(mf, mN);
+        sl.ht[0][x] = f0 * fint(tf, sl.t[xE]);
+        sl.ht[1][x] = f1 * fint(tf, sl.t[xNE]);
+        sl.ht[2][x] = f2 * fint(tf, sl.t[xN]);
+      }
+    }
+    __syncthreads();
+
+    // --- 3. FP64 base state of the update (last stage); qe; t/m in place ------
+    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
+    if (kState) {
+      const int64_t ksb = (int64_t)min(k, a.L - 1) * NC, ke3b = 3 * ksb;
+      bm = ldc(a.qm + ksb, ocl);
+      bt = ldc(a.qt + ksb, ocl);
+      bu0 = ldc(a.qu + ke3b, ocl);
+      bu1 = ldc(a.qu + ke3b + NC, ocl);
+      bu2 = ldc(a.qu + ke3b + 2 * NC, ocl);
+    }
+    if (lay && active) {
+      if (tri_cell) {
+        if (rj >= 1) sl.qe[0][x] = fint(sl.pvU[x - R], sl.pvL[x]);
+        sl.qe[1][x] = fint(sl.pvU[x], sl.pvL[x]);
+        if (ri >= 1) sl.qe[2][x] = fint(sl.pvU[x], sl.pvL[x - 1]);
+      }
+      sl.t[x] = (float)(t / m);
+    }
+    __syncthreads();
+
+    if (corner) {
+      const bool mine[4] = {cSW, cSE, cNW, cNE};
+      if (lay && lt < 4 && mine[lt])
+        pentagon_fixup(lt, sl.u[0], sl.u[1], sl.u[2], sl.m, sl.qe[0], sl.qe[1], sl.qe[2], a.g, NC, base, S);
+      __syncthreads();
+    }
+
+    // --- 4. trends of owned nodes and edges, pole master; update -------------
+    if (lay && own) {
+      const int xW = y - 1, xS = y - R, xSW = y - R - 1, xE = y + 1, xN = y + R, xNE = y + R + 1;
+      const int xNW = y + R - 1, xSE = y - R + 1;
+      const float hi = gs.hinv[y];
+      const float divm =
+          (sl.hm[0][y] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][y] + sl.hm[2][y] - sl.hm[2][xS]) * hi;
+      const float divt =
+          (sl.ht[0][y] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][y] + sl.ht[2][y] - sl.ht[2][xS]) * hi;
+      stc(a.om + ks, oc, (To)(bm - a.h * (double)divm));
+      stc(a.ot + ks, oc, (To)(bt - a.h * (double)divt));
+      float qp[3];
+      weights_f32(gs, y, 0, w[0]);
+      weights_f32(gs, y, 1, w[1]);
+      weights_f32(gs, y, 2, w[2]);
+#define HM(e, xx) sl.hm[e][xx]
+#define QE(e, xx) sl.qe[e][xx]
+      const float qr = QE(0, y);
+      qp[0] = HM(1, y) * fint(QE(1, y), qr) * w[0][0] + HM(2, y) * fint(QE(2, y), qr) * w[0][1] +
+              HM(0, xW) * fint(QE(0, xW), qr) * w[0][2] + HM(1, xSW) * fint(QE(1, xSW), qr) * w[0][3] +
+              HM(2, xS) * fint(QE(2, xS), qr) * w[0][4] + HM(1, xS) * fint(QE(1, xS), qr) * w[3][0] +
+              HM(2, xSE) * fint(QE(2, xSE), qr) * w[3][1] + HM(0, xE) * fint(QE(0, xE), qr) * w[3][2] +
+              HM(1, xE) * fint(QE(1, xE), qr) * w[3][3] + HM(2, xE) * fint(QE(2, xE), qr) * w[3][4];
+      const float qd = QE(1, y);
+      qp[1] = HM(2, y) * fint(QE(2, y), qd) * w[1][0] + HM(0, xW) * fint(QE(0, xW), qd) * w[1][1] +
+              HM(1, xSW) * fint(QE(1, xSW), qd) * w[1][2] + HM(2, xS) * fint(QE(2, xS), qd) * w[1][3] +
+              HM(0, y) * fint(QE(0, y), qd) * w[1][4] + HM(2, xE) * fint(QE(2, xE), qd) * w[4][0] +
+              HM(0, xNE) * fint(QE(0, xNE), qd) * w[4][1] + HM(1, xNE) * fint(QE(1, xNE), qd) * w[4][2] +
+              HM(2, xNE) * fint(QE(2, xNE), qd) * w[4][3] + HM(0, xN) * fint(QE(0, xN), qd) * w[4][4];
+      const float qu = QE(2, y);
+      qp[2] = HM(0, xW) * fint(QE(0, xW), qu) * w[2][0] + HM(1, xSW) * fint(QE(1, xSW), qu) * w[2][1] +
+              HM(2, xS) * fint(QE(2, xS), qu) * w[2][2] + HM(0, y) * fint(QE(0, y), qu) * w[2][3] +
+              HM(1, y) * fint(QE(1, y), qu) * w[2][4] + HM(0, xN) * fint(QE(0, xN), qu) * w[5][0] +
+              HM(1, xN) * fint(QE(1, xN), qu) * w[5][1] + HM(2, xN) * fint(QE(2, xN), qu) * w[5][2] +
+              HM(0, xNW) * fint(QE(0, xNW), qu) * w[5][3] + HM(1, xW) * fint(QE(1, xW), qu) * w[5][4];
+#undef HM
+#undef QE
+      const float th0 = sl.t[y];
+      const float the[3] = {fint(th0, sl.t[xE]), fint(th0, sl.t[xNE]), fint(th0, sl.t[xN])};
+      const double b0 = sl.bern[y], x0 = sl.exn[y];
+      const float il[3] = {frcp(gs.len[0][y]), frcp(gs.len[1][y]), frcp(gs.len[2][y])};
+      const float dB[3] = {(float)(sl.bern[xE] - b0), (float)(b0 - sl.bern[xNE]), (float)(sl.bern[xN] - b0)};
+      const float dX[3] = {(float)(sl.exn[xE] - x0), (float)(x0 - sl.exn[xNE]), (float)(sl.exn[xN] - x0)};
+      const double bu[3] = {bu0, bu1, bu2};
+      for (int e = 0; e < 3; ++e) {
+        const float du = (-qp[e] - dB[e] - the[e] * dX[e]) * il[e];
+        stc(a.ou + ke3 + e * NC, oc, (To)(bu[e] + a.h * (double)du));
+      }
+    }
+    if (kState && !(lay && own)) asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
+    if (lay && pole) {
+      const float hi = gs.hinv[x];
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+      const float divm =
+          (sl.hm[0][x] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][x] + sl.hm[2][x] - sl.hm[2][xS]) * hi;
+      const float divt =
+          (sl.ht[0][x] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][x] + sl.ht[2][x] - sl.ht[2][xS]) * hi;
+      const double pm = kState ? ldc(a.qm + ks, xc) : 0.0, pt = kState ? ldc(a.qt + ks, xc) : 0.0;
+      stc(a.om + ks, xc, (To)(pm - a.h * (double)divm));
+      stc(a.ot + ks, xc, (To)(pt - a.h * (double)divt));
+    }
+    __syncthreads();
+    // --- 5. stage the group's next layer --------------------------------------
+    m = nm;
+    t = nt;
+    u0 = (float)nu0;
+    u1 = (float)nu1;
+    u2 = (float)nu2;
+    if (active && k + 2 < a.L) {
+      sl.mt[x] = m - t;
+      sl.md[x] = m;
+      sl.m[x] = (float)m;
+      sl.t[x] = (float)t;
+      sl.u[0][x] = u0;
+      sl.u[1][x] = u1;
+      sl.u[2][x] = u2;
+    }
+  }
+}
+
+// FP32 copies of the stencil geometry for the FP32-bulk kernel
+__global__ void geo_f32_kernel(int64_t NC, const double* __restrict__ len, const double* __restrict__ ped,
+                               const double* __restrict__ pedf, const double* __restrict__ area,
+                               const double* __restrict__ hinv, const double* __restrict__ cor, float* flen,
+                               float* flp, float* fpedf, float* farea, float* fhinv, float* fcor) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= NC) return;
+  for (int e = 0; e < 3; ++e) {
+    flen[e * NC + c] = (float)len[e * NC + c];
+    flp[e * NC + c] = (float)(len[e * NC + c] * ped[e * NC + c]);
+    fpedf[e * NC + c] = (float)pedf[e * NC + c];
+  }
+  for (int p = 0; p < 6; ++p) farea[p * NC + c] = (float)area[p * NC + c];
+  fhinv[c] = (float)hinv[c];
+  fcor[c] = (float)cor[c];
+  fcor[NC + c] = (float)cor[NC + c];
+}
 
 // ---------------------------------------------------------------------------
 // split-explicit step with implicit free surface (mode split)
@@ -1236,17 +1647,19 @@ int check_slot(wtg_ctx* c, int slot) {
   return 0;
 }
 
-int launch_ghost_fill(wtg_ctx* c, const wtg_ctx::Slot& s) {
+template <class F>
+int launch_ghost_fill(wtg_ctx* c, F* m, F* t, F* u) {
   const int L = c->params.nlayers * c->nmembers;
   int64_t total = (int64_t)(c->nn + c->ne) * L;
   if (total == 0) return 0;
   int threads = 256;
   int64_t blocks = (total + threads - 1) / threads;
-  hipLaunchKernelGGL(ghost_fill_kernel, dim3((unsigned)blocks), dim3(threads), 0, c->stream, L, c->info.ncells,
-                     c->d_nd, c->nn, c->d_ed, c->d_es, c->ne, s.m, s.t, s.u);
+  hipLaunchKernelGGL(ghost_fill_kernel<F>, dim3((unsigned)blocks), dim3(threads), 0, c->stream, L, c->info.ncells,
+                     c->d_nd, c->nn, c->d_ed, c->d_es, c->ne, m, t, u);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
+int launch_ghost_fill(wtg_ctx* c, const wtg_ctx::Slot& s) { return launch_ghost_fill(c, s.m, s.t, s.u); }
 
 int launch_copy_frozen(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to) {
   const int L = c->params.nlayers * c->nmembers;
@@ -1297,12 +1710,122 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
   return 0;
 }
 
+// FP32 geometry copies and the two FP32 stage buffers of the FP32-bulk mode
+int f32_ready(wtg_ctx* c) {
+  if (c->f32_done) return 0;
+  const int64_t NC = c->info.ncells;
+  const size_t L = (size_t)c->params.nlayers * c->nmembers;
+  int rc = 0;
+  if ((rc = dalloc(c, &c->f_len, 3 * NC)) || (rc = dalloc(c, &c->f_lp, 3 * NC)) ||
+      (rc = dalloc(c, &c->f_pedf, 3 * NC)) || (rc = dalloc(c, &c->f_area, 6 * NC)) ||
+      (rc = dalloc(c, &c->f_hinv, NC)) || (rc = dalloc(c, &c->f_cor, 2 * NC)))
+    return rc;
+  for (auto& s : c->fstage) {
+    if ((rc = dalloc(c, &s.m, L * NC)) || (rc = dalloc(c, &s.t, L * NC)) || (rc = dalloc(c, &s.u, 3 * L * NC)))
+      return rc;
+    HIPCHK(c, hipMemsetAsync(s.m, 0, L * NC * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(s.t, 0, L * NC * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(s.u, 0, 3 * L * NC * sizeof(float), c->stream));
+  }
+  hipLaunchKernelGGL(geo_f32_kernel, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, c->stream, NC, c->d_len,
+                     c->d_ped, c->d_pedf, c->d_area, c->d_hinv, c->d_cor, c->f_len, c->f_lp, c->f_pedf, c->f_area,
+                     c->f_hinv, c->f_cor);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->f32_done = true;
+  return 0;
+}
+
+template <bool HasD, class To>
+int launch_stage_mixed(wtg_ctx* c, const wtg_ctx::Slot& q, const wtg_ctx::SlotF* d, To* om, To* ot, To* ou,
+                       double h) {
+  MixedArgs<To> a{};
+  a.n = c->info.n;
+  a.S = c->info.S;
+  a.L = c->params.nlayers;
+  a.ntile = a.n / T;
+  a.NC = c->info.ncells;
+  a.qm = q.m;
+  a.qt = q.t;
+  a.qu = q.u;
+  // the first stage has no increment: its pointers are never read
+  a.dm = d ? d->m : c->fstage[0].m;
+  a.dt = d ? d->t : c->fstage[0].t;
+  a.du = d ? d->u : c->fstage[0].u;
+  a.om = om;
+  a.ot = ot;
+  a.ou = ou;
+  a.h = h;
+  a.f = GeoF{c->f_len, c->f_lp, c->f_pedf, c->f_area, c->f_hinv, c->f_cor};
+  a.g = Geo{c->d_len, c->d_ped, c->d_pedf, c->d_area, c->d_hinv, c->d_tri, c->d_cor, c->d_topo};
+  a.grav = c->params.grav;
+  a.inv_rho0 = 1.0 / c->params.ref_density;
+  a.p_top = c->params.p_top;
+  const dim3 grid((unsigned)(10 * a.ntile * a.ntile), (unsigned)c->nmembers);
+  hipLaunchKernelGGL((trisk_stage_mixed_kernel<HasD, To>), grid, dim3(THREADS3), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// FP32-bulk RK4: the low-storage stages with FP32 increments d over the FP64
+// state q (y_k = q + d_k, d_{k+1} = h_{k+1} f(y_k)); the last stage writes
+// q + dt f(y_4) into the other FP64 buffer, because every stage reads q at
+// the halo of its tile, where the neighbouring tiles would be updating it in
+// place.  The state ping-pongs between the slot and the context's first FP64
+// stage buffer and is copied back after an odd number of steps.  Halo slots
+// that are never ghost filled keep d = 0 (the buffers start zeroed and
+// stages write owned cells and pole masters only).
+int rk4_mixed_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
+  if (int rc = f32_ready(c)) return rc;
+  const wtg_ctx::Slot& sol = c->slots[slot];
+  const wtg_ctx::Slot* cur = &sol;
+  const wtg_ctx::Slot* nxt = &c->slots[c->nslots];
+  auto &f1 = c->fstage[0], &f2 = c->fstage[1];
+  if (nsteps > 0)
+    if (int rc = launch_copy_frozen(c, sol, *nxt)) return rc;
+  int launch = 0;
+  auto rec = [&](int k) -> int {
+    if (ev) HIPCHK(c, hipEventRecord(ev[2 * launch + k], c->stream));
+    return 0;
+  };
+  for (int s = 0; s < nsteps; ++s) {
+    const auto& q = *cur;
+    const auto& o = *nxt;
+    int rc = 0;
+    if ((rc = rec(0)) || (rc = launch_stage_mixed<false, float>(c, q, nullptr, f1.m, f1.t, f1.u, dt / 4.0)) ||
+        (rc = rec(1)) || (rc = launch_ghost_fill(c, f1.m, f1.t, f1.u)))
+      return rc;
+    ++launch;
+    if ((rc = rec(0)) || (rc = launch_stage_mixed<true, float>(c, q, &f1, f2.m, f2.t, f2.u, dt / 3.0)) ||
+        (rc = rec(1)) || (rc = launch_ghost_fill(c, f2.m, f2.t, f2.u)))
+      return rc;
+    ++launch;
+    if ((rc = rec(0)) || (rc = launch_stage_mixed<true, float>(c, q, &f2, f1.m, f1.t, f1.u, dt / 2.0)) ||
+        (rc = rec(1)) || (rc = launch_ghost_fill(c, f1.m, f1.t, f1.u)))
+      return rc;
+    ++launch;
+    if ((rc = rec(0)) || (rc = launch_stage_mixed<true, double>(c, q, &f1, o.m, o.t, o.u, dt)) || (rc = rec(1)) ||
+        (rc = launch_ghost_fill(c, o)))
+      return rc;
+    ++launch;
+    std::swap(cur, nxt);
+  }
+  if (cur != &sol) {
+    const size_t ns = (size_t)c->params.nlayers * c->nmembers * c->info.ncells;
+    HIPCHK(c, hipMemcpyAsync(sol.m, cur->m, ns * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sol.t, cur->t, ns * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sol.u, cur->u, 3 * ns * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return 0;
+}
+
 // nsteps RK4 steps; when ev is given, ev[2i] / ev[2i+1] are recorded around
 // the i-th stage-kernel launch (no synchronisation between launches)
 int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "nsteps < 0");
+  if (c->precision == WTG_PRECISION_FP32_BULK) return rk4_mixed_impl(c, slot, dt, nsteps, ev);
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
   const auto& w2 = c->slots[c->nslots + 1];
@@ -1726,6 +2249,16 @@ int wtg_rk4_timed(wtg_ctx* c, int slot, double dt, int nsteps, double* total_ms,
     if (e) hipEventDestroy(e);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_set_precision(wtg_ctx* c, int precision) {
+  if (!c) return WTG_E_ARG;
+  if (precision != WTG_PRECISION_FP64 && precision != WTG_PRECISION_FP32_BULK)
+    return fail(c, WTG_E_ARG, "unknown precision mode %d", precision);
+  if (precision == WTG_PRECISION_FP32_BULK)
+    if (int rc = f32_ready(c)) return rc;
+  c->precision = precision;
   return 0;
 }
 
