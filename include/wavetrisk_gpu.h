@@ -151,6 +151,23 @@ int wtg_rk4_timed(wtg_ctx* ctx, int slot, double dt, int nsteps, double* total_m
 #define WTG_PRECISION_FP32_BULK 1
 int wtg_set_precision(wtg_ctx* ctx, int precision);
 
+/* Launch shape of the FP32-bulk stage kernel (no counterpart in the
+ * reference): WTG_SHAPE_LAYER_PAIRS (one 768-thread block per 16 x 16 tile,
+ * two layers in lock step, one owned cell per thread in the trend phase),
+ * WTG_SHAPE_EDGE_TASKS (the same with the trend phase split into edge tasks
+ * over all threads), WTG_SHAPE_ONE_LAYER (384 threads, one layer at a time,
+ * three blocks per CU), or WTG_SHAPE_AUTO (default): the first FP32-bulk step
+ * times every shape on the context's state and keeps the fastest.  All shapes
+ * compute the same arithmetic per cell.  wtg_get_stage_shape reports the
+ * shape in use (WTG_SHAPE_AUTO before tuning) and, in ms[0..2], the tuner's
+ * mean launch time of shapes 1..3 (0 when the shape was set by hand). */
+#define WTG_SHAPE_AUTO 0
+#define WTG_SHAPE_LAYER_PAIRS 1
+#define WTG_SHAPE_EDGE_TASKS 2
+#define WTG_SHAPE_ONE_LAYER 3
+int wtg_set_stage_shape(wtg_ctx* ctx, int shape);
+int wtg_get_stage_shape(wtg_ctx* ctx, int* shape, float* ms);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

@@ -15,6 +15,8 @@ errors against the oracle's FP64 result on the same inputs:
 A dropped or mis-signed term in any FP32 stencil moves these errors to
 1e-3 or more.  Ensembles in FP32-bulk mode equal single-member runs bit for
 bit, and switching back to FP64 gives the bit-exact FP64 path again.
+Every launch shape of the FP32-bulk kernel (the autotuner's candidates)
+meets the same bars.
 """
 import numpy as np
 import pytest
@@ -52,11 +54,15 @@ def _case(name, g):
     return 1, 300.0, topo, m, t, u
 
 
-def _run(g, L, dt, topo, m, t, u, nsteps, precision, members=1):
+SHAPES = {"layer pairs": W.SHAPE_LAYER_PAIRS, "edge tasks": W.SHAPE_EDGE_TASKS, "one layer": W.SHAPE_ONE_LAYER}
+
+
+def _run(g, L, dt, topo, m, t, u, nsteps, precision, members=1, shape=W.SHAPE_AUTO):
     ctx = W.Context(g, nlayers=L, members=members)
     ctx.set_topography(topo)
     ctx.upload_state(0, m, t, u)
     ctx.set_precision(precision)
+    ctx.set_stage_shape(shape)
     ctx.rk4(0, dt, nsteps)
     return ctx.download_state(0)
 
@@ -72,13 +78,34 @@ def test_fp32_bulk_against_oracle(name):
     for nsteps, bars in ((1, {"mass": 1e-7, "temp": 1e-7, "velo": 1e-6, "eta/depth": 1e-7}),
                          (100, {"mass": 1e-6, "temp": 1e-6, "velo": 1e-5, "eta/depth": 1e-6})):
         want = o.rk4(m0.copy(), t0.copy(), u0.copy(), dt, nsteps)
-        got = _run(g, L, dt, topo, m, t, u, nsteps, W.PRECISION_FP32_BULK)
-        err = _errors(g, got, want, topo)
-        print(f"  {nsteps} steps: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
-        for k, v in err.items():
-            assert v <= bars[k], f"{name}, {nsteps} steps: {k} error {v:.3e} above {bars[k]:.0e}"
-        # the mode is in effect: FP32 stencils do not reproduce FP64 bits
-        assert err["mass"] > 0 or err["velo"] > 0
+        for sname, shape in SHAPES.items():
+            got = _run(g, L, dt, topo, m, t, u, nsteps, W.PRECISION_FP32_BULK, shape=shape)
+            err = _errors(g, got, want, topo)
+            print(f"  {nsteps} steps, {sname}: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
+            for k, v in err.items():
+                assert v <= bars[k], f"{name}, {nsteps} steps, {sname}: {k} error {v:.3e} above {bars[k]:.0e}"
+            # the mode is in effect: FP32 stencils do not reproduce FP64 bits
+            assert err["mass"] > 0 or err["velo"] > 0
+
+
+def test_stage_shape_autotuner():
+    print("FP32-bulk stage autotuner J5 L4", flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    topo, m, t, u = cases.ocean(g, 4, seed=5)
+    ctx = W.Context(g, nlayers=4)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.set_precision(W.PRECISION_FP32_BULK)
+    assert ctx.stage_shape()[0] == W.SHAPE_AUTO
+    ctx.rk4(0, 600.0, 1)
+    shape, ms = ctx.stage_shape()
+    print(f"  chosen shape {shape}, ms {ms}", flush=True)
+    assert all(v > 0 for v in ms)
+    assert shape == 1 + int(np.argmin(ms))
+    ctx.set_stage_shape(W.SHAPE_ONE_LAYER)
+    assert ctx.stage_shape() == (W.SHAPE_ONE_LAYER, [0.0, 0.0, 0.0])
+    with pytest.raises(W.WtgError, match="shape"):
+        ctx.set_stage_shape(9)
 
 
 def test_fp32_bulk_full_size_J8_L30():
@@ -101,10 +128,12 @@ def test_fp32_bulk_ensemble_equals_single_runs_and_fp64_switch_back():
     draws = [cases.ocean(g, L, seed=20 + k) for k in range(M)]
     topo = np.stack([d[0] for d in draws])
     state = [np.concatenate([d[k] for d in draws]) for k in (1, 2, 3)]
-    em, et, eu = _run(g, L, dt, topo, *state, 3, W.PRECISION_FP32_BULK, members=M)
+    # one fixed shape: the shapes agree to rounding, not bit for bit, and the
+    # autotuner's choice could differ between the contexts
+    em, et, eu = _run(g, L, dt, topo, *state, 3, W.PRECISION_FP32_BULK, members=M, shape=W.SHAPE_LAYER_PAIRS)
     own, own_e = g.layout.owned_cells(), g.layout.owned_edge_cells()
     for k, (tp, mass, temp, velo) in enumerate(draws):
-        sm, st, su = _run(g, L, dt, tp, mass, temp, velo, 3, W.PRECISION_FP32_BULK)
+        sm, st, su = _run(g, L, dt, tp, mass, temp, velo, 3, W.PRECISION_FP32_BULK, shape=W.SHAPE_LAYER_PAIRS)
         sl = slice(k * L, (k + 1) * L)
         assert np.array_equal(em[sl][:, own], sm[:, own])
         assert np.array_equal(et[sl][:, own], st[:, own])

@@ -24,7 +24,8 @@ from .layout import Layout
 
 __all__ = ["Grid", "Context", "Hierarchy", "MgParams", "MLP", "Layout", "WtgError", "library_path", "load_library",
            "FS_ETA", "FS_RHS", "FS_ETA_OLD", "FS_DIVF", "FS_DIVF_OLD", "SOLVE_BICGSTAB", "SOLVE_JACOBI", "SOLVE_FMG",
-           "SOLVE_SRJ", "SOLVE_VCYCLE", "SOLVE_LEARNED", "PRECISION_FP64", "PRECISION_FP32_BULK"]
+           "SOLVE_SRJ", "SOLVE_VCYCLE", "SOLVE_LEARNED", "PRECISION_FP64", "PRECISION_FP32_BULK",
+           "SHAPE_AUTO", "SHAPE_LAYER_PAIRS", "SHAPE_EDGE_TASKS", "SHAPE_ONE_LAYER"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libwavetrisk_gpu.so"
@@ -74,6 +75,8 @@ SIGNATURES = {
     "wtg_rk4": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "wtg_rk4_timed": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _D, _D]),
     "wtg_set_precision": (C.c_int, [_P, C.c_int]),
+    "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
+    "wtg_get_stage_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
     "wtg_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
@@ -116,6 +119,8 @@ SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED 
 # precision of Context.rk4 / rk4_timed: FP64 (bit-exact with the reference's
 # RK4 of trend_ml) or FP32 bulk stencils with FP64 column scan and updates
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
+# launch shape of the FP32-bulk stage kernel (AUTO: timed on the first step)
+SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS, SHAPE_ONE_LAYER = 0, 1, 2, 3
 
 
 class _FsParams(C.Structure):
@@ -285,6 +290,17 @@ class Context:
     def set_precision(self, precision: int):
         """PRECISION_FP64 or PRECISION_FP32_BULK for rk4 / rk4_timed."""
         _check(_lib.wtg_set_precision(self._h, precision), self._h, "wtg_set_precision")
+
+    def set_stage_shape(self, shape: int):
+        """SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS or SHAPE_ONE_LAYER
+        for the FP32-bulk stage kernel."""
+        _check(_lib.wtg_set_stage_shape(self._h, shape), self._h, "wtg_set_stage_shape")
+
+    def stage_shape(self):
+        """(shape in use, [tuner ms of shapes 1..3])."""
+        sh, ms = C.c_int(), (C.c_float * 3)()
+        _check(_lib.wtg_get_stage_shape(self._h, C.byref(sh), ms), self._h, "wtg_get_stage_shape")
+        return sh.value, list(ms)
 
     def rk4_timed(self, slot: int, dt: float, nsteps: int):
         tot, ker = C.c_double(), C.c_double()

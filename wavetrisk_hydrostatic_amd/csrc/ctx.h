@@ -66,6 +66,9 @@ struct wtg_ctx {
     float *m = nullptr, *t = nullptr, *u = nullptr;
   };
   SlotF fstage[2];
+  int mixed_shape = WTG_SHAPE_LAYER_PAIRS;  // launch shape of the FP32-bulk stage kernel
+  bool mixed_tuned = false;                 // false: timed on the first FP32-bulk step
+  float tuned_ms[3] = {0.f, 0.f, 0.f};      // the autotuner's times of shapes 1..3
   // free-surface problem
   double* d_lenf = nullptr;      // primal lengths of the flux (owner's for halo edges)
   int64_t* d_ncl = nullptr;      // node compute list: (dst, src) pairs, masters then ghost copies

@@ -1187,10 +1187,11 @@ __device__ __forceinline__ MixedArgs<To> member_args(const MixedArgs<To>& a0) {
 struct GeoSF {
   float len[3][RR], lp[3][RR], area[6][RR], hinv[RR];
 };
-struct LayerSF {  // t holds t/m from phase 3 on
+struct LayerSF {
   double bern[RR], exn[RR];
   double mt[RR], md[RR];  // m - t and m in FP64 for the other group's column scan
   float m[RR], t[RR], u[3][RR];
+  float th[RR];           // potential temperature t/m
   float pvL[RR], pvU[RR];
   float qe[3][RR], hm[3][RR], ht[3][RR];
 };
@@ -1227,9 +1228,68 @@ __device__ __forceinline__ double yval(const double* q, const float* d, int64_t 
   return HasD ? ldc(q + ks, c) + (double)ldc(d + ks, c) : ldc(q + ks, c);
 }
 
+// Velocity trend of edge e of the owned cell at LDS index yy (Qperp, the
+// Bernoulli and Exner gradients), in FP32 from the layer planes; the Qperp
+// weights of the cell (offset e) and of its neighbour (offset e + 3) are built
+// from the area planes
+__device__ __forceinline__ float edge_du(const LayerSF& sl, const GeoSF& gs, int yy, int e) {
+  const int xW = yy - 1, xS = yy - R, xSW = yy - R - 1, xE = yy + 1, xN = yy + R, xNE = yy + R + 1;
+  const int xNW = yy + R - 1, xSE = yy - R + 1;
+  const float th0 = sl.th[yy];
+  const double b0 = sl.bern[yy], x0 = sl.exn[yy];
+  float w0[5], w1[5], qp, the, dB, dX, il;
+#define HM(e, xx) sl.hm[e][xx]
+#define QE(e, xx) sl.qe[e][xx]
+  if (e == 0) {
+    weights_f32(gs, yy, 0, w0);
+    weights_f32(gs, yy + 1, 3, w1);
+    const float qr = QE(0, yy);
+    qp = HM(1, yy) * fint(QE(1, yy), qr) * w0[0] + HM(2, yy) * fint(QE(2, yy), qr) * w0[1] +
+         HM(0, xW) * fint(QE(0, xW), qr) * w0[2] + HM(1, xSW) * fint(QE(1, xSW), qr) * w0[3] +
+         HM(2, xS) * fint(QE(2, xS), qr) * w0[4] + HM(1, xS) * fint(QE(1, xS), qr) * w1[0] +
+         HM(2, xSE) * fint(QE(2, xSE), qr) * w1[1] + HM(0, xE) * fint(QE(0, xE), qr) * w1[2] +
+         HM(1, xE) * fint(QE(1, xE), qr) * w1[3] + HM(2, xE) * fint(QE(2, xE), qr) * w1[4];
+    the = fint(th0, sl.th[xE]);
+    dB = (float)(sl.bern[xE] - b0);
+    dX = (float)(sl.exn[xE] - x0);
+    il = frcp(gs.len[0][yy]);
+  } else if (e == 1) {
+    weights_f32(gs, yy, 1, w0);
+    weights_f32(gs, yy + R + 1, 4, w1);
+    const float qd = QE(1, yy);
+    qp = HM(2, yy) * fint(QE(2, yy), qd) * w0[0] + HM(0, xW) * fint(QE(0, xW), qd) * w0[1] +
+         HM(1, xSW) * fint(QE(1, xSW), qd) * w0[2] + HM(2, xS) * fint(QE(2, xS), qd) * w0[3] +
+         HM(0, yy) * fint(QE(0, yy), qd) * w0[4] + HM(2, xE) * fint(QE(2, xE), qd) * w1[0] +
+         HM(0, xNE) * fint(QE(0, xNE), qd) * w1[1] + HM(1, xNE) * fint(QE(1, xNE), qd) * w1[2] +
+         HM(2, xNE) * fint(QE(2, xNE), qd) * w1[3] + HM(0, xN) * fint(QE(0, xN), qd) * w1[4];
+    the = fint(th0, sl.th[xNE]);
+    dB = (float)(b0 - sl.bern[xNE]);
+    dX = (float)(x0 - sl.exn[xNE]);
+    il = frcp(gs.len[1][yy]);
+  } else {
+    weights_f32(gs, yy, 2, w0);
+    weights_f32(gs, yy + R, 5, w1);
+    const float qu = QE(2, yy);
+    qp = HM(0, xW) * fint(QE(0, xW), qu) * w0[0] + HM(1, xSW) * fint(QE(1, xSW), qu) * w0[1] +
+         HM(2, xS) * fint(QE(2, xS), qu) * w0[2] + HM(0, yy) * fint(QE(0, yy), qu) * w0[3] +
+         HM(1, yy) * fint(QE(1, yy), qu) * w0[4] + HM(0, xN) * fint(QE(0, xN), qu) * w1[0] +
+         HM(1, xN) * fint(QE(1, xN), qu) * w1[1] + HM(2, xN) * fint(QE(2, xN), qu) * w1[2] +
+         HM(0, xNW) * fint(QE(0, xNW), qu) * w1[3] + HM(1, xW) * fint(QE(1, xW), qu) * w1[4];
+    the = fint(th0, sl.th[xN]);
+    dB = (float)(sl.bern[xN] - b0);
+    dX = (float)(sl.exn[xN] - x0);
+    il = frcp(gs.len[2][yy]);
+  }
+#undef HM
+#undef QE
+  return (-qp - dB - the * dX) * il;
+}
+
 // HasD: the stage has an increment (stages 2-4); To: float = the next
-// stage's increment, double = the new state
-template <bool HasD, class To>
+// stage's increment, double = the new state; Edge: phase 4 as edge tasks
+// (each group's 384 threads take the 3 x 256 edge trends, two each, and the
+// first 256 also the node trends) instead of one owned cell per thread
+template <bool HasD, class To, bool Edge>
 __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<To> a_in) {
 #pragma clang fp contract(fast)
   const MixedArgs<To> a = member_args(a_in);
@@ -1342,11 +1402,17 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
   __syncthreads();
 
   float w[6][5];
-  if (own) {
+  if (!Edge && own) {
     weights_f32(gs, y + 1, 3, w[3]);
     weights_f32(gs, y + R + 1, 4, w[4]);
     weights_f32(gs, y + R, 5, w[5]);
   }
+  // edge tasks of this thread (Edge): task t -> edge t / 256 of owned cell t % 256
+  const int tA = lt, tB = lt + GROUP;
+  const int eA = tA / (T * T), eB = tB / (T * T);
+  const int cA = tA % (T * T), cB = tB % (T * T);
+  const int yA = (1 + cA / T) * R + 1 + cA % T, yB = (1 + cB / T) * R + 1 + cB % T;
+  const uint32_t gA = (uint32_t)gcell(1 + cA % T, 1 + cA / T), gB = (uint32_t)gcell(1 + cB % T, 1 + cB / T);
 
   const int niter = (a.L + 1) / 2;
   for (int it = 0; it < niter; ++it) {
@@ -1355,15 +1421,24 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
     const bool pair = 2 * it + 1 < a.L;
     const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
     __syncthreads();
-    // --- 1. prefetch the group's next layer -----------------------------------
+    // --- 1. prefetch the group's next layer (q and d stay apart until phase 5,
+    //        so nothing waits for these loads before then) ---------------------
     double nm, nt, nu0, nu1, nu2;
+    float dm = 0.f, dt = 0.f, du0 = 0.f, du1 = 0.f, du2 = 0.f;
     {
       const int64_t ks2 = (int64_t)min(k + 2, a.L - 1) * NC, ke32 = 3 * ks2;
-      nm = yval<HasD>(a.qm, a.dm, ks2, xc);
-      nt = yval<HasD>(a.qt, a.dt, ks2, xc);
-      nu0 = yval<HasD>(a.qu, a.du, ke32, xc);
-      nu1 = yval<HasD>(a.qu, a.du, ke32 + NC, xc);
-      nu2 = yval<HasD>(a.qu, a.du, ke32 + 2 * NC, xc);
+      nm = ldc(a.qm + ks2, xc);
+      nt = ldc(a.qt + ks2, xc);
+      nu0 = ldc(a.qu + ke32, xc);
+      nu1 = ldc(a.qu + ke32 + NC, xc);
+      nu2 = ldc(a.qu + ke32 + 2 * NC, xc);
+      if (HasD) {
+        dm = ldc(a.dm + ks2, xc);
+        dt = ldc(a.dt + ks2, xc);
+        du0 = ldc(a.du + ke32, xc);
+        du1 = ldc(a.du + ke32 + NC, xc);
+        du2 = ldc(a.du + ke32 + 2 * NC, xc);
+      }
     }
 
     // --- 2. column scan (FP64); PV, fluxes, kinetic energy (FP32) -------------
@@ -1422,9 +1497,14 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
       const int64_t ksb = (int64_t)min(k, a.L - 1) * NC, ke3b = 3 * ksb;
       bm = ldc(a.qm + ksb, ocl);
       bt = ldc(a.qt + ksb, ocl);
-      bu0 = ldc(a.qu + ke3b, ocl);
-      bu1 = ldc(a.qu + ke3b + NC, ocl);
-      bu2 = ldc(a.qu + ke3b + 2 * NC, ocl);
+      if (Edge) {
+        bu0 = ldc(a.qu + ke3b + eA * NC, gA);
+        bu1 = ldc(a.qu + ke3b + eB * NC, gB);
+      } else {
+        bu0 = ldc(a.qu + ke3b, ocl);
+        bu1 = ldc(a.qu + ke3b + NC, ocl);
+        bu2 = ldc(a.qu + ke3b + 2 * NC, ocl);
+      }
     }
     if (lay && active) {
       if (tri_cell) {
@@ -1432,7 +1512,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
         sl.qe[1][x] = fint(sl.pvU[x], sl.pvL[x]);
         if (ri >= 1) sl.qe[2][x] = fint(sl.pvU[x], sl.pvL[x - 1]);
       }
-      sl.t[x] = (float)(t / m);
+      sl.th[x] = (float)(t / m);
     }
     __syncthreads();
 
@@ -1444,7 +1524,23 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
     }
 
     // --- 4. trends of owned nodes and edges, pole master; update -------------
-    if (lay && own) {
+    if (Edge && lay) {
+      if (own) {
+        const int xW = y - 1, xS = y - R, xSW = y - R - 1;
+        const float hi = gs.hinv[y];
+        const float divm =
+            (sl.hm[0][y] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][y] + sl.hm[2][y] - sl.hm[2][xS]) * hi;
+        const float divt =
+            (sl.ht[0][y] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][y] + sl.ht[2][y] - sl.ht[2][xS]) * hi;
+        stc(a.om + ks, oc, (To)(bm - a.h * (double)divm));
+        stc(a.ot + ks, oc, (To)(bt - a.h * (double)divt));
+      }
+      const float duA = edge_du(sl, gs, yA, eA);
+      stc(a.ou + ke3 + eA * NC, gA, (To)(bu0 + a.h * (double)duA));
+      const float duB = edge_du(sl, gs, yB, eB);
+      stc(a.ou + ke3 + eB * NC, gB, (To)(bu1 + a.h * (double)duB));
+    }
+    if (!Edge && lay && own) {
       const int xW = y - 1, xS = y - R, xSW = y - R - 1, xE = y + 1, xN = y + R, xNE = y + R + 1;
       const int xNW = y + R - 1, xSE = y - R + 1;
       const float hi = gs.hinv[y];
@@ -1480,8 +1576,8 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
               HM(0, xNW) * fint(QE(0, xNW), qu) * w[5][3] + HM(1, xW) * fint(QE(1, xW), qu) * w[5][4];
 #undef HM
 #undef QE
-      const float th0 = sl.t[y];
-      const float the[3] = {fint(th0, sl.t[xE]), fint(th0, sl.t[xNE]), fint(th0, sl.t[xN])};
+      const float th0 = sl.th[y];
+      const float the[3] = {fint(th0, sl.th[xE]), fint(th0, sl.th[xNE]), fint(th0, sl.th[xN])};
       const double b0 = sl.bern[y], x0 = sl.exn[y];
       const float il[3] = {frcp(gs.len[0][y]), frcp(gs.len[1][y]), frcp(gs.len[2][y])};
       const float dB[3] = {(float)(sl.bern[xE] - b0), (float)(b0 - sl.bern[xNE]), (float)(sl.bern[xN] - b0)};
@@ -1492,7 +1588,8 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
         stc(a.ou + ke3 + e * NC, oc, (To)(bu[e] + a.h * (double)du));
       }
     }
-    if (kState && !(lay && own)) asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
+    if (kState && !(lay && (own || Edge))) asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
+    if (kState && Edge && lay && !own) asm volatile("" ::"v"(bm), "v"(bt));
     if (lay && pole) {
       const float hi = gs.hinv[x];
       const int xW = x - 1, xS = x - R, xSW = x - R - 1;
@@ -1504,13 +1601,13 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
       stc(a.om + ks, xc, (To)(pm - a.h * (double)divm));
       stc(a.ot + ks, xc, (To)(pt - a.h * (double)divt));
     }
-    __syncthreads();
-    // --- 5. stage the group's next layer --------------------------------------
-    m = nm;
-    t = nt;
-    u0 = (float)nu0;
-    u1 = (float)nu1;
-    u2 = (float)nu2;
+    // --- 5. stage the group's next layer (phase 4 reads none of these planes,
+    //        and the next iteration's barrier orders them before phase 2) ------
+    m = nm + (double)dm;
+    t = nt + (double)dt;
+    u0 = (float)(nu0 + (double)du0);
+    u1 = (float)(nu1 + (double)du1);
+    u2 = (float)(nu2 + (double)du2);
     if (active && k + 2 < a.L) {
       sl.mt[x] = m - t;
       sl.md[x] = m;
@@ -1520,6 +1617,280 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_mixed_kernel(MixedArgs<T
       sl.u[1][x] = u1;
       sl.u[2][x] = u2;
     }
+  }
+}
+
+// One layer at a time per block (the FP32-bulk kernel's second shape): a
+// single group of 384 threads per tile walks the layers bottom to top, so the
+// column scan stays in registers and LDS holds one layer (49 KB): three
+// blocks share a CU and cover each other's barriers, where the layer-pair
+// shape holds one 768-thread block per CU in lock step.
+struct LayerS1 {
+  double bern[RR], exn[RR];
+  float m[RR], t[RR], u[3][RR], th[RR];
+  float pvL[RR], pvU[RR];
+  float qe[3][RR], hm[3][RR], ht[3][RR];
+};
+constexpr int THREADS1 = 384;
+
+template <bool HasD, class To>
+__global__ __launch_bounds__(THREADS1, 5) void trisk_stage_mixed1_kernel(MixedArgs<To> a_in) {
+#pragma clang fp contract(fast)
+  const MixedArgs<To> a = member_args(a_in);
+  constexpr bool kState = sizeof(To) == 8;
+  __shared__ GeoSF gs;
+  __shared__ union {
+    LayerS1 l;
+    double flat[sizeof(LayerS1) / sizeof(double)];
+  } lsu;
+  LayerS1& sl = lsu.l;
+  const int lt = threadIdx.x;
+  const int64_t NC = a.NC;
+  const int S = a.S;
+
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int tpl = a.ntile * a.ntile;
+  const int lz = tile / tpl;
+  const int rem = tile % tpl;
+  const int i0 = (rem % a.ntile) * T, j0 = (rem / a.ntile) * T;
+  const int64_t base = (int64_t)lz * S * S + (int64_t)(j0 - 1 + H) * S + (i0 - 1 + H);
+  auto gcell = [&](int ri, int rj) -> int64_t { return base + (int64_t)rj * S + ri; };
+
+  const bool cSW = (i0 == 0 && j0 == 0), cSE = (i0 + T == a.n && j0 == 0);
+  const bool cNW = (i0 == 0 && j0 + T == a.n), cNE = (i0 + T == a.n && j0 + T == a.n);
+  const bool corner = cSW || cSE || cNW || cNE;
+
+  const bool active = lt < RR;
+  const int ri = lt % R, rj = lt / R;
+  const int x = lt;
+  const uint32_t xc = (uint32_t)(active ? gcell(ri, rj) : base);
+  const bool tri_cell = active && ri <= T + 1 && rj <= T + 1;
+  const bool p_cell = active && ri >= 1 && rj >= 1 && ri <= T + 1 && rj <= T + 1;
+  const bool pole = (lz == 0 && cSE && ri == T + 1 && rj == 1) || (lz == 5 && cNW && ri == 1 && rj == T + 1);
+  const bool own = lt < T * T;
+  const int oi = 1 + lt % T, oj = 1 + lt / T;
+  const int y = oj * R + oi;
+  const uint32_t oc = (uint32_t)gcell(oi, oj);
+  const int lo = lt % (T * T);
+  const uint32_t ocl = (uint32_t)gcell(1 + lo % T, 1 + lo / T);
+
+  float pf0 = 0.f, pf1 = 0.f, pf2 = 0.f, cr0 = 0.f, cr1 = 0.f;
+  if (active) {
+    for (int e = 0; e < 3; ++e) {
+      gs.len[e][x] = ldc(a.f.len + e * NC, xc);
+      gs.lp[e][x] = ldc(a.f.lp + e * NC, xc);
+    }
+    for (int p = 0; p < 6; ++p) gs.area[p][x] = ldc(a.f.area + p * NC, xc);
+    gs.hinv[x] = ldc(a.f.hinv, xc);
+    pf0 = ldc(a.f.pedf, xc);
+    pf1 = ldc(a.f.pedf + NC, xc);
+    pf2 = ldc(a.f.pedf + 2 * NC, xc);
+    cr0 = ldc(a.f.cor, xc);
+    cr1 = ldc(a.f.cor + NC, xc);
+  }
+
+  // column state in FP64 (cal_surf_press, set_surf_geopot)
+  double P = 0, G = 0;
+  {
+    constexpr int KMAX = (int)(sizeof(lsu.flat) / sizeof(double)) / RR;
+    double s = 0.0;
+    for (int k0 = 0; k0 < a.L; k0 += KMAX) {
+      const int kn = min(KMAX, a.L - k0);
+      if (active) {
+        for (int kk = 0; kk < kn; ++kk) {
+          const int64_t ks = (int64_t)(k0 + kk) * NC;
+          lsu.flat[kk * RR + x] = yval<HasD>(a.qm, a.dm, ks, xc) - yval<HasD>(a.qt, a.dt, ks, xc);
+        }
+      }
+      __syncthreads();
+      if (p_cell)
+        for (int kk = 0; kk < kn; ++kk) s = s + lsu.flat[kk * RR + x];
+      __syncthreads();
+    }
+    if (p_cell) {
+      P = a.grav * s + a.p_top;
+      G = a.grav * ldc(a.g.topo, xc);
+    }
+  }
+  const double g_rho = a.grav * a.inv_rho0;
+
+  double m, t;
+  float u0, u1, u2;
+  m = yval<HasD>(a.qm, a.dm, 0, xc);
+  t = yval<HasD>(a.qt, a.dt, 0, xc);
+  u0 = (float)yval<HasD>(a.qu, a.du, 0, xc);
+  u1 = (float)yval<HasD>(a.qu, a.du, NC, xc);
+  u2 = (float)yval<HasD>(a.qu, a.du, 2 * NC, xc);
+
+  // the Qperp weights are rebuilt from LDS in each layer (registers are
+  // what limits this shape to three blocks per CU)
+  float w[6][5];
+
+  for (int k = 0; k < a.L; ++k) {
+    const int64_t ks = (int64_t)k * NC, ke3 = 3 * ks;
+    // --- 1. stage the layer; prefetch the next one and the update's base ------
+    const float mf = (float)m, tf = (float)t;
+    if (active) {
+      sl.m[x] = mf;
+      sl.t[x] = tf;
+      sl.u[0][x] = u0;
+      sl.u[1][x] = u1;
+      sl.u[2][x] = u2;
+    }
+    double nm, nt, nu0, nu1, nu2;
+    float dm = 0.f, dt = 0.f, du0 = 0.f, du1 = 0.f, du2 = 0.f;
+    {
+      const int64_t ks2 = (int64_t)min(k + 1, a.L - 1) * NC, ke32 = 3 * ks2;
+      nm = ldc(a.qm + ks2, xc);
+      nt = ldc(a.qt + ks2, xc);
+      nu0 = ldc(a.qu + ke32, xc);
+      nu1 = ldc(a.qu + ke32 + NC, xc);
+      nu2 = ldc(a.qu + ke32 + 2 * NC, xc);
+      if (HasD) {
+        dm = ldc(a.dm + ks2, xc);
+        dt = ldc(a.dt + ks2, xc);
+        du0 = ldc(a.du + ke32, xc);
+        du1 = ldc(a.du + ke32 + NC, xc);
+        du2 = ldc(a.du + ke32 + 2 * NC, xc);
+      }
+    }
+    double bm = 0, bt = 0, bu0 = 0, bu1 = 0, bu2 = 0;
+    if (kState) {
+      bm = ldc(a.qm + ks, ocl);
+      bt = ldc(a.qt + ks, ocl);
+      bu0 = ldc(a.qu + ke3, ocl);
+      bu1 = ldc(a.qu + ke3 + NC, ocl);
+      bu2 = ldc(a.qu + ke3 + 2 * NC, ocl);
+    }
+    double pr = 0, phi_k = 0;
+    if (p_cell) {
+      double p_upper = P - a.grav * (m - t);
+      pr = 0.5 * (P + p_upper);
+      double gpl = G;
+      G = gpl + g_rho * m;
+      phi_k = 0.5 * (G + gpl);
+      P = p_upper;
+    }
+    __syncthreads();
+
+    // --- 2. PV, fluxes; kinetic energy, Bernoulli, Exner ----------------------
+    if (p_cell) {
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+      const float uS = sl.u[2][xS], uSW = sl.u[1][xSW], uW = sl.u[0][xW];
+      const float ke = (u2 * u2 * gs.lp[2][x] + u1 * u1 * gs.lp[1][x] + u0 * u0 * gs.lp[0][x] +
+                        uS * uS * gs.lp[2][xS] + uSW * uSW * gs.lp[1][xSW] + uW * uW * gs.lp[0][xW]) *
+                       (0.25f * gs.hinv[x]);
+      sl.bern[x] = (double)ke + phi_k + pr * a.inv_rho0;
+      sl.exn[x] = -phi_k;
+    }
+    if (tri_cell) {
+      const int xE = x + 1, xN = x + R, xNE = x + R + 1;
+      const float upRT = u0 * gs.len[0][x], upDG = u1 * gs.len[1][x], upUP = u2 * gs.len[2][x];
+      const float circL = upRT + sl.u[2][xE] * gs.len[2][xE] + upDG;
+      const float circU = -(upDG + upUP + sl.u[0][xN] * gs.len[0][xN]);
+      const float mE = sl.m[xE], mNE = sl.m[xNE], mN = sl.m[xN];
+      const float massL = mf * gs.area[0][x] + mE * gs.area[2][xE] + mNE * gs.area[4][xNE];
+      const float massU = mf * gs.area[1][x] + mNE * gs.area[3][xNE] + mN * gs.area[5][xN];
+      sl.pvL[x] = (cr0 + circL) * frcp(massL);
+      sl.pvU[x] = (cr1 + circU) * frcp(massU);
+      const float f0 = u0 * pf0, f1 = u1 * pf1, f2 = u2 * pf2;
+      sl.hm[0][x] = f0 * fint(mf, mE);
+      sl.hm[1][x] = f1 * fint(mf, mNE);
+      sl.hm[2][x] = f2 * fint(mf, mN);
+      sl.ht[0][x] = f0 * fint(tf, sl.t[xE]);
+      sl.ht[1][x] = f1 * fint(tf, sl.t[xNE]);
+      sl.ht[2][x] = f2 * fint(tf, sl.t[xN]);
+    }
+    if (active) sl.th[x] = (float)(t / m);
+    __syncthreads();
+
+    // --- 3. qe ------------------------------------------------------------------
+    if (tri_cell) {
+      if (rj >= 1) sl.qe[0][x] = fint(sl.pvU[x - R], sl.pvL[x]);
+      sl.qe[1][x] = fint(sl.pvU[x], sl.pvL[x]);
+      if (ri >= 1) sl.qe[2][x] = fint(sl.pvU[x], sl.pvL[x - 1]);
+    }
+    __syncthreads();
+    if (corner) {
+      const bool mine[4] = {cSW, cSE, cNW, cNE};
+      if (lt < 4 && mine[lt])
+        pentagon_fixup(lt, sl.u[0], sl.u[1], sl.u[2], sl.m, sl.qe[0], sl.qe[1], sl.qe[2], a.g, NC, base, S);
+      __syncthreads();
+    }
+
+    // --- 4. trends of owned nodes and edges, pole master; update --------------
+    // (the next layer's staging writes only m, t, u, which this phase does not
+    // read; its barrier orders this phase before the next phase 2)
+    if (own) {
+      const int xW = y - 1, xS = y - R, xSW = y - R - 1, xE = y + 1, xN = y + R, xNE = y + R + 1;
+      const int xNW = y + R - 1, xSE = y - R + 1;
+      const float hi = gs.hinv[y];
+      const float divm =
+          (sl.hm[0][y] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][y] + sl.hm[2][y] - sl.hm[2][xS]) * hi;
+      const float divt =
+          (sl.ht[0][y] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][y] + sl.ht[2][y] - sl.ht[2][xS]) * hi;
+      stc(a.om + ks, oc, (To)(bm - a.h * (double)divm));
+      stc(a.ot + ks, oc, (To)(bt - a.h * (double)divt));
+      float qp[3];
+      weights_f32(gs, y, 0, w[0]);
+      weights_f32(gs, y, 1, w[1]);
+      weights_f32(gs, y, 2, w[2]);
+      weights_f32(gs, y + 1, 3, w[3]);
+      weights_f32(gs, y + R + 1, 4, w[4]);
+      weights_f32(gs, y + R, 5, w[5]);
+#define HM(e, xx) sl.hm[e][xx]
+#define QE(e, xx) sl.qe[e][xx]
+      const float qr = QE(0, y);
+      qp[0] = HM(1, y) * fint(QE(1, y), qr) * w[0][0] + HM(2, y) * fint(QE(2, y), qr) * w[0][1] +
+              HM(0, xW) * fint(QE(0, xW), qr) * w[0][2] + HM(1, xSW) * fint(QE(1, xSW), qr) * w[0][3] +
+              HM(2, xS) * fint(QE(2, xS), qr) * w[0][4] + HM(1, xS) * fint(QE(1, xS), qr) * w[3][0] +
+              HM(2, xSE) * fint(QE(2, xSE), qr) * w[3][1] + HM(0, xE) * fint(QE(0, xE), qr) * w[3][2] +
+              HM(1, xE) * fint(QE(1, xE), qr) * w[3][3] + HM(2, xE) * fint(QE(2, xE), qr) * w[3][4];
+      const float qd = QE(1, y);
+      qp[1] = HM(2, y) * fint(QE(2, y), qd) * w[1][0] + HM(0, xW) * fint(QE(0, xW), qd) * w[1][1] +
+              HM(1, xSW) * fint(QE(1, xSW), qd) * w[1][2] + HM(2, xS) * fint(QE(2, xS), qd) * w[1][3] +
+              HM(0, y) * fint(QE(0, y), qd) * w[1][4] + HM(2, xE) * fint(QE(2, xE), qd) * w[4][0] +
+              HM(0, xNE) * fint(QE(0, xNE), qd) * w[4][1] + HM(1, xNE) * fint(QE(1, xNE), qd) * w[4][2] +
+              HM(2, xNE) * fint(QE(2, xNE), qd) * w[4][3] + HM(0, xN) * fint(QE(0, xN), qd) * w[4][4];
+      const float qu = QE(2, y);
+      qp[2] = HM(0, xW) * fint(QE(0, xW), qu) * w[2][0] + HM(1, xSW) * fint(QE(1, xSW), qu) * w[2][1] +
+              HM(2, xS) * fint(QE(2, xS), qu) * w[2][2] + HM(0, y) * fint(QE(0, y), qu) * w[2][3] +
+              HM(1, y) * fint(QE(1, y), qu) * w[2][4] + HM(0, xN) * fint(QE(0, xN), qu) * w[5][0] +
+              HM(1, xN) * fint(QE(1, xN), qu) * w[5][1] + HM(2, xN) * fint(QE(2, xN), qu) * w[5][2] +
+              HM(0, xNW) * fint(QE(0, xNW), qu) * w[5][3] + HM(1, xW) * fint(QE(1, xW), qu) * w[5][4];
+#undef HM
+#undef QE
+      const float th0 = sl.th[y];
+      const float the[3] = {fint(th0, sl.th[xE]), fint(th0, sl.th[xNE]), fint(th0, sl.th[xN])};
+      const double b0 = sl.bern[y], x0 = sl.exn[y];
+      const float il[3] = {frcp(gs.len[0][y]), frcp(gs.len[1][y]), frcp(gs.len[2][y])};
+      const float dB[3] = {(float)(sl.bern[xE] - b0), (float)(b0 - sl.bern[xNE]), (float)(sl.bern[xN] - b0)};
+      const float dX[3] = {(float)(sl.exn[xE] - x0), (float)(x0 - sl.exn[xNE]), (float)(sl.exn[xN] - x0)};
+      const double bu[3] = {bu0, bu1, bu2};
+      for (int e = 0; e < 3; ++e) {
+        const float du = (-qp[e] - dB[e] - the[e] * dX[e]) * il[e];
+        stc(a.ou + ke3 + e * NC, oc, (To)(bu[e] + a.h * (double)du));
+      }
+    }
+    if (kState && !own) asm volatile("" ::"v"(bm), "v"(bt), "v"(bu0), "v"(bu1), "v"(bu2));
+    if (pole) {
+      const float hi = gs.hinv[x];
+      const int xW = x - 1, xS = x - R, xSW = x - R - 1;
+      const float divm =
+          (sl.hm[0][x] - sl.hm[0][xW] + sl.hm[1][xSW] - sl.hm[1][x] + sl.hm[2][x] - sl.hm[2][xS]) * hi;
+      const float divt =
+          (sl.ht[0][x] - sl.ht[0][xW] + sl.ht[1][xSW] - sl.ht[1][x] + sl.ht[2][x] - sl.ht[2][xS]) * hi;
+      const double pm = kState ? ldc(a.qm + ks, xc) : 0.0, pt = kState ? ldc(a.qt + ks, xc) : 0.0;
+      stc(a.om + ks, xc, (To)(pm - a.h * (double)divm));
+      stc(a.ot + ks, xc, (To)(pt - a.h * (double)divt));
+    }
+    m = nm + (double)dm;
+    t = nt + (double)dt;
+    u0 = (float)(nu0 + (double)du0);
+    u1 = (float)(nu1 + (double)du1);
+    u2 = (float)(nu2 + (double)du2);
   }
 }
 
@@ -1762,9 +2133,52 @@ int launch_stage_mixed(wtg_ctx* c, const wtg_ctx::Slot& q, const wtg_ctx::SlotF*
   a.inv_rho0 = 1.0 / c->params.ref_density;
   a.p_top = c->params.p_top;
   const dim3 grid((unsigned)(10 * a.ntile * a.ntile), (unsigned)c->nmembers);
-  hipLaunchKernelGGL((trisk_stage_mixed_kernel<HasD, To>), grid, dim3(THREADS3), 0, c->stream, a);
+  if (c->mixed_shape == WTG_SHAPE_ONE_LAYER)
+    hipLaunchKernelGGL((trisk_stage_mixed1_kernel<HasD, To>), grid, dim3(THREADS1), 0, c->stream, a);
+  else if (c->mixed_shape == WTG_SHAPE_EDGE_TASKS)
+    hipLaunchKernelGGL((trisk_stage_mixed_kernel<HasD, To, true>), grid, dim3(THREADS3), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((trisk_stage_mixed_kernel<HasD, To, false>), grid, dim3(THREADS3), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+// The autotuner of the FP32-bulk stage: times every launch shape on the
+// context's own state (an intermediate stage, 3 launches each after a warm-up
+// launch) and keeps the fastest.  The launches write only the second FP32
+// increment buffer, which the next RK stage overwrites before reading it.
+int tune_mixed_shape(wtg_ctx* c, const wtg_ctx::Slot& q) {
+  const int shapes[3] = {WTG_SHAPE_LAYER_PAIRS, WTG_SHAPE_EDGE_TASKS, WTG_SHAPE_ONE_LAYER};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail(c, WTG_E_DEVICE, "hipEventCreate failed");
+  }
+  int rc = 0, best = shapes[0];
+  float best_ms = 0.f;
+  auto& f1 = c->fstage[0];
+  auto& f2 = c->fstage[1];
+  for (int i = 0; i < 3 && !rc; ++i) {
+    c->mixed_shape = shapes[i];
+    rc = launch_stage_mixed<true, float>(c, q, &f1, f2.m, f2.t, f2.u, 1.0);
+    if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(c, WTG_E_DEVICE, "event record failed");
+    for (int r = 0; r < 3 && !rc; ++r) rc = launch_stage_mixed<true, float>(c, q, &f1, f2.m, f2.t, f2.u, 1.0);
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      rc = fail(c, WTG_E_DEVICE, "event synchronise failed");
+    float ms = 0.f;
+    if (!rc) hipEventElapsedTime(&ms, e0, e1);
+    c->tuned_ms[i] = ms / 3.0f;
+    if (!rc && (i == 0 || ms < best_ms)) {
+      best_ms = ms;
+      best = shapes[i];
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->mixed_shape = best;
+  c->mixed_tuned = true;
+  return rc;
 }
 
 // FP32-bulk RK4: the low-storage stages with FP32 increments d over the FP64
@@ -1778,6 +2192,8 @@ int launch_stage_mixed(wtg_ctx* c, const wtg_ctx::Slot& q, const wtg_ctx::SlotF*
 int rk4_mixed_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
   if (int rc = f32_ready(c)) return rc;
   const wtg_ctx::Slot& sol = c->slots[slot];
+  if (!c->mixed_tuned && nsteps > 0)
+    if (int rc = tune_mixed_shape(c, sol)) return rc;
   const wtg_ctx::Slot* cur = &sol;
   const wtg_ctx::Slot* nxt = &c->slots[c->nslots];
   auto &f1 = c->fstage[0], &f2 = c->fstage[1];
@@ -2259,6 +2675,29 @@ int wtg_set_precision(wtg_ctx* c, int precision) {
   if (precision == WTG_PRECISION_FP32_BULK)
     if (int rc = f32_ready(c)) return rc;
   c->precision = precision;
+  return 0;
+}
+
+int wtg_set_stage_shape(wtg_ctx* c, int shape) {
+  if (!c) return WTG_E_ARG;
+  if (shape == WTG_SHAPE_AUTO) {
+    c->mixed_tuned = false;
+    c->mixed_shape = WTG_SHAPE_LAYER_PAIRS;
+    return 0;
+  }
+  if (shape != WTG_SHAPE_LAYER_PAIRS && shape != WTG_SHAPE_EDGE_TASKS && shape != WTG_SHAPE_ONE_LAYER)
+    return fail(c, WTG_E_ARG, "unknown stage shape %d", shape);
+  c->mixed_shape = shape;
+  c->mixed_tuned = true;
+  c->tuned_ms[0] = c->tuned_ms[1] = c->tuned_ms[2] = 0.f;
+  return 0;
+}
+
+int wtg_get_stage_shape(wtg_ctx* c, int* shape, float* ms) {
+  if (!c || !shape) return WTG_E_ARG;
+  *shape = c->mixed_tuned ? c->mixed_shape : WTG_SHAPE_AUTO;
+  if (ms)
+    for (int i = 0; i < 3; ++i) ms[i] = c->tuned_ms[i];
   return 0;
 }
 
