@@ -127,6 +127,7 @@ def main():
     bulk = fp32_bulk(grid, topo, mass, temp, velo, L, args.dt, args.warmup, args.steps, m_out, u_out,
                      dof_per_step, ms_per_step)
     ens = None if args.no_ensemble else ensemble(grid, topo, mass, temp, velo, L, args.dt, args.members)
+    small = None if args.no_cpu else shallow_water_c1()
     ctx.synchronize()
     split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
     fs = None if args.no_free_surface else free_surface(args.fs_level)
@@ -163,6 +164,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp32_bulk": bulk,
+        "shallow_water_J5": small,
         "ensemble": ens,
         "split_step": split,
         "free_surface": fs,
@@ -218,6 +220,50 @@ def fp32_bulk(grid, topo, mass, temp, velo, L, dt, warmup, steps, m64, u64, dof_
                                                 np.max(np.abs(u64[:, :, own_e])))},
             "note": "PV, fluxes, kinetic energy, divergence, Qperp in FP32; column scan, Bernoulli/Exner "
                     "gradients and state updates FP64; RK stages as FP32 increments over the FP64 state"}
+
+
+def shallow_water_c1(nsteps=100):
+    """The reference's CPU-runnable case: single-layer shallow water (TC5,
+    zonal flow over a mountain) on the uniform J5 grid, 100 RK4 steps at
+    CFL 0.8; launch-bound on the GPU, so the steps are also timed replayed
+    from a captured HIP graph.  The CPU oracle runs the same 100 steps."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+    from oracle.oracle import Oracle, geometry_of
+
+    print("bench: J5 shallow water, eager and graph launches", file=sys.stderr, flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    topo, m, t, u = cases.tc5(g, 1)
+    own_e = g.layout.owned_edge_cells()
+    depth = float(np.max(m.sum(0))) / 1028.0
+    dt = 0.8 * float(g.array("len")[:, own_e].min()) / (float(np.sqrt(9.80616 * depth)) + 20.0)
+    dof = g.layout.n_dof_per_layer
+    out = {"workload": f"J5 TC5 single layer, {nsteps} RK4 steps, dt {dt:.1f} s (CFL 0.8)", "dof_per_step": dof}
+    for name, prec, graph in (("fp64_eager", W.PRECISION_FP64, False), ("fp64_graph", W.PRECISION_FP64, True),
+                              ("fp32_bulk_graph", W.PRECISION_FP32_BULK, True)):
+        ctx = W.Context(g, nlayers=1, nslots=1)
+        ctx.set_topography(topo)
+        ctx.upload_state(0, m, t, u)
+        ctx.set_precision(prec)
+        ctx.set_graph(graph)
+        ctx.rk4(0, dt, 4)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.rk4(0, dt, nsteps)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        ctx.close()
+        out[name] = {"ms_per_step": ms, "dof_updates_per_s": dof / (ms / 1e3)}
+    o = Oracle(g.layout, geometry_of(g), topo)
+    mm, tt, uu = m.copy(), t.copy(), u.copy()
+    o.ghost_fill(mm, tt, uu)
+    t0 = time.perf_counter()
+    o.rk4(mm, tt, uu, dt, nsteps)
+    secs = time.perf_counter() - t0
+    out["cpu_oracle_1_thread"] = {"ms_per_step": secs * 1e3 / nsteps, "dof_updates_per_s": dof * nsteps / secs}
+    return out
 
 
 def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10):

@@ -168,6 +168,13 @@ int wtg_set_precision(wtg_ctx* ctx, int precision);
 int wtg_set_stage_shape(wtg_ctx* ctx, int shape);
 int wtg_get_stage_shape(wtg_ctx* ctx, int* shape, float* ms);
 
+/* enable != 0: wtg_rk4 replays its steps from a HIP graph captured on the
+ * context's stream (one step, or two in FP32-bulk mode, per graph; captured
+ * on first use and again when slot, dt, precision or launch shape change).
+ * Results are identical to the eager launches.  No counterpart in the
+ * reference.  wtg_rk4_timed always launches eagerly (it times each stage). */
+int wtg_set_graph(wtg_ctx* ctx, int enable);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "wtg_rk4_timed": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _D, _D]),
     "wtg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
+    "wtg_set_graph": (C.c_int, [_P, C.c_int]),
     "wtg_get_stage_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
@@ -290,6 +291,10 @@ class Context:
     def set_precision(self, precision: int):
         """PRECISION_FP64 or PRECISION_FP32_BULK for rk4 / rk4_timed."""
         _check(_lib.wtg_set_precision(self._h, precision), self._h, "wtg_set_precision")
+
+    def set_graph(self, enable: bool):
+        """rk4 replays its steps from a captured HIP graph (identical results)."""
+        _check(_lib.wtg_set_graph(self._h, int(bool(enable))), self._h, "wtg_set_graph")
 
     def set_stage_shape(self, shape: int):
         """SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS or SHAPE_ONE_LAYER

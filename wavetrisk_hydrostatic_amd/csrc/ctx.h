@@ -69,6 +69,11 @@ struct wtg_ctx {
   int mixed_shape = WTG_SHAPE_LAYER_PAIRS;  // launch shape of the FP32-bulk stage kernel
   bool mixed_tuned = false;                 // false: timed on the first FP32-bulk step
   float tuned_ms[3] = {0.f, 0.f, 0.f};      // the autotuner's times of shapes 1..3
+  // wtg_rk4 from a captured HIP graph (wtg_set_graph) and the key it was captured for
+  bool use_graph = false;
+  hipGraphExec_t rk_graph = nullptr;
+  int g_slot = -1, g_precision = -1, g_shape = -1;
+  double g_dt = 0.0;
   // free-surface problem
   double* d_lenf = nullptr;      // primal lengths of the flux (owner's for halo edges)
   int64_t* d_ncl = nullptr;      // node compute list: (dst, src) pairs, masters then ghost copies

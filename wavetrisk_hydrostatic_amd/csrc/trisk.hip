@@ -2189,16 +2189,11 @@ int tune_mixed_shape(wtg_ctx* c, const wtg_ctx::Slot& q) {
 // stage buffer and is copied back after an odd number of steps.  Halo slots
 // that are never ghost filled keep d = 0 (the buffers start zeroed and
 // stages write owned cells and pole masters only).
-int rk4_mixed_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
-  if (int rc = f32_ready(c)) return rc;
+int rk4_mixed_steps(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
   const wtg_ctx::Slot& sol = c->slots[slot];
-  if (!c->mixed_tuned && nsteps > 0)
-    if (int rc = tune_mixed_shape(c, sol)) return rc;
   const wtg_ctx::Slot* cur = &sol;
   const wtg_ctx::Slot* nxt = &c->slots[c->nslots];
   auto &f1 = c->fstage[0], &f2 = c->fstage[1];
-  if (nsteps > 0)
-    if (int rc = launch_copy_frozen(c, sol, *nxt)) return rc;
   int launch = 0;
   auto rec = [&](int k) -> int {
     if (ev) HIPCHK(c, hipEventRecord(ev[2 * launch + k], c->stream));
@@ -2235,18 +2230,33 @@ int rk4_mixed_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) 
   return 0;
 }
 
-// nsteps RK4 steps; when ev is given, ev[2i] / ev[2i+1] are recorded around
-// the i-th stage-kernel launch (no synchronisation between launches)
-int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
+// Set-up of RK4 steps on `slot`, outside any graph capture: the frozen halo
+// slots of the stage buffers; in FP32-bulk mode also the FP32 buffers and the
+// launch-shape tuning
+int rk4_prepare(wtg_ctx* c, int slot, int nsteps) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "nsteps < 0");
-  if (c->precision == WTG_PRECISION_FP32_BULK) return rk4_mixed_impl(c, slot, dt, nsteps, ev);
+  const auto& sol = c->slots[slot];
+  if (c->precision == WTG_PRECISION_FP32_BULK) {
+    if (int rc = f32_ready(c)) return rc;
+    if (nsteps == 0) return 0;
+    if (!c->mixed_tuned)
+      if (int rc = tune_mixed_shape(c, sol)) return rc;
+    return launch_copy_frozen(c, sol, c->slots[c->nslots]);
+  }
+  if (int rc = launch_copy_frozen(c, sol, c->slots[c->nslots])) return rc;
+  return launch_copy_frozen(c, sol, c->slots[c->nslots + 1]);
+}
+
+// nsteps RK4 steps (kernels only, after rk4_prepare); when ev is given,
+// ev[2i] / ev[2i+1] are recorded around the i-th stage-kernel launch (no
+// synchronisation between launches)
+int rk4_steps(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
+  if (c->precision == WTG_PRECISION_FP32_BULK) return rk4_mixed_steps(c, slot, dt, nsteps, ev);
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
   const auto& w2 = c->slots[c->nslots + 1];
-  if (int rc = launch_copy_frozen(c, sol, w1)) return rc;
-  if (int rc = launch_copy_frozen(c, sol, w2)) return rc;
   const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
   int launch = 0;
   for (int s = 0; s < nsteps; ++s) {
@@ -2259,6 +2269,53 @@ int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
       if (int rc = launch_ghost_fill(c, *os[st])) return rc;
     }
   }
+  return 0;
+}
+
+int rk4_impl(wtg_ctx* c, int slot, double dt, int nsteps, hipEvent_t* ev) {
+  if (int rc = rk4_prepare(c, slot, nsteps)) return rc;
+  return rk4_steps(c, slot, dt, nsteps, ev);
+}
+
+// RK4 steps replayed from a captured HIP graph: one step (FP64) or two
+// (FP32-bulk, whose state returns to the slot every second step) per graph;
+// a remaining odd FP32-bulk step runs eagerly.  The graph holds kernel nodes
+// only and is captured again when the slot, dt, precision or shape changes.
+int rk4_graph_impl(wtg_ctx* c, int slot, double dt, int nsteps) {
+  if (int rc = rk4_prepare(c, slot, nsteps)) return rc;
+  const int per = c->precision == WTG_PRECISION_FP32_BULK ? 2 : 1;
+  const int ng = nsteps / per, rest = nsteps % per;
+  if (ng > 0) {
+    const bool same = c->rk_graph && c->g_slot == slot && c->g_dt == dt && c->g_precision == c->precision &&
+                      c->g_shape == c->mixed_shape;
+    if (!same) {
+      if (c->rk_graph) {
+        hipGraphExecDestroy(c->rk_graph);
+        c->rk_graph = nullptr;
+      }
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = rk4_steps(c, slot, dt, per, nullptr);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ec != hipSuccess) return fail(c, WTG_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&c->rk_graph, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        c->rk_graph = nullptr;
+        return fail(c, WTG_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+      }
+      c->g_slot = slot;
+      c->g_dt = dt;
+      c->g_precision = c->precision;
+      c->g_shape = c->mixed_shape;
+    }
+    for (int i = 0; i < ng; ++i) HIPCHK(c, hipGraphLaunch(c->rk_graph, c->stream));
+  }
+  if (rest) return rk4_steps(c, slot, dt, rest, nullptr);
   return 0;
 }
 
@@ -2547,6 +2604,7 @@ int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmem
 void wtg_destroy(wtg_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->rk_graph) hipGraphExecDestroy(c->rk_graph);
   for (void* p : c->allocs) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -2626,8 +2684,15 @@ int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter
   return 0;
 }
 
+int wtg_set_graph(wtg_ctx* c, int enable) {
+  if (!c) return WTG_E_ARG;
+  c->use_graph = enable != 0;
+  return 0;
+}
+
 int wtg_rk4(wtg_ctx* c, int slot, double dt, int nsteps) {
-  if (int rc = rk4_impl(c, slot, dt, nsteps, nullptr)) return rc;
+  if (!c) return WTG_E_ARG;
+  if (int rc = c->use_graph ? rk4_graph_impl(c, slot, dt, nsteps) : rk4_impl(c, slot, dt, nsteps, nullptr)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
