@@ -235,6 +235,12 @@ int wtg_rhs_elliptic(wtg_ctx* ctx);
  * (max_iter + 1 if rel_tol was not reached), *rel_resid the last relative
  * residual. */
 int wtg_elliptic_solve(wtg_ctx* ctx, int method, double rel_tol, int max_iter, int* iters, double* rel_resid);
+/* enable != 0: BiCGSTAB replays each full chunk of its gated iterations
+ * (8 iterations, 96 kernels) from a HIP graph captured on first use, instead
+ * of launching the kernels one by one; the iterates and the iteration count
+ * are those of the eager launches.  0 (default): eager launches.  No
+ * counterpart in the reference. */
+int wtg_fs_set_graph(wtg_ctx* ctx, int enable);
 
 /* dt_step_split => RK4_split (time_integr.f90:360-382, main.f90:209): nsteps
  * split-explicit steps of size dt (wtg_fs_set_params) on state slot `slot`

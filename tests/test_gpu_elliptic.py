@@ -102,6 +102,32 @@ def test_bicgstab_matches_reference(name):
     assert o.l2(r) / o.l2(rhs) <= tol * (1 + 1e-6)
 
 
+@pytest.mark.parametrize("name", ["J4", "J5"])
+def test_graph_bicgstab_equals_eager_launches(name):
+    """BiCGSTAB chunks replayed from a captured graph (wtg_fs_set_graph) against
+    eager launches: same iterates, residual and iteration count, bit for bit,
+    over repeated solves (graph replays) and a solve stopped by max_iter."""
+    print(f"graph BiCGSTAB {name}", flush=True)
+    out = {}
+    for graph in (True, False):
+        f, g, o, ctx, masters = setup(name)
+        ctx.fs_set_graph(graph)
+        eo = o.node_ghost(np.nan_to_num(f["eta_old"]))
+        rhs = o.node_ghost(np.nan_to_num(f["rhs"]))
+        runs = []
+        for tol, maxit in ((float(f["tol"]), 400), (1e-14, 3), (float(f["tol"]), 400), (1e-14, 20)):
+            ctx.fs_upload(W.FS_ETA_OLD, eo)
+            ctx.fs_upload(W.FS_ETA, eo)
+            ctx.fs_upload(W.FS_RHS, rhs)
+            iters, res = ctx.elliptic_solve(W.SOLVE_BICGSTAB, tol, maxit)
+            runs.append((iters, res, ctx.fs_download(W.FS_ETA)))
+        out[graph] = runs
+    for (i1, r1, s1), (i2, r2, s2) in zip(out[True], out[False]):
+        assert i1 == i2 and r1 == r2
+        assert np.array_equal(s1, s2)
+    assert out[True][1][0] == 4  # max_iter + 1: not converged
+
+
 def test_jacobi_sweeps_equal_oracle_J4():
     f, g, o, ctx, masters = setup("J4")
     eo = o.node_ghost(np.nan_to_num(f["eta_old"]))

@@ -77,6 +77,7 @@ SIGNATURES = {
     "wtg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
     "wtg_set_graph": (C.c_int, [_P, C.c_int]),
+    "wtg_fs_set_graph": (C.c_int, [_P, C.c_int]),
     "wtg_get_stage_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
@@ -291,6 +292,10 @@ class Context:
     def set_precision(self, precision: int):
         """PRECISION_FP64 or PRECISION_FP32_BULK for rk4 / rk4_timed."""
         _check(_lib.wtg_set_precision(self._h, precision), self._h, "wtg_set_precision")
+
+    def fs_set_graph(self, enable: bool):
+        """BiCGSTAB chunks replayed from a captured HIP graph (default off)."""
+        _check(_lib.wtg_fs_set_graph(self._h, int(bool(enable))), self._h, "wtg_fs_set_graph")
 
     def set_graph(self, enable: bool):
         """rk4 replays its steps from a captured HIP graph (identical results)."""

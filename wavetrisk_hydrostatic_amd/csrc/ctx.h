@@ -85,9 +85,21 @@ struct wtg_ctx {
   bool fs_params_set = false;
   double* fs[FS_NFIELDS] = {};   // allocated on first use
   double* d_part = nullptr;      // reduction partials
+  unsigned* d_count = nullptr;   // blocks finished in a dot pass (the last one reduces)
+  // dot passes of at most this many blocks finish in their last block (one
+  // launch); larger ones use a final one-block kernel, because the last
+  // block's device-scope fence costs L2 write-backs that grow with the grid
+  int lastblock_max = 64;
   double* d_scal = nullptr;      // device scalars of the solvers
   double* h_scal = nullptr;      // pinned host copy (hipHostMalloc)
   int part_blocks = 0;
+  // BiCGSTAB chunks of gated iterations replayed from a captured graph
+  // (wtg_fs_set_graph), keyed by the solution and right-hand side arrays
+  bool fs_graph = false;
+  hipGraphExec_t bicg_graph = nullptr;
+  const double* bicg_u = nullptr;
+  const double* bicg_f = nullptr;
+  double bicg_key[2] = {0.0, 0.0};  // operator coefficient and gravity of the captured graph
   std::vector<void*> allocs;
 };
 

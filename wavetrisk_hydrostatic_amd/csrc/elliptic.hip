@@ -297,28 +297,179 @@ __global__ void fs_scalar_kernel(int op, int gated_loop, double arg, double* __r
   scalar_op(op, gated_loop, arg, s);
 }
 
-// u = 1 u + 1 (alph p + omga s) on the masters, in place (bicgstab's
-// corr = alph p + omga s; u = u + corr, fused)
-__global__ void fs_bicg_u_kernel(const double* __restrict__ gate, int nmaster, const int64_t* __restrict__ ncl,
-                                 double* __restrict__ u, const double* __restrict__ p, const double* __restrict__ sv,
-                                 const double* __restrict__ scal) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nmaster || gated(gate)) return;
-  const int64_t c = ncl[2 * t + 1];
-  const double corr = (1.0 * scal[SC_ALPH]) * p[c] + (1.0 * scal[SC_OMGA]) * sv[c];
-  u[c] = 1.0 * u[c] + 1.0 * corr;
+
+// ---------------------------------------------------------------------------
+// One-launch dot products and the fused BiCGSTAB steps
+// ---------------------------------------------------------------------------
+// A dot pass writes one partial sum per block (fs_dot_kernel's order), and
+// the block that finishes last -- counted with an atomic, no block waits --
+// reduces the partials in fs_dot_final_kernel's order and runs the scalar
+// step.  The partials are read with device-scope loads (they were written by
+// other compute units).  The counter is reset by that last block.
+
+// tree over this block's RED_MAX accumulators; thread k < RED_MAX writes part[b][k]
+__device__ __forceinline__ void dot_block_tree(double acc[RED_MAX], double (*sh)[FS_NT], double* part, int b) {
+  for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = FS_NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (part && threadIdx.x < RED_MAX) part[b * RED_MAX + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// p_new = res + b (p - omga Ap) (bicgstab's corr = p - omga Ap; p = res + b corr, fused), out of place
-__global__ void fs_bicg_p_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl,
-                                 const double* __restrict__ res, const double* __restrict__ p,
-                                 const double* __restrict__ Ap, const double* __restrict__ scal,
-                                 double* __restrict__ pn) {
+__device__ __forceinline__ void dot_last_block(int nbred, int nd, const double* part, int o0, int o1, int o2,
+                                               double* __restrict__ scal, int post_op, int gated_loop,
+                                               unsigned* count, double (*sh)[FS_NT]) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(count, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double acc[RED_MAX] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nbred; b += FS_NT)
+    for (int k = 0; k < RED_MAX; ++k)
+      acc[k] = acc[k] + __hip_atomic_load(part + b * RED_MAX + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  dot_block_tree(acc, sh, nullptr, 0);
+  if (threadIdx.x == 0) {
+    const int o[RED_MAX] = {o0, o1, o2};
+    for (int k = 0; k < nd; ++k) scal[o[k]] = sh[k][0];
+    if (post_op != OP_NONE) scalar_op(post_op, gated_loop, 0.0, scal);
+    atomicExch(count, 0u);
+  }
+}
+
+// weighted dot products into scal[o_k], then post_op (one launch)
+__global__ __launch_bounds__(FS_NT) void fs_dot1_kernel(const double* __restrict__ gate, int gated_loop, int nred,
+                                                        const int64_t* __restrict__ rcell,
+                                                        const double* __restrict__ rw, int nd,
+                                                        const double* __restrict__ a0, const double* __restrict__ b0,
+                                                        const double* __restrict__ a1, const double* __restrict__ b1,
+                                                        const double* __restrict__ a2, const double* __restrict__ b2,
+                                                        double* __restrict__ part, int o0, int o1, int o2,
+                                                        double* __restrict__ scal, int post_op, unsigned* count) {
+  if (gated(gate)) return;
+  __shared__ double sh[RED_MAX][FS_NT];
+  const double* A[RED_MAX] = {a0, a1, a2};
+  const double* B[RED_MAX] = {b0, b1, b2};
+  double acc[RED_MAX] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * FS_NT + threadIdx.x; i < nred; i += gridDim.x * FS_NT) {
+    const int64_t c = rcell[i];
+    const double w = rw[i];
+    for (int k = 0; k < RED_MAX; ++k)
+      if (k < nd) acc[k] = acc[k] + w * (A[k][c] * B[k][c]);
+  }
+  dot_block_tree(acc, sh, part, blockIdx.x);
+  if (count) dot_last_block(gridDim.x, nd, part, o0, o1, o2, scal, post_op, gated_loop, count, sh);
+}
+
+// elliptic_lo at cell c from the values q7 of the field at c and its six
+// neighbours {c, E, NE, N, W, SW, S}: the expression of ell_apply_at
+__device__ __forceinline__ double ell_flux_v(const FsGeo& g, const double* __restrict__ eo, double grav, int64_t x,
+                                             int e, double qx, double qn) {
+  const int64_t nb = e == 0 ? x + 1 : (e == 1 ? x + g.S + 1 : x + g.S);
+  const double cx = grav * (fabs(g.topo[x]) * 1.0 + eo[x]);
+  const double cn = grav * (fabs(g.topo[nb]) * 1.0 + eo[nb]);
+  const double d = e == 1 ? (qx / 1.0 - qn / 1.0) : (qn / 1.0 - qx / 1.0);
+  return interp(cx, cn) * d / g.lenf[e * g.NC + x] * g.pedf[e * g.NC + x];
+}
+__device__ __forceinline__ double ell_apply_v(const FsGeo& g, const double* __restrict__ eo, double grav,
+                                              double coef, int64_t c, const double q7[7]) {
+  const int64_t cW = c - 1, cSW = c - g.S - 1, cS = c - g.S;
+  const double dv = (ell_flux_v(g, eo, grav, c, 0, q7[0], q7[1]) - ell_flux_v(g, eo, grav, cW, 0, q7[4], q7[0]) +
+                     ell_flux_v(g, eo, grav, cSW, 1, q7[5], q7[0]) - ell_flux_v(g, eo, grav, c, 1, q7[0], q7[2]) +
+                     ell_flux_v(g, eo, grav, c, 2, q7[0], q7[3]) - ell_flux_v(g, eo, grav, cS, 2, q7[6], q7[0])) *
+                    g.hinv[c];
+  return coef * dv - q7[0];
+}
+__device__ __forceinline__ void cells7(const FsGeo& g, int64_t c, int64_t x7[7]) {
+  x7[0] = c;
+  x7[1] = c + 1;
+  x7[2] = c + g.S + 1;
+  x7[3] = c + g.S;
+  x7[4] = c - 1;
+  x7[5] = c - g.S - 1;
+  x7[6] = c - g.S;
+}
+
+// s = res - alph Ap and As = L s in one pass (s is evaluated at the stencil's
+// seven cells; res and Ap are ghost filled, so halo values equal the owners')
+__global__ void fs_bicg_s_apply_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl,
+                                       FsGeo g, const double* __restrict__ res, const double* __restrict__ Ap,
+                                       const double* __restrict__ eo, double grav, double coef,
+                                       const double* __restrict__ scal, double* __restrict__ s_out,
+                                       double* __restrict__ As_out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || gated(gate)) return;
-  const int64_t c = ncl[2 * t + 1];
-  const double corr = 1.0 * p[c] + (-1.0 * scal[SC_OMGA]) * Ap[c];
-  pn[ncl[2 * t]] = 1.0 * res[c] + (1.0 * scal[SC_B]) * corr;
+  const double a2 = -1.0 * scal[SC_ALPH];
+  const int64_t c = ncl[2 * t + 1], dst = ncl[2 * t];
+  int64_t x7[7];
+  cells7(g, c, x7);
+  double q7[7];
+  for (int k = 0; k < 7; ++k) q7[k] = 1.0 * res[x7[k]] + a2 * Ap[x7[k]];
+  s_out[dst] = q7[0];
+  As_out[dst] = ell_apply_v(g, eo, grav, coef, c, q7);
+}
+
+// p_new = res + b (p - omga Ap) and Ap_new = L p_new in one pass
+__global__ void fs_bicg_p_apply_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl,
+                                       FsGeo g, const double* __restrict__ res, const double* __restrict__ p,
+                                       const double* __restrict__ Ap, const double* __restrict__ eo, double grav,
+                                       double coef, const double* __restrict__ scal, double* __restrict__ pn_out,
+                                       double* __restrict__ Apn_out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || gated(gate)) return;
+  const double bb = 1.0 * scal[SC_B], mo = -1.0 * scal[SC_OMGA];
+  const int64_t c = ncl[2 * t + 1], dst = ncl[2 * t];
+  int64_t x7[7];
+  cells7(g, c, x7);
+  double q7[7];
+  for (int k = 0; k < 7; ++k) {
+    const double corr = 1.0 * p[x7[k]] + mo * Ap[x7[k]];
+    q7[k] = 1.0 * res[x7[k]] + bb * corr;
+  }
+  pn_out[dst] = q7[0];
+  Apn_out[dst] = ell_apply_v(g, eo, grav, coef, c, q7);
+}
+
+// u = u + (alph p + omga s) on the masters, res = s - omga As, and the dots
+// (res, res) -> T0, (res0, res) -> T3 with err, the convergence test and the
+// next rho and b (OP_ERR_BETA) in one launch; res is evaluated where the dots
+// read it.  Blocks b < nbred also reduce (the partition of fs_dot1_kernel).
+__global__ __launch_bounds__(FS_NT) void fs_bicg_res_kernel(
+    const double* __restrict__ gate, int nmaster, int nncl, const int64_t* __restrict__ ncl, int nred, int nbred,
+    const int64_t* __restrict__ rcell, const double* __restrict__ rw, double* __restrict__ u,
+    const double* __restrict__ p, const double* __restrict__ s, const double* __restrict__ As,
+    const double* __restrict__ res0, double* __restrict__ scal, double* __restrict__ res_out,
+    double* __restrict__ part, unsigned* count) {
+  if (gated(gate)) return;
+  __shared__ double sh[RED_MAX][FS_NT];
+  const double alph = scal[SC_ALPH], omga = scal[SC_OMGA];
+  const int i = blockIdx.x * FS_NT + threadIdx.x;
+  if (i < nmaster) {
+    const int64_t c = ncl[2 * i + 1];
+    const double corr = (1.0 * alph) * p[c] + (1.0 * omga) * s[c];
+    u[c] = 1.0 * u[c] + 1.0 * corr;
+  }
+  const double mo = -1.0 * omga;
+  if (i < nncl) {
+    const int64_t c = ncl[2 * i + 1];
+    res_out[ncl[2 * i]] = 1.0 * s[c] + mo * As[c];
+  }
+  double acc[RED_MAX] = {0.0, 0.0, 0.0};
+  if ((int)blockIdx.x < nbred)
+    for (int j = blockIdx.x * FS_NT + threadIdx.x; j < nred; j += nbred * FS_NT) {
+      const int64_t c = rcell[j];
+      const double w = rw[j];
+      const double r = 1.0 * s[c] + mo * As[c];
+      acc[0] = acc[0] + w * (r * r);
+      acc[1] = acc[1] + w * (res0[c] * r);
+    }
+  dot_block_tree(acc, sh, (int)blockIdx.x < nbred ? part : nullptr, blockIdx.x);
+  if (count) dot_last_block(nbred, 2, part, SC_T0, SC_T3, 0, scal, OP_ERR_BETA, 1, count, sh);
 }
 
 // ===========================================================================
@@ -534,6 +685,8 @@ int fs_ready(wtg_ctx* c) {
     WTG_HIPCHK(c, hipMemsetAsync(c->fs[f], 0, NC * sizeof(double), c->stream));
   }
   if (int rc = wtg_dalloc<double>(c, &c->d_part, (size_t)RED_BLOCKS * RED_MAX)) return rc;
+  if (int rc = wtg_dalloc<unsigned>(c, &c->d_count, 1)) return rc;
+  WTG_HIPCHK(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned), c->stream));
   if (int rc = wtg_dalloc<double>(c, &c->d_scal, SC_NSCAL)) return rc;
   WTG_HIPCHK(c, hipMemsetAsync(c->d_scal, 0, SC_NSCAL * sizeof(double), c->stream));
   WTG_HIPCHK(c, hipHostMalloc((void**)&c->h_scal, SC_NSCAL * sizeof(double), hipHostMallocDefault));
@@ -594,6 +747,13 @@ int launch_dots_g(wtg_ctx* c, bool gate, int post_op, int nd, const double* a0, 
                   const double* a1 = nullptr, const double* b1 = nullptr, int o1 = 0, const double* a2 = nullptr,
                   const double* b2 = nullptr, int o2 = 0) {
   const int nb = std::min(RED_BLOCKS, (int)nblk(c->nred));
+  if (nb <= c->lastblock_max) {
+    hipLaunchKernelGGL(fs_dot1_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr, gate ? 1 : 0,
+                       c->nred, c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0,
+                       b2 ? b2 : b0, c->d_part, o0, o1, o2, c->d_scal, post_op, c->d_count);
+    WTG_HIPCHK(c, hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(fs_dot_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr, c->nred,
                      c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0, b2 ? b2 : b0,
                      c->d_part);
@@ -632,7 +792,11 @@ int read_scalar(wtg_ctx* c, int i, double* out) {
 // iterations are queued in chunks without host synchronisation: every kernel
 // of an iteration is gated by the device flag that the convergence test sets,
 // so the iterations queued after convergence do nothing and the iterates and
-// the iteration count are those of the sequential loop.  u is updated on its
+// the iteration count are those of the sequential loop.  An iteration is five
+// launches: dp(Ap, res0) -> alph; s = res - alph Ap with As = L s; dp(As, s),
+// dp(As, As) -> omga; the update of u, res = s - omga As and its two dot
+// products -> err, rho, b; p = res + b (p - omga Ap) with Ap = L p (the dot
+// passes of large levels add a one-block final kernel).  u is updated on its
 // masters in place and its halo copies refreshed after the loop.
 constexpr int BICG_CHUNK = 8;
 
@@ -651,24 +815,72 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   RC(launch_scalar(c, OP_LOOP_INIT, false, tol));
   int queued = 0;
   bool done = false;
-  while (queued < maxit && !done) {
-    const int k = std::min(BICG_CHUNK, maxit - queued);
+  // one chunk of gated iterations; the pointer swap of p returns to its
+  // start after an even count, so a full chunk can be replayed from a graph
+  double *Ap_alt = W[7];
+  const int nbred = std::min(RED_BLOCKS, (int)nblk(c->nred));
+  auto chunk = [&](int k) -> int {
     for (int i = 0; i < k; ++i) {
       RC(launch_dots_g(c, true, OP_ALPH, 1, Ap, res0, SC_T0));
-      RC(launch_lc(c, 1.0, -1, res, -1.0, SC_ALPH, Ap, s, true));  // s = res - alph Ap
-      RC(launch_apply(c, s, As, true));
-      RC(launch_dots_g(c, true, OP_OMGA, 2, As, s, SC_T1, As, As, SC_T2));
-      hipLaunchKernelGGL(fs_bicg_u_kernel, dim3(nblk(c->nmaster)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nmaster,
-                         c->d_ncl, u, p, s, c->d_scal);  // u = u + (alph p + omga s)
+      hipLaunchKernelGGL(fs_bicg_s_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
+                         c->d_ncl, fs_geo(c), res, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->d_scal, s,
+                         As);  // s = res - alph Ap; As = L s
       WTG_HIPCHK(c, hipGetLastError());
-      RC(launch_lc(c, 1.0, -1, s, -1.0, SC_OMGA, As, res, true));  // res = s - omga As
-      // err and the convergence test; unless converged, rho and b
-      RC(launch_dots_g(c, true, OP_ERR_BETA, 2, res, res, SC_T0, res0, res, SC_T3));
-      hipLaunchKernelGGL(fs_bicg_p_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
-                         c->d_ncl, res, p, Ap, c->d_scal, p_alt);  // p = res + b (p - omga Ap)
+      RC(launch_dots_g(c, true, OP_OMGA, 2, As, s, SC_T1, As, As, SC_T2));
+      const bool last = nbred <= c->lastblock_max;
+      hipLaunchKernelGGL(fs_bicg_res_kernel, dim3(std::max((unsigned)nbred, nblk(c->nncl))), dim3(FS_NT), 0,
+                         c->stream, c->d_scal, c->nmaster, c->nncl, c->d_ncl, c->nred, nbred, c->d_rcell, c->d_rw, u,
+                         p, s, As, res0, c->d_scal, res, c->d_part, last ? c->d_count : nullptr);  // u, res, dots
+      WTG_HIPCHK(c, hipGetLastError());
+      if (!last) {  // err, the convergence test, rho and b
+        hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1), dim3(FS_NT), 0, c->stream, 1, nbred, 2, c->d_part, SC_T0,
+                           SC_T3, 0, c->d_scal, OP_ERR_BETA);
+        WTG_HIPCHK(c, hipGetLastError());
+      }
+      hipLaunchKernelGGL(fs_bicg_p_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
+                         c->d_ncl, fs_geo(c), res, p, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->d_scal,
+                         p_alt, Ap_alt);  // p = res + b (p - omga Ap); Ap = L p
       WTG_HIPCHK(c, hipGetLastError());
       std::swap(p, p_alt);
-      RC(launch_apply(c, p, Ap, true));
+      std::swap(Ap, Ap_alt);
+    }
+    return 0;
+  };
+  static_assert(BICG_CHUNK % 2 == 0, "a chunk must return p and Ap to their buffers");
+  while (queued < maxit && !done) {
+    const int k = std::min(BICG_CHUNK, maxit - queued);
+    if (c->fs_graph && k == BICG_CHUNK) {
+      // the operator's coefficients are kernel arguments of the graph
+      const double key[2] = {fs_coef(c), c->params.grav};
+      if (!c->bicg_graph || c->bicg_u != u || c->bicg_f != f || c->bicg_key[0] != key[0] ||
+          c->bicg_key[1] != key[1]) {
+        if (c->bicg_graph) {
+          hipGraphExecDestroy(c->bicg_graph);
+          c->bicg_graph = nullptr;
+        }
+        WTG_HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = chunk(k);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+          if (graph) hipGraphDestroy(graph);
+          return rc;
+        }
+        if (ec != hipSuccess) return wtg_fail(c, WTG_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&c->bicg_graph, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+          c->bicg_graph = nullptr;
+          return wtg_fail(c, WTG_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
+        c->bicg_u = u;
+        c->bicg_f = f;
+        c->bicg_key[0] = key[0];
+        c->bicg_key[1] = key[1];
+      }
+      WTG_HIPCHK(c, hipGraphLaunch(c->bicg_graph, c->stream));
+    } else {
+      RC(chunk(k));
     }
     queued += k;
     WTG_HIPCHK(c, hipMemcpyAsync(c->h_scal + SC_ERR, c->d_scal + SC_ERR, (SC_TOL - SC_ERR) * sizeof(double),
@@ -716,6 +928,12 @@ int jacobi_solve(wtg_ctx* c, double* u, const double* f, double tol, int maxit, 
 }  // namespace
 
 int wtg_fs_ready(wtg_ctx* c) { return fs_ready(c); }
+
+int wtg_fs_set_graph(wtg_ctx* c, int enable) {
+  if (!c) return WTG_E_ARG;
+  c->fs_graph = enable != 0;
+  return 0;
+}
 
 // hierarchy of free-surface contexts, coarse (0) to fine (nl - 1)
 struct wtg_mg {

@@ -2605,6 +2605,7 @@ void wtg_destroy(wtg_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->rk_graph) hipGraphExecDestroy(c->rk_graph);
+  if (c->bicg_graph) hipGraphExecDestroy(c->bicg_graph);
   for (void* p : c->allocs) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
   if (c->stream) hipStreamDestroy(c->stream);
