@@ -172,11 +172,11 @@ def main():
     print(json.dumps(out))
 
 
-def measured_traffic(J: int, L: int):
+def measured_traffic(J: int, L: int, kernel: str = "trisk_stage"):
     """HBM bytes per launch of the stage kernel from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE + WRITE_SIZE per dispatch,
     scaled by a plain copy's readings), or None if it is absent."""
-    path = os.path.join(ROOT, "profiles", f"trisk_stage_J{J}_L{L}_hbm.txt")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_J{J}_L{L}_hbm.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -204,6 +204,9 @@ def fp32_bulk(grid, topo, mass, temp, velo, L, dt, warmup, steps, m64, u64, dof_
         ctx.rk4(0, dt, warmup)
     ctx.synchronize()
     total_ms, kernel_ms = ctx.rk4_timed(0, dt, steps)
+    sh, tms = ctx.stage_shape()
+    shape = {"chosen": {1: "layer pairs", 2: "edge tasks", 3: "one layer"}.get(sh, "auto"),
+             "tuner_ms": {"layer pairs": tms[0], "edge tasks": tms[1], "one layer": tms[2]}}
     m, _, u = ctx.download_state(0)
     ctx.close()
     lay = grid.layout
@@ -211,8 +214,19 @@ def fp32_bulk(grid, topo, mass, temp, velo, L, dt, warmup, steps, m64, u64, dof_
     depth = m64[:, own].sum(0) / 1028.0
     eta_err = np.max(np.abs(m[:, own].sum(0) - m64[:, own].sum(0))) / 1028.0
     ms = total_ms / steps
+    # algorithmic bytes of the mean stage: per node-layer and stage, 60 / 80 /
+    # 80 / 100 B (FP64 state read, FP32 increment read and written, FP64 state
+    # written in the last stage), and 80 B of geometry per node
+    nodes = 10 * lay.n * lay.n
+    alg = (0.25 * (60 + 80 + 80 + 100) * L + 80) * nodes
+    achieved = alg / (kernel_ms / 1e3) / 1e9
     return {"ms_per_step": ms, "dof_updates_per_s": dof_per_step / (ms / 1e3), "speedup_vs_fp64": ms64 / ms,
             "stage_kernel_ms": kernel_ms,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(lay.level, L, "trisk_stage_mixed"),
+                         "kernel": "trisk_stage_mixed_kernel", "algorithmic_bytes_per_launch": alg},
+            "stage_shape": shape,
             "drift_vs_fp64": {"steps": warmup + steps,
                               "surface_height_rel_depth": eta_err / float(np.max(depth)),
                               "mass_rel": float(np.max(np.abs(m[:, own] - m64[:, own])) / np.max(np.abs(m64[:, own]))),
