@@ -658,11 +658,16 @@ __global__ void lc_features_kernel(int npatch, const int64_t* __restrict__ cells
 }
 
 // e(patch node) = sigma * y (other cells keep their value)
-__global__ void lc_scatter_kernel(int npatch, const int64_t* __restrict__ cells, const float* __restrict__ y,
-                                  const double* __restrict__ sigma, double* __restrict__ e) {
-  const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
-  if (p >= npatch) return;
-  e[cells[(int64_t)p * 64 + lane]] = (double)y[(int64_t)p * 64 + lane] * sigma[p];
+// e := the network's correction y sigma on the patch nodes, 0 on the cells
+// outside every patch (the pole masters), over the node compute list so the
+// halo copies are written in the same pass; slot[cell] = patch * 64 + node
+__global__ void lc_scatter_kernel(int n, const int64_t* __restrict__ ncl, const int* __restrict__ slot,
+                                  const float* __restrict__ y, const double* __restrict__ sigma,
+                                  double* __restrict__ e) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int s = slot[ncl[2 * t + 1]];
+  e[ncl[2 * t]] = s >= 0 ? (double)y[s] * sigma[s / 64] : 0.0;
 }
 
 
@@ -945,6 +950,7 @@ struct wtg_mg {
   wtg_mlp* mlp = nullptr;
   int npatch = 0;
   int64_t* d_cells = nullptr;  // [npatch][64] cell of each patch node
+  int* d_slot = nullptr;       // [ncells] patch * 64 + node of each patch cell, -1 elsewhere
   double* d_fnode = nullptr;   // sin(latitude) per cell
   double* d_sigma = nullptr;   // [npatch]
   float* d_feat = nullptr;     // [npatch][192] (feature extraction)
@@ -1172,20 +1178,19 @@ int lc_features(wtg_mg* m, const double* r, bool to_mlp) {
 }
 
 // e := ghost-filled learned correction of residual r on the finest level
-int lc_correction(wtg_mg* m, const double* r, double* tmp, double* e) {
+int lc_correction(wtg_mg* m, const double* r, double* e) {
   wtg_ctx* c = m->lv[m->nl - 1];
   RC(lc_features(m, r, true));
-  if (int rc = wtg_mlp_forward_on(m->mlp, c->stream, m->npatch))
+  if (int rc = wtg_mlp_infer_on(m->mlp, c->stream, m->npatch))
     return wtg_fail(c, rc, "learned correction: %s", wtg_mlp_error_text(m->mlp));
   uint16_t* x0;
   float* y;
   int maxb;
   wtg_mlp_buffers(m->mlp, &x0, &y, &maxb);
-  RC(mg_fill(c, tmp, 0.0));
-  hipLaunchKernelGGL(lc_scatter_kernel, dim3((unsigned)((m->npatch + 3) / 4)), dim3(256), 0, c->stream, m->npatch,
-                     m->d_cells, y, m->d_sigma, tmp);
+  hipLaunchKernelGGL(lc_scatter_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, m->d_slot,
+                     y, m->d_sigma, e);
   WTG_HIPCHK(c, hipGetLastError());
-  return launch_copy(c, tmp, e);  // halo copies
+  return 0;
 }
 
 // Cycles on the finest level from its current u: a plain V-cycle each, or
@@ -1207,13 +1212,12 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
   res_out[l] = err;
   for (int it = 1; it <= P.max_vcycle && err > P.fine_tol; ++it) {
     if (learned) {
-      double *r = c->fs[FS_W0 + 1], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3], *tmp = c->fs[FS_W0 + 4];
+      double *r = c->fs[FS_W0 + 1], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3];
       RC(launch_residual(c, V[l].u, V[l].f, r));
-      RC(lc_correction(m, r, tmp, e));
+      RC(lc_correction(m, r, e));
       RC(launch_apply(c, e, w));
       // the step is computed and applied on the device (a = 0 leaves u unchanged)
-      RC(launch_dots(c, 2, r, w, SC_T0, w, w, SC_T1));
-      RC(launch_scalar(c, OP_MRSTEP));
+      RC(launch_dots_g(c, false, OP_MRSTEP, 2, r, w, SC_T0, w, w, SC_T1));
       RC(launch_lc(c, 1.0, -1, V[l].u, 1.0, SC_MR, e, V[l].u_alt));
       std::swap(V[l].u, V[l].u_alt);
     }
@@ -1452,14 +1456,18 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
             for (int a = 0; a < 8; ++a) cells[p * 64 + b * 8 + a] = (int64_t)fl.cell(lz, 8 * pi + a, 8 * pj + b);
         }
     const size_t NC = fl.ncells();
+    std::vector<int> slot(NC, -1);
+    for (size_t q = 0; q < cells.size(); ++q) slot[cells[q]] = (int)q;
     std::vector<double> fn(NC);
     for (size_t c = 0; c < NC; ++c) fn[c] = fl.X[2 * NC + c] / fl.radius;
     bool ok = hipMalloc((void**)&m->d_cells, cells.size() * sizeof(int64_t)) == hipSuccess &&
               hipMalloc((void**)&m->d_fnode, NC * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&m->d_sigma, (size_t)m->npatch * sizeof(double)) == hipSuccess &&
-              hipMalloc((void**)&m->d_feat, (size_t)m->npatch * 192 * sizeof(float)) == hipSuccess;
+              hipMalloc((void**)&m->d_feat, (size_t)m->npatch * 192 * sizeof(float)) == hipSuccess &&
+              hipMalloc((void**)&m->d_slot, NC * sizeof(int)) == hipSuccess;
     ok = ok && hipMemcpy(m->d_cells, cells.data(), cells.size() * sizeof(int64_t), hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(m->d_fnode, fn.data(), NC * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+         hipMemcpy(m->d_fnode, fn.data(), NC * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(m->d_slot, slot.data(), NC * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
       wtg_mg_destroy(m.release());
       return wtg_fail(nullptr, WTG_E_DEVICE, "patch table upload failed");
@@ -1476,7 +1484,7 @@ void wtg_mg_destroy(wtg_mg* m) {
   for (auto* c : m->lv) c->stream = c->own_stream;
   for (double* a : m->ovl)
     if (a) hipFree(a);
-  for (void* a : {(void*)m->d_cells, (void*)m->d_fnode, (void*)m->d_sigma, (void*)m->d_feat})
+  for (void* a : {(void*)m->d_cells, (void*)m->d_fnode, (void*)m->d_sigma, (void*)m->d_feat, (void*)m->d_slot})
     if (a) hipFree(a);
   delete m;
 }

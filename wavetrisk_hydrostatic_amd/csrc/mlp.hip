@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -152,6 +153,94 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
         if (EPI == EPI_BIAS_GELU) a.H[(int64_t)m * a.ldc + n] = f2bf(gelu(z));
       }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Inference in one launch: each 256-thread block takes 64 rows through the
+// three layers, the hidden activations staying in LDS.  The MFMA tiles, the
+// fragment layout, the k order and the epilogue arithmetic are those of
+// gemm_bf16_kernel, so the output equals the three-launch forward pass bit
+// for bit; nothing is kept for a backward pass.
+// ---------------------------------------------------------------------------
+constexpr int HP = WTG_MLP_HIDDEN + 8;  // LDS pitch of a hidden activation row (bf16)
+
+// One layer for the block's 64 rows: Z = A[64 x K] W[K x N] + bias, then
+// either H = bf16(GELU(Z)) into an LDS activation block (TO_LDS) or y = Z for
+// the valid rows.  A comes from global bf16 rows (FROM_GLOBAL: Ag, staged per
+// k chunk, rows past `rows` read as zero) or from an LDS activation block
+// (Al, pitch HP); W (global, [K][N]) is staged per k chunk as [n][k]; wave w
+// owns rows 32 (w / 2) .. +32 and columns (N / 2)(w % 2) .. +N/2.
+template <int N, bool FROM_GLOBAL, bool TO_LDS>
+__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ Ag, int rows, const uint16_t* Al, int K,
+                                            const uint16_t* __restrict__ W, const float* __restrict__ bias,
+                                            uint16_t (*As)[LP], uint16_t (*Bs)[LP], uint16_t* H, float* y) {
+  constexpr int NJ = N / 32;
+  const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+  const int wm = (wave / 2) * 32, wn = (wave % 2) * (N / 2);
+  float4_t acc[2 * NJ];
+  for (int q = 0; q < 2 * NJ; ++q) acc[q] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    if (FROM_GLOBAL) {  // 64 x 32 chunk of the input rows
+      const int m = t / 4, kk = (t % 4) * 8;
+      uint4 q = make_uint4(0u, 0u, 0u, 0u);
+      if (m < rows) q = *reinterpret_cast<const uint4*>(Ag + (int64_t)m * K + k0 + kk);
+      *reinterpret_cast<uint4*>(&As[m][kk]) = q;
+    }
+    for (int p = 0; p < N / 64; ++p) {  // 32 x N chunk of W, stored [n][k]
+      const int kk = t / 8, n = p * 64 + (t % 8) * 8;
+      const uint4 q = *reinterpret_cast<const uint4*>(W + (int64_t)(k0 + kk) * N + n);
+      const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+      for (int i = 0; i < 4; ++i) {
+        Bs[n + 2 * i][kk] = (uint16_t)(w4[i] & 0xffffu);
+        Bs[n + 2 * i + 1][kk] = (uint16_t)(w4[i] >> 16);
+      }
+    }
+    __syncthreads();
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kq = ks + 4 * (lane / 16);
+      short4_t af[2], bfr[NJ];
+      for (int i = 0; i < 2; ++i)
+        af[i] = FROM_GLOBAL ? *reinterpret_cast<const short4_t*>(&As[wm + i * 16 + lane % 16][kq])
+                            : *reinterpret_cast<const short4_t*>(Al + (wm + i * 16 + lane % 16) * HP + k0 + kq);
+      for (int j = 0; j < NJ; ++j) bfr[j] = *reinterpret_cast<const short4_t*>(&Bs[wn + j * 16 + lane % 16][kq]);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < NJ; ++j)
+          acc[i * NJ + j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i * NJ + j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < NJ; ++j) {
+      const int n = wn + j * 16 + lane % 16;
+      const float b = bias[n];
+      for (int r = 0; r < 4; ++r) {
+        const int m = wm + i * 16 + 4 * (lane / 16) + r;
+        const float z = acc[i * NJ + j][r] + b;
+        if (TO_LDS)
+          H[m * HP + n] = f2bf(gelu(z));
+        else if (m < rows)
+          y[(int64_t)m * N + n] = z;
+      }
+    }
+}
+
+__global__ __launch_bounds__(GT) void mlp_infer_kernel(int B, const uint16_t* __restrict__ x,
+                                                       const uint16_t* __restrict__ wb, const float* __restrict__ p,
+                                                       float* __restrict__ y) {
+  __shared__ __align__(16) uint16_t As[BM][LP];
+  __shared__ __align__(16) uint16_t Bs[WTG_MLP_HIDDEN][LP];
+  __shared__ __align__(16) uint16_t H1[BM * HP];
+  __shared__ __align__(16) uint16_t H2[BM * HP];
+  constexpr int NI_ = WTG_MLP_IN, NH_ = WTG_MLP_HIDDEN, NO_ = WTG_MLP_OUT;
+  constexpr int OB1_ = NI_ * NH_, OW2_ = OB1_ + NH_, OB2_ = OW2_ + NH_ * NH_, OW3_ = OB2_ + NH_,
+                OB3_ = OW3_ + NH_ * NO_;
+  const int m0 = blockIdx.x * BM, rows = min(BM, B - m0);
+  infer_layer<NH_, true, true>(x + (int64_t)m0 * NI_, rows, H1, NI_, wb, p + OB1_, As, Bs, H1, y);
+  __syncthreads();
+  infer_layer<NH_, false, true>(x, rows, H1, NH_, wb + NI_ * NH_, p + OB2_, As, Bs, H2, y);
+  __syncthreads();
+  infer_layer<NO_, false, false>(x, rows, H2, NH_, wb + NI_ * NH_ + NH_ * NH_, p + OB3_, As, Bs, H2,
+                                 y + (int64_t)m0 * NO_);
 }
 
 __global__ void to_bf16_kernel(int64_t n, const float* __restrict__ x, uint16_t* __restrict__ y) {
@@ -386,15 +475,15 @@ void wtg_mlp_buffers(wtg_mlp* m, uint16_t** x0, float** y, int* max_batch) {
   *max_batch = m->max_batch;
 }
 
-int wtg_mlp_forward_on(wtg_mlp* m, hipStream_t stream, int B) {
-  const int Bp = (B + 63) / 64 * 64;
-  if (Bp > m->max_batch) return mfail(m, WTG_E_ARG, "batch %d exceeds the maximum %d", B, m->max_batch);
-  hipStream_t own = m->stream;
-  m->stream = stream;
-  const int rc = forward(m, Bp);
-  m->stream = own;
-  return rc;
+int wtg_mlp_infer_on(wtg_mlp* m, hipStream_t stream, int B) {
+  if (B < 0 || B > m->max_batch) return mfail(m, WTG_E_ARG, "batch %d exceeds the maximum %d", B, m->max_batch);
+  if (B == 0) return 0;
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((B + BM - 1) / BM)), dim3(GT), 0, stream, B, m->x0, m->wb, m->p,
+                     m->y);
+  MCHK(m, hipGetLastError());
+  return 0;
 }
+
 
 const char* wtg_mlp_error_text(const wtg_mlp* m) { return m->err.c_str(); }
 
@@ -512,7 +601,7 @@ int wtg_mlp_forward(wtg_mlp* m, const float* x, int batch, float* y) {
   if (!m || !x || !y) return WTG_E_ARG;
   if (batch < 1) return mfail(m, WTG_E_ARG, "batch < 1");
   MRC(upload_rows(m, x, nullptr, batch));
-  MRC(forward(m, batch));
+  MRC(wtg_mlp_infer_on(m, m->stream, batch));
   MCHK(m, hipMemcpyAsync(y, m->y, (size_t)batch * NO * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
   return 0;
@@ -531,7 +620,7 @@ int wtg_mlp_forward_timed(wtg_mlp* m, int batch, int nrep, double* ms_per_pass) 
   }
   int rc = 0;
   if (hipEventRecord(e0, m->stream) != hipSuccess) rc = mfail(m, WTG_E_DEVICE, "hipEventRecord failed");
-  for (int i = 0; i < nrep && !rc; ++i) rc = forward(m, batch);
+  for (int i = 0; i < nrep && !rc; ++i) rc = wtg_mlp_infer_on(m, m->stream, batch);
   if (!rc && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
     rc = mfail(m, WTG_E_DEVICE, "event synchronisation failed");
   if (!rc) {

