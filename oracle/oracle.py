@@ -31,6 +31,7 @@ def load():
         _lib.wto_ghost_fill.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.wto_trend.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         _lib.wto_rk4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.wto_num_threads.restype = C.c_int
         _lib.wto_node_ghost.argtypes = [C.c_void_p, C.c_void_p]
         _lib.wto_elliptic_apply.argtypes = [C.c_void_p] * 5
         _lib.wto_elliptic_diag.argtypes = [C.c_void_p] * 4
@@ -51,6 +52,11 @@ def load():
         _lib.wto_rk4_split.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
         _lib.wto_rk4_split.restype = C.c_int
     return _lib
+
+
+def num_threads() -> int:
+    """Threads of the oracle's OpenMP cell loops (OMP_NUM_THREADS)."""
+    return int(load().wto_num_threads())
 
 
 class FsParams(C.Structure):

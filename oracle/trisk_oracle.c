@@ -22,10 +22,28 @@
 #define H 2
 #define CELL(lz, i, j) ((int64_t)(lz) * S * S + (int64_t)((j) + H) * S + (int64_t)((i) + H))
 
+/* The cell loops of one pass write distinct cells and read only the inputs of
+ * that pass, so they run over the host's cores (OpenMP) with results that do
+ * not depend on the thread count. */
+#ifdef _OPENMP
+#include <omp.h>
+#define WTO_PARALLEL_LZ_J _Pragma("omp parallel for collapse(2) schedule(static)")
+#define WTO_PARALLEL_K_C _Pragma("omp parallel for collapse(2) schedule(static)")
+int wto_num_threads(void) { return omp_get_max_threads(); }
+#else
+#define WTO_PARALLEL_LZ_J
+#define WTO_PARALLEL_K_C
+int wto_num_threads(void) { return 1; }
+#endif
+
 static inline double interp(double a, double b) { return 0.5 * (a + b); }
 
 void wto_ghost_fill(const wto_grid* g, int L, double* mass, double* temp, double* velo) {
   const int64_t NC = g->ncells;
+  /* a halo slot's source is an owned slot, so the copies are independent */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int k = 0; k < L; ++k) {
     double* m = mass + (int64_t)k * NC;
     double* t = temp + (int64_t)k * NC;
@@ -83,6 +101,9 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
   memset(dvelo, 0, sizeof(double) * 3 * NC * L);
 
   /* dual lengths used for fluxes: a halo edge carries its owner's flux */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int64_t c = 0; c < NC; ++c)
     for (int e = 0; e < 3; ++e) {
       int64_t slot = 3 * c + e, s = g->edge_src[slot];
@@ -93,6 +114,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
   for (int lz = 0; lz < 10; ++lz)
     for (int j = -1; j <= n + 1; ++j)
       for (int i = -1; i <= n + 1; ++i) w.geopot[CELL(lz, i, j)] = grav * g->topo[CELL(lz, i, j)];
+  WTO_PARALLEL_LZ_J
   for (int lz = 0; lz < 10; ++lz)
     for (int j = 0; j <= n; ++j)
       for (int i = 0; i <= n; ++i) {
@@ -116,6 +138,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
 #define UP_(e, c) (U(e, c) * LEN(e, c))
 #define UD_(e, c) (U(e, c) * PED(e, c))
     /* integrate_pressure_up on [0, n]^2 */
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = 0; j <= n; ++j)
         for (int i = 0; i <= n; ++i) {
@@ -128,6 +151,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
           w.press_lower[c] = p_upper;
         }
     /* potential vorticity on triangles LORT(i,j), UPLT(i,j), (i,j) in [-1, n]^2 */
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = -1; j <= n; ++j)
         for (int i = -1; i <= n; ++i) {
@@ -140,6 +164,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
           w.pvU[c] = (cor[NC + c] + circU) / massU;
         }
     /* qe and fluxes on every edge a stencil reads */
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = -1; j <= n; ++j)
         for (int i = -1; i <= n; ++i) {
@@ -218,6 +243,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
       }
     }
     /* kinetic energy, Bernoulli, Exner on [0, n]^2 */
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = 0; j <= n; ++j)
         for (int i = 0; i <= n; ++i) {
@@ -231,6 +257,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
           w.exner[c] = -phi_k;
         }
     /* scalar trends: -div(flux) on owned cells and the pole masters */
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = 0; j <= n; ++j)
         for (int i = 0; i <= n; ++i) {
@@ -246,6 +273,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
     /* velocity trends on owned edges: -Qperp, then gradients (du_source, du_grad) */
 #define HM(e, c) (w.hm[(int64_t)(e) * NC + (c)])
 #define QE(e, c) (w.qe[(int64_t)(e) * NC + (c)])
+    WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
       for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
@@ -334,6 +362,7 @@ void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, doub
       double* ot = st == 3 ? temp : t1;
       double* ou = st == 3 ? velo : u1;
       const double h = coef[st];
+      WTO_PARALLEL_K_C
       for (int k = 0; k < L; ++k)
         for (int64_t c = 0; c < NC; ++c) {
           if (!is_prognostic(g, c)) continue;

@@ -46,6 +46,9 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
  * the state is ghost filled on entry and on exit. */
 void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, double* temp, double* velo);
 
+/* threads the oracle's cell loops use (OpenMP; 1 without it) */
+int wto_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
