@@ -131,6 +131,7 @@ def main():
 
     bulk = fp32_bulk(grid, topo, mass, temp, velo, L, args.dt, args.warmup, args.steps, m_out, u_out,
                      dof_per_step, ms_per_step)
+    shards = sharding_model(grid, ctx, L)
     ens = None if args.no_ensemble else ensemble(grid, topo, mass, temp, velo, L, args.dt, args.members)
     small = None if args.no_cpu else shallow_water_c1()
     ctx.synchronize()
@@ -169,6 +170,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp32_bulk": bulk,
+        "sharding_model": shards,
         "shallow_water_J5": small,
         "ensemble": ens,
         "split_step": split,
@@ -239,6 +241,25 @@ def fp32_bulk(grid, topo, mass, temp, velo, L, dt, warmup, steps, m64, u64, dof_
                                                 np.max(np.abs(u64[:, :, own_e])))},
             "note": "PV, fluxes, kinetic energy, divergence, Qperp in FP32; column scan, Bernoulli/Exner "
                     "gradients and state updates FP64; RK stages as FP32 increments over the FP64 state"}
+
+
+def sharding_model(grid, ctx, L, xgmi_gbs=153.0):
+    """Whole-lozenge shards of the J8 L30 state: ghost entries per layer that
+    stay in a shard and that would cross to another GPU, the bytes one RK
+    stage would send (all layers: mass, temp, velocity, FP64), the time of
+    that at one xGMI link's bandwidth, and the measured time of the ghost
+    fill's local and would-be-remote launches on this GPU."""
+    print("bench: sharding model of the ghost fill", file=sys.stderr, flush=True)
+    out = {"shard": "whole lozenges, lozenge lz on shard lz * nshards / 10", "xgmi_link_gbs": xgmi_gbs}
+    for ns in (2, 5, 10):
+        nl, nr, el, er = grid.shard_halo(ns)
+        remote_bytes = L * (2 * nr + er) * 8
+        ms_l, ms_r = ctx.ghost_fill_split_timed(0, ns, 20)
+        out[str(ns)] = {"node_entries_local": nl, "node_entries_remote": nr, "edge_entries_local": el,
+                        "edge_entries_remote": er, "remote_bytes_per_stage": remote_bytes,
+                        "remote_ms_at_one_link": remote_bytes / (xgmi_gbs * 1e9) * 1e3,
+                        "fill_ms_local": ms_l, "fill_ms_remote_entries": ms_r}
+    return out
 
 
 def shallow_water_c1(nsteps=100):

@@ -119,6 +119,17 @@ int wtg_download_state(wtg_ctx* ctx, int slot, double* mass, double* temp, doubl
 /* update_bdry on one slot: halo := owner values (edges with sign). */
 int wtg_ghost_fill(wtg_ctx* ctx, int slot);
 
+/* Sharding model (no counterpart in the reference; one GPU runs it): shards
+ * of whole lozenges, lozenge lz in shard lz * nshards / 10 (the reference's
+ * 16 domains of a lozenge together, comm.f90's lists cut at lozenge edges).
+ * wtg_grid_shard_halo counts the ghost entries of one layer: counts[0] node
+ * entries whose source is in the destination's shard, counts[1] the others
+ * (the would-be remote ones), counts[2], counts[3] the same for edges.
+ * wtg_ghost_fill_split_timed fills slot `slot` as those two launches, each
+ * timed over `reps` launches (mean ms); the result equals wtg_ghost_fill. */
+int wtg_grid_shard_halo(const wtg_grid* grid, int nshards, int64_t counts[4]);
+int wtg_ghost_fill_split_timed(wtg_ctx* ctx, int slot, int nshards, int reps, double* ms_local, double* ms_remote);
+
 /* trend_ml: dq_slot := trend of q_slot (q_slot must be ghost filled).
  * The trend is written on owned cells/edges and the pole masters. */
 int wtg_trend(wtg_ctx* ctx, int q_slot, int dq_slot);

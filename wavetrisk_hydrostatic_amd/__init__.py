@@ -77,6 +77,8 @@ SIGNATURES = {
     "wtg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
     "wtg_set_graph": (C.c_int, [_P, C.c_int]),
+    "wtg_grid_shard_halo": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    "wtg_ghost_fill_split_timed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D]),
     "wtg_fs_set_graph": (C.c_int, [_P, C.c_int]),
     "wtg_get_stage_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
@@ -223,6 +225,13 @@ class Grid:
             return out  # slot = 3*cell + e
         return out.reshape(comp, nc) if comp > 1 else out
 
+    def shard_halo(self, nshards: int):
+        """Ghost entries of one layer for whole-lozenge shards: (node local,
+        node remote, edge local, edge remote)."""
+        counts = (C.c_int64 * 4)()
+        _check(load_library().wtg_grid_shard_halo(self._h, nshards, counts), None, "wtg_grid_shard_halo")
+        return tuple(int(v) for v in counts)
+
     @property
     def handle(self):
         return self._h
@@ -282,6 +291,13 @@ class Context:
 
     def ghost_fill(self, slot: int):
         _check(_lib.wtg_ghost_fill(self._h, slot), self._h, "wtg_ghost_fill")
+
+    def ghost_fill_split_timed(self, slot: int, nshards: int, reps: int = 20):
+        """The ghost fill as local and would-be-remote launches: (ms, ms)."""
+        a, b = C.c_double(), C.c_double()
+        _check(_lib.wtg_ghost_fill_split_timed(self._h, slot, nshards, reps, C.byref(a), C.byref(b)), self._h,
+               "wtg_ghost_fill_split_timed")
+        return a.value, b.value
 
     def trend(self, q_slot: int, dq_slot: int):
         _check(_lib.wtg_trend(self._h, q_slot, dq_slot), self._h, "wtg_trend")

@@ -2462,6 +2462,31 @@ int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double*
   return 0;
 }
 
+// ghost entries of a uniform level by whether their source lies in the same
+// shard as their destination, shards being whole lozenges: lozenge lz goes to
+// shard lz * nshards / 10 (the reference's domains of one lozenge together)
+int wtg_grid_shard_halo(const wtg_grid* grid, int nshards, int64_t counts[4]) {
+  if (!grid || !counts) return fail(nullptr, WTG_E_ARG, "null argument");
+  if (nshards < 1 || nshards > 10) return fail(nullptr, WTG_E_ARG, "nshards %d out of range [1,10]", nshards);
+  const auto& g = grid->lv;
+  const int64_t NC = (int64_t)g.ncells(), SS = (int64_t)g.S * g.S;
+  const int n = g.n;
+  auto shard = [&](int64_t cell) { return (int)(cell / SS) * nshards / 10; };
+  for (int q = 0; q < 4; ++q) counts[q] = 0;
+  for (int64_t c = 0; c < NC; ++c) {
+    const int64_t r = c % SS;
+    const int j = (int)(r / g.S) - H, i = (int)(r % g.S) - H;
+    if (i >= 0 && i < n && j >= 0 && j < n) continue;
+    const int64_t s = g.node_src[c];
+    if (s >= 0 && s != c) ++counts[shard(s) == shard(c) ? 0 : 1];
+    for (int e = 0; e < 3; ++e) {
+      const int64_t slot = 3 * c + e, se = g.edge_src[slot];
+      if (se >= 0 && se != slot) ++counts[shard(se / 3) == shard(c) ? 2 : 3];
+    }
+  }
+  return 0;
+}
+
 void wtg_grid_destroy(wtg_grid* grid) { delete grid; }
 
 int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_ctx** out) {
@@ -2650,6 +2675,78 @@ int wtg_download_state(wtg_ctx* c, int slot, double* mass, double* temp, double*
   HIPCHK(c, hipMemcpyAsync(velo, s.u, 3 * ns * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+// the ghost fill of `slot` as two launches, the entries whose source lies in
+// the destination's shard and those whose source does not (whole-lozenge
+// shards as wtg_grid_shard_halo); each timed over `reps` launches
+int wtg_ghost_fill_split_timed(wtg_ctx* c, int slot, int nshards, int reps, double* ms_local, double* ms_remote) {
+  if (int rc = check_slot(c, slot)) return rc;
+  if (!ms_local || !ms_remote) return fail(c, WTG_E_ARG, "null argument");
+  if (nshards < 1 || nshards > 10) return fail(c, WTG_E_ARG, "nshards %d out of range [1,10]", nshards);
+  if (reps < 1 || reps > 10000) return fail(c, WTG_E_ARG, "reps %d out of range [1,10000]", reps);
+  const int64_t NC = c->info.ncells, SS = (int64_t)c->info.S * c->info.S;
+  auto shard = [&](int64_t cell) { return (int)(cell / SS) * nshards / 10; };
+  std::vector<int64_t> nd(2 * (size_t)c->nn), ed(2 * (size_t)c->ne);
+  std::vector<int8_t> es(c->ne);
+  HIPCHK(c, hipMemcpy(nd.data(), c->d_nd, nd.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ed.data(), c->d_ed, ed.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(es.data(), c->d_es, es.size(), hipMemcpyDeviceToHost));
+  std::vector<int64_t> ndp[2], edp[2];
+  std::vector<int8_t> esp[2];
+  for (int q = 0; q < c->nn; ++q) {
+    const int r = shard(nd[2 * q]) == shard(nd[2 * q + 1]) ? 0 : 1;
+    ndp[r].push_back(nd[2 * q]);
+    ndp[r].push_back(nd[2 * q + 1]);
+  }
+  for (int q = 0; q < c->ne; ++q) {  // plane indices e * NC + cell
+    const int r = shard(ed[2 * q] % NC) == shard(ed[2 * q + 1] % NC) ? 0 : 1;
+    edp[r].push_back(ed[2 * q]);
+    edp[r].push_back(ed[2 * q + 1]);
+    esp[r].push_back(es[q]);
+  }
+  const auto& s = c->slots[slot];
+  const int L = c->params.nlayers * c->nmembers;
+  double ms[2] = {0.0, 0.0};
+  int rc = 0;
+  for (int r = 0; r < 2 && !rc; ++r) {
+    int64_t *dn = nullptr, *de = nullptr;
+    int8_t* dsg = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int nn = (int)(ndp[r].size() / 2), ne = (int)(edp[r].size() / 2);
+    if (hipMalloc(&dn, std::max<size_t>(ndp[r].size(), 1) * 8) != hipSuccess ||
+        hipMalloc(&de, std::max<size_t>(edp[r].size(), 1) * 8) != hipSuccess ||
+        hipMalloc(&dsg, std::max<size_t>(esp[r].size(), 1)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+      rc = fail(c, WTG_E_DEVICE, "allocation of the split lists failed");
+    if (!rc && (hipMemcpy(dn, ndp[r].data(), ndp[r].size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(de, edp[r].data(), edp[r].size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dsg, esp[r].data(), esp[r].size(), hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(c, WTG_E_DEVICE, "upload of the split lists failed");
+    const int64_t total = (int64_t)(nn + ne) * L;
+    if (!rc && total > 0) {
+      const unsigned blocks = (unsigned)((total + 255) / 256);
+      hipEventRecord(e0, c->stream);
+      for (int k = 0; k < reps; ++k)
+        hipLaunchKernelGGL(ghost_fill_kernel<double>, dim3(blocks), dim3(256), 0, c->stream, L, NC, dn, nn, de, dsg,
+                           ne, s.m, s.t, s.u);
+      hipEventRecord(e1, c->stream);
+      if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        rc = fail(c, WTG_E_DEVICE, "split ghost fill failed");
+      float t = 0.f;
+      if (!rc) hipEventElapsedTime(&t, e0, e1);
+      ms[r] = t / reps;
+    }
+    hipStreamSynchronize(c->stream);
+    if (dn) hipFree(dn);
+    if (de) hipFree(de);
+    if (dsg) hipFree(dsg);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  }
+  *ms_local = ms[0];
+  *ms_remote = ms[1];
+  return rc;
 }
 
 int wtg_ghost_fill(wtg_ctx* c, int slot) {
