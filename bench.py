@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-free-surface", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--members", type=int, default=8, help="ensemble members of the batched-ensemble line")
+    ap.add_argument("--members", type=int, default=16, help="ensemble members of the batched-ensemble line")
     ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -310,32 +310,51 @@ def shallow_water_c1(nsteps=100):
 def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10):
     """The batched ensemble (every layer kernel launched once for all
     members): members copies of the J8 L30 state, each with its own seeded
-    bathymetry perturbation (+-50 m), FP64 RK4 steps; DOF updates/s of the
-    whole ensemble."""
+    bathymetry perturbation (+-50 m), RK4 steps in FP64 and in FP32-bulk; DOF
+    updates/s of the whole ensemble, and the drift of each member's FP32-bulk
+    state from its FP64 state after the same steps from the same start."""
     import numpy as np
 
     import wavetrisk_hydrostatic_amd as W
 
     print(f"bench: ensemble of {members} members", file=sys.stderr, flush=True)
-    ctx = W.Context(grid, nlayers=L, members=members, nslots=1)
     rng = np.random.default_rng(7)
     topos = np.stack([topo + 50.0 * np.tanh(rng.standard_normal()) * np.cos(np.arange(topo.size) * 1e-4 * (m + 1))
                       for m in range(members)])
-    ctx.set_topography(topos)
-    ctx.upload_state(0, np.tile(mass, (members, 1)), np.tile(temp, (members, 1)), np.tile(velo, (members, 1, 1)))
-    ctx.rk4(0, dt, 2)
-    ctx.synchronize()
-    total_ms, kernel_ms = ctx.rk4_timed(0, dt, nsteps)
-    ctx.set_precision(W.PRECISION_FP32_BULK)
-    ctx.rk4(0, dt, 2)
-    ctx.synchronize()
-    total_b, kernel_b = ctx.rk4_timed(0, dt, nsteps)
-    ctx.close()
+    state = (np.tile(mass, (members, 1)), np.tile(temp, (members, 1)), np.tile(velo, (members, 1, 1)))
+    res = {}
+    for prec in (W.PRECISION_FP64, W.PRECISION_FP32_BULK):
+        ctx = W.Context(grid, nlayers=L, members=members, nslots=1)
+        ctx.set_topography(topos)
+        ctx.upload_state(0, *state)
+        ctx.set_precision(prec)
+        ctx.rk4(0, dt, 2)
+        ctx.synchronize()
+        total_ms, kernel_ms = ctx.rk4_timed(0, dt, nsteps)
+        res[prec] = (total_ms, kernel_ms, ctx.download_state(0))
+        ctx.close()
     dof = members * L * grid.layout.n_dof_per_layer
-    return {"members": members, "ms_per_step": total_ms / nsteps, "dof_updates_per_s": dof * nsteps / (total_ms / 1e3),
-            "stage_kernel_ms": kernel_ms,
-            "fp32_bulk": {"ms_per_step": total_b / nsteps, "dof_updates_per_s": dof * nsteps / (total_b / 1e3),
-                          "stage_kernel_ms": kernel_b},
+    lay = grid.layout
+    own, own_e = lay.owned_cells(), lay.owned_edge_cells()
+    m64, _, u64 = res[W.PRECISION_FP64][2]
+    m32, _, u32 = res[W.PRECISION_FP32_BULK][2]
+    drift = []
+    for k in range(members):
+        sl = slice(k * L, (k + 1) * L)
+        depth = m64[sl][:, own].sum(0) / 1028.0
+        eta = np.max(np.abs(m32[sl][:, own].sum(0) - m64[sl][:, own].sum(0))) / 1028.0 / np.max(depth)
+        du = np.max(np.abs(u32[sl][:, :, own_e] - u64[sl][:, :, own_e])) / np.max(np.abs(u64[sl][:, :, own_e]))
+        drift.append({"surface_height_rel_depth": float(eta), "velo_rel": float(du)})
+    t64, k64, _ = res[W.PRECISION_FP64]
+    t32, k32, _ = res[W.PRECISION_FP32_BULK]
+    return {"members": members, "ms_per_step": t64 / nsteps, "dof_updates_per_s": dof * nsteps / (t64 / 1e3),
+            "stage_kernel_ms": k64,
+            "fp32_bulk": {"ms_per_step": t32 / nsteps, "dof_updates_per_s": dof * nsteps / (t32 / 1e3),
+                          "stage_kernel_ms": k32,
+                          "drift_vs_fp64_per_member": {"steps": 2 + nsteps, "max_surface_height_rel_depth":
+                                                       max(d["surface_height_rel_depth"] for d in drift),
+                                                       "max_velo_rel": max(d["velo_rel"] for d in drift),
+                                                       "members": drift}},
             "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
 
 
