@@ -366,19 +366,35 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
 
     from wavetrisk_hydrostatic_amd import FS_ETA
 
+    from wavetrisk_hydrostatic_amd import PRECISION_FP32_BULK, PRECISION_FP64
+
     print(f"bench: split-explicit steps, dt {dt} s", file=sys.stderr, flush=True)
     ctx.fs_set_params(dt, 0.6, 0.8)
-    ctx.upload_state(1, mass, temp, velo)
-    ctx.fs_upload(FS_ETA, mass.sum(0) / 1028.0 + topo)
-    ctx.rk4_split(1, 1, 1e-9, 400)  # warm-up
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    its = [ctx.rk4_split(1, 1, 1e-9, 400) for _ in range(nsteps)]
-    ms = (time.perf_counter() - t0) * 1e3 / nsteps
     dof = L * grid.layout.n_dof_per_layer
-    return {"dt": dt, "theta1": 0.6, "theta2": 0.8, "rel_tol": 1e-9, "ms_per_step": ms,
-            "dof_updates_per_s": dof / (ms / 1e3), "bicgstab_iterations": [int(i) for i in its],
-            "note": "host wall clock over 5 steps; BiCGSTAB queues 8 gated iterations per host convergence test"}
+    runs, final = {}, {}
+    for name, prec in (("fp64", PRECISION_FP64), ("fp32_bulk", PRECISION_FP32_BULK)):
+        ctx.set_precision(prec)
+        ctx.upload_state(1, mass, temp, velo)
+        ctx.fs_upload(FS_ETA, mass.sum(0) / 1028.0 + topo)
+        ctx.rk4_split(1, 1, 1e-9, 400)  # warm-up (and, FP32-bulk, the stage autotuner)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        its = [ctx.rk4_split(1, 1, 1e-9, 400) for _ in range(nsteps)]
+        ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        runs[name] = {"ms_per_step": ms, "dof_updates_per_s": dof / (ms / 1e3),
+                      "bicgstab_iterations": [int(i) for i in its]}
+        final[name] = (ctx.download_state(1)[0], ctx.fs_download(FS_ETA))
+    ctx.set_precision(PRECISION_FP64)
+    own = grid.layout.owned_cells()
+    depth = np.max(final["fp64"][0][:, own].sum(0) / 1028.0)
+    drift = float(np.max(np.abs(final["fp32_bulk"][1][own] - final["fp64"][1][own])) / depth)
+    return {"dt": dt, "theta1": 0.6, "theta2": 0.8, "rel_tol": 1e-9, **runs["fp64"],
+            "fp32_bulk": {**runs["fp32_bulk"], "speedup_vs_fp64": runs["fp64"]["ms_per_step"] /
+                          runs["fp32_bulk"]["ms_per_step"],
+                          "surface_height_drift_rel_depth": drift, "steps_compared": nsteps + 1},
+            "note": "host wall clock over 5 steps after one warm-up step from the same state; BiCGSTAB queues 8 "
+                    "gated iterations per host convergence test; FP32-bulk: FP32 trend stencils, FP64 stage "
+                    "states, AXPYs and free surface"}
 
 
 def free_surface(fine: int):

@@ -1947,21 +1947,23 @@ __device__ __forceinline__ double grad_eta_e(const double* __restrict__ eta, con
 }
 
 // scalar_star and u_star (barotropic_2d.f90:9-50) over the masters:
-// out = base + h tr (scalars), out = base + h (tr + theta1 g grad eta) (edges)
+// out = base + h tr (scalars), out = base + h (tr + theta1 g grad eta) (edges);
+// the trend is FP64, or FP32 in the FP32-bulk mode (the sums stay FP64)
+template <class Tr>
 __global__ void split_axpy_kernel(int nmaster, const int64_t* __restrict__ ncl, int L, int64_t NC, int S, double h,
                                   double th1, double grav, const double* __restrict__ eta,
                                   const double* __restrict__ len, const double* bm, const double* bt,
-                                  const double* bu, const double* tm, const double* tt, const double* tu, double* om,
+                                  const double* bu, const Tr* tm, const Tr* tt, const Tr* tu, double* om,
                                   double* ot, double* ou) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)nmaster * L) return;
   const int k = (int)(idx / nmaster), i = (int)(idx % nmaster);
   const int64_t c = ncl[2 * i + 1], x = (int64_t)k * NC + c;
-  om[x] = bm[x] + h * tm[x];
-  ot[x] = bt[x] + h * tt[x];
+  om[x] = bm[x] + h * (double)tm[x];
+  ot[x] = bt[x] + h * (double)tt[x];
   for (int e = 0; e < 3; ++e) {
     const int64_t y = ((int64_t)k * 3 + e) * NC + c;
-    ou[y] = bu[y] + h * (tu[y] + th1 * grad_eta_e(eta, len, NC, S, c, e, grav));
+    ou[y] = bu[y] + h * ((double)tu[y] + th1 * grad_eta_e(eta, len, NC, S, c, e, grav));
   }
 }
 
@@ -2354,6 +2356,17 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
   const double dt = c->fs_dt, th1 = c->fs_theta1, grav = c->params.grav;
   if (int rc = launch_copy_frozen(c, sol, w1)) return rc;
   if (int rc = launch_copy_frozen(c, sol, w2)) return rc;
+  // FP32-bulk: the trend stencils run in FP32 into the first FP32 stage
+  // buffer; the stage states, the AXPYs and the free surface stay FP64
+  const bool f32 = c->precision == WTG_PRECISION_FP32_BULK;
+  if (f32) {
+    if (int rc = f32_ready(c)) return rc;
+    if (!c->mixed_tuned && nsteps > 0) {
+      if (int rc = launch_ghost_fill(c, sol)) return rc;
+      if (int rc = tune_mixed_shape(c, sol)) return rc;
+    }
+  }
+  const auto& ftr = c->fstage[0];
   const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
   const wtg_ctx::Slot* ins[4] = {&sol, &w1, &w2, &w1};
   const wtg_ctx::Slot* trs[4] = {&w1, &w2, &w1, &w2};
@@ -2369,10 +2382,17 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
       const auto &in = *ins[st], &tr = *trs[st], &out = *outs[st];
       if (int rc = baro_correction(c, in)) return rc;
       if (int rc = launch_ghost_fill(c, in)) return rc;
-      if (int rc = launch_stage(c, in, in, tr, 0.0, 1)) return rc;
-      hipLaunchKernelGGL(split_axpy_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC,
-                         c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, tr.m, tr.t,
-                         tr.u, out.m, out.t, out.u);
+      if (f32) {
+        if (int rc = launch_stage_mixed<false, float>(c, in, nullptr, ftr.m, ftr.t, ftr.u, 1.0)) return rc;
+        hipLaunchKernelGGL(split_axpy_kernel<float>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC,
+                           c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, ftr.m,
+                           ftr.t, ftr.u, out.m, out.t, out.u);
+      } else {
+        if (int rc = launch_stage(c, in, in, tr, 0.0, 1)) return rc;
+        hipLaunchKernelGGL(split_axpy_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L,
+                           NC, c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, tr.m,
+                           tr.t, tr.u, out.m, out.t, out.u);
+      }
       HIPCHK(c, hipGetLastError());
       if (int rc = launch_ghost_fill(c, out)) return rc;
     }
