@@ -448,12 +448,19 @@ def free_surface(fine: int):
                 "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15}
 
     pv = LC.vcycle_params(1e-9, 40)
+    # V-cycles and learned cycles with FP32 smoothing (FP64 residuals and update)
+    prob.h.set_precision(W.PRECISION_FP32_BULK)
+    fp32 = {"vcycle": timed(SOLVE_VCYCLE, pv), "learned": timed(SOLVE_LEARNED, pv, learned=True),
+            "note": "each cycle a defect correction: FP64 residual, correction smoothed and transferred in FP32 "
+                    "fields, coarsest BiCGSTAB FP64"}
+    prob.h.set_precision(W.PRECISION_FP64)
     out = {"workload": f"elliptic_lo on J{fine - 2}..J{fine} (finest {10 * 4 ** fine + 2} nodes), dt 1800 s, "
                        "seeded ocean draw 4242, tolerance 1e-9",
            "vcycle": timed(SOLVE_VCYCLE, pv),
            "learned": timed(SOLVE_LEARNED, pv, learned=True),
            "fmg": timed(SOLVE_FMG, MgParams.defaults(coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)),
            "network": source,
+           "fp32_smoothing": fp32,
            "mlp_forward_64_rhs": mlp_line}
     prob.close()
     mlp.close()

@@ -314,6 +314,15 @@ int wtg_mg_solve(wtg_mg* mg, int method, const wtg_mg_params* p, int* iters, dou
  * eta^n to p->fine_tol (at most p->max_vcycle).  *iters = cycles of the
  * last step. */
 int wtg_mg_rk4_split(wtg_mg* mg, int slot, int nsteps, int method, const wtg_mg_params* p, int* iters);
+/* Precision of the V-cycles of WTG_SOLVE_FMG, WTG_SOLVE_VCYCLE and
+ * WTG_SOLVE_LEARNED (the reference's v_cycle, lin_solve.f90:102-139, is
+ * FP64 only).  WTG_PRECISION_FP64 (the default) runs them as the reference;
+ * WTG_PRECISION_FP32_BULK makes each cycle a defect correction: the FP64
+ * residual R = f - L u, its correction equation L e = R smoothed, restricted
+ * and prolonged in FP32 fields, the coarsest level's BiCGSTAB in FP64, and
+ * u := u + e and the residual that judges convergence in FP64.  The FP32
+ * fields are allocated on every level at the first call with this mode. */
+int wtg_mg_set_precision(wtg_mg* mg, int precision);
 /* One level of forward_scalar_transform (wavelet.f90:104-150): on level
  * `fine`, wc_field := Compute_scalar_wavelets of s_field; on level fine - 1,
  * coarse_field := Restrict_scalar of (s_field, wc_field).  s_field must be

@@ -102,6 +102,7 @@ SIGNATURES = {
     "wtg_mg_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "wtg_mg_inverse": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "wtg_mg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
+    "wtg_mg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_mg_npatches": (C.c_int, [_P]),
     "wtg_mg_set_learned": (C.c_int, [_P, _P]),
     "wtg_mg_features": (C.c_int, [_P, C.c_int, _P, _P, _P]),
@@ -437,6 +438,11 @@ class Hierarchy:
     def set_learned(self, mlp: "MLP | None"):
         self._mlp = mlp
         _check(_lib.wtg_mg_set_learned(self._h, mlp._h if mlp is not None else None), self._top, "wtg_mg_set_learned")
+
+    def set_precision(self, precision: int):
+        """PRECISION_FP64 or PRECISION_FP32_BULK (FP32 smoothing and transfers
+        of each V-cycle's correction, FP64 residuals and solution update)"""
+        _check(_lib.wtg_mg_set_precision(self._h, precision), self._top, "wtg_mg_set_precision")
 
     def features(self, field: int):
         """network input rows [npatch, 192], patch RMS sigma [npatch] and patch

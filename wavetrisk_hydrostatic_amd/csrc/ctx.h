@@ -33,6 +33,18 @@ enum FsField {
   FS_NFIELDS
 };
 
+// FP32 fields of the multigrid correction cycle (WTG_PRECISION_FP32_BULK)
+enum Mg32Field {
+  M32_CORR = 0,  // correction, two buffers
+  M32_CORR2,
+  M32_RES,       // residual of the correction equation, two buffers
+  M32_RES2,
+  M32_WC,        // scalar wavelet coefficients
+  M32_PF,        // prolongation of the coarser correction
+  M32_R,         // FP64 residual of the cycle's start, rounded (finest level)
+  M32_NFIELDS
+};
+
 struct wtg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;      // stream of every launch (a hierarchy points it at its finest level's)
@@ -84,6 +96,7 @@ struct wtg_ctx {
   double fs_dt = 0, fs_theta1 = 1, fs_theta2 = 1;
   bool fs_params_set = false;
   double* fs[FS_NFIELDS] = {};   // allocated on first use
+  float* fs32[M32_NFIELDS] = {}; // allocated on the first FP32 correction cycle
   double* d_part = nullptr;      // reduction partials
   unsigned* d_count = nullptr;   // blocks finished in a dot pass (the last one reduces)
   // dot passes of at most this many blocks finish in their last block (one

@@ -56,7 +56,8 @@ struct FsGeo {
 
 // flux of edge e of cell x (step1 itype 2, ops.f90:202-216):
 //   g interp(|topo| + eta_old) * (difference of q along the edge) / len * pedlen
-__device__ __forceinline__ double ell_flux(const FsGeo& g, const double* __restrict__ q,
+template <class F>
+__device__ __forceinline__ double ell_flux(const FsGeo& g, const F* __restrict__ q,
                                            const double* __restrict__ eo, double grav, int64_t x, int e) {
   const int64_t nb = e == 0 ? x + 1 : (e == 1 ? x + g.S + 1 : x + g.S);
   const double cx = grav * (fabs(g.topo[x]) * 1.0 + eo[x]);
@@ -66,7 +67,8 @@ __device__ __forceinline__ double ell_flux(const FsGeo& g, const double* __restr
 }
 
 // elliptic_lo at cell c: coef * div(flux) - q  (div: ops.f90:1325-1343)
-__device__ __forceinline__ double ell_apply_at(const FsGeo& g, const double* __restrict__ q,
+template <class F>
+__device__ __forceinline__ double ell_apply_at(const FsGeo& g, const F* __restrict__ q,
                                                const double* __restrict__ eo, double grav, double coef, int64_t c) {
   const int64_t cW = c - 1, cSW = c - g.S - 1, cS = c - g.S;
   const double dv = (ell_flux(g, q, eo, grav, c, 0) - ell_flux(g, q, eo, grav, cW, 0) +
@@ -101,37 +103,42 @@ __global__ void fs_diag_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, 
 }
 
 // Jacobi_iteration (lin_solve.f90:301-340, cal_jacobi 342-359): out = u + w (f - L u) / diag
-__global__ void fs_jacobi_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const double* __restrict__ u,
-                                 const double* __restrict__ f, const double* __restrict__ eo, double grav, double coef,
-                                 double dt2, double th12, double w, double* __restrict__ out) {
+// (fields of type F: double, or float for the FP32 correction cycle; the
+// arithmetic is FP64 either way)
+template <class F>
+__global__ void fs_jacobi_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const F* __restrict__ u,
+                                 const F* __restrict__ f, const double* __restrict__ eo, double grav, double coef,
+                                 double dt2, double th12, double w, F* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = ncl[2 * t + 1];
   const double Au = ell_apply_at(g, u, eo, grav, coef, c);
   const double dg = ell_diag_at(g, eo, grav, dt2, th12, c);
-  out[ncl[2 * t]] = u[c] + w * (f[c] - Au) / dg;
+  out[ncl[2 * t]] = (F)(u[c] + w * (f[c] - Au) / dg);
 }
 
 // residual (lin_solve.f90:563-618): out = f - L u
-__global__ void fs_residual_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const double* __restrict__ u,
-                                   const double* __restrict__ f, const double* __restrict__ eo, double grav,
-                                   double coef, double* __restrict__ out) {
+template <class Fu, class Ff, class Fo>
+__global__ void fs_residual_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const Fu* __restrict__ u,
+                                   const Ff* __restrict__ f, const double* __restrict__ eo, double grav,
+                                   double coef, Fo* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = ncl[2 * t + 1];
-  out[ncl[2 * t]] = f[c] - ell_apply_at(g, u, eo, grav, coef, c);
+  out[ncl[2 * t]] = (Fo)(f[c] - ell_apply_at(g, u, eo, grav, coef, c));
 }
 
 // lc (lin_solve.f90:520-561): out = a1 s1 + a2 s2, a = k (i < 0) or k * scal[i]
+template <class F1, class F2, class Fo>
 __global__ void fs_lc_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, double k1, int i1,
-                             const double* __restrict__ s1, double k2, int i2, const double* __restrict__ s2,
-                             const double* __restrict__ scal, double* __restrict__ out) {
+                             const F1* __restrict__ s1, double k2, int i2, const F2* __restrict__ s2,
+                             const double* __restrict__ scal, Fo* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || gated(gate)) return;
   const double a1 = i1 < 0 ? k1 : k1 * scal[i1];
   const double a2 = i2 < 0 ? k2 : k2 * scal[i2];
   const int64_t c = ncl[2 * t + 1];
-  out[ncl[2 * t]] = a1 * s1[c] + a2 * s2[c];
+  out[ncl[2 * t]] = (Fo)(a1 * s1[c] + a2 * s2[c]);
 }
 
 // halo copies of a field from its masters (entries nmaster.. of the list)
@@ -141,12 +148,13 @@ __global__ void fs_ghost_copy_kernel(int nmaster, int n, const int64_t* __restri
   f[ncl[2 * t]] = f[ncl[2 * t + 1]];
 }
 
-// out = src (master and halo cells alike)
-__global__ void fs_copy_kernel(int n, const int64_t* __restrict__ ncl, const double* __restrict__ s,
-                               double* __restrict__ out) {
+// out = src (master and halo cells alike), converted to the output type
+template <class Fi, class Fo>
+__global__ void fs_copy_kernel(int n, const int64_t* __restrict__ ncl, const Fi* __restrict__ s,
+                               Fo* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  out[ncl[2 * t]] = s[ncl[2 * t + 1]];
+  out[ncl[2 * t]] = (Fo)s[ncl[2 * t + 1]];
 }
 
 // flux_divergence (barotropic_2d.f90:326-366): div of sum_k u_k interp(mass_k) * pedlen
@@ -499,7 +507,8 @@ __device__ __forceinline__ void mg_ij(int64_t c, int S, int& lz, int& i, int& j)
 
 // Interp_node (wavelet.f90:1487-1498) at fine midpoint x from its four
 // coincident neighbours: the edge's end points and the two opposite nodes
-__device__ __forceinline__ double mg_interp(const MgGeo& g, const double* __restrict__ s, int64_t x, int i, int j) {
+template <class F>
+__device__ __forceinline__ double mg_interp(const MgGeo& g, const F* __restrict__ s, int64_t x, int i, int j) {
   const int64_t S = g.Sf, NC = g.NCf;
   int64_t id1, id2, id3, id4;
   if ((i & 1) && !(j & 1)) {  // RT
@@ -515,7 +524,8 @@ __device__ __forceinline__ double mg_interp(const MgGeo& g, const double* __rest
 
 // lifting around fine coincident cell c (Restrict_scalar wavelet.f90:542-567,
 // twelve terms; Prolong_scalar 586-611 omits id2NE)
-__device__ __forceinline__ double mg_lift(const MgGeo& g, const double* __restrict__ wc, int64_t c, bool with_2ne) {
+template <class F>
+__device__ __forceinline__ double mg_lift(const MgGeo& g, const F* __restrict__ wc, int64_t c, bool with_2ne) {
   const int64_t S = g.Sf, NC = g.NCf;
   const double *a1 = g.ovl, *a2 = g.ovl + NC, *a3 = g.ovl + 2 * NC, *a4 = g.ovl + 3 * NC;
   const int64_t E = c + 1, NE = c + 1 + S, N2E = c + 2 + S, N2NE = c + 1 + 2 * S, N = c + S, W = c - 1,
@@ -532,7 +542,8 @@ __device__ __forceinline__ double mg_lift(const MgGeo& g, const double* __restri
 // second end point) and the five DG midpoints of the second ring (a3 south,
 // a4 north); the prolongation drops the second-ring term of lozenge 1 (south)
 // / 7 (north), the pole's id2NE
-__device__ __forceinline__ double mg_pole_lift(const MgGeo& g, const double* __restrict__ wc, bool south,
+template <class F>
+__device__ __forceinline__ double mg_pole_lift(const MgGeo& g, const F* __restrict__ wc, bool south,
                                                bool prolong) {
   const int64_t S = g.Sf, NC = g.NCf, T = S * S;
   const int n = g.nf;
@@ -555,19 +566,21 @@ __device__ __forceinline__ double mg_pole_lift(const MgGeo& g, const double* __r
 
 // Compute_scalar_wavelets (wavelet.f90:648-670) over the fine compute list:
 // wc = s - Interp_node at midpoints, 0 at coincident nodes
-__global__ void mg_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ s,
-                                  double* __restrict__ wc) {
+template <class F>
+__global__ void mg_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ s,
+                                  F* __restrict__ wc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t x = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
-  wc[ncl[2 * t]] = ((i | j) & 1) ? s[x] - mg_interp(g, s, x, i, j) : 0.0;
+  wc[ncl[2 * t]] = (F)(((i | j) & 1) ? s[x] - mg_interp(g, s, x, i, j) : 0.0);
 }
 
 // Restrict_scalar over the coarse compute list: s_c = s_f(child) + lift * hex_inv_c
-__global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ sf,
-                                   const double* __restrict__ wc, double* __restrict__ sc) {
+template <class F>
+__global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ sf,
+                                   const F* __restrict__ wc, F* __restrict__ sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t p = ncl[2 * t + 1];
@@ -576,13 +589,14 @@ __global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo
   const int64_t c = (int64_t)lz * g.Sf * g.Sf + (int64_t)(2 * j + 2) * g.Sf + (2 * i + 2);
   const bool pole = p == g.sp_c || p == g.np_c;
   const double lift = pole ? mg_pole_lift(g, wc, p == g.sp_c, false) : mg_lift(g, wc, c, true);
-  sc[ncl[2 * t]] = sf[c] + lift * g.hinv_c[p];
+  sc[ncl[2 * t]] = (F)(sf[c] + lift * g.hinv_c[p]);
 }
 
 // Prolong_scalar over the fine compute list, coincident nodes:
 // s_f = s_c(parent) - lift * hex_inv_c
-__global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ sc,
-                                       const double* __restrict__ wc, double* __restrict__ sf) {
+template <class F>
+__global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ sc,
+                                       const F* __restrict__ wc, F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t x = ncl[2 * t + 1];
@@ -592,28 +606,30 @@ __global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, M
   const int64_t p = (int64_t)lz * g.Sc * g.Sc + (int64_t)(j / 2 + 2) * g.Sc + (i / 2 + 2);
   const bool pole = x == g.sp_f || x == g.np_f;
   const double lift = pole ? mg_pole_lift(g, wc, x == g.sp_f, true) : mg_lift(g, wc, x, false);
-  sf[ncl[2 * t]] = sc[p] - lift * g.hinv_c[p];
+  sf[ncl[2 * t]] = (F)(sc[p] - lift * g.hinv_c[p]);
 }
 
 // Reconstruct_scalar (wavelet.f90:613-646) over the fine compute list,
 // midpoints: s_f = Interp_node(coincident s_f) + wc.  Reads only coincident
 // cells and writes only midpoints, so it runs in place after the even pass.
-__global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ wc,
-                                      double* __restrict__ sf) {
+template <class F>
+__global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ wc,
+                                      F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t x = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
   if (!((i | j) & 1)) return;
-  sf[ncl[2 * t]] = mg_interp(g, sf, x, i, j) + wc[x];
+  sf[ncl[2 * t]] = (F)(mg_interp(g, sf, x, i, j) + wc[x]);
 }
 
 // out := v on every cell of the array (work fields; not a memset, so the
 // sequence stays capturable in a graph)
-__global__ void mg_fill_kernel(int64_t n, double v, double* __restrict__ out) {
+template <class F>
+__global__ void mg_fill_kernel(int64_t n, double v, F* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[t] = v;
+  if (t < n) out[t] = (F)v;
 }
 
 
@@ -718,31 +734,35 @@ int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false) {
   return 0;
 }
 
-int launch_residual(wtg_ctx* c, const double* u, const double* f, double* out) {
-  hipLaunchKernelGGL(fs_residual_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
+template <class Fu, class Ff, class Fo>
+int launch_residual(wtg_ctx* c, const Fu* u, const Ff* f, Fo* out) {
+  hipLaunchKernelGGL((fs_residual_kernel<Fu, Ff, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
                      u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
-int launch_jacobi(wtg_ctx* c, const double* u, const double* f, double w, double* out) {
-  hipLaunchKernelGGL(fs_jacobi_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), u,
+template <class F>
+int launch_jacobi(wtg_ctx* c, const F* u, const F* f, double w, F* out) {
+  hipLaunchKernelGGL(fs_jacobi_kernel<F>, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), u,
                      f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->fs_dt * c->fs_dt,
                      c->fs_theta1 * c->fs_theta2, w, out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
-int launch_lc(wtg_ctx* c, double k1, int i1, const double* s1, double k2, int i2, const double* s2, double* out,
+template <class F1, class F2, class Fo>
+int launch_lc(wtg_ctx* c, double k1, int i1, const F1* s1, double k2, int i2, const F2* s2, Fo* out,
               bool gate = false) {
-  hipLaunchKernelGGL(fs_lc_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
+  hipLaunchKernelGGL((fs_lc_kernel<F1, F2, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
                      c->nncl, c->d_ncl, k1, i1, s1, k2, i2, s2, c->d_scal, out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
-int launch_copy(wtg_ctx* c, const double* s, double* out) {
-  hipLaunchKernelGGL(fs_copy_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, s, out);
+template <class Fi, class Fo>
+int launch_copy(wtg_ctx* c, const Fi* s, Fo* out) {
+  hipLaunchKernelGGL((fs_copy_kernel<Fi, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, s, out);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -954,6 +974,9 @@ struct wtg_mg {
   double* d_fnode = nullptr;   // sin(latitude) per cell
   double* d_sigma = nullptr;   // [npatch]
   float* d_feat = nullptr;     // [npatch][192] (feature extraction)
+  // WTG_PRECISION_FP32_BULK: the V-cycles' correction (smoothing, transfers)
+  // in FP32 fields; residuals of the solution and its update stay FP64
+  int precision = WTG_PRECISION_FP64;
 };
 
 namespace {
@@ -972,33 +995,36 @@ MgGeo mg_geo(const wtg_mg* m, int l) {
                f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole};
 }
 
-int mg_fill(wtg_ctx* c, double* a, double v) {
+template <class F>
+int mg_fill(wtg_ctx* c, F* a, double v) {
   const int64_t n = c->info.ncells;
-  hipLaunchKernelGGL(mg_fill_kernel, dim3((unsigned)((n + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream, n, v, a);
+  hipLaunchKernelGGL(mg_fill_kernel<F>, dim3((unsigned)((n + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream, n, v, a);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
 // wc := wavelets of s on level l, s_c := restriction of (s, wc) to level l - 1
-int mg_forward(wtg_mg* m, int l, const double* s, double* wc, double* s_c) {
+template <class F>
+int mg_forward(wtg_mg* m, int l, const F* s, F* wc, F* s_c) {
   wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
   const MgGeo g = mg_geo(m, l);
-  hipLaunchKernelGGL(mg_wavelet_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s, wc);
+  hipLaunchKernelGGL(mg_wavelet_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s, wc);
   WTG_HIPCHK(f, hipGetLastError());
-  hipLaunchKernelGGL(mg_restrict_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, f->stream, c->nncl, c->d_ncl, g, s, wc,
+  hipLaunchKernelGGL(mg_restrict_kernel<F>, dim3(nblk(c->nncl)), dim3(FS_NT), 0, f->stream, c->nncl, c->d_ncl, g, s, wc,
                      s_c);
   WTG_HIPCHK(f, hipGetLastError());
   return 0;
 }
 
 // s_f := inverse transform of (s_c, wc) from level l - 1 to level l
-int mg_inverse(wtg_mg* m, int l, const double* s_c, const double* wc, double* s_f) {
+template <class F>
+int mg_inverse(wtg_mg* m, int l, const F* s_c, const F* wc, F* s_f) {
   wtg_ctx* f = m->lv[l];
   const MgGeo g = mg_geo(m, l);
-  hipLaunchKernelGGL(mg_prolong_even_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
+  hipLaunchKernelGGL(mg_prolong_even_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
                      wc, s_f);
   WTG_HIPCHK(f, hipGetLastError());
-  hipLaunchKernelGGL(mg_prolong_odd_kernel, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
+  hipLaunchKernelGGL(mg_prolong_odd_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
                      s_f);
   WTG_HIPCHK(f, hipGetLastError());
   return 0;
@@ -1012,7 +1038,8 @@ struct MgLv {
   double *u, *u_alt, *f, *res, *res_alt, *corr, *corr_alt;
 };
 
-int lv_jacobi(MgLv& v, double*& u, double*& alt, const double* f, double w, int n) {
+template <class F>
+int lv_jacobi(MgLv& v, F*& u, F*& alt, const F* f, double w, int n) {
   for (int i = 0; i < n; ++i) {
     RC(launch_jacobi(v.c, u, f, w, alt));
     std::swap(u, alt);
@@ -1065,9 +1092,12 @@ int mg_finish(std::vector<MgLv>& V) {
   return 0;
 }
 
+int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err);
+
 // v_cycle (lin_solve.f90:102-139) on level l: u := u + V-cycle correction,
 // Jacobi post-smoothing, *err = ||f - L u|| / nrm_f
 int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
+  if (m->precision == WTG_PRECISION_FP32_BULK) return vcycle32(m, V, l, P, nrm_f, err);
   for (int j = 0; j <= l; ++j) RC(mg_fill(V[j].c, V[j].corr, 0.0));
   RC(launch_residual(V[l].c, V[l].u, V[l].f, V[l].res));
   for (int j = l; j >= 1; --j) {
@@ -1089,6 +1119,67 @@ int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, doubl
   RC(launch_lc(V[l].c, 1.0, -1, V[l].u, 1.0, -1, V[l].corr, V[l].u_alt));
   std::swap(V[l].u, V[l].u_alt);
   RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, 1.0, P.post_iter));
+  return lv_res_err(V[l], V[l].f, V[l].u, nrm_f, err);
+}
+
+// FP32 fields of the correction cycle on every level of the hierarchy
+int fs32_ready(wtg_mg* m) {
+  for (wtg_ctx* c : m->lv) {
+    if (c->fs32[0]) continue;
+    const int64_t NC = c->info.ncells;
+    for (int f = 0; f < M32_NFIELDS; ++f) {
+      int rc = wtg_dalloc<float>(c, &c->fs32[f], NC);
+      if (!rc && hipMemsetAsync(c->fs32[f], 0, NC * sizeof(float), c->stream) != hipSuccess)
+        rc = wtg_fail(c, WTG_E_DEVICE, "hipMemsetAsync of an FP32 multigrid field failed");
+      if (rc) {
+        m->lv[m->nl - 1]->err = c->err;  // errors are reported through the finest level
+        return rc;
+      }
+    }
+  }
+  return 0;
+}
+
+// The V-cycle of vcycle() as a defect correction with FP32 smoothing:
+// R = f - L u in FP64, rounded to FP32; the correction equation L e = R is
+// smoothed, restricted and prolonged in FP32 fields (FP64 arithmetic in
+// registers), the coarsest level solved by FP64 BiCGSTAB; the post-smoothing
+// sweeps on u are the same sweeps on e against R (Jacobi is affine:
+// J(u + e; f) = u + J(e; f - L u)), and u := u + e in FP64, so the solution
+// and its residual never round to FP32.
+int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
+  std::vector<float *> corr(l + 1), corr_alt(l + 1), res(l + 1), res_alt(l + 1);
+  for (int j = 0; j <= l; ++j) {
+    float** b = V[j].c->fs32;
+    corr[j] = b[M32_CORR], corr_alt[j] = b[M32_CORR2], res[j] = b[M32_RES], res_alt[j] = b[M32_RES2];
+    if (j >= 1) RC(mg_fill(V[j].c, corr[j], 0.0));
+  }
+  RC(mg_fill(V[0].c, V[0].corr, 0.0));
+  float* R = V[l].c->fs32[M32_R];
+  RC(launch_residual(V[l].c, V[l].u, V[l].f, R));
+  for (int j = l; j >= 1; --j) {
+    const float* rhs = j == l ? R : res[j];
+    RC(lv_jacobi(V[j], corr[j], corr_alt[j], rhs, 1.0, P.down_iter));
+    RC(launch_residual(V[j].c, corr[j], rhs, res_alt[j]));  // res := rhs - L corr
+    std::swap(res[j], res_alt[j]);
+    RC(mg_forward(m, j, (const float*)res[j], V[j].c->fs32[M32_WC], res[j - 1]));
+  }
+  int cit;
+  double cerr;
+  RC(launch_copy(V[0].c, (const float*)res[0], V[0].res));
+  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr));
+  RC(launch_copy(V[0].c, (const double*)V[0].corr, corr[0]));
+  for (int j = 1; j <= l; ++j) {
+    float *pf = V[j].c->fs32[M32_PF], *wc = V[j].c->fs32[M32_WC];
+    RC(mg_fill(V[j].c, wc, 0.0));
+    RC(mg_inverse(m, j, (const float*)corr[j - 1], (const float*)wc, pf));
+    RC(launch_lc(V[j].c, 1.0, -1, (const float*)corr[j], 1.0, -1, (const float*)pf, corr_alt[j]));
+    std::swap(corr[j], corr_alt[j]);
+    RC(lv_jacobi(V[j], corr[j], corr_alt[j], (const float*)res[j], 1.0, P.up_iter));
+  }
+  RC(lv_jacobi(V[l], corr[l], corr_alt[l], (const float*)R, 1.0, P.post_iter));
+  RC(launch_lc(V[l].c, 1.0, -1, (const double*)V[l].u, 1.0, -1, (const float*)corr[l], V[l].u_alt));
+  std::swap(V[l].u, V[l].u_alt);
   return lv_res_err(V[l], V[l].f, V[l].u, nrm_f, err);
 }
 
@@ -1534,6 +1625,17 @@ int wtg_mg_inverse(wtg_mg* m, int fine, int coarse_field, int wc_field, int fine
 
 
 int wtg_mg_npatches(const wtg_mg* m) { return m ? m->npatch : WTG_E_ARG; }
+
+int wtg_mg_set_precision(wtg_mg* m, int precision) {
+  if (!m) return WTG_E_ARG;
+  wtg_ctx* top = m->lv[m->nl - 1];
+  if (precision != WTG_PRECISION_FP64 && precision != WTG_PRECISION_FP32_BULK)
+    return wtg_fail(top, WTG_E_ARG, "unknown precision mode %d", precision);
+  if (precision == WTG_PRECISION_FP32_BULK)
+    if (int rc = fs32_ready(m)) return rc;
+  m->precision = precision;
+  return 0;
+}
 
 int wtg_mg_set_learned(wtg_mg* m, wtg_mlp* mlp) {
   if (!m) return WTG_E_ARG;
