@@ -89,10 +89,29 @@ __device__ __forceinline__ double ell_diag_at(const FsGeo& g, const double* __re
 
 __global__ void fs_apply_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, FsGeo g,
                                 const double* __restrict__ q, const double* __restrict__ eo, double grav, double coef,
-                                double* __restrict__ out) {
+                                double* __restrict__ out, double* __restrict__ out2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || gated(gate)) return;
-  out[ncl[2 * t]] = ell_apply_at(g, q, eo, grav, coef, ncl[2 * t + 1]);
+  const double v = ell_apply_at(g, q, eo, grav, coef, ncl[2 * t + 1]);
+  out[ncl[2 * t]] = v;
+  if (out2) out2[ncl[2 * t]] = v;
+}
+
+// start of bicgstab (lin_solve.f90:371-388): res = f - L u, res0 = p = s =
+// res; zero_u: the initial guess is 0 (u := 0 and L u = 0, u is not read)
+__global__ void fs_bicg_init_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, double* __restrict__ u,
+                                    const double* __restrict__ f, const double* __restrict__ eo, double grav,
+                                    double coef, int zero_u, double* __restrict__ res, double* __restrict__ res0,
+                                    double* __restrict__ p, double* __restrict__ s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = ncl[2 * t + 1], d = ncl[2 * t];
+  const double r = zero_u ? f[c] - 0.0 : f[c] - ell_apply_at(g, (const double*)u, eo, grav, coef, c);
+  res[d] = r;
+  res0[d] = r;
+  p[d] = r;
+  s[d] = r;
+  if (zero_u) u[d] = 0.0;
 }
 
 __global__ void fs_diag_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const double* __restrict__ eo,
@@ -105,15 +124,20 @@ __global__ void fs_diag_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, 
 // Jacobi_iteration (lin_solve.f90:301-340, cal_jacobi 342-359): out = u + w (f - L u) / diag
 // (fields of type F: double, or float for the FP32 correction cycle; the
 // arithmetic is FP64 either way)
-template <class F>
+// Zero: the sweep starts from u = 0 (L 0 = 0), u is not read
+template <class F, bool Zero = false>
 __global__ void fs_jacobi_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const F* __restrict__ u,
                                  const F* __restrict__ f, const double* __restrict__ eo, double grav, double coef,
                                  double dt2, double th12, double w, F* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = ncl[2 * t + 1];
-  const double Au = ell_apply_at(g, u, eo, grav, coef, c);
   const double dg = ell_diag_at(g, eo, grav, dt2, th12, c);
+  if (Zero) {
+    out[ncl[2 * t]] = (F)(0.0 + w * (f[c] - 0.0) / dg);
+    return;
+  }
+  const double Au = ell_apply_at(g, u, eo, grav, coef, c);
   out[ncl[2 * t]] = (F)(u[c] + w * (f[c] - Au) / dg);
 }
 
@@ -594,7 +618,8 @@ __global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo
 
 // Prolong_scalar over the fine compute list, coincident nodes:
 // s_f = s_c(parent) - lift * hex_inv_c
-template <class F>
+// ZeroW: all wavelets zero (prol_fun), wc is not read
+template <class F, bool ZeroW = false>
 __global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ sc,
                                        const F* __restrict__ wc, F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,6 +629,10 @@ __global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, M
   mg_ij(x, g.Sf, lz, i, j);
   if ((i | j) & 1) return;
   const int64_t p = (int64_t)lz * g.Sc * g.Sc + (int64_t)(j / 2 + 2) * g.Sc + (i / 2 + 2);
+  if (ZeroW) {
+    sf[ncl[2 * t]] = sc[p];
+    return;
+  }
   const bool pole = x == g.sp_f || x == g.np_f;
   const double lift = pole ? mg_pole_lift(g, wc, x == g.sp_f, true) : mg_lift(g, wc, x, false);
   sf[ncl[2 * t]] = (F)(sc[p] - lift * g.hinv_c[p]);
@@ -612,7 +641,7 @@ __global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, M
 // Reconstruct_scalar (wavelet.f90:613-646) over the fine compute list,
 // midpoints: s_f = Interp_node(coincident s_f) + wc.  Reads only coincident
 // cells and writes only midpoints, so it runs in place after the even pass.
-template <class F>
+template <class F, bool ZeroW = false>
 __global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ wc,
                                       F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,7 +650,7 @@ __global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, Mg
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
   if (!((i | j) & 1)) return;
-  sf[ncl[2 * t]] = (F)(mg_interp(g, sf, x, i, j) + wc[x]);
+  sf[ncl[2 * t]] = ZeroW ? (F)mg_interp(g, sf, x, i, j) : (F)(mg_interp(g, sf, x, i, j) + wc[x]);
 }
 
 // out := v on every cell of the array (work fields; not a memset, so the
@@ -727,9 +756,9 @@ int fs_need_params(wtg_ctx* c) {
 
 double fs_coef(const wtg_ctx* c) { return c->fs_theta1 * c->fs_theta2 * (c->fs_dt * c->fs_dt); }
 
-int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false) {
+int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false, double* out2 = nullptr) {
   hipLaunchKernelGGL(fs_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
-                     c->nncl, c->d_ncl, fs_geo(c), q, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out);
+                     c->nncl, c->d_ncl, fs_geo(c), q, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out, out2);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -742,9 +771,9 @@ int launch_residual(wtg_ctx* c, const Fu* u, const Ff* f, Fo* out) {
   return 0;
 }
 
-template <class F>
+template <class F, bool Zero = false>
 int launch_jacobi(wtg_ctx* c, const F* u, const F* f, double w, F* out) {
-  hipLaunchKernelGGL(fs_jacobi_kernel<F>, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), u,
+  hipLaunchKernelGGL((fs_jacobi_kernel<F, Zero>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), u,
                      f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->fs_dt * c->fs_dt,
                      c->fs_theta1 * c->fs_theta2, w, out);
   WTG_HIPCHK(c, hipGetLastError());
@@ -825,17 +854,19 @@ int read_scalar(wtg_ctx* c, int i, double* out) {
 // masters in place and its halo copies refreshed after the loop.
 constexpr int BICG_CHUNK = 8;
 
-int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int* iters, double* err_out) {
+// zero_u: start from u = 0 (u is overwritten); sync_end: wait for the
+// final ghost copy of u before returning (callers that stay on the stream
+// need not)
+int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int* iters, double* err_out,
+             bool zero_u = false, bool sync_end = true) {
   double** W = c->fs + FS_W0;
   double *res = W[0], *res0 = W[1], *p = W[2], *s = W[3], *Ap = W[4], *As = W[5], *p_alt = W[6];
-  RC(launch_dots(c, 1, f, f, SC_T0));  // nrm_f = l2(f)
-  RC(launch_scalar(c, OP_NRMF));
-  RC(launch_residual(c, u, f, res));
-  RC(launch_copy(c, res, res0));
-  RC(launch_copy(c, res0, p));
-  RC(launch_copy(c, res0, s));
-  RC(launch_apply(c, p, Ap));
-  RC(launch_copy(c, Ap, As));
+  RC(launch_dots_g(c, false, OP_NRMF, 1, f, f, SC_T0));  // nrm_f = l2(f)
+  hipLaunchKernelGGL(fs_bicg_init_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl,
+                     fs_geo(c), u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), zero_u ? 1 : 0, res, res0, p,
+                     s);
+  WTG_HIPCHK(c, hipGetLastError());
+  RC(launch_apply(c, p, Ap, false, As));  // Ap = As = L p
   RC(launch_dots(c, 1, res0, res, SC_RHO));
   RC(launch_scalar(c, OP_LOOP_INIT, false, tol));
   int queued = 0;
@@ -872,8 +903,12 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
     return 0;
   };
   static_assert(BICG_CHUNK % 2 == 0, "a chunk must return p and Ap to their buffers");
+  // the first chunk: the iterations the previous solve on this context took
+  // (repeated solves, such as the coarse level of V-cycles, need about as
+  // many), so few gated no-op iterations are queued past convergence
+  const int first = c->bicg_hint > 0 ? c->bicg_hint : BICG_CHUNK;
   while (queued < maxit && !done) {
-    const int k = std::min(BICG_CHUNK, maxit - queued);
+    const int k = std::min(queued == 0 ? first : BICG_CHUNK, maxit - queued);
     if (c->fs_graph && k == BICG_CHUNK) {
       // the operator's coefficients are kernel arguments of the graph
       const double key[2] = {fs_coef(c), c->params.grav};
@@ -919,9 +954,10 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
                        c->d_ncl, u);
     WTG_HIPCHK(c, hipGetLastError());
   }
-  WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (sync_end) WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   *iters = done ? (int)c->h_scal[SC_ITER] : maxit + 1;
   *err_out = queued > 0 ? c->h_scal[SC_ERR] : 0.0;
+  if (done) c->bicg_hint = *iters;
   return 0;
 }
 
@@ -1017,14 +1053,15 @@ int mg_forward(wtg_mg* m, int l, const F* s, F* wc, F* s_c) {
 }
 
 // s_f := inverse transform of (s_c, wc) from level l - 1 to level l
-template <class F>
+// ZeroW: all wavelets zero (wc may be null)
+template <class F, bool ZeroW = false>
 int mg_inverse(wtg_mg* m, int l, const F* s_c, const F* wc, F* s_f) {
   wtg_ctx* f = m->lv[l];
   const MgGeo g = mg_geo(m, l);
-  hipLaunchKernelGGL(mg_prolong_even_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
+  hipLaunchKernelGGL((mg_prolong_even_kernel<F, ZeroW>), dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
                      wc, s_f);
   WTG_HIPCHK(f, hipGetLastError());
-  hipLaunchKernelGGL(mg_prolong_odd_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
+  hipLaunchKernelGGL((mg_prolong_odd_kernel<F, ZeroW>), dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
                      s_f);
   WTG_HIPCHK(f, hipGetLastError());
   return 0;
@@ -1068,10 +1105,18 @@ int lv_l2(MgLv& v, const double* a, double* out) {
 
 // zero-wavelet prolongation (prolong / prol_fun, lin_solve.f90:244-269) of
 // level l - 1's coarse field into level l
-int lv_prolong0(wtg_mg* m, std::vector<MgLv>& V, int l, const double* coarse, double* fine) {
-  double* wc = V[l].c->fs[MG_WC];
-  RC(mg_fill(V[l].c, wc, 0.0));
-  return mg_inverse(m, l, coarse, wc, fine);
+template <class F>
+int lv_prolong0(wtg_mg* m, int l, const F* coarse, F* fine) {
+  return mg_inverse<F, true>(m, l, coarse, nullptr, fine);
+}
+
+// n Jacobi sweeps from u = 0 (the first reads no u; n = 0 leaves u = 0)
+template <class F>
+int lv_jacobi0(MgLv& v, F*& u, F*& alt, const F* f, double w, int n) {
+  if (n == 0) return mg_fill(v.c, u, 0.0);
+  RC((launch_jacobi<F, true>(v.c, u, f, w, alt)));
+  std::swap(u, alt);
+  return lv_jacobi(v, u, alt, f, w, n - 1);
 }
 
 std::vector<MgLv> mg_levels(wtg_mg* m) {
@@ -1098,20 +1143,19 @@ int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, dou
 // Jacobi post-smoothing, *err = ||f - L u|| / nrm_f
 int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
   if (m->precision == WTG_PRECISION_FP32_BULK) return vcycle32(m, V, l, P, nrm_f, err);
-  for (int j = 0; j <= l; ++j) RC(mg_fill(V[j].c, V[j].corr, 0.0));
   RC(launch_residual(V[l].c, V[l].u, V[l].f, V[l].res));
   for (int j = l; j >= 1; --j) {
-    RC(lv_jacobi(V[j], V[j].corr, V[j].corr_alt, V[j].res, 1.0, P.down_iter));
+    RC(lv_jacobi0(V[j], V[j].corr, V[j].corr_alt, (const double*)V[j].res, 1.0, P.down_iter));  // from corr = 0
     RC(launch_residual(V[j].c, V[j].corr, V[j].res, V[j].res_alt));  // res := res - L corr
     std::swap(V[j].res, V[j].res_alt);
     RC(mg_forward(m, j, V[j].res, V[j].c->fs[MG_WC], V[j - 1].res));
   }
   int cit;
   double cerr;
-  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr));
+  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr, true, false));
   for (int j = 1; j <= l; ++j) {
     double* pf = V[j].c->fs[MG_PF];
-    RC(lv_prolong0(m, V, j, V[j - 1].corr, pf));
+    RC(lv_prolong0(m, j, (const double*)V[j - 1].corr, pf));
     RC(launch_lc(V[j].c, 1.0, -1, V[j].corr, 1.0, -1, pf, V[j].corr_alt));
     std::swap(V[j].corr, V[j].corr_alt);
     RC(lv_jacobi(V[j], V[j].corr, V[j].corr_alt, V[j].res, 1.0, P.up_iter));
@@ -1152,14 +1196,12 @@ int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, dou
   for (int j = 0; j <= l; ++j) {
     float** b = V[j].c->fs32;
     corr[j] = b[M32_CORR], corr_alt[j] = b[M32_CORR2], res[j] = b[M32_RES], res_alt[j] = b[M32_RES2];
-    if (j >= 1) RC(mg_fill(V[j].c, corr[j], 0.0));
   }
-  RC(mg_fill(V[0].c, V[0].corr, 0.0));
   float* R = V[l].c->fs32[M32_R];
   RC(launch_residual(V[l].c, V[l].u, V[l].f, R));
   for (int j = l; j >= 1; --j) {
     const float* rhs = j == l ? R : res[j];
-    RC(lv_jacobi(V[j], corr[j], corr_alt[j], rhs, 1.0, P.down_iter));
+    RC(lv_jacobi0(V[j], corr[j], corr_alt[j], rhs, 1.0, P.down_iter));  // from corr = 0
     RC(launch_residual(V[j].c, corr[j], rhs, res_alt[j]));  // res := rhs - L corr
     std::swap(res[j], res_alt[j]);
     RC(mg_forward(m, j, (const float*)res[j], V[j].c->fs32[M32_WC], res[j - 1]));
@@ -1167,12 +1209,11 @@ int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, dou
   int cit;
   double cerr;
   RC(launch_copy(V[0].c, (const float*)res[0], V[0].res));
-  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr));
+  RC(bicgstab(V[0].c, V[0].corr, V[0].res, P.coarse_tol, P.coarse_iter, &cit, &cerr, true, false));
   RC(launch_copy(V[0].c, (const double*)V[0].corr, corr[0]));
   for (int j = 1; j <= l; ++j) {
-    float *pf = V[j].c->fs32[M32_PF], *wc = V[j].c->fs32[M32_WC];
-    RC(mg_fill(V[j].c, wc, 0.0));
-    RC(mg_inverse(m, j, (const float*)corr[j - 1], (const float*)wc, pf));
+    float* pf = V[j].c->fs32[M32_PF];
+    RC(lv_prolong0(m, j, (const float*)corr[j - 1], pf));
     RC(launch_lc(V[j].c, 1.0, -1, (const float*)corr[j], 1.0, -1, (const float*)pf, corr_alt[j]));
     std::swap(corr[j], corr_alt[j]);
     RC(lv_jacobi(V[j], corr[j], corr_alt[j], (const float*)res[j], 1.0, P.up_iter));
@@ -1198,7 +1239,7 @@ int mg_fmg(wtg_mg* m, const wtg_mg_params& P, int* iters, double* res_out) {
   RC(bicgstab(V[0].c, V[0].u, V[0].f, P.coarse_tol, P.coarse_iter, &iters[0], &res_out[0]));
   for (int l = 1; l < nl; ++l) {
     if (nrm1[l] < P.fine_tol) continue;
-    RC(lv_prolong0(m, V, l, V[l - 1].u, V[l].u));
+    RC(lv_prolong0(m, l, (const double*)V[l - 1].u, V[l].u));
     RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, 1.0, P.pre_iter));
     for (int it = 1; it <= P.max_vcycle; ++it) {
       RC(vcycle(m, V, l, P, nrm_f[l], &res_out[l]));
@@ -1233,7 +1274,7 @@ int mg_srj(wtg_mg* m, const wtg_mg_params& P, int* iters, double* res_out) {
   }
   RC(bicgstab(V[0].c, V[0].u, V[0].f, P.coarse_tol, P.coarse_iter, &iters[0], &res_out[0]));
   for (int l = 1; l < nl; ++l) {
-    RC(lv_prolong0(m, V, l, V[l - 1].u, V[l].u));
+    RC(lv_prolong0(m, l, (const double*)V[l - 1].u, V[l].u));
     int ii = 0;
     for (int it = 1; it <= max_it; ++it) {
       if (++ii > M) {
