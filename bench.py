@@ -59,11 +59,16 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--members", type=int, default=16, help="ensemble members of the batched-ensemble line")
     ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
+    ap.add_argument("--only-free-surface", action="store_true",
+                    help="run only the free-surface line and print it (for profiling the solver)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding of the lozenge domains is a design note, not a code path", file=sys.stderr)
         sys.exit(2)
+    if args.only_free_surface:
+        print(json.dumps({"free_surface": free_surface(args.fs_level)}))
+        return
 
     import numpy as np
 
