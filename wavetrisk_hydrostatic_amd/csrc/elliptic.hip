@@ -219,6 +219,30 @@ __global__ void fs_rhs_kernel(int n, const int64_t* __restrict__ ncl, const doub
 
 // weighted dot products over the reduction cells: part[b*RED_MAX + k] =
 // sum over this block's cells of w * (a_k * b_k); a fixed-order tree
+// Sum over the block of acc[k] into sh[k][0] (read by thread 0): pairwise
+// tree, thread t adding t + s for s = FS_NT/2 .. 1.  The steps across waves
+// go through LDS; the last six (s = 32 .. 1) run in wave 0 with shuffles,
+// which pair the same lanes, so the sums are those of the all-LDS tree bit
+// for bit, without their barriers.
+__device__ __forceinline__ void block_tree(double acc[RED_MAX], double (*sh)[FS_NT]) {
+  static_assert(FS_NT >= 128 && FS_NT % 64 == 0, "tree over whole waves");
+  for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = FS_NT / 2; s >= 64; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 64) {
+    double v[RED_MAX];
+    for (int k = 0; k < RED_MAX; ++k) v[k] = sh[k][threadIdx.x];
+    for (int s = 32; s > 0; s >>= 1)
+      for (int k = 0; k < RED_MAX; ++k) v[k] = v[k] + __shfl_down(v[k], s, 64);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < RED_MAX; ++k) sh[k][0] = v[k];
+  }
+}
+
 __global__ __launch_bounds__(FS_NT) void fs_dot_kernel(const double* __restrict__ gate, int nred,
                                                        const int64_t* __restrict__ rcell,
                                                        const double* __restrict__ rw, int nd,
@@ -237,14 +261,9 @@ __global__ __launch_bounds__(FS_NT) void fs_dot_kernel(const double* __restrict_
     for (int k = 0; k < RED_MAX; ++k)
       if (k < nd) acc[k] = acc[k] + w * (A[k][c] * B[k][c]);
   }
-  for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int s = FS_NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x < RED_MAX) part[blockIdx.x * RED_MAX + threadIdx.x] = sh[threadIdx.x][0];
+  block_tree(acc, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < RED_MAX; ++k) part[blockIdx.x * RED_MAX + k] = sh[k][0];
 }
 
 // scal[out_k] = sum of the partials of dot k (one block, fixed order)
@@ -259,13 +278,7 @@ __global__ __launch_bounds__(FS_NT) void fs_dot_final_kernel(int gated_loop, int
   double acc[RED_MAX] = {0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nblocks; b += FS_NT)
     for (int k = 0; k < RED_MAX; ++k) acc[k] = acc[k] + part[b * RED_MAX + k];
-  for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int s = FS_NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
-    __syncthreads();
-  }
+  block_tree(acc, sh);
   if (threadIdx.x == 0) {
     const int o[RED_MAX] = {o0, o1, o2};
     for (int k = 0; k < nd; ++k) scal[o[k]] = sh[k][0];
@@ -341,14 +354,9 @@ __global__ void fs_scalar_kernel(int op, int gated_loop, double arg, double* __r
 
 // tree over this block's RED_MAX accumulators; thread k < RED_MAX writes part[b][k]
 __device__ __forceinline__ void dot_block_tree(double acc[RED_MAX], double (*sh)[FS_NT], double* part, int b) {
-  for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int s = FS_NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int k = 0; k < RED_MAX; ++k) sh[k][threadIdx.x] = sh[k][threadIdx.x] + sh[k][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (part && threadIdx.x < RED_MAX) part[b * RED_MAX + threadIdx.x] = sh[threadIdx.x][0];
+  block_tree(acc, sh);
+  if (part && threadIdx.x == 0)
+    for (int k = 0; k < RED_MAX; ++k) part[b * RED_MAX + k] = sh[k][0];
 }
 
 __device__ __forceinline__ void dot_last_block(int nbred, int nd, const double* part, int o0, int o1, int o2,
