@@ -107,6 +107,7 @@ struct wtg_ctx {
   double* h_scal = nullptr;      // pinned host copy (hipHostMalloc)
   int part_blocks = 0;
   int bicg_hint = 0;             // iterations of the last converged BiCGSTAB solve (its first chunk)
+  double bicg_hint_tol = 0.0;    // and its tolerance
   // BiCGSTAB chunks of gated iterations replayed from a captured graph
   // (wtg_fs_set_graph), keyed by the solution and right-hand side arrays
   bool fs_graph = false;

@@ -912,9 +912,10 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   };
   static_assert(BICG_CHUNK % 2 == 0, "a chunk must return p and Ap to their buffers");
   // the first chunk: the iterations the previous solve on this context took
-  // (repeated solves, such as the coarse level of V-cycles, need about as
-  // many), so few gated no-op iterations are queued past convergence
-  const int first = c->bicg_hint > 0 ? c->bicg_hint : BICG_CHUNK;
+  // to the same tolerance (repeated solves, such as the coarse level of
+  // V-cycles or the free surface of successive steps, need about as many),
+  // so few gated no-op iterations are queued past convergence
+  const int first = c->bicg_hint > 0 && c->bicg_hint_tol == tol ? c->bicg_hint : BICG_CHUNK;
   while (queued < maxit && !done) {
     const int k = std::min(queued == 0 ? first : BICG_CHUNK, maxit - queued);
     if (c->fs_graph && k == BICG_CHUNK) {
@@ -965,7 +966,10 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   if (sync_end) WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   *iters = done ? (int)c->h_scal[SC_ITER] : maxit + 1;
   *err_out = queued > 0 ? c->h_scal[SC_ERR] : 0.0;
-  if (done) c->bicg_hint = *iters;
+  if (done) {
+    c->bicg_hint = *iters;
+    c->bicg_hint_tol = tol;
+  }
   return 0;
 }
 
@@ -1145,16 +1149,23 @@ int mg_finish(std::vector<MgLv>& V) {
   return 0;
 }
 
-int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err);
+int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err,
+             const double* r_in);
 
 // v_cycle (lin_solve.f90:102-139) on level l: u := u + V-cycle correction,
 // Jacobi post-smoothing, *err = ||f - L u|| / nrm_f
-int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
-  if (m->precision == WTG_PRECISION_FP32_BULK) return vcycle32(m, V, l, P, nrm_f, err);
-  RC(launch_residual(V[l].c, V[l].u, V[l].f, V[l].res));
+// r_in: f - L u of the current u when the caller holds it (the previous
+// cycle's convergence test computed it), else null
+int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err,
+           const double* r_in = nullptr) {
+  if (l == 0) r_in = nullptr;  // one level: the coarse solve reads V[0].res
+  // one level has nothing to smooth: the FP64 coarse solve is the cycle
+  if (m->precision == WTG_PRECISION_FP32_BULK && l >= 1) return vcycle32(m, V, l, P, nrm_f, err, r_in);
+  if (!r_in) RC(launch_residual(V[l].c, V[l].u, V[l].f, V[l].res));
   for (int j = l; j >= 1; --j) {
-    RC(lv_jacobi0(V[j], V[j].corr, V[j].corr_alt, (const double*)V[j].res, 1.0, P.down_iter));  // from corr = 0
-    RC(launch_residual(V[j].c, V[j].corr, V[j].res, V[j].res_alt));  // res := res - L corr
+    const double* rhs = j == l && r_in ? r_in : V[j].res;
+    RC(lv_jacobi0(V[j], V[j].corr, V[j].corr_alt, rhs, 1.0, P.down_iter));  // from corr = 0
+    RC(launch_residual(V[j].c, V[j].corr, rhs, V[j].res_alt));  // res := res - L corr
     std::swap(V[j].res, V[j].res_alt);
     RC(mg_forward(m, j, V[j].res, V[j].c->fs[MG_WC], V[j - 1].res));
   }
@@ -1199,14 +1210,18 @@ int fs32_ready(wtg_mg* m) {
 // sweeps on u are the same sweeps on e against R (Jacobi is affine:
 // J(u + e; f) = u + J(e; f - L u)), and u := u + e in FP64, so the solution
 // and its residual never round to FP32.
-int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err) {
+int vcycle32(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, double nrm_f, double* err,
+             const double* r_in) {
   std::vector<float *> corr(l + 1), corr_alt(l + 1), res(l + 1), res_alt(l + 1);
   for (int j = 0; j <= l; ++j) {
     float** b = V[j].c->fs32;
     corr[j] = b[M32_CORR], corr_alt[j] = b[M32_CORR2], res[j] = b[M32_RES], res_alt[j] = b[M32_RES2];
   }
   float* R = V[l].c->fs32[M32_R];
-  RC(launch_residual(V[l].c, V[l].u, V[l].f, R));
+  if (r_in)
+    RC(launch_copy(V[l].c, r_in, R));
+  else
+    RC(launch_residual(V[l].c, V[l].u, V[l].f, R));
   for (int j = l; j >= 1; --j) {
     const float* rhs = j == l ? R : res[j];
     RC(lv_jacobi0(V[j], corr[j], corr_alt[j], rhs, 1.0, P.down_iter));  // from corr = 0
@@ -1350,10 +1365,11 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
   double err;
   RC(lv_res_err(V[l], V[l].f, V[l].u, nrm_f, &err));
   res_out[l] = err;
+  // lv_res_err leaves f - L u of the current u in the top level's FS_W0
   for (int it = 1; it <= P.max_vcycle && err > P.fine_tol; ++it) {
+    const double* r_top = c->fs[FS_W0];
     if (learned) {
-      double *r = c->fs[FS_W0 + 1], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3];
-      RC(launch_residual(c, V[l].u, V[l].f, r));
+      double *r = c->fs[FS_W0], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3];
       RC(lc_correction(m, r, e));
       RC(launch_apply(c, e, w));
       // the step is computed and applied on the device (a = 0 leaves u unchanged)
@@ -1361,7 +1377,7 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
       RC(launch_lc(c, 1.0, -1, V[l].u, 1.0, SC_MR, e, V[l].u_alt));
       std::swap(V[l].u, V[l].u_alt);
     }
-    RC(vcycle(m, V, l, P, nrm_f, &err));
+    RC(vcycle(m, V, l, P, nrm_f, &err, learned ? nullptr : r_top));  // the learned step moved u
     iters[l] = it;
     res_out[l] = err;
   }
