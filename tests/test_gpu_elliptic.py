@@ -128,6 +128,35 @@ def test_graph_bicgstab_equals_eager_launches(name):
     assert out[True][1][0] == 4  # max_iter + 1: not converged
 
 
+@pytest.mark.parametrize("name", ["J5"])
+def test_solves_in_sequence_equal_fresh_solves(name):
+    """The first chunk of gated iterations follows the previous solve on the
+    context (at the same tolerance): a sequence of solves at different
+    tolerances and iteration limits on one context gives every solve's
+    iterates, residual and count bit for bit as a fresh context does."""
+    print(f"BiCGSTAB solves in sequence {name}", flush=True)
+    seq = ((1e-2, 400), (1e-12, 400), (1e-2, 400), (1e-2, 2), (1e-12, 400), (1e-6, 400))
+    f, g, o, ctx, masters = setup(name)
+    eo = o.node_ghost(np.nan_to_num(f["eta_old"]))
+    rhs = o.node_ghost(np.nan_to_num(f["rhs"]))
+
+    def solve(c, tol, maxit):
+        c.fs_upload(W.FS_ETA_OLD, eo)
+        c.fs_upload(W.FS_ETA, eo)
+        c.fs_upload(W.FS_RHS, rhs)
+        iters, res = c.elliptic_solve(W.SOLVE_BICGSTAB, tol, maxit)
+        return iters, res, c.fs_download(W.FS_ETA)
+
+    for tol, maxit in seq:
+        i1, r1, s1 = solve(ctx, tol, maxit)
+        fresh = setup(name)[3]
+        i2, r2, s2 = solve(fresh, tol, maxit)
+        fresh.close()
+        print(f"  tol {tol:.0e} maxit {maxit}: {i1} iterations, residual {r1:.3e}", flush=True)
+        assert i1 == i2 and r1 == r2 and np.array_equal(s1, s2)
+        assert (r1 <= tol) == (i1 <= maxit)
+
+
 def test_jacobi_sweeps_equal_oracle_J4():
     f, g, o, ctx, masters = setup("J4")
     eo = o.node_ghost(np.nan_to_num(f["eta_old"]))

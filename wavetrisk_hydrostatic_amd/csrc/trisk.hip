@@ -19,6 +19,7 @@
 // without FMA contraction, so results match the CPU oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1094,24 +1095,47 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
 }
 
 // halo := owner (nodes: mass and temp; edges: velocity with sign), all
-// layers; F64 state slots and the FP32 stage buffers (F = float)
+// layers; F64 state slots and the FP32 stage buffers (F = float).  Thread =
+// (list entry, chunk of GF_KC layers), entries fastest, so neighbouring
+// threads copy neighbouring list entries of one layer plane; a thread reads
+// its chunk's sources before it writes (a list's destinations are never its
+// sources, the entries are independent).
+constexpr int GF_KC = 8;
 template <class F>
 __global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__ nd, int nn,
                                   const int64_t* __restrict__ ed, const int8_t* __restrict__ es, int ne, F* m, F* t,
                                   F* u) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)(nn + ne) * L;
-  if (idx >= total) return;
-  int k = (int)(idx % L);
-  int64_t e = idx / L;
+  const int64_t E = (int64_t)nn + ne;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= E * ((L + GF_KC - 1) / GF_KC)) return;
+  const int64_t e = idx % E;
+  const int k0 = (int)(idx / E) * GF_KC, nk = min(GF_KC, L - k0);
   if (e < nn) {
-    int64_t dst = nd[2 * e], src = nd[2 * e + 1];
-    m[(int64_t)k * NC + dst] = m[(int64_t)k * NC + src];
-    t[(int64_t)k * NC + dst] = t[(int64_t)k * NC + src];
+    const int64_t dst = nd[2 * e], src = nd[2 * e + 1];
+    F vm[GF_KC], vt[GF_KC];
+#pragma unroll
+    for (int i = 0; i < GF_KC; ++i)
+      if (i < nk) {
+        vm[i] = m[(int64_t)(k0 + i) * NC + src];
+        vt[i] = t[(int64_t)(k0 + i) * NC + src];
+      }
+#pragma unroll
+    for (int i = 0; i < GF_KC; ++i)
+      if (i < nk) {
+        m[(int64_t)(k0 + i) * NC + dst] = vm[i];
+        t[(int64_t)(k0 + i) * NC + dst] = vt[i];
+      }
   } else {
-    e -= nn;
-    int64_t dst = ed[2 * e], src = ed[2 * e + 1];  // plane indices e*NC + cell
-    u[(int64_t)k * 3 * NC + dst] = (F)es[e] * u[(int64_t)k * 3 * NC + src];
+    const int64_t j = e - nn;
+    const int64_t dst = ed[2 * j], src = ed[2 * j + 1];  // plane indices e*NC + cell
+    const F sg = (F)es[j];
+    F vu[GF_KC];
+#pragma unroll
+    for (int i = 0; i < GF_KC; ++i)
+      if (i < nk) vu[i] = u[(int64_t)(k0 + i) * 3 * NC + src];
+#pragma unroll
+    for (int i = 0; i < GF_KC; ++i)
+      if (i < nk) u[(int64_t)(k0 + i) * 3 * NC + dst] = sg * vu[i];
   }
 }
 
@@ -2023,7 +2047,7 @@ int check_slot(wtg_ctx* c, int slot) {
 template <class F>
 int launch_ghost_fill(wtg_ctx* c, F* m, F* t, F* u) {
   const int L = c->params.nlayers * c->nmembers;
-  int64_t total = (int64_t)(c->nn + c->ne) * L;
+  int64_t total = (int64_t)(c->nn + c->ne) * ((L + GF_KC - 1) / GF_KC);
   if (total == 0) return 0;
   int threads = 256;
   int64_t blocks = (total + threads - 1) / threads;
@@ -2587,6 +2611,27 @@ int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmem
     }
   }
   c->nmaster = (int)(ncl.size() / 2);
+  // ghost entries in destination order: the entries are independent copies,
+  // and neighbouring threads of the fill then touch neighbouring halo cells
+  {
+    std::vector<size_t> o(nd.size() / 2);
+    for (size_t q = 0; q < o.size(); ++q) o[q] = q;
+    std::sort(o.begin(), o.end(), [&](size_t a, size_t b) { return nd[2 * a] < nd[2 * b]; });
+    std::vector<int64_t> nd2(nd.size());
+    for (size_t q = 0; q < o.size(); ++q) nd2[2 * q] = nd[2 * o[q]], nd2[2 * q + 1] = nd[2 * o[q] + 1];
+    nd.swap(nd2);
+    std::vector<size_t> oe(ed.size() / 2);
+    for (size_t q = 0; q < oe.size(); ++q) oe[q] = q;
+    std::sort(oe.begin(), oe.end(), [&](size_t a, size_t b) { return ed[2 * a] < ed[2 * b]; });
+    std::vector<int64_t> ed2(ed.size());
+    std::vector<int8_t> es2(es.size());
+    for (size_t q = 0; q < oe.size(); ++q) {
+      ed2[2 * q] = ed[2 * oe[q]], ed2[2 * q + 1] = ed[2 * oe[q] + 1];
+      es2[q] = es[oe[q]];
+    }
+    ed.swap(ed2);
+    es.swap(es2);
+  }
   for (size_t q = 0; q < nd.size(); q += 2) {
     ncl.push_back(nd[q]);
     ncl.push_back(nd[q + 1]);
@@ -2743,7 +2788,7 @@ int wtg_ghost_fill_split_timed(wtg_ctx* c, int slot, int nshards, int reps, doub
                 hipMemcpy(de, edp[r].data(), edp[r].size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(dsg, esp[r].data(), esp[r].size(), hipMemcpyHostToDevice) != hipSuccess))
       rc = fail(c, WTG_E_DEVICE, "upload of the split lists failed");
-    const int64_t total = (int64_t)(nn + ne) * L;
+    const int64_t total = (int64_t)(nn + ne) * ((L + GF_KC - 1) / GF_KC);
     if (!rc && total > 0) {
       const unsigned blocks = (unsigned)((total + 255) / 256);
       hipEventRecord(e0, c->stream);
