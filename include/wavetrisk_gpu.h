@@ -99,8 +99,9 @@ int wtg_create(const wtg_grid* grid, const wtg_params* params, int nslots, wtg_c
  * scalars [member][layer][cell], velocity [member][layer][edge][cell], and
  * every layer kernel runs all members in one launch (members on the grid's
  * second dimension); each member has its own topography.  wtg_trend,
- * wtg_rk4 and the ghost fill take the whole ensemble; the free-surface calls
- * need nmembers = 1. */
+ * wtg_rk4, wtg_rk4_split and the ghost fill take the whole ensemble (the
+ * free-surface fields then hold [member][cell]); the other free-surface
+ * calls and hierarchies need nmembers = 1. */
 int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmembers, int nslots, wtg_ctx** out);
 void wtg_destroy(wtg_ctx* ctx);
 
@@ -217,8 +218,9 @@ typedef struct {
 } wtg_fs_params;
 
 int wtg_fs_set_params(wtg_ctx* ctx, const wtg_fs_params* p);
-/* host <-> device copy of one free-surface field ([ncells] doubles; halo
- * cells are filled from their owners on upload) */
+/* host <-> device copy of one free-surface field ([ncells] doubles, or
+ * [nmembers][ncells] for an ensemble; halo cells are filled from their
+ * owners on upload) */
 int wtg_fs_upload(wtg_ctx* ctx, int field, const double* host);
 int wtg_fs_download(wtg_ctx* ctx, int field, double* host);
 
@@ -262,7 +264,10 @@ int wtg_fs_set_graph(wtg_ctx* ctx, int enable);
  * free_surface_update (464-478: rhs_elliptic, eta_old := eta, the elliptic
  * solve of WTG_SOLVE_BICGSTAB to rel_tol -- the reference's SRJ on a single
  * level --, barotropic_correction, u_update).  Porosity 1, mean state 0.
- * *iters = BiCGSTAB iterations of the last step. */
+ * *iters = BiCGSTAB iterations of the last step.  Ensembles (no counterpart
+ * in the reference): one trend launch for all members, each member's
+ * column and free-surface steps on its own fields, the members' solves one
+ * after another; *iters = the most any member took in the last step. */
 int wtg_rk4_split(wtg_ctx* ctx, int slot, int nsteps, double rel_tol, int max_iter, int* iters);
 
 /* ---- multi-level free surface: scalar wavelet transforms, FMG, SRJ -------

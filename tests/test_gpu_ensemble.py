@@ -44,10 +44,62 @@ def test_ensemble_members_equal_single_runs():
             assert np.array_equal(su[:, :, own_e], wu[:, :, own_e])
 
 
-def test_ensemble_rejects_free_surface():
+def test_ensemble_free_surface_restrictions():
+    """An ensemble's free surface runs through the split step only: the
+    single-field solver entry points and the hierarchy refuse it."""
     g = W.Grid(level=4, min_level=4, domain_level=1)
     ens = W.Context(g, nlayers=2, members=2)
     ens.set_topography(np.full(g.layout.ncells, -1000.0))
     ens.fs_set_params(100.0)
-    with pytest.raises(W.WtgError, match="single-member"):
-        ens.rk4_split(0, 1)
+    with pytest.raises(W.WtgError, match="ensemble"):
+        ens.elliptic_apply(W.FS_ETA, W.FS_RHS)
+    with pytest.raises(W.WtgError, match="ensemble"):
+        W.Hierarchy([ens])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+def test_ensemble_split_steps_equal_single_runs(precision):
+    """Split-explicit steps (RK4_split with the implicit free surface) of an
+    ensemble: one trend launch for all members, each member's free surface
+    solved on its own view.  Every member equals the same member stepped
+    alone bit for bit (state and free surface); the reported iteration count
+    is the largest of the members'."""
+    print(f"ensemble split steps J5 L3 x 3 ({precision})", flush=True)
+    prec = W.PRECISION_FP64 if precision == "fp64" else W.PRECISION_FP32_BULK
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, M, dt = 3, 3, 1800.0
+    draws = [cases.ocean(g, L, seed=30 + m) for m in range(M)]
+    etas = [d[1].sum(0) / 1028.0 + d[0] for d in draws]
+    ens = W.Context(g, nlayers=L, members=M)
+    ens.set_topography(np.stack([d[0] for d in draws]))
+    ens.upload_state(0, *(np.concatenate([d[k] for d in draws]) for k in (1, 2, 3)))
+    ens.fs_set_params(dt, 0.6, 0.8)
+    ens.fs_upload(W.FS_ETA, np.stack(etas))
+    ens.set_precision(prec)
+    ens.set_stage_shape(W.SHAPE_LAYER_PAIRS)
+    it_e = ens.rk4_split(0, 2, 1e-11, 400)
+    em, et, eu = ens.download_state(0)
+    ee = ens.fs_download(W.FS_ETA)
+    own, own_e = g.layout.owned_cells(), g.layout.owned_edge_cells()
+    its = []
+    for m, (topo, mass, temp, velo) in enumerate(draws):
+        one = W.Context(g, nlayers=L)
+        one.set_topography(topo)
+        one.upload_state(0, mass, temp, velo)
+        one.fs_set_params(dt, 0.6, 0.8)
+        one.fs_upload(W.FS_ETA, etas[m])
+        one.set_precision(prec)
+        one.set_stage_shape(W.SHAPE_LAYER_PAIRS)
+        its.append(one.rk4_split(0, 2, 1e-11, 400))
+        sm, st, su = one.download_state(0)
+        sl = slice(m * L, (m + 1) * L)
+        assert np.array_equal(em[sl][:, own], sm[:, own])
+        assert np.array_equal(et[sl][:, own], st[:, own])
+        assert np.array_equal(eu[sl][:, :, own_e], su[:, :, own_e])
+        assert np.array_equal(ee[m][own], one.fs_download(W.FS_ETA)[own])
+        one.close()
+    print(f"  iterations: ensemble {it_e}, members {its}", flush=True)
+    assert it_e == max(its)
+    with pytest.raises(W.WtgError, match="ensemble"):
+        ens.elliptic_solve(W.SOLVE_BICGSTAB, 1e-9, 10)
+    ens.close()

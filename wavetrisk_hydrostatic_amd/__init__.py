@@ -343,12 +343,14 @@ class Context:
         _check(_lib.wtg_fs_set_params(self._h, C.byref(p)), self._h, "wtg_fs_set_params")
 
     def fs_upload(self, field: int, values):
+        """[ncells], or [members, ncells] for an ensemble"""
         a = np.ascontiguousarray(values, dtype=np.float64)
-        assert a.shape == (self.layout.ncells,)
+        shape = (self.layout.ncells,) if self.members == 1 else (self.members, self.layout.ncells)
+        assert a.shape == shape, (a.shape, shape)
         _check(_lib.wtg_fs_upload(self._h, field, _ptr(a)), self._h, "wtg_fs_upload")
 
     def fs_download(self, field: int) -> np.ndarray:
-        a = np.empty(self.layout.ncells)
+        a = np.empty(self.layout.ncells if self.members == 1 else (self.members, self.layout.ncells))
         _check(_lib.wtg_fs_download(self._h, field, _ptr(a)), self._h, "wtg_fs_download")
         return a
 

@@ -116,6 +116,12 @@ struct wtg_ctx {
   const double* bicg_f = nullptr;
   double bicg_key[2] = {0.0, 0.0};  // operator coefficient and gravity of the captured graph
   std::vector<void*> allocs;
+  // ensembles: one single-member view per member for the free surface of
+  // the split-explicit step (state, topography and free-surface fields at
+  // the member's offset; geometry, lists and solver scalars shared, the
+  // members' solves running one after another on the stream).  Owned here,
+  // built on first use; a view owns no device memory.
+  std::vector<wtg_ctx*> views;
 };
 
 extern thread_local std::string g_create_error;
@@ -138,8 +144,13 @@ inline int wtg_dalloc(wtg_ctx* c, Tp** p, size_t count) {
   return 0;
 }
 
-// allocates the free-surface fields on first use (elliptic.hip)
+// allocates the free-surface fields on first use (elliptic.hip); an
+// ensemble context holds them member-major ([nmembers][ncells] per field)
 int wtg_fs_ready(wtg_ctx* c);
+// flux_divergence into WTG_FS_DIVF and rhs_elliptic into WTG_FS_RHS of a
+// single-member context (or view), queued without synchronising (elliptic.hip)
+int wtg_fs_flux_div_async(wtg_ctx* c, int slot);
+int wtg_fs_rhs_async(wtg_ctx* c);
 
 // split-explicit steps (trisk.hip) with the free-surface solve supplied by
 // the caller: solve(user, c, iters) solves elliptic_lo(WTG_FS_ETA) =

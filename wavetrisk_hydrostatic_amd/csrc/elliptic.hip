@@ -737,7 +737,7 @@ int fs_ready(wtg_ctx* c) {
   if (!c) return WTG_E_ARG;
   if (!c->topo_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_topography must be called first");
   if (c->fs[0]) return 0;
-  const int64_t NC = c->info.ncells;
+  const int64_t NC = c->info.ncells * c->nmembers;  // member-major for ensembles
   for (int f = 0; f < FS_NFIELDS; ++f) {
     if (int rc = wtg_dalloc<double>(c, &c->fs[f], NC)) return rc;
     WTG_HIPCHK(c, hipMemsetAsync(c->fs[f], 0, NC * sizeof(double), c->stream));
@@ -754,6 +754,13 @@ int fs_ready(wtg_ctx* c) {
 
 int fs_field(wtg_ctx* c, int f, int nuser = FS_NUSER) {
   if (f < 0 || f >= nuser) return wtg_fail(c, WTG_E_ARG, "free-surface field %d out of range [0,%d)", f, nuser);
+  return 0;
+}
+
+// the solver entry points act on one member's fields
+int fs_single(wtg_ctx* c) {
+  if (c->nmembers != 1)
+    return wtg_fail(c, WTG_E_ARG, "ensemble contexts support the free surface through wtg_rk4_split only");
   return 0;
 }
 
@@ -1415,6 +1422,28 @@ int mg_check_level(wtg_mg* m, int fine) {
 
 }  // namespace
 
+int wtg_fs_flux_div_async(wtg_ctx* c, int slot) {
+  RC(fs_ready(c));
+  if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "the free-surface fields are per member: use a single-member context");
+  if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
+  const auto& s = c->slots[slot];
+  hipLaunchKernelGGL(fs_flux_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
+                     c->params.nlayers, s.m, s.u, c->fs[FS_DIVF]);
+  WTG_HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int wtg_fs_rhs_async(wtg_ctx* c) {
+  RC(fs_ready(c));
+  RC(fs_need_params(c));
+  if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "the free-surface fields are per member: use a single-member context");
+  hipLaunchKernelGGL(fs_rhs_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, c->fs[FS_ETA],
+                     c->fs[FS_DIVF], c->fs[FS_DIVF_OLD], c->fs_dt, c->fs_theta2, c->params.ref_density,
+                     c->fs[FS_RHS]);
+  WTG_HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
 extern "C" {
 
 int wtg_fs_set_params(wtg_ctx* c, const wtg_fs_params* p) {
@@ -1431,13 +1460,14 @@ int wtg_fs_upload(wtg_ctx* c, int field, const double* host) {
   RC(fs_ready(c));
   RC(fs_field(c, field));
   if (!host) return wtg_fail(c, WTG_E_ARG, "null array");
-  const int64_t NC = c->info.ncells;
-  std::vector<double> t(host, host + NC);
-  for (int64_t cc = 0; cc < NC; ++cc) {
-    const int64_t s = c->node_src[cc];
-    if (s >= 0) t[cc] = host[s];
-  }
-  WTG_HIPCHK(c, hipMemcpyAsync(c->fs[field], t.data(), NC * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int64_t NC = c->info.ncells, n = NC * c->nmembers;
+  std::vector<double> t(host, host + n);
+  for (int64_t m = 0; m < n; m += NC)
+    for (int64_t cc = 0; cc < NC; ++cc) {
+      const int64_t s = c->node_src[cc];
+      if (s >= 0) t[m + cc] = host[m + s];
+    }
+  WTG_HIPCHK(c, hipMemcpyAsync(c->fs[field], t.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1446,13 +1476,15 @@ int wtg_fs_download(wtg_ctx* c, int field, double* host) {
   RC(fs_ready(c));
   RC(fs_field(c, field));
   if (!host) return wtg_fail(c, WTG_E_ARG, "null array");
-  WTG_HIPCHK(c, hipMemcpyAsync(host, c->fs[field], c->info.ncells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  WTG_HIPCHK(c, hipMemcpyAsync(host, c->fs[field], c->info.ncells * c->nmembers * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int wtg_elliptic_apply(wtg_ctx* c, int in_field, int out_field) {
   RC(fs_ready(c));
+  RC(fs_single(c));
   RC(fs_need_params(c));
   RC(fs_field(c, in_field));
   RC(fs_field(c, out_field));
@@ -1465,6 +1497,7 @@ int wtg_elliptic_apply(wtg_ctx* c, int in_field, int out_field) {
 
 int wtg_elliptic_diag(wtg_ctx* c, int out_field) {
   RC(fs_ready(c));
+  RC(fs_single(c));
   RC(fs_need_params(c));
   RC(fs_field(c, out_field));
   if (out_field == FS_ETA_OLD) return wtg_fail(c, WTG_E_ARG, "output field must differ from WTG_FS_ETA_OLD");
@@ -1478,6 +1511,7 @@ int wtg_elliptic_diag(wtg_ctx* c, int out_field) {
 
 int wtg_jacobi(wtg_ctx* c, int u_field, int f_field, double w, int nsweeps) {
   RC(fs_ready(c));
+  RC(fs_single(c));
   RC(fs_need_params(c));
   RC(fs_field(c, u_field));
   RC(fs_field(c, f_field));
@@ -1496,6 +1530,7 @@ int wtg_jacobi(wtg_ctx* c, int u_field, int f_field, double w, int nsweeps) {
 
 int wtg_fs_dot(wtg_ctx* c, int a_field, int b_field, double* out) {
   RC(fs_ready(c));
+  RC(fs_single(c));
   RC(fs_field(c, a_field));
   RC(fs_field(c, b_field));
   if (!out) return wtg_fail(c, WTG_E_ARG, "null output");
@@ -1504,30 +1539,20 @@ int wtg_fs_dot(wtg_ctx* c, int a_field, int b_field, double* out) {
 }
 
 int wtg_flux_divergence(wtg_ctx* c, int slot) {
-  RC(fs_ready(c));
-  if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "the free surface is built for single-member contexts");
-  if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
-  const auto& s = c->slots[slot];
-  hipLaunchKernelGGL(fs_flux_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
-                     c->params.nlayers, s.m, s.u, c->fs[FS_DIVF]);
-  WTG_HIPCHK(c, hipGetLastError());
+  RC(wtg_fs_flux_div_async(c, slot));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int wtg_rhs_elliptic(wtg_ctx* c) {
-  RC(fs_ready(c));
-  RC(fs_need_params(c));
-  hipLaunchKernelGGL(fs_rhs_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, c->fs[FS_ETA],
-                     c->fs[FS_DIVF], c->fs[FS_DIVF_OLD], c->fs_dt, c->fs_theta2, c->params.ref_density,
-                     c->fs[FS_RHS]);
-  WTG_HIPCHK(c, hipGetLastError());
+  RC(wtg_fs_rhs_async(c));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int wtg_elliptic_solve(wtg_ctx* c, int method, double rel_tol, int max_iter, int* iters, double* rel_resid) {
   RC(fs_ready(c));
+  RC(fs_single(c));
   RC(fs_need_params(c));
   if (!iters || !rel_resid) return wtg_fail(c, WTG_E_ARG, "null output");
   if (max_iter < 0) return wtg_fail(c, WTG_E_ARG, "max_iter < 0");
@@ -1562,6 +1587,7 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
     if (l && grids[l]->lv.level != grids[l - 1]->lv.level + 1)
       return wtg_fail(nullptr, WTG_E_ARG, "levels %d and %d are not consecutive", l - 1, l);
     if (c->device != levels[0]->device) return wtg_fail(nullptr, WTG_E_ARG, "level %d is on another device", l);
+    if (c->nmembers != 1) return wtg_fail(nullptr, WTG_E_ARG, "level %d is an ensemble context", l);
     if (!c->topo_set || !c->fs_params_set)
       return wtg_fail(nullptr, WTG_E_STATE, "level %d: topography and free-surface parameters must be set", l);
     for (int k = 0; k < l; ++k)

@@ -2357,6 +2357,62 @@ int baro_correction(wtg_ctx* c, const wtg_ctx::Slot& s) {
   return 0;
 }
 
+// the single-member views of an ensemble context for the free surface
+// (built once; their scalar parameters follow the context's on every call)
+int member_views(wtg_ctx* c, std::vector<wtg_ctx*>& out) {
+  out.clear();
+  if (c->nmembers == 1) {
+    out.push_back(c);
+    return 0;
+  }
+  if (c->views.empty()) {
+    const int64_t NC = c->info.ncells, L = c->params.nlayers;
+    std::vector<int64_t> src;
+    src.swap(c->node_src);  // not needed by a view, and large
+    for (int m = 0; m < c->nmembers; ++m) {
+      wtg_ctx* v = new wtg_ctx(*c);
+      v->allocs.clear();
+      v->views.clear();
+      v->own_stream = nullptr;
+      v->rk_graph = nullptr;
+      v->bicg_graph = nullptr;
+      v->use_graph = false;
+      v->fs_graph = false;
+      v->nmembers = 1;
+      v->d_topo = c->d_topo + m * NC;
+      for (auto& sl : v->slots) {
+        sl.m += m * L * NC;
+        sl.t += m * L * NC;
+        sl.u += 3 * m * L * NC;
+      }
+      for (int f = 0; f < FS_NFIELDS; ++f) v->fs[f] = c->fs[f] + m * NC;
+      for (auto& f32 : v->fs32) f32 = nullptr;
+      c->views.push_back(v);
+    }
+    src.swap(c->node_src);
+  }
+  const int64_t NC = c->info.ncells, L = c->params.nlayers;
+  for (int m = 0; m < c->nmembers; ++m) {
+    wtg_ctx* v = c->views[m];
+    for (int k = 0; k < 2; ++k) {  // the FP32 buffers may be allocated after the views
+      const auto& sf = c->fstage[k];
+      v->fstage[k] = {sf.m ? sf.m + m * L * NC : nullptr, sf.t ? sf.t + m * L * NC : nullptr,
+                      sf.u ? sf.u + 3 * m * L * NC : nullptr};
+    }
+    v->params = c->params;
+    v->fs_dt = c->fs_dt;
+    v->fs_theta1 = c->fs_theta1;
+    v->fs_theta2 = c->fs_theta2;
+    v->fs_params_set = c->fs_params_set;
+    v->topo_set = c->topo_set;
+    v->precision = c->precision;
+    v->mixed_shape = c->mixed_shape;
+    v->stream = c->stream;
+    out.push_back(v);
+  }
+  return 0;
+}
+
 int bicgstab_solve(void* user, wtg_ctx* c, int* iters) {
   const double* tm = (const double*)user;  // {tol, maxit}
   double res = 0.0;
@@ -2370,13 +2426,14 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "negative step count");
-  if (c->nmembers != 1) return fail(c, WTG_E_ARG, "the free surface is built for single-member contexts");
+  if (c->nmembers != 1 && solve != bicgstab_solve)
+    return fail(c, WTG_E_ARG, "ensembles solve the free surface with BiCGSTAB (wtg_rk4_split)");
   if (int rc = wtg_fs_ready(c)) return rc;
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
   const auto& w2 = c->slots[c->nslots + 1];
   const int L = c->params.nlayers;
-  const int64_t NC = c->info.ncells;
+  const int64_t NC = c->info.ncells, NCM = NC * c->nmembers;
   const double dt = c->fs_dt, th1 = c->fs_theta1, grav = c->params.grav;
   if (int rc = launch_copy_frozen(c, sol, w1)) return rc;
   if (int rc = launch_copy_frozen(c, sol, w2)) return rc;
@@ -2390,46 +2447,73 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
       if (int rc = tune_mixed_shape(c, sol)) return rc;
     }
   }
+  // one launch for the trend of every member; the 2-D free surface and the
+  // per-column steps on each member's view
+  std::vector<wtg_ctx*> V;
+  if (int rc = member_views(c, V)) return rc;
+  auto view_err = [&](wtg_ctx* v, int rc) {
+    if (rc && v != c) c->err = v->err;
+    return rc;
+  };
   const auto& ftr = c->fstage[0];
   const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
-  const wtg_ctx::Slot* ins[4] = {&sol, &w1, &w2, &w1};
-  const wtg_ctx::Slot* trs[4] = {&w1, &w2, &w1, &w2};
-  const wtg_ctx::Slot* outs[4] = {&w1, &w2, &w1, &sol};
+  const int ins[4] = {slot, c->nslots, c->nslots + 1, c->nslots};
+  const int trs[4] = {c->nslots, c->nslots + 1, c->nslots, c->nslots + 1};
+  const int outs[4] = {c->nslots, c->nslots + 1, c->nslots, slot};
   const unsigned nb = (unsigned)(((int64_t)c->nmaster * L + 255) / 256);
+  *iters = 0;
   for (int s = 0; s < nsteps; ++s) {
     if (int rc = launch_ghost_fill(c, sol)) return rc;
     // flux divergence of the layer-integrated velocity at the previous step
-    if (int rc = wtg_flux_divergence(c, slot)) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->fs[FS_DIVF_OLD], c->fs[FS_DIVF], NC * sizeof(double), hipMemcpyDeviceToDevice,
+    for (wtg_ctx* v : V)
+      if (int rc = wtg_fs_flux_div_async(v, slot)) return view_err(v, rc);
+    HIPCHK(c, hipMemcpyAsync(c->fs[FS_DIVF_OLD], c->fs[FS_DIVF], NCM * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
     for (int st = 0; st < 4; ++st) {
-      const auto &in = *ins[st], &tr = *trs[st], &out = *outs[st];
-      if (int rc = baro_correction(c, in)) return rc;
+      const auto &in = c->slots[ins[st]], &tr = c->slots[trs[st]], &out = c->slots[outs[st]];
+      for (wtg_ctx* v : V)
+        if (int rc = baro_correction(v, v->slots[ins[st]])) return view_err(v, rc);
       if (int rc = launch_ghost_fill(c, in)) return rc;
       if (f32) {
         if (int rc = launch_stage_mixed<false, float>(c, in, nullptr, ftr.m, ftr.t, ftr.u, 1.0)) return rc;
-        hipLaunchKernelGGL(split_axpy_kernel<float>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC,
-                           c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, ftr.m,
-                           ftr.t, ftr.u, out.m, out.t, out.u);
       } else {
         if (int rc = launch_stage(c, in, in, tr, 0.0, 1)) return rc;
-        hipLaunchKernelGGL(split_axpy_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L,
-                           NC, c->info.S, coef[st], th1, grav, c->fs[FS_ETA], c->d_len, sol.m, sol.t, sol.u, tr.m,
-                           tr.t, tr.u, out.m, out.t, out.u);
       }
-      HIPCHK(c, hipGetLastError());
+      for (wtg_ctx* v : V) {
+        const auto &vs = v->slots[slot], &vt = v->slots[trs[st]], &vo = v->slots[outs[st]];
+        const auto& vf = v->fstage[0];
+        if (f32)
+          hipLaunchKernelGGL(split_axpy_kernel<float>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC,
+                             c->info.S, coef[st], th1, grav, v->fs[FS_ETA], c->d_len, vs.m, vs.t, vs.u, vf.m, vf.t,
+                             vf.u, vo.m, vo.t, vo.u);
+        else
+          hipLaunchKernelGGL(split_axpy_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L,
+                             NC, c->info.S, coef[st], th1, grav, v->fs[FS_ETA], c->d_len, vs.m, vs.t, vs.u, vt.m,
+                             vt.t, vt.u, vo.m, vo.t, vo.u);
+        HIPCHK(c, hipGetLastError());
+      }
       if (int rc = launch_ghost_fill(c, out)) return rc;
     }
     // free_surface_update: eta_update, barotropic_correction, u_update
-    if (int rc = wtg_flux_divergence(c, slot)) return rc;
-    if (int rc = wtg_rhs_elliptic(c)) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NC * sizeof(double), hipMemcpyDeviceToDevice,
+    for (wtg_ctx* v : V) {
+      if (int rc = wtg_fs_flux_div_async(v, slot)) return view_err(v, rc);
+      if (int rc = wtg_fs_rhs_async(v)) return view_err(v, rc);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NCM * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
-    if (int rc = solve(user, c, iters)) return rc;
-    if (int rc = baro_correction(c, sol)) return rc;
-    hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC, c->info.S,
-                       th1 * dt, grav, c->fs[FS_ETA], c->d_len, sol.u);
-    HIPCHK(c, hipGetLastError());
+    int step_it = 0;  // the most iterations any member's solve took in this step
+    for (wtg_ctx* v : V) {
+      int it = 0;
+      if (int rc = solve(user, v, &it)) return view_err(v, rc);
+      step_it = std::max(step_it, it);
+    }
+    *iters = step_it;
+    for (wtg_ctx* v : V) {
+      if (int rc = baro_correction(v, v->slots[slot])) return view_err(v, rc);
+      hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC, c->info.S,
+                         th1 * dt, grav, v->fs[FS_ETA], c->d_len, v->slots[slot].u);
+      HIPCHK(c, hipGetLastError());
+    }
     if (int rc = launch_ghost_fill(c, sol)) return rc;
   }
   return 0;
@@ -2696,6 +2780,7 @@ void wtg_destroy(wtg_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->rk_graph) hipGraphExecDestroy(c->rk_graph);
   if (c->bicg_graph) hipGraphExecDestroy(c->bicg_graph);
+  for (wtg_ctx* v : c->views) delete v;  // views own no device memory
   for (void* p : c->allocs) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
   if (c->stream) hipStreamDestroy(c->stream);
