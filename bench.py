@@ -376,7 +376,8 @@ def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10, split_step
                                     dof / (split[W.PRECISION_FP64]["ms_per_step"] / 1e3)},
                            "fp32_bulk": {**split[W.PRECISION_FP32_BULK], "dof_updates_per_s":
                                          dof / (split[W.PRECISION_FP32_BULK]["ms_per_step"] / 1e3)},
-                           "note": "host wall clock; the members' free-surface solves run one after another"},
+                           "note": "host wall clock; the members' BiCGSTAB solves batched (one launch per step of "
+                                   "the iteration for all members)"},
             "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
 
 

@@ -266,8 +266,10 @@ int wtg_fs_set_graph(wtg_ctx* ctx, int enable);
  * level --, barotropic_correction, u_update).  Porosity 1, mean state 0.
  * *iters = BiCGSTAB iterations of the last step.  Ensembles (no counterpart
  * in the reference): one trend launch for all members, each member's
- * column and free-surface steps on its own fields, the members' solves one
- * after another; *iters = the most any member took in the last step. */
+ * column and free-surface steps on its own fields, the members' BiCGSTAB
+ * solves batched (one launch per step of the iteration for all members,
+ * each member stopping at its own convergence test); *iters = the most any
+ * member took in the last step. */
 int wtg_rk4_split(wtg_ctx* ctx, int slot, int nsteps, double rel_tol, int max_iter, int* iters);
 
 /* ---- multi-level free surface: scalar wavelet transforms, FMG, SRJ -------

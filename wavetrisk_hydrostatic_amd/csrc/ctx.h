@@ -150,6 +150,10 @@ int wtg_fs_ready(wtg_ctx* c);
 // flux_divergence into WTG_FS_DIVF and rhs_elliptic into WTG_FS_RHS of a
 // single-member context (or view), queued without synchronising (elliptic.hip)
 int wtg_fs_flux_div_async(wtg_ctx* c, int slot);
+// BiCGSTAB on WTG_FS_ETA / WTG_FS_RHS of every member of a context at once
+// (one launch per step of the iteration for all members, each member gated
+// by its own convergence test); *iters, *err: the largest over the members
+int wtg_fs_bicgstab_all(wtg_ctx* c, double tol, int maxit, int* iters, double* err);
 int wtg_fs_rhs_async(wtg_ctx* c);
 
 // split-explicit steps (trisk.hip) with the free-surface solve supplied by

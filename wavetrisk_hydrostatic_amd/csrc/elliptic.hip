@@ -43,6 +43,15 @@ enum Scal {
 // the host's convergence test without changing the iterates
 __device__ __forceinline__ bool gated(const double* __restrict__ gate) { return gate && gate[SC_DONE] != 0.0; }
 
+// Batched solves of an ensemble (one launch for all members): member
+// blockIdx.y, its node fields at blockIdx.y * ms (ms = cells per member),
+// its solver scalars, dot partials and block counter in its own slices.
+// One member: blockIdx.y = 0, no offset.
+template <class Tp>
+__device__ __forceinline__ Tp* mem(Tp* p, int64_t ms) {
+  return p ? p + (int64_t)blockIdx.y * ms : p;
+}
+
 // scalar steps of the solvers (scalar_op below)
 enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF, OP_LOOP_INIT, OP_ERR_BETA, OP_MRSTEP, OP_NONE };
 
@@ -89,9 +98,11 @@ __device__ __forceinline__ double ell_diag_at(const FsGeo& g, const double* __re
 
 __global__ void fs_apply_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, FsGeo g,
                                 const double* __restrict__ q, const double* __restrict__ eo, double grav, double coef,
-                                double* __restrict__ out, double* __restrict__ out2) {
+                                double* __restrict__ out, double* __restrict__ out2, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  gate = mem(gate, SC_NSCAL);
   if (t >= n || gated(gate)) return;
+  g.topo = mem(g.topo, ms), q = mem(q, ms), eo = mem(eo, ms), out = mem(out, ms), out2 = mem(out2, ms);
   const double v = ell_apply_at(g, q, eo, grav, coef, ncl[2 * t + 1]);
   out[ncl[2 * t]] = v;
   if (out2) out2[ncl[2 * t]] = v;
@@ -102,9 +113,11 @@ __global__ void fs_apply_kernel(const double* __restrict__ gate, int n, const in
 __global__ void fs_bicg_init_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, double* __restrict__ u,
                                     const double* __restrict__ f, const double* __restrict__ eo, double grav,
                                     double coef, int zero_u, double* __restrict__ res, double* __restrict__ res0,
-                                    double* __restrict__ p, double* __restrict__ s) {
+                                    double* __restrict__ p, double* __restrict__ s, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  g.topo = mem(g.topo, ms), u = mem(u, ms), f = mem(f, ms), eo = mem(eo, ms);
+  res = mem(res, ms), res0 = mem(res0, ms), p = mem(p, ms), s = mem(s, ms);
   const int64_t c = ncl[2 * t + 1], d = ncl[2 * t];
   const double r = zero_u ? f[c] - 0.0 : f[c] - ell_apply_at(g, (const double*)u, eo, grav, coef, c);
   res[d] = r;
@@ -166,9 +179,11 @@ __global__ void fs_lc_kernel(const double* __restrict__ gate, int n, const int64
 }
 
 // halo copies of a field from its masters (entries nmaster.. of the list)
-__global__ void fs_ghost_copy_kernel(int nmaster, int n, const int64_t* __restrict__ ncl, double* __restrict__ f) {
+__global__ void fs_ghost_copy_kernel(int nmaster, int n, const int64_t* __restrict__ ncl, double* __restrict__ f,
+                                     int64_t ms) {
   const int t = nmaster + blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  f = mem(f, ms);
   f[ncl[2 * t]] = f[ncl[2 * t + 1]];
 }
 
@@ -249,8 +264,11 @@ __global__ __launch_bounds__(FS_NT) void fs_dot_kernel(const double* __restrict_
                                                        const double* __restrict__ a0, const double* __restrict__ b0,
                                                        const double* __restrict__ a1, const double* __restrict__ b1,
                                                        const double* __restrict__ a2, const double* __restrict__ b2,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part, int64_t ms) {
+  gate = mem(gate, SC_NSCAL);
   if (gated(gate)) return;
+  a0 = mem(a0, ms), b0 = mem(b0, ms), a1 = mem(a1, ms), b1 = mem(b1, ms), a2 = mem(a2, ms), b2 = mem(b2, ms);
+  part = mem(part, (int64_t)RED_BLOCKS * RED_MAX);
   __shared__ double sh[RED_MAX][FS_NT];
   const double* A[RED_MAX] = {a0, a1, a2};
   const double* B[RED_MAX] = {b0, b1, b2};
@@ -273,6 +291,8 @@ __device__ void scalar_op(int op, int gated_loop, double arg, double* __restrict
 __global__ __launch_bounds__(FS_NT) void fs_dot_final_kernel(int gated_loop, int nblocks, int nd,
                                                              const double* __restrict__ part, int o0, int o1, int o2,
                                                              double* __restrict__ scal, int post_op) {
+  scal = mem(scal, SC_NSCAL);
+  part = mem(part, (int64_t)RED_BLOCKS * RED_MAX);
   if (gated_loop && scal[SC_DONE] != 0.0) return;
   __shared__ double sh[RED_MAX][FS_NT];
   double acc[RED_MAX] = {0.0, 0.0, 0.0};
@@ -339,7 +359,7 @@ __device__ void scalar_op(int op, int gated_loop, double arg, double* __restrict
 
 __global__ void fs_scalar_kernel(int op, int gated_loop, double arg, double* __restrict__ s) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  scalar_op(op, gated_loop, arg, s);
+  scalar_op(op, gated_loop, arg, mem(s, SC_NSCAL));
 }
 
 
@@ -390,8 +410,12 @@ __global__ __launch_bounds__(FS_NT) void fs_dot1_kernel(const double* __restrict
                                                         const double* __restrict__ a1, const double* __restrict__ b1,
                                                         const double* __restrict__ a2, const double* __restrict__ b2,
                                                         double* __restrict__ part, int o0, int o1, int o2,
-                                                        double* __restrict__ scal, int post_op, unsigned* count) {
+                                                        double* __restrict__ scal, int post_op, unsigned* count,
+                                                        int64_t ms) {
+  gate = mem(gate, SC_NSCAL);
   if (gated(gate)) return;
+  a0 = mem(a0, ms), b0 = mem(b0, ms), a1 = mem(a1, ms), b1 = mem(b1, ms), a2 = mem(a2, ms), b2 = mem(b2, ms);
+  part = mem(part, (int64_t)RED_BLOCKS * RED_MAX), scal = mem(scal, SC_NSCAL), count = mem(count, 1);
   __shared__ double sh[RED_MAX][FS_NT];
   const double* A[RED_MAX] = {a0, a1, a2};
   const double* B[RED_MAX] = {b0, b1, b2};
@@ -441,9 +465,12 @@ __global__ void fs_bicg_s_apply_kernel(const double* __restrict__ gate, int n, c
                                        FsGeo g, const double* __restrict__ res, const double* __restrict__ Ap,
                                        const double* __restrict__ eo, double grav, double coef,
                                        const double* __restrict__ scal, double* __restrict__ s_out,
-                                       double* __restrict__ As_out) {
+                                       double* __restrict__ As_out, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  gate = mem(gate, SC_NSCAL);
   if (t >= n || gated(gate)) return;
+  g.topo = mem(g.topo, ms), res = mem(res, ms), Ap = mem(Ap, ms), eo = mem(eo, ms);
+  scal = mem(scal, SC_NSCAL), s_out = mem(s_out, ms), As_out = mem(As_out, ms);
   const double a2 = -1.0 * scal[SC_ALPH];
   const int64_t c = ncl[2 * t + 1], dst = ncl[2 * t];
   int64_t x7[7];
@@ -459,9 +486,12 @@ __global__ void fs_bicg_p_apply_kernel(const double* __restrict__ gate, int n, c
                                        FsGeo g, const double* __restrict__ res, const double* __restrict__ p,
                                        const double* __restrict__ Ap, const double* __restrict__ eo, double grav,
                                        double coef, const double* __restrict__ scal, double* __restrict__ pn_out,
-                                       double* __restrict__ Apn_out) {
+                                       double* __restrict__ Apn_out, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  gate = mem(gate, SC_NSCAL);
   if (t >= n || gated(gate)) return;
+  g.topo = mem(g.topo, ms), res = mem(res, ms), p = mem(p, ms), Ap = mem(Ap, ms), eo = mem(eo, ms);
+  scal = mem(scal, SC_NSCAL), pn_out = mem(pn_out, ms), Apn_out = mem(Apn_out, ms);
   const double bb = 1.0 * scal[SC_B], mo = -1.0 * scal[SC_OMGA];
   const int64_t c = ncl[2 * t + 1], dst = ncl[2 * t];
   int64_t x7[7];
@@ -484,8 +514,11 @@ __global__ __launch_bounds__(FS_NT) void fs_bicg_res_kernel(
     const int64_t* __restrict__ rcell, const double* __restrict__ rw, double* __restrict__ u,
     const double* __restrict__ p, const double* __restrict__ s, const double* __restrict__ As,
     const double* __restrict__ res0, double* __restrict__ scal, double* __restrict__ res_out,
-    double* __restrict__ part, unsigned* count) {
+    double* __restrict__ part, unsigned* count, int64_t ms) {
+  gate = mem(gate, SC_NSCAL);
   if (gated(gate)) return;
+  u = mem(u, ms), p = mem(p, ms), s = mem(s, ms), As = mem(As, ms), res0 = mem(res0, ms), res_out = mem(res_out, ms);
+  scal = mem(scal, SC_NSCAL), part = mem(part, (int64_t)RED_BLOCKS * RED_MAX), count = mem(count, 1);
   __shared__ double sh[RED_MAX][FS_NT];
   const double alph = scal[SC_ALPH], omga = scal[SC_OMGA];
   const int i = blockIdx.x * FS_NT + threadIdx.x;
@@ -742,12 +775,14 @@ int fs_ready(wtg_ctx* c) {
     if (int rc = wtg_dalloc<double>(c, &c->fs[f], NC)) return rc;
     WTG_HIPCHK(c, hipMemsetAsync(c->fs[f], 0, NC * sizeof(double), c->stream));
   }
-  if (int rc = wtg_dalloc<double>(c, &c->d_part, (size_t)RED_BLOCKS * RED_MAX)) return rc;
-  if (int rc = wtg_dalloc<unsigned>(c, &c->d_count, 1)) return rc;
-  WTG_HIPCHK(c, hipMemsetAsync(c->d_count, 0, sizeof(unsigned), c->stream));
-  if (int rc = wtg_dalloc<double>(c, &c->d_scal, SC_NSCAL)) return rc;
-  WTG_HIPCHK(c, hipMemsetAsync(c->d_scal, 0, SC_NSCAL * sizeof(double), c->stream));
-  WTG_HIPCHK(c, hipHostMalloc((void**)&c->h_scal, SC_NSCAL * sizeof(double), hipHostMallocDefault));
+  // solver scalars, dot partials and block counters: one slice per member
+  const int M = c->nmembers;
+  if (int rc = wtg_dalloc<double>(c, &c->d_part, (size_t)RED_BLOCKS * RED_MAX * M)) return rc;
+  if (int rc = wtg_dalloc<unsigned>(c, &c->d_count, M)) return rc;
+  WTG_HIPCHK(c, hipMemsetAsync(c->d_count, 0, M * sizeof(unsigned), c->stream));
+  if (int rc = wtg_dalloc<double>(c, &c->d_scal, (size_t)SC_NSCAL * M)) return rc;
+  WTG_HIPCHK(c, hipMemsetAsync(c->d_scal, 0, SC_NSCAL * M * sizeof(double), c->stream));
+  WTG_HIPCHK(c, hipHostMalloc((void**)&c->h_scal, SC_NSCAL * M * sizeof(double), hipHostMallocDefault));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -772,8 +807,9 @@ int fs_need_params(wtg_ctx* c) {
 double fs_coef(const wtg_ctx* c) { return c->fs_theta1 * c->fs_theta2 * (c->fs_dt * c->fs_dt); }
 
 int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false, double* out2 = nullptr) {
-  hipLaunchKernelGGL(fs_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
-                     c->nncl, c->d_ncl, fs_geo(c), q, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out, out2);
+  hipLaunchKernelGGL(fs_apply_kernel, dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream,
+                     gate ? c->d_scal : nullptr, c->nncl, c->d_ncl, fs_geo(c), q, c->fs[FS_ETA_OLD], c->params.grav,
+                     fs_coef(c), out, out2, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -817,18 +853,19 @@ int launch_dots_g(wtg_ctx* c, bool gate, int post_op, int nd, const double* a0, 
                   const double* b2 = nullptr, int o2 = 0) {
   const int nb = std::min(RED_BLOCKS, (int)nblk(c->nred));
   if (nb <= c->lastblock_max) {
-    hipLaunchKernelGGL(fs_dot1_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr, gate ? 1 : 0,
-                       c->nred, c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0,
-                       b2 ? b2 : b0, c->d_part, o0, o1, o2, c->d_scal, post_op, c->d_count);
+    hipLaunchKernelGGL(fs_dot1_kernel, dim3(nb, c->nmembers), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
+                       gate ? 1 : 0, c->nred, c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0,
+                       a2 ? a2 : a0, b2 ? b2 : b0, c->d_part, o0, o1, o2, c->d_scal, post_op, c->d_count,
+                       c->info.ncells);
     WTG_HIPCHK(c, hipGetLastError());
     return 0;
   }
-  hipLaunchKernelGGL(fs_dot_kernel, dim3(nb), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr, c->nred,
-                     c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0, b2 ? b2 : b0,
-                     c->d_part);
+  hipLaunchKernelGGL(fs_dot_kernel, dim3(nb, c->nmembers), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
+                     c->nred, c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0, a2 ? a2 : a0,
+                     b2 ? b2 : b0, c->d_part, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
-  hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1), dim3(FS_NT), 0, c->stream, gate ? 1 : 0, nb, nd, c->d_part, o0, o1,
-                     o2, c->d_scal, post_op);
+  hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1, c->nmembers), dim3(FS_NT), 0, c->stream, gate ? 1 : 0, nb, nd,
+                     c->d_part, o0, o1, o2, c->d_scal, post_op);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -840,7 +877,7 @@ int launch_dots(wtg_ctx* c, int nd, const double* a0, const double* b0, int o0, 
 }
 
 int launch_scalar(wtg_ctx* c, int op, bool gate = false, double arg = 0.0) {
-  hipLaunchKernelGGL(fs_scalar_kernel, dim3(1), dim3(64), 0, c->stream, op, gate ? 1 : 0, arg, c->d_scal);
+  hipLaunchKernelGGL(fs_scalar_kernel, dim3(1, c->nmembers), dim3(64), 0, c->stream, op, gate ? 1 : 0, arg, c->d_scal);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -877,9 +914,11 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   double** W = c->fs + FS_W0;
   double *res = W[0], *res0 = W[1], *p = W[2], *s = W[3], *Ap = W[4], *As = W[5], *p_alt = W[6];
   RC(launch_dots_g(c, false, OP_NRMF, 1, f, f, SC_T0));  // nrm_f = l2(f)
-  hipLaunchKernelGGL(fs_bicg_init_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl,
+  const int M = c->nmembers;  // an ensemble solves every member's problem in the same launches
+  const int64_t ms = c->info.ncells;
+  hipLaunchKernelGGL(fs_bicg_init_kernel, dim3(nblk(c->nncl), M), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl,
                      fs_geo(c), u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), zero_u ? 1 : 0, res, res0, p,
-                     s);
+                     s, ms);
   WTG_HIPCHK(c, hipGetLastError());
   RC(launch_apply(c, p, Ap, false, As));  // Ap = As = L p
   RC(launch_dots(c, 1, res0, res, SC_RHO));
@@ -893,24 +932,25 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   auto chunk = [&](int k) -> int {
     for (int i = 0; i < k; ++i) {
       RC(launch_dots_g(c, true, OP_ALPH, 1, Ap, res0, SC_T0));
-      hipLaunchKernelGGL(fs_bicg_s_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
-                         c->d_ncl, fs_geo(c), res, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->d_scal, s,
-                         As);  // s = res - alph Ap; As = L s
+      hipLaunchKernelGGL(fs_bicg_s_apply_kernel, dim3(nblk(c->nncl), M), dim3(FS_NT), 0, c->stream, c->d_scal,
+                         c->nncl, c->d_ncl, fs_geo(c), res, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c),
+                         c->d_scal, s, As, ms);  // s = res - alph Ap; As = L s
       WTG_HIPCHK(c, hipGetLastError());
       RC(launch_dots_g(c, true, OP_OMGA, 2, As, s, SC_T1, As, As, SC_T2));
       const bool last = nbred <= c->lastblock_max;
-      hipLaunchKernelGGL(fs_bicg_res_kernel, dim3(std::max((unsigned)nbred, nblk(c->nncl))), dim3(FS_NT), 0,
+      hipLaunchKernelGGL(fs_bicg_res_kernel, dim3(std::max((unsigned)nbred, nblk(c->nncl)), M), dim3(FS_NT), 0,
                          c->stream, c->d_scal, c->nmaster, c->nncl, c->d_ncl, c->nred, nbred, c->d_rcell, c->d_rw, u,
-                         p, s, As, res0, c->d_scal, res, c->d_part, last ? c->d_count : nullptr);  // u, res, dots
+                         p, s, As, res0, c->d_scal, res, c->d_part, last ? c->d_count : nullptr,
+                         ms);  // u, res, dots
       WTG_HIPCHK(c, hipGetLastError());
       if (!last) {  // err, the convergence test, rho and b
-        hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1), dim3(FS_NT), 0, c->stream, 1, nbred, 2, c->d_part, SC_T0,
-                           SC_T3, 0, c->d_scal, OP_ERR_BETA);
+        hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1, M), dim3(FS_NT), 0, c->stream, 1, nbred, 2, c->d_part,
+                           SC_T0, SC_T3, 0, c->d_scal, OP_ERR_BETA);
         WTG_HIPCHK(c, hipGetLastError());
       }
-      hipLaunchKernelGGL(fs_bicg_p_apply_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->d_scal, c->nncl,
-                         c->d_ncl, fs_geo(c), res, p, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->d_scal,
-                         p_alt, Ap_alt);  // p = res + b (p - omga Ap); Ap = L p
+      hipLaunchKernelGGL(fs_bicg_p_apply_kernel, dim3(nblk(c->nncl), M), dim3(FS_NT), 0, c->stream, c->d_scal,
+                         c->nncl, c->d_ncl, fs_geo(c), res, p, Ap, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c),
+                         c->d_scal, p_alt, Ap_alt, ms);  // p = res + b (p - omga Ap); Ap = L p
       WTG_HIPCHK(c, hipGetLastError());
       std::swap(p, p_alt);
       std::swap(Ap, Ap_alt);
@@ -925,7 +965,7 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   const int first = c->bicg_hint > 0 && c->bicg_hint_tol == tol ? c->bicg_hint : BICG_CHUNK;
   while (queued < maxit && !done) {
     const int k = std::min(queued == 0 ? first : BICG_CHUNK, maxit - queued);
-    if (c->fs_graph && k == BICG_CHUNK) {
+    if (c->fs_graph && k == BICG_CHUNK && M == 1) {
       // the operator's coefficients are kernel arguments of the graph
       const double key[2] = {fs_coef(c), c->params.grav};
       if (!c->bicg_graph || c->bicg_u != u || c->bicg_f != f || c->bicg_key[0] != key[0] ||
@@ -959,20 +999,32 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
       RC(chunk(k));
     }
     queued += k;
-    WTG_HIPCHK(c, hipMemcpyAsync(c->h_scal + SC_ERR, c->d_scal + SC_ERR, (SC_TOL - SC_ERR) * sizeof(double),
-                                 hipMemcpyDeviceToHost, c->stream));
+    if (M == 1)
+      WTG_HIPCHK(c, hipMemcpyAsync(c->h_scal + SC_ERR, c->d_scal + SC_ERR, (SC_TOL - SC_ERR) * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+    else
+      WTG_HIPCHK(c, hipMemcpyAsync(c->h_scal, c->d_scal, (size_t)SC_NSCAL * M * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
     WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
-    done = c->h_scal[SC_DONE] != 0.0;
+    done = true;  // every member converged
+    for (int m = 0; m < M; ++m) done = done && c->h_scal[m * SC_NSCAL + SC_DONE] != 0.0;
   }
   const int nghost = c->nncl - c->nmaster;
   if (nghost > 0) {
-    hipLaunchKernelGGL(fs_ghost_copy_kernel, dim3(nblk(nghost)), dim3(FS_NT), 0, c->stream, c->nmaster, c->nncl,
-                       c->d_ncl, u);
+    hipLaunchKernelGGL(fs_ghost_copy_kernel, dim3(nblk(nghost), M), dim3(FS_NT), 0, c->stream, c->nmaster, c->nncl,
+                       c->d_ncl, u, ms);
     WTG_HIPCHK(c, hipGetLastError());
   }
   if (sync_end) WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
-  *iters = done ? (int)c->h_scal[SC_ITER] : maxit + 1;
-  *err_out = queued > 0 ? c->h_scal[SC_ERR] : 0.0;
+  // ensembles: the most iterations and the largest residual of the members
+  int itmax = 0;
+  double emax = 0.0;
+  for (int m = 0; m < M; ++m) {
+    itmax = std::max(itmax, (int)c->h_scal[m * SC_NSCAL + SC_ITER]);
+    emax = std::max(emax, c->h_scal[m * SC_NSCAL + SC_ERR]);
+  }
+  *iters = done ? itmax : maxit + 1;
+  *err_out = queued > 0 ? emax : 0.0;
   if (done) {
     c->bicg_hint = *iters;
     c->bicg_hint_tol = tol;
@@ -1421,6 +1473,12 @@ int mg_check_level(wtg_mg* m, int fine) {
 }
 
 }  // namespace
+
+int wtg_fs_bicgstab_all(wtg_ctx* c, double tol, int maxit, int* iters, double* err) {
+  RC(fs_ready(c));
+  RC(fs_need_params(c));
+  return bicgstab(c, c->fs[FS_ETA], c->fs[FS_RHS], tol, maxit, iters, err);
+}
 
 int wtg_fs_flux_div_async(wtg_ctx* c, int slot) {
   RC(fs_ready(c));

@@ -2447,8 +2447,8 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
       if (int rc = tune_mixed_shape(c, sol)) return rc;
     }
   }
-  // one launch for the trend of every member; the 2-D free surface and the
-  // per-column steps on each member's view
+  // one launch for the trend of every member and for each step of the
+  // members' BiCGSTAB; the other 2-D and per-column steps on each member's view
   std::vector<wtg_ctx*> V;
   if (int rc = member_views(c, V)) return rc;
   auto view_err = [&](wtg_ctx* v, int rc) {
@@ -2501,13 +2501,13 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
     }
     HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NCM * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
-    int step_it = 0;  // the most iterations any member's solve took in this step
-    for (wtg_ctx* v : V) {
-      int it = 0;
-      if (int rc = solve(user, v, &it)) return view_err(v, rc);
-      step_it = std::max(step_it, it);
+    if (c->nmembers == 1) {
+      if (int rc = solve(user, c, iters)) return rc;
+    } else {  // every member's BiCGSTAB in the same launches
+      const double* tm = (const double*)user;  // {tol, maxit} of bicgstab_solve
+      double err = 0.0;
+      if (int rc = wtg_fs_bicgstab_all(c, tm[0], (int)tm[1], iters, &err)) return rc;
     }
-    *iters = step_it;
     for (wtg_ctx* v : V) {
       if (int rc = baro_correction(v, v->slots[slot])) return view_err(v, rc);
       hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, NC, c->info.S,
