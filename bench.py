@@ -408,6 +408,40 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
                       "bicgstab_iterations": [int(i) for i in its]}
         final[name] = (ctx.download_state(1)[0], ctx.fs_download(FS_ETA))
     ctx.set_precision(PRECISION_FP64)
+    # the same step with the free surface solved by V-cycles on the hierarchy
+    # J(J-3)..J (the reference's multigrid setting): coarse levels one layer
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+
+    J = grid.layout.level
+    mg_runs = {}
+    grids = [W.Grid(level=l, min_level=J - 3, domain_level=2) for l in range(J - 3, J)]
+    levels = []
+    for g in grids:
+        tp, _, _, _ = cases.ocean(g, 1, seed=3)
+        c = W.Context(g, nlayers=1)
+        c.set_topography(tp)
+        c.fs_set_params(dt, 0.6, 0.8)
+        levels.append(c)
+    top = W.Context(W.Grid(level=J, min_level=J - 3, domain_level=2), nlayers=L)
+    top.set_topography(topo)
+    top.fs_set_params(dt, 0.6, 0.8)
+    h = W.Hierarchy(levels + [top])
+    p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)
+    for name, prec in (("fp64", PRECISION_FP64), ("fp32_bulk", PRECISION_FP32_BULK)):
+        top.set_precision(prec)
+        top.upload_state(0, mass, temp, velo)
+        top.fs_upload(FS_ETA, mass.sum(0) / 1028.0 + topo)
+        h.rk4_split(0, 1, W.SOLVE_VCYCLE, p)  # warm-up
+        top.synchronize()
+        t0 = time.perf_counter()
+        cyc = [h.rk4_split(0, 1, W.SOLVE_VCYCLE, p) for _ in range(nsteps)]
+        ms_mg = (time.perf_counter() - t0) * 1e3 / nsteps
+        mg_runs[name] = {"ms_per_step": ms_mg, "dof_updates_per_s": dof / (ms_mg / 1e3),
+                         "vcycles": [int(i) for i in cyc]}
+    h.close()
+    for c in levels + [top]:
+        c.close()
     own = grid.layout.owned_cells()
     depth = np.max(final["fp64"][0][:, own].sum(0) / 1028.0)
     drift = float(np.max(np.abs(final["fp32_bulk"][1][own] - final["fp64"][1][own])) / depth)
@@ -415,6 +449,7 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
             "fp32_bulk": {**runs["fp32_bulk"], "speedup_vs_fp64": runs["fp64"]["ms_per_step"] /
                           runs["fp32_bulk"]["ms_per_step"],
                           "surface_height_drift_rel_depth": drift, "steps_compared": nsteps + 1},
+            "vcycle_hierarchy": {"levels": f"J{J - 3}..J{J}", "coarse_tol": 1e-2, **mg_runs},
             "note": "host wall clock over 5 steps after one warm-up step from the same state; BiCGSTAB queues 8 "
                     "gated iterations per host convergence test; FP32-bulk: FP32 trend stencils, FP64 stage "
                     "states, AXPYs and free surface"}
