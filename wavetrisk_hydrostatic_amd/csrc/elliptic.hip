@@ -211,15 +211,34 @@ __device__ __forceinline__ double vflux(const FsGeo& g, int L, const double* __r
   return h * g.pedf[e * g.NC + x];
 }
 
-__global__ void fs_flux_div_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, int L, const double* __restrict__ m,
-                                   const double* __restrict__ u, double* __restrict__ out) {
+// The divergence in two passes, each edge's layer sum evaluated once:
+// fs_edge_flux_kernel writes vflux(x, e) for every cell x = (lz, i, j) with
+// i, j in [-1, n] (the masters, poles included, and their W, SW, S
+// neighbours; every neighbour read stays in the lozenge's tile), then
+// fs_div_kernel sums (F0(c) - F0(W) + F1(SW) - F1(c) + F2(c) - F2(S)) hex_inv
+// in the order of div (ops.f90:1325-1343).
+__global__ void fs_edge_flux_kernel(int nlz, FsGeo g, int L, const double* __restrict__ m,
+                                    const double* __restrict__ u, double* __restrict__ F0, double* __restrict__ F1,
+                                    double* __restrict__ F2) {
+  const int w = nlz + 2;  // cells per row and rows per lozenge in the pass
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)10 * w * w) return;
+  const int lz = (int)(t / ((int64_t)w * w)), r = (int)(t % ((int64_t)w * w));
+  const int i = r % w - 1, j = r / w - 1;
+  const int64_t x = (int64_t)lz * g.S * g.S + (int64_t)(j + 2) * g.S + (i + 2);
+  F0[x] = vflux(g, L, m, u, x, 0);
+  F1[x] = vflux(g, L, m, u, x, 1);
+  F2[x] = vflux(g, L, m, u, x, 2);
+}
+
+__global__ void fs_div_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const double* __restrict__ F0,
+                              const double* __restrict__ F1, const double* __restrict__ F2,
+                              double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = ncl[2 * t + 1];
   const int64_t cW = c - 1, cSW = c - g.S - 1, cS = c - g.S;
-  out[ncl[2 * t]] = (vflux(g, L, m, u, c, 0) - vflux(g, L, m, u, cW, 0) + vflux(g, L, m, u, cSW, 1) -
-                     vflux(g, L, m, u, c, 1) + vflux(g, L, m, u, c, 2) - vflux(g, L, m, u, cS, 2)) *
-                    g.hinv[c];
+  out[ncl[2 * t]] = (F0[c] - F0[cW] + F1[cSW] - F1[c] + F2[c] - F2[cS]) * g.hinv[c];
 }
 
 // rhs_elliptic (barotropic_2d.f90:168-184)
@@ -1485,8 +1504,14 @@ int wtg_fs_flux_div_async(wtg_ctx* c, int slot) {
   if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "the free-surface fields are per member: use a single-member context");
   if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
   const auto& s = c->slots[slot];
-  hipLaunchKernelGGL(fs_flux_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
-                     c->params.nlayers, s.m, s.u, c->fs[FS_DIVF]);
+  // edge fluxes into the first three solver work fields (no solve is running)
+  double *F0 = c->fs[FS_W0], *F1 = c->fs[FS_W0 + 1], *F2 = c->fs[FS_W0 + 2];
+  const int64_t ne = (int64_t)10 * (c->info.n + 2) * (c->info.n + 2);
+  hipLaunchKernelGGL(fs_edge_flux_kernel, dim3((unsigned)((ne + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream,
+                     c->info.n, fs_geo(c), c->params.nlayers, s.m, s.u, F0, F1, F2);
+  WTG_HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(fs_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), F0,
+                     F1, F2, c->fs[FS_DIVF]);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
