@@ -63,7 +63,7 @@ def main():
                     help="run only the free-surface line and print it (for profiling the solver)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding of the lozenge domains is a design note, not a code path", file=sys.stderr)
         sys.exit(2)
     if args.only_free_surface:

@@ -22,8 +22,8 @@ lozenge-tile layout:
   levels_J4.npz      hierarchy J4..J6 (harness dump_levels.f90): forward and
                      inverse scalar wavelet transforms and an FMG solve
 
-Only this container has /root/reference; the fixtures are committed and the
-GPU box reads them.
+The fixtures are committed: the tests read them, never the reference build,
+which exists only where the reference sources are.
 
     python tests/golden/make_golden.py
 """
