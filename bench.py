@@ -506,6 +506,22 @@ def free_surface(fine: int):
                 "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15}
 
     pv = LC.vcycle_params(1e-9, 40)
+    pv1 = LC.vcycle_params(1e-9, 40)
+    pv1.learned_cycles = 1  # the network's correction before the first cycle only
+    # two more seeded draws: plain V-cycles, learned cycles, learned first cycle
+    draws = {}
+    for seed in (7, 99):
+        pr = LC.Problem(grids, seed=seed)
+        row = {}
+        for name, meth, par, lrn in (("vcycle", SOLVE_VCYCLE, pv, False), ("learned", SOLVE_LEARNED, pv, True),
+                                     ("learned_first_cycle", SOLVE_LEARNED, pv1, True)):
+            pr.h.set_learned(mlp if lrn else None)
+            runs = [pr.solve(meth, par) for _ in range(4)][1:]
+            row[name] = {"cycles": int(runs[-1][0][-1]), "rel_resid": float(runs[-1][1][-1]),
+                         "ms": sorted(r[2] for r in runs)[1]}
+        pr.h.set_learned(None)
+        pr.close()
+        draws[f"seed_{seed}"] = row
     # V-cycles and learned cycles with FP32 smoothing (FP64 residuals and update)
     prob.h.set_precision(W.PRECISION_FP32_BULK)
     fp32 = {"vcycle": timed(SOLVE_VCYCLE, pv), "learned": timed(SOLVE_LEARNED, pv, learned=True),
@@ -518,6 +534,8 @@ def free_surface(fine: int):
            "learned": timed(SOLVE_LEARNED, pv, learned=True),
            "fmg": timed(SOLVE_FMG, MgParams.defaults(coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)),
            "network": source,
+           "learned_first_cycle": timed(SOLVE_LEARNED, pv1, learned=True),
+           "draws": draws,
            "fp32_smoothing": fp32,
            "mlp_forward_64_rhs": mlp_line}
     prob.close()

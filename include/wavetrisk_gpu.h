@@ -292,6 +292,9 @@ typedef struct {
   int max_vcycle;     /* FMG: V-cycles per level */
   int pre_iter, down_iter, up_iter, post_iter;  /* FMG: Jacobi sweeps */
   int max_srj_iter;   /* SRJ: sweeps per level (rounded down to a multiple of 8) */
+  int learned_cycles; /* WTG_SOLVE_LEARNED: the network's correction precedes the
+                         first learned_cycles cycles only (no counterpart in the
+                         reference; default: every cycle) */
 } wtg_mg_params;
 
 /* the reference's module defaults (coarse_tol = fine_tol = 1e-3, ...) */

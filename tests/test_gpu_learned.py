@@ -48,6 +48,32 @@ def test_features_match_definition(grids):
         prob.close()
 
 
+def test_learned_cycles_zero_equal_plain_vcycles(grids):
+    """learned_cycles = 0: no learned step is taken, the cycles are the plain
+    V-cycles bit for bit; learned_cycles = 1 takes it before the first cycle
+    only and still reaches the FP64 tolerance."""
+    print("learned: learned_cycles 0 and 1", flush=True)
+    prob = LC.Problem(grids, seed=12)
+    mlp = W.MLP(max_batch=prob.h.npatches, seed=4)
+    try:
+        p = LC.vcycle_params(1e-9, 30)
+        i0, r0, _ = prob.solve(W.SOLVE_VCYCLE, p)
+        u0 = prob.top.fs_download(W.FS_ETA)
+        prob.h.set_learned(mlp)
+        p.learned_cycles = 0
+        i1, r1, _ = prob.solve(W.SOLVE_LEARNED, p)
+        assert list(i1) == list(i0) and np.array_equal(r1, r0)
+        assert np.array_equal(prob.top.fs_download(W.FS_ETA), u0)
+        p.learned_cycles = 1
+        i2, r2, _ = prob.solve(W.SOLVE_LEARNED, p)
+        print(f"  plain {i0[-1]} cycles, learned first cycle {i2[-1]} cycles ({r2[-1]:.2e})", flush=True)
+        assert r2[-1] <= 1e-9 and i2[-1] <= i0[-1] + 1
+    finally:
+        prob.h.set_learned(None)
+        prob.close()
+        mlp.close()
+
+
 def test_learned_cycles_converge_with_any_network(grids):
     print("learned: untrained network", flush=True)
     prob = LC.Problem(grids, seed=2)

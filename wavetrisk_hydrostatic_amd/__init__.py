@@ -137,7 +137,7 @@ class MgParams(C.Structure):
     (lin_solve.f90:11-26); MgParams.defaults() gives the reference's."""
     _fields_ = [("coarse_tol", C.c_double), ("fine_tol", C.c_double), ("coarse_iter", C.c_int),
                 ("max_vcycle", C.c_int), ("pre_iter", C.c_int), ("down_iter", C.c_int), ("up_iter", C.c_int),
-                ("post_iter", C.c_int), ("max_srj_iter", C.c_int)]
+                ("post_iter", C.c_int), ("max_srj_iter", C.c_int), ("learned_cycles", C.c_int)]
 
     @classmethod
     def defaults(cls, **overrides):

@@ -1446,7 +1446,8 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
   // lv_res_err leaves f - L u of the current u in the top level's FS_W0
   for (int it = 1; it <= P.max_vcycle && err > P.fine_tol; ++it) {
     const double* r_top = c->fs[FS_W0];
-    if (learned) {
+    const bool step = learned && it <= P.learned_cycles;
+    if (step) {
       double *r = c->fs[FS_W0], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3];
       RC(lc_correction(m, r, e));
       RC(launch_apply(c, e, w));
@@ -1455,7 +1456,7 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
       RC(launch_lc(c, 1.0, -1, V[l].u, 1.0, SC_MR, e, V[l].u_alt));
       std::swap(V[l].u, V[l].u_alt);
     }
-    RC(vcycle(m, V, l, P, nrm_f, &err, learned ? nullptr : r_top));  // the learned step moved u
+    RC(vcycle(m, V, l, P, nrm_f, &err, step ? nullptr : r_top));  // the learned step moved u
     iters[l] = it;
     res_out[l] = err;
   }
@@ -1656,6 +1657,7 @@ void wtg_mg_default_params(wtg_mg_params* p) {
   p->up_iter = 2;
   p->post_iter = 4;
   p->max_srj_iter = 200;
+  p->learned_cycles = 1 << 30;
 }
 
 int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlevels, wtg_mg** out) {
