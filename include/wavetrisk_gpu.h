@@ -342,6 +342,13 @@ int wtg_mg_forward(wtg_mg* mg, int fine, int s_field, int wc_field, int coarse_f
  * `fine`, fine_field := Prolong_scalar / Reconstruct_scalar of level
  * fine - 1's coarse_field and level fine's wc_field. */
 int wtg_mg_inverse(wtg_mg* mg, int fine, int coarse_field, int wc_field, int fine_field);
+/* The velocity restriction of forward_wavelet_transform (Restrict_velo,
+ * wavelet.f90:1154-1174) on uniform levels: every coarse edge of coarse
+ * slot := the mean of the two fine edges of fine slot that bisect it, all
+ * layers (and members); then the coarse slot's ghost fill.  The levels
+ * hold the same layers; coarse is the next coarser level.  Errors are
+ * reported through wtg_last_error(fine). */
+int wtg_restrict_velo(wtg_ctx* fine, int fine_slot, wtg_ctx* coarse, int coarse_slot);
 
 /* The finest level's 8x8 patches of owned nodes (n a multiple of 8): 10
  * (n/8)^2 of them, lozenge-major, then patch row, then patch column; nodes

@@ -103,6 +103,7 @@ SIGNATURES = {
     "wtg_mg_inverse": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "wtg_mg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "wtg_mg_set_precision": (C.c_int, [_P, C.c_int]),
+    "wtg_restrict_velo": (C.c_int, [_P, C.c_int, _P, C.c_int]),
     "wtg_mg_npatches": (C.c_int, [_P]),
     "wtg_mg_set_learned": (C.c_int, [_P, _P]),
     "wtg_mg_features": (C.c_int, [_P, C.c_int, _P, _P, _P]),
@@ -341,6 +342,11 @@ class Context:
     def fs_set_params(self, dt: float, theta1: float = 1.0, theta2: float = 1.0):
         p = _FsParams(dt, theta1, theta2)
         _check(_lib.wtg_fs_set_params(self._h, C.byref(p)), self._h, "wtg_fs_set_params")
+
+    def restrict_velo(self, slot: int, coarse: "Context", coarse_slot: int):
+        """velocity of `slot` restricted onto the next coarser level's slot
+        (Restrict_velo, wavelet.f90:1154-1174), ghost filled there"""
+        _check(_lib.wtg_restrict_velo(self._h, slot, coarse._h, coarse_slot), self._h, "wtg_restrict_velo")
 
     def fs_upload(self, field: int, values):
         """[ncells], or [members, ncells] for an ensemble"""

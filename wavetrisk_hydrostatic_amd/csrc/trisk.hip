@@ -1139,6 +1139,28 @@ __global__ void ghost_fill_kernel(int L, int64_t NC, const int64_t* __restrict__
   }
 }
 
+// Restrict_velo (wavelet.f90:1154-1174) over the owned coarse nodes: a
+// coarse edge is the mean of the two fine edges that bisect it, in the
+// lozenge frame of the child (2i, 2j) -- RT: child, child + E; DG:
+// child + NE, child; UP: child, child + N.  One thread per (node, layer).
+__global__ void restrict_velo_kernel(int nc, int Sc, int Sf, int64_t NCc, int64_t NCf, int L,
+                                     const double* __restrict__ uf, double* __restrict__ uc) {
+  const int64_t per = (int64_t)10 * nc * nc;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * L) return;
+  const int k = (int)(t / per);
+  const int64_t r = t % per;
+  const int lz = (int)(r / ((int64_t)nc * nc)), rr = (int)(r % ((int64_t)nc * nc));
+  const int j = rr / nc, i = rr % nc;
+  const int64_t p = (int64_t)lz * Sc * Sc + (int64_t)(j + 2) * Sc + (i + 2);
+  const int64_t c = (int64_t)lz * Sf * Sf + (int64_t)(2 * j + 2) * Sf + (2 * i + 2);
+  const double* f = uf + (int64_t)k * 3 * NCf;
+  double* o = uc + (int64_t)k * 3 * NCc;
+  o[p] = 0.5 * (f[c] + f[c + 1]);
+  o[NCc + p] = 0.5 * (f[NCf + c + Sf + 1] + f[NCf + c]);
+  o[2 * NCc + p] = 0.5 * (f[2 * NCf + c] + f[2 * NCf + c + Sf]);
+}
+
 // copy frozen halo slots (never ghost filled) from one state to another
 __global__ void copy_frozen_kernel(int L, int64_t NC, const int64_t* __restrict__ fn, int nn,
                                    const int64_t* __restrict__ fe, int ne, const double* sm_, const double* st_,
@@ -2929,6 +2951,27 @@ int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter
   double tm[2] = {rel_tol, (double)max_iter};
   if (int rc = wtg_split_steps(c, slot, nsteps, bicgstab_solve, tm, iters)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_restrict_velo(wtg_ctx* fine, int fine_slot, wtg_ctx* coarse, int coarse_slot) {
+  if (!fine || !coarse) return WTG_E_ARG;
+  if (int rc = check_slot(fine, fine_slot)) return rc;
+  if (int rc = check_slot(coarse, coarse_slot)) return fail(fine, rc, "coarse slot: %s", coarse->err.c_str());
+  if (coarse->info.n * 2 != fine->info.n) return fail(fine, WTG_E_ARG, "the coarse level is not the next coarser one");
+  const int L = fine->params.nlayers * fine->nmembers;
+  if (coarse->params.nlayers * coarse->nmembers != L) return fail(fine, WTG_E_ARG, "the levels hold different layers");
+  if (coarse->device != fine->device) return fail(fine, WTG_E_ARG, "the levels are on different devices");
+  const int nc = coarse->info.n;
+  const int64_t total = (int64_t)10 * nc * nc * L;
+  // on the coarse level's stream, after the fine level's work
+  HIPCHK(fine, hipStreamSynchronize(fine->stream));
+  hipLaunchKernelGGL(restrict_velo_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, coarse->stream, nc,
+                     coarse->info.S, fine->info.S, coarse->info.ncells, fine->info.ncells, L,
+                     fine->slots[fine_slot].u, coarse->slots[coarse_slot].u);
+  HIPCHK(fine, hipGetLastError());
+  if (int rc = launch_ghost_fill(coarse, coarse->slots[coarse_slot])) return fail(fine, rc, "%s", coarse->err.c_str());
+  HIPCHK(fine, hipStreamSynchronize(coarse->stream));
   return 0;
 }
 
