@@ -349,6 +349,11 @@ int wtg_mg_inverse(wtg_mg* mg, int fine, int coarse_field, int wc_field, int fin
  * hold the same layers; coarse is the next coarser level.  Errors are
  * reported through wtg_last_error(fine). */
 int wtg_restrict_velo(wtg_ctx* fine, int fine_slot, wtg_ctx* coarse, int coarse_slot);
+/* Sub-sampling restriction (Restrict_ss, wavelet.f90:823-837, as for the
+ * coarse Bernoulli and Exner functions): on level fine - 1, coarse_field :=
+ * level fine's fine_field at the coincident nodes (poles included), halo
+ * copies written in the same launch. */
+int wtg_mg_restrict_ss(wtg_mg* mg, int fine, int fine_field, int coarse_field);
 
 /* The finest level's 8x8 patches of owned nodes (n a multiple of 8): 10
  * (n/8)^2 of them, lozenge-major, then patch row, then patch column; nodes

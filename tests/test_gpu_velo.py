@@ -50,3 +50,36 @@ def test_restrict_velo_equals_oracle():
         coarse.restrict_velo(0, fine, 0)
     fine.close()
     coarse.close()
+
+
+def test_restrict_ss_samples_coincident_nodes():
+    """Sub-sampling restriction (Restrict_ss, wavelet.f90:823-837): coarse
+    nodes coincide with the fine level's even nodes, so restricting the fine
+    node coordinates gives the coarse coordinates (poles included), and every
+    coarse master and halo copy equals the fine value at its child."""
+    print("sub-sampling restriction J5 -> J4", flush=True)
+    grids = [W.Grid(level=l, min_level=4, domain_level=1) for l in (4, 5)]
+    ctxs = []
+    for g in grids:
+        c = W.Context(g, nlayers=1)
+        c.set_topography(np.full(g.layout.ncells, -1000.0))
+        c.fs_set_params(100.0)
+        ctxs.append(c)
+    h = W.Hierarchy(ctxs)
+    try:
+        xf, xc = grids[1].array("x").reshape(3, -1), grids[0].array("x").reshape(3, -1)
+        lc = grids[0].layout
+        masters = lc.owned_cells()  # owned nodes and the two poles
+        for d in range(3):
+            ctxs[1].fs_upload(W.FS_DIVF, xf[d])
+            h.restrict_ss(1, W.FS_DIVF, W.FS_DIVF_OLD)
+            got = ctxs[0].fs_download(W.FS_DIVF_OLD)
+            assert np.max(np.abs(got[masters] - xc[d][masters])) <= 1e-15 * 6.371e6
+        # halo copies equal their masters
+        src = grids[0].array("node_src")
+        halo = np.nonzero((src >= 0) & (src != np.arange(len(src))))[0]
+        assert np.array_equal(got[halo], got[src[halo]])
+    finally:
+        h.close()
+        for c in ctxs:
+            c.close()

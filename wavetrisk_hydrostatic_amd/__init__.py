@@ -104,6 +104,7 @@ SIGNATURES = {
     "wtg_mg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "wtg_mg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_restrict_velo": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "wtg_mg_restrict_ss": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "wtg_mg_npatches": (C.c_int, [_P]),
     "wtg_mg_set_learned": (C.c_int, [_P, _P]),
     "wtg_mg_features": (C.c_int, [_P, C.c_int, _P, _P, _P]),
@@ -446,6 +447,11 @@ class Hierarchy:
     def set_learned(self, mlp: "MLP | None"):
         self._mlp = mlp
         _check(_lib.wtg_mg_set_learned(self._h, mlp._h if mlp is not None else None), self._top, "wtg_mg_set_learned")
+
+    def restrict_ss(self, fine: int, fine_field: int, coarse_field: int):
+        """coarse_field of level fine - 1 := fine_field of level fine at the
+        coincident nodes (Restrict_ss, wavelet.f90:823-837)"""
+        _check(_lib.wtg_mg_restrict_ss(self._h, fine, fine_field, coarse_field), self._top, "wtg_mg_restrict_ss")
 
     def set_precision(self, precision: int):
         """PRECISION_FP64 or PRECISION_FP32_BULK (FP32 smoothing and transfers

@@ -676,6 +676,18 @@ __global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo
   sc[ncl[2 * t]] = (F)(sf[c] + lift * g.hinv_c[p]);
 }
 
+// Restrict_ss (wavelet.f90:823-837) over the coarse compute list: the
+// coarse value is the fine value at the coincident child (poles included)
+__global__ void mg_restrict_ss_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const double* __restrict__ sf,
+                                      double* __restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t p = ncl[2 * t + 1];
+  int lz, i, j;
+  mg_ij(p, g.Sc, lz, i, j);
+  sc[ncl[2 * t]] = sf[(int64_t)lz * g.Sf * g.Sf + (int64_t)(2 * j + 2) * g.Sf + (2 * i + 2)];
+}
+
 // Prolong_scalar over the fine compute list, coincident nodes:
 // s_f = s_c(parent) - lift * hex_inv_c
 // ZeroW: all wavelets zero (prol_fun), wc is not read
@@ -1799,6 +1811,19 @@ int wtg_mg_inverse(wtg_mg* m, int fine, int coarse_field, int wc_field, int fine
   return 0;
 }
 
+
+int wtg_mg_restrict_ss(wtg_mg* m, int fine, int fine_field, int coarse_field) {
+  if (!m) return WTG_E_ARG;
+  if (int rc = mg_check_level(m, fine)) return rc;
+  wtg_ctx *f = m->lv[fine], *c = m->lv[fine - 1];
+  if (int rc = fs_field(f, fine_field)) return mg_err(m, rc, fine);
+  if (int rc = fs_field(c, coarse_field)) return mg_err(m, rc, fine - 1);
+  hipLaunchKernelGGL(mg_restrict_ss_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, f->stream, c->nncl, c->d_ncl,
+                     mg_geo(m, fine), f->fs[fine_field], c->fs[coarse_field]);
+  WTG_HIPCHK(f, hipGetLastError());
+  WTG_HIPCHK(f, hipStreamSynchronize(f->stream));
+  return 0;
+}
 
 int wtg_mg_npatches(const wtg_mg* m) { return m ? m->npatch : WTG_E_ARG; }
 
