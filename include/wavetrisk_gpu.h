@@ -77,13 +77,26 @@ int wtg_grid_info_get(const wtg_grid* grid, wtg_grid_info* info);
  * "len", "pedlen" [3][ncells], "area" [6][ncells], "hex_inv" [ncells],
  * "triarea", "coriolis" [2][ncells] (double); "node_src" [ncells],
  * "edge_src" [3*ncells] (int64; -1 = frozen halo slot, see below);
- * "edge_sign" [3*ncells] (int8).  bytes must equal the array's size. */
+ * "edge_sign" [3*ncells] (int8); "wide_src" [10*(n+10)^2] (int64): the
+ * owned cell of every lattice point (lz, i, j), i, j in [-5, n+5), index
+ * lz*(n+10)^2 + (j+5)*(n+10) + i+5, -1 where the lattice does not continue.
+ * bytes must equal the array's size. */
 int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes);
 /* Overlap areas of the scalar wavelet transform between a grid and the next
  * coarser one (get_overl_areas / set_RF_wgts, wavelet.f90:1408-1539):
  * [4][fine ncells] doubles, a(1..4) at fine nodes that are coarse edge
  * midpoints, 0 elsewhere.  Both grids need the same min_level/domain_level. */
 int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double* out, size_t bytes);
+/* Inter-level weights of the velocity and flux transforms between a grid and
+ * the next coarser one, as the reference computes them when it refines
+ * (set_WT_wgts / outer_velo_weights, set_RF_wgts / basic_F_restr_wgt,
+ * wavelet.f90:1176-1360, 1512-1705; replaces their host computation):
+ *   "split" [2][fine ncells]  split parts of the overlap areas
+ *   "iu"    [9][fine ncells]  I_u_wgt at the second end point of each outer child edge
+ *   "rf"    [4][6][coarse ncells]  R_F_wgt per coarse node: RT side k=0, RT k=1,
+ *           UP k=0, UP k=1, two triplets each (id_enc order). */
+int wtg_grid_transfer_weights(const wtg_grid* fine, const wtg_grid* coarse, const char* name, double* out,
+                              size_t bytes);
 void wtg_grid_destroy(wtg_grid* grid);
 
 /* ---- device context ------------------------------------------------------ */
@@ -354,6 +367,44 @@ int wtg_restrict_velo(wtg_ctx* fine, int fine_slot, wtg_ctx* coarse, int coarse_
  * level fine's fine_field at the coincident nodes (poles included), halo
  * copies written in the same launch. */
 int wtg_mg_restrict_ss(wtg_mg* mg, int fine, int fine_field, int coarse_field);
+
+/* ---- multi-level explicit step on a full hierarchy ------------------------
+ * One context per level, coarse to fine (consecutive levels, the same layers
+ * and slots, FP64, single member), each holding its level's state in the
+ * same slot.  On a full hierarchy (the reference's uniform refinement with
+ * zero thresholds: every mask TOLRNZ) every coarse node has children, so the
+ * coarse levels' Bernoulli and Exner functions, fluxes and velocity sources
+ * are restrictions of the finer level's.  The contexts' streams are pointed
+ * at the finest level's until wtg_ml_destroy; destroy the hierarchy before
+ * its contexts. */
+typedef struct wtg_ml wtg_ml;
+/* Uploads the inter-level weights (overlap areas, their split parts, I_u_wgt,
+ * R_F_wgt; wtg_grid_transfer_weights) and allocates the work buffers. */
+int wtg_ml_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlevels, wtg_ml** out);
+void wtg_ml_destroy(wtg_ml* ml);
+/* forward_wavelet_transform (wavelet.f90:20-101) of the slot: from the
+ * finest level down, scalar wavelets, Restrict_scalar and Restrict_velo
+ * into the coarser levels' slot, then the velocity wavelets of every level;
+ * the wavelets stay in the hierarchy (wtg_ml_wavelets). */
+int wtg_ml_forward(wtg_ml* ml, int slot);
+/* inverse_wavelet_transform (wavelet.f90:208-296) of the slot from its
+ * coarsest level and the hierarchy's wavelets: every finer level rebuilt. */
+int wtg_ml_inverse(wtg_ml* ml, int slot);
+/* Copies the wavelets of level (1..nlevels-1) to the host (to_device = 0) or
+ * from it (1): mass, temp [L][ncells], velo [L][3][ncells]; null skips. */
+int wtg_ml_wavelets(wtg_ml* ml, int level, int to_device, double* mass, double* temp, double* velo);
+/* trend_ml (multi_level.f90:5-188) on every level: the finest level's
+ * trend, the coarser levels' by compute-or-restrict (flux_cpt_restr,
+ * scalar_cpt_restr, u_source_cpt_restr); the trend of each level goes to
+ * dq_slot (owned cells, pole masters and owned edges). */
+int wtg_ml_trend(wtg_ml* ml, int q_slot, int dq_slot);
+/* WT_after_step (adapt.f90:192-237) of the slot: the wavelets of every
+ * level from its state, then the inverse transform. */
+int wtg_ml_wt_after_step(wtg_ml* ml, int slot);
+/* RK4 (time_integr.f90:78-103) on the hierarchy: per stage trend_ml,
+ * RK_sub_step on every level, WT_after_step.  Uses each context's two
+ * stage buffers. */
+int wtg_ml_rk4(wtg_ml* ml, int slot, double dt, int nsteps);
 
 /* The finest level's 8x8 patches of owned nodes (n a multiple of 8): 10
  * (n/8)^2 of them, lozenge-major, then patch row, then patch column; nodes

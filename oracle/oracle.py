@@ -297,3 +297,88 @@ class MgOracle:
         res = np.zeros(nl)
         self.lib.wto_srj(C.byref(self.h), int(max_srj_iter), U, F, iters.ctypes.data, res.ctypes.data)
         return u, iters, res
+
+
+class _Ml(C.Structure):
+    _fields_ = [("nlevels", C.c_int), ("grid", C.c_void_p * 16), ("ovl", C.c_void_p * 16), ("split", C.c_void_p * 16),
+                ("iu", C.c_void_p * 16), ("rf", C.c_void_p * 16), ("wide_src", C.c_void_p * 16), ("WH", C.c_int * 16)]
+
+
+class MlOracle:
+    """The multi-level explicit step (oracle/ml_oracle.c) on a full hierarchy
+    of grids (coarse to fine): per-level state lists of [L, nc] scalars and
+    [L, 3, nc] velocities."""
+
+    def __init__(self, grids, topos, grav=9.80616, rho0=1028.0, p_top=0.0):
+        self.grids = grids
+        self.levels = [Oracle(g.layout, geometry_of(g), t, grav=grav, rho0=rho0, p_top=p_top)
+                       for g, t in zip(grids, topos)]
+        nl = len(grids)
+        self._keep = []
+        self.h = _Ml()
+        self.h.nlevels = nl
+        for l in range(nl):
+            self.h.grid[l] = C.addressof(self.levels[l].g)
+            if l:
+                g, gc = grids[l], grids[l - 1]
+                arrs = {"ovl": _f64(g.overlap_areas(gc)).ravel(), "split": _f64(g.transfer_weights(gc, "split")).ravel(),
+                        "iu": _f64(g.transfer_weights(gc, "iu")).ravel(), "wide": g.wide_src()}
+                rf = _f64(g.transfer_weights(gc, "rf")).ravel()
+                self._keep += [arrs, rf]
+                self.h.ovl[l] = arrs["ovl"].ctypes.data
+                self.h.split[l] = arrs["split"].ctypes.data
+                self.h.iu[l] = arrs["iu"].ctypes.data
+                self.h.wide_src[l] = arrs["wide"].ctypes.data
+                self.h.WH[l] = g.WIDE_HALO
+                self.h.rf[l - 1] = rf.ctypes.data
+        self.lib = load()
+        lib = self.lib
+        lib.wto_velo_wavelets.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        lib.wto_velo_reconstruct.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        lib.wto_flux_restrict.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        lib.wto_trend_ml.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        lib.wto_wt_after_step.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        lib.wto_rk4_ml.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 3
+
+    def _ptrs(self, arrs):
+        return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+    def _copy(self, states):
+        return [_f64(a).copy() for a in states]
+
+    def ghost_fill(self, mass, temp, velo):
+        for l, lv in enumerate(self.levels):
+            lv.ghost_fill(mass[l], temp[l], velo[l])
+
+    def velo_wavelets(self, l, u_fine, u_coarse):
+        uf, uc = _f64(u_fine), _f64(u_coarse)
+        wc = np.zeros_like(uf)
+        self.lib.wto_velo_wavelets(C.byref(self.h), l, uf.ctypes.data, uc.ctypes.data, wc.ctypes.data)
+        return wc
+
+    def velo_reconstruct(self, l, u_coarse, wc, u_fine):
+        uf = _f64(u_fine).copy()
+        self.lib.wto_velo_reconstruct(C.byref(self.h), l, _f64(u_coarse).ctypes.data, _f64(wc).ctypes.data,
+                                      uf.ctypes.data)
+        return uf
+
+    def trend(self, mass, temp, velo):
+        mass, temp, velo = self._copy(mass), self._copy(temp), self._copy(velo)
+        L = mass[0].shape[0]
+        dm, dt, du = [np.zeros_like(a) for a in mass], [np.zeros_like(a) for a in temp], [np.zeros_like(a) for a in velo]
+        self.lib.wto_trend_ml(C.byref(self.h), L, self._ptrs(mass), self._ptrs(temp), self._ptrs(velo),
+                              self._ptrs(dm), self._ptrs(dt), self._ptrs(du))
+        return dm, dt, du
+
+    def wt_after_step(self, mass, temp, velo):
+        mass, temp, velo = self._copy(mass), self._copy(temp), self._copy(velo)
+        L = mass[0].shape[0]
+        self.lib.wto_wt_after_step(C.byref(self.h), L, self._ptrs(mass), self._ptrs(temp), self._ptrs(velo))
+        return mass, temp, velo
+
+    def rk4(self, mass, temp, velo, dt, nsteps):
+        mass, temp, velo = self._copy(mass), self._copy(temp), self._copy(velo)
+        L = mass[0].shape[0]
+        self.lib.wto_rk4_ml(C.byref(self.h), L, float(dt), int(nsteps), self._ptrs(mass), self._ptrs(temp),
+                            self._ptrs(velo))
+        return mass, temp, velo

@@ -84,6 +84,11 @@ typedef struct {
 
 void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
                double* dmass, double* dtemp, double* dvelo) {
+  wto_trend_parts(g, L, mass, temp, velo, dmass, dtemp, dvelo, NULL);
+}
+
+void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
+                     double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep) {
   const int n = g->n, S = g->S;
   const int64_t NC = g->ncells;
   const double grav = g->grav, rho0 = g->rho0;
@@ -256,6 +261,12 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
           w.bern[c] = ke + phi_k + w.press[c] / (rho0 * 1.0);
           w.exner[c] = -phi_k;
         }
+    if (keep) {
+      memcpy(keep->hflux_m + (int64_t)k * 3 * NC, w.hm, sizeof(double) * 3 * NC);
+      memcpy(keep->hflux_t + (int64_t)k * 3 * NC, w.ht, sizeof(double) * 3 * NC);
+      memcpy(keep->bern + (int64_t)k * NC, w.bern, sizeof(double) * NC);
+      memcpy(keep->exner + (int64_t)k * NC, w.exner, sizeof(double) * NC);
+    }
     /* scalar trends: -div(flux) on owned cells and the pole masters */
     WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
@@ -316,6 +327,7 @@ void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp,
           for (int e = 0; e < 3; ++e) {
             double src = -q[e] + 0.0 * LEN(e, id);
             du[e * NC + id] = src / LEN(e, id) - gB[e] - the[e] * gE[e];
+            if (keep) keep->du_src[((int64_t)k * 3 + e) * NC + id] = src;
           }
         }
 #undef HM

@@ -42,6 +42,17 @@ void wto_ghost_fill(const wto_grid* g, int L, double* mass, double* temp, double
 void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
                double* dmass, double* dtemp, double* dvelo);
 
+/* wto_trend with the quantities the coarser levels of a hierarchy restrict
+ * (multi_level.f90:49-188), per layer: the mass and temperature fluxes
+ * h_flux [L][3][ncells] on every edge the divergence reads, Bernoulli and
+ * Exner [L][ncells] on [0, n]^2, and the velocity source (-Qperp, before
+ * du_grad divides by the length) [L][3][ncells] on owned edges. */
+typedef struct {
+  double *hflux_m, *hflux_t, *bern, *exner, *du_src;
+} wto_trend_keep;
+void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
+                     double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep);
+
 /* nsteps low-storage RK4 steps (time_integr.f90:78-103, 206-225) in place;
  * the state is ghost filled on entry and on exit. */
 void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, double* temp, double* velo);

@@ -21,6 +21,9 @@ lozenge-tile layout:
                      barotropic_correction, grad_eta and one RK4_split step
   levels_J4.npz      hierarchy J4..J6 (harness dump_levels.f90): forward and
                      inverse scalar wavelet transforms and an FMG solve
+  mlstep_J4.npz      hierarchy J4..J5 (harness dump_mlstep.f90): transfer
+                     weights, velocity wavelets, inverse transform, trend_ml
+                     on every level, WT_after_step and two RK4 steps
 
 The fixtures are committed: the tests read them, never the reference build,
 which exists only where the reference sources are.
@@ -425,10 +428,152 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
     print(f"levels_J{J}.npz written: FMG iterations {out['fmg_iters']}")
 
 
+def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
+    """Multi-level explicit step on the full hierarchy J..maxlev (harness
+    dump_mlstep.f90, uniform refinement with zero thresholds): per level l,
+    on the owned cells and the two pole masters, topography, the initial
+    state s0 (with every halo copy, for the never-filled slots), its
+    velocity wavelets w0, the inverse transform i0 of (w0, s0), trend_ml t0,
+    WT_after_step of s0 + dt/4 t0 (a0) and the state after each RK4 step;
+    the masks; and, on each finer level, the transfer weights I_u_wgt,
+    overlap split parts and R_F_wgt at the reference's fine positions."""
+    from scipy.spatial import cKDTree
+    sys.path.insert(0, ROOT)
+    import wavetrisk_hydrostatic_amd as W
+    param, ml, dl = PARAMS[J]
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "ml.bin")
+        run_ref(param, f"&run maxlev={maxlev}, zlev={L}, nsteps={nsteps}, step={dt}d0, out='OUT' /\n", p,
+                prog="dump_mlstep")
+        recs = read_records(p)
+    meta = {t: a for t, a in recs if t in ("min_level", "params", "nelts")}
+    _, DL, PS, L_, ND, ls, le = (int(v) for v in meta["min_level"])
+    assert L_ == L
+    nsub = 4 ** DL
+    whole, state, ptabs, levids = {}, {}, {}, {}
+    cur = None
+    for t, a in recs:
+        if t == "ptab":
+            ptabs.setdefault(int(a[0]), []).append((int(a[1]) - 1, a[2:]))
+        elif t == "lev_ids":
+            cur = {"ids": a[1:] - 1}
+            levids.setdefault(int(a[0]), []).append(cur)
+        elif t == "lev_poles":
+            cur["poles"] = a
+        elif t in ("mask_n", "mask_e", "node_x", "node_y", "node_z", "topo", "ovl_a", "ovl_split", "iu_wgt",
+                   "rf_wgt"):
+            whole.setdefault(t, []).append(a)
+        elif t not in meta:
+            state.setdefault(t, []).append(a)
+    for t, m in (("ovl_a", 4), ("ovl_split", 2), ("iu_wgt", 9), ("rf_wgt", 3)):
+        whole[t] = [np.stack(whole[t][d * m:(d + 1) * m]) for d in range(ND)]
+    for t in list(state):
+        state[t] = [[state[t][k * ND + d] for d in range(ND)] for k in range(L)]
+    H_ = 2
+    m = PS + 2 * H_
+    out = dict(level_start=ls, level_end=le, min_level=ml, domain_level=dl, nlayers=L, dt=dt, nsteps=nsteps,
+               radius=meta["params"][0], omega=meta["params"][1], grav=meta["params"][2],
+               ref_density=meta["params"][3])
+    for l in range(ls, le + 1):
+        g = W.Grid(level=l, min_level=ml, domain_level=dl)
+        lay = g.layout
+        n, S, NC = lay.n, lay.S, lay.ncells
+        X = g.array("x")
+        own = lay.owned_cells(with_poles=False)
+        tree = cKDTree(X[:, own].T)
+        # (lz, i, j) -> [(dist, dd, element)] over every patch table of the level
+        cover = {}
+        for dd, tab in ptabs[l]:
+            t = tab.reshape(m, m) - 0
+            e0 = int(t[H_][H_])
+            P = np.array([whole["node_x"][dd][e0], whole["node_y"][dd][e0], whole["node_z"][dd][e0]])
+            dist, ix = tree.query(P)
+            assert dist < 1e-6
+            c0 = int(own[ix])
+            lz, r = c0 // (S * S), c0 % (S * S)
+            I0, J0 = r % S - H_, r // S - H_
+            for jj in range(-H_, PS + H_):
+                for ii in range(-H_, PS + H_):
+                    I, Jj = I0 + ii, J0 + jj
+                    if not (-H_ <= I < n + H_ and -H_ <= Jj < n + H_):
+                        continue
+                    e = int(t[jj + H_][ii + H_])
+                    if e < 0:
+                        continue
+                    dist_ = max(0, -ii, ii - (PS - 1)) + max(0, -jj, jj - (PS - 1))
+                    cover.setdefault((lz, I, Jj), []).append((dist_, dd, e))
+        for k in cover:
+            cover[k].sort()
+        poles = {lay.south_pole: None, lay.north_pole: None}
+        # pole masters: the lozenge-0 (south) and lozenge-5 (north) copies
+        for dd, dom in enumerate(levids[l]):
+            sp, npl = dom["poles"]
+            if sp > 0 and dd // nsub == 0:
+                poles[lay.south_pole] = (dd, sp - 1)
+            if npl > 0 and dd // nsub == 5:
+                poles[lay.north_pole] = (dd, npl - 1)
+        owned = {}
+        for lz in range(10):
+            for j in range(n):
+                for i in range(n):
+                    hits = [x for x in cover[(lz, i, j)] if x[0] == 0] or cover[(lz, i, j)]
+                    owned[lz * S * S + (j + H_) * S + i + H_] = hits[0][1:]
+        for c, v in poles.items():
+            owned[c] = v
+        halo = {}
+        for (lz, i, j), lst in cover.items():
+            if 0 <= i < n and 0 <= j < n:
+                continue
+            c = lz * S * S + (j + H_) * S + i + H_
+            if c in poles:
+                continue
+            halo[c] = lst[0][1:]
+
+        def field(arrs, mult, with_halo=False, per_layer=True):
+            nk = L if per_layer else 1
+            a = np.full((nk, mult, NC), np.nan)
+            for k in range(nk):
+                src = arrs[k] if per_layer else arrs
+                for c, (dd, e) in owned.items():
+                    a[k, :, c] = src[dd][mult * e:mult * e + mult]
+                if with_halo:
+                    for c, (dd, e) in halo.items():
+                        a[k, :, c] = src[dd][mult * e:mult * e + mult]
+            a = a if mult == 3 else a[:, 0]
+            return a if per_layer else a[0]
+
+        def comp(tag, ncomp):
+            a = np.full((ncomp, NC), np.nan)
+            for c, (dd, e) in owned.items():
+                a[:, c] = whole[tag][dd][:, e]
+            return a
+
+        out[f"topo_{l}"] = field(whole["topo"], 1, per_layer=False)
+        mn = field([a.astype(np.float64) for a in whole["mask_n"]], 1, per_layer=False)
+        me = field([a.astype(np.float64) for a in whole["mask_e"]], 3, per_layer=False)
+        out[f"mask_n_{l}"] = np.nan_to_num(mn, nan=-1).astype(np.int8)
+        out[f"mask_e_{l}"] = np.nan_to_num(me, nan=-1).astype(np.int8)
+        for pre, halo_ in (("s0", True), ("t0", False), ("a0", False)) + tuple(
+                (f"s{s:02d}", False) for s in range(1, nsteps + 1)):
+            out[f"{pre}_mass_{l}"] = field(state[f"{pre}_mass"], 1, halo_)
+            out[f"{pre}_temp_{l}"] = field(state[f"{pre}_temp"], 1, halo_)
+            out[f"{pre}_velo_{l}"] = field(state[f"{pre}_velo"], 3, halo_)
+        if l > ls:  # the wavelets and the inverse transform of the finer level
+            out[f"w0_mass_{l}"] = field(state["w0_mass"], 1)
+            out[f"w0_velo_{l}"] = field(state["w0_velo"], 3)
+            out[f"i0_velo_{l}"] = field(state["i0_velo"], 3)
+        if l > ls:
+            out[f"iu_{l}"] = comp("iu_wgt", 9)
+            out[f"split_{l}"] = comp("ovl_split", 2)
+            out[f"rf_{l}"] = comp("rf_wgt", 3)
+    np.savez_compressed(os.path.join(OUT, f"mlstep_J{J}.npz"), **out)
+    print(f"mlstep_J{J}.npz written")
+
+
 def main():
     if not os.path.isdir("/root/reference/src"):
         sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
-    what = sys.argv[1:] or ["grid", "trisk", "elliptic", "levels"]
+    what = sys.argv[1:] or ["grid", "trisk", "elliptic", "levels", "mlstep"]
     if "grid" in what:
         for J in sorted(PARAMS):
             grid_fixture(J)
@@ -440,6 +585,8 @@ def main():
             elliptic_fixture(*c)
     if "levels" in what:
         levels_fixture()
+    if "mlstep" in what:
+        mlstep_fixture()
 
 
 if __name__ == "__main__":

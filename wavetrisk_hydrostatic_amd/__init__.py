@@ -64,6 +64,7 @@ SIGNATURES = {
     "wtg_grid_get": (C.c_int, [_P, C.c_char_p, _P, C.c_size_t]),
     "wtg_grid_destroy": (None, [_P]),
     "wtg_grid_overlap_areas": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "wtg_grid_transfer_weights": (C.c_int, [_P, _P, C.c_char_p, _P, C.c_size_t]),
     "wtg_create": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.POINTER(_P)]),
     "wtg_create_ensemble": (C.c_int, [_P, C.POINTER(_Params), C.c_int, C.c_int, C.POINTER(_P)]),
     "wtg_destroy": (None, [_P]),
@@ -97,6 +98,14 @@ SIGNATURES = {
     "wtg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "wtg_mg_default_params": (None, [_P]),
     "wtg_mg_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "wtg_ml_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "wtg_ml_destroy": (None, [_P]),
+    "wtg_ml_forward": (C.c_int, [_P, C.c_int]),
+    "wtg_ml_inverse": (C.c_int, [_P, C.c_int]),
+    "wtg_ml_wavelets": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P]),
+    "wtg_ml_trend": (C.c_int, [_P, C.c_int, C.c_int]),
+    "wtg_ml_wt_after_step": (C.c_int, [_P, C.c_int]),
+    "wtg_ml_rk4": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "wtg_mg_destroy": (None, [_P]),
     "wtg_mg_solve": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "wtg_mg_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -218,6 +227,26 @@ class Grid:
         _check(load_library().wtg_grid_overlap_areas(self._h, coarse._h, _ptr(out), out.nbytes), None,
                "wtg_grid_overlap_areas")
         return out.reshape(4, self.layout.ncells)
+
+    WIDE_HALO = 5
+
+    def wide_src(self) -> np.ndarray:
+        """Owned cell of every lattice point within WIDE_HALO cells of each
+        lozenge's owned square, [10, n+2W, n+2W] flattened (-1: no lattice)."""
+        ws = self.layout.n + 2 * self.WIDE_HALO
+        out = np.empty(10 * ws * ws, dtype=np.int64)
+        _check(load_library().wtg_grid_get(self._h, b"wide_src", _ptr(out), out.nbytes), None, "wtg_grid_get(wide_src)")
+        return out
+
+    def transfer_weights(self, coarse: "Grid", name: str) -> np.ndarray:
+        """Velocity/flux transfer weights to the next coarser grid: "split"
+        [2, ncells], "iu" [9, ncells] (fine cells), "rf" [4, 6, coarse ncells]."""
+        shape = {"split": (2, self.layout.ncells), "iu": (9, self.layout.ncells),
+                 "rf": (4, 6, coarse.layout.ncells)}[name]
+        out = np.empty(int(np.prod(shape)))
+        _check(load_library().wtg_grid_transfer_weights(self._h, coarse._h, name.encode(), _ptr(out), out.nbytes),
+               None, "wtg_grid_transfer_weights")
+        return out.reshape(shape)
 
     def array(self, name: str) -> np.ndarray:
         comp, dt = self._SHAPES[name]
@@ -476,6 +505,73 @@ class Hierarchy:
 
 
 MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_NPARAMS = 192, 128, 64, 49472
+
+
+class MultiLevel:
+    """The multi-level explicit step on a full hierarchy of uniform levels
+    (wtg_ml_*): contexts coarse to fine with the same layers and slots, each
+    holding its level's state.  While it exists the contexts run on the
+    finest one's stream; close it before the contexts."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        n = len(self.contexts)
+        grids = (_P * n)(*[c.grid.handle.value for c in self.contexts])
+        ctxs = (_P * n)(*[c._h.value for c in self.contexts])
+        h = _P()
+        _check(load_library().wtg_ml_create(grids, ctxs, n, C.byref(h)), None, "wtg_ml_create")
+        self._h = h
+
+    @property
+    def _top(self):
+        return self.contexts[-1]._h
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.wtg_ml_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, slot: int, mass, temp, velo):
+        """per-level lists of [L, nc] / [L, 3, nc] arrays"""
+        for c, m, t, u in zip(self.contexts, mass, temp, velo):
+            c.upload_state(slot, m, t, u)
+
+    def download(self, slot: int):
+        out = [c.download_state(slot) for c in self.contexts]
+        return [o[0] for o in out], [o[1] for o in out], [o[2] for o in out]
+
+    def forward(self, slot: int):
+        _check(_lib.wtg_ml_forward(self._h, slot), self._top, "wtg_ml_forward")
+
+    def inverse(self, slot: int):
+        _check(_lib.wtg_ml_inverse(self._h, slot), self._top, "wtg_ml_inverse")
+
+    def wavelets(self, level: int):
+        c = self.contexts[level]
+        (L, nc), (_, _, _) = c._shape()
+        m, t, u = np.empty((L, nc)), np.empty((L, nc)), np.empty((L, 3, nc))
+        _check(_lib.wtg_ml_wavelets(self._h, level, 0, _ptr(m), _ptr(t), _ptr(u)), self._top, "wtg_ml_wavelets")
+        return m, t, u
+
+    def set_wavelets(self, level: int, mass, temp, velo):
+        c = self.contexts[level]
+        (L, nc), _ = c._shape()
+        m = np.ascontiguousarray(mass, dtype=np.float64).reshape(L, nc)
+        t = np.ascontiguousarray(temp, dtype=np.float64).reshape(L, nc)
+        u = np.ascontiguousarray(velo, dtype=np.float64).reshape(L, 3, nc)
+        _check(_lib.wtg_ml_wavelets(self._h, level, 1, _ptr(m), _ptr(t), _ptr(u)), self._top, "wtg_ml_wavelets")
+
+    def trend(self, q_slot: int, dq_slot: int):
+        _check(_lib.wtg_ml_trend(self._h, q_slot, dq_slot), self._top, "wtg_ml_trend")
+
+    def wt_after_step(self, slot: int):
+        _check(_lib.wtg_ml_wt_after_step(self._h, slot), self._top, "wtg_ml_wt_after_step")
+
+    def rk4(self, slot: int, dt: float, nsteps: int = 1):
+        _check(_lib.wtg_ml_rk4(self._h, slot, dt, nsteps), self._top, "wtg_ml_rk4")
 
 
 class MLP:

@@ -161,3 +161,21 @@ int wtg_fs_rhs_async(wtg_ctx* c);
 // WTG_FS_RHS on c with WTG_FS_ETA_OLD set, the old eta as initial guess
 typedef int (*wtg_fs_solver)(void* user, wtg_ctx* c, int* iters);
 int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters);
+
+// intermediate quantities of the finest level's stage that a hierarchy's
+// coarser levels restrict (trisk.hip; null members are not kept)
+struct WtgStageKeep {
+  double *hm, *ht;     // [L][3][NC] mass / temperature fluxes on region cells [-1, n]^2
+  double *bern, *exn;  // [L][NC] Bernoulli and Exner functions on [0, n]^2
+  double* src;         // [L][3][NC] velocity source (-Qperp) on owned edges
+  double *dm, *dt;     // [L][NC] scalar trends on owned cells and the pole masters
+};
+// one RK stage (or trend, trend_only = 1) of the FP64 layer-pair kernel
+// with optional kept quantities
+int wtg_launch_stage_keep(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, const wtg_ctx::Slot& out,
+                          double h, int trend_only, const WtgStageKeep* keep);
+// ghost fill of L layers of node fields m, t (skipped when m is null) and of
+// an edge field u (skipped when null) with the context's lists
+int wtg_ghost_fill_layers(wtg_ctx* c, int L, double* m, double* t, double* u);
+// copies the never-filled halo slots of one state to another
+int wtg_copy_frozen_slots(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to);

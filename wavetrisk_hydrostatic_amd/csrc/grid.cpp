@@ -322,6 +322,41 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
   // both end nodes of every tile edge are known
   const int EH = HALO + 1, ES = n + 2 * EH;
   auto ecell = [&](int lz, int i, int j) { return (size_t)lz * ES * ES + (size_t)(j + EH) * ES + (size_t)(i + EH); };
+  // owned cell at lozenge-frame point (lz, i, j) off the tile's owned square
+  // (-1 where the lattice cannot be continued across the boundary)
+  auto resolve = [&](int lz, int i, int j) -> int64_t {
+    int ic = std::min(std::max(i, 0), n), jc = std::min(std::max(j, 0), n);
+    int di = i - ic, dj = j - jc;
+    if (di == 0 && dj == 0) return owned_cell_at(nat_of(lz, ic, jc));  // on the natural boundary row/column
+    // anchor strictly inside the side where possible (avoid vertices)
+    int64_t found = -1;
+    for (auto& o : others(lz, ic, jc)) {
+      int l2 = o[0], i2 = o[1], j2 = o[2];
+      // along-edge direction: a lattice neighbour of the anchor that is
+      // also on the shared boundary
+      for (int k = 0; k < 6 && found < 0; ++k) {
+        int ai = ic + cyc[k][0], aj = jc + cyc[k][1];
+        if (ai < 0 || ai > n || aj < 0 || aj > n) continue;
+        V3 pa = nat_of(lz, ai, aj);
+        for (int kk = 0; kk < 6; ++kk) {
+          int bi = i2 + cyc[kk][0], bj = j2 + cyc[kk][1];
+          if (bi < 0 || bi > n || bj < 0 || bj > n) continue;
+          if (!same(nat_of(l2, bi, bj), pa)) continue;
+          int r = ((kk - k) % 6 + 6) % 6;
+          int ra, rb;
+          rot(r, di, dj, ra, rb);
+          int ti = i2 + ra, tj = j2 + rb;
+          if (ti < 0 || ti > n || tj < 0 || tj > n) break;
+          // the target must lie on the far side of the shared edge
+          int64_t oc = owned_cell_at(nat_of(l2, ti, tj));
+          if (oc >= 0) found = oc;
+          break;
+        }
+      }
+      if (found >= 0) break;
+    }
+    return found;
+  };
   std::vector<int64_t> esrc((size_t)10 * ES * ES, -1);
   for (int lz = 0; lz < 10; ++lz)
     for (int j = -EH; j < n + EH; ++j)
@@ -335,40 +370,20 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
           esrc[c] = (int64_t)cell(lz, i, j);
           continue;
         }
-        int ic = std::min(std::max(i, 0), n), jc = std::min(std::max(j, 0), n);
-        int di = i - ic, dj = j - jc;
-        if (di == 0 && dj == 0) {  // on the natural boundary row/column
-          esrc[c] = owned_cell_at(nat_of(lz, ic, jc));
-          continue;
-        }
-        // anchor strictly inside the side where possible (avoid vertices)
-        int64_t found = -1;
-        for (auto& o : others(lz, ic, jc)) {
-          int l2 = o[0], i2 = o[1], j2 = o[2];
-          // along-edge direction: a lattice neighbour of the anchor that is
-          // also on the shared boundary
-          for (int k = 0; k < 6 && found < 0; ++k) {
-            int ai = ic + cyc[k][0], aj = jc + cyc[k][1];
-            if (ai < 0 || ai > n || aj < 0 || aj > n) continue;
-            V3 pa = nat_of(lz, ai, aj);
-            for (int kk = 0; kk < 6; ++kk) {
-              int bi = i2 + cyc[kk][0], bj = j2 + cyc[kk][1];
-              if (bi < 0 || bi > n || bj < 0 || bj > n) continue;
-              if (!same(nat_of(l2, bi, bj), pa)) continue;
-              int r = ((kk - k) % 6 + 6) % 6;
-              int ra, rb;
-              rot(r, di, dj, ra, rb);
-              int ti = i2 + ra, tj = j2 + rb;
-              if (ti < 0 || ti > n || tj < 0 || tj > n) break;
-              // the target must lie on the far side of the shared edge
-              int64_t oc = owned_cell_at(nat_of(l2, ti, tj));
-              if (oc >= 0) found = oc;
-              break;
-            }
-          }
-          if (found >= 0) break;
-        }
-        esrc[c] = found;
+        esrc[c] = resolve(lz, i, j);
+      }
+  // a wider frame of plain lattice continuation, for the few stencils that
+  // reach more than HALO cells beyond a lozenge side (flux restriction)
+  g.WH = HALO + 3;
+  g.WS = n + 2 * g.WH;
+  g.wide_src.assign((size_t)10 * g.WS * g.WS, -1);
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = -g.WH; j < n + g.WH; ++j)
+      for (int i = -g.WH; i < n + g.WH; ++i) {
+        const size_t c = g.wcell(lz, i, j);
+        if (i >= 0 && i < n && j >= 0 && j < n) g.wide_src[c] = (int64_t)cell(lz, i, j);
+        else if (i >= -EH && j >= -EH && i < n + EH && j < n + EH) g.wide_src[c] = esrc[ecell(lz, i, j)];
+        else g.wide_src[c] = resolve(lz, i, j);
       }
   (void)dir_index;
   // --- 4. pole neighbourhoods ------------------------------------------------
@@ -672,6 +687,10 @@ Level build_level(int level, int min_level, int domain_level, double radius, dou
     g.cle[3 * x] = CL[x].x; g.cle[3 * x + 1] = CL[x].y; g.cle[3 * x + 2] = CL[x].z;
     g.cue[3 * x] = CU[x].x; g.cue[3 * x + 1] = CU[x].y; g.cue[3 * x + 2] = CU[x].z;
   }
+  g.mpe.resize(9 * ENC);
+  for (size_t x = 0; x < 3 * ENC; ++x) {
+    g.mpe[3 * x] = MP[x].x; g.mpe[3 * x + 1] = MP[x].y; g.mpe[3 * x + 2] = MP[x].z;
+  }
 
   // --- 7. copy the tile window out of the virtual frame ------------------------
   g.X.assign(3 * NC, 0.0);
@@ -767,77 +786,93 @@ constexpr int HEX_S_OFFS[3] = {2, 0, 4};
 
 }  // namespace
 
+// get_overl_areas (wavelet.f90:1408-1475) for the child of coarse node
+// (ip, jp) of lozenge lz on edge e: the fine node (2ip + [e<2], 2jp + [e>0])
+// gets area[0..7] (a(1..4), then the split parts area(5..8)) and typ[0..1]
+// (INSIDE 0, OUTER1 1, OUTER2 2 for the sides k = 0, 1).  Returns false when
+// the stencil leaves the extended frames.
+bool overl_node(const Level& fine, const Level& coarse, int lz, int ip, int jp, int e, double area[8], int typ[2]) {
+  const double R = fine.radius;
+  auto V = [](const std::vector<double>& v, size_t x) { return V3{v[3 * x], v[3 * x + 1], v[3 * x + 2]}; };
+  const int nf = fine.n, nc = coarse.n;
+  auto fcc = [&](int i, int j, const Tri& t) {
+    size_t x = fine.ecell(lz, i + t.di, j + t.dj);
+    return t.t == 0 ? V(fine.cle, x) : V(fine.cue, x);
+  };
+  auto ccc = [&](int i, int j, const Tri& t) {
+    size_t x = coarse.ecell(lz, i + t.di, j + t.dj);
+    return t.t == 0 ? V(coarse.cle, x) : V(coarse.cue, x);
+  };
+  const int ic = 2 * ip + (e == 0 || e == 1 ? 1 : 0), jc = 2 * jp + (e == 1 || e == 2 ? 1 : 0);
+  if (ic - 1 < -fine.EH || jc - 1 < -fine.EH || ic > nf + 1 || jc > nf + 1) return false;
+  if (ip - 1 < -coarse.EH || jp - 1 < -coarse.EH || ip + 1 > nc + 1 || jp + 1 > nc + 1) return false;
+  V3 hex[6];
+  const int so = HEX_S_OFFS[2 - e];
+  for (int k = 0; k < 6; ++k) hex[k] = fcc(ic, jc, NO_ADJ[k + so]);
+  // coarse circumcentres around the parent edge: tri(row, col)
+  V3 tri[3][2];
+  tri[0][0] = ccc(ip, jp, BFLY[e][3]);
+  tri[1][0] = ccc(ip, jp, ADJ[e][1]);
+  tri[2][0] = ccc(ip, jp, BFLY[e][2]);
+  tri[0][1] = ccc(ip, jp, BFLY[e][1]);
+  tri[1][1] = ccc(ip, jp, ADJ[e][0]);
+  tri[2][1] = ccc(ip, jp, BFLY[e][0]);
+  const V3 pt = V(fine.xe, fine.ecell(lz, ic, jc));
+  for (int k = 0; k < 8; ++k) area[k] = 0.0;
+  typ[0] = typ[1] = 0;
+  area[0] = triarea(hex[5], hex[0], pt, R);
+  area[1] = triarea(hex[2], hex[3], pt, R);
+  for (int i = 1; i <= 2; ++i) {
+    const int c = i - 1;                     // tri column
+    const V3 h1 = hex[3 * i - 3], h2 = hex[3 * i - 2], h3 = hex[3 * i - 1];  // hex(3i-2), hex(3i-1), hex(3i)
+    V3 p0, p1;
+    bool d0 = arc_inters(tri[0][c], tri[1][c], h1, h2, p0, R);
+    bool d1 = arc_inters(tri[2][c], tri[1][c], h3, h2, p1, R);
+    double& ai = area[i - 1];
+    double& aj = area[2 - i];   // area(-i+3)
+    if (d0 && d1) {
+      area[i + 3] = triarea(p0, tri[1][c], h2, R);
+      area[i + 5] = triarea(tri[1][c], h2, p1, R);
+      area[i + 1] = area[i + 3] + area[i + 5];
+      ai = ai + triarea(h1, p0, pt, R) + triarea(p0, pt, tri[1][c], R);
+      aj = aj + triarea(p1, h3, pt, R) + triarea(tri[1][c], pt, p1, R);
+      typ[2 - i] = 0;
+    } else if (!d0 && !d1) {
+      area[i + 1] = 0.0;
+      d0 = arc_inters(tri[1][0], tri[1][1], h1, h2, p0, R);
+      d1 = arc_inters(tri[1][1], tri[1][0], h2, h3, p1, R);
+      if (!d0 && d1) {
+        ai = ai + triarea(h1, h2, pt, R) + triarea(h2, p1, pt, R);
+        aj = aj + triarea(p1, h3, pt, R);
+        typ[2 - i] = 2;
+      } else if (d0 && !d1) {
+        ai = ai + triarea(h1, p0, pt, R);
+        aj = aj + triarea(h2, h3, pt, R) + triarea(p0, h2, pt, R);
+        typ[2 - i] = 1;
+      }
+    }
+  }
+  return true;
+}
+
 std::vector<double> overlap_areas(const Level& fine, const Level& coarse) {
   if (fine.n != 2 * coarse.n) throw std::runtime_error("overlap_areas: levels are not consecutive");
   const size_t NC = fine.ncells();
-  const double R = fine.radius;
   std::vector<double> a(4 * NC, 0.0);
-  auto V = [](const std::vector<double>& v, size_t x) { return V3{v[3 * x], v[3 * x + 1], v[3 * x + 2]}; };
   const int nf = fine.n, nc = coarse.n;
-  for (int lz = 0; lz < 10; ++lz) {
-    auto fcc = [&](int i, int j, const Tri& t) {
-      size_t x = fine.ecell(lz, i + t.di, j + t.dj);
-      return t.t == 0 ? V(fine.cle, x) : V(fine.cue, x);
-    };
-    auto ccc = [&](int i, int j, const Tri& t) {
-      size_t x = coarse.ecell(lz, i + t.di, j + t.dj);
-      return t.t == 0 ? V(coarse.cle, x) : V(coarse.cue, x);
-    };
+  for (int lz = 0; lz < 10; ++lz)
     // parents whose children fall in the fine tile window with a complete stencil
     for (int jp = -1; jp <= nc; ++jp)
       for (int ip = -1; ip <= nc; ++ip)
         for (int e = 0; e < 3; ++e) {
           const int ic = 2 * ip + (e == 0 || e == 1 ? 1 : 0), jc = 2 * jp + (e == 1 || e == 2 ? 1 : 0);
           if (ic < -HALO || jc < -HALO || ic >= nf + HALO || jc >= nf + HALO) continue;
-          // hexagon corners: fine circumcentres around the child (needs fine cells >= -EH)
-          if (ic - 1 < -fine.EH || jc - 1 < -fine.EH || ic > nf + 1 || jc > nf + 1) continue;
-          if (ip - 1 < -coarse.EH || jp - 1 < -coarse.EH || ip + 1 > nc + 1 || jp + 1 > nc + 1) continue;
-          V3 hex[6];
-          const int so = HEX_S_OFFS[2 - e];
-          for (int k = 0; k < 6; ++k) hex[k] = fcc(ic, jc, NO_ADJ[k + so]);
-          // coarse circumcentres around the parent edge: tri(row, col)
-          V3 tri[3][2];
-          tri[0][0] = ccc(ip, jp, BFLY[e][3]);
-          tri[1][0] = ccc(ip, jp, ADJ[e][1]);
-          tri[2][0] = ccc(ip, jp, BFLY[e][2]);
-          tri[0][1] = ccc(ip, jp, BFLY[e][1]);
-          tri[1][1] = ccc(ip, jp, ADJ[e][0]);
-          tri[2][1] = ccc(ip, jp, BFLY[e][0]);
-          const V3 pt = V(fine.xe, fine.ecell(lz, ic, jc));
-          double area[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          area[0] = triarea(hex[5], hex[0], pt, R);
-          area[1] = triarea(hex[2], hex[3], pt, R);
-          for (int i = 1; i <= 2; ++i) {
-            const int c = i - 1;                     // tri column
-            const V3 h1 = hex[3 * i - 3], h2 = hex[3 * i - 2], h3 = hex[3 * i - 1];  // hex(3i-2), hex(3i-1), hex(3i)
-            V3 p0, p1;
-            bool d0 = arc_inters(tri[0][c], tri[1][c], h1, h2, p0, R);
-            bool d1 = arc_inters(tri[2][c], tri[1][c], h3, h2, p1, R);
-            double& ai = area[i - 1];
-            double& aj = area[2 - i];   // area(-i+3)
-            if (d0 && d1) {
-              area[i + 3] = triarea(p0, tri[1][c], h2, R);
-              area[i + 5] = triarea(tri[1][c], h2, p1, R);
-              area[i + 1] = area[i + 3] + area[i + 5];
-              ai = ai + triarea(h1, p0, pt, R) + triarea(p0, pt, tri[1][c], R);
-              aj = aj + triarea(p1, h3, pt, R) + triarea(tri[1][c], pt, p1, R);
-            } else if (!d0 && !d1) {
-              area[i + 1] = 0.0;
-              d0 = arc_inters(tri[1][0], tri[1][1], h1, h2, p0, R);
-              d1 = arc_inters(tri[1][1], tri[1][0], h2, h3, p1, R);
-              if (!d0 && d1) {
-                ai = ai + triarea(h1, h2, pt, R) + triarea(h2, p1, pt, R);
-                aj = aj + triarea(p1, h3, pt, R);
-              } else if (d0 && !d1) {
-                ai = ai + triarea(h1, p0, pt, R);
-                aj = aj + triarea(h2, h3, pt, R) + triarea(p0, h2, pt, R);
-              }
-            }
-          }
+          double area[8];
+          int typ[2];
+          if (!overl_node(fine, coarse, lz, ip, jp, e, area, typ)) continue;
           const size_t cf = fine.cell(lz, ic, jc);
           for (int k = 0; k < 4; ++k) a[k * NC + cf] = area[k];
         }
-  }
   // Halo nodes hold their owner's areas, as the reference's node
   // communication leaves them (get_areas / set_areas, refine_patch.f90:
   // 436-467): across a rotated lozenge join the midpoint changes type (RT,
@@ -865,6 +900,332 @@ std::vector<double> overlap_areas(const Level& fine, const Level& coarse) {
         for (int k = 0; k < 4; ++k) a[k * NC + c] = a[perm[k] * NC + (size_t)src];
       }
   return a;
+}
+
+
+// ---------------------------------------------------------------------------
+// Transfer weights of the velocity and flux transforms
+// ---------------------------------------------------------------------------
+namespace {
+
+// 6 x 6 dense solve by LU with partial pivoting in the order of LAPACK's
+// DGETF2 (pivot = first largest |a|, column scaled by the pivot's
+// reciprocal, rank-1 update) and DGETRS (row swaps, unit lower then upper
+// triangular solve, column-oriented as the reference BLAS DTRSM).  G is
+// column-major (G[c][r]); b is overwritten by the solution.
+void dgesv6(double G[6][6], double b[6]) {
+  int ipiv[6];
+  for (int j = 0; j < 6; ++j) {
+    int jp = j;
+    double amax = std::fabs(G[j][j]);
+    for (int i = j + 1; i < 6; ++i)
+      if (std::fabs(G[j][i]) > amax) { amax = std::fabs(G[j][i]); jp = i; }
+    ipiv[j] = jp;
+    if (G[j][jp] != 0.0) {
+      if (jp != j)
+        for (int c = 0; c < 6; ++c) std::swap(G[c][j], G[c][jp]);
+      const double sfmin = 2.2250738585072014e-308;
+      if (std::fabs(G[j][j]) >= sfmin) {
+        const double r = 1.0 / G[j][j];
+        for (int i = j + 1; i < 6; ++i) G[j][i] = r * G[j][i];
+      } else {
+        for (int i = j + 1; i < 6; ++i) G[j][i] = G[j][i] / G[j][j];
+      }
+    }
+    for (int c = j + 1; c < 6; ++c) {
+      if (G[c][j] != 0.0) {
+        const double t = -1.0 * G[c][j];
+        for (int i = j + 1; i < 6; ++i) G[c][i] = G[c][i] + G[j][i] * t;
+      }
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    if (ipiv[i] != i) std::swap(b[i], b[ipiv[i]]);
+  for (int k = 0; k < 6; ++k)
+    if (b[k] != 0.0)
+      for (int i = k + 1; i < 6; ++i) b[i] = b[i] - b[k] * G[k][i];
+  for (int k = 5; k >= 0; --k)
+    if (b[k] != 0.0) {
+      b[k] = b[k] / G[k][k];
+      for (int i = 0; i < k; ++i) b[i] = b[i] - b[k] * G[k][i];
+    }
+}
+
+inline V3 scal(double a, V3 v) { return {a * v.x, a * v.y, a * v.z}; }
+// sphere.f90:256-268 normalize_Coord (zero below radius * 1e-13)
+inline V3 normalize(V3 v, double R) {
+  const double n = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  if (n >= R * 1e-13) return {v.x / n, v.y / n, v.z / n};
+  return {0, 0, 0};
+}
+
+// wavelet.f90:1761-1775 coords_to_rowd: direction (u, v) of dirvec in the
+// local frame (x, y) and the midpoint's local coordinates
+inline void row_d(V3 midpt, V3 dir, V3 x, V3 y, double out[6]) {
+  double q0 = inner(dir, x), q1 = inner(dir, y);
+  const double n = std::sqrt(q0 * q0 + q1 * q1);
+  const double u = q0 / n, v = q1 / n;
+  const double a = inner(midpt, x), b = inner(midpt, y);
+  out[0] = u; out[1] = u * a; out[2] = u * b; out[3] = v; out[4] = v * a; out[5] = v * b;
+}
+
+// shared.f90:282-327 tables (0-based edge index e = RT 0, DG 1, UP 2)
+constexpr int HEXS[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {-1, 0, 0}, {-1, -1, 1},
+                             {0, -1, 2}, {0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {-1, 0, 0}};
+constexpr int NGHB[10][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 0}, {-1, -1}, {0, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 0}};
+constexpr int ENDPT[3][2][2] = {{{0, 0}, {1, 0}}, {{1, 1}, {0, 0}}, {{0, 0}, {0, 1}}};   // [e][k][i|j]
+constexpr int OPPNO[3][2][2] = {{{0, -1}, {1, 1}}, {{0, 1}, {1, 0}}, {{1, 1}, {-1, 0}}};
+
+// flux interpolation points (shared.f90:85-115)
+enum { UZM = 0, UPZ, UMZ, UZP, VMM, VPM, VMP, VPP, WMM, WPM, WMP, WPP, WMMM, WPPP, VMPP, VPMM };
+
+struct Frame {
+  const Level& g;
+  int lz;
+  V3 node(int i, int j) const {
+    const size_t x = g.ecell(lz, i, j);
+    return {g.xe[3 * x], g.xe[3 * x + 1], g.xe[3 * x + 2]};
+  }
+  V3 cc(int i, int j, int t) const {
+    const size_t x = g.ecell(lz, i, j);
+    const std::vector<double>& v = t == 0 ? g.cle : g.cue;
+    return {v[3 * x], v[3 * x + 1], v[3 * x + 2]};
+  }
+  V3 mid(int i, int j, int e) const {
+    const size_t x = 3 * g.ecell(lz, i, j) + e;
+    return {g.mpe[3 * x], g.mpe[3 * x + 1], g.mpe[3 * x + 2]};
+  }
+  bool in(int i, int j) const { return i >= -g.EH && j >= -g.EH && i < g.n + g.EH && j < g.n + g.EH; }
+};
+
+// outer_velo_weights (wavelet.f90:1604-1705) for coarse node (i0, j0), edge
+// e0: the nine interpolation weights minus Iu_Base_Wgt
+void outer_velo_weights(const Frame& C, int i0, int j0, int e0, double out[9]) {
+  static const double base[9] = {16.0 / 16, -1.0 / 16, 1.0 / 16, 1.0 / 16, -1.0 / 16,
+                                 -1.0 / 16, -1.0 / 16, 1.0 / 16, 1.0 / 16};
+  const double R = C.g.radius;
+  const int n = C.g.n;
+  // coordinates with the pentagon substitutions of get_coord (wavelet.f90:1670-1703)
+  auto coord = [&](int i, int j) -> V3 {
+    if (i == -1 && j == -1) {
+      if (e0 == 0) return C.node(-1, 0);
+      if (e0 == 2) return C.node(0, -1);
+      return C.node(i, j);
+    }
+    if (i == -1 && j == n) return C.node(0, n + 1);
+    if (i == n && j == -1) return C.node(n + 1, 0);
+    if (i == n + 1 && j == n + 1) return C.node(n + 1, n);
+    return C.node(i, j);
+  };
+  const V3 e1 = C.node(i0 + ENDPT[e0][0][0], j0 + ENDPT[e0][0][1]);
+  const V3 e2 = C.node(i0 + ENDPT[e0][1][0], j0 + ENDPT[e0][1][1]);
+  const V3 mp = C.mid(i0, j0, e0);
+  const V3 x = normalize(sub(e2, e1), R);
+  const V3 y = normalize(cross(x, mp), R);
+  auto row = [&](int o1i, int o1j, const int* hs, double r[6]) {
+    const int i = i0 + o1i + hs[0], j = j0 + o1j + hs[1], e = hs[2];
+    const V3 p1 = coord(i + ENDPT[e][0][0], j + ENDPT[e][0][1]);
+    const V3 p2 = coord(i + ENDPT[e][1][0], j + ENDPT[e][1][1]);
+    row_d(mid_pt(p1, p2, R), sub(p2, p1), x, y, r);
+  };
+  const int so = HEX_S_OFFS[e0];
+  double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 1; k <= 2; ++k) {
+    const int* ek = ENDPT[e0][k - 1];
+    const int* ek2 = ENDPT[e0][2 - k];   // end_pt(:, -k+3)
+    const int* ok = OPPNO[e0][k - 1];
+    double G[6][6], r1[6], r2[6];
+    const int zero[3] = {0, 0, e0};
+    row(0, 0, zero, G[0]);
+    row(ek2[0], ek2[1], HEXS[so + 2 + 3 * k - 3], G[1]);
+    row(ek[0], ek[1], HEXS[so + 3 - (3 * k - 3)], G[2]);
+    row(ok[0], ok[1], HEXS[so + 1 + 3 * k - 3], r1);
+    row(ek[0], ek[1], HEXS[so + 2 + 3 * k - 3], r2);
+    for (int m = 0; m < 6; ++m) G[3][m] = r1[m] - r2[m];
+    row(ek2[0], ek2[1], HEXS[so + 3 - (3 * k - 3)], r1);
+    row(ok[0], ok[1], HEXS[so + 4 - (3 * k - 3)], r2);
+    for (int m = 0; m < 6; ++m) G[4][m] = r1[m] - r2[m];
+    row(ek[0], ek[1], HEXS[so + 5 - (3 * k - 3)], r1);
+    row(ek2[0], ek2[1], HEXS[so + 0 + 3 * k - 3], r2);
+    for (int m = 0; m < 6; ++m) G[5][m] = r1[m] - r2[m];
+    double b[6];
+    row_d(mid_pt(mp, e2, R), sub(e2, e1), x, y, b);
+    dgesv6(G, b);
+    w[0] = w[0] + b[0];
+    w[2 * k - 1] = w[2 * k - 1] + b[1];
+    w[2 * k] = w[2 * k] + b[2];
+    w[2 * k + 3] = b[3];
+    w[2 * k + 4] = b[4];
+    w[-2 * k + 5] = w[-2 * k + 5] + b[5];
+    w[-2 * k + 6] = w[-2 * k + 6] - b[5];
+  }
+  for (int m = 0; m < 9; ++m) out[m] = 0.5 * w[m] - base[m];
+}
+
+// basic_F_restr_wgt (wavelet.f90:1176-1360) for coarse node (ip, jp) and
+// edge e (RT or UP), child (i0, j0) = (2ip, 2jp): the four weight triplets
+// in the order of id_enc (1..4); typ = the overlap types of the edge's child
+void flux_restr_weights(const Frame& C, const Frame& F, int ip, int jp, int e, const int typ[2], double out[4][3]) {
+  const double R = F.g.radius;
+  const int i0 = 2 * ip, j0 = 2 * jp;
+  int i, j;
+  if (e == 2) { i = i0; j = j0 + 1; }
+  else { i = i0 + 1; j = j0; }
+  const int so = HEX_S_OFFS[e];
+  int ije[16][3];
+  auto set = [&](int id, int bi, int bj, int col) {
+    ije[id][0] = bi + HEXS[col][0]; ije[id][1] = bj + HEXS[col][1]; ije[id][2] = HEXS[col][2];
+  };
+  set(UMZ, i, j, so + 1); set(UPZ, i, j, so + 4); set(WMP, i, j, so + 0);
+  set(VPP, i, j, so + 5); set(WPM, i, j, so + 3); set(VMM, i, j, so + 2);
+  const int mpi = i + NGHB[so + 0][0], mpj = j + NGHB[so + 0][1];
+  const int ppi = i + NGHB[so + 5][0], ppj = j + NGHB[so + 5][1];
+  const int pmi = i + NGHB[so + 3][0], pmj = j + NGHB[so + 3][1];
+  const int mmi = i + NGHB[so + 2][0], mmj = j + NGHB[so + 2][1];
+  set(VMP, mpi, mpj, so + 2); set(VMPP, mpi, mpj, so + 5); set(UZP, mpi, mpj, so + 4);
+  set(WPPP, ppi, ppj, so + 0); set(WPP, ppi, ppj, so + 3);
+  set(VPM, pmi, pmj, so + 5); set(VPMM, pmi, pmj, so + 2); set(UZM, pmi, pmj, so + 1);
+  set(WMMM, mmi, mmj, so + 3); set(WMM, mmi, mmj, so + 0);
+
+  // coords_to_row of basic_F_restr_wgt: edge (i, j, e) of the fine level
+  auto row = [&](const int* q, V3 x, V3 y, double r[6]) {
+    const int ee = q[2];
+    const V3 p1 = F.node(q[0] + ENDPT[ee][0][0], q[1] + ENDPT[ee][0][1]);
+    const V3 p2 = F.node(q[0] + ENDPT[ee][1][0], q[1] + ENDPT[ee][1][1]);
+    const Tri& t1 = ADJ[ee][0];
+    const Tri& t2 = ADJ[ee][1];
+    const double ped = dist(F.cc(q[0] + t1.di, q[1] + t1.dj, t1.t), F.cc(q[0] + t2.di, q[1] + t2.dj, t2.t), R);
+    row_d(mid_pt(p1, p2, R), sub(p2, p1), x, y, r);
+    for (int m = 0; m < 6; ++m) r[m] = r[m] * ped;
+  };
+  auto solve = [&](int k1, const int* lcsd, V3 endpt_o, const int st[9], double w[6]) {
+    const Tri& ta = ADJ[lcsd[2]][-k1 + 1];   // adj_tri(:, -k1+2, e)
+    const V3 c1 = F.cc(lcsd[0] + ta.di, lcsd[1] + ta.dj, ta.t);
+    const V3 m1 = F.mid(lcsd[0], lcsd[1], lcsd[2]);
+    const V3 x = normalize(sub(m1, c1), R);
+    const V3 y = normalize(cross(x, c1), R);
+    double G[6][6], r1[6], r2[6];
+    row(ije[st[0]], x, y, G[0]);
+    row(ije[st[1]], x, y, G[1]);
+    row(ije[st[2]], x, y, G[2]);
+    for (int c = 0; c < 3; ++c) {
+      row(ije[st[3 + 2 * c]], x, y, r1);
+      row(ije[st[4 + 2 * c]], x, y, r2);
+      for (int m = 0; m < 6; ++m) G[3 + c][m] = r1[m] - r2[m];
+    }
+    // coords_to_row_perp of (c1, endpt_o)
+    const V3 mm = mid_pt(c1, endpt_o, R);
+    row_d(mm, cross(sub(endpt_o, c1), mm), x, y, w);
+    const double d = dist(c1, endpt_o, R);
+    for (int m = 0; m < 6; ++m) w[m] = w[m] * d;
+    dgesv6(G, w);
+  };
+  auto permute = [&](double w[6]) {
+    double t[6];
+    for (int m = 0; m < 6; ++m) t[m] = w[m];
+    if (e == 0) { w[0] = t[1]; w[1] = t[2]; w[2] = t[0]; w[3] = t[4]; w[4] = t[5]; w[5] = t[3]; }
+    else { w[0] = t[2]; w[1] = t[0]; w[2] = t[1]; w[3] = t[5]; w[4] = t[3]; w[5] = t[4]; }
+  };
+  const double eps = R * 1e-13;
+  // side k = 1
+  {
+    const Tri& tp = ADJ[e][1];
+    const V3 cp = C.cc(ip + tp.di, jp + tp.dj, tp.t);
+    const int* q = ije[UZP];
+    const Tri& tq = ADJ[q[2]][0];
+    if (dist(cp, F.cc(q[0] + tq.di, q[1] + tq.dj, tq.t), R) < eps) {
+      for (int m = 0; m < 3; ++m) out[0][m] = out[1][m] = 0.0;
+    } else {
+      const int* lcsd = typ[1] == 1 ? ije[VPP] : typ[1] == 2 ? ije[WMP] : ije[UZP];
+      static const int st[9] = {VPP, WMP, UZP, UPZ, WPP, VMP, UMZ, WPPP, VMPP};
+      double w[6];
+      solve(1, lcsd, cp, st, w);
+      permute(w);
+      for (int m = 0; m < 3; ++m) { out[0][m] = w[m]; out[1][m] = w[3 + m]; }
+    }
+  }
+  // side k = 0
+  {
+    const Tri& tp = ADJ[e][0];
+    const V3 cp = C.cc(ip + tp.di, jp + tp.dj, tp.t);
+    const int* q = ije[UZM];
+    const Tri& tq = ADJ[q[2]][1];
+    if (dist(cp, F.cc(q[0] + tq.di, q[1] + tq.dj, tq.t), R) < eps) {
+      for (int m = 0; m < 3; ++m) out[2][m] = out[3][m] = 0.0;
+    } else {
+      const int* lcsd = typ[0] == 1 ? ije[VMM] : typ[0] == 2 ? ije[WPM] : ije[UZM];
+      static const int st[9] = {VMM, WPM, UZM, UMZ, WMM, VPM, UPZ, WMMM, VPMM};
+      double w[6];
+      solve(0, lcsd, cp, st, w);
+      permute(w);
+      for (int m = 0; m < 3; ++m) { out[2][m] = w[m]; out[3][m] = w[3 + m]; }
+    }
+  }
+}
+
+}  // namespace
+
+TransferWeights transfer_weights(const Level& fine, const Level& coarse) {
+  if (fine.n != 2 * coarse.n) throw std::runtime_error("transfer_weights: levels are not consecutive");
+  TransferWeights w;
+  const size_t NF = fine.ncells(), NCc = coarse.ncells();
+  w.nc = (int)NCc;
+  w.split.assign(2 * NF, 0.0);
+  w.iu.assign(9 * NF, 0.0);
+  w.rf.assign(24 * NCc, 0.0);
+  const int nc = coarse.n, nf = fine.n;
+  for (int lz = 0; lz < 10; ++lz) {
+    const Frame C{coarse, lz}, F{fine, lz};
+    for (int jp = -1; jp <= nc; ++jp)
+      for (int ip = -1; ip <= nc; ++ip)
+        for (int e = 0; e < 3; ++e) {
+          const int ic = 2 * ip + (e == 0 || e == 1 ? 1 : 0), jc = 2 * jp + (e == 1 || e == 2 ? 1 : 0);
+          if (ic < -HALO || jc < -HALO || ic >= nf + HALO || jc >= nf + HALO) continue;
+          double area[8];
+          int typ[2];
+          if (!overl_node(fine, coarse, lz, ip, jp, e, area, typ)) continue;
+          const size_t cf = fine.cell(lz, ic, jc);
+          w.split[cf] = area[4];
+          w.split[NF + cf] = area[5];
+          const bool owned = ip >= 0 && jp >= 0 && ip < nc && jp < nc;
+          if (!owned) continue;
+          // I_u_wgt of the outer child edge whose second end point is the
+          // child (set_WT_wgts, wavelet.f90:1585-1602)
+          const int i2 = 2 * ip + ENDPT[e][1][0], j2 = 2 * jp + ENDPT[e][1][1];
+          double iu[9];
+          outer_velo_weights(C, ip, jp, e, iu);
+          const size_t c2 = fine.cell(lz, i2, j2);
+          for (int m = 0; m < 9; ++m) w.iu[m * NF + c2] = iu[m];
+          if (e == 1) continue;
+          double rf[4][3];
+          flux_restr_weights(C, F, ip, jp, e, typ, rf);
+          const size_t cp = coarse.cell(lz, ip, jp);
+          const int s0 = e == 0 ? 0 : 2;   // RT: sets 0 (k=0), 1 (k=1); UP: 2, 3
+          // id_enc (3, 4) belong to side k = 0, (1, 2) to side k = 1
+          for (int m = 0; m < 3; ++m) {
+            w.rf[((s0 + 0) * 6 + m) * NCc + cp] = rf[2][m];
+            w.rf[((s0 + 0) * 6 + 3 + m) * NCc + cp] = rf[3][m];
+            w.rf[((s0 + 1) * 6 + m) * NCc + cp] = rf[0][m];
+            w.rf[((s0 + 1) * 6 + 3 + m) * NCc + cp] = rf[1][m];
+          }
+        }
+  }
+  // halo split parts: the owner's, unswapped (set_areas, refine_patch.f90:436-450)
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = -HALO; j < nf + HALO; ++j)
+      for (int i = -HALO; i < nf + HALO; ++i) {
+        if (i >= 0 && i < nf && j >= 0 && j < nf) continue;
+        const size_t c = fine.cell(lz, i, j);
+        if (c == fine.south_pole || c == fine.north_pole) continue;
+        const int64_t src = fine.node_src[c];
+        if (src < 0 || (i & 1) + (j & 1) == 0) {
+          w.split[c] = w.split[NF + c] = 0.0;
+          continue;
+        }
+        w.split[c] = w.split[(size_t)src];
+        w.split[NF + c] = w.split[NF + (size_t)src];
+      }
+  return w;
 }
 
 }  // namespace wtg

@@ -40,6 +40,14 @@ struct Level {
   // kept for the inter-level geometry of the wavelet transforms
   int EH = 0, ES = 0;
   std::vector<double> xe, cle, cue;  // [10*ES*ES][3]
+  // owned cell of every lattice point within WH cells of the owned square
+  // (-1 where the lattice does not continue), frame [-WH, n+WH)^2
+  int WH = 0, WS = 0;
+  std::vector<int64_t> wide_src;
+  size_t wcell(int lz, int i, int j) const {
+    return (size_t)lz * WS * WS + (size_t)(j + WH) * WS + (size_t)(i + WH);
+  }
+  std::vector<double> mpe;           // edge midpoints [10*ES*ES][RT|DG|UP][3] (cells [-1, n+1])
   size_t ecell(int lz, int i, int j) const {
     return (size_t)lz * ES * ES + (size_t)(j + EH) * ES + (size_t)(i + EH);
   }
@@ -58,5 +66,27 @@ std::vector<double> overlap_areas(const Level& fine, const Level& coarse);
 // (sub-domain corners first) and, through PATCH_SIZE = 2^(MIN_LEVEL -
 // DOMAIN_LEVEL - 1), one ghost-list convention beside the south pole.
 Level build_level(int level, int min_level, int domain_level, double radius, double omega);
+
+// get_overl_areas for the child on edge e of coarse node (ip, jp) of lozenge
+// lz: area[0..7] = a(1..4) and the split parts area(5..8), typ[0..1] =
+// INSIDE 0 / OUTER1 1 / OUTER2 2 of the two sides.  False when the stencil
+// leaves the extended frames.
+bool overl_node(const Level& fine, const Level& coarse, int lz, int ip, int jp, int e, double area[8], int typ[2]);
+
+// Inter-level weights of the velocity and flux transforms, computed on the
+// host as the reference computes them at refinement (refine_patch.f90:
+// 250-257, wavelet.f90:1176-1300, 1512-1705), with the 6 x 6 systems solved
+// in LAPACK DGETF2/DGETRS order (dgesv.f90).
+struct TransferWeights {
+  int nc = 0;                  // coarse cells
+  std::vector<double> split;   // [2][fine ncells]: overlap split parts a(5..6), halos as the owner's
+  std::vector<double> iu;      // [9][fine ncells]: I_u_wgt of the outer child edges (owned fine cells)
+  // R_F_wgt per coarse node and side: [4][6][coarse ncells], sets
+  // 0 = RT k=0 (fine (2i+1, 2j-2), (2i+1, 2j-1)), 1 = RT k=1 ((2i, 2j), (2i, 2j+1)),
+  // 2 = UP k=0 ((2i+1, 2j), (2i+1, 2j+1)), 3 = UP k=1 ((2i-2, 2j), (2i-2, 2j+1)),
+  // each two triplets
+  std::vector<double> rf;
+};
+TransferWeights transfer_weights(const Level& fine, const Level& coarse);
 
 }  // namespace wtg

@@ -64,6 +64,12 @@ struct StageArgs {
   int trend_only;              // 1: out = trend
   Geo g;
   double grav, rho0, p_top;
+  // optional copies of the intermediate quantities that the coarser levels
+  // of a hierarchy restrict (null: not kept); FP64 pair kernel only
+  double *k_hm, *k_ht;     // [L][3][NC] mass / temperature fluxes, region cells [-1, n]^2
+  double *k_bern, *k_exn;  // [L][NC] Bernoulli, Exner on [0, n]^2
+  double *k_src;           // [L][3][NC] velocity source (-Qperp) on owned edges
+  double *k_dm, *k_dt;     // [L][NC] scalar trends on owned cells and the pole masters
 };
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
@@ -964,6 +970,10 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
                     gs.hinv[x] / 4.0;
         sl.bern[x] = ke + phi_k + pr / (a.rho0 * 1.0);
         sl.exn[x] = -phi_k;
+        if (a.k_bern) {
+          sto(a.k_bern + ks, xb, sl.bern[x]);
+          sto(a.k_exn + ks, xb, sl.exn[x]);
+        }
       }
       if (tri_cell) {
         const int xE = x + 1, xN = x + R, xNE = x + R + 1;
@@ -982,6 +992,12 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
         sl.ht[0][x] = u0 * f0 * interp(t, sl.t[xE]) + 0.0;
         sl.ht[1][x] = u1 * f1 * interp(t, sl.t[xNE]) + 0.0;
         sl.ht[2][x] = u2 * f2 * interp(t, sl.t[xN]) + 0.0;
+        if (a.k_hm) {
+          for (int e = 0; e < 3; ++e) {
+            sto(a.k_hm + ke3 + e * NC, xb, sl.hm[e][x]);
+            sto(a.k_ht + ke3 + e * NC, xb, sl.ht[e][x]);
+          }
+        }
       }
     }
     __syncthreads();
@@ -1024,6 +1040,10 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       double dmv = -divm, dtv = -divt;
       sto(a.om + ks, ob, a.trend_only ? dmv : bm + a.h * dmv);
       sto(a.ot + ks, ob, a.trend_only ? dtv : bt + a.h * dtv);
+      if (a.k_dm) {
+        sto(a.k_dm + ks, ob, dmv);
+        sto(a.k_dt + ks, ob, dtv);
+      }
       double qp[3];
       weights_lds(gs, y, 0, w[0]);
       weights_lds(gs, y, 1, w[1]);
@@ -1061,6 +1081,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
         double src = -qp[e] + 0.0 * ll[e];
         double du = src / ll[e] - gB[e] - the[e] * gX[e];
         sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
+        if (a.k_src) sto(a.k_src + ke3 + e * NC, ob, src);
       }
     }
     if (!(lay && own)) {
@@ -1076,6 +1097,10 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       double dmv = -divm, dtv = -divt;
       sto(a.om + ks, xb, a.trend_only ? dmv : ldo(a.qm + ks, xb) + a.h * dmv);
       sto(a.ot + ks, xb, a.trend_only ? dtv : ldo(a.qt + ks, xb) + a.h * dtv);
+      if (a.k_dm) {
+        sto(a.k_dm + ks, xb, dmv);
+        sto(a.k_dt + ks, xb, dtv);
+      }
     }
     __syncthreads();
     // --- 5. stage the group's next layer --------------------------------------
@@ -2093,8 +2118,19 @@ int launch_copy_frozen(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slo
 }
 
 int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, const wtg_ctx::Slot& out, double h,
-                 int trend_only) {
+                 int trend_only, const WtgStageKeep* keep = nullptr) {
   StageArgs a{};
+  if (keep) {
+    if (c->stage_version != 3 || c->nmembers != 1)
+      return fail(c, WTG_E_STATE, "kept stage quantities need the layer-pair kernel on a single-member context");
+    a.k_hm = keep->hm;
+    a.k_ht = keep->ht;
+    a.k_bern = keep->bern;
+    a.k_exn = keep->exn;
+    a.k_src = keep->src;
+    a.k_dm = keep->dm;
+    a.k_dt = keep->dt;
+  }
   a.n = c->info.n;
   a.S = c->info.S;
   a.L = c->params.nlayers;
@@ -2443,6 +2479,27 @@ int bicgstab_solve(void* user, wtg_ctx* c, int* iters) {
 
 }  // namespace
 
+int wtg_launch_stage_keep(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, const wtg_ctx::Slot& out,
+                          double h, int trend_only, const WtgStageKeep* keep) {
+  return launch_stage(c, y, q, out, h, trend_only, keep);
+}
+
+int wtg_ghost_fill_layers(wtg_ctx* c, int L, double* m, double* t, double* u) {
+  const int nn = m ? c->nn : 0, ne = u ? c->ne : 0;
+  const int64_t total = (int64_t)(nn + ne) * ((L + GF_KC - 1) / GF_KC);
+  if (total == 0) return 0;
+  const int threads = 256;
+  const int64_t blocks = (total + threads - 1) / threads;
+  hipLaunchKernelGGL(ghost_fill_kernel<double>, dim3((unsigned)blocks), dim3(threads), 0, c->stream, L, c->info.ncells,
+                     c->d_nd, nn, c->d_ed, c->d_es, ne, m, t, u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int wtg_copy_frozen_slots(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to) {
+  return launch_copy_frozen(c, from, to);
+}
+
 int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters) {
   if (int rc = check_slot(c, slot)) return rc;
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
@@ -2592,6 +2649,7 @@ int wtg_grid_get(const wtg_grid* grid, const char* name, void* out, size_t bytes
   else if (s == "edge_src") { src = g.edge_src.data(); n = g.edge_src.size() * 8; }
   else if (s == "edge_sign") { src = g.edge_sign.data(); n = g.edge_sign.size(); }
   else if (s == "red_weight") { src = g.red_weight.data(); n = g.red_weight.size() * 8; }
+  else if (s == "wide_src") { src = g.wide_src.data(); n = g.wide_src.size() * 8; }
   else return fail(nullptr, WTG_E_ARG, "unknown grid array '%s'", name);
   if (bytes != n) return fail(nullptr, WTG_E_ARG, "grid array '%s' has %zu bytes, %zu given", name, n, bytes);
   std::memcpy(out, src, n);
@@ -2606,6 +2664,26 @@ int wtg_grid_overlap_areas(const wtg_grid* fine, const wtg_grid* coarse, double*
   try {
     std::vector<double> a = wtg::overlap_areas(fine->lv, coarse->lv);
     std::memcpy(out, a.data(), n);
+  } catch (const std::exception& e) {
+    return fail(nullptr, WTG_E_ARG, "%s", e.what());
+  }
+  return 0;
+}
+
+int wtg_grid_transfer_weights(const wtg_grid* fine, const wtg_grid* coarse, const char* name, double* out,
+                              size_t bytes) {
+  if (!fine || !coarse || !name || !out) return fail(nullptr, WTG_E_ARG, "null argument");
+  if (fine->lv.level != coarse->lv.level + 1) return fail(nullptr, WTG_E_ARG, "grids are not consecutive levels");
+  try {
+    wtg::TransferWeights w = wtg::transfer_weights(fine->lv, coarse->lv);
+    const std::vector<double>* v = nullptr;
+    if (!std::strcmp(name, "split")) v = &w.split;
+    else if (!std::strcmp(name, "iu")) v = &w.iu;
+    else if (!std::strcmp(name, "rf")) v = &w.rf;
+    else return fail(nullptr, WTG_E_ARG, "unknown transfer weights '%s'", name);
+    const size_t n = v->size() * sizeof(double);
+    if (bytes != n) return fail(nullptr, WTG_E_ARG, "transfer weights '%s' have %zu bytes, %zu given", name, n, bytes);
+    std::memcpy(out, v->data(), n);
   } catch (const std::exception& e) {
     return fail(nullptr, WTG_E_ARG, "%s", e.what());
   }
