@@ -411,7 +411,9 @@ int wtg_ml_rk4(wtg_ml* ml, int slot, double dt, int nsteps);
  * row-major within a patch. */
 int wtg_mg_npatches(const wtg_mg* mg);
 /* Attaches (NULL detaches) the network of WTG_SOLVE_LEARNED; it must take a
- * batch of one row per patch.  The hierarchy does not own it. */
+ * batch of one row per patch.  The hierarchy does not own it: the network
+ * must outlive the attachment (detach it, or destroy the hierarchy, before
+ * wtg_mlp_destroy). */
 typedef struct wtg_mlp wtg_mlp;
 int wtg_mg_set_learned(wtg_mg* mg, wtg_mlp* mlp);
 /* The network's input rows for the finest level's field `field`:

@@ -115,6 +115,8 @@ struct wtg_ctx {
   const double* bicg_u = nullptr;
   const double* bicg_f = nullptr;
   double bicg_key[2] = {0.0, 0.0};  // operator coefficient and gravity of the captured graph
+  const double* bicg_p = nullptr;   // buffers holding p and Ap when the graph was captured
+  const double* bicg_Ap = nullptr;
   std::vector<void*> allocs;
   // ensembles: one single-member view per member for the free surface of
   // the split-explicit step (state, topography and free-surface fields at

@@ -921,8 +921,13 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
     if (c->fs_graph && k == BICG_CHUNK && M == 1) {
       // the operator's coefficients are kernel arguments of the graph
       const double key[2] = {fs_coef(c), c->params.grav};
+      // the buffers that hold p and Ap at the chunk's start are part of the
+      // key: after a chunk of odd length (a first chunk sized from the
+      // previous solve) they are the other pair
       if (!c->bicg_graph || c->bicg_u != u || c->bicg_f != f || c->bicg_key[0] != key[0] ||
-          c->bicg_key[1] != key[1]) {
+          c->bicg_key[1] != key[1] || c->bicg_p != p || c->bicg_Ap != Ap) {
+        const double* p0 = p;
+        const double* Ap0 = Ap;
         if (c->bicg_graph) {
           hipGraphExecDestroy(c->bicg_graph);
           c->bicg_graph = nullptr;
@@ -946,6 +951,8 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
         c->bicg_f = f;
         c->bicg_key[0] = key[0];
         c->bicg_key[1] = key[1];
+        c->bicg_p = p0;
+        c->bicg_Ap = Ap0;
       }
       WTG_HIPCHK(c, hipGraphLaunch(c->bicg_graph, c->stream));
     } else {
@@ -1690,13 +1697,18 @@ void wtg_mg_destroy(wtg_mg* m) {
   delete m;
 }
 
+static int mg_check_params(wtg_ctx* top, const wtg_mg_params* p) {
+  if (p->coarse_iter < 0 || p->max_vcycle < 0 || p->pre_iter < 0 || p->down_iter < 0 || p->up_iter < 0 ||
+      p->post_iter < 0 || p->max_srj_iter < 0 || p->learned_cycles < 0)
+    return wtg_fail(top, WTG_E_ARG, "negative iteration count");
+  return 0;
+}
+
 int wtg_mg_solve(wtg_mg* m, int method, const wtg_mg_params* p, int* iters, double* rel_resid) {
   if (!m) return WTG_E_ARG;
   wtg_ctx* top = m->lv[m->nl - 1];
   if (!p || !iters || !rel_resid) return wtg_fail(top, WTG_E_ARG, "null argument");
-  if (p->coarse_iter < 0 || p->max_vcycle < 0 || p->pre_iter < 0 || p->down_iter < 0 || p->up_iter < 0 ||
-      p->post_iter < 0 || p->max_srj_iter < 0)
-    return wtg_fail(top, WTG_E_ARG, "negative iteration count");
+  if (int rc = mg_check_params(top, p)) return rc;
   if (method == WTG_SOLVE_FMG) return mg_fmg(m, *p, iters, rel_resid);
   if (method == WTG_SOLVE_SRJ) return mg_srj(m, *p, iters, rel_resid);
   if (method == WTG_SOLVE_VCYCLE) return mg_cycles(m, *p, false, iters, rel_resid);
@@ -1799,6 +1811,7 @@ int wtg_mg_rk4_split(wtg_mg* m, int slot, int nsteps, int method, const wtg_mg_p
   if (!m) return WTG_E_ARG;
   wtg_ctx* top = m->lv[m->nl - 1];
   if (!p || !iters) return wtg_fail(top, WTG_E_ARG, "null argument");
+  if (int rc = mg_check_params(top, p)) return rc;
   if (method != WTG_SOLVE_VCYCLE && method != WTG_SOLVE_LEARNED)
     return wtg_fail(top, WTG_E_ARG, "split steps on a hierarchy take WTG_SOLVE_VCYCLE or WTG_SOLVE_LEARNED, not %d",
                     method);

@@ -2883,7 +2883,9 @@ void wtg_destroy(wtg_ctx* c) {
   for (wtg_ctx* v : c->views) delete v;  // views own no device memory
   for (void* p : c->allocs) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
-  if (c->stream) hipStreamDestroy(c->stream);
+  // the context's own stream (a hierarchy may have pointed c->stream at its
+  // finest level's, which that level destroys)
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
 }
 
