@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-free-surface", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-multilevel", action="store_true")
     ap.add_argument("--members", type=int, default=16, help="ensemble members of the batched-ensemble line")
     ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
     ap.add_argument("--only-free-surface", action="store_true",
@@ -142,6 +143,7 @@ def main():
     ctx.synchronize()
     split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
     fs = None if args.no_free_surface else free_surface(args.fs_level)
+    ml = None if args.no_multilevel else multilevel()
 
     out = {
         "metric": METRIC,
@@ -180,6 +182,7 @@ def main():
         "ensemble": ens,
         "split_step": split,
         "free_surface": fs,
+        "multilevel": ml,
     }
     print(json.dumps(out))
 
@@ -453,6 +456,71 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
             "note": "host wall clock over 5 steps after one warm-up step from the same state; BiCGSTAB queues 8 "
                     "gated iterations per host convergence test; FP32-bulk: FP32 trend stencils, FP64 stage "
                     "states, AXPYs and free surface"}
+
+
+def multilevel(cases_=((5, 7, 20), (5, 8, 30)), dt=300.0, nsteps=3):
+    """The multi-level step on a full hierarchy (every node active, as the
+    reference runs a uniform hierarchy with zero thresholds): explicit RK4
+    with trend_ml and WT_after_step over levels, and RK4_split with the FMG
+    free surface to 1e-9 on every level -- config 2's J5..J7 L20 ocean and
+    config 5's member shape J5..J8 L30.  Host wall clock over nsteps steps
+    after one warm-up step; DOF counted on the finest level and on all."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+
+    out = {}
+    for lo, hi, L in cases_:
+        print(f"bench: multi-level steps J{lo}..J{hi} L{L}", file=sys.stderr, flush=True)
+        grids = [W.Grid(level=l, min_level=lo, domain_level=2) for l in range(lo, hi + 1)]
+        ctxs, st, topos = [], [], []
+        for g in grids:
+            topo, m, t, u = cases.ocean(g, L, seed=21)
+            c = W.Context(g, nlayers=L, nslots=2)
+            c.set_topography(topo)
+            c.fs_set_params(dt, 0.6, 0.8)
+            ctxs.append(c)
+            st.append((m, t, u))
+            topos.append(topo)
+        h = W.Hierarchy(ctxs)
+        ml = W.MultiLevel(ctxs)
+        ml.upload(0, [s[0] for s in st], [s[1] for s in st], [s[2] for s in st])
+        ml.forward(0)
+        start = ml.download(0)
+        dof_fine = L * grids[-1].layout.n_dof_per_layer
+        dof_all = sum(L * g.layout.n_dof_per_layer for g in grids)
+        # explicit RK4 over the hierarchy
+        ml.upload(0, *start)
+        ml.rk4(0, dt, 1)
+        ctxs[-1].synchronize()
+        t0 = time.perf_counter()
+        ml.rk4(0, dt, nsteps)
+        ms_rk4 = (time.perf_counter() - t0) * 1e3 / nsteps
+        # RK4_split with FMG to 1e-9
+        p = W.MgParams.defaults(coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400, max_vcycle=50)
+        ml.upload(0, *start)
+        for i, c in enumerate(ctxs):
+            c.fs_upload(W.FS_ETA, start[0][i].sum(0) / 1028.0 + topos[i])
+        ml.rk4_split(h, 0, 1, p)
+        ctxs[-1].synchronize()
+        t0 = time.perf_counter()
+        its = [ml.rk4_split(h, 0, 1, p) for _ in range(nsteps)]
+        ms_split = (time.perf_counter() - t0) * 1e3 / nsteps
+        m, _, u = ml.download(0)
+        finite = bool(all(np.isfinite(a).all() for a in m) and all(np.isfinite(a).all() for a in u))
+        ml.close()
+        h.close()
+        for c in ctxs:
+            c.close()
+        out[f"J{lo}..J{hi}_L{L}"] = {
+            "dof_per_step_finest": dof_fine, "dof_per_step_all_levels": dof_all,
+            "rk4": {"ms_per_step": ms_rk4, "dof_updates_per_s_finest": dof_fine / (ms_rk4 / 1e3)},
+            "rk4_split_fmg": {"ms_per_step": ms_split, "dof_updates_per_s_finest": dof_fine / (ms_split / 1e3),
+                              "fmg_vcycles_summed_over_levels": [int(i) for i in its], "finite": finite}}
+    out["note"] = (f"dt {dt} s, theta1 0.6, theta2 0.8, FMG to 1e-9 on every level (the reference's parameters "
+                   f"otherwise), FP64; host wall clock over {nsteps} steps after one warm-up step")
+    return out
 
 
 def free_surface(fine: int):

@@ -284,6 +284,11 @@ int wtg_fs_set_graph(wtg_ctx* ctx, int enable);
  * each member stopping at its own convergence test); *iters = the most any
  * member took in the last step. */
 int wtg_rk4_split(wtg_ctx* ctx, int slot, int nsteps, double rel_tol, int max_iter, int* iters);
+/* The same with nstages stages of RK_split: 4 = RK4_split, 3 = RK3_split
+ * (time_integr.f90:384-405), 2 = RK2_split (:407-427), 1 = Euler_split
+ * (:429-441); stage coefficients dt/nstages, ..., dt/2, dt as the
+ * reference's low-storage schemes. */
+int wtg_rk_split(wtg_ctx* ctx, int slot, int nstages, int nsteps, double rel_tol, int max_iter, int* iters);
 
 /* ---- multi-level free surface: scalar wavelet transforms, FMG, SRJ -------
  * A hierarchy is a list of free-surface contexts on consecutive uniform
@@ -405,6 +410,20 @@ int wtg_ml_wt_after_step(wtg_ml* ml, int slot);
  * RK_sub_step on every level, WT_after_step.  Uses each context's two
  * stage buffers. */
 int wtg_ml_rk4(wtg_ml* ml, int slot, double dt, int nsteps);
+/* RK4_split (time_integr.f90:360-382, 444-478) on the hierarchy, the
+ * reference's mode-split step of a multi-level ocean: per stage
+ * barotropic_correction on every level, trend_ml, scalar_star / u_star with
+ * each level's external pressure gradient (coarse free surfaces sub-sampled,
+ * cpt_or_restr_eta), WT_after_step; then flux_divergence over levels (the
+ * layer-integrated flux restricted by flux_cpt_restr), rhs_elliptic on every
+ * level, the FMG free-surface solve on mg (the same contexts, wtg_mg_create,
+ * with wtg_fs_set_params on every level; WTG_FS_ETA of each level is its free
+ * surface), barotropic_correction, u_update and WT_after_step including the
+ * free surface.  *iters: FMG V-cycles summed over levels and steps. */
+int wtg_ml_rk4_split(wtg_ml* ml, wtg_mg* mg, int slot, int nsteps, const wtg_mg_params* p, int* iters);
+/* RK3_split, RK2_split, Euler_split on the hierarchy (nstages 3, 2, 1; 4 is
+ * wtg_ml_rk4_split) */
+int wtg_ml_rk_split(wtg_ml* ml, wtg_mg* mg, int slot, int nstages, int nsteps, const wtg_mg_params* p, int* iters);
 
 /* The finest level's 8x8 patches of owned nodes (n a multiple of 8): 10
  * (n/8)^2 of them, lozenge-major, then patch row, then patch column; nodes

@@ -183,6 +183,11 @@ int wto_bicgstab(const wto_grid* g, const wto_fs_params* p, const double* eta_ol
 }
 
 void wto_flux_divergence(const wto_grid* g, int L, const double* mass, const double* velo, double* divF) {
+  wto_flux_divergence_parts(g, L, mass, velo, divF, NULL);
+}
+
+void wto_flux_divergence_parts(const wto_grid* g, int L, const double* mass, const double* velo, double* divF,
+                               double* flux_out) {
   const int64_t NC = g->ncells;
   const int S = g->S;
   double* lenf = (double*)malloc(3 * NC * sizeof(double));
@@ -218,6 +223,7 @@ void wto_flux_divergence(const wto_grid* g, int L, const double* mass, const dou
               g->hex_inv[c];
   }
   wto_node_ghost(g, divF);
+  if (flux_out) memcpy(flux_out, flux, 3 * NC * sizeof(double));
   free(lenf);
   free(pedf);
   free(flux);

@@ -26,6 +26,10 @@ int wto_bicgstab(const wto_grid* g, const wto_fs_params* p, const double* eta_ol
                  const double* f, double nrm_f, double tol, int maxit, double* err_out);
 /* divergence of the layer-integrated mass flux sum_k u_k interp(mass_k) pedlen */
 void wto_flux_divergence(const wto_grid* g, int L, const double* mass, const double* velo, double* divF);
+/* the same, also returning the layer-integrated edge fluxes [3][ncells] of
+ * the four cells around every master (0 elsewhere) */
+void wto_flux_divergence_parts(const wto_grid* g, int L, const double* mass, const double* velo, double* divF,
+                               double* flux_out);
 /* rhs = -eta + dt (theta2 divF + (1 - theta2) divF_old) / rho0 */
 void wto_rhs_elliptic(const wto_grid* g, const wto_fs_params* p, double rho0, const double* eta, const double* divF,
                       const double* divF_old, double* rhs);

@@ -34,6 +34,7 @@
 
 #include "elliptic_oracle.h"
 #include "mg_oracle.h"
+#include "split_oracle.h"
 
 #define H 2
 
@@ -635,4 +636,161 @@ void wto_rk4_ml(const wto_ml* h, int L, double dt, int nsteps, double** mass, do
     free(dtm[l]);
     free(du[l]);
   }
+}
+
+/* ---------------------------------------------------------------------------
+ * RK4_split on the hierarchy (time_integr.f90:360-382, 444-478)
+ * ------------------------------------------------------------------------- */
+/* flux_divergence (barotropic_2d.f90:326-366) on every level: the finest
+ * level's vertically integrated flux (step1 itype 4) and its divergence,
+ * coarser levels from the restricted flux (cpt_or_restr_flux with dscalar =
+ * the finer divergence); divF[l] ghost filled */
+static void flux_divergence_ml(const wto_ml* h, int L, double* const* mass, double* const* velo, double** divF) {
+  const int top = h->nlevels - 1;
+  const wto_grid* gt = h->grid[top];
+  const int64_t NT = gt->ncells;
+  double* hf = calloc(3 * NT, sizeof(double));
+  wto_flux_divergence_parts(gt, L, mass[top], velo[top], divF[top], hf);
+  edge_ghost(gt, hf);
+  double* fine = hf;
+  for (int l = top - 1; l >= 0; --l) {
+    const wto_grid* gc = h->grid[l];
+    double* hc = calloc(3 * gc->ncells, sizeof(double));
+    wto_flux_restrict(h, l + 1, fine, divF[l + 1], hc);
+    memset(divF[l], 0, sizeof(double) * gc->ncells);
+    divergence(gc, hc, divF[l]);
+    /* cal_div is +div: undo the scalar trend's sign */
+    for (int64_t c = 0; c < gc->ncells; ++c) divF[l][c] = -divF[l][c];
+    wto_node_ghost(gc, divF[l]);
+    free(fine);
+    fine = hc;
+  }
+  free(fine);
+}
+
+/* grad_eta (barotropic_2d.f90:457-474) on every level: eta copied, its
+ * coarse values on [0, n]^2 replaced by the finer level's at the coincident
+ * node (cpt_or_restr_eta), then g grad on the level's edges */
+static void grad_eta_ml(const wto_ml* h, double* const* eta, double** hf) {
+  const int top = h->nlevels - 1;
+  double* e2[16];
+  for (int l = top; l >= 0; --l) {
+    const wto_grid* g = h->grid[l];
+    e2[l] = malloc(sizeof(double) * g->ncells);
+    memcpy(e2[l], eta[l], sizeof(double) * g->ncells);
+    if (l < top) {
+      const wto_grid* gf = h->grid[l + 1];
+      for (int lz = 0; lz < 10; ++lz)
+        for (int j = 0; j <= g->n; ++j)
+          for (int i = 0; i <= g->n; ++i) e2[l][cell_of(g, lz, i, j)] = e2[l + 1][cell_of(gf, lz, 2 * i, 2 * j)];
+    }
+    wto_grad_eta(g, e2[l], hf[l]);
+  }
+  for (int l = 0; l <= top; ++l) free(e2[l]);
+}
+
+int wto_rk_split_ml(const wto_ml* h, wto_mg* mg, int L, int nstages, double** mass, double** temp, double** velo,
+                    double** eta, int* fmg_iters) {
+  const int nl = h->nlevels;
+  const double dt = mg->p.dt, th1 = mg->p.theta1, th2 = mg->p.theta2;
+  double *m1[16], *t1[16], *u1[16], *dm[16], *dtm[16], *du[16], *hf[16], *dF[16], *dFold[16], *rhs[16], *eo[16];
+  for (int l = 0; l < nl; ++l) {
+    const wto_grid* g = h->grid[l];
+    const int64_t NC = g->ncells;
+    m1[l] = malloc(8 * NC * L);
+    t1[l] = malloc(8 * NC * L);
+    u1[l] = malloc(8 * 3 * NC * L);
+    dm[l] = malloc(8 * NC * L);
+    dtm[l] = malloc(8 * NC * L);
+    du[l] = malloc(8 * 3 * NC * L);
+    hf[l] = calloc(3 * NC, 8);
+    dF[l] = calloc(NC, 8);
+    dFold[l] = calloc(NC, 8);
+    rhs[l] = calloc(NC, 8);
+    eo[l] = malloc(8 * NC);
+    wto_ghost_fill(g, L, mass[l], temp[l], velo[l]);
+    wto_node_ghost(g, eta[l]);
+  }
+  if (th2 != 1.0) flux_divergence_ml(h, L, mass, velo, dFold);
+  for (int l = 0; l < nl; ++l) {
+    const int64_t NC = h->grid[l]->ncells;
+    memcpy(m1[l], mass[l], 8 * NC * L);
+    memcpy(t1[l], temp[l], 8 * NC * L);
+    memcpy(u1[l], velo[l], 8 * 3 * NC * L);
+  }
+  grad_eta_ml(h, eta, hf); /* eta is fixed through the stages */
+  for (int st = 0; st < nstages; ++st) {
+    double** ym = st == 0 ? mass : m1;
+    double** yt = st == 0 ? temp : t1;
+    double** yu = st == 0 ? velo : u1;
+    for (int l = 0; l < nl; ++l) wto_barotropic_correction(h->grid[l], L, eta[l], ym[l], yt[l]);
+    wto_trend_ml(h, L, ym, yt, yu, dm, dtm, du);
+    const int last = st == nstages - 1;
+    double** om = last ? mass : m1;
+    double** ot = last ? temp : t1;
+    double** ou = last ? velo : u1;
+    const double hh = dt / (double)(nstages - st); /* h/4, h/3, h/2, h */
+    for (int l = 0; l < nl; ++l) {
+      const wto_grid* g = h->grid[l];
+      const int64_t NC = g->ncells;
+      for (int k = 0; k < L; ++k)
+        for (int64_t c = 0; c < NC; ++c) {
+          if (!prognostic(g, c)) continue;
+          const int64_t x = (int64_t)k * NC + c;
+          om[l][x] = mass[l][x] + hh * dm[l][x];
+          ot[l][x] = temp[l][x] + hh * dtm[l][x];
+          for (int e = 0; e < 3; ++e) {
+            const int64_t y = ((int64_t)k * 3 + e) * NC + c;
+            ou[l][y] = velo[l][y] + hh * (du[l][y] + th1 * hf[l][e * NC + c]);
+          }
+        }
+    }
+    wto_wt_after_step(h, L, om, ot, ou);
+  }
+  /* free_surface_update: eta_update (rhs_elliptic, FMG), barotropic
+   * correction, u_update, WT_after_step with the free surface */
+  flux_divergence_ml(h, L, mass, velo, dF);
+  for (int l = 0; l < nl; ++l) {
+    wto_rhs_elliptic(h->grid[l], &mg->p, h->grid[l]->rho0, eta[l], dF[l], dFold[l], rhs[l]);
+    memcpy(eo[l], eta[l], 8 * h->grid[l]->ncells);
+    mg->eta_old[l] = eo[l];
+  }
+  double res[16];
+  wto_fmg(mg, eta, rhs, fmg_iters, res);
+  for (int l = 0; l < nl; ++l) wto_barotropic_correction(h->grid[l], L, eta[l], mass[l], temp[l]);
+  grad_eta_ml(h, eta, hf);
+  for (int l = 0; l < nl; ++l) {
+    const wto_grid* g = h->grid[l];
+    const int64_t NC = g->ncells;
+    for (int k = 0; k < L; ++k)
+      for (int64_t c = 0; c < NC; ++c) {
+        if (!prognostic(g, c)) continue;
+        for (int e = 0; e < 3; ++e) {
+          const int64_t y = ((int64_t)k * 3 + e) * NC + c;
+          velo[l][y] = velo[l][y] - th1 * dt * hf[l][e * NC + c];
+        }
+      }
+  }
+  /* the free surface's own transform (the extra layer of WT_after_step) */
+  double* wce[16];
+  for (int l = 1; l < nl; ++l) {
+    wce[l] = calloc(h->grid[l]->ncells, 8);
+    wto_node_ghost(h->grid[l], eta[l]);
+    wto_scalar_wavelets(h->grid[l], h->ovl[l], eta[l], wce[l]);
+  }
+  wto_wt_after_step(h, L, mass, temp, velo);
+  for (int l = 1; l < nl; ++l) {
+    wto_prolong(h->grid[l], h->grid[l - 1], h->ovl[l], eta[l - 1], wce[l], eta[l]);
+    free(wce[l]);
+  }
+  for (int l = 0; l < nl; ++l) {
+    free(m1[l]); free(t1[l]); free(u1[l]); free(dm[l]); free(dtm[l]); free(du[l]);
+    free(hf[l]); free(dF[l]); free(dFold[l]); free(rhs[l]); free(eo[l]);
+  }
+  return 0;
+}
+
+int wto_rk4_split_ml(const wto_ml* h, wto_mg* mg, int L, double** mass, double** temp, double** velo,
+                     double** eta, int* fmg_iters) {
+  return wto_rk_split_ml(h, mg, L, 4, mass, temp, velo, eta, fmg_iters);
 }

@@ -2,6 +2,7 @@
  * levels (see ml_oracle.c).  TEST INFRASTRUCTURE ONLY. */
 #ifndef WTG_ML_ORACLE_H
 #define WTG_ML_ORACLE_H
+#include "mg_oracle.h"
 #include "trisk_oracle.h"
 
 #ifdef __cplusplus
@@ -42,6 +43,19 @@ void wto_trend_ml(const wto_ml* h, int L, double* const* mass, double* const* te
 void wto_wt_after_step(const wto_ml* h, int L, double** mass, double** temp, double** velo);
 /* nsteps RK4 steps (time_integr.f90:78-103) on the hierarchy, in place */
 void wto_rk4_ml(const wto_ml* h, int L, double dt, int nsteps, double** mass, double** temp, double** velo);
+
+/* one RK4_split step (time_integr.f90:360-382, 444-478) on the hierarchy:
+ * per stage barotropic_correction on every level, trend_ml, scalar_star and
+ * u_star with the external pressure gradient of every level (coarse eta
+ * sub-sampled), WT_after_step; then flux_divergence over levels, rhs_elliptic,
+ * FMG (mg: the same levels, its dt/theta/solver parameters; its eta_old
+ * pointers are set here), barotropic_correction, u_update and WT_after_step
+ * with the free surface.  eta[l] per level; fmg_iters[l] = FMG's counts. */
+int wto_rk4_split_ml(const wto_ml* h, wto_mg* mg, int L, double** mass, double** temp, double** velo,
+                     double** eta, int* fmg_iters);
+/* the same with nstages = 1..4 (Euler_split, RK2_split, RK3_split, RK4_split) */
+int wto_rk_split_ml(const wto_ml* h, wto_mg* mg, int L, int nstages, double** mass, double** temp, double** velo,
+                    double** eta, int* fmg_iters);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,8 @@ def load():
         _lib.wto_grad_eta.argtypes = [C.c_void_p] * 3
         _lib.wto_rk4_split.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
         _lib.wto_rk4_split.restype = C.c_int
+        _lib.wto_rk_split.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
+        _lib.wto_rk_split.restype = C.c_int
     return _lib
 
 
@@ -200,12 +202,14 @@ class EllipticOracle(Oracle):
         self.lib.wto_grad_eta(C.byref(self.g), eta.ctypes.data, out.ctypes.data)
         return out
 
-    def rk4_split(self, mass, temp, velo, eta, tol=1e-13, maxit=400):
+    def rk4_split(self, mass, temp, velo, eta, tol=1e-13, maxit=400, nstages=4):
         """one RK4_split step of size dt (the FsParams dt); returns the new
-        (mass, temp, velo, eta) and the BiCGSTAB iteration count"""
+        (mass, temp, velo, eta) and the BiCGSTAB iteration count.  nstages 3,
+        2, 1: RK3_split, RK2_split, Euler_split"""
         mass, temp, velo, eta = (_f64(a).copy() for a in (mass, temp, velo, eta))
-        it = self.lib.wto_rk4_split(C.byref(self.g), C.byref(self.fs), self.weight.ctypes.data, mass.shape[0], tol,
-                                    maxit, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data, eta.ctypes.data)
+        it = self.lib.wto_rk_split(C.byref(self.g), C.byref(self.fs), self.weight.ctypes.data, mass.shape[0],
+                                   nstages, tol, maxit, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data,
+                                   eta.ctypes.data)
         return mass, temp, velo, eta, it
 
     def rhs(self, eta, divF, divF_old):
@@ -382,3 +386,17 @@ class MlOracle:
         self.lib.wto_rk4_ml(C.byref(self.h), L, float(dt), int(nsteps), self._ptrs(mass), self._ptrs(temp),
                             self._ptrs(velo))
         return mass, temp, velo
+
+    def rk4_split(self, mg: "MgOracle", mass, temp, velo, eta, nstages=4):
+        """one RK4_split step on the hierarchy with the FMG free surface of mg
+        (the same grids; its dt, theta1, theta2 and solver tolerances);
+        nstages 3, 2, 1: RK3_split, RK2_split, Euler_split"""
+        mass, temp, velo = self._copy(mass), self._copy(temp), self._copy(velo)
+        eta = [_f64(e).copy() for e in eta]
+        L = mass[0].shape[0]
+        nl = len(eta)
+        iters = np.zeros(nl, dtype=np.int32)
+        self.lib.wto_rk_split_ml.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 5
+        self.lib.wto_rk_split_ml(C.byref(self.h), C.byref(mg.h), L, nstages, self._ptrs(mass), self._ptrs(temp),
+                                 self._ptrs(velo), self._ptrs(eta), iters.ctypes.data)
+        return mass, temp, velo, eta, iters

@@ -19,8 +19,10 @@
 !   bc_mass, bc_temp, bc_velo   barotropic_correction of the state
 !   ge_flux                     grad_eta: g grad(eta) on the edges (horiz_flux(S_TEMP))
 !   sp_mass, sp_temp, sp_velo, sp_eta   the state after one RK4_split step
+!                                       (nstage = 3, 2, 1: RK3_split, RK2_split,
+!                                       Euler_split, time_integr.f90:384-451)
 !
-! Namelist &run (stdin): zlev, step (dt), theta_1, theta_2, wjac, tol_solve, maxit, split, out.
+! Namelist &run (stdin): zlev, step (dt), theta_1, theta_2, wjac, tol_solve, maxit, split, nstage, out.
 module ell_hooks
   use domain_mod
   implicit none
@@ -139,12 +141,13 @@ program dump_elliptic
   implicit none
 
   character(255) :: out
-  integer        :: zlev, maxit, d, k, l, u, iters, j, nmax, split
+  integer        :: zlev, maxit, d, k, l, u, iters, j, nmax, split, nstage
   real(8)        :: step, theta_1, theta_2, wjac, tol_solve, err, nrm
   type(Float_Field) :: Lq, dgl, uu
-  namelist /run/ zlev, step, theta_1, theta_2, wjac, tol_solve, maxit, split, out
+  namelist /run/ zlev, step, theta_1, theta_2, wjac, tol_solve, maxit, split, nstage, out
 
   zlev = 2; step = 600d0; theta_1 = 1d0; theta_2 = 1d0; wjac = 0.7d0; tol_solve = 1d-9; maxit = 200; split = 0
+  nstage = 4
   out = 'ell.bin'
   read (5, nml=run)
 
@@ -301,7 +304,16 @@ program dump_elliptic
      fine_tol   = tol_solve
      coarse_iter = maxit
      log_iter = .false.
-     call RK4_split (dt)
+     select case (nstage)
+     case (3)
+        call RK3_split (dt)
+     case (2)
+        call RK2_split (dt)
+     case (1)
+        call Euler_split (dt)
+     case default
+        call RK4_split (dt)
+     end select
      call update_bdry (sol(:,1:zmax), NONE, 1)
      call dump_layers ('sp')
      call dump_field ('sp_eta', sol(S_MASS,zlevels+1))

@@ -15,7 +15,13 @@
 !     t0 (trend_ml of s0, every level), a0 (WT_after_step of s0 + dt/4 t0),
 !     and the state after each of nsteps RK4 steps (time_integr.f90:78-103).
 !
-! Namelist &run (stdin): maxlev, zlev, nsteps, step, out.
+! With split = 1 the model runs in mode-split form (mode_split, zmax =
+! zlevels+1: the free surface eta is the extra layer's mass) and the steps
+! are RK4_split (time_integr.f90:360-382) with the FMG free-surface solve to
+! tol_solve (lin_solve.f90:33-140); eta is dumped per level as 'eta'.
+!
+! Namelist &run (stdin): maxlev, zlev, nsteps, step, split, th1, th2,
+! tol_solve, out.
 module ml_hooks
   use domain_mod
   implicit none
@@ -69,6 +75,13 @@ contains
          - (radius*omega*u0 + 0.5d0*u0**2) * sin (lat)**2 / grav_accel
   end function surface
 
+  real(8) function eta_init (p)
+    type(Coord) :: p
+    real(8)     :: lon, lat
+    call cart2sph (p, lon, lat)
+    eta_init = 0.5d0 * exp (-((lon - 0.5d0)**2 + (lat - 0.3d0)**2) / 0.04d0) + 0.05d0 * cos (lat)**2 * sin (3d0*lon)
+  end function eta_init
+
   real(8) function layer_density (k)
     ! a single layer is lighter than the reference density, so that its
     ! temperature variable (and its flux restriction) is not zero
@@ -113,6 +126,14 @@ contains
             proj_vel (flow, dom%node%elts(id+1),   dom%node%elts(idN+1)) * (1d0 - 0.05d0*dble(k-1))
        sol_mean(S_VELO,k)%data(d)%elts(EDGE*id+RT+1:EDGE*id+UP+1) = 0d0
     end do
+    if (mode_split) then
+       sol(S_MASS,zlevels+1)%data(d)%elts(id+1) = eta_init (dom%node%elts(id+1))
+       sol(S_TEMP,zlevels+1)%data(d)%elts(id+1) = 0d0
+       sol(S_VELO,zlevels+1)%data(d)%elts(EDGE*id+RT+1:EDGE*id+UP+1) = 0d0
+       sol_mean(S_MASS,zlevels+1)%data(d)%elts(id+1) = 0d0
+       sol_mean(S_TEMP,zlevels+1)%data(d)%elts(id+1) = 0d0
+       sol_mean(S_VELO,zlevels+1)%data(d)%elts(EDGE*id+RT+1:EDGE*id+UP+1) = 0d0
+    end if
   end subroutine init_sol
 end module ml_ic
 
@@ -130,12 +151,13 @@ program dump_mlstep
   external :: set_thresholds_zero
 
   character(255) :: out
-  integer        :: maxlev, zlev, nsteps, d, k, l, n, u
-  real(8)        :: step
+  integer        :: maxlev, zlev, nsteps, d, k, l, n, u, split
+  real(8)        :: step, th1, th2, tol_solve
   type(Float_Field), dimension(:,:), allocatable :: cp
-  namelist /run/ maxlev, zlev, nsteps, step, out
+  namelist /run/ maxlev, zlev, nsteps, step, split, th1, th2, tol_solve, out
 
   maxlev = MIN_LEVEL + 1; zlev = 2; nsteps = 2; step = 600d0; out = 'ml.bin'
+  split = 0; th1 = 1d0; th2 = 1d0; tol_solve = 1d-9
   read (5, nml=run)
 
   call init_shared_mod
@@ -144,7 +166,14 @@ program dump_mlstep
   level_start   = MIN_LEVEL
   level_end     = MIN_LEVEL
   compressible  = .false.
-  mode_split    = .false.
+  mode_split    = split /= 0
+  theta1        = th1
+  theta2        = th2
+  dt            = step
+  fine_tol      = tol_solve
+  coarse_tol    = tol_solve
+  coarse_iter   = 400
+  max_vcycle    = 50
   optimize_grid = NO_OPTIM
   Laplace_sclr  = 0; Laplace_divu = 0; Laplace_rotu = 0
   radius        = 6.371d6
@@ -165,6 +194,7 @@ program dump_mlstep
   physics_scalar_flux => no_scalar_flux
   physics_velo_source => no_velo_source
   update => no_update
+  elliptic_solver => FMG
 
   call init_comm_mod
   call init_init_mod
@@ -217,6 +247,18 @@ program dump_mlstep
   call dump_whole
   call dump_state ('s0_', sol)
   call dump_state ('w0_', wav_coeff)
+  if (mode_split) then
+     call update_bdry (sol(S_MASS,zlevels+1), NONE, 1)
+     call dump_eta ('e00_')
+     do n = 1, nsteps
+        call RK4_split (dt)
+        call update_bdry (sol(:,1:zmax), NONE, 1)
+        call dump_state ('s'//achar(48+n/10)//achar(48+mod(n,10))//'_', sol)
+        call dump_eta ('e'//achar(48+n/10)//achar(48+mod(n,10))//'_')
+     end do
+     close (u)
+     stop
+  end if
 
   ! inverse transform of (w0, s0) on a copy
   allocate (cp(1:N_VARIABLE,1:zlevels))
@@ -323,6 +365,14 @@ contains
        end do
     end do
   end subroutine dump_state
+
+  subroutine dump_eta (pre)
+    character(*) :: pre
+    integer :: dd
+    do dd = 1, size(grid)
+       call wr_f (pre//'eta', sol(S_MASS,zlevels+1)%data(dd)%elts(1:grid(dd)%node%length))
+    end do
+  end subroutine dump_eta
 
   function pole_ids (dd, l)
     ! elements of the level-l patches of domain dd (with one boundary row) that are poles

@@ -80,8 +80,11 @@ void wto_grad_eta(const wto_grid* g, const double* eta, double* hf) {
   }
 }
 
-int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, double tol, int maxit,
-                  double* mass, double* temp, double* velo, double* eta) {
+/* Euler_split, RK2_split, RK3_split, RK4_split (time_integr.f90:360-451): the
+ * low-storage stages RK_split(h/(n-k)) from sol through q1 back to sol, then
+ * free_surface_update */
+int wto_rk_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, int nstages, double tol,
+                 int maxit, double* mass, double* temp, double* velo, double* eta) {
   const int64_t NC = g->ncells, ns = NC * L, ne = 3 * NC * L;
   double *m1 = malloc(8 * ns), *t1 = malloc(8 * ns), *u1 = malloc(8 * ne);
   double *dm = malloc(8 * ns), *dtm = malloc(8 * ns), *du = malloc(8 * ne);
@@ -93,18 +96,18 @@ int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, 
   memcpy(m1, mass, 8 * ns);
   memcpy(t1, temp, 8 * ns);
   memcpy(u1, velo, 8 * ne);
-  const double coef[4] = {p->dt / 4.0, p->dt / 3.0, p->dt / 2.0, p->dt};
   wto_grad_eta(g, eta, hf); /* eta is fixed through the stages */
-  for (int st = 0; st < 4; ++st) {
+  for (int st = 0; st < nstages; ++st) {
     double* ym = st == 0 ? mass : m1;
     double* yt = st == 0 ? temp : t1;
     double* yu = st == 0 ? velo : u1;
     wto_barotropic_correction(g, L, eta, ym, yt);
     wto_trend(g, L, ym, yt, yu, dm, dtm, du);
-    double* om = st == 3 ? mass : m1;
-    double* ot = st == 3 ? temp : t1;
-    double* ou = st == 3 ? velo : u1;
-    const double h = coef[st];
+    const int last = st == nstages - 1;
+    double* om = last ? mass : m1;
+    double* ot = last ? temp : t1;
+    double* ou = last ? velo : u1;
+    const double h = p->dt / (double)(nstages - st); /* h/4, h/3, h/2, h */
     for (int k = 0; k < L; ++k)
       for (int64_t c = 0; c < NC; ++c) {
         if (!is_master(g, c)) continue;
@@ -139,4 +142,9 @@ int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, 
   wto_ghost_fill(g, L, mass, temp, velo);
   free(m1); free(t1); free(u1); free(dm); free(dtm); free(du); free(hf); free(dF); free(dFold); free(rhs); free(eo);
   return iters;
+}
+
+int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, double tol, int maxit,
+                  double* mass, double* temp, double* velo, double* eta) {
+  return wto_rk_split(g, p, wgt, L, 4, tol, maxit, mass, temp, velo, eta);
 }

@@ -17,6 +17,10 @@ void wto_grad_eta(const wto_grid* g, const double* eta, double* hflux);
  * solved by BiCGSTAB to tol (maxit iterations); returns its iteration count */
 int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, double tol, int maxit,
                   double* mass, double* temp, double* velo, double* eta);
+/* the same with nstages = 1..4 stages: Euler_split, RK2_split, RK3_split,
+ * RK4_split (stage coefficients dt/nstages, ..., dt/2, dt) */
+int wto_rk_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, int nstages, double tol,
+                 int maxit, double* mass, double* temp, double* velo, double* eta);
 
 #ifdef __cplusplus
 }

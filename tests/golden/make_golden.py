@@ -24,6 +24,8 @@ lozenge-tile layout:
   mlstep_J4.npz      hierarchy J4..J5 (harness dump_mlstep.f90): transfer
                      weights, velocity wavelets, inverse transform, trend_ml
                      on every level, WT_after_step and two RK4 steps
+  mlsplit_J4.npz     the same hierarchy in mode-split form: two RK4_split
+                     steps with the FMG free-surface solve, eta per level
 
 The fixtures are committed: the tests read them, never the reference build,
 which exists only where the reference sources are.
@@ -276,12 +278,13 @@ ELLIPTIC_CASES = [
 ]
 
 
-def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=0):
+def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=0, nstage=4, only=None):
+    """only: return the split step's state (sp_*) instead of writing a file"""
     param, ml, dl = PARAMS[J]
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "e.bin")
         run_ref(param, f"&run zlev={L}, step={dt}, theta_1={th1}, theta_2={th2}, wjac={wjac}, tol_solve={tol}, "
-                       f"maxit=400, split={split}, out='OUT' /\n", p, prog="dump_elliptic")
+                       f"maxit=400, split={split}, nstage={nstage}, out='OUT' /\n", p, prog="dump_elliptic")
         o = load_dump(p)
     n, S, ND, L_, cover = lozenge_cover(o)
     assert L_ == L
@@ -336,6 +339,8 @@ def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=0):
             ge[:, cell_index(S, *key)] = o["ge_flux"][dd][3 * e:3 * e + 3]
         extra["ge_flux"] = ge
         extra["temp0"] = layers("s0_temp", 1)[:, 0]
+    if only:
+        return {k: v for k, v in extra.items() if k.startswith("sp_")}
     prm = o["params"]
     np.savez_compressed(os.path.join(OUT, f"elliptic_{name}.npz"), **extra, level=J, min_level=ml, domain_level=dl,
                         nlayers=L, radius=prm[0], omega=prm[1], grav=prm[2], ref_density=prm[3], dt=prm[4],
@@ -346,6 +351,19 @@ def elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=0):
                         jac=node(o["e_jac"]), sol=node(o["e_sol"]), iters=int(o["e_iters"][0][0]),
                         err=o["e_err"][0])
     print(f"elliptic_{name}.npz written")
+
+
+def split_schemes_fixture():
+    """RK3_split, RK2_split and Euler_split (time_integr.f90:384-451): one
+    step of each from the initial state of elliptic_splitJ4, the state and
+    free surface after it (sp<n>_*, n = stages)"""
+    name, J, L, dt, th1, th2, wjac, tol, _ = ELLIPTIC_CASES[2]
+    data = {}
+    for n in (3, 2, 1):
+        for k, v in elliptic_fixture(name, J, L, dt, th1, th2, wjac, tol, split=1, nstage=n, only=True).items():
+            data[k.replace("sp_", f"sp{n}_")] = v
+    np.savez_compressed(os.path.join(OUT, "split_schemes_J4.npz"), **data)
+    print("split_schemes_J4.npz written")
 
 
 def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
@@ -428,7 +446,7 @@ def levels_fixture(J=4, maxlev=6, L=2, dt=600.0, tol=1e-9):
     print(f"levels_J{J}.npz written: FMG iterations {out['fmg_iters']}")
 
 
-def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
+def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2, split=0, th1=1.0, th2=1.0, tol=1e-9, name=None):
     """Multi-level explicit step on the full hierarchy J..maxlev (harness
     dump_mlstep.f90, uniform refinement with zero thresholds): per level l,
     on the owned cells and the two pole masters, topography, the initial
@@ -443,8 +461,8 @@ def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
     param, ml, dl = PARAMS[J]
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "ml.bin")
-        run_ref(param, f"&run maxlev={maxlev}, zlev={L}, nsteps={nsteps}, step={dt}d0, out='OUT' /\n", p,
-                prog="dump_mlstep")
+        run_ref(param, f"&run maxlev={maxlev}, zlev={L}, nsteps={nsteps}, step={dt}d0, split={split}, "
+                       f"th1={th1}d0, th2={th2}d0, tol_solve={tol}, out='OUT' /\n", p, prog="dump_mlstep")
         recs = read_records(p)
     meta = {t: a for t, a in recs if t in ("min_level", "params", "nelts")}
     _, DL, PS, L_, ND, ls, le = (int(v) for v in meta["min_level"])
@@ -468,12 +486,15 @@ def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
     for t, m in (("ovl_a", 4), ("ovl_split", 2), ("iu_wgt", 9), ("rf_wgt", 3)):
         whole[t] = [np.stack(whole[t][d * m:(d + 1) * m]) for d in range(ND)]
     for t in list(state):
-        state[t] = [[state[t][k * ND + d] for d in range(ND)] for k in range(L)]
+        if t.endswith("_eta"):
+            state[t] = [state[t]]  # one 2-D field per domain
+        else:
+            state[t] = [[state[t][k * ND + d] for d in range(ND)] for k in range(L)]
     H_ = 2
     m = PS + 2 * H_
     out = dict(level_start=ls, level_end=le, min_level=ml, domain_level=dl, nlayers=L, dt=dt, nsteps=nsteps,
                radius=meta["params"][0], omega=meta["params"][1], grav=meta["params"][2],
-               ref_density=meta["params"][3])
+               ref_density=meta["params"][3], split=split, theta1=th1, theta2=th2, tol=tol)
     for l in range(ls, le + 1):
         g = W.Grid(level=l, min_level=ml, domain_level=dl)
         lay = g.layout
@@ -553,12 +574,16 @@ def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
         me = field([a.astype(np.float64) for a in whole["mask_e"]], 3, per_layer=False)
         out[f"mask_n_{l}"] = np.nan_to_num(mn, nan=-1).astype(np.int8)
         out[f"mask_e_{l}"] = np.nan_to_num(me, nan=-1).astype(np.int8)
-        for pre, halo_ in (("s0", True), ("t0", False), ("a0", False)) + tuple(
-                (f"s{s:02d}", False) for s in range(1, nsteps + 1)):
+        stages = (("s0", True),) + ((("t0", False), ("a0", False)) if not split else ()) + tuple(
+            (f"s{s:02d}", False) for s in range(1, nsteps + 1))
+        for pre, halo_ in stages:
             out[f"{pre}_mass_{l}"] = field(state[f"{pre}_mass"], 1, halo_)
             out[f"{pre}_temp_{l}"] = field(state[f"{pre}_temp"], 1, halo_)
             out[f"{pre}_velo_{l}"] = field(state[f"{pre}_velo"], 3, halo_)
-        if l > ls:  # the wavelets and the inverse transform of the finer level
+        if split:  # the free surface per level
+            for s in range(0, nsteps + 1):
+                out[f"e{s:02d}_eta_{l}"] = field([state[f"e{s:02d}_eta"][0]], 1, True)
+        if l > ls and not split:  # the wavelets and the inverse transform of the finer level
             out[f"w0_mass_{l}"] = field(state["w0_mass"], 1)
             out[f"w0_velo_{l}"] = field(state["w0_velo"], 3)
             out[f"i0_velo_{l}"] = field(state["i0_velo"], 3)
@@ -566,14 +591,15 @@ def mlstep_fixture(J=4, maxlev=5, L=1, dt=600.0, nsteps=2):
             out[f"iu_{l}"] = comp("iu_wgt", 9)
             out[f"split_{l}"] = comp("ovl_split", 2)
             out[f"rf_{l}"] = comp("rf_wgt", 3)
-    np.savez_compressed(os.path.join(OUT, f"mlstep_J{J}.npz"), **out)
-    print(f"mlstep_J{J}.npz written")
+    name = name or f"mlstep_J{J}"
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print(f"{name}.npz written")
 
 
 def main():
     if not os.path.isdir("/root/reference/src"):
         sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
-    what = sys.argv[1:] or ["grid", "trisk", "elliptic", "levels", "mlstep"]
+    what = sys.argv[1:] or ["grid", "trisk", "elliptic", "schemes", "levels", "mlstep", "mlsplit"]
     if "grid" in what:
         for J in sorted(PARAMS):
             grid_fixture(J)
@@ -583,10 +609,14 @@ def main():
     if "elliptic" in what:
         for c in ELLIPTIC_CASES:
             elliptic_fixture(*c)
+    if "schemes" in what:
+        split_schemes_fixture()
     if "levels" in what:
         levels_fixture()
     if "mlstep" in what:
         mlstep_fixture()
+    if "mlsplit" in what:
+        mlstep_fixture(nsteps=2, split=1, th1=0.6, th2=0.8, name="mlsplit_J4")
 
 
 if __name__ == "__main__":

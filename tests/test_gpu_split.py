@@ -124,3 +124,89 @@ def test_split_step_on_hierarchy(method):
     assert maxrel(e2[masters], eo[masters]) <= 1e-9
     assert maxrel(m2[:, masters], mo[:, masters]) <= 1e-9
     assert maxrel(u2[:, :, own], uo[:, :, own]) <= 1e-9
+
+
+def test_fused_split_stage_equals_trend_plus_axpy():
+    """The FP64 split-explicit stage fused into the stage kernel's update
+    gives the same bits as the trend launch plus split_axpy_kernel (contexts
+    created with WTG_SPLIT_FUSED=0 take the unfused path): J5 L4, two steps,
+    a single context and a two-member ensemble."""
+    import os
+    print("fused split stage vs trend + AXPY, J5 L4", flush=True)
+    from wavetrisk_hydrostatic_amd import cases
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L = 4
+    topo, mass, temp, velo = cases.ocean(g, L, seed=13)
+    eta = mass.sum(0) / 1028.0 + topo
+    out = {}
+    for fused in (True, False):
+        if fused:
+            os.environ.pop("WTG_SPLIT_FUSED", None)
+        else:
+            os.environ["WTG_SPLIT_FUSED"] = "0"
+        try:
+            ctx = W.Context(g, nlayers=L)
+            ens = W.Context(g, nlayers=L, members=2)
+        finally:
+            os.environ.pop("WTG_SPLIT_FUSED", None)
+        ctx.set_topography(topo)
+        ctx.fs_set_params(900.0, 0.6, 0.8)
+        ctx.upload_state(0, mass, temp, velo)
+        ctx.fs_upload(W.FS_ETA, eta)
+        it = ctx.rk4_split(0, 2, 1e-11, 400)
+        ens.set_topography(np.stack([topo, topo]))
+        ens.fs_set_params(900.0, 0.6, 0.8)
+        ens.upload_state(0, np.concatenate([mass, mass]), np.concatenate([temp, temp]),
+                         np.concatenate([velo, velo]))
+        ens.fs_upload(W.FS_ETA, np.stack([eta, eta]))
+        ite = ens.rk4_split(0, 2, 1e-11, 400)
+        out[fused] = (it, ite, ctx.download_state(0), ctx.fs_download(W.FS_ETA), ens.download_state(0),
+                      ens.fs_download(W.FS_ETA))
+        ctx.close()
+        ens.close()
+    a, b = out[True], out[False]
+    assert a[0] == b[0] and a[1] == b[1]
+    own, owne = g.layout.owned_cells(), g.layout.owned_edge_cells()
+    for x, y in ((a[2], b[2]), (a[4], b[4])):
+        assert np.array_equal(x[0][..., own], y[0][..., own]) and np.array_equal(x[1][..., own], y[1][..., own])
+        assert np.array_equal(x[2][..., owne], y[2][..., owne])
+    assert np.array_equal(a[3][own], b[3][own]) and np.array_equal(a[5][:, own], b[5][:, own])
+
+
+@pytest.mark.parametrize("nstages", [3, 2, 1])
+def test_other_split_schemes_match_reference_and_oracle(nstages):
+    """RK3_split, RK2_split, Euler_split (wtg_rk_split) on the J4 case of
+    tests/golden/split_schemes_J4.npz: the oracle's BiCGSTAB count, the
+    state within 1e-12 and the free surface within 5e-12 of the reference
+    and of the oracle (the bars of tests/test_split_oracle.py)"""
+    print(f"split step J4, {nstages} stages", flush=True)
+    f = golden("elliptic_splitJ4.npz")
+    s = golden("split_schemes_J4.npz")
+    g = grid_of_fixture(f)
+    topo = np.nan_to_num(f["topo"])
+    kw = dict(grav=float(f["grav"]), rho0=float(f["ref_density"]), dt=float(f["dt"]), theta1=float(f["theta1"]),
+              theta2=float(f["theta2"]))
+    o = EllipticOracle(g.layout, geometry_of(g), topo, **kw)
+    mass, temp, velo = (np.nan_to_num(f[k]).copy() for k in ("mass", "temp0", "velo"))
+    o.ghost_fill(mass, temp, velo)
+    eta = o.node_ghost(np.nan_to_num(f["eta_old"]))
+    ctx = W.Context(g, nlayers=mass.shape[0], grav=kw["grav"], ref_density=kw["rho0"])
+    ctx.set_topography(topo)
+    ctx.fs_set_params(kw["dt"], kw["theta1"], kw["theta2"])
+    ctx.upload_state(0, mass, temp, velo)
+    ctx.fs_upload(W.FS_ETA, eta)
+    it = ctx.rk4_split(0, 1, float(f["tol"]), 400, nstages=nstages)
+    m2, t2, u2 = ctx.download_state(0)
+    e2 = ctx.fs_download(W.FS_ETA)
+    mo, to, uo, eo, ito = o.rk4_split(mass, temp, velo, eta, tol=float(f["tol"]), maxit=400, nstages=nstages)
+    assert it == ito
+    masters = g.layout.owned_cells()
+    own = g.layout.owned_cells(with_poles=False)
+    p = f"sp{nstages}_"
+    for a, b, r, cells in ((m2, mo, s[p + "mass"], masters), (t2, to, s[p + "temp"], masters),
+                           (u2, uo, s[p + "velo"], own)):
+        assert maxrel(a[..., cells], r[..., cells]) <= 1e-12
+        assert maxrel(a[..., cells], b[..., cells]) <= 1e-12
+    assert maxrel(e2[masters], s[p + "eta"][masters]) <= 5e-12
+    assert maxrel(e2[masters], eo[masters]) <= 5e-12
+    ctx.close()

@@ -75,3 +75,24 @@ def test_split_step_properties(case):
     finally:
         o.fs.theta1 = -o.fs.theta1
     assert maxrel(u_bad[:, :, own], f["sp_velo"][:, :, own]) > 1e-6
+
+
+@pytest.mark.parametrize("nstages", [3, 2, 1])
+def test_other_split_schemes_match_reference(case, nstages):
+    """RK3_split, RK2_split and Euler_split (tests/golden/split_schemes_J4.npz:
+    one step of each from the same initial state): state within 1e-12, free
+    surface within 5e-12 (both solve it to a relative residual of 1e-13; the
+    solution error is that times the operator's conditioning, 1.3e-12 for
+    RK3_split); every scheme differs from RK4_split by more than 1e-6 in
+    velocity, so the stage count and coefficients are pinned"""
+    f, g, o, mass, temp, velo, eta = case
+    s = golden("split_schemes_J4.npz")
+    m2, t2, u2, e2, it = o.rk4_split(mass, temp, velo, eta, tol=float(f["tol"]), maxit=400, nstages=nstages)
+    masters = g.layout.owned_cells()
+    own = g.layout.owned_cells(with_poles=False)
+    p = f"sp{nstages}_"
+    assert maxrel(m2[:, masters], s[p + "mass"][:, masters]) <= 1e-12
+    assert maxrel(t2[:, masters], s[p + "temp"][:, masters]) <= 1e-12
+    assert maxrel(u2[:, :, own], s[p + "velo"][:, :, own]) <= 1e-12
+    assert maxrel(e2[masters], s[p + "eta"][masters]) <= 5e-12
+    assert maxrel(u2[:, :, own], f["sp_velo"][:, :, own]) > 1e-6

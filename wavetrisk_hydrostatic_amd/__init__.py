@@ -96,6 +96,7 @@ SIGNATURES = {
     "wtg_rhs_elliptic": (C.c_int, [_P]),
     "wtg_elliptic_solve": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _D]),
     "wtg_rk4_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
+    "wtg_rk_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "wtg_mg_default_params": (None, [_P]),
     "wtg_mg_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     "wtg_ml_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
@@ -106,6 +107,8 @@ SIGNATURES = {
     "wtg_ml_trend": (C.c_int, [_P, C.c_int, C.c_int]),
     "wtg_ml_wt_after_step": (C.c_int, [_P, C.c_int]),
     "wtg_ml_rk4": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
+    "wtg_ml_rk4_split": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
+    "wtg_ml_rk_split": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
     "wtg_mg_destroy": (None, [_P]),
     "wtg_mg_solve": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "wtg_mg_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -410,11 +413,18 @@ class Context:
     def rhs_elliptic(self):
         _check(_lib.wtg_rhs_elliptic(self._h), self._h, "wtg_rhs_elliptic")
 
-    def rk4_split(self, slot: int, nsteps: int = 1, rel_tol: float = 1e-9, max_iter: int = 400) -> int:
+    def rk4_split(self, slot: int, nsteps: int = 1, rel_tol: float = 1e-9, max_iter: int = 400,
+                  nstages: int = 4) -> int:
         """split-explicit RK4 steps with implicit free surface (FS_ETA); dt
-        from fs_set_params; returns the last step's BiCGSTAB iterations"""
+        from fs_set_params; returns the last step's BiCGSTAB iterations.
+        nstages 3, 2, 1: RK3_split, RK2_split, Euler_split (wtg_rk_split)"""
         it = C.c_int()
-        _check(_lib.wtg_rk4_split(self._h, slot, nsteps, rel_tol, max_iter, C.byref(it)), self._h, "wtg_rk4_split")
+        if nstages == 4:
+            _check(_lib.wtg_rk4_split(self._h, slot, nsteps, rel_tol, max_iter, C.byref(it)), self._h,
+                   "wtg_rk4_split")
+        else:
+            _check(_lib.wtg_rk_split(self._h, slot, nstages, nsteps, rel_tol, max_iter, C.byref(it)), self._h,
+                   "wtg_rk_split")
         return it.value
 
     def elliptic_solve(self, method: int = SOLVE_BICGSTAB, rel_tol: float = 1e-9, max_iter: int = 200):
@@ -572,6 +582,22 @@ class MultiLevel:
 
     def rk4(self, slot: int, dt: float, nsteps: int = 1):
         _check(_lib.wtg_ml_rk4(self._h, slot, dt, nsteps), self._top, "wtg_ml_rk4")
+
+    def rk4_split(self, hierarchy: "Hierarchy", slot: int, nsteps: int = 1, params: MgParams | None = None,
+                  nstages: int = 4) -> int:
+        """RK4_split steps on every level with the FMG free surface of
+        `hierarchy` (a Hierarchy of the same contexts; dt, theta1, theta2
+        from each context's fs_set_params, the free surface in FS_ETA of
+        every level); returns the FMG V-cycles summed over levels and steps"""
+        p = params if params is not None else MgParams.defaults()
+        it = C.c_int()
+        if nstages == 4:
+            _check(_lib.wtg_ml_rk4_split(self._h, hierarchy._h, slot, nsteps, C.byref(p), C.byref(it)), self._top,
+                   "wtg_ml_rk4_split")
+        else:
+            _check(_lib.wtg_ml_rk_split(self._h, hierarchy._h, slot, nstages, nsteps, C.byref(p), C.byref(it)),
+                   self._top, "wtg_ml_rk_split")
+        return it.value
 
 
 class MLP:

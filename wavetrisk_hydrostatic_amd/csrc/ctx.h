@@ -53,6 +53,7 @@ struct wtg_ctx {
   wtg_params params{};
   int nslots = 0;
   int nmembers = 1;       // ensemble members, each nlayers layers, one after another in every slot
+  bool split_fused = true;  // FP64 split-explicit stages: trend and AXPY in one launch
   int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
   bool topo_set = false;
   std::string err;
@@ -162,7 +163,8 @@ int wtg_fs_rhs_async(wtg_ctx* c);
 // the caller: solve(user, c, iters) solves elliptic_lo(WTG_FS_ETA) =
 // WTG_FS_RHS on c with WTG_FS_ETA_OLD set, the old eta as initial guess
 typedef int (*wtg_fs_solver)(void* user, wtg_ctx* c, int* iters);
-int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters);
+int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters,
+                    int nstages = 4);
 
 // intermediate quantities of the finest level's stage that a hierarchy's
 // coarser levels restrict (trisk.hip; null members are not kept)
@@ -181,3 +183,22 @@ int wtg_launch_stage_keep(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slo
 int wtg_ghost_fill_layers(wtg_ctx* c, int L, double* m, double* t, double* u);
 // copies the never-filled halo slots of one state to another
 int wtg_copy_frozen_slots(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to);
+// split-explicit pieces on one level (trisk.hip): barotropic_correction of
+// a state with the level's WTG_FS_ETA (masters only); out = base + h (trend
+// + theta1 g grad eta_grad) (scalar_star, u_star); u -= theta1 dt g grad
+// eta_grad (u_update)
+int wtg_baro_correction_slot(wtg_ctx* c, const wtg_ctx::Slot& s);
+int wtg_split_axpy(wtg_ctx* c, double h, const double* eta_grad, const wtg_ctx::Slot& base, const wtg_ctx::Slot& tr,
+                   const wtg_ctx::Slot& out);
+int wtg_u_update(wtg_ctx* c, const double* eta_grad, const wtg_ctx::Slot& s);
+// flux_divergence of a slot into given buffers (elliptic.hip): the
+// layer-integrated edge fluxes [3][ncells] on cells [-1, n]^2 and their
+// divergence (compute list, halo copies included)
+int wtg_fs_flux_div_into(wtg_ctx* c, int slot, double* flux3, double* divf);
+// the free surface's own WT_after_step on a hierarchy (elliptic.hip): the
+// scalar wavelets of WTG_FS_ETA on every level, then the inverse transform
+// coarse to fine
+struct wtg_mg;
+int wtg_mg_eta_transform(wtg_mg* m);
+// true when the solver hierarchy runs on exactly these contexts, coarse to fine
+bool wtg_mg_same_levels(const wtg_mg* m, wtg_ctx* const* lv, int n);

@@ -1469,6 +1469,41 @@ int wtg_fs_rhs_async(wtg_ctx* c) {
   return 0;
 }
 
+int wtg_fs_flux_div_into(wtg_ctx* c, int slot, double* flux3, double* divf) {
+  const auto& s = c->slots[slot];
+  const int64_t NC = c->info.ncells;
+  const int64_t ne = (int64_t)10 * (c->info.n + 2) * (c->info.n + 2);
+  hipLaunchKernelGGL(fs_edge_flux_kernel, dim3((unsigned)((ne + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream,
+                     c->info.n, fs_geo(c), c->params.nlayers, s.m, s.u, flux3, flux3 + NC, flux3 + 2 * NC);
+  WTG_HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(fs_div_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), flux3,
+                     flux3 + NC, flux3 + 2 * NC, divf);
+  WTG_HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+bool wtg_mg_same_levels(const wtg_mg* m, wtg_ctx* const* lv, int n) {
+  if (!m || m->nl != n) return false;
+  for (int l = 0; l < n; ++l)
+    if (m->lv[l] != lv[l]) return false;
+  return true;
+}
+
+int wtg_mg_eta_transform(wtg_mg* m) {
+  for (int l = 1; l < m->nl; ++l) {
+    wtg_ctx* f = m->lv[l];
+    const MgGeo g = mg_geo(m, l);
+    hipLaunchKernelGGL(mg_wavelet_kernel<double>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g,
+                       (const double*)f->fs[FS_ETA], f->fs[MG_WC]);
+    WTG_HIPCHK(f, hipGetLastError());
+  }
+  for (int l = 1; l < m->nl; ++l)
+    RC(mg_err(m, mg_inverse(m, l, (const double*)m->lv[l - 1]->fs[FS_ETA], (const double*)m->lv[l]->fs[MG_WC],
+                            m->lv[l]->fs[FS_ETA]),
+              l));
+  return 0;
+}
+
 extern "C" {
 
 int wtg_fs_set_params(wtg_ctx* c, const wtg_fs_params* p) {

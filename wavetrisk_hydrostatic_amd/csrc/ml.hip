@@ -41,6 +41,7 @@ struct wtg_ml {
   std::vector<double*> wcs;    // [2L][NC]: scalar wavelets (l >= 1)
   std::vector<double*> wcu;    // [L][3][NC]: velocity wavelets (l >= 1)
   double *bern = nullptr, *exn = nullptr;  // finest level [L][NC]
+  std::vector<double*> eg;     // [NC] free surface of the gradient (coarse levels sub-sampled)
   std::vector<void*> allocs;
 };
 
@@ -506,7 +507,7 @@ __global__ void ml_coarse_scalar_kernel(int n, int nmaster, const int64_t* __res
                                         const double* __restrict__ hinv, const double* __restrict__ hcl,
                                         double* __restrict__ dsl, const double* __restrict__ qm,
                                         const double* __restrict__ qt, double* __restrict__ om,
-                                        double* __restrict__ ot, double h, int trend_only) {
+                                        double* __restrict__ ot, double h, int trend_only, int plus_div) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int y = blockIdx.y, k = y % L;
@@ -514,7 +515,7 @@ __global__ void ml_coarse_scalar_kernel(int n, int nmaster, const int64_t* __res
   const int64_t dst = ncl[2 * t], c = ncl[2 * t + 1];
   const int64_t cW = c - 1, cS = c - S, cSW = c - S - 1;
   const double div = (hf[c] - hf[cW] + hf[NC + cSW] - hf[NC + c] + hf[2 * NC + c] - hf[2 * NC + cS]) * hinv[c];
-  const double d = -div;
+  const double d = plus_div ? div : -div;
   dsl[(int64_t)y * NC + dst] = d;
   if (t < nmaster && om) {
     const int64_t x = (int64_t)k * NC + dst;
@@ -569,6 +570,19 @@ __global__ void ml_coarse_velo_kernel(PairGeo g, int L, int scale, int Sfine, in
     const int64_t x = ((int64_t)k * 3 + e) * NC + p;
     ou[x] = trend_only ? du : qu[x] + h * du;
   }
+}
+
+// cpt_or_restr_eta (barotropic_2d.f90:417-455): the coarse copy of the free
+// surface takes the finer copy's value at the coincident node on [0, n]^2
+__global__ void ml_eta_subsample_kernel(int nc, int Sc, int Sf, const double* __restrict__ ef,
+                                        double* __restrict__ ec) {
+  const int w = nc + 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)10 * w * w) return;
+  const int lz = (int)(t / ((int64_t)w * w)), r = (int)(t % ((int64_t)w * w));
+  const int i = r % w, j = r / w;
+  ec[(int64_t)lz * Sc * Sc + (int64_t)(j + 2) * Sc + (i + 2)] =
+      ef[(int64_t)lz * Sf * Sf + (int64_t)(2 * j + 2) * Sf + (2 * i + 2)];
 }
 
 // ---------------------------------------------------------------------------
@@ -726,7 +740,7 @@ int stage(wtg_ml* m, const std::vector<const wtg_ctx::Slot*>& y, const std::vect
     const int64_t NC = c->info.ncells;
     hipLaunchKernelGGL(ml_coarse_scalar_kernel, dim3(nblk(c->nncl), 2 * L), dim3(NT), 0, mstream(m), c->nncl,
                        c->nmaster, c->d_ncl, c->info.S, NC, L, c->d_hinv, m->hflux[l], m->dsc[l], q[l]->m, q[l]->t,
-                       out[l]->m, out[l]->t, h, trend_only);
+                       out[l]->m, out[l]->t, h, trend_only, 0);
     MLCHK(m, hipGetLastError());
     hipLaunchKernelGGL(ml_coarse_velo_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), L), dim3(NT), 0, mstream(m), pg, L,
                        1 << (top - l), ft->info.S, NT_, m->bern, m->exn, m->src[l + 1], m->src[l], y[l]->m, y[l]->t,
@@ -789,6 +803,7 @@ int wtg_ml_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
   m->src.assign(nlevels, nullptr);
   m->wcs.assign(nlevels, nullptr);
   m->wcu.assign(nlevels, nullptr);
+  m->eg.assign(nlevels, nullptr);
   wtg_ml* mp = m.get();
   auto bail = [&](int rc) {
     g_create_error = mp->lv[mp->nl - 1]->err;
@@ -816,7 +831,7 @@ int wtg_ml_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
       if ((rc = ml_alloc(mp, &m->wcs[l], 2 * L * NC)) || (rc = ml_alloc(mp, &m->wcu[l], 3 * L * NC))) return bail(rc);
     }
     if ((rc = ml_alloc(mp, &m->hflux[l], 6 * L * NC)) || (rc = ml_alloc(mp, &m->dsc[l], 2 * L * NC)) ||
-        (rc = ml_alloc(mp, &m->src[l], 3 * L * NC)))
+        (rc = ml_alloc(mp, &m->src[l], 3 * L * NC)) || (rc = ml_alloc(mp, &m->eg[l], NC)))
       return bail(rc);
     // fluxes outside the stage kernel's region and never-filled slots stay zero
     if (hipMemset(m->hflux[l], 0, 6 * L * NC * sizeof(double)) != hipSuccess ||
@@ -945,4 +960,135 @@ int wtg_ml_rk4(wtg_ml* m, int slot, double dt, int nsteps) {
   }
   MLCHK(m, hipStreamSynchronize(mstream(m)));
   return 0;
+}
+
+namespace {
+
+// flux_divergence (barotropic_2d.f90:326-366) of a slot on every level into
+// the field `out` of each level: the finest level's layer-integrated fluxes
+// and divergence, coarser levels from the restricted flux (cpt_or_restr_flux
+// with the finer divergence as dscalar), each divergence ghost filled
+int flux_divergence_ml(wtg_ml* m, int slot, int out) {
+  const int top = m->nl - 1;
+  wtg_ctx* ft = m->lv[top];
+  if (int rc = wtg_fs_flux_div_into(ft, slot, m->hflux[top], ft->fs[out])) return rc;
+  if (int rc = wtg_ghost_fill_layers(ft, 1, nullptr, nullptr, m->hflux[top])) return rc;
+  for (int l = top - 1; l >= 0; --l) {
+    wtg_ctx* c = m->lv[l];
+    const PairGeo pg = pgeo(m, l + 1);
+    hipLaunchKernelGGL(ml_flux_restrict_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), 1), dim3(NT), 0, mstream(m), pg,
+                       1, m->hflux[l + 1], (const double*)m->lv[l + 1]->fs[out], m->hflux[l]);
+    MLCHK(m, hipGetLastError());
+    if (int rc = wtg_ghost_fill_layers(c, 1, nullptr, nullptr, m->hflux[l])) return rc;
+    hipLaunchKernelGGL(ml_coarse_scalar_kernel, dim3(nblk(c->nncl), 1), dim3(NT), 0, mstream(m), c->nncl, c->nmaster,
+                       c->d_ncl, c->info.S, c->info.ncells, 1, c->d_hinv, m->hflux[l], c->fs[out],
+                       (const double*)nullptr, (const double*)nullptr, (double*)nullptr, (double*)nullptr, 0.0, 1, 1);
+    MLCHK(m, hipGetLastError());
+  }
+  return 0;
+}
+
+// the free surface each level's external pressure gradient uses: the
+// finest level's own, coarser levels a copy with [0, n]^2 sub-sampled from
+// the next finer copy (grad_eta, barotropic_2d.f90:457-474)
+int eta_for_gradient(wtg_ml* m) {
+  const int top = m->nl - 1;
+  m->eg[top] = nullptr;
+  for (int l = top - 1; l >= 0; --l) {
+    wtg_ctx *c = m->lv[l], *f = m->lv[l + 1];
+    MLCHK(m, hipMemcpyAsync(m->eg[l], c->fs[FS_ETA], (size_t)c->info.ncells * sizeof(double), hipMemcpyDeviceToDevice,
+                            mstream(m)));
+    const double* ef = l + 1 == top ? f->fs[FS_ETA] : m->eg[l + 1];
+    const int w = c->info.n + 1;
+    hipLaunchKernelGGL(ml_eta_subsample_kernel, dim3(nblk((int64_t)10 * w * w)), dim3(NT), 0, mstream(m), c->info.n,
+                       c->info.S, f->info.S, ef, m->eg[l]);
+    MLCHK(m, hipGetLastError());
+  }
+  return 0;
+}
+
+const double* grad_field(wtg_ml* m, int l) { return l == m->nl - 1 ? m->lv[l]->fs[FS_ETA] : m->eg[l]; }
+
+}  // namespace
+
+int wtg_ml_rk_split(wtg_ml* m, wtg_mg* mg, int slot, int nstages, int nsteps, const wtg_mg_params* p, int* iters) {
+  if (int rc = check(m, slot)) return rc;
+  if (!mg || !p || !iters) return mlfail(m, WTG_E_ARG, "null argument");
+  if (nsteps < 0) return mlfail(m, WTG_E_ARG, "nsteps < 0");
+  if (nstages < 1 || nstages > 4) return mlfail(m, WTG_E_ARG, "split schemes have 1 to 4 stages");
+  const int nl = m->nl, top = nl - 1, ns = m->lv[0]->nslots;
+  if (!wtg_mg_same_levels(mg, m->lv.data(), nl))
+    return mlfail(m, WTG_E_ARG, "the solver hierarchy must run on the same contexts");
+  for (int l = 0; l < nl; ++l) {
+    wtg_ctx* c = m->lv[l];
+    if (!c->fs_params_set) return wtg_fail(m->lv[top], WTG_E_STATE, "level %d: free-surface parameters not set", l);
+    if (!c->fs[FS_ETA]) return wtg_fail(m->lv[top], WTG_E_STATE, "level %d: free-surface fields not allocated", l);
+    if (c->fs_dt != m->lv[top]->fs_dt || c->fs_theta1 != m->lv[top]->fs_theta1 ||
+        c->fs_theta2 != m->lv[top]->fs_theta2)
+      return wtg_fail(m->lv[top], WTG_E_ARG, "level %d: free-surface parameters differ from the finest level's", l);
+  }
+  const auto sol = slots(m, slot), w1 = slots(m, ns), w2 = slots(m, ns + 1);
+  for (int l = 0; l < nl; ++l) {
+    if (int rc = wtg_copy_frozen_slots(m->lv[l], *sol[l], *w1[l])) return rc;
+    if (int rc = wtg_copy_frozen_slots(m->lv[l], *sol[l], *w2[l])) return rc;
+  }
+  const double dt = m->lv[top]->fs_dt, th2 = m->lv[top]->fs_theta2;
+  // stage input, trend buffer and output, as the single-level split step:
+  // sol -> q1 -> ... -> sol with h/n, ..., h/2, h
+  double coef[4];
+  const std::vector<const wtg_ctx::Slot*>*ins[4], *trs[4], *outs[4];
+  for (int k = 0; k < nstages; ++k) {
+    coef[k] = dt / (double)(nstages - k);
+    outs[k] = k == nstages - 1 ? &sol : (k % 2 == 0 ? &w1 : &w2);
+    ins[k] = k == 0 ? &sol : outs[k - 1];
+    trs[k] = outs[k] != &sol ? outs[k] : (ins[k] == &w1 ? &w2 : &w1);
+  }
+  *iters = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    if (int rc = ghost_all(m, sol)) return rc;
+    for (int l = 0; l < nl; ++l) {
+      wtg_ctx* c = m->lv[l];
+      if (int rc = wtg_ghost_fill_layers(c, 1, c->fs[FS_ETA], c->fs[FS_ETA], nullptr)) return rc;
+    }
+    if (th2 != 1.0)
+      if (int rc = flux_divergence_ml(m, slot, FS_DIVF_OLD)) return rc;
+    if (int rc = eta_for_gradient(m)) return rc;
+    for (int st = 0; st < nstages; ++st) {
+      const auto &in = *ins[st], &tr = *trs[st], &out = *outs[st];
+      for (int l = 0; l < nl; ++l)
+        if (int rc = wtg_baro_correction_slot(m->lv[l], *in[l])) return rc;
+      if (int rc = ghost_all(m, in)) return rc;
+      if (int rc = stage(m, in, in, tr, 0.0, 1)) return rc;
+      for (int l = 0; l < nl; ++l)
+        if (int rc = wtg_split_axpy(m->lv[l], coef[st], grad_field(m, l), *sol[l], *tr[l], *out[l])) return rc;
+      if (int rc = wt_after_step(m, out)) return rc;
+    }
+    // free_surface_update: eta_update, barotropic_correction, u_update,
+    // WT_after_step with the free surface
+    if (int rc = flux_divergence_ml(m, slot, FS_DIVF)) return rc;
+    for (int l = 0; l < nl; ++l) {
+      wtg_ctx* c = m->lv[l];
+      if (int rc = wtg_fs_rhs_async(c)) return rc;
+      MLCHK(m, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], (size_t)c->info.ncells * sizeof(double),
+                              hipMemcpyDeviceToDevice, mstream(m)));
+    }
+    // FMG reports the iterations and the residual of every level
+    std::vector<int> its(nl, 0);
+    std::vector<double> res(nl, 0.0);
+    if (int rc = wtg_mg_solve(mg, WTG_SOLVE_FMG, p, its.data(), res.data())) return rc;
+    for (int l = 0; l < nl; ++l) *iters += its[l];
+    for (int l = 0; l < nl; ++l)
+      if (int rc = wtg_baro_correction_slot(m->lv[l], *sol[l])) return rc;
+    if (int rc = eta_for_gradient(m)) return rc;
+    for (int l = 0; l < nl; ++l)
+      if (int rc = wtg_u_update(m->lv[l], grad_field(m, l), *sol[l])) return rc;
+    if (int rc = wtg_mg_eta_transform(mg)) return rc;
+    if (int rc = wt_after_step(m, sol)) return rc;
+  }
+  MLCHK(m, hipStreamSynchronize(mstream(m)));
+  return 0;
+}
+
+int wtg_ml_rk4_split(wtg_ml* m, wtg_mg* mg, int slot, int nsteps, const wtg_mg_params* p, int* iters) {
+  return wtg_ml_rk_split(m, mg, slot, 4, nsteps, p, iters);
 }

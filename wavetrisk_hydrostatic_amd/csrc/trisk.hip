@@ -70,6 +70,10 @@ struct StageArgs {
   double *k_bern, *k_exn;  // [L][NC] Bernoulli, Exner on [0, n]^2
   double *k_src;           // [L][3][NC] velocity source (-Qperp) on owned edges
   double *k_dm, *k_dt;     // [L][NC] scalar trends on owned cells and the pole masters
+  // split-explicit stage (scalar_star / u_star fused into the update): the
+  // velocity update adds theta1 g grad eta to the trend (eta [NC], per member)
+  const double* eta;
+  double th1;
 };
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
@@ -83,6 +87,7 @@ __device__ __forceinline__ StageArgs member_args(const StageArgs& a0) {
   a.qm += s1; a.qt += s1; a.qu += s3;
   a.om += s1; a.ot += s1; a.ou += s3;
   a.g.topo += (int64_t)blockIdx.y * a0.NC;  // each member has its own bathymetry
+  if (a.eta) a.eta += (int64_t)blockIdx.y * a0.NC;
   return a;
 }
 
@@ -799,6 +804,12 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
   w[4] = 0.5 - acc4 * hi;
 }
 
+// KEEP: also store the stage's fluxes, Bernoulli / Exner functions, velocity
+// source and scalar trends (the k_* arrays) for the coarser levels of a
+// hierarchy; SPLIT: the split-explicit stage, out = q + h (trend + theta1 g
+// grad eta) on edges (u_star, barotropic_2d.f90:30-50, in split_axpy_kernel's
+// order).  Separate instantiations, so that the plain step carries neither.
+template <bool KEEP, bool SPLIT>
 __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_in) {
   const StageArgs a = member_args(a_in);
   __shared__ GeoS gs;
@@ -970,7 +981,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
                     gs.hinv[x] / 4.0;
         sl.bern[x] = ke + phi_k + pr / (a.rho0 * 1.0);
         sl.exn[x] = -phi_k;
-        if (a.k_bern) {
+        if (KEEP) {
           sto(a.k_bern + ks, xb, sl.bern[x]);
           sto(a.k_exn + ks, xb, sl.exn[x]);
         }
@@ -992,7 +1003,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
         sl.ht[0][x] = u0 * f0 * interp(t, sl.t[xE]) + 0.0;
         sl.ht[1][x] = u1 * f1 * interp(t, sl.t[xNE]) + 0.0;
         sl.ht[2][x] = u2 * f2 * interp(t, sl.t[xN]) + 0.0;
-        if (a.k_hm) {
+        if (KEEP) {
           for (int e = 0; e < 3; ++e) {
             sto(a.k_hm + ke3 + e * NC, xb, sl.hm[e][x]);
             sto(a.k_ht + ke3 + e * NC, xb, sl.ht[e][x]);
@@ -1040,7 +1051,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       double dmv = -divm, dtv = -divt;
       sto(a.om + ks, ob, a.trend_only ? dmv : bm + a.h * dmv);
       sto(a.ot + ks, ob, a.trend_only ? dtv : bt + a.h * dtv);
-      if (a.k_dm) {
+      if (KEEP) {
         sto(a.k_dm + ks, ob, dmv);
         sto(a.k_dt + ks, ob, dtv);
       }
@@ -1077,11 +1088,18 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       double gB[3] = {(sl.bern[xE] - b0) / ll[0], (b0 - sl.bern[xNE]) / ll[1], (sl.bern[xN] - b0) / ll[2]};
       double gX[3] = {(sl.exn[xE] - x0) / ll[0], (x0 - sl.exn[xNE]) / ll[1], (sl.exn[xN] - x0) / ll[2]};
       const double bu[3] = {bu0, bu1, bu2};
+      double ge[3] = {0.0, 0.0, 0.0};
+      if (SPLIT) {  // grad_eta_e: g (eta(E) - eta) / len, g (eta - eta(NE)) / len, g (eta(N) - eta) / len
+        const double e0 = ldo(a.eta, ob);
+        ge[0] = a.grav * (ldo(a.eta, ob + 8u) - e0) / ll[0];
+        ge[1] = a.grav * (e0 - ldo(a.eta, ob + 8u * (uint32_t)(S + 1))) / ll[1];
+        ge[2] = a.grav * (ldo(a.eta, ob + 8u * (uint32_t)S) - e0) / ll[2];
+      }
       for (int e = 0; e < 3; ++e) {
         double src = -qp[e] + 0.0 * ll[e];
         double du = src / ll[e] - gB[e] - the[e] * gX[e];
-        sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * du);
-        if (a.k_src) sto(a.k_src + ke3 + e * NC, ob, src);
+        sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * (SPLIT ? du + a.th1 * ge[e] : du));
+        if (KEEP) sto(a.k_src + ke3 + e * NC, ob, src);
       }
     }
     if (!(lay && own)) {
@@ -1097,7 +1115,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       double dmv = -divm, dtv = -divt;
       sto(a.om + ks, xb, a.trend_only ? dmv : ldo(a.qm + ks, xb) + a.h * dmv);
       sto(a.ot + ks, xb, a.trend_only ? dtv : ldo(a.qt + ks, xb) + a.h * dtv);
-      if (a.k_dm) {
+      if (KEEP) {
         sto(a.k_dm + ks, xb, dmv);
         sto(a.k_dt + ks, xb, dtv);
       }
@@ -2118,8 +2136,14 @@ int launch_copy_frozen(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slo
 }
 
 int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, const wtg_ctx::Slot& out, double h,
-                 int trend_only, const WtgStageKeep* keep = nullptr) {
+                 int trend_only, const WtgStageKeep* keep = nullptr, const double* split_eta = nullptr) {
   StageArgs a{};
+  if (split_eta) {
+    if (c->stage_version != 3 || keep || trend_only)
+      return fail(c, WTG_E_STATE, "the fused split-explicit stage needs the layer-pair kernel");
+    a.eta = split_eta;
+    a.th1 = c->fs_theta1;
+  }
   if (keep) {
     if (c->stage_version != 3 || c->nmembers != 1)
       return fail(c, WTG_E_STATE, "kept stage quantities need the layer-pair kernel on a single-member context");
@@ -2159,8 +2183,12 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
     hipLaunchKernelGGL(trisk_stage_kernel, grid, dim3(THREADS), 0, c->stream, a);
   else if (c->stage_version == 2)
     hipLaunchKernelGGL(trisk_stage_kernel2, grid, dim3(THREADS2), 0, c->stream, a);
+  else if (keep)
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<true, false>), grid, dim3(THREADS3), 0, c->stream, a);
+  else if (split_eta)
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, true>), grid, dim3(THREADS3), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(trisk_stage_pair_kernel, grid, dim3(THREADS3), 0, c->stream, a);
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, false>), grid, dim3(THREADS3), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -2500,8 +2528,31 @@ int wtg_copy_frozen_slots(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::
   return launch_copy_frozen(c, from, to);
 }
 
-int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters) {
+int wtg_baro_correction_slot(wtg_ctx* c, const wtg_ctx::Slot& s) { return baro_correction(c, s); }
+
+int wtg_split_axpy(wtg_ctx* c, double h, const double* eta_grad, const wtg_ctx::Slot& base, const wtg_ctx::Slot& tr,
+                   const wtg_ctx::Slot& out) {
+  const int L = c->params.nlayers;
+  const unsigned nb = (unsigned)(((int64_t)c->nmaster * L + 255) / 256);
+  hipLaunchKernelGGL(split_axpy_kernel<double>, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L,
+                     c->info.ncells, c->info.S, h, c->fs_theta1, c->params.grav, eta_grad, c->d_len, base.m, base.t,
+                     base.u, tr.m, tr.t, tr.u, out.m, out.t, out.u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int wtg_u_update(wtg_ctx* c, const double* eta_grad, const wtg_ctx::Slot& s) {
+  const int L = c->params.nlayers;
+  const unsigned nb = (unsigned)(((int64_t)c->nmaster * L + 255) / 256);
+  hipLaunchKernelGGL(u_update_kernel, dim3(nb), dim3(256), 0, c->stream, c->nmaster, c->d_ncl, L, c->info.ncells,
+                     c->info.S, c->fs_theta1 * c->fs_dt, c->params.grav, eta_grad, c->d_len, s.u);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void* user, int* iters, int nstages) {
   if (int rc = check_slot(c, slot)) return rc;
+  if (nstages < 1 || nstages > 4) return fail(c, WTG_E_ARG, "split schemes have 1 to 4 stages, not %d", nstages);
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "negative step count");
@@ -2519,6 +2570,8 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
   // FP32-bulk: the trend stencils run in FP32 into the first FP32 stage
   // buffer; the stage states, the AXPYs and the free surface stay FP64
   const bool f32 = c->precision == WTG_PRECISION_FP32_BULK;
+  // FP64 with the layer-pair kernel: the stage's update is the split AXPY
+  const bool fused = !f32 && c->stage_version == 3 && c->split_fused;
   if (f32) {
     if (int rc = f32_ready(c)) return rc;
     if (!c->mixed_tuned && nsteps > 0) {
@@ -2535,10 +2588,18 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
     return rc;
   };
   const auto& ftr = c->fstage[0];
-  const double coef[4] = {dt / 4.0, dt / 3.0, dt / 2.0, dt};
-  const int ins[4] = {slot, c->nslots, c->nslots + 1, c->nslots};
-  const int trs[4] = {c->nslots, c->nslots + 1, c->nslots, c->nslots + 1};
-  const int outs[4] = {c->nslots, c->nslots + 1, c->nslots, slot};
+  // RK_split stages (time_integr.f90:360-451) with h/n, ..., h/2, h: sol ->
+  // q1 -> ... -> sol, q1 alternating between the two work slots (a stage
+  // kernel does not write the state it reads); the trend of the unfused path
+  // goes to the stage's output slot, or the other work slot when that is sol
+  double coef[4];
+  int ins[4], trs[4], outs[4];
+  for (int k = 0; k < nstages; ++k) {
+    coef[k] = dt / (double)(nstages - k);
+    outs[k] = k == nstages - 1 ? slot : (k % 2 == 0 ? c->nslots : c->nslots + 1);
+    ins[k] = k == 0 ? slot : outs[k - 1];
+    trs[k] = outs[k] != slot ? outs[k] : (ins[k] == c->nslots ? c->nslots + 1 : c->nslots);
+  }
   const unsigned nb = (unsigned)(((int64_t)c->nmaster * L + 255) / 256);
   *iters = 0;
   for (int s = 0; s < nsteps; ++s) {
@@ -2548,11 +2609,16 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
       if (int rc = wtg_fs_flux_div_async(v, slot)) return view_err(v, rc);
     HIPCHK(c, hipMemcpyAsync(c->fs[FS_DIVF_OLD], c->fs[FS_DIVF], NCM * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
-    for (int st = 0; st < 4; ++st) {
+    for (int st = 0; st < nstages; ++st) {
       const auto &in = c->slots[ins[st]], &tr = c->slots[trs[st]], &out = c->slots[outs[st]];
       for (wtg_ctx* v : V)
         if (int rc = baro_correction(v, v->slots[ins[st]])) return view_err(v, rc);
       if (int rc = launch_ghost_fill(c, in)) return rc;
+      if (fused && ins[st] != outs[st]) {  // trend and u_star / scalar_star in one launch
+        if (int rc = launch_stage(c, in, sol, out, coef[st], 0, nullptr, c->fs[FS_ETA])) return rc;
+        if (int rc = launch_ghost_fill(c, out)) return rc;
+        continue;
+      }
       if (f32) {
         if (int rc = launch_stage_mixed<false, float>(c, in, nullptr, ftr.m, ftr.t, ftr.u, 1.0)) return rc;
       } else {
@@ -2741,6 +2807,9 @@ int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmem
     int sv = std::atoi(v);
     c->stage_version = (sv == 1 || sv == 2) ? sv : 3;
   }
+  // WTG_SPLIT_FUSED=0: the split-explicit stage as a trend launch and a
+  // separate AXPY (the two give the same bits)
+  if (const char* v = std::getenv("WTG_SPLIT_FUSED")) c->split_fused = std::atoi(v) != 0;
   const auto& g = grid->lv;
   const int64_t NC = c->info.ncells;
   // flux dual lengths: a halo edge that is ghost filled carries its owner's flux
@@ -3024,14 +3093,18 @@ int wtg_trend(wtg_ctx* c, int q_slot, int dq_slot) {
   return 0;
 }
 
-int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter, int* iters) {
+int wtg_rk_split(wtg_ctx* c, int slot, int nstages, int nsteps, double rel_tol, int max_iter, int* iters) {
   if (!c || !iters) return WTG_E_ARG;
   *iters = 0;
   if (max_iter < 0) return fail(c, WTG_E_ARG, "max_iter < 0");
   double tm[2] = {rel_tol, (double)max_iter};
-  if (int rc = wtg_split_steps(c, slot, nsteps, bicgstab_solve, tm, iters)) return rc;
+  if (int rc = wtg_split_steps(c, slot, nsteps, bicgstab_solve, tm, iters, nstages)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int wtg_rk4_split(wtg_ctx* c, int slot, int nsteps, double rel_tol, int max_iter, int* iters) {
+  return wtg_rk_split(c, slot, 4, nsteps, rel_tol, max_iter, iters);
 }
 
 int wtg_restrict_velo(wtg_ctx* fine, int fine_slot, wtg_ctx* coarse, int coarse_slot) {
