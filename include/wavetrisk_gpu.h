@@ -247,6 +247,17 @@ int wtg_jacobi(wtg_ctx* ctx, int u_field, int f_field, double w, int nsweeps);
 /* dp (lin_solve.f90:477-503): sum over the reference's (0,1) node sweep;
  * l2 (lin_solve.f90:433-461) is sqrt(dot(a, a)) */
 int wtg_fs_dot(wtg_ctx* ctx, int a_field, int b_field, double* out);
+/* wtg_fs_dot for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops); no counterpart in the reference.  nd (1..3)
+ * weighted dot products sum_c w_c a[k][c] b[k][c] over the level's
+ * reduction cells, in wtg_fs_dot's order (bit-identical results): a[k], b[k]
+ * are device arrays [ncells] of the context's level, out a device array of nd
+ * doubles.  Queued on the context's stream, no synchronisation; allocates
+ * nothing (no hipMalloc, no free-surface fields).  The block partials live in
+ * one file-scope device array per GPU (12 KiB, 512 partials of 3 sums, kept
+ * until the process ends): calls that overlap on one GPU race.  nd > 3 is
+ * refused. */
+int wtg_fs_dots_noalloc(wtg_ctx* ctx, int nd, const double* const* a, const double* const* b, double* out);
 /* WTG_FS_DIVF := flux_divergence of state slot `slot` (barotropic_2d.f90:326-366) */
 int wtg_flux_divergence(wtg_ctx* ctx, int slot);
 /* WTG_FS_RHS := -eta + dt (theta2 divF + (1 - theta2) divF_old) / ref_density

@@ -84,6 +84,7 @@ SIGNATURES = {
     "wtg_get_stage_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "wtg_state_device_ptrs": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "wtg_synchronize": (C.c_int, [_P]),
+    "wtg_fs_dots_noalloc": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "wtg_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "wtg_fs_set_params": (C.c_int, [_P, _P]),
     "wtg_fs_upload": (C.c_int, [_P, C.c_int, _P]),
@@ -370,6 +371,15 @@ class Context:
 
     def synchronize(self):
         _check(_lib.wtg_synchronize(self._h), self._h, "wtg_synchronize")
+
+    def fs_dots_noalloc(self, a_ptrs, b_ptrs, out_ptr: int):
+        """wtg_fs_dots_noalloc: len(a_ptrs) weighted dot products of device
+        arrays (addresses) into the device array at out_ptr, queued on the
+        context's stream (call synchronize before reading out)."""
+        nd = len(a_ptrs)
+        A = (_P * max(nd, 1))(*[_P(int(p)) for p in a_ptrs])
+        B = (_P * max(nd, 1))(*[_P(int(p)) for p in b_ptrs])
+        _check(_lib.wtg_fs_dots_noalloc(self._h, nd, A, B, _P(int(out_ptr))), self._h, "wtg_fs_dots_noalloc")
 
     # --- free-surface elliptic problem (node fields [ncells]) ---------------
     def fs_set_params(self, dt: float, theta1: float = 1.0, theta2: float = 1.0):

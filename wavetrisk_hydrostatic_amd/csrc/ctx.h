@@ -183,6 +183,9 @@ int wtg_launch_stage_keep(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slo
 int wtg_ghost_fill_layers(wtg_ctx* c, int L, double* m, double* t, double* u);
 // copies the never-filled halo slots of one state to another
 int wtg_copy_frozen_slots(wtg_ctx* c, const wtg_ctx::Slot& from, const wtg_ctx::Slot& to);
+// device address of the file-scope scratch of wtg_fs_dots_noalloc on a
+// device, taken once per device (elliptic.hip); 0 on success
+int wtg_noalloc_scratch_init(int device);
 // split-explicit pieces on one level (trisk.hip): barotropic_correction of
 // a state with the level's WTG_FS_ETA (masters only); out = base + h (trend
 // + theta1 g grad eta_grad) (scalar_star, u_star); u -= theta1 dt g grad

@@ -1504,6 +1504,27 @@ int wtg_mg_eta_transform(wtg_mg* m) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// dot products that allocate nothing (wtg_fs_dots_noalloc): the block
+// partials live in a file-scope device array instead of the context's
+// buffers.  Host code never passes `scratch` itself to a kernel: its device
+// address on each GPU comes from hipGetSymbolAddress, taken once per device
+// when the first context is created there (wtg_noalloc_scratch_init).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[RED_BLOCKS * RED_MAX];  // 12 KiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+int wtg_noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return WTG_E_DEVICE;
+  if (g_scratch[device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return WTG_E_DEVICE;
+  g_scratch[device] = (double*)ptr;
+  return 0;
+}
+
 extern "C" {
 
 int wtg_fs_set_params(wtg_ctx* c, const wtg_fs_params* p) {
@@ -1585,6 +1606,32 @@ int wtg_jacobi(wtg_ctx* c, int u_field, int f_field, double w, int nsweeps) {
   }
   if (u != c->fs[u_field]) RC(launch_copy(c, u, c->fs[u_field]));
   WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int wtg_fs_dots_noalloc(wtg_ctx* c, int nd, const double* const* a, const double* const* b, double* out) {
+  if (!c) return WTG_E_ARG;
+  if (nd < 1 || nd > RED_MAX)
+    return wtg_fail(c, WTG_E_ARG, "%d dot products asked; the scratch holds the partial sums of at most %d", nd,
+                    RED_MAX);
+  if (!a || !b || !out) return wtg_fail(c, WTG_E_ARG, "null argument");
+  for (int k = 0; k < nd; ++k)
+    if (!a[k] || !b[k]) return wtg_fail(c, WTG_E_ARG, "null field of dot product %d", k);
+  if (c->nmembers != 1) return wtg_fail(c, WTG_E_ARG, "ensemble contexts are not supported");
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES || !g_scratch[c->device])
+    return wtg_fail(c, WTG_E_STATE, "no scratch address for device %d", c->device);
+  if (c->nred < 1) return wtg_fail(c, WTG_E_STATE, "the level has no reduction cells");
+  double* part = g_scratch[c->device];
+  const int nb = std::min(RED_BLOCKS, (int)nblk(c->nred));
+  // every partial that the final pass reads is written by the first pass:
+  // nothing in the scratch has to start at zero
+  hipLaunchKernelGGL(fs_dot_kernel, dim3(nb, 1), dim3(FS_NT), 0, c->stream, (const double*)nullptr, c->nred,
+                     c->d_rcell, c->d_rw, nd, a[0], b[0], nd > 1 ? a[1] : a[0], nd > 1 ? b[1] : b[0],
+                     nd > 2 ? a[2] : a[0], nd > 2 ? b[2] : b[0], part, c->info.ncells);
+  WTG_HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(fs_dot_final_kernel, dim3(1, 1), dim3(FS_NT), 0, c->stream, 0, nb, nd, (const double*)part, 0, 1,
+                     2, out, (int)OP_NONE);
+  WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 

@@ -2799,6 +2799,10 @@ int wtg_create_ensemble(const wtg_grid* grid, const wtg_params* params, int nmem
   if (hipGetDevice(&c->device) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipGetDevice failed");
   if (hipStreamCreate(&c->stream) != hipSuccess) return fail(nullptr, WTG_E_DEVICE, "hipStreamCreate failed");
   c->own_stream = c->stream;
+  if (wtg_noalloc_scratch_init(c->device) != 0) {
+    hipStreamDestroy(c->stream);
+    return fail(nullptr, WTG_E_DEVICE, "hipGetSymbolAddress of the dot-product scratch failed on device %d", c->device);
+  }
   wtg_grid_info_get(grid, &c->info);
   c->params = *params;
   c->nslots = nslots;
