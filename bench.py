@@ -62,6 +62,12 @@ def main():
     ap.add_argument("--fs-level", type=int, default=7, help="finest level of the free-surface hierarchy")
     ap.add_argument("--only-free-surface", action="store_true",
                     help="run only the free-surface line and print it (for profiling the solver)")
+    ap.add_argument("--only-multilevel", action="store_true",
+                    help="run only the multi-level line and print it (for profiling the hierarchy's kernels)")
+    ap.add_argument("--ml-case", default="", help="with --only-multilevel: lo,hi,layers of one hierarchy (e.g. 5,8,30)")
+    ap.add_argument("--ml-steps", type=int, default=3, help="steps timed per multi-level leg")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched right-hand-side line and print it (for profiling the batched solver)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -69,6 +75,18 @@ def main():
         sys.exit(2)
     if args.only_free_surface:
         print(json.dumps({"free_surface": free_surface(args.fs_level)}))
+        return
+    if args.only_multilevel:
+        cs = ((tuple(int(v) for v in args.ml_case.split(",")),) if args.ml_case else ((5, 7, 20), (5, 8, 30)))
+        print(json.dumps({"multilevel": multilevel(cs, nsteps=args.ml_steps)}))
+        return
+    if args.only_batched:
+        from wavetrisk_hydrostatic_amd import learned as LC
+        grids = LC.hierarchy_grids(args.fs_level)
+        npatch = 10 * (grids[-1].layout.n // 8) ** 2
+        mlp = LC.load_network(npatch, os.path.join(os.path.dirname(LC.WEIGHTS), f"learned_J{args.fs_level}.npz"))
+        print(json.dumps({"batched_64_rhs": free_surface_batched(grids, mlp, nrhs=64)}))
+        mlp.close()
         return
 
     import numpy as np
@@ -606,8 +624,47 @@ def free_surface(fine: int):
            "draws": draws,
            "fp32_smoothing": fp32,
            "mlp_forward_64_rhs": mlp_line}
+    out["batched_64_rhs"] = free_surface_batched(grids, mlp, nrhs=64)
     prob.close()
     mlp.close()
+    return out
+
+
+def free_surface_batched(grids, mlp, nrhs=64):
+    """Config 4's batch: nrhs right-hand sides (seeded draws, each with its
+    own bathymetry and old free surface) on the same hierarchy, solved to
+    1e-9 in the same launches (an ensemble hierarchy: one launch per operator
+    of the V-cycle for every right-hand side, one network inference for all
+    their patches); plain V-cycles against learned cycles (the same network,
+    its step before every cycle and before the first only).  Wall-clock ms
+    for the whole batch (median of 3 after a warm-up) and per right-hand side;
+    cycles per right-hand side."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import learned as LC
+
+    print(f"bench: free surface, {nrhs} right-hand sides batched", file=sys.stderr, flush=True)
+    prob = LC.Problem(grids, [5000 + k for k in range(nrhs)])
+    big = W.MLP(max_batch=prob.h.npatches * nrhs, seed=1)
+    big.params = mlp.params
+    pv = LC.vcycle_params(1e-9, 40)
+    pv1 = LC.vcycle_params(1e-9, 40)
+    pv1.learned_cycles = 1
+    out = {"rhs": nrhs}
+    for name, meth, par, lrn in (("vcycle", W.SOLVE_VCYCLE, pv, False), ("learned", W.SOLVE_LEARNED, pv, True),
+                                 ("learned_first_cycle", W.SOLVE_LEARNED, pv1, True)):
+        prob.h.set_learned(big if lrn else None)
+        runs = [prob.solve(meth, par) for _ in range(4)][1:]
+        cyc, res = prob.h.member_results()
+        ms = sorted(r[2] for r in runs)[1]
+        out[name] = {"ms_batch": ms, "ms_per_rhs": ms / nrhs, "cycles_mean": float(np.mean(cyc)),
+                     "cycles_max": int(np.max(cyc)), "cycles": [int(c) for c in cyc],
+                     "max_rel_resid": float(np.max(res))}
+    prob.h.set_learned(None)
+    prob.close()
+    big.close()
+    out["note"] = "right-hand sides as ensemble members of one hierarchy; a converged member is left alone"
     return out
 
 

@@ -340,6 +340,18 @@ void wtg_mg_destroy(wtg_mg* mg);
  * on level 0, V-cycles (FMG) or Jacobi sweeps (SRJ) above; rel_resid[l] the
  * final ||f - L u|| / ||f|| per level ([nlevels] each). */
 int wtg_mg_solve(wtg_mg* mg, int method, const wtg_mg_params* p, int* iters, double* rel_resid);
+/* A hierarchy of ensemble contexts (wtg_create_ensemble, the same member
+ * count on every level; no counterpart in the reference, which solves one
+ * right-hand side at a time): wtg_mg_solve with WTG_SOLVE_VCYCLE or
+ * WTG_SOLVE_LEARNED solves every member's problem ([nmembers][ncells] free-
+ * surface fields) in the same launches -- one launch per operator of the
+ * V-cycle, the members' coarse BiCGSTAB batched, one network inference for
+ * all members' patches (the network's batch must hold npatches x nmembers
+ * rows).  Each member's residual is tested after every cycle; a converged
+ * member's fields are left alone from then on, so its iterates and cycle
+ * count are those of its own solve.  iters / rel_resid of the finest level:
+ * the most cycles and the largest residual; per member: */
+int wtg_mg_member_results(const wtg_mg* mg, int* cycles, double* rel_resid);
 /* WTG_SOLVE_VCYCLE: plain V-cycles (v_cycle, lin_solve.f90:102-139) on the
  * finest level from its current u until fine_tol or max_vcycle cycles.
  * WTG_SOLVE_LEARNED (no counterpart in the reference): the same cycles, each

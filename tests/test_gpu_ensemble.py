@@ -45,16 +45,31 @@ def test_ensemble_members_equal_single_runs():
 
 
 def test_ensemble_free_surface_restrictions():
-    """An ensemble's free surface runs through the split step only: the
-    single-field solver entry points and the hierarchy refuse it."""
+    """An ensemble's free surface runs through the split step or a
+    member-batched hierarchy: the single-field solver entry points refuse
+    it; a hierarchy needs the same member count on every level and solves by
+    V-cycles or learned cycles only (FMG and SRJ follow one right-hand side's
+    per-level decisions)."""
     g = W.Grid(level=4, min_level=4, domain_level=1)
+    g5 = W.Grid(level=5, min_level=4, domain_level=1)
     ens = W.Context(g, nlayers=2, members=2)
     ens.set_topography(np.full(g.layout.ncells, -1000.0))
     ens.fs_set_params(100.0)
     with pytest.raises(W.WtgError, match="ensemble"):
         ens.elliptic_apply(W.FS_ETA, W.FS_RHS)
-    with pytest.raises(W.WtgError, match="ensemble"):
-        W.Hierarchy([ens])
+    three = W.Context(g5, nlayers=1, members=3)
+    three.set_topography(np.full(g5.layout.ncells, -1000.0))
+    three.fs_set_params(100.0)
+    with pytest.raises(W.WtgError, match="members"):
+        W.Hierarchy([ens, three])
+    h = W.Hierarchy([ens])
+    try:
+        with pytest.raises(W.WtgError, match="ensemble hierarchy"):
+            h.solve(W.SOLVE_FMG)
+        with pytest.raises(W.WtgError, match="ensemble hierarchy"):
+            h.set_precision(W.PRECISION_FP32_BULK)
+    finally:
+        h.close()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])

@@ -100,6 +100,7 @@ SIGNATURES = {
     "wtg_rk_split": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "wtg_mg_default_params": (None, [_P]),
     "wtg_mg_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "wtg_mg_member_results": (C.c_int, [_P, _P, _P]),
     "wtg_ml_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
     "wtg_ml_destroy": (None, [_P]),
     "wtg_ml_forward": (C.c_int, [_P, C.c_int]),
@@ -488,6 +489,15 @@ class Hierarchy:
         _check(_lib.wtg_mg_rk4_split(self._h, slot, nsteps, method, C.byref(p), C.byref(it)), self._top,
                "wtg_mg_rk4_split")
         return it.value
+
+    def member_results(self):
+        """per member of an ensemble hierarchy: (cycles, relative residual) of
+        the last member-batched solve"""
+        M = self.contexts[-1].members
+        cyc = np.zeros(M, dtype=np.int32)
+        res = np.zeros(M)
+        _check(_lib.wtg_mg_member_results(self._h, _ptr(cyc), _ptr(res)), self._top, "wtg_mg_member_results")
+        return cyc, res
 
     @property
     def npatches(self) -> int:

@@ -53,6 +53,9 @@ struct wtg_ctx {
   wtg_params params{};
   int nslots = 0;
   int nmembers = 1;       // ensemble members, each nlayers layers, one after another in every slot
+  // set by a member-batched hierarchy solve on each of its levels: per
+  // member 1 once that member's cycles have converged (device [nmembers])
+  const double* mg_mdone = nullptr;
   bool split_fused = true;  // FP64 split-explicit stages: trend and AXPY in one launch
   int stage_version = 3;  // WTG_STAGE_KERNEL: 1 = tile loop, 2 = one thread per region cell, 3 = layer pairs
   bool topo_set = false;

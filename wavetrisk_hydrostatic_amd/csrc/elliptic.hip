@@ -53,6 +53,14 @@ __device__ __forceinline__ Tp* mem(Tp* p, int64_t ms) {
   return p ? p + (int64_t)blockIdx.y * ms : p;
 }
 
+// Member-batched multigrid (a hierarchy of ensemble contexts, blockIdx.y =
+// member): a member whose cycles have converged (mdone[member] != 0) keeps
+// its fields -- kernels that write the next buffer of a solution copy it
+// through, the others return; null mdone: no member has converged
+__device__ __forceinline__ bool mdone_of(const double* __restrict__ mdone) {
+  return mdone && mdone[blockIdx.y] != 0.0;
+}
+
 // scalar steps of the solvers (scalar_op below)
 enum ScalOp { OP_ALPH = 0, OP_OMGA, OP_ERR, OP_BETA, OP_NRMF, OP_LOOP_INIT, OP_ERR_BETA, OP_MRSTEP, OP_NONE };
 
@@ -140,12 +148,18 @@ __global__ void fs_diag_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, 
 // arithmetic is FP64 either way)
 // Zero: the sweep starts from u = 0 (L 0 = 0), u is not read
 template <class F, bool Zero = false>
-__global__ void fs_jacobi_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const F* __restrict__ u,
-                                 const F* __restrict__ f, const double* __restrict__ eo, double grav, double coef,
-                                 double dt2, double th12, double w, F* __restrict__ out) {
+__global__ void fs_jacobi_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl, FsGeo g,
+                                 const F* __restrict__ u, const F* __restrict__ f, const double* __restrict__ eo,
+                                 double grav, double coef, double dt2, double th12, double w, F* __restrict__ out,
+                                 int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  g.topo = mem(g.topo, ms), u = mem(u, ms), f = mem(f, ms), eo = mem(eo, ms), out = mem(out, ms);
   const int64_t c = ncl[2 * t + 1];
+  if (mdone_of(mdone)) {  // converged member: the sweep leaves u as it is
+    out[ncl[2 * t]] = Zero ? (F)0.0 : u[c];
+    return;
+  }
   const double dg = ell_diag_at(g, eo, grav, dt2, th12, c);
   if (Zero) {
     out[ncl[2 * t]] = (F)(0.0 + w * (f[c] - 0.0) / dg);
@@ -157,25 +171,34 @@ __global__ void fs_jacobi_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g
 
 // residual (lin_solve.f90:563-618): out = f - L u
 template <class Fu, class Ff, class Fo>
-__global__ void fs_residual_kernel(int n, const int64_t* __restrict__ ncl, FsGeo g, const Fu* __restrict__ u,
-                                   const Ff* __restrict__ f, const double* __restrict__ eo, double grav,
-                                   double coef, Fo* __restrict__ out) {
+__global__ void fs_residual_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl, FsGeo g,
+                                   const Fu* __restrict__ u, const Ff* __restrict__ f,
+                                   const double* __restrict__ eo, double grav, double coef, Fo* __restrict__ out,
+                                   int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || mdone_of(mdone)) return;
+  g.topo = mem(g.topo, ms), u = mem(u, ms), f = mem(f, ms), eo = mem(eo, ms), out = mem(out, ms);
   const int64_t c = ncl[2 * t + 1];
   out[ncl[2 * t]] = (Fo)(f[c] - ell_apply_at(g, u, eo, grav, coef, c));
 }
 
 // lc (lin_solve.f90:520-561): out = a1 s1 + a2 s2, a = k (i < 0) or k * scal[i]
 template <class F1, class F2, class Fo>
-__global__ void fs_lc_kernel(const double* __restrict__ gate, int n, const int64_t* __restrict__ ncl, double k1, int i1,
-                             const F1* __restrict__ s1, double k2, int i2, const F2* __restrict__ s2,
-                             const double* __restrict__ scal, Fo* __restrict__ out) {
+__global__ void fs_lc_kernel(const double* __restrict__ gate, const double* __restrict__ mdone, int n,
+                             const int64_t* __restrict__ ncl, double k1, int i1, const F1* __restrict__ s1, double k2,
+                             int i2, const F2* __restrict__ s2, const double* __restrict__ scal, Fo* __restrict__ out,
+                             int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  gate = mem(gate, SC_NSCAL);
   if (t >= n || gated(gate)) return;
+  scal = mem(scal, SC_NSCAL), s1 = mem(s1, ms), s2 = mem(s2, ms), out = mem(out, ms);
+  const int64_t c = ncl[2 * t + 1];
+  if (mdone_of(mdone)) {  // converged member: out is its first operand (u of u + a e)
+    out[ncl[2 * t]] = (Fo)s1[c];
+    return;
+  }
   const double a1 = i1 < 0 ? k1 : k1 * scal[i1];
   const double a2 = i2 < 0 ? k2 : k2 * scal[i2];
-  const int64_t c = ncl[2 * t + 1];
   out[ncl[2 * t]] = (Fo)(a1 * s1[c] + a2 * s2[c]);
 }
 
@@ -191,9 +214,10 @@ __global__ void fs_ghost_copy_kernel(int nmaster, int n, const int64_t* __restri
 // out = src (master and halo cells alike), converted to the output type
 template <class Fi, class Fo>
 __global__ void fs_copy_kernel(int n, const int64_t* __restrict__ ncl, const Fi* __restrict__ s,
-                               Fo* __restrict__ out) {
+                               Fo* __restrict__ out, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  s = mem(s, ms), out = mem(out, ms);
   out[ncl[2 * t]] = (Fo)s[ncl[2 * t + 1]];
 }
 
@@ -573,10 +597,11 @@ __global__ __launch_bounds__(FS_NT) void fs_bicg_res_kernel(
 // Compute_scalar_wavelets (wavelet.f90:648-670) over the fine compute list:
 // wc = s - Interp_node at midpoints, 0 at coincident nodes
 template <class F>
-__global__ void mg_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ s,
-                                  F* __restrict__ wc) {
+__global__ void mg_wavelet_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl, MgGeo g,
+                                  const F* __restrict__ s, F* __restrict__ wc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || mdone_of(mdone)) return;
+  s = mem(s, g.NCf), wc = mem(wc, g.NCf);
   const int64_t x = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
@@ -585,10 +610,11 @@ __global__ void mg_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo 
 
 // Restrict_scalar over the coarse compute list: s_c = s_f(child) + lift * hex_inv_c
 template <class F>
-__global__ void mg_restrict_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ sf,
-                                   const F* __restrict__ wc, F* __restrict__ sc) {
+__global__ void mg_restrict_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl, MgGeo g,
+                                   const F* __restrict__ sf, const F* __restrict__ wc, F* __restrict__ sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || mdone_of(mdone)) return;
+  sf = mem(sf, g.NCf), wc = mem(wc, g.NCf), sc = mem(sc, g.NCc);
   const int64_t p = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(p, g.Sc, lz, i, j);
@@ -614,10 +640,12 @@ __global__ void mg_restrict_ss_kernel(int n, const int64_t* __restrict__ ncl, Mg
 // s_f = s_c(parent) - lift * hex_inv_c
 // ZeroW: all wavelets zero (prol_fun), wc is not read
 template <class F, bool ZeroW = false>
-__global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ sc,
-                                       const F* __restrict__ wc, F* __restrict__ sf) {
+__global__ void mg_prolong_even_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl,
+                                       MgGeo g, const F* __restrict__ sc, const F* __restrict__ wc,
+                                       F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || mdone_of(mdone)) return;
+  sc = mem(sc, g.NCc), wc = mem(wc, g.NCf), sf = mem(sf, g.NCf);
   const int64_t x = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
@@ -636,15 +664,43 @@ __global__ void mg_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, M
 // midpoints: s_f = Interp_node(coincident s_f) + wc.  Reads only coincident
 // cells and writes only midpoints, so it runs in place after the even pass.
 template <class F, bool ZeroW = false>
-__global__ void mg_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, const F* __restrict__ wc,
-                                      F* __restrict__ sf) {
+__global__ void mg_prolong_odd_kernel(const double* __restrict__ mdone, int n, const int64_t* __restrict__ ncl,
+                                      MgGeo g, const F* __restrict__ wc, F* __restrict__ sf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  if (t >= n || mdone_of(mdone)) return;
+  wc = mem(wc, g.NCf), sf = mem(sf, g.NCf);
   const int64_t x = ncl[2 * t + 1];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
   if (!((i | j) & 1)) return;
   sf[ncl[2 * t]] = ZeroW ? (F)mg_interp(g, sf, x, i, j) : (F)(mg_interp(g, sf, x, i, j) + wc[x]);
+}
+
+// member-batched cycles: the converged members' BiCGSTAB starts converged
+// (SC_DONE), so its gated iterations leave their coarse fields alone
+__global__ void mg_mark_done_kernel(int M, const double* __restrict__ mdone, double* __restrict__ scal) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < M && mdone[k] != 0.0) scal[(int64_t)k * SC_NSCAL + SC_DONE] = 1.0;
+}
+
+// member-batched cycles: relative residual sqrt(dot) / nrm of every member
+// (dot in its SC_T0); a member converges once at or below tol (sticky)
+__global__ void mg_member_test_kernel(int M, const double* __restrict__ scal, double tol, double* __restrict__ stat,
+                                      int init_nrm) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  double* done = stat;
+  double* err = stat + M;
+  double* nrm = stat + 2 * M;
+  const double d = scal[(int64_t)k * SC_NSCAL + SC_T0];
+  if (init_nrm) {  // nrm_f = l2(f), 1 if zero (lin_solve.f90:73)
+    const double nf = sqrt(d);
+    nrm[k] = nf == 0.0 ? 1.0 : nf;
+    done[k] = 0.0;
+    return;
+  }
+  err[k] = sqrt(d) / nrm[k];
+  if (done[k] == 0.0 && err[k] <= tol) done[k] = 1.0;
 }
 
 // out := v on every cell of the array (work fields; not a memset, so the
@@ -668,9 +724,12 @@ template <bool BF16>
 __global__ void lc_features_kernel(int npatch, const int64_t* __restrict__ cells, const double* __restrict__ r,
                                    const double* __restrict__ topo, const double* __restrict__ eo,
                                    const double* __restrict__ fnode, double* __restrict__ sigma,
-                                   float* __restrict__ outf, uint16_t* __restrict__ outb) {
+                                   float* __restrict__ outf, uint16_t* __restrict__ outb, int64_t ms) {
   const int p = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
   if (p >= npatch) return;
+  // member blockIdx.y: its fields at member * ms, its rows after the previous member's
+  r = mem(r, ms), topo = mem(topo, ms), eo = mem(eo, ms), sigma = mem(sigma, (int64_t)npatch);
+  outf = mem(outf, (int64_t)npatch * 192), outb = mem(outb, (int64_t)npatch * 192);
   const int64_t c = cells[(int64_t)p * 64 + lane];
   const double v = r[c];
   double ss = v * v;
@@ -702,9 +761,10 @@ __global__ void lc_features_kernel(int npatch, const int64_t* __restrict__ cells
 // halo copies are written in the same pass; slot[cell] = patch * 64 + node
 __global__ void lc_scatter_kernel(int n, const int64_t* __restrict__ ncl, const int* __restrict__ slot,
                                   const float* __restrict__ y, const double* __restrict__ sigma,
-                                  double* __restrict__ e) {
+                                  double* __restrict__ e, int npatch, int64_t ms) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  y = mem(y, (int64_t)npatch * 64), sigma = mem(sigma, (int64_t)npatch), e = mem(e, ms);
   const int s = slot[ncl[2 * t + 1]];
   e[ncl[2 * t]] = s >= 0 ? (double)y[s] * sigma[s / 64] : 0.0;
 }
@@ -769,17 +829,18 @@ int launch_apply(wtg_ctx* c, const double* q, double* out, bool gate = false, do
 
 template <class Fu, class Ff, class Fo>
 int launch_residual(wtg_ctx* c, const Fu* u, const Ff* f, Fo* out) {
-  hipLaunchKernelGGL((fs_residual_kernel<Fu, Ff, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c),
-                     u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), out);
+  hipLaunchKernelGGL((fs_residual_kernel<Fu, Ff, Fo>), dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream,
+                     c->mg_mdone, c->nncl, c->d_ncl, fs_geo(c), u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c),
+                     out, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
 template <class F, bool Zero = false>
 int launch_jacobi(wtg_ctx* c, const F* u, const F* f, double w, F* out) {
-  hipLaunchKernelGGL((fs_jacobi_kernel<F, Zero>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, fs_geo(c), u,
-                     f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c), c->fs_dt * c->fs_dt,
-                     c->fs_theta1 * c->fs_theta2, w, out);
+  hipLaunchKernelGGL((fs_jacobi_kernel<F, Zero>), dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream,
+                     c->mg_mdone, c->nncl, c->d_ncl, fs_geo(c), u, f, c->fs[FS_ETA_OLD], c->params.grav, fs_coef(c),
+                     c->fs_dt * c->fs_dt, c->fs_theta1 * c->fs_theta2, w, out, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -787,15 +848,17 @@ int launch_jacobi(wtg_ctx* c, const F* u, const F* f, double w, F* out) {
 template <class F1, class F2, class Fo>
 int launch_lc(wtg_ctx* c, double k1, int i1, const F1* s1, double k2, int i2, const F2* s2, Fo* out,
               bool gate = false) {
-  hipLaunchKernelGGL((fs_lc_kernel<F1, F2, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
-                     c->nncl, c->d_ncl, k1, i1, s1, k2, i2, s2, c->d_scal, out);
+  hipLaunchKernelGGL((fs_lc_kernel<F1, F2, Fo>), dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream,
+                     gate ? c->d_scal : nullptr, c->mg_mdone, c->nncl, c->d_ncl, k1, i1, s1, k2, i2, s2, c->d_scal, out,
+                     c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
 
 template <class Fi, class Fo>
 int launch_copy(wtg_ctx* c, const Fi* s, Fo* out) {
-  hipLaunchKernelGGL((fs_copy_kernel<Fi, Fo>), dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, s, out);
+  hipLaunchKernelGGL((fs_copy_kernel<Fi, Fo>), dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream, c->nncl,
+                     c->d_ncl, s, out, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -876,6 +939,11 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
   RC(launch_apply(c, p, Ap, false, As));  // Ap = As = L p
   RC(launch_dots(c, 1, res0, res, SC_RHO));
   RC(launch_scalar(c, OP_LOOP_INIT, false, tol));
+  if (c->mg_mdone) {
+    hipLaunchKernelGGL(mg_mark_done_kernel, dim3((c->nmembers + 63) / 64), dim3(64), 0, c->stream, c->nmembers,
+                       c->mg_mdone, c->d_scal);
+    WTG_HIPCHK(c, hipGetLastError());
+  }
   int queued = 0;
   bool done = false;
   // one chunk of gated iterations; the pointer swap of p returns to its
@@ -1044,6 +1112,12 @@ struct wtg_mg {
   // WTG_PRECISION_FP32_BULK: the V-cycles' correction (smoothing, transfers)
   // in FP32 fields; residuals of the solution and its update stay FP64
   int precision = WTG_PRECISION_FP64;
+  // a hierarchy of ensemble contexts (nmembers M on every level): each
+  // member's problem in the same launches, members gated once converged
+  int M = 1;
+  double* d_mstat = nullptr;  // [3][M]: converged flag, relative residual, l2 of f
+  std::vector<int> m_cycles;  // per member, last cycles solve
+  std::vector<double> m_resid;
 };
 
 namespace {
@@ -1059,12 +1133,12 @@ int mg_err(wtg_mg* m, int rc, int l) {
 MgGeo mg_geo(const wtg_mg* m, int l) {
   const wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
   return MgGeo{m->ovl[l], f->d_hinv, c->d_hinv, f->info.ncells, f->info.S, f->info.n, c->info.S,
-               f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole};
+               f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole, c->info.ncells};
 }
 
 template <class F>
 int mg_fill(wtg_ctx* c, F* a, double v) {
-  const int64_t n = c->info.ncells;
+  const int64_t n = c->info.ncells * c->nmembers;
   hipLaunchKernelGGL(mg_fill_kernel<F>, dim3((unsigned)((n + FS_NT - 1) / FS_NT)), dim3(FS_NT), 0, c->stream, n, v, a);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
@@ -1075,10 +1149,11 @@ template <class F>
 int mg_forward(wtg_mg* m, int l, const F* s, F* wc, F* s_c) {
   wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
   const MgGeo g = mg_geo(m, l);
-  hipLaunchKernelGGL(mg_wavelet_kernel<F>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s, wc);
+  hipLaunchKernelGGL(mg_wavelet_kernel<F>, dim3(nblk(f->nncl), f->nmembers), dim3(FS_NT), 0, f->stream, f->mg_mdone,
+                     f->nncl, f->d_ncl, g, s, wc);
   WTG_HIPCHK(f, hipGetLastError());
-  hipLaunchKernelGGL(mg_restrict_kernel<F>, dim3(nblk(c->nncl)), dim3(FS_NT), 0, f->stream, c->nncl, c->d_ncl, g, s, wc,
-                     s_c);
+  hipLaunchKernelGGL(mg_restrict_kernel<F>, dim3(nblk(c->nncl), f->nmembers), dim3(FS_NT), 0, f->stream, f->mg_mdone,
+                     c->nncl, c->d_ncl, g, s, wc, s_c);
   WTG_HIPCHK(f, hipGetLastError());
   return 0;
 }
@@ -1089,11 +1164,11 @@ template <class F, bool ZeroW = false>
 int mg_inverse(wtg_mg* m, int l, const F* s_c, const F* wc, F* s_f) {
   wtg_ctx* f = m->lv[l];
   const MgGeo g = mg_geo(m, l);
-  hipLaunchKernelGGL((mg_prolong_even_kernel<F, ZeroW>), dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, s_c,
-                     wc, s_f);
+  hipLaunchKernelGGL((mg_prolong_even_kernel<F, ZeroW>), dim3(nblk(f->nncl), f->nmembers), dim3(FS_NT), 0, f->stream,
+                     f->mg_mdone, f->nncl, f->d_ncl, g, s_c, wc, s_f);
   WTG_HIPCHK(f, hipGetLastError());
-  hipLaunchKernelGGL((mg_prolong_odd_kernel<F, ZeroW>), dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g, wc,
-                     s_f);
+  hipLaunchKernelGGL((mg_prolong_odd_kernel<F, ZeroW>), dim3(nblk(f->nncl), f->nmembers), dim3(FS_NT), 0, f->stream,
+                     f->mg_mdone, f->nncl, f->d_ncl, g, wc, s_f);
   WTG_HIPCHK(f, hipGetLastError());
   return 0;
 }
@@ -1201,6 +1276,7 @@ int vcycle(wtg_mg* m, std::vector<MgLv>& V, int l, const wtg_mg_params& P, doubl
   RC(launch_lc(V[l].c, 1.0, -1, V[l].u, 1.0, -1, V[l].corr, V[l].u_alt));
   std::swap(V[l].u, V[l].u_alt);
   RC(lv_jacobi(V[l], V[l].u, V[l].u_alt, V[l].f, 1.0, P.post_iter));
+  if (m->M > 1) return 0;  // member-batched cycles test every member's residual in the caller
   return lv_res_err(V[l], V[l].f, V[l].u, nrm_f, err);
 }
 
@@ -1341,12 +1417,15 @@ int lc_features(wtg_mg* m, const double* r, bool to_mlp) {
   int maxb = 0;
   if (to_mlp) wtg_mlp_buffers(m->mlp, &x0, &y, &maxb);
   const unsigned nb = (unsigned)((m->npatch + 3) / 4);
+  // an ensemble hierarchy: the rows of member k follow member k - 1's
+  const int64_t ms = c->info.ncells;
   if (to_mlp)
-    hipLaunchKernelGGL(lc_features_kernel<true>, dim3(nb), dim3(256), 0, c->stream, m->npatch, m->d_cells, r, c->d_topo,
-                       c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, (float*)nullptr, x0);
+    hipLaunchKernelGGL(lc_features_kernel<true>, dim3(nb, c->nmembers), dim3(256), 0, c->stream, m->npatch, m->d_cells,
+                       r, c->d_topo, c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, (float*)nullptr, x0, ms);
   else
-    hipLaunchKernelGGL(lc_features_kernel<false>, dim3(nb), dim3(256), 0, c->stream, m->npatch, m->d_cells, r,
-                       c->d_topo, c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, m->d_feat, (uint16_t*)nullptr);
+    hipLaunchKernelGGL(lc_features_kernel<false>, dim3(nb, c->nmembers), dim3(256), 0, c->stream, m->npatch,
+                       m->d_cells, r, c->d_topo, c->fs[FS_ETA_OLD], m->d_fnode, m->d_sigma, m->d_feat,
+                       (uint16_t*)nullptr, ms);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1354,15 +1433,19 @@ int lc_features(wtg_mg* m, const double* r, bool to_mlp) {
 // e := ghost-filled learned correction of residual r on the finest level
 int lc_correction(wtg_mg* m, const double* r, double* e) {
   wtg_ctx* c = m->lv[m->nl - 1];
-  RC(lc_features(m, r, true));
-  if (int rc = wtg_mlp_infer_on(m->mlp, c->stream, m->npatch))
-    return wtg_fail(c, rc, "learned correction: %s", wtg_mlp_error_text(m->mlp));
   uint16_t* x0;
   float* y;
   int maxb;
   wtg_mlp_buffers(m->mlp, &x0, &y, &maxb);
-  hipLaunchKernelGGL(lc_scatter_kernel, dim3(nblk(c->nncl)), dim3(FS_NT), 0, c->stream, c->nncl, c->d_ncl, m->d_slot,
-                     y, m->d_sigma, e);
+  const int rows = m->npatch * c->nmembers;  // every member's patches in one inference
+  if (rows > maxb)
+    return wtg_fail(c, WTG_E_ARG, "the network's batch (%d rows) is smaller than the %d patches of the members", maxb,
+                    rows);
+  RC(lc_features(m, r, true));
+  if (int rc = wtg_mlp_infer_on(m->mlp, c->stream, rows))
+    return wtg_fail(c, rc, "learned correction: %s", wtg_mlp_error_text(m->mlp));
+  hipLaunchKernelGGL(lc_scatter_kernel, dim3(nblk(c->nncl), c->nmembers), dim3(FS_NT), 0, c->stream, c->nncl,
+                     c->d_ncl, m->d_slot, y, m->d_sigma, e, m->npatch, c->info.ncells);
   WTG_HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1404,6 +1487,82 @@ int mg_cycles(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, doubl
   return mg_finish(V);
 }
 
+// mg_cycles for a hierarchy of ensemble contexts: every member's V-cycles
+// (and learned steps) in the same launches.  After each cycle every member's
+// residual is tested on the device; a member at or below fine_tol is
+// converged and from then on every kernel of the cycle leaves its fields as
+// they are (its solution buffers are copied through), so each member's
+// iterates and cycle count are those of its own solve.  iters[top] / res_out
+// [top]: the most cycles and the largest residual over the members; per
+// member in m_cycles / m_resid.
+int mg_cycles_members(wtg_mg* m, const wtg_mg_params& P, bool learned, int* iters, double* res_out) {
+  std::vector<MgLv> V = mg_levels(m);
+  const int l = m->nl - 1, M = m->M;
+  wtg_ctx* c = V[l].c;
+  for (int j = 0; j < m->nl; ++j) iters[j] = 0, res_out[j] = 0.0;
+  double* done = m->d_mstat;
+  const unsigned tb = (unsigned)((M + 63) / 64);
+  auto test = [&](int init) -> int {
+    RC(launch_dots(c, 1, init ? V[l].f : c->fs[FS_W0], init ? V[l].f : c->fs[FS_W0], SC_T0));
+    hipLaunchKernelGGL(mg_member_test_kernel, dim3(tb), dim3(64), 0, c->stream, M, c->d_scal, P.fine_tol,
+                       m->d_mstat, init);
+    WTG_HIPCHK(c, hipGetLastError());
+    return 0;
+  };
+  std::vector<double> h(2 * (size_t)M);
+  auto read = [&]() -> int {
+    WTG_HIPCHK(c, hipMemcpyAsync(h.data(), m->d_mstat, 2 * M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  RC(test(1));  // nrm_f per member, every member active
+  for (auto* x : m->lv) x->mg_mdone = done;
+  auto finish = [&](int rc) {
+    for (auto* x : m->lv) x->mg_mdone = nullptr;
+    return rc;
+  };
+  // r = f - L u of every member into FS_W0, its relative residual
+  if (int rc = launch_residual(c, V[l].u, V[l].f, c->fs[FS_W0])) return finish(rc);
+  if (int rc = test(0)) return finish(rc);
+  if (int rc = read()) return finish(rc);
+  m->m_cycles.assign(M, 0);
+  m->m_resid.assign(h.begin() + M, h.end());
+  auto all_done = [&]() {
+    for (int k = 0; k < M; ++k)
+      if (h[k] == 0.0) return false;
+    return true;
+  };
+  for (int it = 1; it <= P.max_vcycle && !all_done(); ++it) {
+    const double* r_top = c->fs[FS_W0];
+    const bool step = learned && it <= P.learned_cycles;
+    if (step) {
+      double *r = c->fs[FS_W0], *e = c->fs[FS_W0 + 2], *w = c->fs[FS_W0 + 3];
+      if (int rc = lc_correction(m, r, e)) return finish(rc);
+      if (int rc = launch_apply(c, e, w)) return finish(rc);
+      if (int rc = launch_dots_g(c, false, OP_MRSTEP, 2, r, w, SC_T0, w, w, SC_T1)) return finish(rc);
+      if (int rc = launch_lc(c, 1.0, -1, V[l].u, 1.0, SC_MR, e, V[l].u_alt)) return finish(rc);
+      std::swap(V[l].u, V[l].u_alt);
+    }
+    double unused;
+    if (int rc = vcycle(m, V, l, P, 1.0, &unused, step ? nullptr : r_top)) return finish(rc);
+    if (int rc = launch_residual(c, V[l].u, V[l].f, c->fs[FS_W0])) return finish(rc);
+    if (int rc = test(0)) return finish(rc);
+    const std::vector<double> before(h.begin(), h.begin() + M);
+    if (int rc = read()) return finish(rc);
+    for (int k = 0; k < M; ++k)
+      if (before[k] == 0.0) {  // active during this cycle
+        m->m_cycles[k] = it;
+        m->m_resid[k] = h[M + k];
+      }
+  }
+  for (int k = 0; k < M; ++k) {
+    iters[l] = std::max(iters[l], m->m_cycles[k]);
+    res_out[l] = std::max(res_out[l], m->m_resid[k]);
+  }
+  // the converged members' coarse BiCGSTAB flags stay set until its next start
+  return finish(mg_finish(V));
+}
+
 // the free-surface solve of a split-explicit step on the hierarchy: eta^n
 // restricted to every coarser level as its operator's eta_old (the
 // reference's coarse levels hold restrictions of sol), then cycles on the
@@ -1421,7 +1580,10 @@ int mg_split_solve(void* user, wtg_ctx* c, int* iters) {
     RC(mg_forward(m, l, m->lv[l]->fs[FS_ETA_OLD], m->lv[l]->fs[MG_WC], m->lv[l - 1]->fs[FS_ETA_OLD]));
   std::vector<int> it(m->nl);
   std::vector<double> res(m->nl);
-  RC(mg_cycles(m, s->P, s->learned, it.data(), res.data()));
+  if (m->M > 1)
+    RC(mg_cycles_members(m, s->P, s->learned, it.data(), res.data()));
+  else
+    RC(mg_cycles(m, s->P, s->learned, it.data(), res.data()));
   *iters = it[m->nl - 1];
   (void)c;
   return 0;
@@ -1493,8 +1655,8 @@ int wtg_mg_eta_transform(wtg_mg* m) {
   for (int l = 1; l < m->nl; ++l) {
     wtg_ctx* f = m->lv[l];
     const MgGeo g = mg_geo(m, l);
-    hipLaunchKernelGGL(mg_wavelet_kernel<double>, dim3(nblk(f->nncl)), dim3(FS_NT), 0, f->stream, f->nncl, f->d_ncl, g,
-                       (const double*)f->fs[FS_ETA], f->fs[MG_WC]);
+    hipLaunchKernelGGL(mg_wavelet_kernel<double>, dim3(nblk(f->nncl), f->nmembers), dim3(FS_NT), 0, f->stream,
+                       (const double*)nullptr, f->nncl, f->d_ncl, g, (const double*)f->fs[FS_ETA], f->fs[MG_WC]);
     WTG_HIPCHK(f, hipGetLastError());
   }
   for (int l = 1; l < m->nl; ++l)
@@ -1695,7 +1857,8 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
     if (l && grids[l]->lv.level != grids[l - 1]->lv.level + 1)
       return wtg_fail(nullptr, WTG_E_ARG, "levels %d and %d are not consecutive", l - 1, l);
     if (c->device != levels[0]->device) return wtg_fail(nullptr, WTG_E_ARG, "level %d is on another device", l);
-    if (c->nmembers != 1) return wtg_fail(nullptr, WTG_E_ARG, "level %d is an ensemble context", l);
+    if (c->nmembers != levels[0]->nmembers)
+      return wtg_fail(nullptr, WTG_E_ARG, "level %d has %d members, level 0 %d", l, c->nmembers, levels[0]->nmembers);
     if (!c->topo_set || !c->fs_params_set)
       return wtg_fail(nullptr, WTG_E_STATE, "level %d: topography and free-surface parameters must be set", l);
     for (int k = 0; k < l; ++k)
@@ -1703,6 +1866,7 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
   }
   auto m = std::make_unique<wtg_mg>();
   m->nl = nlevels;
+  m->M = levels[0]->nmembers;
   m->lv.assign(levels, levels + nlevels);
   m->ovl.assign(nlevels, nullptr);
   wtg_ctx* top = levels[nlevels - 1];
@@ -1752,7 +1916,7 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
     for (size_t c = 0; c < NC; ++c) fn[c] = fl.X[2 * NC + c] / fl.radius;
     bool ok = hipMalloc((void**)&m->d_cells, cells.size() * sizeof(int64_t)) == hipSuccess &&
               hipMalloc((void**)&m->d_fnode, NC * sizeof(double)) == hipSuccess &&
-              hipMalloc((void**)&m->d_sigma, (size_t)m->npatch * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&m->d_sigma, (size_t)m->npatch * m->M * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&m->d_feat, (size_t)m->npatch * 192 * sizeof(float)) == hipSuccess &&
               hipMalloc((void**)&m->d_slot, NC * sizeof(int)) == hipSuccess;
     ok = ok && hipMemcpy(m->d_cells, cells.data(), cells.size() * sizeof(int64_t), hipMemcpyHostToDevice) == hipSuccess &&
@@ -1762,6 +1926,11 @@ int wtg_mg_create(const wtg_grid* const* grids, wtg_ctx* const* levels, int nlev
       wtg_mg_destroy(m.release());
       return wtg_fail(nullptr, WTG_E_DEVICE, "patch table upload failed");
     }
+  }
+  if (hipMalloc((void**)&m->d_mstat, 3 * (size_t)m->M * sizeof(double)) != hipSuccess) {
+    m->d_mstat = nullptr;
+    wtg_mg_destroy(m.release());
+    return wtg_fail(nullptr, WTG_E_DEVICE, "hipMalloc of the member status failed");
   }
   for (auto* c : m->lv) c->stream = top->own_stream;
   *out = m.release();
@@ -1774,7 +1943,8 @@ void wtg_mg_destroy(wtg_mg* m) {
   for (auto* c : m->lv) c->stream = c->own_stream;
   for (double* a : m->ovl)
     if (a) hipFree(a);
-  for (void* a : {(void*)m->d_cells, (void*)m->d_fnode, (void*)m->d_sigma, (void*)m->d_feat, (void*)m->d_slot})
+  for (void* a : {(void*)m->d_cells, (void*)m->d_fnode, (void*)m->d_sigma, (void*)m->d_feat, (void*)m->d_slot,
+                  (void*)m->d_mstat})
     if (a) hipFree(a);
   delete m;
 }
@@ -1791,6 +1961,13 @@ int wtg_mg_solve(wtg_mg* m, int method, const wtg_mg_params* p, int* iters, doub
   wtg_ctx* top = m->lv[m->nl - 1];
   if (!p || !iters || !rel_resid) return wtg_fail(top, WTG_E_ARG, "null argument");
   if (int rc = mg_check_params(top, p)) return rc;
+  if (m->M > 1) {  // an ensemble hierarchy: every member's cycles batched
+    if (method != WTG_SOLVE_VCYCLE && method != WTG_SOLVE_LEARNED)
+      return wtg_fail(top, WTG_E_ARG, "an ensemble hierarchy solves with WTG_SOLVE_VCYCLE or WTG_SOLVE_LEARNED");
+    if (method == WTG_SOLVE_LEARNED && !m->mlp)
+      return wtg_fail(top, WTG_E_STATE, "no network attached (wtg_mg_set_learned)");
+    return mg_cycles_members(m, *p, method == WTG_SOLVE_LEARNED, iters, rel_resid);
+  }
   if (method == WTG_SOLVE_FMG) return mg_fmg(m, *p, iters, rel_resid);
   if (method == WTG_SOLVE_SRJ) return mg_srj(m, *p, iters, rel_resid);
   if (method == WTG_SOLVE_VCYCLE) return mg_cycles(m, *p, false, iters, rel_resid);
@@ -1799,6 +1976,14 @@ int wtg_mg_solve(wtg_mg* m, int method, const wtg_mg_params* p, int* iters, doub
     return mg_cycles(m, *p, true, iters, rel_resid);
   }
   return wtg_fail(top, WTG_E_ARG, "unknown multigrid method %d", method);
+}
+
+int wtg_mg_member_results(const wtg_mg* m, int* cycles, double* rel_resid) {
+  if (!m || !cycles || !rel_resid) return WTG_E_ARG;
+  if ((int)m->m_cycles.size() != m->M)
+    return wtg_fail(m->lv[m->nl - 1], WTG_E_STATE, "no member-batched solve has run on this hierarchy");
+  for (int k = 0; k < m->M; ++k) cycles[k] = m->m_cycles[k], rel_resid[k] = m->m_resid[k];
+  return 0;
 }
 
 int wtg_mg_forward(wtg_mg* m, int fine, int s_field, int wc_field, int coarse_field) {
@@ -1848,6 +2033,8 @@ int wtg_mg_set_precision(wtg_mg* m, int precision) {
   wtg_ctx* top = m->lv[m->nl - 1];
   if (precision != WTG_PRECISION_FP64 && precision != WTG_PRECISION_FP32_BULK)
     return wtg_fail(top, WTG_E_ARG, "unknown precision mode %d", precision);
+  if (precision == WTG_PRECISION_FP32_BULK && m->M > 1)
+    return wtg_fail(top, WTG_E_ARG, "FP32 smoothing is not available for an ensemble hierarchy");
   if (precision == WTG_PRECISION_FP32_BULK)
     if (int rc = fs32_ready(m)) return rc;
   m->precision = precision;
@@ -1863,8 +2050,9 @@ int wtg_mg_set_learned(wtg_mg* m, wtg_mlp* mlp) {
     float* y;
     int maxb;
     wtg_mlp_buffers(mlp, &x0, &y, &maxb);
-    if (maxb < m->npatch)
-      return wtg_fail(top, WTG_E_ARG, "the network takes %d rows, the finest level has %d patches", maxb, m->npatch);
+    if (maxb < m->npatch * m->M)
+      return wtg_fail(top, WTG_E_ARG, "the network takes %d rows, the finest level has %d patches (all members)",
+                      maxb, m->npatch * m->M);
   }
   m->mlp = mlp;
   return 0;
@@ -1874,6 +2062,7 @@ int wtg_mg_features(wtg_mg* m, int field, float* feats, double* sigma, int64_t* 
   if (!m) return WTG_E_ARG;
   wtg_ctx* top = m->lv[m->nl - 1];
   if (!m->npatch) return wtg_fail(top, WTG_E_ARG, "the finest level's n is not a multiple of 8");
+  if (m->M > 1) return wtg_fail(top, WTG_E_ARG, "features of an ensemble hierarchy are not available");
   RC(fs_field(top, field));
   if (!feats || !sigma) return wtg_fail(top, WTG_E_ARG, "null output");
   RC(lc_features(m, top->fs[field], false));

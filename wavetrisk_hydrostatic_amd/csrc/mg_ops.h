@@ -16,6 +16,7 @@ struct MgGeo {
   int64_t NCf;
   int Sf, nf, Sc;
   int64_t sp_f, np_f, sp_c, np_c;  // pole masters
+  int64_t NCc;                     // coarse cells (member stride of coarse fields)
 };
 
 __device__ __forceinline__ void mg_ij(int64_t c, int S, int& lz, int& i, int& j) {

@@ -465,38 +465,193 @@ __device__ __forceinline__ double sum3d(const double* __restrict__ hf, const Ind
   return s;
 }
 
-__global__ void ml_flux_restrict_kernel(PairGeo g, int L, const double* __restrict__ hfl, const double* __restrict__ dfl,
-                                        double* __restrict__ hcl) {
-  int lz, ip, jp, y;
-  if (!parent_of(g, lz, ip, jp, y, L)) return;  // y = field * L + layer
-  const double* hf = hfl + (int64_t)y * 3 * g.NCf;
-  const double* df = dfl + (int64_t)y * g.NCf;
-  double* hc = hcl + (int64_t)y * 3 * g.NCc;
+// The restriction of one coarse node reads the same fine positions and
+// geometry for every field and layer: those are resolved once per thread
+// (fine-plane offsets as 32-bit indices, the area and overlap weights of
+// part_coarse_flux and coarse_flux evaluated in the same order as above),
+// then the thread runs over FR_KC consecutive field-layers (blockIdx.y =
+// chunk), each a sequence of loads and the reference's sums.
+constexpr int FR_KC = 6;
+
+struct FluxEdgeW {  // part_coarse_flux and coarse_flux weights of one fine edge
+  double area0, area1, ol0, ol1, ol2, ol3;  // normalised areas, overlap parts
+  double c5, wpp, wpm, wmp, wmm;            // coarse_flux: sum of 5 products, half-weights
+};
+
+__device__ __forceinline__ void flux_edge_prep(const PairGeo& g, int lz, int i, int j, int e, const Ind& x,
+                                               FluxEdgeW& w, int& mz, int& pz, int b2[4]) {
+  const int64_t NF = g.NCf;
+  const double* a = g.ovl;
+  const double* sp = g.split;
+  const double* hi = g.hinv_f;
+  const int64_t c = cellf(g, lz, i, j);
+  double area0 = a[c], area1 = a[NF + c];
+  double ol0 = sp[c], ol1 = sp[NF + c];
+  double ol2 = a[2 * NF + c] - ol0, ol3 = a[3 * NF + c] - ol1;
+  area0 = area0 + ol0 + ol3;
+  area1 = area1 + ol1 + ol2;
+  const double tot = area0 + area1;
+  w.area0 = area0 / tot;
+  w.area1 = area1 / tot;
+  w.ol0 = sp[x.pp];
+  w.ol1 = sp[NF + x.mm];
+  w.ol2 = a[2 * NF + x.mp] - sp[x.mp];
+  w.ol3 = a[3 * NF + x.pm] - sp[NF + x.pm];
+  const int so = c_HSO[e];
+  mz = (int)cellf(g, lz, i + c_NGHB[so + 1][0], j + c_NGHB[so + 1][1]);
+  pz = (int)cellf(g, lz, i + c_NGHB[so + 4][0], j + c_NGHB[so + 4][1]);
+  const int64_t id = c, mp = x.mp, pp = x.pp, pm = x.pm, mm = x.mm;
+#define A(k, cc) a[(int64_t)((k) - 1) * NF + (cc)]
+  w.c5 = A(1, id) * A(2, id) * hi[id] + A(2, mp) * A(3, mp) * hi[mp] + A(1, pp) * A(3, pp) * hi[pp] +
+         A(1, pm) * A(4, pm) * hi[pm] + A(2, mm) * A(4, mm) * hi[mm];
+  w.wpp = A(3, pp) * A(4, pp) * hi[pp] * 0.5;
+  w.wpm = A(3, pm) * A(4, pm) * hi[pm] * 0.5;
+  w.wmp = A(3, mp) * A(4, mp) * hi[mp] * 0.5;
+  w.wmm = A(3, mm) * A(4, mm) * hi[mm] * 0.5;
+#undef A
+  // the butterfly's second ring (dval): the tile, else the wide map, else 0 (-1)
+  for (int q = 0; q < 4; ++q) {
+    const int bi = i + c_BFLY2[e][q][0], bj = j + c_BFLY2[e][q][1];
+    if (bi >= -2 && bj >= -2 && bi < g.nf + 2 && bj < g.nf + 2) {
+      b2[q] = (int)cellf(g, lz, bi, bj);
+    } else {
+      const int WS = g.nf + 2 * g.WH;
+      b2[q] = (int)g.wide[(int64_t)lz * WS * WS + (int64_t)(bj + g.WH) * WS + (bi + g.WH)];
+    }
+  }
+}
+
+// loads at 32-bit byte offsets from a wave-uniform base (one VGPR per index)
+__device__ __forceinline__ double ldb(const double* p, uint32_t off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off);
+}
+constexpr uint32_t NO_CELL = 0xffffffffu;
+
+// part_coarse_flux + coarse_flux of one fine edge on one field-layer
+// (offsets in bytes; ed: UPZ, UMZ, VMM, WMP, WPM, VPP; cell: mp, pm, pp, mm)
+__device__ __forceinline__ double flux_edge_apply(const double* hf, const double* df, const FluxEdgeW& w,
+                                                  const uint32_t ed[6], const uint32_t cell[4], uint32_t mz,
+                                                  uint32_t pz, const uint32_t b2[4]) {
+  const double s1 = ldb(hf, ed[0]) * w.area0 + ldb(hf, ed[1]) * w.area1;
+  const double s2 = ldb(hf, ed[2]) + ldb(hf, ed[3]);
+  const double s3 = ldb(hf, ed[4]) + ldb(hf, ed[5]);
+  const double pf = s1 - s2 * w.area1 - s3 * w.area0 + w.ol2 * ldb(df, cell[0]) - w.ol3 * ldb(df, cell[1]) -
+                    w.ol0 * ldb(df, cell[2]) + w.ol1 * ldb(df, cell[3]);
+  const double dmm2 = b2[0] != NO_CELL ? ldb(df, b2[0]) : 0.0;
+  const double dpm2 = b2[1] != NO_CELL ? ldb(df, b2[1]) : 0.0;
+  const double dpp2 = b2[2] != NO_CELL ? ldb(df, b2[2]) : 0.0;
+  const double dmp2 = b2[3] != NO_CELL ? ldb(df, b2[3]) : 0.0;
+  const double dz = ldb(df, mz), dp = ldb(df, pz);
+  const double cf = w.c5 * (dp - dz) + w.wpp * (dpp2 - dz) + w.wpm * (dpm2 - dz) + w.wmp * (dp - dmp2) +
+                    w.wmm * (dp - dmm2);
+  return pf + cf;
+}
+
+// sum3 / sum3d of R_F_wgt set q (rf planes 3q..3q+2 at byte offset po)
+__device__ __forceinline__ double sum3i(const double* hf, const uint32_t a[3], const double* rf, int64_t NCc,
+                                        uint32_t po) {
+  double s = 0.0;
+  s = s + ldb(hf, a[0]) * ldb(rf, po);
+  s = s + ldb(hf, a[1]) * ldb(rf + NCc, po);
+  s = s + ldb(hf, a[2]) * ldb(rf + 2 * NCc, po);
+  return s;
+}
+__device__ __forceinline__ double sum3di(const double* hf, const uint32_t a[3], const uint32_t b[3],
+                                         const double* rf, int64_t NCc, uint32_t po) {
+  double s = 0.0;
+  s = s + (ldb(hf, a[0]) - ldb(hf, b[0])) * ldb(rf, po);
+  s = s + (ldb(hf, a[1]) - ldb(hf, b[1])) * ldb(rf + NCc, po);
+  s = s + (ldb(hf, a[2]) - ldb(hf, b[2])) * ldb(rf + 2 * NCc, po);
+  return s;
+}
+
+// sum3 + sum3d of one R_F_wgt pair (sets q, q + 1) at fine index set x:
+// sign * sum3(a) + sign * sum3d(c, d) in the reference's order
+struct SmOps {
+  uint32_t a[3], c[3], d[3];
+};
+
+// the restriction of coarse edge E of one coarse node (blockIdx.z = E): its
+// part_coarse_flux + coarse_flux and the two sm terms it shares with the
+// neighbouring edges, hc(E) = ((pf + cf) + sm_E) + sm_{E+1}
+template <int E>
+__device__ __forceinline__ void flux_restrict_edge(const PairGeo& g, int nfl, const double* __restrict__ hfl,
+                                                   const double* __restrict__ dfl, double* __restrict__ hcl, int lz,
+                                                   int ip, int jp, int chunk) {
   const int64_t NCc = g.NCc;
-  const double* rf = g.rf;
   const int i = 2 * ip, j = 2 * jp;
   const int64_t p = cellc(g, lz, ip, jp);
-  Ind x;
-  double sm0, sm1, sm2, sm3;
-  get_indices(g, lz, i + 1, j, 0, x);
-  sm0 = -sum3(hf, x, WPM, UZM, VMM, rf, NCc, p) - sum3d(hf, x, VPM, WMMM, UMZ, UPZ, VPMM, WMM, rf + 3 * NCc, NCc, p);
-  sm1 = sum3(hf, x, WMP, UZP, VPP, rf + 6 * NCc, NCc, p) +
-        sum3d(hf, x, VMP, WPPP, UPZ, UMZ, VMPP, WPP, rf + 9 * NCc, NCc, p);
-  get_indices(g, lz, i, j + 1, 2, x);
-  sm2 = -sum3(hf, x, UZM, VMM, WPM, rf + 12 * NCc, NCc, p) -
-        sum3d(hf, x, WMMM, UMZ, VPM, VPMM, WMM, UPZ, rf + 15 * NCc, NCc, p);
-  sm3 = sum3(hf, x, UZP, VPP, WMP, rf + 18 * NCc, NCc, p) +
-        sum3d(hf, x, WPPP, UPZ, VMP, VMPP, WPP, UMZ, rf + 21 * NCc, NCc, p);
-  double pf, cf;
-  pf = part_coarse_flux(g, hf, df, lz, i + 1, j, 0);
-  cf = coarse_flux(g, df, lz, i + 1, j, 0);
-  hc[p] = pf + cf + sm0 + sm1;
-  pf = part_coarse_flux(g, hf, df, lz, i + 1, j + 1, 1);
-  cf = coarse_flux(g, df, lz, i + 1, j + 1, 1);
-  hc[NCc + p] = pf + cf + sm1 + sm2;
-  pf = part_coarse_flux(g, hf, df, lz, i, j + 1, 2);
-  cf = coarse_flux(g, df, lz, i, j + 1, 2);
-  hc[2 * NCc + p] = pf + cf + sm2 + sm3;
+  auto B = [](int64_t k) { return (uint32_t)(k * 8); };
+  // sm_E, sm_{E+1}: sm0, sm1 from the RT child (i+1, j), sm2, sm3 from the UP child (i, j+1)
+  SmOps sa, sb;
+  int qa, qb;
+  double sga, sgb;
+  {
+    Ind x0, x2;
+    if (E <= 1) get_indices(g, lz, i + 1, j, 0, x0);
+    if (E >= 1) get_indices(g, lz, i, j + 1, 2, x2);
+    auto set = [&](SmOps& o, int k) {
+      if (k == 0) o = SmOps{{B(x0.ed[WPM]), B(x0.ed[UZM]), B(x0.ed[VMM])},
+                            {B(x0.ed[VPM]), B(x0.ed[WMMM]), B(x0.ed[UMZ])},
+                            {B(x0.ed[UPZ]), B(x0.ed[VPMM]), B(x0.ed[WMM])}};
+      if (k == 1) o = SmOps{{B(x0.ed[WMP]), B(x0.ed[UZP]), B(x0.ed[VPP])},
+                            {B(x0.ed[VMP]), B(x0.ed[WPPP]), B(x0.ed[UPZ])},
+                            {B(x0.ed[UMZ]), B(x0.ed[VMPP]), B(x0.ed[WPP])}};
+      if (k == 2) o = SmOps{{B(x2.ed[UZM]), B(x2.ed[VMM]), B(x2.ed[WPM])},
+                            {B(x2.ed[WMMM]), B(x2.ed[UMZ]), B(x2.ed[VPM])},
+                            {B(x2.ed[VPMM]), B(x2.ed[WMM]), B(x2.ed[UPZ])}};
+      if (k == 3) o = SmOps{{B(x2.ed[UZP]), B(x2.ed[VPP]), B(x2.ed[WMP])},
+                            {B(x2.ed[WPPP]), B(x2.ed[UPZ]), B(x2.ed[VMP])},
+                            {B(x2.ed[VMPP]), B(x2.ed[WPP]), B(x2.ed[UMZ])}};
+    };
+    set(sa, E);
+    set(sb, E + 1);
+    qa = 6 * E, qb = 6 * (E + 1);
+    sga = (E == 0 || E == 2) ? -1.0 : 1.0;  // sm0 and sm2 enter with a minus sign
+    sgb = (E + 1 == 2) ? -1.0 : 1.0;
+  }
+  FluxEdgeW w;
+  uint32_t ed[6], cl[4], mz, pz, b2[4];
+  {
+    const int fi = E == 2 ? i : i + 1, fj = E == 0 ? j : j + 1;
+    Ind x;
+    get_indices(g, lz, fi, fj, E, x);
+    int m_, p_, bb[4];
+    flux_edge_prep(g, lz, fi, fj, E, x, w, m_, p_, bb);
+    mz = B(m_), pz = B(p_);
+    for (int q = 0; q < 4; ++q) b2[q] = bb[q] >= 0 ? B(bb[q]) : NO_CELL;
+    ed[0] = B(x.ed[UPZ]), ed[1] = B(x.ed[UMZ]), ed[2] = B(x.ed[VMM]);
+    ed[3] = B(x.ed[WMP]), ed[4] = B(x.ed[WPM]), ed[5] = B(x.ed[VPP]);
+    cl[0] = B(x.mp), cl[1] = B(x.pm), cl[2] = B(x.pp), cl[3] = B(x.mm);
+  }
+  const double* rfa = g.rf + (int64_t)qa * NCc;
+  const double* rfb = g.rf + (int64_t)qb * NCc;
+  const uint32_t po = B(p);
+  const int y0 = chunk * FR_KC, y1 = min(y0 + FR_KC, nfl);
+#pragma unroll 1
+  for (int y = y0; y < y1; ++y) {
+    const double* hf = hfl + (int64_t)y * 3 * g.NCf;
+    const double* df = dfl + (int64_t)y * g.NCf;
+    // sign * sum3 + sign * sum3d, as -sum3(..) - sum3d(..) or sum3(..) + sum3d(..)
+    const double a1 = sum3i(hf, sa.a, rfa, NCc, po), a2 = sum3di(hf, sa.c, sa.d, rfa + 3 * NCc, NCc, po);
+    const double sma = sga < 0.0 ? -a1 - a2 : a1 + a2;
+    const double b1 = sum3i(hf, sb.a, rfb, NCc, po), b2v = sum3di(hf, sb.c, sb.d, rfb + 3 * NCc, NCc, po);
+    const double smb = sgb < 0.0 ? -b1 - b2v : b1 + b2v;
+    hcl[(int64_t)y * 3 * NCc + (int64_t)E * NCc + p] = flux_edge_apply(hf, df, w, ed, cl, mz, pz, b2) + sma + smb;
+  }
+}
+
+__global__ __launch_bounds__(256) void ml_flux_restrict_kernel(PairGeo g, int nfl, const double* __restrict__ hfl,
+                                                               const double* __restrict__ dfl,
+                                                               double* __restrict__ hcl) {
+  int lz, ip, jp, chunk;
+  if (!parent_of(g, lz, ip, jp, chunk, 0)) return;
+  if (blockIdx.z == 0)
+    flux_restrict_edge<0>(g, nfl, hfl, dfl, hcl, lz, ip, jp, chunk);
+  else if (blockIdx.z == 1)
+    flux_restrict_edge<1>(g, nfl, hfl, dfl, hcl, lz, ip, jp, chunk);
+  else
+    flux_restrict_edge<2>(g, nfl, hfl, dfl, hcl, lz, ip, jp, chunk);
 }
 
 // coarse scalar trends (cal_scalar_trend, ops.f90:1325-1343) of the
@@ -613,7 +768,7 @@ int ml_upload(wtg_ml* m, Tp** p, const std::vector<Tp>& v) {
 MgGeo mgeo(const wtg_ml* m, int l) {
   const wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
   return MgGeo{m->ovl[l], f->d_hinv, c->d_hinv, f->info.ncells, f->info.S, f->info.n, c->info.S,
-               f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole};
+               f->info.south_pole, f->info.north_pole, c->info.south_pole, c->info.north_pole, c->info.ncells};
 }
 
 PairGeo pgeo(const wtg_ml* m, int l) {
@@ -732,8 +887,8 @@ int stage(wtg_ml* m, const std::vector<const wtg_ctx::Slot*>& y, const std::vect
   for (int l = top - 1; l >= 0; --l) {
     wtg_ctx* c = m->lv[l];
     const PairGeo pg = pgeo(m, l + 1);
-    const dim3 pgrid(nblk((int64_t)10 * pg.nc * pg.nc), 2 * L);
-    hipLaunchKernelGGL(ml_flux_restrict_kernel, pgrid, dim3(NT), 0, mstream(m), pg, L, m->hflux[l + 1], m->dsc[l + 1],
+    const dim3 pgrid(nblk((int64_t)10 * pg.nc * pg.nc), (2 * L + FR_KC - 1) / FR_KC, 3);
+    hipLaunchKernelGGL(ml_flux_restrict_kernel, pgrid, dim3(NT), 0, mstream(m), pg, 2 * L, m->hflux[l + 1], m->dsc[l + 1],
                        m->hflux[l]);
     MLCHK(m, hipGetLastError());
     if (int rc = wtg_ghost_fill_layers(c, 2 * L, nullptr, nullptr, m->hflux[l])) return rc;
@@ -976,7 +1131,7 @@ int flux_divergence_ml(wtg_ml* m, int slot, int out) {
   for (int l = top - 1; l >= 0; --l) {
     wtg_ctx* c = m->lv[l];
     const PairGeo pg = pgeo(m, l + 1);
-    hipLaunchKernelGGL(ml_flux_restrict_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), 1), dim3(NT), 0, mstream(m), pg,
+    hipLaunchKernelGGL(ml_flux_restrict_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), 1, 3), dim3(NT), 0, mstream(m), pg,
                        1, m->hflux[l + 1], (const double*)m->lv[l + 1]->fs[out], m->hflux[l]);
     MLCHK(m, hipGetLastError());
     if (int rc = wtg_ghost_fill_layers(c, 1, nullptr, nullptr, m->hflux[l])) return rc;

@@ -2556,8 +2556,6 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
   if (!c->topo_set) return fail(c, WTG_E_STATE, "wtg_set_topography must be called before time stepping");
   if (!c->fs_params_set) return fail(c, WTG_E_STATE, "wtg_fs_set_params must be called first");
   if (nsteps < 0) return fail(c, WTG_E_ARG, "negative step count");
-  if (c->nmembers != 1 && solve != bicgstab_solve)
-    return fail(c, WTG_E_ARG, "ensembles solve the free surface with BiCGSTAB (wtg_rk4_split)");
   if (int rc = wtg_fs_ready(c)) return rc;
   const auto& sol = c->slots[slot];
   const auto& w1 = c->slots[c->nslots];
@@ -2646,7 +2644,7 @@ int wtg_split_steps(wtg_ctx* c, int slot, int nsteps, wtg_fs_solver solve, void*
     }
     HIPCHK(c, hipMemcpyAsync(c->fs[FS_ETA_OLD], c->fs[FS_ETA], NCM * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
-    if (c->nmembers == 1) {
+    if (c->nmembers == 1 || solve != bicgstab_solve) {  // single member, or a member-batched hierarchy
       if (int rc = solve(user, c, iters)) return rc;
     } else {  // every member's BiCGSTAB in the same launches
       const double* tm = (const double*)user;  // {tol, maxit} of bicgstab_solve

@@ -30,35 +30,51 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "learned_J7.npz")
 
 
-class Problem:
-    """One draw on the uniform hierarchy coarse..fine (contexts + hierarchy)."""
+def draw_fields(grids, seed: int):
+    """per level (topography, old free surface, right-hand side) of one draw"""
+    rng = np.random.default_rng(seed)
+    bseed, eseed = (int(v) for v in rng.integers(0, 2 ** 31, 2))
+    amps = rng.normal(0.0, 1.0, 4)
+    centres = [(rng.uniform(-np.pi, np.pi), rng.uniform(-1.2, 1.2), rng.uniform(0.05, 0.4)) for _ in range(4)]
+    noise_seed = int(rng.integers(0, 2 ** 31))
+    out = []
+    for g in grids:
+        topo, _, _, _ = ocean(g, 1, seed=bseed)
+        lon, lat = lonlat(g.array("x"), g.radius)
+        erng = np.random.default_rng(eseed)
+        lo0, la0 = erng.uniform(-np.pi, np.pi), erng.uniform(-1.0, 1.0)
+        eta_old = 0.5 * np.exp(-(_dist(lon, lat, lo0, la0) / 0.2) ** 2)
+        f = np.zeros(g.layout.ncells)
+        for a, (lo, la, w) in zip(amps, centres):
+            f += a * np.exp(-(_dist(lon, lat, lo, la) / w) ** 2)
+        f += 0.05 * np.random.default_rng(noise_seed + g.layout.n).standard_normal(g.layout.ncells)
+        out.append((topo, eta_old, f))
+    return out
 
-    def __init__(self, grids, seed: int, dt: float = 1800.0):
+
+class Problem:
+    """One draw on the uniform hierarchy coarse..fine (contexts + hierarchy);
+    seeds: a list of draws makes it a batch, one ensemble member per draw
+    (the hierarchy then solves every member's problem in the same launches)."""
+
+    def __init__(self, grids, seed, dt: float = 1800.0):
         self.grids = grids
-        rng = np.random.default_rng(seed)
+        seeds = list(seed) if isinstance(seed, (list, tuple)) else [seed]
+        self.members = M = len(seeds)
+        draws = [draw_fields(grids, s) for s in seeds]
         self.ctxs, self.topos, self.eta_olds = [], [], []
-        bseed, eseed = (int(v) for v in rng.integers(0, 2 ** 31, 2))
-        amps = rng.normal(0.0, 1.0, 4)
-        centres = [(rng.uniform(-np.pi, np.pi), rng.uniform(-1.2, 1.2), rng.uniform(0.05, 0.4)) for _ in range(4)]
-        noise_seed = int(rng.integers(0, 2 ** 31))
-        for g in grids:
-            topo, _, _, _ = ocean(g, 1, seed=bseed)
-            lon, lat = lonlat(g.array("x"), g.radius)
-            erng = np.random.default_rng(eseed)
-            lo0, la0 = erng.uniform(-np.pi, np.pi), erng.uniform(-1.0, 1.0)
-            eta_old = 0.5 * np.exp(-(_dist(lon, lat, lo0, la0) / 0.2) ** 2)
-            f = np.zeros(g.layout.ncells)
-            for a, (lo, la, w) in zip(amps, centres):
-                f += a * np.exp(-(_dist(lon, lat, lo, la) / w) ** 2)
-            f += 0.05 * np.random.default_rng(noise_seed + g.layout.n).standard_normal(g.layout.ncells)
+        for k, g in enumerate(grids):
+            topo = np.stack([d[k][0] for d in draws]) if M > 1 else draws[0][k][0]
+            eta_old = np.stack([d[k][1] for d in draws]) if M > 1 else draws[0][k][1]
+            f = np.stack([d[k][2] for d in draws]) if M > 1 else draws[0][k][2]
             self.topos.append(topo)
             self.eta_olds.append(eta_old)
-            c = Context(g, nlayers=1)
+            c = Context(g, nlayers=1, members=M)
             c.set_topography(topo)
             c.fs_set_params(dt)
             c.fs_upload(FS_ETA_OLD, eta_old)
             c.fs_upload(FS_RHS, f)
-            c.fs_upload(FS_ETA, np.zeros(g.layout.ncells))
+            c.fs_upload(FS_ETA, np.zeros_like(f))
             self.ctxs.append(c)
         self.h = Hierarchy(self.ctxs)
         self.top = self.ctxs[-1]
@@ -70,7 +86,8 @@ class Problem:
 
     def reset(self):
         for c, g in zip(self.ctxs, self.grids):
-            c.fs_upload(FS_ETA, np.zeros(g.layout.ncells))
+            shape = (g.layout.ncells,) if self.members == 1 else (self.members, g.layout.ncells)
+            c.fs_upload(FS_ETA, np.zeros(shape))
 
     def solve(self, method, params):
         self.reset()
