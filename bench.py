@@ -370,6 +370,7 @@ def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10, split_step
         split[prec] = {"ms_per_step": (time.perf_counter() - t0) * 1e3 / split_steps,
                        "max_member_bicgstab_iterations": [int(i) for i in its]}
         ctx.close()
+        split[prec]["hierarchy"] = ensemble_hierarchy_split(grid, topos, state, L, prec, split_steps)
     dof = members * L * grid.layout.n_dof_per_layer
     lay = grid.layout
     own, own_e = lay.owned_cells(), lay.owned_edge_cells()
@@ -400,6 +401,70 @@ def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10, split_step
                            "note": "host wall clock; the members' BiCGSTAB solves batched (one launch per step of "
                                    "the iteration for all members)"},
             "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
+
+
+def _subsample(fine, coarse, f):
+    """a node field of the coarse level: the finer level's value at each
+    coincident node of [0, n]^2 (halo cells are filled by the context)"""
+    import numpy as np
+
+    lz, i, j = coarse.layout.coords()
+    m = (i >= 0) & (j >= 0) & (i <= coarse.layout.n) & (j <= coarse.layout.n)
+    Sf = fine.layout.S
+    out = np.zeros(f.shape[:-1] + (coarse.layout.ncells,))
+    out[..., m] = f[..., lz[m] * Sf * Sf + (2 * j[m] + 2) * Sf + (2 * i[m] + 2)]
+    return out
+
+
+def ensemble_hierarchy_split(grid, topos, state, L, prec, nsteps, dt=900.0):
+    """Config 5's split step with the members' free surfaces solved together
+    on the J(J-3)..J hierarchy (member-batched cycles, coarse level to 1e-2
+    per cycle, 1e-9 on the finest): plain V-cycles and learned cycles (the
+    shipped network); ms per step of the whole ensemble."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import learned as LC
+
+    M = topos.shape[0]
+    J = grid.layout.level
+    grids = [W.Grid(level=l, min_level=J - 3, domain_level=2) for l in range(J - 3, J + 1)]
+    tps = [topos]
+    for k in range(len(grids) - 2, -1, -1):
+        tps.insert(0, _subsample(grids[k + 1], grids[k], tps[0]))
+    ctxs = []
+    for k, g in enumerate(grids):
+        c = W.Context(g, nlayers=L if k == len(grids) - 1 else 1, members=M, nslots=1)
+        c.set_topography(tps[k])
+        c.fs_set_params(dt, 0.6, 0.8)
+        ctxs.append(c)
+    top = ctxs[-1]
+    top.set_precision(prec)
+    h = W.Hierarchy(ctxs)
+    mlp = LC.load_network(h.npatches * M)
+    p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)
+    out = {"levels": f"J{J - 3}..J{J}"}
+    for name, meth in (("vcycle", W.SOLVE_VCYCLE), ("learned", W.SOLVE_LEARNED)):
+        if meth == W.SOLVE_LEARNED:
+            if mlp is None:
+                continue
+            h.set_learned(mlp)
+        top.upload_state(0, *state)
+        top.fs_upload(W.FS_ETA, state[0].reshape(M, L, -1).sum(1) / 1028.0 + topos)
+        h.rk4_split(0, 1, meth, p)  # warm-up
+        top.synchronize()
+        t0 = time.perf_counter()
+        cyc = [h.rk4_split(0, 1, meth, p) for _ in range(nsteps)]
+        ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        out[name] = {"ms_per_step": ms, "dof_updates_per_s": M * L * grid.layout.n_dof_per_layer / (ms / 1e3),
+                     "max_member_cycles": [int(c) for c in cyc]}
+    h.set_learned(None)
+    h.close()
+    if mlp is not None:
+        mlp.close()
+    for c in ctxs:
+        c.close()
+    return out
 
 
 def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
@@ -449,17 +514,27 @@ def split_step(grid, ctx, topo, mass, temp, velo, L, dt=900.0, nsteps=5):
     top.fs_set_params(dt, 0.6, 0.8)
     h = W.Hierarchy(levels + [top])
     p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)
-    for name, prec in (("fp64", PRECISION_FP64), ("fp32_bulk", PRECISION_FP32_BULK)):
+    from wavetrisk_hydrostatic_amd import learned as LC
+    mlp = LC.load_network(h.npatches)
+    runs_mg = [("fp64", PRECISION_FP64, W.SOLVE_VCYCLE), ("fp32_bulk", PRECISION_FP32_BULK, W.SOLVE_VCYCLE)]
+    if mlp is not None:  # the north-star combination: FP32-bulk trends with learned cycles
+        runs_mg += [("fp64_learned", PRECISION_FP64, W.SOLVE_LEARNED),
+                    ("fp32_bulk_learned", PRECISION_FP32_BULK, W.SOLVE_LEARNED)]
+        h.set_learned(mlp)
+    for name, prec, meth in runs_mg:
         top.set_precision(prec)
         top.upload_state(0, mass, temp, velo)
         top.fs_upload(FS_ETA, mass.sum(0) / 1028.0 + topo)
-        h.rk4_split(0, 1, W.SOLVE_VCYCLE, p)  # warm-up
+        h.rk4_split(0, 1, meth, p)  # warm-up
         top.synchronize()
         t0 = time.perf_counter()
-        cyc = [h.rk4_split(0, 1, W.SOLVE_VCYCLE, p) for _ in range(nsteps)]
+        cyc = [h.rk4_split(0, 1, meth, p) for _ in range(nsteps)]
         ms_mg = (time.perf_counter() - t0) * 1e3 / nsteps
         mg_runs[name] = {"ms_per_step": ms_mg, "dof_updates_per_s": dof / (ms_mg / 1e3),
                          "vcycles": [int(i) for i in cyc]}
+    h.set_learned(None)
+    if mlp is not None:
+        mlp.close()
     h.close()
     for c in levels + [top]:
         c.close()

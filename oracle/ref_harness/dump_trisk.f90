@@ -11,7 +11,9 @@
 !     (multi_level.f90:5-47) with its work arrays,
 !   * the state after each of nsteps RK4 steps (time_integr.f90:78-103).
 !
-! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out.
+! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out, ntimed.
+! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
+! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! Output: one stream file of tagged records (tag*16, kind int32 (1=i32,
 ! 2=f64), count int64, data), read by tests/golden/make_golden.py.
 module harness_hooks
@@ -143,12 +145,13 @@ program dump_trisk
 
   character(16)  :: case
   character(255) :: out
-  integer        :: zlev, nsteps, d, k, l, n, u, v
+  integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed
+  integer(8)     :: t_start, t_end, t_rate
   real(8)        :: step
   logical        :: nanfill
-  namelist /run/ case, zlev, nsteps, step, out, nanfill
+  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed
 
-  case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.
+  case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0
   read (5, nml=run)
   ic_case = case
 
@@ -241,6 +244,14 @@ program dump_trisk
      call dump_state ('s'//achar(48+n/10)//achar(48+mod(n,10))//'_')
   end do
   close (u)
+  if (ntimed > 0) then
+     call system_clock (t_start, t_rate)
+     do n = 1, ntimed
+        call RK4 (sol(:,1:zlevels), wav_coeff(:,1:zlevels), trend_ml, step)
+     end do
+     call system_clock (t_end)
+     write (6, '(a,f12.3,a)') 'RK4 ', 1d3 * dble(t_end - t_start) / dble(t_rate) / dble(ntimed), ' ms/step'
+  end if
 contains
   subroutine wr_i (tag, a)
     character(*)          :: tag

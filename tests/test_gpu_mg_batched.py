@@ -122,3 +122,65 @@ def test_ensemble_split_step_on_batched_hierarchy():
         assert np.array_equal(ub[sl][:, :, owne], ua[:, :, owne])
         assert np.array_equal(eb[k][own], ea[own])
     assert itb == max(its)
+
+
+def test_ensemble_j8_l30_learned_split_step():
+    """Config 5's shape: two members of the J8 L30 ocean, each with its own
+    bathymetry, one split-explicit step with the free surface solved by
+    learned cycles on the member-batched J6..J8 hierarchy (any network).
+    Each member equals its own single-member run bit for bit; member 0
+    against the oracle's split step (single-level BiCGSTAB to 1e-12) within
+    the solver tolerance's reach (1e-8)."""
+    print("J8 L30 ensemble of 2, learned split step on J6..J8", flush=True)
+    from wavetrisk_hydrostatic_amd import cases
+    from conftest import maxrel
+    from oracle.oracle import EllipticOracle, geometry_of
+    grids = [W.Grid(level=l, min_level=6, domain_level=2) for l in (6, 7, 8)]
+    L, dt, M = 30, 900.0, 2
+    draws = [[cases.ocean(g, L if lv == 2 else 1, seed=60 + k) for lv, g in enumerate(grids)] for k in range(M)]
+    p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-10, coarse_iter=400, max_vcycle=60)
+
+    def run(members):
+        ctxs = []
+        for lv, g in enumerate(grids):
+            c = W.Context(g, nlayers=L if lv == 2 else 1, members=len(members))
+            topo = np.stack([draws[k][lv][0] for k in members]) if len(members) > 1 else draws[members[0]][lv][0]
+            c.set_topography(topo)
+            c.fs_set_params(dt, 0.6, 0.8)
+            ctxs.append(c)
+        top = ctxs[-1]
+        top.upload_state(0, *(np.concatenate([draws[k][2][v] for k in members]) for v in (1, 2, 3)))
+        eta = [draws[k][2][1].sum(0) / 1028.0 + draws[k][2][0] for k in members]
+        top.fs_upload(W.FS_ETA, np.stack(eta) if len(members) > 1 else eta[0])
+        h = W.Hierarchy(ctxs)
+        mlp = W.MLP(max_batch=h.npatches * len(members), seed=9)
+        try:
+            h.set_learned(mlp)
+            it = h.rk4_split(0, 1, W.SOLVE_LEARNED, p)
+        finally:
+            h.set_learned(None)
+            h.close()
+            mlp.close()
+        res = top.download_state(0), top.fs_download(W.FS_ETA), it
+        for c in ctxs:
+            c.close()
+        return res
+
+    (mb, tb, ub), eb, itb = run([0, 1])
+    g = grids[-1]
+    own, owne = g.layout.owned_cells(), g.layout.owned_edge_cells()
+    for k in range(M):
+        (ma, ta, ua), ea, _ = run([k])
+        sl = slice(k * L, (k + 1) * L)
+        assert np.array_equal(mb[sl][:, own], ma[:, own]) and np.array_equal(ub[sl][:, :, owne], ua[:, :, owne])
+        assert np.array_equal(eb[k][own], ea[own])
+    topo, m0, t0, u0 = draws[0][2]
+    o = EllipticOracle(g.layout, geometry_of(g), topo, dt=dt, theta1=0.6, theta2=0.8)
+    m0, t0, u0 = m0.copy(), t0.copy(), u0.copy()
+    o.ghost_fill(m0, t0, u0)
+    mo, _, uo, eo, _ = o.rk4_split(m0, t0, u0, o.node_ghost(m0.sum(0) / 1028.0 + topo), tol=1e-12, maxit=2000)
+    print(f"  member 0 vs oracle: eta {maxrel(eb[0][own], eo[own]):.2e}, "
+          f"velocity {maxrel(ub[:L][:, :, owne], uo[:, :, owne]):.2e}", flush=True)
+    assert maxrel(eb[0][own], eo[own]) <= 1e-8
+    assert maxrel(mb[:L][:, own], mo[:, own]) <= 1e-8
+    assert maxrel(ub[:L][:, :, owne], uo[:, :, owne]) <= 1e-8

@@ -868,7 +868,10 @@ int launch_dots_g(wtg_ctx* c, bool gate, int post_op, int nd, const double* a0, 
                   const double* a1 = nullptr, const double* b1 = nullptr, int o1 = 0, const double* a2 = nullptr,
                   const double* b2 = nullptr, int o2 = 0) {
   const int nb = std::min(RED_BLOCKS, (int)nblk(c->nred));
-  if (nb <= c->lastblock_max) {
+  // one launch finishing in the last block (its device-scope fence writes the
+  // L2 back): small single-member passes only -- with many members the
+  // blocks' fences cost more than the second launch
+  if (nb <= c->lastblock_max && c->nmembers == 1) {
     hipLaunchKernelGGL(fs_dot1_kernel, dim3(nb, c->nmembers), dim3(FS_NT), 0, c->stream, gate ? c->d_scal : nullptr,
                        gate ? 1 : 0, c->nred, c->d_rcell, c->d_rw, nd, a0, b0, a1 ? a1 : a0, b1 ? b1 : b0,
                        a2 ? a2 : a0, b2 ? b2 : b0, c->d_part, o0, o1, o2, c->d_scal, post_op, c->d_count,
@@ -958,7 +961,7 @@ int bicgstab(wtg_ctx* c, double* u, const double* f, double tol, int maxit, int*
                          c->d_scal, s, As, ms);  // s = res - alph Ap; As = L s
       WTG_HIPCHK(c, hipGetLastError());
       RC(launch_dots_g(c, true, OP_OMGA, 2, As, s, SC_T1, As, As, SC_T2));
-      const bool last = nbred <= c->lastblock_max;
+      const bool last = nbred <= c->lastblock_max && M == 1;
       hipLaunchKernelGGL(fs_bicg_res_kernel, dim3(std::max((unsigned)nbred, nblk(c->nncl)), M), dim3(FS_NT), 0,
                          c->stream, c->d_scal, c->nmaster, c->nncl, c->d_ncl, c->nred, nbred, c->d_rcell, c->d_rw, u,
                          p, s, As, res0, c->d_scal, res, c->d_part, last ? c->d_count : nullptr,

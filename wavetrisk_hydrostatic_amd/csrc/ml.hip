@@ -86,20 +86,31 @@ __device__ __forceinline__ int64_t cellf(const PairGeo& g, int lz, int i, int j)
 }
 
 // ---------------------------------------------------------------------------
-// scalar transform, layer-batched: blockIdx.y = field * L + layer (field 0
-// mass, 1 temperature); compute-list threads evaluate at src, write dst
+// scalar transform, layer-batched: field-layers y = field * L + layer (field
+// 0 mass, 1 temperature), ML_KC consecutive ones per thread (blockIdx.y =
+// chunk), so a thread's list entry and stencil geometry serve all of them;
+// compute-list threads evaluate at src, write dst
 // ---------------------------------------------------------------------------
+constexpr int ML_KC = 6;
+__host__ __device__ constexpr int ml_chunks(int nfl) { return (nfl + ML_KC - 1) / ML_KC; }
+// the velocity transform's gathers need more threads in flight: two layers
+constexpr int ML_KCV = 2;
+__host__ __device__ constexpr int ml_chunks_v(int L) { return (L + ML_KCV - 1) / ML_KCV; }
+
 __global__ void ml_scalar_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
                                          const double* __restrict__ sm, const double* __restrict__ st,
                                          double* __restrict__ wc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int y = blockIdx.y, k = y % L;
-  const double* s = (y < L ? sm : st) + (int64_t)k * g.NCf;
-  const int64_t x = ncl[2 * t + 1];
+  const int64_t x = ncl[2 * t + 1], dst = ncl[2 * t];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
-  wc[(int64_t)y * g.NCf + ncl[2 * t]] = ((i | j) & 1) ? s[x] - mg_interp(g, s, x, i, j) : 0.0;
+  const bool mid = (i | j) & 1;
+  const int y0 = blockIdx.y * ML_KC, y1 = min(y0 + ML_KC, 2 * L);
+  for (int y = y0; y < y1; ++y) {
+    const double* s = (y < L ? sm : st) + (int64_t)(y % L) * g.NCf;
+    wc[(int64_t)y * g.NCf + dst] = mid ? s[x] - mg_interp(g, s, x, i, j) : 0.0;
+  }
 }
 
 __global__ void ml_restrict_scalar_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
@@ -108,17 +119,20 @@ __global__ void ml_restrict_scalar_kernel(int n, const int64_t* __restrict__ ncl
                                           double* __restrict__ ct, int64_t NCc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int y = blockIdx.y, k = y % L;
-  const double* sf = (y < L ? fm : ft) + (int64_t)k * g.NCf;
-  double* sc = (y < L ? cm : ct) + (int64_t)k * NCc;
-  const double* w = wc + (int64_t)y * g.NCf;
-  const int64_t p = ncl[2 * t + 1];
+  const int64_t p = ncl[2 * t + 1], dst = ncl[2 * t];
   int lz, i, j;
   mg_ij(p, g.Sc, lz, i, j);
   const int64_t c = (int64_t)lz * g.Sf * g.Sf + (int64_t)(2 * j + 2) * g.Sf + (2 * i + 2);
   const bool pole = p == g.sp_c || p == g.np_c;
-  const double lift = pole ? mg_pole_lift(g, w, p == g.sp_c, false) : mg_lift(g, w, c, true);
-  sc[ncl[2 * t]] = sf[c] + lift * g.hinv_c[p];
+  const int y0 = blockIdx.y * ML_KC, y1 = min(y0 + ML_KC, 2 * L);
+  for (int y = y0; y < y1; ++y) {
+    const int k = y % L;
+    const double* sf = (y < L ? fm : ft) + (int64_t)k * g.NCf;
+    double* sc = (y < L ? cm : ct) + (int64_t)k * NCc;
+    const double* w = wc + (int64_t)y * g.NCf;
+    const double lift = pole ? mg_pole_lift(g, w, p == g.sp_c, false) : mg_lift(g, w, c, true);
+    sc[dst] = sf[c] + lift * g.hinv_c[p];
+  }
 }
 
 __global__ void ml_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
@@ -127,18 +141,21 @@ __global__ void ml_prolong_even_kernel(int n, const int64_t* __restrict__ ncl, M
                                        double* __restrict__ ft) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int y = blockIdx.y, k = y % L;
-  const int64_t x = ncl[2 * t + 1];
+  const int64_t x = ncl[2 * t + 1], dst = ncl[2 * t];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
   if ((i | j) & 1) return;
-  const double* sc = (y < L ? cm : ct) + (int64_t)k * NCc;
-  double* sf = (y < L ? fm : ft) + (int64_t)k * g.NCf;
-  const double* w = wc + (int64_t)y * g.NCf;
   const int64_t p = (int64_t)lz * g.Sc * g.Sc + (int64_t)(j / 2 + 2) * g.Sc + (i / 2 + 2);
   const bool pole = x == g.sp_f || x == g.np_f;
-  const double lift = pole ? mg_pole_lift(g, w, x == g.sp_f, true) : mg_lift(g, w, x, false);
-  sf[ncl[2 * t]] = sc[p] - lift * g.hinv_c[p];
+  const int y0 = blockIdx.y * ML_KC, y1 = min(y0 + ML_KC, 2 * L);
+  for (int y = y0; y < y1; ++y) {
+    const int k = y % L;
+    const double* sc = (y < L ? cm : ct) + (int64_t)k * NCc;
+    double* sf = (y < L ? fm : ft) + (int64_t)k * g.NCf;
+    const double* w = wc + (int64_t)y * g.NCf;
+    const double lift = pole ? mg_pole_lift(g, w, x == g.sp_f, true) : mg_lift(g, w, x, false);
+    sf[dst] = sc[p] - lift * g.hinv_c[p];
+  }
 }
 
 __global__ void ml_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
@@ -146,13 +163,15 @@ __global__ void ml_prolong_odd_kernel(int n, const int64_t* __restrict__ ncl, Mg
                                       double* __restrict__ ft) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int y = blockIdx.y, k = y % L;
-  const int64_t x = ncl[2 * t + 1];
+  const int64_t x = ncl[2 * t + 1], dst = ncl[2 * t];
   int lz, i, j;
   mg_ij(x, g.Sf, lz, i, j);
   if (!((i | j) & 1)) return;
-  double* sf = (y < L ? fm : ft) + (int64_t)k * g.NCf;
-  sf[ncl[2 * t]] = mg_interp(g, sf, x, i, j) + wc[(int64_t)y * g.NCf + x];
+  const int y0 = blockIdx.y * ML_KC, y1 = min(y0 + ML_KC, 2 * L);
+  for (int y = y0; y < y1; ++y) {
+    double* sf = (y < L ? fm : ft) + (int64_t)(y % L) * g.NCf;
+    sf[dst] = mg_interp(g, sf, x, i, j) + wc[(int64_t)y * g.NCf + x];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -242,62 +261,71 @@ __device__ __forceinline__ bool parent_of(const PairGeo& g, int& lz, int& ip, in
 // Compute_velo_wavelets (wavelet.f90:672-707): 6 outer and 6 inner child edges
 __global__ void ml_velo_wavelet_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                        const double* __restrict__ ufl, double* __restrict__ wcl) {
-  int lz, ip, jp, k;
-  if (!parent_of(g, lz, ip, jp, k, L)) return;
-  const double* uc = ucl + (int64_t)k * 3 * g.NCc;
-  const double* uf = ufl + (int64_t)k * 3 * g.NCf;
-  double* wc = wcl + (int64_t)k * 3 * g.NCf;
+  int lz, ip, jp, chunk;
+  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
   const int ic = 2 * ip, jc = 2 * jp;
+  const int k1 = min(chunk * ML_KCV + ML_KCV, L);
+  for (int k = chunk * ML_KCV; k < k1; ++k) {
+    const double* uc = ucl + (int64_t)k * 3 * g.NCc;
+    const double* uf = ufl + (int64_t)k * 3 * g.NCf;
+    double* wc = wcl + (int64_t)k * 3 * g.NCf;
 #pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const int64_t id1 = cellf(g, lz, ic + c_ENDPT[e][0][0], jc + c_ENDPT[e][0][1]);
-    const int64_t id2 = cellf(g, lz, ic + c_ENDPT[e][1][0], jc + c_ENDPT[e][1][1]);
-    const double u = interp_outer(g, uc, lz, ip, jp, e, id2);
-    const double w2 = uf[(int64_t)e * g.NCf + id2] - u;
-    wc[(int64_t)e * g.NCf + id2] = w2;
-    wc[(int64_t)e * g.NCf + id1] = -w2;
+    for (int e = 0; e < 3; ++e) {
+      const int64_t id1 = cellf(g, lz, ic + c_ENDPT[e][0][0], jc + c_ENDPT[e][0][1]);
+      const int64_t id2 = cellf(g, lz, ic + c_ENDPT[e][1][0], jc + c_ENDPT[e][1][1]);
+      const double u = interp_outer(g, uc, lz, ip, jp, e, id2);
+      const double w2 = uf[(int64_t)e * g.NCf + id2] - u;
+      wc[(int64_t)e * g.NCf + id2] = w2;
+      wc[(int64_t)e * g.NCf + id1] = -w2;
+    }
+    double ui[6];
+    int64_t sl[6];
+    interp_inner(g, uc, uf, lz, ip, jp, ui, sl);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) wc[sl[q]] = uf[sl[q]] - ui[q];
   }
-  double ui[6];
-  int64_t sl[6];
-  interp_inner(g, uc, uf, lz, ip, jp, ui, sl);
-#pragma unroll
-  for (int q = 0; q < 6; ++q) wc[sl[q]] = uf[sl[q]] - ui[q];
 }
 
 // Reconstruct_outer_velo (wavelet.f90:954-976)
 __global__ void ml_velo_outer_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                      const double* __restrict__ wcl, double* __restrict__ ufl) {
-  int lz, ip, jp, k;
-  if (!parent_of(g, lz, ip, jp, k, L)) return;
-  const double* uc = ucl + (int64_t)k * 3 * g.NCc;
-  const double* wc = wcl + (int64_t)k * 3 * g.NCf;
-  double* uf = ufl + (int64_t)k * 3 * g.NCf;
+  int lz, ip, jp, chunk;
+  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
   const int ic = 2 * ip, jc = 2 * jp;
   const int64_t idp = cellc(g, lz, ip, jp);
+  const int k1 = min(chunk * ML_KCV + ML_KCV, L);
+  for (int k = chunk * ML_KCV; k < k1; ++k) {
+    const double* uc = ucl + (int64_t)k * 3 * g.NCc;
+    const double* wc = wcl + (int64_t)k * 3 * g.NCf;
+    double* uf = ufl + (int64_t)k * 3 * g.NCf;
 #pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const int64_t id1 = cellf(g, lz, ic + c_ENDPT[e][0][0], jc + c_ENDPT[e][0][1]);
-    const int64_t id2 = cellf(g, lz, ic + c_ENDPT[e][1][0], jc + c_ENDPT[e][1][1]);
-    const double u = interp_outer(g, uc, lz, ip, jp, e, id2);
-    const double v2 = u + wc[(int64_t)e * g.NCf + id2];
-    uf[(int64_t)e * g.NCf + id2] = v2;
-    uf[(int64_t)e * g.NCf + id1] = 2.0 * uc[(int64_t)e * g.NCc + idp] - v2;
+    for (int e = 0; e < 3; ++e) {
+      const int64_t id1 = cellf(g, lz, ic + c_ENDPT[e][0][0], jc + c_ENDPT[e][0][1]);
+      const int64_t id2 = cellf(g, lz, ic + c_ENDPT[e][1][0], jc + c_ENDPT[e][1][1]);
+      const double u = interp_outer(g, uc, lz, ip, jp, e, id2);
+      const double v2 = u + wc[(int64_t)e * g.NCf + id2];
+      uf[(int64_t)e * g.NCf + id2] = v2;
+      uf[(int64_t)e * g.NCf + id1] = 2.0 * uc[(int64_t)e * g.NCc + idp] - v2;
+    }
   }
 }
 
 // Reconstruct_inner_velo (wavelet.f90:978-998); outer edges ghost filled
 __global__ void ml_velo_inner_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                      const double* __restrict__ wcl, double* __restrict__ ufl) {
-  int lz, ip, jp, k;
-  if (!parent_of(g, lz, ip, jp, k, L)) return;
-  const double* uc = ucl + (int64_t)k * 3 * g.NCc;
-  const double* wc = wcl + (int64_t)k * 3 * g.NCf;
-  double* uf = ufl + (int64_t)k * 3 * g.NCf;
-  double ui[6];
-  int64_t sl[6];
-  interp_inner(g, uc, uf, lz, ip, jp, ui, sl);
+  int lz, ip, jp, chunk;
+  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
+  const int k1 = min(chunk * ML_KCV + ML_KCV, L);
+  for (int k = chunk * ML_KCV; k < k1; ++k) {
+    const double* uc = ucl + (int64_t)k * 3 * g.NCc;
+    const double* wc = wcl + (int64_t)k * 3 * g.NCf;
+    double* uf = ufl + (int64_t)k * 3 * g.NCf;
+    double ui[6];
+    int64_t sl[6];
+    interp_inner(g, uc, uf, lz, ip, jp, ui, sl);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) uf[sl[q]] = ui[q] + wc[sl[q]];
+    for (int q = 0; q < 6; ++q) uf[sl[q]] = ui[q] + wc[sl[q]];
+  }
 }
 
 // pentagon corrections at the three lozenge corners that have them:
@@ -814,11 +842,12 @@ int wavelets(wtg_ml* m, int l, const wtg_ctx::Slot& fine, const wtg_ctx::Slot& c
   wtg_ctx* f = m->lv[l];
   const int L = m->L;
   const MgGeo g = mgeo(m, l);
-  hipLaunchKernelGGL(ml_scalar_wavelet_kernel, dim3(nblk(f->nncl), 2 * L), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl,
+  hipLaunchKernelGGL(ml_scalar_wavelet_kernel, dim3(nblk(f->nncl), ml_chunks(2 * L)), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl,
                      g, L, fine.m, fine.t, m->wcs[l]);
   MLCHK(m, hipGetLastError());
   const PairGeo pg = pgeo(m, l);
-  hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), L), dim3(NT), 0, mstream(m), pg, L,
+  hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L)), dim3(NT), 0,
+                     mstream(m), pg, L,
                      coarse.u, fine.u, m->wcu[l]);
   MLCHK(m, hipGetLastError());
   hipLaunchKernelGGL(ml_velo_penta_kernel, dim3(nblk(10 * L)), dim3(NT), 0, mstream(m), pg, L, 1, coarse.u, m->wcu[l]);
@@ -831,14 +860,14 @@ int inverse(wtg_ml* m, int l, const wtg_ctx::Slot& fine, const wtg_ctx::Slot& co
   wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
   const int L = m->L;
   const MgGeo g = mgeo(m, l);
-  hipLaunchKernelGGL(ml_prolong_even_kernel, dim3(nblk(f->nncl), 2 * L), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl, g,
+  hipLaunchKernelGGL(ml_prolong_even_kernel, dim3(nblk(f->nncl), ml_chunks(2 * L)), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl, g,
                      L, coarse.m, coarse.t, c->info.ncells, m->wcs[l], fine.m, fine.t);
   MLCHK(m, hipGetLastError());
-  hipLaunchKernelGGL(ml_prolong_odd_kernel, dim3(nblk(f->nncl), 2 * L), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl, g,
+  hipLaunchKernelGGL(ml_prolong_odd_kernel, dim3(nblk(f->nncl), ml_chunks(2 * L)), dim3(NT), 0, mstream(m), f->nncl, f->d_ncl, g,
                      L, m->wcs[l], fine.m, fine.t);
   MLCHK(m, hipGetLastError());
   const PairGeo pg = pgeo(m, l);
-  const dim3 pgrid(nblk((int64_t)10 * pg.nc * pg.nc), L);
+  const dim3 pgrid(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L));
   hipLaunchKernelGGL(ml_velo_outer_kernel, pgrid, dim3(NT), 0, mstream(m), pg, L, coarse.u, m->wcu[l], fine.u);
   MLCHK(m, hipGetLastError());
   hipLaunchKernelGGL(ml_velo_penta_kernel, dim3(nblk(10 * L)), dim3(NT), 0, mstream(m), pg, L, -1, coarse.u, fine.u);
@@ -1027,10 +1056,10 @@ int wtg_ml_forward(wtg_ml* m, int slot) {
     wtg_ctx *f = m->lv[l], *c = m->lv[l - 1];
     if (int rc = wtg_ghost_fill_layers(f, L, s[l]->m, s[l]->t, nullptr)) return rc;
     const MgGeo g = mgeo(m, l);
-    hipLaunchKernelGGL(ml_scalar_wavelet_kernel, dim3(nblk(f->nncl), 2 * L), dim3(NT), 0, mstream(m), f->nncl,
+    hipLaunchKernelGGL(ml_scalar_wavelet_kernel, dim3(nblk(f->nncl), ml_chunks(2 * L)), dim3(NT), 0, mstream(m), f->nncl,
                        f->d_ncl, g, L, s[l]->m, s[l]->t, m->wcs[l]);
     MLCHK(m, hipGetLastError());
-    hipLaunchKernelGGL(ml_restrict_scalar_kernel, dim3(nblk(c->nncl), 2 * L), dim3(NT), 0, mstream(m), c->nncl,
+    hipLaunchKernelGGL(ml_restrict_scalar_kernel, dim3(nblk(c->nncl), ml_chunks(2 * L)), dim3(NT), 0, mstream(m), c->nncl,
                        c->d_ncl, g, L, s[l]->m, s[l]->t, m->wcs[l], s[l - 1]->m, s[l - 1]->t, c->info.ncells);
     MLCHK(m, hipGetLastError());
     if (int rc = wtg_restrict_velo(f, slot, c, slot)) return rc;
@@ -1039,8 +1068,8 @@ int wtg_ml_forward(wtg_ml* m, int slot) {
     if (int rc = wtg_ghost_fill_layers(m->lv[l], L, nullptr, nullptr, s[l]->u)) return rc;
   for (int l = 1; l < m->nl; ++l) {
     const PairGeo pg = pgeo(m, l);
-    hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), L), dim3(NT), 0, mstream(m), pg,
-                       L, s[l - 1]->u, s[l]->u, m->wcu[l]);
+    hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L)), dim3(NT), 0,
+                       mstream(m), pg, L, s[l - 1]->u, s[l]->u, m->wcu[l]);
     MLCHK(m, hipGetLastError());
     hipLaunchKernelGGL(ml_velo_penta_kernel, dim3(nblk(10 * L)), dim3(NT), 0, mstream(m), pg, L, 1, s[l - 1]->u,
                        m->wcu[l]);
