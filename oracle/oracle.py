@@ -31,6 +31,8 @@ def load():
         _lib.wto_ghost_fill.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.wto_trend.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         _lib.wto_rk4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.wto_trend_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        _lib.wto_rk4_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 3
         _lib.wto_num_threads.restype = C.c_int
         _lib.wto_node_ghost.argtypes = [C.c_void_p, C.c_void_p]
         _lib.wto_elliptic_apply.argtypes = [C.c_void_p] * 5
@@ -74,6 +76,13 @@ class _G(C.Structure):
                 ("grav", C.c_double), ("rho0", C.c_double), ("p_top", C.c_double)]
 
 
+class Diff(C.Structure):
+    """wto_diff: Laplace orders of the scalars, div u, rot u and viscosities"""
+    _fields_ = [("lsclr", C.c_int), ("ldivu", C.c_int), ("lrotu", C.c_int), ("visc_m", C.c_double),
+                ("visc_t", C.c_double), ("visc_divu", C.c_double), ("visc_rotu", C.c_double),
+                ("plant_owner_halo", C.c_int)]
+
+
 class Oracle:
     """Holds the geometry arrays alive and exposes ghost_fill / trend / rk4."""
 
@@ -106,17 +115,29 @@ class Oracle:
         L = mass.shape[0]
         self.lib.wto_ghost_fill(C.byref(self.g), L, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
 
-    def trend(self, mass, temp, velo):
+    def trend(self, mass, temp, velo, diffusion=None):
+        """diffusion: None, or (lsclr, ldivu, lrotu, visc_m, visc_t, visc_divu,
+        visc_rotu) -- Laplacian diffusion / hyperdiffusion (wto_trend_diff)"""
         mass, temp, velo = self._state(mass, temp, velo)
         L = mass.shape[0]
         dm, dt, du = np.zeros_like(mass), np.zeros_like(temp), np.zeros_like(velo)
+        if diffusion is not None:
+            d = Diff(*diffusion)
+            self.lib.wto_trend_diff(C.byref(self.g), C.byref(d), L, mass.ctypes.data, temp.ctypes.data,
+                                    velo.ctypes.data, dm.ctypes.data, dt.ctypes.data, du.ctypes.data)
+            return dm, dt, du
         self.lib.wto_trend(C.byref(self.g), L, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data,
                            dm.ctypes.data, dt.ctypes.data, du.ctypes.data)
         return dm, dt, du
 
-    def rk4(self, mass, temp, velo, dt: float, nsteps: int):
+    def rk4(self, mass, temp, velo, dt: float, nsteps: int, diffusion=None):
         mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))
         L = mass.shape[0]
+        if diffusion is not None:
+            d = Diff(*diffusion)
+            self.lib.wto_rk4_diff(C.byref(self.g), C.byref(d), L, dt, nsteps, mass.ctypes.data, temp.ctypes.data,
+                                  velo.ctypes.data)
+            return mass, temp, velo
         self.lib.wto_rk4(C.byref(self.g), L, dt, nsteps, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
         return mass, temp, velo
 

@@ -14,6 +14,13 @@
 ! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out, ntimed.
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
+! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
+! diffusion (order 1) or iterated hyperdiffusion (order 2) through the
+! physics hooks, in the form the reference's test cases give them (a scalar
+! flux (-1)^Laplace_sclr visc_sclr grad(q or Laplacian_scalar) pedlen, a
+! velocity source (-1)^(Laplace_rotu-1) (visc_divu grad divu - visc_rotu
+! curl rotu)); the reference computes divu, the Laplacians and rot rot of
+! the vorticity itself (multi_level.f90:26-30, 99, 190-309, 579-617).
 ! Output: one stream file of tagged records (tag*16, kind int32 (1=i32,
 ! 2=f64), count int64, data), read by tests/golden/make_golden.py.
 module harness_hooks
@@ -37,6 +44,74 @@ contains
     integer, dimension(2,N_BDRY+1) :: dims
     no_velo_source = 0d0
   end function no_velo_source
+
+  ! edge fluxes of diffusion of scalar v (step1's physics_scalar_flux):
+  ! edges E, NE, N of id, or (sw) the W, SW, S edges with the neighbours
+  ! passed as idE, idNE, idN
+  function diff_scalar_flux (q, dom, id, idE, idNE, idN, v, zlev, type)
+    real(8), dimension(1:EDGE)                           :: diff_scalar_flux
+    type(Float_Field), dimension(1:N_VARIABLE,1:zlevels) :: q
+    type(domain)                                         :: dom
+    integer                                              :: d, id, idE, idNE, idN, v, zlev
+    logical, optional                                    :: type
+    real(8), dimension(1:EDGE) :: dl, pl, g
+    real(8) :: f0, fE, fNE, fN
+    logical :: sw
+    diff_scalar_flux = 0d0
+    if (Laplace_sclr == 0) return
+    sw = .false.
+    if (present(type)) sw = type
+    d = dom%id + 1
+    if (sw) then
+       pl = (/ dom%pedlen%elts(EDGE*idE+RT+1), dom%pedlen%elts(EDGE*idNE+DG+1), dom%pedlen%elts(EDGE*idN+UP+1) /)
+       dl = (/ -dom%len%elts(EDGE*idE+RT+1), -dom%len%elts(EDGE*idNE+DG+1), -dom%len%elts(EDGE*idN+UP+1) /)
+    else
+       pl = dom%pedlen%elts(EDGE*id+1:EDGE*id+EDGE)
+       dl = dom%len%elts(EDGE*id+1:EDGE*id+EDGE)
+    end if
+    if (Laplace_sclr == 1) then
+       f0 = q(v,zlev)%data(d)%elts(id+1);  fE = q(v,zlev)%data(d)%elts(idE+1)
+       fNE = q(v,zlev)%data(d)%elts(idNE+1); fN = q(v,zlev)%data(d)%elts(idN+1)
+    else
+       f0 = Laplacian_scalar(v)%data(d)%elts(id+1);  fE = Laplacian_scalar(v)%data(d)%elts(idE+1)
+       fNE = Laplacian_scalar(v)%data(d)%elts(idNE+1); fN = Laplacian_scalar(v)%data(d)%elts(idN+1)
+    end if
+    g(RT+1) = (fE - f0) / dl(RT+1)
+    g(DG+1) = (f0 - fNE) / dl(DG+1)
+    g(UP+1) = (fN - f0) / dl(UP+1)
+    diff_scalar_flux = (-1)**Laplace_sclr * visc_sclr(v) * g * pl
+  end function diff_scalar_flux
+
+  ! diffusion of the velocity (du_source's physics_velo_source) at the
+  ! edges of (i, j): divu is grid%divu (order 1) or its Laplacian (order 2),
+  ! vort the vorticity (order 1) or rot rot of it (order 2)
+  function diff_velo_source (dom, i, j, zlev, offs, dims)
+    real(8), dimension(1:EDGE)     :: diff_velo_source
+    type(domain)                   :: dom
+    integer                        :: i, j, zlev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    integer :: id, idE, idN, idNE, idS, idW
+    real(8), dimension(1:EDGE) :: gd, cr
+    diff_velo_source = 0d0
+    if (Laplace_rotu == 0) return
+    id   = idx (i,   j,   offs, dims)
+    idE  = idx (i+1, j,   offs, dims)
+    idN  = idx (i,   j+1, offs, dims)
+    idNE = idx (i+1, j+1, offs, dims)
+    idS  = idx (i,   j-1, offs, dims)
+    idW  = idx (i-1, j,   offs, dims)
+    gd = 0d0
+    if (Laplace_divu /= 0) then
+       gd(RT+1) = (divu(idE+1) - divu(id+1))   / dom%len%elts(EDGE*id+RT+1)
+       gd(DG+1) = (divu(id+1)  - divu(idNE+1)) / dom%len%elts(EDGE*id+DG+1)
+       gd(UP+1) = (divu(idN+1) - divu(id+1))   / dom%len%elts(EDGE*id+UP+1)
+    end if
+    cr(RT+1) = (vort(TRIAG*id +LORT+1) - vort(TRIAG*idS+UPLT+1)) / dom%pedlen%elts(EDGE*id+RT+1)
+    cr(DG+1) = (vort(TRIAG*id +LORT+1) - vort(TRIAG*id +UPLT+1)) / dom%pedlen%elts(EDGE*id+DG+1)
+    cr(UP+1) = (vort(TRIAG*idW+LORT+1) - vort(TRIAG*id +UPLT+1)) / dom%pedlen%elts(EDGE*id+UP+1)
+    diff_velo_source = (-1)**(Laplace_rotu-1) * (visc_divu * gd - visc_rotu * cr)
+  end function diff_velo_source
 end module harness_hooks
 
 module harness_ic
@@ -145,13 +220,14 @@ program dump_trisk
 
   character(16)  :: case
   character(255) :: out
-  integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed
+  integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed, lsclr, ldivu, lrotu
   integer(8)     :: t_start, t_end, t_rate
-  real(8)        :: step
+  real(8)        :: step, vsclr(2), vdivu, vrotu
   logical        :: nanfill
-  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed
+  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu
 
   case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0
+  lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0
   read (5, nml=run)
   ic_case = case
 
@@ -164,7 +240,10 @@ program dump_trisk
   compressible  = .false.
   mode_split    = .false.
   optimize_grid = NO_OPTIM
-  Laplace_sclr  = 0; Laplace_divu = 0; Laplace_rotu = 0
+  Laplace_sclr  = lsclr; Laplace_divu = ldivu; Laplace_rotu = lrotu
+  visc_sclr(S_MASS) = vsclr(1); visc_sclr(S_TEMP) = vsclr(2)
+  visc_divu     = vdivu
+  visc_rotu     = vrotu
   radius        = 6.371d6
   omega         = 7.292d-5
   grav_accel    = 9.80616d0
@@ -184,6 +263,10 @@ program dump_trisk
 
   physics_scalar_flux => no_scalar_flux
   physics_velo_source => no_velo_source
+  if (lsclr /= 0 .or. lrotu /= 0) then
+     physics_scalar_flux => diff_scalar_flux
+     physics_velo_source => diff_velo_source
+  end if
 
   ! --- init_basic (main.f90:23-34), minus io ---
   call init_comm_mod

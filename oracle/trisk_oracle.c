@@ -10,6 +10,15 @@
  *   du_source / Qperp / get_weights                  ops.f90:757-785, 834-901, 1006-1021
  *   du_grad / gradi_e                                ops.f90:787-832, 1280-1301
  *   RK4, RK_sub_step                                 time_integr.f90:78-103, 206-225
+ *   Laplacian diffusion / hyperdiffusion (wto_trend_diff):
+ *     cal_divu_ml, cal_Laplacian_scalars, cal_Laplacian_divu
+ *                                                    multi_level.f90:26-30, 190-309, 579-617
+ *     cal_Laplacian_vector_rot, cal_Laplacian_rotu, post_vort
+ *                                                    multi_level.f90:99, 240-270, ops.f90:1159-1278
+ *     step1 itypes 1, 5, 8 and the vorticity of itype 0  ops.f90:196-201, 236-239, 256-275, 334-338
+ *     the physics hooks of the reference's test cases (scalar flux
+ *     (-1)^Ls visc grad(q or Laplacian q) pedlen, velocity source
+ *     (-1)^(Lr-1) (visc_divu grad divu - visc_rotu curl rotu))
  * with the evaluation order of the reference's expressions (compile with
  * -ffp-contract=off).  Halo-edge fluxes use the owning edge's dual length,
  * as the reference's ghost fill of horiz_flux does (multi_level.f90:111).
@@ -80,15 +89,144 @@ static void weights(const wto_grid* g, int64_t c, int offs, double w[5]) {
 typedef struct {
   double *pvL, *pvU, *qe, *hm, *ht, *ke, *bern, *exner;
   double *press, *press_lower, *geopot, *geopot_lower, *ped_flux;
+  /* diffusion: owner's edge length of halo edges, divu and its Laplacian,
+   * the scalars' Laplacians, the vorticity (or rot rot of it) on LORT / UPLT
+   * of every cell, rot of the vorticity on edges */
+  double *len_flux, *divu, *lapdivu, *lapm, *lapt, *vortL, *vortU, *rotu;
 } work_t;
+
+/* node field of one layer: owners' values into the halo */
+static void node_ghost(const wto_grid* g, double* f) {
+  for (int64_t c = 0; c < g->ncells; ++c) {
+    int64_t s = g->node_src[c];
+    if (s >= 0 && s != c) f[c] = f[s];
+  }
+}
+
+static void edge_ghost(const wto_grid* g, double* u) {
+  const int64_t NC = g->ncells;
+  for (int64_t c = 0; c < NC; ++c)
+    for (int e = 0; e < 3; ++e) {
+      int64_t slot = 3 * c + e, s = g->edge_src[slot];
+      if (s >= 0 && s != slot) u[e * NC + c] = (double)g->edge_sign[slot] * u[(s % 3) * NC + s / 3];
+    }
+}
+
+static int is_master_cell(const wto_grid* g, int64_t c);
+
+/* div (ops.f90:1325-1343) of the edge field F on the masters, then the halo:
+ * cal_div followed by the ghost fill */
+static void div_masters(const wto_grid* g, const double* F, double* out) {
+  const int S = g->S;
+  const int64_t NC = g->ncells;
+  for (int64_t c = 0; c < NC; ++c) {
+    if (!is_master_cell(g, c)) continue;
+    const int64_t cW = c - 1, cS = c - S, cSW = c - S - 1;
+    out[c] = (F[c] - F[cW] + F[NC + cSW] - F[NC + c] + F[2 * NC + c] - F[2 * NC + cS]) * g->hex_inv[c];
+  }
+  node_ghost(g, out);
+}
+
+/* div of the edge field F as cal_divu_ml leaves it (multi_level.f90:608-614):
+ * cal_div over i, j in [0, n] of every lozenge with no ghost fill after it,
+ * so the halo row and column i = n or j = n hold each lozenge's own
+ * divergence of its ghost fluxes.  Away from the corner (n, n) that equals the
+ * owner's value; at that pentagon corner it does not.  The rest of the halo
+ * takes the owners' values (it is never read). */
+static void div_local(const wto_grid* g, const double* F, double* out) {
+  const int n = g->n, S = g->S;
+  const int64_t NC = g->ncells;
+  div_masters(g, F, out);
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = 0; j <= n; ++j)
+      for (int i = 0; i <= n; ++i) {
+        if (i < n && j < n) continue;
+        int64_t c = CELL(lz, i, j);
+        const int64_t cW = c - 1, cS = c - S, cSW = c - S - 1;
+        out[c] = (F[c] - F[cW] + F[NC + cSW] - F[NC + c] + F[2 * NC + c] - F[2 * NC + cS]) * g->hex_inv[c];
+      }
+}
+
+/* Laplacian of a node field (step1 itype 1, then cal_div): gradient flux
+ * (q(nb) - q) / len * pedlen with the owner's lengths on halo edges */
+static void laplacian_nodes(const wto_grid* g, const double* lenf, const double* pedf, const double* q, double* F,
+                            double* out, int plant) {
+  const int n = g->n, S = g->S;
+  const int64_t NC = g->ncells;
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = -1; j <= n; ++j)
+      for (int i = -1; i <= n; ++i) {
+        int64_t c = CELL(lz, i, j);
+        F[c] = (q[CELL(lz, i + 1, j)] - q[c]) / lenf[c] * pedf[c];
+        F[NC + c] = (q[c] - q[CELL(lz, i + 1, j + 1)]) / lenf[NC + c] * pedf[NC + c];
+        F[2 * NC + c] = (q[CELL(lz, i, j + 1)] - q[c]) / lenf[2 * NC + c] * pedf[2 * NC + c];
+      }
+  /* update_bdry of horiz_flux: halo edges take their owners' fluxes, which
+   * differ from the local ones where q holds a lozenge's own corner value */
+  if (!plant) edge_ghost(g, F);
+  div_masters(g, F, out);
+}
+
+/* circulation / area of every LORT and UPLT triangle of cells [-1, n]^2 of
+ * edge field u, and the pentagon corners' merged triangles (post_step1,
+ * post_vort) */
+static void vorticity(const wto_grid* g, const double* u, double* vL, double* vU) {
+  const int n = g->n, S = g->S;
+  const int64_t NC = g->ncells;
+  const double *len = g->len, *tri = g->triarea;
+#define UV(e, c) (u[(int64_t)(e) * NC + (c)] * len[(int64_t)(e) * NC + (c)])
+  for (int lz = 0; lz < 10; ++lz)
+    for (int j = -1; j <= n; ++j)
+      for (int i = -1; i <= n; ++i) {
+        int64_t c = CELL(lz, i, j), cE = CELL(lz, i + 1, j), cN = CELL(lz, i, j + 1);
+        vL[c] = (UV(0, c) + UV(2, cE) + UV(1, c)) / tri[c];
+        vU[c] = -(UV(1, c) + UV(2, c) + UV(0, cN)) / tri[NC + c];
+      }
+  for (int lz = 0; lz < 10; ++lz) {
+    { /* IJMINUS */
+      int64_t idS = CELL(lz, 0, -1), idW = CELL(lz, -1, 0), idSW = CELL(lz, -1, -1);
+      vL[idSW] = (UV(2, idS) - UV(0, idW) + UV(0, idSW)) / tri[idSW];
+      vU[idSW] = vL[idSW];
+    }
+    { /* IPLUSJMINUS */
+      int64_t id = CELL(lz, n, 0), idS = CELL(lz, n, -1), idSW = CELL(lz, n - 1, -1);
+      vL[idSW] = (-UV(0, id) + UV(0, idSW) + UV(1, idSW)) / tri[idSW];
+      vU[idS] = vL[idSW];
+    }
+    { /* IMINUSJPLUS */
+      int64_t id = CELL(lz, 0, n), idW = CELL(lz, -1, n), idSW = CELL(lz, -1, n - 1);
+      vU[idSW] = (UV(2, id) - UV(1, idSW) - UV(2, idSW)) / tri[NC + idSW];
+      vL[idW] = vU[idSW];
+    }
+    { /* IJPLUS */
+      int64_t id = CELL(lz, n, n), idN = CELL(lz, n, n + 1);
+      vL[id] = (UV(0, id) - UV(0, idN) - UV(2, id)) / tri[id];
+      vU[id] = vL[id];
+    }
+  }
+#undef UV
+}
+
+static void trend_impl(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
+                       double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep, const wto_diff* dif);
 
 void wto_trend(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
                double* dmass, double* dtemp, double* dvelo) {
-  wto_trend_parts(g, L, mass, temp, velo, dmass, dtemp, dvelo, NULL);
+  trend_impl(g, L, mass, temp, velo, dmass, dtemp, dvelo, NULL, NULL);
 }
 
 void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
                      double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep) {
+  trend_impl(g, L, mass, temp, velo, dmass, dtemp, dvelo, keep, NULL);
+}
+
+void wto_trend_diff(const wto_grid* g, const wto_diff* dif, int L, const double* mass, const double* temp,
+                    const double* velo, double* dmass, double* dtemp, double* dvelo) {
+  trend_impl(g, L, mass, temp, velo, dmass, dtemp, dvelo, NULL, dif);
+}
+
+static void trend_impl(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
+                       double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep, const wto_diff* dif) {
   const int n = g->n, S = g->S;
   const int64_t NC = g->ncells;
   const double grav = g->grav, rho0 = g->rho0;
@@ -101,6 +239,20 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
   w.hm = (double*)calloc(3 * NC, sizeof(double));
   w.ht = (double*)calloc(3 * NC, sizeof(double));
   w.ped_flux = (double*)calloc(3 * NC, sizeof(double));
+  const int dsc = dif ? dif->lsclr : 0, ddv = dif ? dif->ldivu : 0, drt = dif ? dif->lrotu : 0;
+  double* gflux = NULL; /* gradient fluxes of the Laplacians */
+  w.len_flux = w.divu = w.lapdivu = w.lapm = w.lapt = w.vortL = w.vortU = w.rotu = NULL;
+  if (dif) {
+    w.len_flux = (double*)calloc(3 * NC, sizeof(double));
+    w.divu = (double*)calloc(NC, sizeof(double));
+    w.lapdivu = (double*)calloc(NC, sizeof(double));
+    w.lapm = (double*)calloc(NC, sizeof(double));
+    w.lapt = (double*)calloc(NC, sizeof(double));
+    w.vortL = (double*)calloc(NC, sizeof(double));
+    w.vortU = (double*)calloc(NC, sizeof(double));
+    w.rotu = (double*)calloc(3 * NC, sizeof(double));
+    gflux = (double*)calloc(3 * NC, sizeof(double));
+  }
   memset(dmass, 0, sizeof(double) * NC * L);
   memset(dtemp, 0, sizeof(double) * NC * L);
   memset(dvelo, 0, sizeof(double) * 3 * NC * L);
@@ -113,6 +265,7 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
     for (int e = 0; e < 3; ++e) {
       int64_t slot = 3 * c + e, s = g->edge_src[slot];
       w.ped_flux[e * NC + c] = (s >= 0 && s != slot) ? ped[(s % 3) * NC + s / 3] : ped[e * NC + c];
+      if (w.len_flux) w.len_flux[e * NC + c] = (s >= 0 && s != slot) ? len[(s % 3) * NC + s / 3] : len[e * NC + c];
     }
 
   /* surface pressure and geopotential (cal_surf_press) on [0, n]^2 */
@@ -155,6 +308,38 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
           w.geopot[c] = w.geopot_lower[c] + grav * rho_dz / (rho0 * 1.0);
           w.press_lower[c] = p_upper;
         }
+    /* diffusion (the order of trend_ml, multi_level.f90:26-30): divu of the
+     * velocity and its Laplacian, the scalars' Laplacians, the vorticity and
+     * (order 2) rot rot of it (multi_level.f90:99) */
+    if (ddv) {
+      for (int64_t c = 0; c < NC; ++c)
+        for (int e = 0; e < 3; ++e) gflux[e * NC + c] = U(e, c) * w.ped_flux[e * NC + c];
+      if (dif->plant_owner_halo)
+        div_masters(g, gflux, w.divu);
+      else
+        div_local(g, gflux, w.divu);
+      if (ddv == 2) laplacian_nodes(g, w.len_flux, w.ped_flux, w.divu, gflux, w.lapdivu, dif->plant_owner_halo);
+    }
+    if (dsc == 2) {
+      laplacian_nodes(g, w.len_flux, w.ped_flux, m, gflux, w.lapm, dif->plant_owner_halo);
+      laplacian_nodes(g, w.len_flux, w.ped_flux, t, gflux, w.lapt, dif->plant_owner_halo);
+    }
+    if (drt) {
+      vorticity(g, u, w.vortL, w.vortU);
+      if (drt == 2) {
+        memset(w.rotu, 0, sizeof(double) * 3 * NC);
+        for (int lz = 0; lz < 10; ++lz)
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+              int64_t c = CELL(lz, i, j), cS = CELL(lz, i, j - 1), cW = CELL(lz, i - 1, j);
+              w.rotu[c] = -(w.vortL[c] - w.vortU[cS]) / PED(0, c);
+              w.rotu[NC + c] = PED(1, c) != 0.0 ? -(w.vortL[c] - w.vortU[c]) / PED(1, c) : 0.0;
+              w.rotu[2 * NC + c] = -(w.vortL[cW] - w.vortU[c]) / PED(2, c);
+            }
+        edge_ghost(g, w.rotu);
+        vorticity(g, w.rotu, w.vortL, w.vortU);
+      }
+    }
     /* potential vorticity on triangles LORT(i,j), UPLT(i,j), (i,j) in [-1, n]^2 */
     WTO_PARALLEL_LZ_J
     for (int lz = 0; lz < 10; ++lz)
@@ -180,8 +365,19 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
           if (i >= 0) w.qe[2 * NC + c] = interp(w.pvU[c], w.pvL[CELL(lz, i - 1, j)]);
           for (int e = 0; e < 3; ++e) {
             double udual = U(e, c) * w.ped_flux[e * NC + c];
-            w.hm[e * NC + c] = udual * interp(m[c], m[nb[e]]) + 0.0;
-            w.ht[e * NC + c] = udual * interp(t[c], t[nb[e]]) + 0.0;
+            double pm = 0.0, pt = 0.0;
+            if (dsc) { /* physics_scalar_flux: (-1)^Ls visc grad(q or Laplacian q) pedlen */
+              const double* qm = dsc == 1 ? m : w.lapm;
+              const double* qt = dsc == 1 ? t : w.lapt;
+              const double sg = (dsc % 2) ? -1.0 : 1.0;
+              const double dl = w.len_flux[e * NC + c], pl = w.ped_flux[e * NC + c];
+              const double gm = e == 1 ? (qm[c] - qm[nb[e]]) / dl : (qm[nb[e]] - qm[c]) / dl;
+              const double gt = e == 1 ? (qt[c] - qt[nb[e]]) / dl : (qt[nb[e]] - qt[c]) / dl;
+              pm = sg * dif->visc_m * gm * pl;
+              pt = sg * dif->visc_t * gt * pl;
+            }
+            w.hm[e * NC + c] = udual * interp(m[c], m[nb[e]]) + pm;
+            w.ht[e * NC + c] = udual * interp(t[c], t[nb[e]]) + pt;
           }
         }
     /* pentagon corners (post_step1) */
@@ -324,8 +520,22 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
                           (w.bern[idN] - w.bern[id]) / LEN(2, id)};
           double gE[3] = {(w.exner[idE] - w.exner[id]) / LEN(0, id), (w.exner[id] - w.exner[idNE]) / LEN(1, id),
                           (w.exner[idN] - w.exner[id]) / LEN(2, id)};
+          double phys[3] = {0.0, 0.0, 0.0};
+          if (drt) { /* physics_velo_source: (-1)^(Lr-1) (visc_divu grad divu - visc_rotu curl rotu) */
+            const double* D = ddv == 2 ? w.lapdivu : w.divu;
+            double gd[3] = {0.0, 0.0, 0.0};
+            if (ddv) {
+              gd[0] = (D[idE] - D[id]) / LEN(0, id);
+              gd[1] = (D[id] - D[idNE]) / LEN(1, id);
+              gd[2] = (D[idN] - D[id]) / LEN(2, id);
+            }
+            const double cr[3] = {(w.vortL[id] - w.vortU[idS]) / PED(0, id), (w.vortL[id] - w.vortU[id]) / PED(1, id),
+                                  (w.vortL[idW] - w.vortU[id]) / PED(2, id)};
+            const double sg = (drt % 2) ? 1.0 : -1.0;
+            for (int e = 0; e < 3; ++e) phys[e] = sg * (dif->visc_divu * gd[e] - dif->visc_rotu * cr[e]);
+          }
           for (int e = 0; e < 3; ++e) {
-            double src = -q[e] + 0.0 * LEN(e, id);
+            double src = -q[e] + phys[e] * LEN(e, id);
             du[e * NC + id] = src / LEN(e, id) - gB[e] - the[e] * gE[e];
             if (keep) keep->du_src[((int64_t)k * 3 + e) * NC + id] = src;
           }
@@ -344,6 +554,17 @@ void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double*
   free(w.hm);
   free(w.ht);
   free(w.ped_flux);
+  if (dif) {
+    free(w.len_flux); free(w.divu); free(w.lapdivu); free(w.lapm); free(w.lapt);
+    free(w.vortL); free(w.vortU); free(w.rotu); free(gflux);
+  }
+}
+
+static int is_master_cell(const wto_grid* g, int64_t c) {
+  const int S = g->S, n = g->n;
+  int64_t r = c % ((int64_t)S * S);
+  int j = (int)(r / S) - H, i = (int)(r % S) - H;
+  return (i >= 0 && i < n && j >= 0 && j < n) || c == g->south_pole || c == g->north_pole;
 }
 
 /* owned cells plus the pole masters: the cells an RK sub-step updates */
@@ -354,7 +575,20 @@ static int is_prognostic(const wto_grid* g, int64_t c) {
   return (i >= 0 && i < n && j >= 0 && j < n) || c == g->south_pole || c == g->north_pole;
 }
 
+static void rk4_impl(const wto_grid* g, const wto_diff* dif, int L, double dt, int nsteps, double* mass,
+                     double* temp, double* velo);
+
 void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, double* temp, double* velo) {
+  rk4_impl(g, NULL, L, dt, nsteps, mass, temp, velo);
+}
+
+void wto_rk4_diff(const wto_grid* g, const wto_diff* dif, int L, double dt, int nsteps, double* mass, double* temp,
+                  double* velo) {
+  rk4_impl(g, dif, L, dt, nsteps, mass, temp, velo);
+}
+
+static void rk4_impl(const wto_grid* g, const wto_diff* dif, int L, double dt, int nsteps, double* mass,
+                     double* temp, double* velo) {
   const int64_t NC = g->ncells;
   const int64_t ns = NC * L, ne = 3 * NC * L;
   double *m1 = malloc(sizeof(double) * ns), *t1 = malloc(sizeof(double) * ns), *u1 = malloc(sizeof(double) * ne);
@@ -369,7 +603,7 @@ void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, doub
       const double* ym = st == 0 ? mass : m1;
       const double* yt = st == 0 ? temp : t1;
       const double* yu = st == 0 ? velo : u1;
-      wto_trend(g, L, ym, yt, yu, dm, dtm, du);
+      trend_impl(g, L, ym, yt, yu, dm, dtm, du, NULL, dif);
       double* om = st == 3 ? mass : m1;
       double* ot = st == 3 ? temp : t1;
       double* ou = st == 3 ? velo : u1;

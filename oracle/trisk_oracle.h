@@ -53,6 +53,22 @@ typedef struct {
 void wto_trend_parts(const wto_grid* g, int L, const double* mass, const double* temp, const double* velo,
                      double* dmass, double* dtemp, double* dvelo, wto_trend_keep* keep);
 
+/* Laplacian diffusion (order 1) or iterated hyperdiffusion (order 2) of the
+ * scalars (lsclr), div u (ldivu) and rot u (lrotu) through the physics
+ * hooks in the form of the reference's test cases; the velocity term needs
+ * lrotu != 0 (its sign is (-1)^(lrotu-1)) */
+typedef struct {
+  int lsclr, ldivu, lrotu;
+  double visc_m, visc_t, visc_divu, visc_rotu;
+  /* test-only planted mistake, 0 in every real use: 1 gives the halo of divu
+   * its owners' values and computes the Laplacians' halo fluxes locally */
+  int plant_owner_halo;
+} wto_diff;
+void wto_trend_diff(const wto_grid* g, const wto_diff* dif, int L, const double* mass, const double* temp,
+                    const double* velo, double* dmass, double* dtemp, double* dvelo);
+void wto_rk4_diff(const wto_grid* g, const wto_diff* dif, int L, double dt, int nsteps, double* mass, double* temp,
+                  double* velo);
+
 /* nsteps low-storage RK4 steps (time_integr.f90:78-103, 206-225) in place;
  * the state is ghost filled on entry and on exit. */
 void wto_rk4(const wto_grid* g, int L, double dt, int nsteps, double* mass, double* temp, double* velo);

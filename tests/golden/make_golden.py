@@ -55,6 +55,15 @@ CASES = [
     ("tc5_J5", 5, "tc5", 1, 1, 600.0),
 ]
 
+# Laplacian diffusion (order 1) and iterated hyperdiffusion (order 2) of
+# scalars, div u and rot u through the physics hooks (J4 ocean, 2 layers)
+DIFF_CASES = [
+    ("ocean_diff1_J4", 4, "ocean", 2, 2, 600.0,
+     dict(lsclr=1, ldivu=1, lrotu=1, vsclr=(2e6, 1e6), vdivu=3e6, vrotu=2e6)),
+    ("ocean_diff2_J4", 4, "ocean", 2, 2, 600.0,
+     dict(lsclr=2, ldivu=2, lrotu=2, vsclr=(5e17, 3e17), vdivu=6e17, vrotu=4e17)),
+]
+
 
 def read_records(path):
     raw = open(path, "rb").read()
@@ -214,11 +223,17 @@ def grid_fixture(J):
     print(f"grid_J{J}.npz written")
 
 
-def trisk_fixture(name, J, case, L, nsteps, dt):
+def trisk_fixture(name, J, case, L, nsteps, dt, diff=None):
     param, ml, dl = PARAMS[J]
+    extra = ""
+    if diff:
+        fd = lambda x: f"{x:.17e}".replace("e", "d")  # a Fortran double literal
+        extra = (f", lsclr={diff['lsclr']}, ldivu={diff['ldivu']}, lrotu={diff['lrotu']}, "
+                 f"vsclr={fd(diff['vsclr'][0])},{fd(diff['vsclr'][1])}, vdivu={fd(diff['vdivu'])}, "
+                 f"vrotu={fd(diff['vrotu'])}")
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "d.bin")
-        run_ref(param, f"&run case='{case}', zlev={L}, nsteps={nsteps}, step={dt}, out='OUT' /\n", p)
+        run_ref(param, f"&run case='{case}', zlev={L}, nsteps={nsteps}, step={dt}, out='OUT'{extra} /\n", p)
         o = load_dump(p)
     n, S, ND, L_, cover = lozenge_cover(o)
     assert L_ == L
@@ -266,6 +281,9 @@ def trisk_fixture(name, J, case, L, nsteps, dt):
         data[f"mass{s}"] = field(f"s{s:02d}_mass", 1, False)
         data[f"temp{s}"] = field(f"s{s:02d}_temp", 1, False)
         data[f"velo{s}"] = field(f"s{s:02d}_velo", 3, False)
+    if diff:
+        data.update(laplace=np.array([diff["lsclr"], diff["ldivu"], diff["lrotu"]]),
+                    visc=np.array([diff["vsclr"][0], diff["vsclr"][1], diff["vdivu"], diff["vrotu"]]))
     np.savez_compressed(os.path.join(OUT, f"trisk_{name}.npz"), **data)
     print(f"trisk_{name}.npz written")
 
@@ -605,6 +623,9 @@ def main():
             grid_fixture(J)
     if "trisk" in what:
         for c in CASES:
+            trisk_fixture(*c)
+    if "diffusion" in what or "trisk" in what:
+        for c in DIFF_CASES:
             trisk_fixture(*c)
     if "elliptic" in what:
         for c in ELLIPTIC_CASES:
