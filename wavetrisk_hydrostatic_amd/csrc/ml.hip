@@ -93,8 +93,9 @@ __device__ __forceinline__ int64_t cellf(const PairGeo& g, int lz, int i, int j)
 // ---------------------------------------------------------------------------
 constexpr int ML_KC = 6;
 __host__ __device__ constexpr int ml_chunks(int nfl) { return (nfl + ML_KC - 1) / ML_KC; }
-// the velocity transform's gathers need more threads in flight: two layers
-constexpr int ML_KCV = 2;
+// the velocity transform's gathers need the threads in flight: one layer
+// per thread (two or six ran slower at J8 L30)
+constexpr int ML_KCV = 1;
 __host__ __device__ constexpr int ml_chunks_v(int L) { return (L + ML_KCV - 1) / ML_KCV; }
 
 __global__ void ml_scalar_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
