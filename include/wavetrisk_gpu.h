@@ -200,6 +200,30 @@ int wtg_get_stage_shape(wtg_ctx* ctx, int* shape, float* ms);
  * reference.  wtg_rk4_timed always launches eagerly (it times each stage). */
 int wtg_set_graph(wtg_ctx* ctx, int enable);
 
+/* Laplacian diffusion (order 1) or iterated hyperdiffusion (order 2) in the
+ * trend of wtg_trend, wtg_rk4 and the split-explicit steps; replaces the
+ * reference's Laplace_sclr / Laplace_divu / Laplace_rotu, visc_sclr(S_MASS),
+ * visc_sclr(S_TEMP), visc_divu, visc_rotu (shared.f90:226, 238) with its test
+ * cases' physics hooks (physics_scalar_flux, physics_velo_source;
+ * ops.f90:360, 511, 778; e.g. test/seamount/test_case_module.f90:106-156):
+ *   mass / temperature flux += (-1)^Ls visc grad(q or Laplacian q) pedlen
+ *   velocity source          += (-1)^(Lr-1) (visc_divu grad(div u or its
+ *                               Laplacian) - visc_rotu curl(vorticity or
+ *                               rot rot of it))
+ * computed as trend_ml does (cal_divu_ml, cal_Laplacian_scalars,
+ * cal_Laplacian_divu, cal_Laplacian_vector_rot; multi_level.f90:26-30, 99,
+ * 190-309, 579-617).  Orders are 0 (off), 1 or 2; the velocity term is on
+ * when laplace_rotu is, and laplace_divu needs laplace_rotu (the hook's
+ * sign).  d == NULL or all orders 0 switches diffusion off.  FP64 only: the
+ * FP32-bulk precision, the stage kernels other than the default, and the
+ * multi-level step refuse a context with diffusion on.  The first call
+ * allocates the pre-pass buffers (up to 14 fields of [layers][ncells]). */
+typedef struct {
+  int laplace_sclr, laplace_divu, laplace_rotu;
+  double visc_mass, visc_temp, visc_divu, visc_rotu;
+} wtg_diffusion;
+int wtg_set_diffusion(wtg_ctx* ctx, const wtg_diffusion* d);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

@@ -55,6 +55,8 @@ def load():
         _lib.wto_rk4_split.restype = C.c_int
         _lib.wto_rk_split.argtypes = [C.c_void_p] * 3 + [C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
         _lib.wto_rk_split.restype = C.c_int
+        _lib.wto_rk_split_diff.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
+        _lib.wto_rk_split_diff.restype = C.c_int
     return _lib
 
 
@@ -223,11 +225,18 @@ class EllipticOracle(Oracle):
         self.lib.wto_grad_eta(C.byref(self.g), eta.ctypes.data, out.ctypes.data)
         return out
 
-    def rk4_split(self, mass, temp, velo, eta, tol=1e-13, maxit=400, nstages=4):
+    def rk4_split(self, mass, temp, velo, eta, tol=1e-13, maxit=400, nstages=4, diffusion=None):
         """one RK4_split step of size dt (the FsParams dt); returns the new
         (mass, temp, velo, eta) and the BiCGSTAB iteration count.  nstages 3,
-        2, 1: RK3_split, RK2_split, Euler_split"""
+        2, 1: RK3_split, RK2_split, Euler_split; diffusion: the trend's
+        (Ls, Ld, Lr, visc_m, visc_t, visc_divu, visc_rotu)"""
         mass, temp, velo, eta = (_f64(a).copy() for a in (mass, temp, velo, eta))
+        if diffusion is not None:
+            d = Diff(*diffusion)
+            it = self.lib.wto_rk_split_diff(C.byref(self.g), C.byref(self.fs), C.byref(d), self.weight.ctypes.data,
+                                            mass.shape[0], nstages, tol, maxit, mass.ctypes.data, temp.ctypes.data,
+                                            velo.ctypes.data, eta.ctypes.data)
+            return mass, temp, velo, eta, it
         it = self.lib.wto_rk_split(C.byref(self.g), C.byref(self.fs), self.weight.ctypes.data, mass.shape[0],
                                    nstages, tol, maxit, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data,
                                    eta.ctypes.data)

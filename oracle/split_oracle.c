@@ -83,8 +83,21 @@ void wto_grad_eta(const wto_grid* g, const double* eta, double* hf) {
 /* Euler_split, RK2_split, RK3_split, RK4_split (time_integr.f90:360-451): the
  * low-storage stages RK_split(h/(n-k)) from sol through q1 back to sol, then
  * free_surface_update */
+static int rk_split_impl(const wto_grid* g, const wto_fs_params* p, const wto_diff* dif, const double* wgt, int L,
+                         int nstages, double tol, int maxit, double* mass, double* temp, double* velo, double* eta);
+
 int wto_rk_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, int nstages, double tol,
                  int maxit, double* mass, double* temp, double* velo, double* eta) {
+  return rk_split_impl(g, p, NULL, wgt, L, nstages, tol, maxit, mass, temp, velo, eta);
+}
+
+int wto_rk_split_diff(const wto_grid* g, const wto_fs_params* p, const wto_diff* dif, const double* wgt, int L,
+                      int nstages, double tol, int maxit, double* mass, double* temp, double* velo, double* eta) {
+  return rk_split_impl(g, p, dif, wgt, L, nstages, tol, maxit, mass, temp, velo, eta);
+}
+
+static int rk_split_impl(const wto_grid* g, const wto_fs_params* p, const wto_diff* dif, const double* wgt, int L,
+                         int nstages, double tol, int maxit, double* mass, double* temp, double* velo, double* eta) {
   const int64_t NC = g->ncells, ns = NC * L, ne = 3 * NC * L;
   double *m1 = malloc(8 * ns), *t1 = malloc(8 * ns), *u1 = malloc(8 * ne);
   double *dm = malloc(8 * ns), *dtm = malloc(8 * ns), *du = malloc(8 * ne);
@@ -102,7 +115,10 @@ int wto_rk_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, i
     double* yt = st == 0 ? temp : t1;
     double* yu = st == 0 ? velo : u1;
     wto_barotropic_correction(g, L, eta, ym, yt);
-    wto_trend(g, L, ym, yt, yu, dm, dtm, du);
+    if (dif)
+      wto_trend_diff(g, dif, L, ym, yt, yu, dm, dtm, du);
+    else
+      wto_trend(g, L, ym, yt, yu, dm, dtm, du);
     const int last = st == nstages - 1;
     double* om = last ? mass : m1;
     double* ot = last ? temp : t1;

@@ -21,6 +21,10 @@ int wto_rk4_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, 
  * RK4_split (stage coefficients dt/nstages, ..., dt/2, dt) */
 int wto_rk_split(const wto_grid* g, const wto_fs_params* p, const double* wgt, int L, int nstages, double tol,
                  int maxit, double* mass, double* temp, double* velo, double* eta);
+/* the same with the trend's Laplacian diffusion / hyperdiffusion (trend_ml
+ * calls it in every stage of the split schemes too) */
+int wto_rk_split_diff(const wto_grid* g, const wto_fs_params* p, const wto_diff* dif, const double* wgt, int L,
+                      int nstages, double tol, int maxit, double* mass, double* temp, double* velo, double* eta);
 
 #ifdef __cplusplus
 }

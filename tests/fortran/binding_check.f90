@@ -4,6 +4,8 @@
 !
 !   binding_check grid <in> <out>   grid only (no GPU): info and hex_inv
 !   binding_check rk4  <in> <out>   one level: wtg_rk4 steps on the device
+!   binding_check rk4d <in> <out>   the same with hyperdiffusion (wtg_set_diffusion:
+!                                   orders 2, 2, 2, viscosities 4e16, 2e16, 6e16, 4e16)
 !   binding_check ml   <in> <out>   a hierarchy: wtg_ml_rk4_split steps with
 !                                   the FMG free surface (wtg_mg_*)
 !
@@ -23,6 +25,7 @@ program binding_check
   type(wtg_params) :: prm
   type(wtg_fs_params) :: fsp
   type(wtg_mg_params) :: mgp
+  type(wtg_diffusion) :: dif
   type(c_ptr), target :: grids(16), ctxs(16)
   type(c_ptr) :: ml, mg
   type arr
@@ -72,7 +75,11 @@ program binding_check
      call wtg_check(wtg_upload_state(ctxs(l), 0, s(l)%mass, s(l)%temp, s(l)%velo), ctxs(l))
   end do
   iters = 0
-  if (trim(mode) == 'rk4') then
+  if (trim(mode) == 'rk4' .or. trim(mode) == 'rk4d') then
+     if (trim(mode) == 'rk4d') then
+        dif = wtg_diffusion(2, 2, 2, 4d16, 2d16, 6d16, 4d16)
+        call wtg_check(wtg_set_diffusion(ctxs(1), dif), ctxs(1))
+     end if
      call wtg_check(wtg_rk4(ctxs(1), 0, par(1), nsteps), ctxs(1))
   else if (trim(mode) == 'ml') then
      do l = 1, nl

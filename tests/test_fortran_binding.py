@@ -9,6 +9,9 @@ got back.
                              what the ctypes binding reads
   rk4 mode (GPU)             wtg_rk4 on one level equals the ctypes path's
                              result bit for bit
+  rk4d mode (GPU)            the same with wtg_set_diffusion (hyperdiffusion
+                             of every order 2) set through the module's
+                             wtg_diffusion type
   ml mode (GPU)              wtg_ml_rk4_split on a J4..J5 hierarchy with the
                              FMG free surface (wtg_mg_params from
                              wtg_mg_default_params, tolerances 1e-9) equals
@@ -78,19 +81,22 @@ def test_grid_through_fortran(tmp_path):
 
 @needs_exe
 @pytest.mark.gpu
-def test_rk4_through_fortran_equals_ctypes(tmp_path):
-    print("Fortran binding: wtg_rk4 J5 L4", flush=True)
+@pytest.mark.parametrize("mode", ["rk4", "rk4d"])
+def test_rk4_through_fortran_equals_ctypes(tmp_path, mode):
+    print(f"Fortran binding: wtg_rk4 J5 L4 ({mode})", flush=True)
     g = W.Grid(level=5, min_level=5, domain_level=2)
     L = 4
     topo, m, t, u = cases.ocean(g, L, seed=3)
     inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     _write(inp, [5], 5, 2, L, 2, [600.0, 1.0, 1.0], [(topo, m, t, u, np.zeros(g.layout.ncells))])
-    _run("rk4", inp, out)
+    _run(mode, inp, out)
     res, _ = _read(out, [g], L)
     fm, ft, fu, _ = res[0]
     ctx = W.Context(g, nlayers=L, nslots=2)
     ctx.set_topography(topo)
     ctx.upload_state(0, m, t, u)
+    if mode == "rk4d":
+        ctx.set_diffusion((2, 2, 2), (4e16, 2e16, 6e16, 4e16))
     ctx.rk4(0, 600.0, 2)
     cm, ct, cu = ctx.download_state(0)
     assert np.array_equal(fm, cm) and np.array_equal(ft, ct) and np.array_equal(fu, cu)

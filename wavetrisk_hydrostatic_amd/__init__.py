@@ -54,6 +54,12 @@ class _Params(C.Structure):
     _fields_ = [("nlayers", C.c_int), ("grav", C.c_double), ("ref_density", C.c_double), ("p_top", C.c_double)]
 
 
+class _Diffusion(C.Structure):
+    _fields_ = [("laplace_sclr", C.c_int), ("laplace_divu", C.c_int), ("laplace_rotu", C.c_int),
+                ("visc_mass", C.c_double), ("visc_temp", C.c_double), ("visc_divu", C.c_double),
+                ("visc_rotu", C.c_double)]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 
@@ -78,6 +84,7 @@ SIGNATURES = {
     "wtg_set_precision": (C.c_int, [_P, C.c_int]),
     "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
     "wtg_set_graph": (C.c_int, [_P, C.c_int]),
+    "wtg_set_diffusion": (C.c_int, [_P, C.POINTER(_Diffusion)]),
     "wtg_grid_shard_halo": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "wtg_ghost_fill_split_timed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D]),
     "wtg_fs_set_graph": (C.c_int, [_P, C.c_int]),
@@ -353,6 +360,14 @@ class Context:
     def set_graph(self, enable: bool):
         """rk4 replays its steps from a captured HIP graph (identical results)."""
         _check(_lib.wtg_set_graph(self._h, int(bool(enable))), self._h, "wtg_set_graph")
+
+    def set_diffusion(self, laplace=(0, 0, 0), visc=(0.0, 0.0, 0.0, 0.0)):
+        """Laplacian diffusion (order 1) or hyperdiffusion (order 2) in the
+        trend: laplace = (Laplace_sclr, Laplace_divu, Laplace_rotu), visc =
+        (visc of mass, of temperature, visc_divu, visc_rotu); all orders 0
+        switch it off.  FP64 precision on a single level only."""
+        d = _Diffusion(*[int(x) for x in laplace], *[float(x) for x in visc])
+        _check(_lib.wtg_set_diffusion(self._h, C.byref(d)), self._h, "wtg_set_diffusion")
 
     def set_stage_shape(self, shape: int):
         """SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS or SHAPE_ONE_LAYER

@@ -128,6 +128,18 @@ struct wtg_ctx {
   // members' solves running one after another on the stream).  Owned here,
   // built on first use; a view owns no device memory.
   std::vector<wtg_ctx*> views;
+  // Laplacian diffusion / hyperdiffusion of the trend (wtg_set_diffusion),
+  // off while every order is 0; the pre-pass buffers are allocated for the
+  // orders in use, [layer of every member][...][ncells]
+  wtg_diffusion dif{};
+  bool dif_on = false;
+  double *dif_divu = nullptr, *dif_lapd = nullptr;  // div u and its Laplacian
+  double *dif_lapm = nullptr, *dif_lapt = nullptr;  // Laplacians of mass and temperature
+  double *dif_flux = nullptr;                       // [3] gradient fluxes of a Laplacian
+  double *dif_vort = nullptr;                       // [2] vorticity (or rot rot) on LORT / UPLT
+  double *dif_rotu = nullptr;                       // [3] rot u on edges
+  double *dif_phm = nullptr, *dif_pht = nullptr;    // [3] scalar diffusive fluxes
+  double *dif_phu = nullptr;                        // [3] velocity source
 };
 
 extern thread_local std::string g_create_error;
@@ -149,6 +161,11 @@ inline int wtg_dalloc(wtg_ctx* c, Tp** p, size_t count) {
   *p = (Tp*)q;
   return 0;
 }
+
+// diffusion (diffusion.hip): buffers of the orders in c->dif, and the
+// pre-pass that fills dif_phm / dif_pht / dif_phu from a ghost-filled state
+int wtg_diffusion_alloc(wtg_ctx* c);
+int wtg_diffusion_prepass(wtg_ctx* c, const double* m, const double* t, const double* u);
 
 // allocates the free-surface fields on first use (elliptic.hip); an
 // ensemble context holds them member-major ([nmembers][ncells] per field)

@@ -943,6 +943,10 @@ int check(wtg_ml* m, int slot) {
     if (!c->topo_set) return wtg_fail(m->lv[m->nl - 1], WTG_E_STATE, "level %d: topography not set", l);
     if (c->precision != WTG_PRECISION_FP64)
       return wtg_fail(m->lv[m->nl - 1], WTG_E_STATE, "the multi-level step runs in FP64 (level %d)", l);
+    if (c->dif_on)
+      return wtg_fail(m->lv[m->nl - 1], WTG_E_STATE,
+                      "level %d: diffusion is supported on single levels only (compute-or-restrict of its "
+                      "fluxes over a hierarchy is not built)", l);
   }
   return 0;
 }

@@ -74,6 +74,10 @@ struct StageArgs {
   // velocity update adds theta1 g grad eta to the trend (eta [NC], per member)
   const double* eta;
   double th1;
+  // diffusion hooks (pair kernel, DIFF): scalar diffusive fluxes on the edges
+  // of the region cells and velocity source on the owned edges, [L][3][NC]
+  // (null: that hook is off)
+  const double *phm, *pht, *phu;
 };
 
 __device__ __forceinline__ double interp(double a, double b) { return 0.5 * (a + b); }
@@ -88,6 +92,8 @@ __device__ __forceinline__ StageArgs member_args(const StageArgs& a0) {
   a.om += s1; a.ot += s1; a.ou += s3;
   a.g.topo += (int64_t)blockIdx.y * a0.NC;  // each member has its own bathymetry
   if (a.eta) a.eta += (int64_t)blockIdx.y * a0.NC;
+  if (a.phm) a.phm += s3, a.pht += s3;
+  if (a.phu) a.phu += s3;
   return a;
 }
 
@@ -808,8 +814,10 @@ __device__ __forceinline__ void weights_lds(const GeoS& gs, int y, int offs, dou
 // source and scalar trends (the k_* arrays) for the coarser levels of a
 // hierarchy; SPLIT: the split-explicit stage, out = q + h (trend + theta1 g
 // grad eta) on edges (u_star, barotropic_2d.f90:30-50, in split_axpy_kernel's
-// order).  Separate instantiations, so that the plain step carries neither.
-template <bool KEEP, bool SPLIT>
+// order); DIFF: the diffusion hooks' values (phm, pht, phu) enter the fluxes
+// and the velocity source.  Separate instantiations, so that the plain step
+// carries none of them.
+template <bool KEEP, bool SPLIT, bool DIFF>
 __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_in) {
   const StageArgs a = member_args(a_in);
   __shared__ GeoS gs;
@@ -949,6 +957,18 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
       nu1 = ldo(a.yu + ke32 + NC, xb);
       nu2 = ldo(a.yu + ke32 + 2 * NC, xb);
     }
+    // the layer's diffusion hooks (clamped layer, unconditional like the prefetch)
+    double dpm[3] = {0.0, 0.0, 0.0}, dpt[3] = {0.0, 0.0, 0.0}, dpu[3] = {0.0, 0.0, 0.0};
+    if (DIFF) {
+      const int64_t kd3 = 3 * (int64_t)min(k, a.L - 1) * NC;
+      for (int e = 0; e < 3; ++e) {
+        if (a.phm) {
+          dpm[e] = ldo(a.phm + kd3 + e * NC, xb);
+          dpt[e] = ldo(a.pht + kd3 + e * NC, xb);
+        }
+        if (a.phu) dpu[e] = ldo(a.phu + kd3 + e * NC, obl);
+      }
+    }
 
     // --- 2. column scan; PV, fluxes; kinetic energy, Bernoulli, Exner ---------
     if (lay) {
@@ -997,12 +1017,12 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
         sl.pvL[x] = (gs.cor[0][x] + circL) / massL;
         sl.pvU[x] = (gs.cor[1][x] + circU) / massU;
         const double f0 = gs.pedf[0][x], f1 = gs.pedf[1][x], f2 = gs.pedf[2][x];
-        sl.hm[0][x] = u0 * f0 * interp(m, mE) + 0.0;
-        sl.hm[1][x] = u1 * f1 * interp(m, mNE) + 0.0;
-        sl.hm[2][x] = u2 * f2 * interp(m, mN) + 0.0;
-        sl.ht[0][x] = u0 * f0 * interp(t, sl.t[xE]) + 0.0;
-        sl.ht[1][x] = u1 * f1 * interp(t, sl.t[xNE]) + 0.0;
-        sl.ht[2][x] = u2 * f2 * interp(t, sl.t[xN]) + 0.0;
+        sl.hm[0][x] = u0 * f0 * interp(m, mE) + dpm[0];
+        sl.hm[1][x] = u1 * f1 * interp(m, mNE) + dpm[1];
+        sl.hm[2][x] = u2 * f2 * interp(m, mN) + dpm[2];
+        sl.ht[0][x] = u0 * f0 * interp(t, sl.t[xE]) + dpt[0];
+        sl.ht[1][x] = u1 * f1 * interp(t, sl.t[xNE]) + dpt[1];
+        sl.ht[2][x] = u2 * f2 * interp(t, sl.t[xN]) + dpt[2];
         if (KEEP) {
           for (int e = 0; e < 3; ++e) {
             sto(a.k_hm + ke3 + e * NC, xb, sl.hm[e][x]);
@@ -1096,7 +1116,7 @@ __global__ __launch_bounds__(THREADS3) void trisk_stage_pair_kernel(StageArgs a_
         ge[2] = a.grav * (ldo(a.eta, ob + 8u * (uint32_t)S) - e0) / ll[2];
       }
       for (int e = 0; e < 3; ++e) {
-        double src = -qp[e] + 0.0 * ll[e];
+        double src = -qp[e] + dpu[e] * ll[e];
         double du = src / ll[e] - gB[e] - the[e] * gX[e];
         sto(a.ou + ke3 + e * NC, ob, a.trend_only ? du : bu[e] + a.h * (SPLIT ? du + a.th1 * ge[e] : du));
         if (KEEP) sto(a.k_src + ke3 + e * NC, ob, src);
@@ -2144,6 +2164,14 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
     a.eta = split_eta;
     a.th1 = c->fs_theta1;
   }
+  if (c->dif_on) {
+    if (c->stage_version != 3 || keep)
+      return fail(c, WTG_E_STATE, "diffusion needs the layer-pair stage kernel on a single level");
+    if (int rc = wtg_diffusion_prepass(c, y.m, y.t, y.u)) return rc;
+    a.phm = c->dif_phm;
+    a.pht = c->dif_pht;
+    a.phu = c->dif_phu;
+  }
   if (keep) {
     if (c->stage_version != 3 || c->nmembers != 1)
       return fail(c, WTG_E_STATE, "kept stage quantities need the layer-pair kernel on a single-member context");
@@ -2184,11 +2212,15 @@ int launch_stage(wtg_ctx* c, const wtg_ctx::Slot& y, const wtg_ctx::Slot& q, con
   else if (c->stage_version == 2)
     hipLaunchKernelGGL(trisk_stage_kernel2, grid, dim3(THREADS2), 0, c->stream, a);
   else if (keep)
-    hipLaunchKernelGGL((trisk_stage_pair_kernel<true, false>), grid, dim3(THREADS3), 0, c->stream, a);
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<true, false, false>), grid, dim3(THREADS3), 0, c->stream, a);
+  else if (split_eta && c->dif_on)
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, true, true>), grid, dim3(THREADS3), 0, c->stream, a);
   else if (split_eta)
-    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, true>), grid, dim3(THREADS3), 0, c->stream, a);
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, true, false>), grid, dim3(THREADS3), 0, c->stream, a);
+  else if (c->dif_on)
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, false, true>), grid, dim3(THREADS3), 0, c->stream, a);
   else
-    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, false>), grid, dim3(THREADS3), 0, c->stream, a);
+    hipLaunchKernelGGL((trisk_stage_pair_kernel<false, false, false>), grid, dim3(THREADS3), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -3183,9 +3215,39 @@ int wtg_set_precision(wtg_ctx* c, int precision) {
   if (!c) return WTG_E_ARG;
   if (precision != WTG_PRECISION_FP64 && precision != WTG_PRECISION_FP32_BULK)
     return fail(c, WTG_E_ARG, "unknown precision mode %d", precision);
+  if (precision == WTG_PRECISION_FP32_BULK && c->dif_on)
+    return fail(c, WTG_E_STATE, "diffusion is FP64 only; switch it off before the FP32-bulk precision");
   if (precision == WTG_PRECISION_FP32_BULK)
     if (int rc = f32_ready(c)) return rc;
   c->precision = precision;
+  return 0;
+}
+
+int wtg_set_diffusion(wtg_ctx* c, const wtg_diffusion* d) {
+  if (!c) return WTG_E_ARG;
+  const wtg_diffusion off{};
+  const wtg_diffusion& v = d ? *d : off;
+  for (int o : {v.laplace_sclr, v.laplace_divu, v.laplace_rotu})
+    if (o < 0 || o > 2) return fail(c, WTG_E_ARG, "Laplacian orders are 0, 1 or 2, not %d", o);
+  if (v.laplace_divu && !v.laplace_rotu)
+    return fail(c, WTG_E_ARG, "laplace_divu needs laplace_rotu: the velocity term's sign is (-1)^(laplace_rotu-1)");
+  const bool on = v.laplace_sclr || v.laplace_rotu;
+  if (on && c->precision != WTG_PRECISION_FP64)
+    return fail(c, WTG_E_STATE, "diffusion is FP64 only (the context is in the FP32-bulk precision)");
+  if (on && c->stage_version != 3)
+    return fail(c, WTG_E_STATE, "diffusion needs the layer-pair stage kernel (WTG_STAGE_KERNEL unset)");
+  c->dif = v;
+  c->dif_on = on;
+  if (on)
+    if (int rc = wtg_diffusion_alloc(c)) {
+      c->dif_on = false;
+      return rc;
+    }
+  // a captured RK4 graph holds the previous settings' launches
+  if (c->rk_graph) {
+    hipGraphExecDestroy(c->rk_graph);
+    c->rk_graph = nullptr;
+  }
   return 0;
 }
 

@@ -34,6 +34,12 @@ module wavetrisk_gpu
   ! solver parameters (lin_solve.f90:11-26); fill with wtg_mg_default_params
   ! first so that learned_cycles, which has no reference counterpart, gets
   ! its default
+  ! Laplace_sclr, Laplace_divu, Laplace_rotu (0, 1 or 2) and visc_sclr(S_MASS),
+  ! visc_sclr(S_TEMP), visc_divu, visc_rotu (shared.f90:226, 238)
+  type, bind(C) :: wtg_diffusion
+     integer(c_int) :: laplace_sclr, laplace_divu, laplace_rotu
+     real(c_double) :: visc_mass, visc_temp, visc_divu, visc_rotu
+  end type
   type, bind(C) :: wtg_mg_params
      real(c_double) :: coarse_tol, fine_tol
      integer(c_int) :: coarse_iter, max_vcycle, pre_iter, down_iter, up_iter, post_iter, max_srj_iter, &
@@ -99,6 +105,10 @@ module wavetrisk_gpu
      ! 1: wtg_rk4 replays its steps from a captured HIP graph
      integer(c_int) function wtg_set_graph (ctx, enable) bind(C)
        import; type(c_ptr), value :: ctx; integer(c_int), value :: enable
+     end function
+     ! Laplacian diffusion / hyperdiffusion in the trend (FP64, single level)
+     integer(c_int) function wtg_set_diffusion (ctx, d) bind(C)
+       import; type(c_ptr), value :: ctx; type(wtg_diffusion), intent(in) :: d
      end function
      ! ---- free surface (mode_split) ----
      integer(c_int) function wtg_fs_set_params (ctx, p) bind(C)
