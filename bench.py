@@ -160,7 +160,7 @@ def main():
     small = None if args.no_cpu else shallow_water_c1()
     ctx.synchronize()
     split = None if args.no_free_surface else split_step(grid, ctx, topo, mass, temp, velo, L)
-    fs = None if args.no_free_surface else free_surface(args.fs_level)
+    fs = None if args.no_free_surface else free_surface(args.fs_level, cpu=not args.no_cpu)
     ml = None if args.no_multilevel else multilevel()
 
     out = {
@@ -616,7 +616,26 @@ def multilevel(cases_=((5, 7, 20), (5, 8, 30)), dt=300.0, nsteps=3):
     return out
 
 
-def free_surface(fine: int):
+def free_surface_cpu(prob, dt: float = 1800.0):
+    """CPU baseline of the solve: the oracle's FMG (the reference's method:
+    FMG with BiCGSTAB on the coarsest level, tolerances 1e-9) on the same
+    draw and hierarchy, one thread"""
+    import numpy as np
+
+    from oracle.oracle import MgOracle
+
+    mg = MgOracle(prob.grids, prob.topos, prob.eta_olds, dt=dt, coarse_tol=1e-9, fine_tol=1e-9, coarse_iter=400,
+                  max_vcycle=40)
+    zero = [np.zeros(g.layout.ncells) for g in prob.grids]
+    t0 = time.perf_counter()
+    _, it, res = mg.fmg(zero, prob.rhs)
+    secs = time.perf_counter() - t0
+    return {"value": secs * 1e3, "unit": "ms per FMG solve to 1e-9", "cores": 1, "kind": "port",
+            "sample": "the C oracle's FMG (wto_fmg, one thread) on the same J5..J7 draw",
+            "iterations": [int(v) for v in it], "rel_resid": float(res[-1])}
+
+
+def free_surface(fine: int, cpu: bool = True):
     """The free-surface solve to 1e-9 relative residual on a uniform
     hierarchy J(fine-2)..J(fine) of a seeded ocean draw (dt = 1800 s): plain
     V-cycles from zero, the same cycles with the learned correction (the
@@ -699,6 +718,9 @@ def free_surface(fine: int):
            "draws": draws,
            "fp32_smoothing": fp32,
            "mlp_forward_64_rhs": mlp_line}
+    if cpu:
+        out["fmg_cpu_baseline"] = free_surface_cpu(prob)
+        out["fmg_cpu_baseline"]["gpu_speedup"] = out["fmg_cpu_baseline"]["value"] / out["fmg"]["ms"]
     out["batched_64_rhs"] = free_surface_batched(grids, mlp, nrhs=64)
     prob.close()
     mlp.close()

@@ -9,9 +9,10 @@
 !   * geometry (len, pedlen, areas, triarea, coriolis, topography),
 !   * the state after the first ghost fill and trend_ml's output
 !     (multi_level.f90:5-47) with its work arrays,
-!   * the state after each of nsteps RK4 steps (time_integr.f90:78-103).
+!   * the state after each of nsteps RK4 steps (time_integr.f90:78-103),
+!     or after every `every`-th step (tags numbered step / every).
 !
-! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out, ntimed.
+! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out, ntimed, every.
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
@@ -220,13 +221,13 @@ program dump_trisk
 
   character(16)  :: case
   character(255) :: out
-  integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed, lsclr, ldivu, lrotu
+  integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed, lsclr, ldivu, lrotu, every
   integer(8)     :: t_start, t_end, t_rate
   real(8)        :: step, vsclr(2), vdivu, vrotu
   logical        :: nanfill
-  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu
+  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every
 
-  case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0
+  case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0; every = 1
   lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0
   read (5, nml=run)
   ic_case = case
@@ -324,7 +325,8 @@ program dump_trisk
   do n = 1, nsteps
      call RK4 (sol(:,1:zlevels), wav_coeff(:,1:zlevels), trend_ml, step)
      call update_bdry (sol(:,1:zlevels), NONE, 1)
-     call dump_state ('s'//achar(48+n/10)//achar(48+mod(n,10))//'_')
+     if (mod(n, every) == 0) &
+          call dump_state ('s'//achar(48+(n/every)/10)//achar(48+mod(n/every,10))//'_')
   end do
   close (u)
   if (ntimed > 0) then

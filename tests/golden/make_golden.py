@@ -46,7 +46,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 H = 2
 
 # (param file, MIN_LEVEL, DOMAIN_LEVEL)
-PARAMS = {4: ("param_J4", 4, 1), 5: ("param_J5", 5, 2)}
+PARAMS = {4: ("param_J4", 4, 1), 5: ("param_J5", 5, 2), 6: ("param_J6", 6, 3), 7: ("param_J7", 7, 2)}
 CASES = [
     # name, J, case, layers, steps, dt
     ("tc5_J4", 4, "tc5", 1, 2, 1200.0),
@@ -146,7 +146,9 @@ def cell_index(S, lz, i, j):
     return lz * S * S + (j + H) * S + (i + H)
 
 
-def grid_fixture(J):
+def grid_fixture(J, geometry=True):
+    """grid_J<J>.npz: the reference's ghost maps and its geometry on the cells
+    the TRiSK stencils read; geometry=False: ghosts_J<J>.npz, the maps only"""
     param, ml, dl = PARAMS[J]
     with tempfile.TemporaryDirectory() as tmp:
         p_id, p_nan = os.path.join(tmp, "id.bin"), os.path.join(tmp, "nan.bin")
@@ -184,7 +186,9 @@ def grid_fixture(J):
                 c = cell_index(S, lz, i, j)
                 # geometry on the cells the TRiSK stencils read
                 inner = -1 <= i <= n and -1 <= j <= n
-                if inner:
+                if not geometry:
+                    pass
+                elif inner:
                     geo["pedlen"][:, c] = d["pedlen"][3 * e:3 * e + 3]
                     geo["triarea"][:, c] = d["triarea"][2 * e:2 * e + 2]
                     geo["coriolis"][:, c] = d["coriolis"][2 * e:2 * e + 2]
@@ -196,7 +200,7 @@ def grid_fixture(J):
                 elif i == n + 1 and j <= n:
                     geo["len"][2, c] = d["len"][3 * e + 2]
                     geo["area"][2:5, c] = [d["area_part"][p][e] for p in (2, 3, 4)]
-                if 0 <= i <= n and 0 <= j <= n:
+                if geometry and 0 <= i <= n and 0 <= j <= n:
                     geo["hex_inv"][0, c] = d["hex_inv"][e]
                 # ghost maps of halo cells
                 if 0 <= i < n and 0 <= j < n:
@@ -217,18 +221,22 @@ def grid_fixture(J):
                     if v != 0 and len(cand) == 1:
                         edge_src[3 * c + k] = cand[0][0]
                         edge_sign[3 * c + k] = int(np.sign(v) * cand[0][1])
-    np.savez_compressed(os.path.join(OUT, f"grid_J{J}.npz"), level=J, min_level=ml, domain_level=dl,
+    name = f"grid_J{J}" if geometry else f"ghosts_J{J}"
+    extra = {"geo_" + k: v for k, v in geo.items()} if geometry else {}
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), level=J, min_level=ml, domain_level=dl,
                         radius=o["params"][0], omega=o["params"][1], node_src=node_src, edge_src=edge_src,
-                        edge_sign=edge_sign, **{"geo_" + k: v for k, v in geo.items()})
-    print(f"grid_J{J}.npz written")
+                        edge_sign=edge_sign, **extra)
+    print(f"{name}.npz written")
 
 
-def trisk_fixture(name, J, case, L, nsteps, dt, diff=None):
+def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1):
+    """trisk_<name>.npz: the reference's initial state, trend and the state
+    after each RK4 step (after every `every`-th one: mass<k> is step k*every)"""
     param, ml, dl = PARAMS[J]
-    extra = ""
+    extra = f", every={every}" if every > 1 else ""
     if diff:
         fd = lambda x: f"{x:.17e}".replace("e", "d")  # a Fortran double literal
-        extra = (f", lsclr={diff['lsclr']}, ldivu={diff['ldivu']}, lrotu={diff['lrotu']}, "
+        extra += (f", lsclr={diff['lsclr']}, ldivu={diff['ldivu']}, lrotu={diff['lrotu']}, "
                  f"vsclr={fd(diff['vsclr'][0])},{fd(diff['vsclr'][1])}, vdivu={fd(diff['vdivu'])}, "
                  f"vrotu={fd(diff['vrotu'])}")
     with tempfile.TemporaryDirectory() as tmp:
@@ -272,12 +280,13 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None):
         hits = [x for x in cover[(lz, i, j)] if x[0] == 0] or cover[(lz, i, j)]
         dd, e = hits[0][1:]
         topo[cell_index(S, lz, i, j)] = o["dom"][dd]["topo"][e]
-    data = dict(level=J, min_level=ml, domain_level=dl, nlayers=L, dt=dt, nsteps=nsteps, radius=o["params"][0],
+    data = dict(level=J, min_level=ml, domain_level=dl, nlayers=L, dt=dt, nsteps=nsteps // every, every=every,
+                radius=o["params"][0],
                 omega=o["params"][1], grav=o["params"][2], ref_density=o["params"][3], topo=topo,
                 mass0=field("s0_mass", 1, True), temp0=field("s0_temp", 1, True),
                 velo0=field("s0_velo", 3, True), dmass=field("t0_dmass", 1, False),
                 dtemp=field("t0_dtemp", 1, False), dvelo=field("t0_dvelo", 3, False))
-    for s in range(1, nsteps + 1):
+    for s in range(1, nsteps // every + 1):
         data[f"mass{s}"] = field(f"s{s:02d}_mass", 1, False)
         data[f"temp{s}"] = field(f"s{s:02d}_temp", 1, False)
         data[f"velo{s}"] = field(f"s{s:02d}_velo", 3, False)
@@ -286,6 +295,19 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None):
                     visc=np.array([diff["vsclr"][0], diff["vsclr"][1], diff["vdivu"], diff["vrotu"]]))
     np.savez_compressed(os.path.join(OUT, f"trisk_{name}.npz"), **data)
     print(f"trisk_{name}.npz written")
+
+
+def c1_dt():
+    """CFL 0.8 time step of the J5 TC5 case: 0.8 min(edge length) / (sqrt(g
+    depth) + u0), the depth from the case's mass (the bench's formula)"""
+    sys.path.insert(0, ROOT)
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    topo, m, t, u = cases.tc5(g, 1)
+    own_e = g.layout.owned_edge_cells()
+    depth = float(np.max(m.sum(0))) / 1028.0
+    return 0.8 * float(g.array("len")[:, own_e].min()) / (float(np.sqrt(9.80616 * depth)) + 20.0)
 
 
 ELLIPTIC_CASES = [
@@ -619,14 +641,19 @@ def main():
         sys.exit("the reference sources are not mounted here; fixtures are generated in the build container")
     what = sys.argv[1:] or ["grid", "trisk", "elliptic", "schemes", "levels", "mlstep", "mlsplit"]
     if "grid" in what:
-        for J in sorted(PARAMS):
+        for J in (4, 5):
             grid_fixture(J)
+    if "ghosts" in what:
+        for J in (6, 7):
+            grid_fixture(J, geometry=False)
     if "trisk" in what:
         for c in CASES:
             trisk_fixture(*c)
     if "diffusion" in what or "trisk" in what:
         for c in DIFF_CASES:
             trisk_fixture(*c)
+    if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
+        trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:
         for c in ELLIPTIC_CASES:
             elliptic_fixture(*c)

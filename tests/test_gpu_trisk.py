@@ -74,6 +74,29 @@ def test_rk4_matches_reference(case):
         _cmp_state(got, ref, g.layout, TOL_REF, f"step {s} vs reference")
 
 
+def test_c1_100_steps_match_reference_and_oracle():
+    """C1 at its workload: J5 TC5, 100 RK4 steps at CFL 0.8 from the
+    reference's state; within 1e-12 of the reference after steps 25, 50, 75,
+    100 and within 1e-13 of the oracle's 100 steps"""
+    f = golden("trisk_tc5_c1_J5.npz")
+    print("C1: J5 TC5 100 RK4 steps", flush=True)
+    g = grid_of_fixture(f)
+    topo, mass, temp, velo = fixture_state(f, g)
+    ctx = _ctx(g, f, 1)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, mass, temp, velo)
+    every, dt = int(f["every"]), float(f["dt"])
+    for s in range(1, int(f["nsteps"]) + 1):
+        ctx.rk4(0, dt, every)
+        got = ctx.download_state(0)
+        ref = (f[f"mass{s}"], np.nan_to_num(f[f"temp{s}"]), f[f"velo{s}"])
+        _cmp_state(got, ref, g.layout, TOL_REF, f"step {s * every} vs reference")
+    o = oracle_of(g, topo, f)
+    want = o.rk4(mass, temp, velo, dt, every * int(f["nsteps"]))
+    _cmp_state(got, want, g.layout, 1e-13, "100 steps vs oracle")
+    ctx.close()
+
+
 @pytest.mark.parametrize("level,dl,nlayers,steps", [(6, 3, 5, 3), (7, 2, 2, 2)])
 def test_rk4_matches_oracle_synthetic(level, dl, nlayers, steps):
     print(f"J{level} L{nlayers}", flush=True)

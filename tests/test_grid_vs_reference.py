@@ -1,7 +1,9 @@
 """Grid generator (product host code) against the reference's own grid.
 
 Fixtures grid_J4/J5 hold the reference's geometry on every cell the TRiSK
-stencils read and its ghost maps (tests/golden/make_golden.py).
+stencils read and its ghost maps; ghosts_J6/J7 hold the ghost maps of the
+reference's param_J6 (domain level 3) and param_J7 (domain level 2) builds
+(tests/golden/make_golden.py).
 """
 import numpy as np
 import pytest
@@ -24,9 +26,9 @@ def _key(n, i, j):
     return f(i), f(j)
 
 
-@pytest.mark.parametrize("J", [4, 5])
+@pytest.mark.parametrize("J", [4, 5, 6, 7])
 def test_ghost_maps_match_reference(J):
-    f = golden(f"grid_J{J}.npz")
+    f = golden(f"grid_J{J}.npz" if J <= 5 else f"ghosts_J{J}.npz")
     g = grid_of_fixture(f)
     nc = g.layout.ncells
     ref_n, ref_e, ref_s = f["node_src"], f["edge_src"], f["edge_sign"]

@@ -62,13 +62,14 @@ class Problem:
         seeds = list(seed) if isinstance(seed, (list, tuple)) else [seed]
         self.members = M = len(seeds)
         draws = [draw_fields(grids, s) for s in seeds]
-        self.ctxs, self.topos, self.eta_olds = [], [], []
+        self.ctxs, self.topos, self.eta_olds, self.rhs = [], [], [], []
         for k, g in enumerate(grids):
             topo = np.stack([d[k][0] for d in draws]) if M > 1 else draws[0][k][0]
             eta_old = np.stack([d[k][1] for d in draws]) if M > 1 else draws[0][k][1]
             f = np.stack([d[k][2] for d in draws]) if M > 1 else draws[0][k][2]
             self.topos.append(topo)
             self.eta_olds.append(eta_old)
+            self.rhs.append(f)
             c = Context(g, nlayers=1, members=M)
             c.set_topography(topo)
             c.fs_set_params(dt)
