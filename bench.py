@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--ml-steps", type=int, default=3, help="steps timed per multi-level leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched right-hand-side line and print it (for profiling the batched solver)")
+    ap.add_argument("--only-mlp", action="store_true",
+                    help="run only the network's one-launch inference at 64 right-hand sides of J7 (163 840 rows)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -79,6 +81,9 @@ def main():
     if args.only_multilevel:
         cs = ((tuple(int(v) for v in args.ml_case.split(",")),) if args.ml_case else ((5, 7, 20), (5, 8, 30)))
         print(json.dumps({"multilevel": multilevel(cs, nsteps=args.ml_steps)}))
+        return
+    if args.only_mlp:
+        print(json.dumps({"mlp_forward_64_rhs": mlp_forward(64 * 10 * (2 ** args.fs_level // 8) ** 2)}))
         return
     if args.only_batched:
         from wavetrisk_hydrostatic_amd import learned as LC
@@ -616,6 +621,28 @@ def multilevel(cases_=((5, 7, 20), (5, 8, 30)), dt=300.0, nsteps=3):
     return out
 
 
+def mlp_forward(rows: int, params=None):
+    """The network's one-launch inference (mlp_infer_kernel) on `rows` rows of
+    seeded features: mean ms of 20 launches (HIP events), BF16 FLOP rate and
+    its fraction of MI355X's dense BF16 peak"""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+
+    big = W.MLP(max_batch=rows, seed=1)
+    if params is not None:
+        big.params = params
+    big.forward(np.random.default_rng(0).standard_normal((rows, W.MLP_IN)).astype(np.float32))
+    ms = big.forward_timed(rows, 20)
+    flops = 2.0 * (W.MLP_IN * W.MLP_HIDDEN + W.MLP_HIDDEN * W.MLP_HIDDEN + W.MLP_HIDDEN * W.MLP_OUT) * rows
+    big.close()
+    # bytes: BF16 inputs read, FP32 outputs written
+    hbm = rows * (2 * W.MLP_IN + 4 * W.MLP_OUT)
+    return {"rows": rows, "ms_per_forward": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+            "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15,
+            "hbm_gbs": hbm / (ms * 1e-3) / 1e9, "hbm_bound_ms": hbm / 8e12 * 1e3}
+
+
 def free_surface_cpu(prob, dt: float = 1800.0):
     """CPU baseline of the solve: the oracle's FMG (the reference's method:
     FMG with BiCGSTAB on the coarsest level, tolerances 1e-9) on the same
@@ -675,15 +702,7 @@ def free_surface(fine: int, cpu: bool = True):
         return {"iterations": [int(v) for v in it], "rel_resid": float(res[-1]), "ms": ms}
 
     # the network at config 4's batch: one row per patch for 64 right-hand sides
-    rows = 64 * npatch
-    big = W.MLP(max_batch=rows, seed=1)
-    big.params = mlp.params
-    big.forward(np.random.default_rng(0).standard_normal((rows, W.MLP_IN)).astype(np.float32))
-    ms = big.forward_timed(rows, 20)
-    flops = 2.0 * (192 * 128 + 128 * 128 + 128 * 64) * rows
-    big.close()
-    mlp_line = {"rows": rows, "ms_per_forward": ms, "tflops": flops / (ms * 1e-3) / 1e12,
-                "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15}
+    mlp_line = mlp_forward(64 * npatch, mlp.params)
 
     pv = LC.vcycle_params(1e-9, 40)
     pv1 = LC.vcycle_params(1e-9, 40)

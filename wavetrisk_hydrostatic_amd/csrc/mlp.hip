@@ -3,10 +3,11 @@
 // and activations and FP32 accumulation on the matrix cores, its backward
 // pass and an Adam optimiser, all on the device.
 //
-// Every product is one tiled kernel, C[M x N] = op(A)[M x K] op(B)[K x N]:
-// BF16 operands staged through LDS with k contiguous, 64 x 64 output tiles
-// per 256-thread block, each wave a 32 x 32 quarter of 2 x 2 MFMA tiles of
-// v_mfma_f32_16x16x16_bf16 (FP32 accumulate).  op() transposes an operand by
+// Every product of the training passes is one tiled kernel, C[M x N] =
+// op(A)[M x K] op(B)[K x N]: BF16 operands staged through LDS with k
+// contiguous, 64 x 64 output tiles per 256-thread block, each wave a 32 x 32
+// quarter of 2 x 2 MFMA tiles of gfx950's v_mfma_f32_16x16x32_bf16 (FP32
+// accumulate, one instruction per 32 k).  op() transposes an operand by
 // the way it is loaded, so the forward pass (X W), the weight gradients
 // (X^T dZ, a reduction over the batch) and the input gradients (dZ W^T) share
 // the kernel.  The epilogue adds the bias and applies GELU, writing the FP32
@@ -30,26 +31,64 @@
 
 namespace {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// 16 x 16 x 32 product on the matrix cores.  Lane l holds row l % 16 of A
+// and column l % 16 of B with the 8 k values 8 (l / 16) .. +7 (any k order
+// shared by A and B gives the same dot products), and rows 4 (l / 16) .. +3
+// of column l % 16 of the FP32 accumulator.
+__device__ __forceinline__ float4_t mfma32(const uint16_t* a, const uint16_t* b, float4_t c) {
+  const bf16x8_t av = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(a));
+  const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(b));
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
+__device__ __forceinline__ float4_t mfma32b(const uint16_t* a, uint4 b, float4_t c) {
+  const bf16x8_t av = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(a));
+  const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, b);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
 
 constexpr int GT = 256;            // threads per GEMM block
 constexpr int BM = 64, BN = 64;    // output tile
 constexpr int BK = 32;             // k per LDS stage
 constexpr int LP = BK + 8;         // LDS row pitch (bf16), breaks bank conflicts
 
-// bf16 <-> fp32: round to nearest even (NaN kept quiet)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// bf16 <-> fp32: round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 f2bf4(float a, float b, float c, float d) {
+  const bf16x4_t v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+  return __builtin_bit_cast(uint2, v);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz and Stegun, Handbook of Mathematical Functions, 7.1.28:
+// erf(x) = 1 - (1 + a1 x + ... + a6 x^6)^-16 for x >= 0, |error| <= 3e-7 in
+// exact arithmetic; evaluated in FP32 (the powering amplifies the rounding)
+// and its reciprocal by v_rcp_f32 it stays within 2e-6 of erf on [-6, 6], far
+// below the 4e-3 relative step of
+// the BF16 activations it feeds; odd in x.  Branch-free: the
+// GELU of every activation of the forward and backward passes evaluates it,
+// and a branchy erf made the inference epilogue the kernel's bottleneck.
+__device__ __forceinline__ float erf_as(float x) {
+  const float a = fabsf(x);
+  float p = 0.0000430638f;
+  p = p * a + 0.0002765672f;
+  p = p * a + 0.0001520143f;
+  p = p * a + 0.0092705272f;
+  p = p * a + 0.0422820123f;
+  p = p * a + 0.0705230784f;
+  p = p * a + 1.0f;
+  p = p * p;  // ^2
+  p = p * p;  // ^4
+  p = p * p;  // ^8
+  p = p * p;  // ^16 (inf for large a: the quotient below is then 0)
+  return copysignf(1.0f - __builtin_amdgcn_rcpf(p), x);  // v_rcp_f32 (1 ulp): rcp(inf) = 0
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f));
   const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
   return cdf + z * pdf;
 }
@@ -126,16 +165,12 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
       }
     }
     __syncthreads();
-    // --- 2 x 2 tiles of 16 x 16 x 16, two k steps --------------------------------
-    for (int ks = 0; ks < BK; ks += 16) {
-      const int kq = ks + 4 * (lane / 16);
-      short4_t af[2], bfr[2];
-      for (int i = 0; i < 2; ++i) {  // one 8-byte LDS read per fragment
-        af[i] = *reinterpret_cast<const short4_t*>(&As[wm + i * 16 + lane % 16][kq]);
-        bfr[i] = *reinterpret_cast<const short4_t*>(&Bs[wn + i * 16 + lane % 16][kq]);
-      }
+    // --- 2 x 2 tiles of 16 x 16 x 32: one 16-byte LDS read per fragment -----------
+    {
+      const int kq = 8 * (lane / 16);
       for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = mfma32(&As[wm + i * 16 + lane % 16][kq], &Bs[wn + j * 16 + lane % 16][kq], acc[i][j]);
     }
     __syncthreads();
   }
@@ -156,91 +191,111 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(GemmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Inference in one launch: each 256-thread block takes 64 rows through the
-// three layers, the hidden activations staying in LDS.  The MFMA tiles, the
-// fragment layout, the k order and the epilogue arithmetic are those of
-// gemm_bf16_kernel, so the output equals the three-launch forward pass bit
-// for bit; nothing is kept for a backward pass.
+// Inference in one launch: a persistent block per CU holds the three weight
+// matrices transposed in LDS (n rows of k contiguous BF16, 101 KB with
+// padding) and the biases for the whole launch, and walks tiles of 128 rows;
+// each of its 8 waves takes 16 rows through the three layers, its hidden
+// activations in its own LDS rows, and loads the next tile's input rows while
+// it finishes the current one.  Each product is computed transposed, Z^T =
+// W^T X^T: the weights are the matrix cores' A operand and the 16 rows the
+// columns of B, so a lane's accumulator holds 4 consecutive outputs of one row
+// -- stored as one 8-byte BF16 activation or one 16-byte FP32 output.  Every
+// output is the sum over k of the products of gemm_bf16_kernel's forward pass
+// in the same k chunks and order, with the same epilogue arithmetic.
 // ---------------------------------------------------------------------------
-constexpr int HP = WTG_MLP_HIDDEN + 8;  // LDS pitch of a hidden activation row (bf16)
+constexpr int IT = 512;               // threads per inference block (8 waves, 2 per SIMD)
+constexpr int IW = 16;                // rows per wave
+constexpr int IR = IT / 64 * IW;      // rows per block tile
+constexpr int P1 = WTG_MLP_IN + 8;    // LDS pitches (bf16) of W1^T, W2^T / W3^T and the activations
+constexpr int P2 = WTG_MLP_HIDDEN + 8;
 
-// One layer for the block's 64 rows: Z = A[64 x K] W[K x N] + bias, then
-// either H = bf16(GELU(Z)) into an LDS activation block (TO_LDS) or y = Z for
-// the valid rows.  A comes from global bf16 rows (FROM_GLOBAL: Ag, staged per
-// k chunk, rows past `rows` read as zero) or from an LDS activation block
-// (Al, pitch HP); W (global, [K][N]) is staged per k chunk as [n][k]; wave w
-// owns rows 32 (w / 2) .. +32 and columns (N / 2)(w % 2) .. +N/2.
-template <int N, bool FROM_GLOBAL, bool TO_LDS>
-__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ Ag, int rows, const uint16_t* Al, int K,
-                                            const uint16_t* __restrict__ W, const float* __restrict__ bias,
-                                            uint16_t (*As)[LP], uint16_t (*Bs)[LP], uint16_t* H, float* y) {
-  constexpr int NJ = N / 32;
-  const int t = threadIdx.x, lane = t % 64, wave = t / 64;
-  const int wm = (wave / 2) * 32, wn = (wave % 2) * (N / 2);
-  float4_t acc[2 * NJ];
-  for (int q = 0; q < 2 * NJ; ++q) acc[q] = float4_t{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    if (FROM_GLOBAL) {  // 64 x 32 chunk of the input rows
-      const int m = t / 4, kk = (t % 4) * 8;
-      uint4 q = make_uint4(0u, 0u, 0u, 0u);
-      if (m < rows) q = *reinterpret_cast<const uint4*>(Ag + (int64_t)m * K + k0 + kk);
-      *reinterpret_cast<uint4*>(&As[m][kk]) = q;
-    }
-    for (int p = 0; p < N / 64; ++p) {  // 32 x N chunk of W, stored [n][k]
-      const int kk = t / 8, n = p * 64 + (t % 8) * 8;
-      const uint4 q = *reinterpret_cast<const uint4*>(W + (int64_t)(k0 + kk) * N + n);
-      const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
-      for (int i = 0; i < 4; ++i) {
-        Bs[n + 2 * i][kk] = (uint16_t)(w4[i] & 0xffffu);
-        Bs[n + 2 * i + 1][kk] = (uint16_t)(w4[i] >> 16);
-      }
-    }
-    __syncthreads();
-    for (int ks = 0; ks < BK; ks += 16) {
-      const int kq = ks + 4 * (lane / 16);
-      short4_t af[2], bfr[NJ];
-      for (int i = 0; i < 2; ++i)
-        af[i] = FROM_GLOBAL ? *reinterpret_cast<const short4_t*>(&As[wm + i * 16 + lane % 16][kq])
-                            : *reinterpret_cast<const short4_t*>(Al + (wm + i * 16 + lane % 16) * HP + k0 + kq);
-      for (int j = 0; j < NJ; ++j) bfr[j] = *reinterpret_cast<const short4_t*>(&Bs[wn + j * 16 + lane % 16][kq]);
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < NJ; ++j)
-          acc[i * NJ + j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i * NJ + j], 0, 0, 0);
-    }
-    __syncthreads();
+struct InferSmem {
+  uint16_t w1[WTG_MLP_HIDDEN][P1];
+  uint16_t w2[WTG_MLP_HIDDEN][P2];
+  uint16_t w3[WTG_MLP_OUT][P2];
+  uint16_t h[IT / 64][IW][P2];
+  float b[2 * WTG_MLP_HIDDEN + WTG_MLP_OUT];
+};
+
+// rows (n) of a transposed weight matrix [N][K] into LDS with pitch P, 16 bytes per copy
+template <int N, int K, int P>
+__device__ __forceinline__ void stage_wt(const uint16_t* __restrict__ wt, uint16_t (*dst)[P]) {
+  for (int i = threadIdx.x; i < N * (K / 8); i += IT) {
+    const int n = i / (K / 8), c = (i % (K / 8)) * 8;
+    *reinterpret_cast<uint4*>(&dst[n][c]) = *reinterpret_cast<const uint4*>(wt + (int64_t)n * K + c);
   }
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < NJ; ++j) {
-      const int n = wn + j * 16 + lane % 16;
-      const float b = bias[n];
-      for (int r = 0; r < 4; ++r) {
-        const int m = wm + i * 16 + 4 * (lane / 16) + r;
-        const float z = acc[i * NJ + j][r] + b;
-        if (TO_LDS)
-          H[m * HP + n] = f2bf(gelu(z));
-        else if (m < rows)
-          y[(int64_t)m * N + n] = z;
-      }
-    }
 }
 
-__global__ __launch_bounds__(GT) void mlp_infer_kernel(int B, const uint16_t* __restrict__ x,
-                                                       const uint16_t* __restrict__ wb, const float* __restrict__ p,
+__global__ __launch_bounds__(IT) void mlp_infer_kernel(int B, const uint16_t* __restrict__ x,
+                                                       const uint16_t* __restrict__ wt, const float* __restrict__ p,
                                                        float* __restrict__ y) {
-  __shared__ __align__(16) uint16_t As[BM][LP];
-  __shared__ __align__(16) uint16_t Bs[WTG_MLP_HIDDEN][LP];
-  __shared__ __align__(16) uint16_t H1[BM * HP];
-  __shared__ __align__(16) uint16_t H2[BM * HP];
   constexpr int NI_ = WTG_MLP_IN, NH_ = WTG_MLP_HIDDEN, NO_ = WTG_MLP_OUT;
   constexpr int OB1_ = NI_ * NH_, OW2_ = OB1_ + NH_, OB2_ = OW2_ + NH_ * NH_, OW3_ = OB2_ + NH_,
                 OB3_ = OW3_ + NH_ * NO_;
-  const int m0 = blockIdx.x * BM, rows = min(BM, B - m0);
-  infer_layer<NH_, true, true>(x + (int64_t)m0 * NI_, rows, H1, NI_, wb, p + OB1_, As, Bs, H1, y);
+  constexpr int K1 = NI_ / 32, K2 = NH_ / 32, J1 = NH_ / 16, J3 = NO_ / 16;
+  __shared__ __align__(16) InferSmem s;
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int r16 = lane % 16, q8 = 8 * (lane / 16), q4 = 4 * (lane / 16);
+  stage_wt<NH_, NI_, P1>(wt, s.w1);
+  stage_wt<NH_, NH_, P2>(wt + NI_ * NH_, s.w2);
+  stage_wt<NO_, NH_, P2>(wt + NI_ * NH_ + NH_ * NH_, s.w3);
+  for (int i = threadIdx.x; i < NH_; i += IT) s.b[i] = p[OB1_ + i], s.b[NH_ + i] = p[OB2_ + i];
+  for (int i = threadIdx.x; i < NO_; i += IT) s.b[2 * NH_ + i] = p[OB3_ + i];
   __syncthreads();
-  infer_layer<NH_, false, true>(x, rows, H1, NH_, wb + NI_ * NH_, p + OB2_, As, Bs, H2, y);
-  __syncthreads();
-  infer_layer<NO_, false, false>(x, rows, H2, NH_, wb + NI_ * NH_ + NH_ * NH_, p + OB3_, As, Bs, H2,
-                                 y + (int64_t)m0 * NO_);
+  uint16_t(*h)[P2] = s.h[wave];
+  // this lane's input row fragments of a tile (rows past B read as zero)
+  auto load_x = [&](int m0, uint4 v[K1]) {
+    const int row = m0 + wave * IW + r16;
+    for (int ks = 0; ks < K1; ++ks)
+      v[ks] = row < B ? *reinterpret_cast<const uint4*>(x + (int64_t)row * NI_ + 32 * ks + q8)
+                      : make_uint4(0u, 0u, 0u, 0u);
+  };
+  // bias + GELU of 4 consecutive hidden units n0 .. n0 + 3 of row r16, as BF16 into h
+  auto act = [&](const float4_t& z, const float* bias, int n0) {
+    const float4 bv = *reinterpret_cast<const float4*>(bias + n0);
+    *reinterpret_cast<uint2*>(&h[r16][n0]) =
+        f2bf4(gelu(z[0] + bv.x), gelu(z[1] + bv.y), gelu(z[2] + bv.z), gelu(z[3] + bv.w));
+  };
+  uint4 xa[K1];
+  load_x(blockIdx.x * IR, xa);
+  for (int m0 = blockIdx.x * IR; m0 < B; m0 += gridDim.x * IR) {
+    // layer 1: Z1 = X W1 + b1, H1 = bf16(GELU(Z1))
+    float4_t acc[J1];
+    for (int j = 0; j < J1; ++j) acc[j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < K1; ++ks)
+      for (int j = 0; j < J1; ++j) acc[j] = mfma32b(&s.w1[16 * j + r16][32 * ks + q8], xa[ks], acc[j]);
+    load_x(m0 + gridDim.x * IR, xa);  // the next tile's rows, in flight through layers 2 and 3
+    for (int j = 0; j < J1; ++j) act(acc[j], s.b, 16 * j + q4);
+    // layer 2: the wave's activations in place (its reads of H1 precede its writes of H2)
+    for (int j = 0; j < J1; ++j) acc[j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < K2; ++ks) {
+      const uint4 hb = *reinterpret_cast<const uint4*>(&h[r16][32 * ks + q8]);
+      for (int j = 0; j < J1; ++j) acc[j] = mfma32b(&s.w2[16 * j + r16][32 * ks + q8], hb, acc[j]);
+    }
+    for (int j = 0; j < J1; ++j) act(acc[j], s.b + NH_, 16 * j + q4);
+    // layer 3: y = H2 W3 + b3 for the valid rows, 4 consecutive outputs per lane
+    float4_t ac3[J3];
+    for (int j = 0; j < J3; ++j) ac3[j] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < K2; ++ks) {
+      const uint4 hb = *reinterpret_cast<const uint4*>(&h[r16][32 * ks + q8]);
+      for (int j = 0; j < J3; ++j) ac3[j] = mfma32b(&s.w3[16 * j + r16][32 * ks + q8], hb, ac3[j]);
+    }
+    const int row = m0 + wave * IW + r16;
+    if (row < B)
+      for (int j = 0; j < J3; ++j) {
+        const float4 bv = *reinterpret_cast<const float4*>(s.b + 2 * NH_ + 16 * j + q4);
+        *reinterpret_cast<float4*>(y + (int64_t)row * NO_ + 16 * j + q4) =
+            make_float4(ac3[j][0] + bv.x, ac3[j][1] + bv.y, ac3[j][2] + bv.z, ac3[j][3] + bv.w);
+      }
+  }
+}
+
+// bf16 W^T of a [K][N] FP32 weight matrix: wt[n][k] = bf16(w[k][n])
+__global__ void to_bf16_t_kernel(int K, int N, const float* __restrict__ w, uint16_t* __restrict__ wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * N) return;
+  const int n = i / K, k = i % K;
+  wt[i] = f2bf(w[(int64_t)k * N + n]);
 }
 
 __global__ void to_bf16_kernel(int64_t n, const float* __restrict__ x, uint16_t* __restrict__ y) {
@@ -306,7 +361,8 @@ __global__ void colsum_kernel(int B, int w, const uint16_t* __restrict__ dzb, fl
 // Adam (Kingma & Ba 2015) on the FP32 master parameters, BF16 copies of the weights refreshed
 __global__ void adam_kernel(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
                             float* __restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
-                            const int* __restrict__ wmap, uint16_t* __restrict__ pb) {
+                            const int* __restrict__ wmap, uint16_t* __restrict__ pb, const int* __restrict__ wmapt,
+                            uint16_t* __restrict__ pbt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -315,7 +371,10 @@ __global__ void adam_kernel(int n, float* __restrict__ p, const float* __restric
   m2[i] = b;
   const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
   p[i] = v;
-  if (wmap[i] >= 0) pb[wmap[i]] = f2bf(v);
+  if (wmap[i] >= 0) {
+    pb[wmap[i]] = f2bf(v);
+    pbt[wmapt[i]] = f2bf(v);
+  }
 }
 
 inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -335,7 +394,10 @@ struct wtg_mlp {
   float* g = nullptr;
   float *m1 = nullptr, *m2 = nullptr;
   uint16_t* wb = nullptr;  // bf16 W1 W2 W3
+  uint16_t* wbt = nullptr; // bf16 W1^T W2^T W3^T (inference)
+  int infer_grid = 0;      // persistent inference blocks (one per CU)
   int* wmap = nullptr;     // parameter -> index in wb (-1 for biases)
+  int* wmapt = nullptr;    // parameter -> index in wbt
   int adam_t = 0;
   // activations
   uint16_t *x0 = nullptr, *h1 = nullptr, *h2 = nullptr, *dyb = nullptr, *dz2 = nullptr, *dz1 = nullptr;
@@ -445,6 +507,12 @@ int refresh_bf16(wtg_mlp* m) {
                      m->wb + BW2);
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks(NH * NO)), dim3(256), 0, m->stream, (int64_t)NH * NO, m->p + OW3,
                      m->wb + BW3);
+  hipLaunchKernelGGL(to_bf16_t_kernel, dim3(blocks(NI * NH)), dim3(256), 0, m->stream, NI, NH, m->p + OW1,
+                     m->wbt + BW1);
+  hipLaunchKernelGGL(to_bf16_t_kernel, dim3(blocks(NH * NH)), dim3(256), 0, m->stream, NH, NH, m->p + OW2,
+                     m->wbt + BW2);
+  hipLaunchKernelGGL(to_bf16_t_kernel, dim3(blocks(NH * NO)), dim3(256), 0, m->stream, NH, NO, m->p + OW3,
+                     m->wbt + BW3);
   MCHK(m, hipGetLastError());
   return 0;
 }
@@ -478,8 +546,8 @@ void wtg_mlp_buffers(wtg_mlp* m, uint16_t** x0, float** y, int* max_batch) {
 int wtg_mlp_infer_on(wtg_mlp* m, hipStream_t stream, int B) {
   if (B < 0 || B > m->max_batch) return mfail(m, WTG_E_ARG, "batch %d exceeds the maximum %d", B, m->max_batch);
   if (B == 0) return 0;
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((B + BM - 1) / BM)), dim3(GT), 0, stream, B, m->x0, m->wb, m->p,
-                     m->y);
+  const unsigned grid = (unsigned)std::min<int64_t>((B + IR - 1) / IR, m->infer_grid);
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(IT), 0, stream, B, m->x0, m->wbt, m->p, m->y);
   MCHK(m, hipGetLastError());
   return 0;
 }
@@ -497,6 +565,11 @@ int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out) {
   if (hipGetDevice(&m->device) != hipSuccess) return mfail(nullptr, WTG_E_DEVICE, "no HIP device");
   if (hipStreamCreate(&m->stream) != hipSuccess) return mfail(nullptr, WTG_E_DEVICE, "hipStreamCreate failed");
   m->max_batch = (max_batch + 63) / 64 * 64;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, m->device) != hipSuccess) return mfail(nullptr, WTG_E_DEVICE, "no device properties");
+    m->infer_grid = std::max(1, prop.multiProcessorCount);
+  }
   const int Bp = m->max_batch;
   auto mk = [&](auto** p, size_t n) { return malloc_dev(m.get(), p, n); };
   int rc = 0;
@@ -505,7 +578,9 @@ int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out) {
   rc = rc ? rc : mk(&m->m1, NPAR);
   rc = rc ? rc : mk(&m->m2, NPAR);
   rc = rc ? rc : mk(&m->wb, NWB);
+  rc = rc ? rc : mk(&m->wbt, NWB);
   rc = rc ? rc : mk(&m->wmap, NPAR);
+  rc = rc ? rc : mk(&m->wmapt, NPAR);
   rc = rc ? rc : mk(&m->x0, (size_t)Bp * NI);
   rc = rc ? rc : mk(&m->x0f, (size_t)Bp * NI);
   rc = rc ? rc : mk(&m->h1, (size_t)Bp * NH);
@@ -537,13 +612,14 @@ int wtg_mlp_create(int max_batch, uint64_t seed, wtg_mlp** out) {
   fill(OW1, NI, NH, 1.0);
   fill(OW2, NH, NH, 1.0);
   fill(OW3, NH, NO, 0.1);
-  std::vector<int> wmap(NPAR, -1);
-  for (int i = 0; i < NI * NH; ++i) wmap[OW1 + i] = BW1 + i;
-  for (int i = 0; i < NH * NH; ++i) wmap[OW2 + i] = BW2 + i;
-  for (int i = 0; i < NH * NO; ++i) wmap[OW3 + i] = BW3 + i;
+  std::vector<int> wmap(NPAR, -1), wmapt(NPAR, -1);
+  for (int i = 0; i < NI * NH; ++i) wmap[OW1 + i] = BW1 + i, wmapt[OW1 + i] = BW1 + (i % NH) * NI + i / NH;
+  for (int i = 0; i < NH * NH; ++i) wmap[OW2 + i] = BW2 + i, wmapt[OW2 + i] = BW2 + (i % NH) * NH + i / NH;
+  for (int i = 0; i < NH * NO; ++i) wmap[OW3 + i] = BW3 + i, wmapt[OW3 + i] = BW3 + (i % NO) * NH + i / NO;
   wtg_mlp* mm = m.get();
   if (hipMemcpy(mm->p, p.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(mm->wmap, wmap.data(), NPAR * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(mm->wmapt, wmapt.data(), NPAR * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(mm->m1, 0, NPAR * sizeof(float)) != hipSuccess ||
       hipMemset(mm->m2, 0, NPAR * sizeof(float)) != hipSuccess) {
     wtg_mlp_destroy(m.release());
@@ -698,7 +774,7 @@ int wtg_mlp_train(wtg_mlp* m, int nsteps, int batch, float lr, uint64_t seed, fl
     ++m->adam_t;
     const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
     hipLaunchKernelGGL(adam_kernel, dim3(blocks(NPAR)), dim3(256), 0, m->stream, NPAR, m->p, m->g, m->m1, m->m2, lr,
-                       b1, b2, eps, c1, c2, m->wmap, m->wb);
+                       b1, b2, eps, c1, c2, m->wmap, m->wb, m->wmapt, m->wbt);
     MCHK(m, hipGetLastError());
   }
   MCHK(m, hipStreamSynchronize(m->stream));
