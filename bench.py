@@ -68,6 +68,10 @@ def main():
     ap.add_argument("--ml-steps", type=int, default=3, help="steps timed per multi-level leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched right-hand-side line and print it (for profiling the batched solver)")
+    ap.add_argument("--only-ensemble-drift", action="store_true",
+                    help="run only config 5's drift line: --members J8 L30 members, --drift-steps split steps, FP64 "
+                         "V-cycles against FP32-bulk stencils with learned cycles on the member-batched hierarchy")
+    ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--only-mlp", action="store_true",
                     help="run only the network's one-launch inference at 64 right-hand sides of J7 (163 840 rows)")
     args = ap.parse_args()
@@ -81,6 +85,14 @@ def main():
     if args.only_multilevel:
         cs = ((tuple(int(v) for v in args.ml_case.split(",")),) if args.ml_case else ((5, 7, 20), (5, 8, 30)))
         print(json.dumps({"multilevel": multilevel(cs, nsteps=args.ml_steps)}))
+        return
+    if args.only_ensemble_drift:
+        import wavetrisk_hydrostatic_amd as W
+        from wavetrisk_hydrostatic_amd import cases
+        grid = W.Grid(level=args.level, min_level=args.level, domain_level=2)
+        topo, mass, temp, velo = cases.ocean(grid, args.layers, seed=2026)
+        print(json.dumps({"ensemble_drift": ensemble_drift(grid, topo, mass, temp, velo, args.layers, args.members,
+                                                           args.drift_steps)}))
         return
     if args.only_mlp:
         print(json.dumps({"mlp_forward_64_rhs": mlp_forward(64 * 10 * (2 ** args.fs_level // 8) ** 2)}))
@@ -406,6 +418,88 @@ def ensemble(grid, topo, mass, temp, velo, L, dt, members, nsteps=10, split_step
                            "note": "host wall clock; the members' BiCGSTAB solves batched (one launch per step of "
                                    "the iteration for all members)"},
             "note": "one stage launch covers all members (grid y = member); same initial layers, per-member bathymetry"}
+
+
+def ensemble_drift(grid, topo, mass, temp, velo, L, members, nsteps, dt=900.0):
+    """Config 5 at its length: `members` copies of the J8 L30 ocean with their
+    own bathymetry (as in the ensemble line), `nsteps` split steps with the
+    members' free surfaces on the member-batched J(J-3)..J hierarchy to 1e-9:
+    FP64 stencils with FP64 V-cycles, then FP32-bulk stencils with learned
+    cycles (the shipped network; member-batched cycles smooth in FP64); wall
+    ms per step of the whole ensemble, the largest member cycle count per
+    step, and each member's drift of the FP32-bulk run from its FP64 run at
+    the end (free surface relative to the column depth, mass, velocity)."""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import learned as LC
+
+    M = members
+    print(f"bench: ensemble drift, {M} members, {nsteps} split steps", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(7)
+    topos = np.stack([topo + 50.0 * np.tanh(rng.standard_normal()) * np.cos(np.arange(topo.size) * 1e-4 * (m + 1))
+                      for m in range(M)])
+    J = grid.layout.level
+    grids = [W.Grid(level=l, min_level=J - 3, domain_level=2) for l in range(J - 3, J + 1)]
+    tps = [topos]
+    for k in range(len(grids) - 2, -1, -1):
+        tps.insert(0, _subsample(grids[k + 1], grids[k], tps[0]))
+    p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-9, coarse_iter=400, max_vcycle=40)
+    runs, final = {}, {}
+    for name, prec, learned in (("fp64_vcycle", W.PRECISION_FP64, False),
+                                ("fp32_bulk_learned", W.PRECISION_FP32_BULK, True)):
+        ctxs = []
+        for k, g in enumerate(grids):
+            c = W.Context(g, nlayers=L if k == len(grids) - 1 else 1, members=M, nslots=1)
+            c.set_topography(tps[k])
+            c.fs_set_params(dt, 0.6, 0.8)
+            ctxs.append(c)
+        top = ctxs[-1]
+        top.set_precision(prec)
+        h = W.Hierarchy(ctxs)
+        mlp = LC.load_network(h.npatches * M) if learned else None
+        if learned and mlp is None:
+            raise RuntimeError("no shipped network for the learned cycles")
+        h.set_learned(mlp)
+        top.upload_state(0, np.tile(mass, (M, 1)), np.tile(temp, (M, 1)), np.tile(velo, (M, 1, 1)))
+        top.fs_upload(W.FS_ETA, np.tile(mass.sum(0) / 1028.0, (M, 1)) + topos)
+        top.synchronize()
+        meth = W.SOLVE_LEARNED if learned else W.SOLVE_VCYCLE
+        t0 = time.perf_counter()
+        cyc = [int(h.rk4_split(0, 1, meth, p)) for _ in range(nsteps)]
+        top.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        runs[name] = {"ms_per_step": ms, "dof_updates_per_s": M * L * grid.layout.n_dof_per_layer / (ms / 1e3),
+                      "max_member_cycles_min": min(cyc), "max_member_cycles_max": max(cyc),
+                      "cycles_at_cap": sum(c >= p.max_vcycle for c in cyc)}
+        m, _, u = top.download_state(0)
+        final[name] = (m, u, top.fs_download(W.FS_ETA))
+        h.set_learned(None)
+        h.close()
+        if mlp is not None:
+            mlp.close()
+        for c in ctxs:
+            c.close()
+    lay = grid.layout
+    own, own_e = lay.owned_cells(), lay.owned_edge_cells()
+    m64, u64, e64 = final["fp64_vcycle"]
+    m32, u32, e32 = final["fp32_bulk_learned"]
+    drift = []
+    for k in range(M):
+        sl = slice(k * L, (k + 1) * L)
+        depth = np.max(m64[sl][:, own].sum(0) / 1028.0)
+        drift.append({"surface_height_rel_depth": float(np.max(np.abs(e32[k][own] - e64[k][own])) / depth),
+                      "mass_rel": float(np.max(np.abs(m32[sl][:, own] - m64[sl][:, own])) /
+                                        np.max(np.abs(m64[sl][:, own]))),
+                      "velo_rel": float(np.max(np.abs(u32[sl][:, :, own_e] - u64[sl][:, :, own_e])) /
+                                        np.max(np.abs(u64[sl][:, :, own_e])))})
+    return {"members": M, "steps": nsteps, "dt": dt, "levels": f"J{J - 3}..J{J}", "layers": L, **runs,
+            "speedup_fp32_learned_vs_fp64": runs["fp64_vcycle"]["ms_per_step"] / runs["fp32_bulk_learned"]["ms_per_step"],
+            "drift_vs_fp64": {"max_surface_height_rel_depth": max(d["surface_height_rel_depth"] for d in drift),
+                              "max_mass_rel": max(d["mass_rel"] for d in drift),
+                              "max_velo_rel": max(d["velo_rel"] for d in drift), "per_member": drift},
+            "note": "host wall clock over all steps (no warm-up step: both runs start from the same state); "
+                    "free surface to 1e-9 relative residual per member every step"}
 
 
 def _subsample(fine, coarse, f):

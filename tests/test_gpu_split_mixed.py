@@ -128,3 +128,49 @@ def test_fp32_bulk_split_J8_L30_against_fp64():
     err = _errors(g, res[W.PRECISION_FP32_BULK], res[W.PRECISION_FP64], topo)
     print("  " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
     assert err["eta/depth"] <= 1e-6 and err["mass"] <= 1e-6 and err["velo"] <= 1e-5
+
+
+def test_config2_j7_l20_fp32_bulk_100_steps_against_fp64():
+    """Config 2 at its workload: the J7 ocean with 20 layers, split steps with
+    the free surface solved by V-cycles on the J5..J7 hierarchy to 1e-9
+    relative residual every step (dt 300 s, theta1 0.6, theta2 0.8), 100
+    steps; FP64 (stencils, smoothing) against FP32-bulk stencils with FP32
+    smoothing and FP64 residual refinement.  The FP64 path is the oracle's
+    step to 1e-11 on this hierarchy (tests/test_gpu_ml.py), so its 100 steps
+    stand for the oracle's here (a C run of 100 J7 L20 steps takes minutes).
+    Every step converges (fewer cycles than the cap); after 100 steps the
+    free surface is within 1e-6 of the column depth, mass within 1e-6 and
+    velocity within 1e-5 of the FP64 run."""
+    print("config 2: J5..J7 L20, 100 split steps, V-cycles to 1e-9, FP32-bulk vs FP64", flush=True)
+    grids = [W.Grid(level=l, min_level=5, domain_level=2) for l in (5, 6, 7)]
+    L, dt, nsteps, cap = 20, 300.0, 100, 40
+    draws = [_start(g, L if k == 2 else 1, 17) for k, g in enumerate(grids)]
+    p = W.MgParams.defaults(coarse_tol=1e-2, fine_tol=1e-9, coarse_iter=400, max_vcycle=cap)
+    res, cycles = {}, {}
+    for prec in (W.PRECISION_FP64, W.PRECISION_FP32_BULK):
+        ctxs = []
+        for k, g in enumerate(grids):
+            c = W.Context(g, nlayers=L if k == 2 else 1)
+            c.set_topography(draws[k][0])
+            c.fs_set_params(dt, TH1, TH2)
+            ctxs.append(c)
+        top = ctxs[-1]
+        topo, mass, temp, velo, eta = draws[2]
+        top.upload_state(0, mass, temp, velo)
+        top.fs_upload(W.FS_ETA, eta)
+        top.set_precision(prec)
+        h = W.Hierarchy(ctxs)
+        h.set_precision(prec)
+        try:
+            cycles[prec] = [h.rk4_split(0, 1, W.SOLVE_VCYCLE, p) for _ in range(nsteps)]
+        finally:
+            h.close()
+        res[prec] = (*top.download_state(0), top.fs_download(W.FS_ETA))
+        for c in ctxs:
+            c.close()
+    for prec, cyc in cycles.items():
+        assert all(0 < n < cap for n in cyc), (prec, cyc)
+    err = _errors(grids[-1], res[W.PRECISION_FP32_BULK], res[W.PRECISION_FP64], draws[2][0])
+    print(f"  cycles per step FP64 {min(cycles[0])}-{max(cycles[0])}, FP32-bulk {min(cycles[1])}-{max(cycles[1])}; "
+          + ", ".join(f"{k} {v:.2e}" for k, v in err.items()), flush=True)
+    assert err["eta/depth"] <= 1e-6 and err["mass"] <= 1e-6 and err["velo"] <= 1e-5
