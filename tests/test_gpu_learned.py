@@ -6,7 +6,9 @@ sin(latitude)) to FP32 rounding; with any network the learned cycles reach
 the FP64 tolerance (the minimal-residual step never increases the residual
 and the V-cycle remains the fallback); a network trained on the GPU from
 pairs the FP64 solver generates needs no more cycles than plain V-cycles on
-held-out draws and lowers the residual of each learned step.
+held-out draws and lowers the residual of each learned step; the shipped
+J7 network saves at least one cycle per draw on average on 16 held-out
+draws (a strict gain) and never needs more cycles than plain V-cycles.
 """
 import numpy as np
 import pytest
@@ -112,3 +114,32 @@ def test_trained_network_gains(grids):
         assert i1 <= i0
         gains.append(i0 - i1)
     mlp.close()
+
+
+def test_shipped_network_strict_gain_on_held_out_draws():
+    """The shipped J7 network (trained on draws 1000..1049 by learned.main)
+    on 16 held-out draws of the J5..J7 hierarchy, solved to 1e-9 together
+    (member-batched cycles): no draw needs more learned cycles than plain
+    V-cycles, and the learned cycles save at least one cycle per draw on
+    average (a strict gain; the bench measures 11.25 -> 9.5 at 64 draws)."""
+    print("learned: shipped J7 network on 16 held-out draws", flush=True)
+    grids7 = LC.hierarchy_grids(fine=7, levels=3)
+    seeds = list(range(6000, 6016))
+    prob = LC.Problem(grids7, seeds)
+    mlp = LC.load_network(prob.h.npatches * len(seeds))
+    assert mlp is not None, "the shipped network learned_J7.npz is missing"
+    try:
+        p = LC.vcycle_params(1e-9, 40)
+        prob.solve(W.SOLVE_VCYCLE, p)
+        plain, rp = prob.h.member_results()
+        prob.h.set_learned(mlp)
+        prob.solve(W.SOLVE_LEARNED, p)
+        learned, rl = prob.h.member_results()
+        prob.h.set_learned(None)
+    finally:
+        mlp.close()
+        prob.close()
+    print(f"  plain {list(plain)} -> learned {list(learned)}", flush=True)
+    assert np.all(rp <= 1e-9) and np.all(rl <= 1e-9)
+    assert np.all(learned <= plain)
+    assert plain.mean() - learned.mean() >= 1.0

@@ -96,7 +96,6 @@ __host__ __device__ constexpr int ml_chunks(int nfl) { return (nfl + ML_KC - 1) 
 // the velocity transform's gathers need the threads in flight: one layer
 // per thread (two or six ran slower at J8 L30)
 constexpr int ML_KCV = 1;
-__host__ __device__ constexpr int ml_chunks_v(int L) { return (L + ML_KCV - 1) / ML_KCV; }
 
 __global__ void ml_scalar_wavelet_kernel(int n, const int64_t* __restrict__ ncl, MgGeo g, int L,
                                          const double* __restrict__ sm, const double* __restrict__ st,
@@ -259,11 +258,35 @@ __device__ __forceinline__ bool parent_of(const PairGeo& g, int& lz, int& ip, in
   return true;
 }
 
+// The velocity transform kernels: a 256-thread block is 64 consecutive
+// parents x 4 layers, one wave per layer (blockIdx.y: group of 4 layers), so
+// the state planes are read along i by every wave and the layer-invariant
+// geometry a parent needs (I_u_wgt, lengths, areas: ~420 bytes) is fetched
+// once from L2 for the four layers (the L1 serves the other three waves)
+// instead of once per layer.
+constexpr int VPB = 64;         // parents per block
+constexpr int VLB = NT / VPB;   // layers per block (one wave each)
+inline dim3 velo_grid(int nc, int L) {
+  return dim3((unsigned)(((int64_t)10 * nc * nc + VPB - 1) / VPB), (unsigned)((L + VLB - 1) / VLB));
+}
+__device__ __forceinline__ bool parent_layer_of(const PairGeo& g, int& lz, int& ip, int& jp, int& k, int L) {
+  const int64_t per = (int64_t)10 * g.nc * g.nc;
+  const int64_t t = (int64_t)blockIdx.x * VPB + threadIdx.x % VPB;
+  k = blockIdx.y * VLB + threadIdx.x / VPB;
+  if (t >= per || k >= L) return false;
+  lz = (int)(t / ((int64_t)g.nc * g.nc));
+  const int r = (int)(t % ((int64_t)g.nc * g.nc));
+  jp = r / g.nc;
+  ip = r % g.nc;
+  return true;
+}
+static_assert(ML_KCV == 1, "the velocity transform kernels take one layer per thread");
+
 // Compute_velo_wavelets (wavelet.f90:672-707): 6 outer and 6 inner child edges
 __global__ void ml_velo_wavelet_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                        const double* __restrict__ ufl, double* __restrict__ wcl) {
   int lz, ip, jp, chunk;
-  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
+  if (!parent_layer_of(g, lz, ip, jp, chunk, L)) return;
   const int ic = 2 * ip, jc = 2 * jp;
   const int k1 = min(chunk * ML_KCV + ML_KCV, L);
   for (int k = chunk * ML_KCV; k < k1; ++k) {
@@ -291,7 +314,7 @@ __global__ void ml_velo_wavelet_kernel(PairGeo g, int L, const double* __restric
 __global__ void ml_velo_outer_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                      const double* __restrict__ wcl, double* __restrict__ ufl) {
   int lz, ip, jp, chunk;
-  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
+  if (!parent_layer_of(g, lz, ip, jp, chunk, L)) return;
   const int ic = 2 * ip, jc = 2 * jp;
   const int64_t idp = cellc(g, lz, ip, jp);
   const int k1 = min(chunk * ML_KCV + ML_KCV, L);
@@ -315,7 +338,7 @@ __global__ void ml_velo_outer_kernel(PairGeo g, int L, const double* __restrict_
 __global__ void ml_velo_inner_kernel(PairGeo g, int L, const double* __restrict__ ucl,
                                      const double* __restrict__ wcl, double* __restrict__ ufl) {
   int lz, ip, jp, chunk;
-  if (!parent_of(g, lz, ip, jp, chunk, L)) return;
+  if (!parent_layer_of(g, lz, ip, jp, chunk, L)) return;
   const int k1 = min(chunk * ML_KCV + ML_KCV, L);
   for (int k = chunk * ML_KCV; k < k1; ++k) {
     const double* uc = ucl + (int64_t)k * 3 * g.NCc;
@@ -847,7 +870,7 @@ int wavelets(wtg_ml* m, int l, const wtg_ctx::Slot& fine, const wtg_ctx::Slot& c
                      g, L, fine.m, fine.t, m->wcs[l]);
   MLCHK(m, hipGetLastError());
   const PairGeo pg = pgeo(m, l);
-  hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L)), dim3(NT), 0,
+  hipLaunchKernelGGL(ml_velo_wavelet_kernel, velo_grid(pg.nc, L), dim3(NT), 0,
                      mstream(m), pg, L,
                      coarse.u, fine.u, m->wcu[l]);
   MLCHK(m, hipGetLastError());
@@ -868,7 +891,7 @@ int inverse(wtg_ml* m, int l, const wtg_ctx::Slot& fine, const wtg_ctx::Slot& co
                      L, m->wcs[l], fine.m, fine.t);
   MLCHK(m, hipGetLastError());
   const PairGeo pg = pgeo(m, l);
-  const dim3 pgrid(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L));
+  const dim3 pgrid = velo_grid(pg.nc, L);
   hipLaunchKernelGGL(ml_velo_outer_kernel, pgrid, dim3(NT), 0, mstream(m), pg, L, coarse.u, m->wcu[l], fine.u);
   MLCHK(m, hipGetLastError());
   hipLaunchKernelGGL(ml_velo_penta_kernel, dim3(nblk(10 * L)), dim3(NT), 0, mstream(m), pg, L, -1, coarse.u, fine.u);
@@ -1073,7 +1096,7 @@ int wtg_ml_forward(wtg_ml* m, int slot) {
     if (int rc = wtg_ghost_fill_layers(m->lv[l], L, nullptr, nullptr, s[l]->u)) return rc;
   for (int l = 1; l < m->nl; ++l) {
     const PairGeo pg = pgeo(m, l);
-    hipLaunchKernelGGL(ml_velo_wavelet_kernel, dim3(nblk((int64_t)10 * pg.nc * pg.nc), ml_chunks_v(L)), dim3(NT), 0,
+    hipLaunchKernelGGL(ml_velo_wavelet_kernel, velo_grid(pg.nc, L), dim3(NT), 0,
                        mstream(m), pg, L, s[l - 1]->u, s[l]->u, m->wcu[l]);
     MLCHK(m, hipGetLastError());
     hipLaunchKernelGGL(ml_velo_penta_kernel, dim3(nblk(10 * L)), dim3(NT), 0, mstream(m), pg, L, 1, s[l - 1]->u,
