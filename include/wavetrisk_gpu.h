@@ -224,6 +224,22 @@ typedef struct {
 } wtg_diffusion;
 int wtg_set_diffusion(wtg_ctx* ctx, const wtg_diffusion* d);
 
+/* remap_vertical_coordinates (remap.f90:18-75) of the incompressible layer
+ * model (remap_incompressible, remap.f90:147-216): the Lagrangian layers of
+ * slot `slot` remapped onto the hybrid coordinates z(k) = a_vert[k] eta +
+ * b_vert[k] z_s (k = 0..nlayers, bottom up, host arrays; the reference's
+ * a_vert / b_vert after initialize_a_b_vert), every member of an ensemble:
+ * the buoyancy and the edge velocities (not at the poles) remapped with the
+ * reference's remap_type "0" (WTG_REMAP_PIECEWISE_CONSTANT, remap0) or "1"
+ * (WTG_REMAP_PIECEWISE_LINEAR, remap1), the new mass rho0 dz; the slot is
+ * ghost filled before and after.  The reference's default "PPR" and its
+ * sigma_z coordinates are not built.  At most WTG_REMAP_MAX_LAYERS layers;
+ * the first work slot holds the old masses meanwhile. */
+#define WTG_REMAP_PIECEWISE_CONSTANT 0
+#define WTG_REMAP_PIECEWISE_LINEAR 1
+#define WTG_REMAP_MAX_LAYERS 64
+int wtg_remap(wtg_ctx* ctx, int slot, int scheme, const double* a_vert, const double* b_vert);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

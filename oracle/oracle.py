@@ -32,6 +32,8 @@ def load():
         _lib.wto_trend.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         _lib.wto_rk4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.wto_trend_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        _lib.wto_remap.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 5
+        _lib.wto_remap_column.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 4
         _lib.wto_rk4_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 3
         _lib.wto_num_threads.restype = C.c_int
         _lib.wto_node_ghost.argtypes = [C.c_void_p, C.c_void_p]
@@ -131,6 +133,15 @@ class Oracle:
         self.lib.wto_trend(C.byref(self.g), L, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data,
                            dm.ctypes.data, dt.ctypes.data, du.ctypes.data)
         return dm, dt, du
+
+    def remap(self, mass, temp, velo, scheme: int, a_vert, b_vert):
+        """remap_vertical_coordinates (incompressible, schemes 0 and 1) of a
+        ghost-filled state onto z = a_vert eta + b_vert z_s"""
+        mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))
+        a, b = _f64(a_vert).copy(), _f64(b_vert).copy()
+        self.lib.wto_remap(C.byref(self.g), int(scheme), mass.shape[0], a.ctypes.data, b.ctypes.data,
+                           mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
+        return mass, temp, velo
 
     def rk4(self, mass, temp, velo, dt: float, nsteps: int, diffusion=None):
         mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))

@@ -13,6 +13,10 @@
 !     or after every `every`-th step (tags numbered step / every).
 !
 ! Namelist &run (stdin): case ('tc5' | 'ocean'), zlev, nsteps, dt, out, ntimed, every.
+! vremap ('0', '1', '2PPM', 'PPR', ...; default none): after the steps, the
+! reference's remap_vertical_coordinates (remap.f90:18-75) onto the hybrid
+! coordinates of coords ('uniform' | 'chebyshev': b_vert as the test cases'
+! initialize_a_b_vert, a_vert = 1 - b_vert), dumped with the tag 'r0_'.
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
@@ -215,6 +219,7 @@ end module harness_ic
 
 program dump_trisk
   use time_integr_mod
+  use remap_mod
   use harness_hooks
   use harness_ic
   implicit none
@@ -225,10 +230,12 @@ program dump_trisk
   integer(8)     :: t_start, t_end, t_rate
   real(8)        :: step, vsclr(2), vdivu, vrotu
   logical        :: nanfill
-  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every
+  character(16)  :: vremap, coords
+  namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every, &
+                 vremap, coords
 
   case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0; every = 1
-  lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0
+  lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0; vremap = ''; coords = 'uniform' 
   read (5, nml=run)
   ic_case = case
 
@@ -328,6 +335,21 @@ program dump_trisk
      if (mod(n, every) == 0) &
           call dump_state ('s'//achar(48+(n/every)/10)//achar(48+mod(n/every,10))//'_')
   end do
+  if (len_trim(vremap) > 0) then
+     do k = 0, zlevels
+        if (trim(coords) == 'chebyshev') then
+           b_vert(k) = (1d0 + cos (dble(k)/dble(zlevels) * MATH_PI)) / 2d0
+        else
+           b_vert(k) = 1d0 - dble(k)/dble(zlevels)
+        end if
+     end do
+     a_vert = 1d0 - b_vert
+     remap_type = trim(vremap)
+     sigma_z = .false.
+     call remap_vertical_coordinates
+     call update_bdry (sol(:,1:zlevels), NONE, 1)
+     call dump_state ('r0_')
+  end if
   close (u)
   if (ntimed > 0) then
      call system_clock (t_start, t_rate)

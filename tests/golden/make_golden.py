@@ -229,11 +229,15 @@ def grid_fixture(J, geometry=True):
     print(f"{name}.npz written")
 
 
-def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1):
+def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
     """trisk_<name>.npz: the reference's initial state, trend and the state
-    after each RK4 step (after every `every`-th one: mass<k> is step k*every)"""
+    after each RK4 step (after every `every`-th one: mass<k> is step k*every);
+    remap = (scheme, coords): also the state after remap_vertical_coordinates
+    of the last step's state (remapped_mass, ...) and the target coordinates"""
     param, ml, dl = PARAMS[J]
     extra = f", every={every}" if every > 1 else ""
+    if remap:
+        extra += f", vremap='{remap[0]}', coords='{remap[1]}'"
     if diff:
         fd = lambda x: f"{x:.17e}".replace("e", "d")  # a Fortran double literal
         extra += (f", lsclr={diff['lsclr']}, ldivu={diff['ldivu']}, lrotu={diff['lrotu']}, "
@@ -290,6 +294,17 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1):
         data[f"mass{s}"] = field(f"s{s:02d}_mass", 1, False)
         data[f"temp{s}"] = field(f"s{s:02d}_temp", 1, False)
         data[f"velo{s}"] = field(f"s{s:02d}_velo", 3, False)
+    if remap:
+        data["remapped_mass"] = field("r0_mass", 1, False)
+        data["remapped_temp"] = field("r0_temp", 1, False)
+        data["remapped_velo"] = field("r0_velo", 3, False)
+        k = np.arange(L + 1)
+        b = (1.0 + np.cos(k / L * np.pi)) / 2.0 if remap[1] == "chebyshev" else 1.0 - k / L
+        data.update(remap_scheme=remap[0], b_vert=b, a_vert=1.0 - b)
+        keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
+                "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "remapped_mass",
+                "remapped_temp", "remapped_velo", "remap_scheme", "a_vert", "b_vert")
+        data = {k: v for k, v in data.items() if k in keep}
     if diff:
         data.update(laplace=np.array([diff["lsclr"], diff["ldivu"], diff["lrotu"]]),
                     visc=np.array([diff["vsclr"][0], diff["vsclr"][1], diff["vdivu"], diff["vrotu"]]))
@@ -652,6 +667,9 @@ def main():
     if "diffusion" in what or "trisk" in what:
         for c in DIFF_CASES:
             trisk_fixture(*c)
+    if "remap" in what:  # remap_vertical_coordinates (schemes 0 and 1) after two RK4 steps of the J4 ocean
+        for sch, crd in (("1", "chebyshev"), ("0", "uniform")):
+            trisk_fixture(f"ocean_remap{sch}_J4", 4, "ocean", 6, 1, 1200.0, remap=(sch, crd))
     if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
         trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:

@@ -85,6 +85,7 @@ SIGNATURES = {
     "wtg_set_stage_shape": (C.c_int, [_P, C.c_int]),
     "wtg_set_graph": (C.c_int, [_P, C.c_int]),
     "wtg_set_diffusion": (C.c_int, [_P, C.POINTER(_Diffusion)]),
+    "wtg_remap": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     "wtg_grid_shard_halo": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "wtg_ghost_fill_split_timed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D]),
     "wtg_fs_set_graph": (C.c_int, [_P, C.c_int]),
@@ -147,6 +148,7 @@ SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED 
 # precision of Context.rk4 / rk4_timed: FP64 (bit-exact with the reference's
 # RK4 of trend_ml) or FP32 bulk stencils with FP64 column scan and updates
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
+REMAP_PIECEWISE_CONSTANT, REMAP_PIECEWISE_LINEAR, REMAP_MAX_LAYERS = 0, 1, 64
 # launch shape of the FP32-bulk stage kernel (AUTO: timed on the first step)
 SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS, SHAPE_ONE_LAYER = 0, 1, 2, 3
 
@@ -360,6 +362,17 @@ class Context:
     def set_graph(self, enable: bool):
         """rk4 replays its steps from a captured HIP graph (identical results)."""
         _check(_lib.wtg_set_graph(self._h, int(bool(enable))), self._h, "wtg_set_graph")
+
+    def remap(self, slot: int, scheme: int, a_vert, b_vert):
+        """remap_vertical_coordinates of slot onto z = a_vert eta + b_vert z_s
+        (nlayers + 1 interfaces, bottom up): REMAP_PIECEWISE_CONSTANT or
+        REMAP_PIECEWISE_LINEAR (the reference's remap_type "0" / "1")"""
+        a = np.ascontiguousarray(a_vert, dtype=np.float64)
+        b = np.ascontiguousarray(b_vert, dtype=np.float64)
+        L = self.params.nlayers
+        if a.shape != (L + 1,) or b.shape != (L + 1,):
+            raise ValueError(f"a_vert and b_vert need {L + 1} interfaces")
+        _check(_lib.wtg_remap(self._h, slot, scheme, _ptr(a), _ptr(b)), self._h, "wtg_remap")
 
     def set_diffusion(self, laplace=(0, 0, 0), visc=(0.0, 0.0, 0.0, 0.0)):
         """Laplacian diffusion (order 1) or hyperdiffusion (order 2) in the

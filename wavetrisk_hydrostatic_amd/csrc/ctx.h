@@ -140,6 +140,8 @@ struct wtg_ctx {
   double *dif_rotu = nullptr;                       // [3] rot u on edges
   double *dif_phm = nullptr, *dif_pht = nullptr;    // [3] scalar diffusive fluxes
   double *dif_phu = nullptr;                        // [3] velocity source
+  // wtg_remap: a_vert then b_vert, [WTG_REMAP_MAX_LAYERS + 1] each (first use)
+  double* d_remap_ab = nullptr;
 };
 
 extern thread_local std::string g_create_error;

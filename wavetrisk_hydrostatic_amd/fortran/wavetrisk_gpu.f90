@@ -15,6 +15,8 @@ module wavetrisk_gpu
   integer(c_int), parameter :: WTG_SOLVE_BICGSTAB = 0, WTG_SOLVE_JACOBI = 1, WTG_SOLVE_FMG = 2, &
                                WTG_SOLVE_SRJ = 3, WTG_SOLVE_VCYCLE = 4, WTG_SOLVE_LEARNED = 5
   integer(c_int), parameter :: WTG_PRECISION_FP64 = 0, WTG_PRECISION_FP32_BULK = 1
+  ! remap_type "0" and "1" of remap_vertical_coordinates
+  integer(c_int), parameter :: WTG_REMAP_PIECEWISE_CONSTANT = 0, WTG_REMAP_PIECEWISE_LINEAR = 1
 
   type, bind(C) :: wtg_grid_desc
      integer(c_int) :: level, min_level, domain_level
@@ -109,6 +111,11 @@ module wavetrisk_gpu
      ! Laplacian diffusion / hyperdiffusion in the trend (FP64, single level)
      integer(c_int) function wtg_set_diffusion (ctx, d) bind(C)
        import; type(c_ptr), value :: ctx; type(wtg_diffusion), intent(in) :: d
+     end function
+     ! remap_vertical_coordinates onto z = a_vert eta + b_vert z_s (a_vert, b_vert(0:zlevels))
+     integer(c_int) function wtg_remap (ctx, slot, scheme, a_vert, b_vert) bind(C)
+       import; type(c_ptr), value :: ctx; integer(c_int), value :: slot, scheme
+       real(c_double), intent(in) :: a_vert(*), b_vert(*)
      end function
      ! ---- free surface (mode_split) ----
      integer(c_int) function wtg_fs_set_params (ctx, p) bind(C)
