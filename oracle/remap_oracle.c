@@ -5,6 +5,8 @@
  *   remap0 (piecewise constant)                         remap.f90:259-285
  *   remap1 (piecewise linear, harmonic-mean slopes, ten reconciliation sweeps
  *           with a minmod limiter)                      remap.f90:287-380
+ *   remap2PPM (Colella-Woodward piecewise parabolic, slope-limited)
+ *                                                       remap.f90:382-495
  *   remap_incompressible: old and new interface heights of the node's column
  *     and of the three edge columns (mean of the node's and the neighbour's),
  *     buoyancy and edge velocities remapped, new mass = rho0 dz, new mass-
@@ -94,11 +96,81 @@ static void remap1(int N, double* vn, const double* zn, const double* vo, const 
   free(Hz);
 }
 
+/* remap2PPM (remap.f90:382-495): arrays (0:N) of the reference at [k];
+ * var(k), Hz(k) at [k - 1].  FC holds slopes, then curvatures, then (in
+ * increasing k, as the reference overwrites it in place) the fluxes. */
+static void remap2ppm(int N, double* vn, const double* zn, const double* vo, const double* zo) {
+  double *Hz = calloc(N + 1, sizeof(double)), *aL = calloc(N + 1, sizeof(double)), *aR = calloc(N + 1, sizeof(double));
+  double *CF = calloc(N + 1, sizeof(double)), *FC = calloc(N + 1, sizeof(double)), *FC1 = calloc(N + 1, sizeof(double));
+#define V(k) vo[(k) - 1]
+#define HZ(k) Hz[(k) - 1]
+  for (int k = 1; k <= N; ++k) HZ(k) = zo[k] - zo[k - 1];
+  for (int k = 1; k <= N - 1; ++k) {
+    const double cff = 1.0 / (HZ(k) + HZ(k + 1));
+    CF[k] = cff * (V(k + 1) * HZ(k) + V(k) * HZ(k + 1));
+    FC[k] = cff * (V(k + 1) - V(k));
+  }
+  for (int k = 2; k <= N - 1; ++k) {
+    double cff = HZ(k) * ((2.0 * HZ(k - 1) + HZ(k)) * FC[k] + (2.0 * HZ(k + 1) + HZ(k)) * FC[k - 1]) /
+                 (HZ(k - 1) + HZ(k) + HZ(k + 1));
+    const double cffR = 2.0 * (V(k + 1) - V(k)), cffL = 2.0 * (V(k) - V(k - 1));
+    if (cffR * cffL > 0.0) {
+      if (fabs(cffL) < fabs(cff)) cff = cffL;
+      if (fabs(cffR) < fabs(cff)) cff = cffR;
+    } else {
+      cff = 0.0;
+    }
+    FC1[k] = cff;
+  }
+  FC1[N] = HZ(N) * FC[N - 1];
+  FC1[1] = HZ(1) * FC[1];
+  for (int k = 1; k <= N - 1; ++k) {
+    const int k1 = k - 1 > 1 ? k - 1 : 1, k2 = k + 2 < N ? k + 2 : N;
+    const double cff = HZ(k1) + HZ(k) + HZ(k + 1) + HZ(k2);
+    const double cffL = (HZ(k1) + HZ(k)) / (cff * (2.0 * HZ(k) + HZ(k + 1)));
+    const double cffR = (HZ(k + 1) + HZ(k2)) / (cff * (HZ(k) + 2.0 * HZ(k + 1)));
+    aR[k] = CF[k] + 2.0 * HZ(k) * HZ(k + 1) * (cffL - cffR) * FC[k] - cffL * HZ(k) * FC1[k + 1] +
+            cffR * HZ(k + 1) * FC1[k];
+    aL[k + 1] = aR[k];
+  }
+  aR[N] = 2.0 * V(N) - aL[N];
+  aL[1] = 2.0 * V(1) - aR[1];
+  for (int k = 1; k <= N; ++k) {
+    CF[k] = 0.5 * (aR[k] - aL[k]);
+    FC[k] = 0.5 * (aR[k] + aL[k]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    double alpha, cff, cffR, cffL;
+    if (dz > 0.0) {
+      alpha = HZ(k + 1);
+      cff = aL[k + 1];
+      cffR = CF[k + 1];
+      cffL = FC[k + 1];
+    } else {
+      alpha = -HZ(k);
+      cff = aR[k];
+      cffR = -CF[k];
+      cffL = FC[k];
+    }
+    alpha = dz / alpha;
+    FC[k] = dz * (cff + alpha * (cffR - cffL * (3.0 - 2.0 * alpha)));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (HZ(k) * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+#undef V
+#undef HZ
+  free(Hz); free(aL); free(aR); free(CF); free(FC); free(FC1);
+}
+
 void wto_remap_column(int scheme, int N, double* vn, const double* zn, const double* vo, const double* zo) {
   if (scheme == 0)
     remap0(N, vn, zn, vo, zo);
-  else
+  else if (scheme == 1)
     remap1(N, vn, zn, vo, zo);
+  else
+    remap2ppm(N, vn, zn, vo, zo);
 }
 
 /* find_coordinates_incompressible of column c from the saved masses */

@@ -300,7 +300,7 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
         data["remapped_velo"] = field("r0_velo", 3, False)
         k = np.arange(L + 1)
         b = (1.0 + np.cos(k / L * np.pi)) / 2.0 if remap[1] == "chebyshev" else 1.0 - k / L
-        data.update(remap_scheme=remap[0], b_vert=b, a_vert=1.0 - b)
+        data.update(remap_scheme={"0": 0, "1": 1, "2PPM": 2}[remap[0]], b_vert=b, a_vert=1.0 - b)
         keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
                 "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "remapped_mass",
                 "remapped_temp", "remapped_velo", "remap_scheme", "a_vert", "b_vert")
@@ -668,8 +668,8 @@ def main():
         for c in DIFF_CASES:
             trisk_fixture(*c)
     if "remap" in what:  # remap_vertical_coordinates (schemes 0 and 1) after two RK4 steps of the J4 ocean
-        for sch, crd in (("1", "chebyshev"), ("0", "uniform")):
-            trisk_fixture(f"ocean_remap{sch}_J4", 4, "ocean", 6, 1, 1200.0, remap=(sch, crd))
+        for sch, crd in (("1", "chebyshev"), ("0", "uniform"), ("2PPM", "chebyshev")):
+            trisk_fixture(f"ocean_remap{sch.lower()}_J4", 4, "ocean", 6, 1, 1200.0, remap=(sch, crd))
     if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
         trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:

@@ -1,8 +1,9 @@
 """Vertical remap of the Lagrangian layers on the GPU (wtg_remap; needs a GPU).
 
 Bars: against the reference's remap_vertical_coordinates (fixtures
-trisk_ocean_remap{1,0}_J4: remap_type "1" onto Chebyshev and "0" onto uniform
-hybrid coordinates after one RK4 step of the J4 ocean, 6 layers) within 1e-15
+trisk_ocean_remap{1,0,2ppm}_J4: remap_type "1" and "2PPM" onto Chebyshev and
+"0" onto uniform hybrid coordinates after one RK4 step of the J4 ocean, 6
+layers) within 1e-15
 relative; against the oracle on a J7 ocean with 20 layers within 1e-15 (same
 evaluation order, no FMA contraction: observed bit for bit); every member of
 an ensemble equal to its own single-member run bit for bit; at J8 with 30
@@ -26,7 +27,7 @@ def _coords(L, kind):
     return 1.0 - b, b
 
 
-@pytest.mark.parametrize("case", ["ocean_remap1_J4", "ocean_remap0_J4"])
+@pytest.mark.parametrize("case", ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4"])
 def test_remap_matches_reference(case):
     print(f"remap {case}", flush=True)
     f = golden(f"trisk_{case}.npz")
@@ -48,7 +49,8 @@ def test_remap_matches_reference(case):
 
 
 # the synthetic ocean's layers are uniform: both schemes remap onto Chebyshev coordinates
-@pytest.mark.parametrize("scheme,kind", [(W.REMAP_PIECEWISE_LINEAR, "chebyshev"), (W.REMAP_PIECEWISE_CONSTANT, "chebyshev")])
+@pytest.mark.parametrize("scheme,kind", [(W.REMAP_PIECEWISE_LINEAR, "chebyshev"), (W.REMAP_PIECEWISE_CONSTANT, "chebyshev"),
+                                         (W.REMAP_PARABOLIC_PPM, "chebyshev")])
 def test_remap_j7_l20_equals_oracle(scheme, kind):
     print(f"remap J7 L20 scheme {scheme} onto {kind} coordinates", flush=True)
     from oracle.oracle import Oracle, geometry_of
@@ -125,7 +127,7 @@ def test_remap_refusals():
     c.upload_state(0, m, t, u)
     a, b = _coords(3, "uniform")
     with pytest.raises(W.WtgError):
-        c.remap(0, 2, a, b)  # the reference's remap_type "2PPM" and beyond are not built
+        c.remap(0, 3, a, b)  # the reference's remap_type "2S", "2W", "4" and "PPR" are not built
     with pytest.raises(ValueError):
         c.remap(0, W.REMAP_PIECEWISE_LINEAR, a[:-1], b[:-1])
     c.close()

@@ -5,10 +5,11 @@ Fixtures tests/golden/trisk_ocean_remap{1,0}_J4.npz come from the
 reference's remap_vertical_coordinates (remap.f90:18-75, incompressible
 branch 147-257) run by the harness (oracle/ref_harness/dump_trisk.f90) after
 one RK4 step of the J4 ocean with 6 layers: remap_type "1" (piecewise linear)
-onto Chebyshev hybrid coordinates and remap_type "0" (piecewise constant)
-onto uniform ones (b_vert as the test cases' initialize_a_b_vert, a_vert =
-1 - b_vert).  The reference's default, "PPR" (Engwirda's piecewise-parabolic
-library), is not restated.
+and "2PPM" (Colella-Woodward piecewise parabolic) onto Chebyshev hybrid
+coordinates and remap_type "0" (piecewise constant) onto uniform ones (b_vert
+as the test cases' initialize_a_b_vert, a_vert = 1 - b_vert).  The
+reference's default, "PPR" (Engwirda's piecewise-polynomial library), is not
+restated.
 
 Bars: the oracle equals the reference to 1e-15 relative (observed: bit for
 bit); closed-form properties of the finite-volume remap hold to rounding:
@@ -24,7 +25,7 @@ import pytest
 from conftest import golden, maxrel
 from helpers import grid_of_fixture, oracle_of
 
-CASES = ["ocean_remap1_J4", "ocean_remap0_J4"]
+CASES = ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4"]
 
 
 def load(case):
@@ -74,7 +75,7 @@ def test_remap_conserves_columns_and_is_idempotent(case):
 @pytest.mark.parametrize("case", CASES)
 def test_planted_errors_are_caught(case):
     f, g, o, st, scheme = load(case)
-    other = o.remap(*st, 1 - scheme, f["a_vert"], f["b_vert"])
+    other = o.remap(*st, (scheme + 1) % 3, f["a_vert"], f["b_vert"])
     assert max(errors(f, g, other)) > 1e-9
     b = f["b_vert"].copy()
     b[1:-1] += 1e-6  # the inner interfaces moved by 1e-6 of the column depth

@@ -148,7 +148,7 @@ SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED 
 # precision of Context.rk4 / rk4_timed: FP64 (bit-exact with the reference's
 # RK4 of trend_ml) or FP32 bulk stencils with FP64 column scan and updates
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
-REMAP_PIECEWISE_CONSTANT, REMAP_PIECEWISE_LINEAR, REMAP_MAX_LAYERS = 0, 1, 64
+REMAP_PIECEWISE_CONSTANT, REMAP_PIECEWISE_LINEAR, REMAP_PARABOLIC_PPM, REMAP_MAX_LAYERS = 0, 1, 2, 64
 # launch shape of the FP32-bulk stage kernel (AUTO: timed on the first step)
 SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS, SHAPE_ONE_LAYER = 0, 1, 2, 3
 

@@ -1,9 +1,10 @@
 // Vertical remap of the Lagrangian layers (wtg_remap): remap_vertical_coordinates
 // (remap.f90:18-75) of the incompressible layer model (remap_incompressible,
 // remap.f90:147-216) with the reference's remap_type "0" (remap0,
-// piecewise constant, remap.f90:259-285) or "1" (remap1, piecewise linear
+// piecewise constant, remap.f90:259-285), "1" (remap1, piecewise linear
 // with harmonic-mean slopes and ten minmod reconciliation sweeps,
-// remap.f90:287-380).
+// remap.f90:287-380) or "2PPM" (remap2PPM, Colella-Woodward piecewise
+// parabolic with limited slopes, remap.f90:382-495).
 //
 // One thread per column (owned node or pole master; grid y = ensemble
 // member): the old interface heights z_old(k) = z_s + sum of mass / rho0 and
@@ -37,7 +38,7 @@ struct RemapArgs {
 };
 
 struct Work {
-  double FC[RMAX + 1], aL[RMAX], aR[RMAX], Hz[RMAX];
+  double FC[RMAX + 1], aL[RMAX + 1], aR[RMAX + 1], Hz[RMAX + 1], CF[RMAX + 1], FC1[RMAX + 1];
 };
 
 __device__ void remap0(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
@@ -88,12 +89,73 @@ __device__ void remap1(int N, double* vn, const double* zn, const double* vo, co
   for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k - 1] * vo[k - 1] + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
 }
 
+// remap2PPM: the reference's (0:N) arrays at [k], var(k) and Hz(k) at [k - 1];
+// FC holds slopes, then curvatures, then the fluxes (in increasing k, as the
+// reference overwrites it in place)
+__device__ void remap2ppm(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
+  double *FC = w.FC, *aL = w.aL, *aR = w.aR, *Hz = w.Hz, *CF = w.CF, *FC1 = w.FC1;
+  for (int k = 0; k <= N; ++k) FC[k] = CF[k] = FC1[k] = aL[k] = aR[k] = 0.0;
+#define V(k) vo[(k) - 1]
+#define HZ(k) Hz[(k) - 1]
+  for (int k = 1; k <= N; ++k) HZ(k) = zo[k] - zo[k - 1];
+  for (int k = 1; k <= N - 1; ++k) {
+    const double cff = 1.0 / (HZ(k) + HZ(k + 1));
+    CF[k] = cff * (V(k + 1) * HZ(k) + V(k) * HZ(k + 1));
+    FC[k] = cff * (V(k + 1) - V(k));
+  }
+  for (int k = 2; k <= N - 1; ++k) {
+    double cff = HZ(k) * ((2.0 * HZ(k - 1) + HZ(k)) * FC[k] + (2.0 * HZ(k + 1) + HZ(k)) * FC[k - 1]) /
+                 (HZ(k - 1) + HZ(k) + HZ(k + 1));
+    const double cffR = 2.0 * (V(k + 1) - V(k)), cffL = 2.0 * (V(k) - V(k - 1));
+    if (cffR * cffL > 0.0) {
+      if (fabs(cffL) < fabs(cff)) cff = cffL;
+      if (fabs(cffR) < fabs(cff)) cff = cffR;
+    } else {
+      cff = 0.0;
+    }
+    FC1[k] = cff;
+  }
+  FC1[N] = HZ(N) * FC[N - 1];
+  FC1[1] = HZ(1) * FC[1];
+  for (int k = 1; k <= N - 1; ++k) {
+    const int k1 = max(k - 1, 1), k2 = min(k + 2, N);
+    const double cff = HZ(k1) + HZ(k) + HZ(k + 1) + HZ(k2);
+    const double cffL = (HZ(k1) + HZ(k)) / (cff * (2.0 * HZ(k) + HZ(k + 1)));
+    const double cffR = (HZ(k + 1) + HZ(k2)) / (cff * (HZ(k) + 2.0 * HZ(k + 1)));
+    aR[k] = CF[k] + 2.0 * HZ(k) * HZ(k + 1) * (cffL - cffR) * FC[k] - cffL * HZ(k) * FC1[k + 1] +
+            cffR * HZ(k + 1) * FC1[k];
+    aL[k + 1] = aR[k];
+  }
+  aR[N] = 2.0 * V(N) - aL[N];
+  aL[1] = 2.0 * V(1) - aR[1];
+  for (int k = 1; k <= N; ++k) {
+    CF[k] = 0.5 * (aR[k] - aL[k]);
+    FC[k] = 0.5 * (aR[k] + aL[k]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    const bool up = dz > 0.0;
+    const double alpha = dz / (up ? HZ(k + 1) : -HZ(k));
+    const double cff = up ? aL[k + 1] : aR[k];
+    const double cffR = up ? CF[k + 1] : -CF[k];
+    const double cffL = up ? FC[k + 1] : FC[k];
+    FC[k] = dz * (cff + alpha * (cffR - cffL * (3.0 - 2.0 * alpha)));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (HZ(k) * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+#undef V
+#undef HZ
+}
+
 __device__ __forceinline__ void remap_col(int scheme, int N, double* vn, const double* zn, const double* vo,
                                           const double* zo, Work& w) {
   if (scheme == 0)
     remap0(N, vn, zn, vo, zo, w);
-  else
+  else if (scheme == 1)
     remap1(N, vn, zn, vo, zo, w);
+  else
+    remap2ppm(N, vn, zn, vo, zo, w);
 }
 
 // find_coordinates_incompressible (remap.f90:235-257) of column c
@@ -158,8 +220,9 @@ __global__ __launch_bounds__(RT) void remap_kernel(RemapArgs a) {
 extern "C" int wtg_remap(wtg_ctx* c, int slot, int scheme, const double* a_vert, const double* b_vert) {
   if (!c || !a_vert || !b_vert) return WTG_E_ARG;
   if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
-  if (scheme != WTG_REMAP_PIECEWISE_CONSTANT && scheme != WTG_REMAP_PIECEWISE_LINEAR)
-    return wtg_fail(c, WTG_E_ARG, "remap scheme %d is not built (0: remap0, 1: remap1)", scheme);
+  if (scheme != WTG_REMAP_PIECEWISE_CONSTANT && scheme != WTG_REMAP_PIECEWISE_LINEAR &&
+      scheme != WTG_REMAP_PARABOLIC_PPM)
+    return wtg_fail(c, WTG_E_ARG, "remap scheme %d is not built (0: remap0, 1: remap1, 2: remap2PPM)", scheme);
   const int L = c->params.nlayers, M = c->nmembers;
   if (L > RMAX) return wtg_fail(c, WTG_E_ARG, "the remap takes at most %d layers, not %d", RMAX, L);
   if (!c->topo_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_topography must be called first");
