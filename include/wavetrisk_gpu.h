@@ -231,14 +231,17 @@ int wtg_set_diffusion(wtg_ctx* ctx, const wtg_diffusion* d);
  * a_vert / b_vert after initialize_a_b_vert), every member of an ensemble:
  * the buoyancy and the edge velocities (not at the poles) remapped with the
  * reference's remap_type "0" (WTG_REMAP_PIECEWISE_CONSTANT, remap0), "1"
- * (WTG_REMAP_PIECEWISE_LINEAR, remap1) or "2PPM" (WTG_REMAP_PARABOLIC_PPM,
- * remap2PPM), the new mass rho0 dz; the slot is ghost filled before and
- * after.  The reference's default "PPR", its "2S", "2W", "4" and its sigma_z
- * coordinates are not built.  At most WTG_REMAP_MAX_LAYERS layers;
- * the first work slot holds the old masses meanwhile. */
+ * (WTG_REMAP_PIECEWISE_LINEAR, remap1), "2PPM" (WTG_REMAP_PARABOLIC_PPM,
+ * remap2PPM) or its default "PPR" (WTG_REMAP_PPR, remapPPR at
+ * remap.f90:950-1010: order 2, mono limiter, loose ends), the new mass
+ * rho0 dz; the slot is ghost filled before and after.  The reference's
+ * "2S", "2W", "4" and its sigma_z coordinates are not built.  At most
+ * WTG_REMAP_MAX_LAYERS layers; the first work slot holds the old masses
+ * meanwhile. */
 #define WTG_REMAP_PIECEWISE_CONSTANT 0
 #define WTG_REMAP_PIECEWISE_LINEAR 1
 #define WTG_REMAP_PARABOLIC_PPM 2
+#define WTG_REMAP_PPR 3
 #define WTG_REMAP_MAX_LAYERS 64
 int wtg_remap(wtg_ctx* ctx, int slot, int scheme, const double* a_vert, const double* b_vert);
 

@@ -135,8 +135,9 @@ class Oracle:
         return dm, dt, du
 
     def remap(self, mass, temp, velo, scheme: int, a_vert, b_vert):
-        """remap_vertical_coordinates (incompressible, schemes 0 and 1) of a
-        ghost-filled state onto z = a_vert eta + b_vert z_s"""
+        """remap_vertical_coordinates (incompressible; scheme 0 remap0, 1 remap1,
+        2 remap2PPM, 3 remapPPR) of a ghost-filled state onto
+        z = a_vert eta + b_vert z_s"""
         mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))
         a, b = _f64(a_vert).copy(), _f64(b_vert).copy()
         self.lib.wto_remap(C.byref(self.g), int(scheme), mass.shape[0], a.ctypes.data, b.ctypes.data,
@@ -153,6 +154,17 @@ class Oracle:
             return mass, temp, velo
         self.lib.wto_rk4(C.byref(self.g), L, dt, nsteps, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
         return mass, temp, velo
+
+
+def remap_column(scheme: int, var_old, z_old, z_new):
+    """One column of the vertical remap (remap.f90 remap0 / remap1 /
+    remap2PPM / remapPPR): cell values on the N cells of z_new (N + 1
+    interface heights, bottom up) from those on the cells of z_old."""
+    lib = load()
+    vo, zo, zn = _f64(var_old).copy(), _f64(z_old).copy(), _f64(z_new).copy()
+    vn = np.zeros_like(vo)
+    lib.wto_remap_column(int(scheme), vo.size, vn.ctypes.data, zn.ctypes.data, vo.ctypes.data, zo.ctypes.data)
+    return vn
 
 
 def geometry_of(grid) -> dict:

@@ -17,6 +17,8 @@
 ! reference's remap_vertical_coordinates (remap.f90:18-75) onto the hybrid
 ! coordinates of coords ('uniform' | 'chebyshev': b_vert as the test cases'
 ! initialize_a_b_vert, a_vert = 1 - b_vert), dumped with the tag 'r0_'.
+! uniform_first (logical): before that remap, a piecewise-constant remap onto
+! uniform coordinates (so the old layers are evenly spaced), dumped as 'p0_'.
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
@@ -229,13 +231,14 @@ program dump_trisk
   integer        :: zlev, nsteps, d, k, l, n, u, v, ntimed, lsclr, ldivu, lrotu, every
   integer(8)     :: t_start, t_end, t_rate
   real(8)        :: step, vsclr(2), vdivu, vrotu
-  logical        :: nanfill
+  logical        :: nanfill, uniform_first
   character(16)  :: vremap, coords
   namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every, &
-                 vremap, coords
+                 vremap, coords, uniform_first
 
   case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0; every = 1
-  lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0; vremap = ''; coords = 'uniform' 
+  lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0; vremap = ''; coords = 'uniform'
+  uniform_first = .false.
   read (5, nml=run)
   ic_case = case
 
@@ -335,6 +338,17 @@ program dump_trisk
      if (mod(n, every) == 0) &
           call dump_state ('s'//achar(48+(n/every)/10)//achar(48+mod(n/every,10))//'_')
   end do
+  if (len_trim(vremap) > 0 .and. uniform_first) then
+     do k = 0, zlevels
+        b_vert(k) = 1d0 - dble(k)/dble(zlevels)
+     end do
+     a_vert = 1d0 - b_vert
+     remap_type = '0'
+     sigma_z = .false.
+     call remap_vertical_coordinates
+     call update_bdry (sol(:,1:zlevels), NONE, 1)
+     call dump_state ('p0_')
+  end if
   if (len_trim(vremap) > 0) then
      do k = 0, zlevels
         if (trim(coords) == 'chebyshev') then

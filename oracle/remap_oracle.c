@@ -13,6 +13,12 @@
  *     weighted buoyancy = rho0 dz theta                 remap.f90:147-216
  *   find_coordinates_incompressible (hybrid a/b coordinates)
  *                                                       remap.f90:235-257
+ *   remapPPR (Engwirda's PPR library at order 2: p3e cubic edge values with
+ *           loose end fits, PPM cells with the mono limiter, integral
+ *           remap with the compensated conservation fix)
+ *                                                       remap.f90:950-1010,
+ *           ppr/rmap1d.f90, rcon1d.f90, p3e.f90, p1e.f90, pbc.f90,
+ *           ppm.f90, plm.f90 (plsv, plsc), inv.f90 (slv_3x3, slv_4x4)
  *   apply_no_bdry: owned nodes, then the poles without their velocities
  *                                                       domain.f90:324-341
  */
@@ -164,13 +170,303 @@ static void remap2ppm(int N, double* vn, const double* zn, const double* vo, con
   free(Hz); free(aL); free(aR); free(CF); free(FC); free(FC1);
 }
 
+/* ---- remapPPR (remap.f90:950-1010): Engwirda's PPR library at order 2 ----
+ * cubic edge values (p3e.f90) with loose one-sided end fits (pbc.f90 lbc/ubc),
+ * monotone piecewise-parabolic cells (ppm.f90 ppm/ppmfn, plm.f90 plsv/plsc),
+ * exact integration of the cell polynomials over the new cells with the
+ * compensated conservation fix (rmap1d.f90 rmap1d/imap1d).  Arrays below are
+ * indexed as the library's: edges 1..N+1, cells 1..N (element 0 unused). */
+
+/* integer powers as the reference's compiled x**k evaluates them: products
+ * from the left (x**4 as a square of squares differs in the last bit) */
+static double ppr_p2(double x) { return x * x; }
+static double ppr_p3(double x) { return (x * x) * x; }
+static double ppr_p4(double x) { return ((x * x) * x) * x; }
+
+/* slv_3x3 (inv.f90:64-124, 400-445): adjugate and determinant, solve in place */
+static int ppr_solve3(const double A[3][3], double v[3], double eps) {
+  const double a2233 = A[1][1] * A[2][2], a2332 = A[1][2] * A[2][1], a2133 = A[1][0] * A[2][2];
+  const double a2331 = A[1][2] * A[2][0], a2132 = A[1][0] * A[2][1], a2231 = A[1][1] * A[2][0];
+  const double det = A[0][0] * (a2233 - a2332) - A[0][1] * (a2133 - a2331) + A[0][2] * (a2132 - a2231);
+  const double a1233 = A[0][1] * A[2][2], a1332 = A[0][2] * A[2][1], a1223 = A[0][1] * A[1][2];
+  const double a1322 = A[0][2] * A[1][1], a1133 = A[0][0] * A[2][2], a1331 = A[0][2] * A[2][0];
+  const double a1123 = A[0][0] * A[1][2], a1321 = A[0][2] * A[1][0], a1132 = A[0][0] * A[2][1];
+  const double a1231 = A[0][1] * A[2][0], a1122 = A[0][0] * A[1][1], a1221 = A[0][1] * A[1][0];
+  const double adj[3][3] = {{(a2233 - a2332), -(a1233 - a1332), (a1223 - a1322)},
+                            {-(a2133 - a2331), (a1133 - a1331), -(a1123 - a1321)},
+                            {(a2132 - a2231), -(a1132 - a1231), (a1122 - a1221)}};
+  if (!(fabs(det) > eps)) return 0;
+  double t[3];
+  for (int r = 0; r < 3; ++r) t[r] = (adj[r][0] * v[0] + adj[r][1] * v[1] + adj[r][2] * v[2]) / det;
+  v[0] = t[0]; v[1] = t[1]; v[2] = t[2];
+  return 1;
+}
+
+/* C += s (A B) for 2 x 2 blocks (inv.f90 mul_2x2; its +1 / -1 branches are the
+ * same numbers as the general one) */
+static void ppr_mul2(const double A[2][2], const double B[2][2], double s, double C[2][2]) {
+  for (int c = 0; c < 2; ++c)
+    for (int r = 0; r < 2; ++r) {
+      const double p = A[r][0] * B[0][c] + A[r][1] * B[1][c];
+      C[r][c] = s == 1.0 ? C[r][c] + p : s == -1.0 ? C[r][c] - p : C[r][c] + s * p;
+    }
+}
+
+/* slv_4x4 (inv.f90:447-600): block LDU through 2 x 2 adjugates; the second
+ * singularity test repeats the first (as the library does) */
+static int ppr_solve4(const double A[4][4], double v[4], double eps) {
+  const double A11[2][2] = {{A[0][0], A[0][1]}, {A[1][0], A[1][1]}}, A12[2][2] = {{A[0][2], A[0][3]}, {A[1][2], A[1][3]}};
+  const double A21[2][2] = {{A[2][0], A[2][1]}, {A[3][0], A[3][1]}};
+  const double adet = A11[0][0] * A11[1][1] - A11[0][1] * A11[1][0];
+  const double ai[2][2] = {{A11[1][1], -A11[0][1]}, {-A11[1][0], A11[0][0]}};
+  if (!(fabs(adet) > eps)) return 0;
+  double L[2][2] = {{0, 0}, {0, 0}}, U[2][2] = {{0, 0}, {0, 0}};
+  double Sm[2][2] = {{A[2][2], A[2][3]}, {A[3][2], A[3][3]}};
+  ppr_mul2(A21, ai, 1.0, L);
+  ppr_mul2(ai, A12, 1.0, U);
+  ppr_mul2(L, A12, -1.0 / adet, Sm);
+  const double sdet = Sm[0][0] * Sm[1][1] - Sm[0][1] * Sm[1][0];
+  const double si[2][2] = {{Sm[1][1], -Sm[0][1]}, {-Sm[1][0], Sm[0][0]}};
+  v[2] = v[2] - (L[0][0] * v[0] + L[0][1] * v[1]) / adet;
+  v[3] = v[3] - (L[1][0] * v[0] + L[1][1] * v[1]) / adet;
+  const double t0 = (ai[0][0] * v[0] + ai[0][1] * v[1]) / adet, t1 = (ai[1][0] * v[0] + ai[1][1] * v[1]) / adet;
+  v[0] = t0; v[1] = t1;
+  const double t2 = (si[0][0] * v[2] + si[0][1] * v[3]) / sdet, t3 = (si[1][0] * v[2] + si[1][1] * v[3]) / sdet;
+  v[2] = t2; v[3] = t3;
+  v[0] = v[0] - (U[0][0] * v[2] + U[0][1] * v[3]) / adet;
+  v[1] = v[1] - (U[1][0] * v[2] + U[1][1] * v[3]) / adet;
+  return 1;
+}
+
+/* loose end fit (pbc.f90 lbc / ubc): the quadratic whose means match the three
+ * cells c-1, c, c+1, evaluated at the edges c-1 .. c+1 (low end, lo = 1) or
+ * c .. c+2 (high end, lo = 0); dx has nd = 1 entry when the column is uniform */
+static void ppr_end_fit(int lo, int c, const double* dx, int nd, const double* f, double dmin, double* edge) {
+  double xhat, h[3];
+  if (nd > 1) {
+    xhat = fmax(dx[c], dmin) * 0.5;
+    h[0] = dx[c - 1]; h[1] = dx[c]; h[2] = dx[c + 1];
+  } else {
+    xhat = fmax(dx[1], dmin) * 0.5;
+    h[0] = h[1] = h[2] = dx[1];
+  }
+  const double s[4] = {-(h[0] + h[1] * 0.5) / xhat, -1.0, 1.0, (h[2] + h[1] * 0.5) / xhat};
+  double A[3][3], v[3];
+  for (int r = 0; r < 3; ++r) {
+    A[r][0] = s[r + 1] / 1.0 - s[r] / 1.0;
+    A[r][1] = ppr_p2(s[r + 1]) / 2.0 - ppr_p2(s[r]) / 2.0;
+    A[r][2] = ppr_p3(s[r + 1]) / 3.0 - ppr_p3(s[r]) / 3.0;
+    v[r] = h[r] * f[c - 1 + r] / xhat;
+  }
+  const int e0 = lo ? c - 1 : c;  /* first edge written */
+  if (ppr_solve3(A, v, 1.0e-14 * dmin)) {
+    for (int k = 0; k < 3; ++k) {
+      const double x = s[k + (lo ? 0 : 1)];
+      edge[e0 + k] = 1.0 * v[0] + (x * 1.0) * v[1] + (ppr_p2(x) * 1.0) * v[2];
+    }
+  } else if (lo) {
+    edge[c - 1] = f[c - 1] * 1.0;
+    edge[c] = f[c - 1] * 0.5 + f[c] * 0.5;
+    edge[c + 1] = f[c] * 0.5 + f[c + 1] * 0.5;
+  } else {
+    edge[c] = f[c - 1] * 0.5 + f[c] * 0.5;
+    edge[c + 1] = f[c] * 0.5 + f[c + 1] * 0.5;
+    edge[c + 2] = f[c + 1] * 1.0;
+  }
+}
+
+/* edge values: p3e.f90 (p1e.f90 for at most three cells) */
+static void ppr_edges(int N, const double* dx, int nd, const double* f, double dmin, double* edge) {
+  const int npos = N + 1;
+  if (npos <= 4) {  /* p1e: two-point means inside, the end cells' values at the ends */
+    for (int i = 2; i <= npos - 1; ++i) {
+      if (nd == 1) {
+        const double d = dx[1] * 2.0;
+        edge[i] = (dx[1] * f[i - 1] + dx[1] * f[i]) / d;
+      } else {
+        const double hl = fmax(dx[i - 1], dmin), h0 = fmax(dx[i], dmin);
+        edge[i] = (h0 * f[i - 1] + hl * f[i]) / (hl + h0);
+      }
+    }
+    edge[1] = f[1];
+    edge[npos] = f[npos - 1];
+    return;
+  }
+  ppr_end_fit(1, 2, dx, nd, f, dmin, edge);
+  ppr_end_fit(0, npos - 2, dx, nd, f, dmin, edge);
+  for (int i = 3; i <= npos - 2; ++i) {
+    if (nd == 1) {
+      edge[i] = (-(1.0 * f[i - 2]) + 7.0 * f[i - 1] + 7.0 * f[i] - 1.0 * f[i + 1]) / 12.0;
+      continue;
+    }
+    const double h[4] = {dx[i - 2], dx[i - 1], dx[i], dx[i + 1]};
+    const double xhat = 0.5 * fmax(h[1], dmin) + 0.5 * fmax(h[2], dmin);
+    const double s[5] = {-(h[0] + h[1]) / xhat, -h[1] / xhat, 0.0, h[2] / xhat, (h[2] + h[3]) / xhat};
+    double iv[5][4];
+    for (int k = 0; k < 5; ++k) {
+      iv[k][0] = s[k] / 1.0;
+      iv[k][1] = ppr_p2(s[k]) / 2.0;
+      iv[k][2] = ppr_p3(s[k]) / 3.0;
+      iv[k][3] = ppr_p4(s[k]) / 4.0;
+    }
+    double A[4][4], v[4];
+    for (int r = 0; r < 4; ++r) {
+      for (int p = 0; p < 4; ++p) A[r][p] = iv[r + 1][p] - iv[r][p];
+      v[r] = dx[i - 2 + r] * f[i - 2 + r] / xhat;
+    }
+    if (ppr_solve4(A, v, 1.0e-14 * dmin))
+      edge[i] = v[0];
+    else
+      edge[i] = f[i - 1] * 0.5 + f[i] * 0.5;
+  }
+}
+
+/* limited slope of the cell (plm.f90 plsv for a non-uniform column, plsc for a
+ * uniform one): only the centred slope dfds(0) is used by ppmfn */
+static double ppr_slope(int uniform, double fl, double hl, double f0, double h0, double fr, double hr) {
+  const double dm = f0 - fl, dp = fr - f0;
+  if (!(dm * dp > 0.0)) return 0.0;
+  double el, er;
+  if (uniform) {
+    el = (fl + f0) * 0.5;
+    er = (f0 + fr) * 0.5;
+  } else {
+    el = (h0 * fl + hl * f0) / (hl + h0);
+    er = (hr * f0 + h0 * fr) / (h0 + hr);
+  }
+  const double d0 = 0.5 * (er - el);
+  double scal = fmin(fabs(dm), fabs(dp)) / fmax(fabs(d0), 1.0e-14);
+  scal = fmin(scal, 1.0);
+  return scal * d0;
+}
+
+static void ppr_parabola(double f0, double el, double er, double c[3]) {
+  c[0] = 1.5 * f0 - 0.25 * (er + el);
+  c[1] = 0.5 * (er - el);
+  c[2] = -(1.5 * f0) + 0.75 * (er + el);
+}
+
+/* ppm.f90 ppm + ppmfn with the mono limiter: coefficients of 1, s, s^2 on the
+ * cell mapped to s in [-1, 1] */
+static void ppr_cells(int N, const double* dx, int nd, const double* f, const double* edge, double (*fh)[3]) {
+  if (N == 1) {
+    fh[1][0] = f[1]; fh[1][1] = 0.0; fh[1][2] = 0.0;
+    return;
+  }
+  for (int i = 1; i <= N; ++i) {
+    const int il = i - 1 > 1 ? i - 1 : 1, ir = i + 1 < N ? i + 1 : N;
+    const double f0 = f[i], fl = f[il], fr = f[ir];
+    double el = edge[i], er = edge[i + 1];
+    const double d0 = nd > 1 ? ppr_slope(0, fl, dx[il], f0, dx[i], fr, dx[ir]) : ppr_slope(1, fl, 0, f0, 0, fr, 0);
+    double u[3], l[3];
+    ppr_parabola(f0, el, er, u);
+    if ((fr - f0) * (f0 - fl) < 0.0) {
+      l[0] = f0; l[1] = 0.0; l[2] = 0.0;
+    } else {
+      if ((fl - el) * (el - f0) <= 0.0) el = f0 - d0;
+      if ((fr - er) * (er - f0) <= 0.0) er = f0 + d0;
+      ppr_parabola(f0, el, er, l);
+      if (fabs(l[2]) > fabs(l[1]) * 0.5) {
+        const double turn = -0.5 * l[1] / l[2];
+        if (turn >= -1.0 && turn <= 0.0) {
+          er = 3.0 * f0 - 2.0 * el;
+          ppr_parabola(f0, el, er, l);
+        } else if (turn > 0.0 && turn <= 1.0) {
+          el = 3.0 * f0 - 2.0 * er;
+          ppr_parabola(f0, el, er, l);
+        }
+      }
+    }
+    for (int p = 0; p < 3; ++p) fh[i][p] = 0.0 * u[p] + 1.0 * l[p];
+  }
+}
+
+/* add b to the compensated sum (s, e) (imap1d's Kahan-Babuska-Neumaier loop) */
+static void ppr_kbn(double* s, double* e, double b) {
+  const double t = *s + b;
+  if (fabs(*s) >= fabs(b))
+    *e = *e + ((*s - t) + b);
+  else
+    *e = *e + ((b - t) + *s);
+  *s = t;
+}
+
+static void remap_ppr(int N, double* vn, const double* zn, const double* vo, const double* zo) {
+  const int npos = N + 1;
+  const double* x = zo - 1;   /* x[1..npos] */
+  const double* xn = zn - 1;
+  const double* f = vo - 1;   /* f[1..N] */
+  double* dx = (double*)calloc(npos + 1, sizeof(double));
+  double* edge = (double*)calloc(npos + 1, sizeof(double));
+  double (*fh)[3] = (double (*)[3])calloc(npos + 1, sizeof(double[3]));
+  double* fnew = (double*)calloc(npos + 1, sizeof(double));
+  /* rmap1d: uniform spacing to 1e-14 keeps one spacing */
+  const double same = (x[npos] - x[1]) / (npos - 1), xtol = same * 1.0e-14;
+  int uniform = 1;
+  for (int i = 1; i <= npos - 1; ++i) {
+    dx[i] = x[i + 1] - x[i];
+    if (fabs(dx[i] - same) > xtol) uniform = 0;
+  }
+  const int nd = uniform ? 1 : N;
+  /* rcon1d: dmin = 1e-14 times the mean spacing */
+  double dmid = dx[1];
+  if (nd > 1) {
+    for (int i = 2; i <= npos - 1; ++i) dmid = dmid + dx[i];
+    dmid = dmid / (npos - 1);
+  }
+  const double dmin = 1.0e-14 * dmid;
+  if (N >= 2) ppr_edges(N, dx, nd, f, dmin, edge);
+  ppr_cells(N, dx, nd, f, edge, fh);
+  /* imap1d with three degrees of freedom */
+  int kmin = 1, kmax = 1, p0 = 1, p1 = 1;
+  for (int k = 1; k <= N; ++k) {
+    if (p1 < 1) p1 = 1;
+    for (p0 = p1; p0 <= npos - 1; ++p0)
+      if (x[p0 + 1] > xn[k]) break;
+    for (p1 = p0; p1 <= npos - 1; ++p1)
+      if (x[p1] >= xn[k + 1]) break;
+    p1 = p1 - 1;
+    const double khat = fmax(xn[k + 1] - xn[k], xtol);
+    double acc = 0.0;
+    for (int i = p0; i <= p1; ++i) {
+      const double lo = fmax(x[i], xn[k]), hi = fmin(x[i + 1], xn[k + 1]);
+      const double mid = x[i + 1] * 0.5 + x[i] * 0.5, half = x[i + 1] * 0.5 - x[i] * 0.5;
+      const double sl = (lo - mid) / fmax(half, xtol), sh = (hi - mid) / fmax(half, xtol);
+      const double w0 = sh / 1.0 - sl / 1.0, w1 = ppr_p2(sh) / 2.0 - ppr_p2(sl) / 2.0;
+      const double w2 = ppr_p3(sh) / 3.0 - ppr_p3(sl) / 3.0;
+      double in = w0 * fh[i][0] + w1 * fh[i][1] + w2 * fh[i][2];
+      in = in * half;
+      acc = acc + in;
+    }
+    fnew[k] = acc / khat;
+    if (fnew[k] > fnew[kmax])
+      kmax = k;
+    else if (fnew[k] < fnew[kmin])
+      kmin = k;
+  }
+  double so = 0.0, eo = 0.0, sn = 0.0, en = 0.0;
+  for (int i = 1; i <= N; ++i) ppr_kbn(&so, &eo, (x[i + 1] - x[i]) * f[i]);
+  so = so + eo;
+  for (int i = 1; i <= N; ++i) ppr_kbn(&sn, &en, (xn[i + 1] - xn[i]) * fnew[i]);
+  sn = sn + en;
+  const double err = so - sn;
+  if (err > 0.0)
+    fnew[kmin] = fnew[kmin] + err / (xn[kmin + 1] - xn[kmin]);
+  else if (err < 0.0)
+    fnew[kmax] = fnew[kmax] + err / (xn[kmax + 1] - xn[kmax]);
+  for (int k = 1; k <= N; ++k) vn[k - 1] = fnew[k];
+  free(dx); free(edge); free(fh); free(fnew);
+}
+
 void wto_remap_column(int scheme, int N, double* vn, const double* zn, const double* vo, const double* zo) {
   if (scheme == 0)
     remap0(N, vn, zn, vo, zo);
   else if (scheme == 1)
     remap1(N, vn, zn, vo, zo);
-  else
+  else if (scheme == 2)
     remap2ppm(N, vn, zn, vo, zo);
+  else
+    remap_ppr(N, vn, zn, vo, zo);
 }
 
 /* find_coordinates_incompressible of column c from the saved masses */

@@ -238,6 +238,8 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
     extra = f", every={every}" if every > 1 else ""
     if remap:
         extra += f", vremap='{remap[0]}', coords='{remap[1]}'"
+        if len(remap) > 2:  # a piecewise-constant remap onto uniform coordinates first
+            extra += ", uniform_first=.true."
     if diff:
         fd = lambda x: f"{x:.17e}".replace("e", "d")  # a Fortran double literal
         extra += (f", lsclr={diff['lsclr']}, ldivu={diff['ldivu']}, lrotu={diff['lrotu']}, "
@@ -295,12 +297,16 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
         data[f"temp{s}"] = field(f"s{s:02d}_temp", 1, False)
         data[f"velo{s}"] = field(f"s{s:02d}_velo", 3, False)
     if remap:
+        if len(remap) > 2:  # the input of the remap is the uniformly layered state
+            data[f"mass{nsteps}"] = field("p0_mass", 1, False)
+            data[f"temp{nsteps}"] = field("p0_temp", 1, False)
+            data[f"velo{nsteps}"] = field("p0_velo", 3, False)
         data["remapped_mass"] = field("r0_mass", 1, False)
         data["remapped_temp"] = field("r0_temp", 1, False)
         data["remapped_velo"] = field("r0_velo", 3, False)
         k = np.arange(L + 1)
         b = (1.0 + np.cos(k / L * np.pi)) / 2.0 if remap[1] == "chebyshev" else 1.0 - k / L
-        data.update(remap_scheme={"0": 0, "1": 1, "2PPM": 2}[remap[0]], b_vert=b, a_vert=1.0 - b)
+        data.update(remap_scheme={"0": 0, "1": 1, "2PPM": 2, "PPR": 3}[remap[0]], b_vert=b, a_vert=1.0 - b)
         keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
                 "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "remapped_mass",
                 "remapped_temp", "remapped_velo", "remap_scheme", "a_vert", "b_vert")
@@ -667,9 +673,13 @@ def main():
     if "diffusion" in what or "trisk" in what:
         for c in DIFF_CASES:
             trisk_fixture(*c)
-    if "remap" in what:  # remap_vertical_coordinates (schemes 0 and 1) after two RK4 steps of the J4 ocean
-        for sch, crd in (("1", "chebyshev"), ("0", "uniform"), ("2PPM", "chebyshev")):
+    if "remap" in what:  # remap_vertical_coordinates after one RK4 step of the J4 ocean
+        for sch, crd in (("1", "chebyshev"), ("0", "uniform"), ("2PPM", "chebyshev"), ("PPR", "chebyshev")):
             trisk_fixture(f"ocean_remap{sch.lower()}_J4", 4, "ocean", 6, 1, 1200.0, remap=(sch, crd))
+        # PPR from uniformly spaced old layers (the uniform-spacing branch) and with three layers (two-point
+        # edge values)
+        trisk_fixture("ocean_remapppr_uniform_J4", 4, "ocean", 6, 1, 1200.0, remap=("PPR", "chebyshev", "uniform"))
+        trisk_fixture("ocean_remapppr_l3_J4", 4, "ocean", 3, 1, 1200.0, remap=("PPR", "chebyshev"))
     if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
         trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:

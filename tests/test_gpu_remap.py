@@ -1,10 +1,10 @@
 """Vertical remap of the Lagrangian layers on the GPU (wtg_remap; needs a GPU).
 
 Bars: against the reference's remap_vertical_coordinates (fixtures
-trisk_ocean_remap{1,0,2ppm}_J4: remap_type "1" and "2PPM" onto Chebyshev and
-"0" onto uniform hybrid coordinates after one RK4 step of the J4 ocean, 6
-layers) within 1e-15
-relative; against the oracle on a J7 ocean with 20 layers within 1e-15 (same
+trisk_ocean_remap{1,0,2ppm,ppr}_J4: remap_type "1", "2PPM" and "PPR" onto
+Chebyshev and "0" onto uniform hybrid coordinates after one RK4 step of the
+J4 ocean, 6 layers; PPR also from evenly spaced old layers and with three
+layers) within 1e-15 relative; against the oracle on a J7 ocean with 20 layers within 1e-15 (same
 evaluation order, no FMA contraction: observed bit for bit); every member of
 an ensemble equal to its own single-member run bit for bit; at J8 with 30
 layers the column mass and the column buoyancy content are conserved to
@@ -27,7 +27,8 @@ def _coords(L, kind):
     return 1.0 - b, b
 
 
-@pytest.mark.parametrize("case", ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4"])
+@pytest.mark.parametrize("case", ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4", "ocean_remapppr_J4",
+                                  "ocean_remapppr_uniform_J4", "ocean_remapppr_l3_J4"])
 def test_remap_matches_reference(case):
     print(f"remap {case}", flush=True)
     f = golden(f"trisk_{case}.npz")
@@ -48,9 +49,10 @@ def test_remap_matches_reference(case):
         assert maxrel(gu[k, :, oe], f["remapped_velo"][k, :, oe]) <= 1e-15
 
 
-# the synthetic ocean's layers are uniform: both schemes remap onto Chebyshev coordinates
+# the synthetic ocean's layers are uniform (PPR's uniform-spacing branch): every scheme remaps onto Chebyshev
+# coordinates
 @pytest.mark.parametrize("scheme,kind", [(W.REMAP_PIECEWISE_LINEAR, "chebyshev"), (W.REMAP_PIECEWISE_CONSTANT, "chebyshev"),
-                                         (W.REMAP_PARABOLIC_PPM, "chebyshev")])
+                                         (W.REMAP_PARABOLIC_PPM, "chebyshev"), (W.REMAP_PPR, "chebyshev")])
 def test_remap_j7_l20_equals_oracle(scheme, kind):
     print(f"remap J7 L20 scheme {scheme} onto {kind} coordinates", flush=True)
     from oracle.oracle import Oracle, geometry_of
@@ -75,8 +77,9 @@ def test_remap_j7_l20_equals_oracle(scheme, kind):
     assert maxrel(got[0][:, oc], m[:, oc]) > 1e-3  # not a no-op
 
 
-def test_remap_ensemble_members_equal_single_runs():
-    print("remap ensemble of 2, J5 L6", flush=True)
+@pytest.mark.parametrize("scheme", [W.REMAP_PIECEWISE_LINEAR, W.REMAP_PPR])
+def test_remap_ensemble_members_equal_single_runs(scheme):
+    print(f"remap ensemble of 2, J5 L6, scheme {scheme}", flush=True)
     g = W.Grid(level=5, min_level=5, domain_level=2)
     L = 6
     draws = [cases.ocean(g, L, seed=30 + k) for k in range(2)]
@@ -84,22 +87,23 @@ def test_remap_ensemble_members_equal_single_runs():
     ens = W.Context(g, nlayers=L, members=2)
     ens.set_topography(np.stack([d[0] for d in draws]))
     ens.upload_state(0, *(np.concatenate([d[v] for d in draws]) for v in (1, 2, 3)))
-    ens.remap(0, W.REMAP_PIECEWISE_LINEAR, a, b)
+    ens.remap(0, scheme, a, b)
     em, et, eu = ens.download_state(0)
     ens.close()
     for k, (topo, m, t, u) in enumerate(draws):
         c = W.Context(g, nlayers=L)
         c.set_topography(topo)
         c.upload_state(0, m, t, u)
-        c.remap(0, W.REMAP_PIECEWISE_LINEAR, a, b)
+        c.remap(0, scheme, a, b)
         sm, st, su = c.download_state(0)
         c.close()
         sl = slice(k * L, (k + 1) * L)
         assert np.array_equal(em[sl], sm) and np.array_equal(et[sl], st) and np.array_equal(eu[sl], su)
 
 
-def test_remap_j8_l30_conserves_columns():
-    print("remap J8 L30: column mass and buoyancy content", flush=True)
+@pytest.mark.parametrize("scheme", [W.REMAP_PIECEWISE_LINEAR, W.REMAP_PPR])
+def test_remap_j8_l30_conserves_columns(scheme):
+    print(f"remap J8 L30 scheme {scheme}: column mass and buoyancy content", flush=True)
     g = W.Grid(level=8, min_level=8, domain_level=2)
     L = 30
     topo, m, t, u = cases.ocean(g, L, seed=8)
@@ -107,7 +111,7 @@ def test_remap_j8_l30_conserves_columns():
     ctx = W.Context(g, nlayers=L)
     ctx.set_topography(topo)
     ctx.upload_state(0, m, t, u)
-    ctx.remap(0, W.REMAP_PIECEWISE_LINEAR, a, b)
+    ctx.remap(0, scheme, a, b)
     gm, gt, gu = ctx.download_state(0)
     ctx.close()
     oc = g.layout.owned_cells()
@@ -127,7 +131,7 @@ def test_remap_refusals():
     c.upload_state(0, m, t, u)
     a, b = _coords(3, "uniform")
     with pytest.raises(W.WtgError):
-        c.remap(0, 3, a, b)  # the reference's remap_type "2S", "2W", "4" and "PPR" are not built
+        c.remap(0, 4, a, b)  # the reference's remap_type "2S", "2W" and "4" are not built
     with pytest.raises(ValueError):
         c.remap(0, W.REMAP_PIECEWISE_LINEAR, a[:-1], b[:-1])
     c.close()

@@ -3,8 +3,10 @@
 // remap.f90:147-216) with the reference's remap_type "0" (remap0,
 // piecewise constant, remap.f90:259-285), "1" (remap1, piecewise linear
 // with harmonic-mean slopes and ten minmod reconciliation sweeps,
-// remap.f90:287-380) or "2PPM" (remap2PPM, Colella-Woodward piecewise
-// parabolic with limited slopes, remap.f90:382-495).
+// remap.f90:287-380), "2PPM" (remap2PPM, Colella-Woodward piecewise
+// parabolic with limited slopes, remap.f90:382-495) or "PPR" (remapPPR, the
+// reference's default: Engwirda's PPR library at order 2 with the mono
+// limiter and loose end conditions, remap.f90:950-1010).
 //
 // One thread per column (owned node or pole master; grid y = ensemble
 // member): the old interface heights z_old(k) = z_s + sum of mass / rho0 and
@@ -37,8 +39,9 @@ struct RemapArgs {
   double rho0;
 };
 
+// remapPPR indexes edges 1..N+1 in these arrays (hence RMAX + 2)
 struct Work {
-  double FC[RMAX + 1], aL[RMAX + 1], aR[RMAX + 1], Hz[RMAX + 1], CF[RMAX + 1], FC1[RMAX + 1];
+  double FC[RMAX + 2], aL[RMAX + 2], aR[RMAX + 2], Hz[RMAX + 2], CF[RMAX + 2], FC1[RMAX + 2];
 };
 
 __device__ void remap0(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
@@ -148,14 +151,303 @@ __device__ void remap2ppm(int N, double* vn, const double* zn, const double* vo,
 #undef HZ
 }
 
+// ---- remapPPR (remap.f90:950-1010): Engwirda's PPR library at order 2 ----
+// cubic edge values (ppr/p3e.f90; p1e.f90 for at most three cells) with loose
+// one-sided quadratic end fits (pbc.f90 lbc/ubc), monotone parabolas in the
+// cells (ppm.f90 ppm/ppmfn with plm.f90 plsv/plsc slopes), exact integration of
+// the cell parabolas over the new cells and the compensated conservation fix
+// (rmap1d.f90 rmap1d/imap1d).  The macros index as the library does: edges
+// 1..N+1, cells 1..N.
+
+// the reference's compiled x**k: products from the left
+__device__ __forceinline__ double pw2(double x) { return x * x; }
+__device__ __forceinline__ double pw3(double x) { return (x * x) * x; }
+__device__ __forceinline__ double pw4(double x) { return ((x * x) * x) * x; }
+
+// slv_3x3 (inv.f90:64-124, 400-445): adjugate over determinant, in place
+__device__ bool solve3(const double A[3][3], double v[3], double eps) {
+  const double a2233 = A[1][1] * A[2][2], a2332 = A[1][2] * A[2][1], a2133 = A[1][0] * A[2][2];
+  const double a2331 = A[1][2] * A[2][0], a2132 = A[1][0] * A[2][1], a2231 = A[1][1] * A[2][0];
+  const double det = A[0][0] * (a2233 - a2332) - A[0][1] * (a2133 - a2331) + A[0][2] * (a2132 - a2231);
+  if (!(fabs(det) > eps)) return false;
+  const double a1233 = A[0][1] * A[2][2], a1332 = A[0][2] * A[2][1], a1223 = A[0][1] * A[1][2];
+  const double a1322 = A[0][2] * A[1][1], a1133 = A[0][0] * A[2][2], a1331 = A[0][2] * A[2][0];
+  const double a1123 = A[0][0] * A[1][2], a1321 = A[0][2] * A[1][0], a1132 = A[0][0] * A[2][1];
+  const double a1231 = A[0][1] * A[2][0], a1122 = A[0][0] * A[1][1], a1221 = A[0][1] * A[1][0];
+  const double t0 = ((a2233 - a2332) * v[0] + -(a1233 - a1332) * v[1] + (a1223 - a1322) * v[2]) / det;
+  const double t1 = (-(a2133 - a2331) * v[0] + (a1133 - a1331) * v[1] + -(a1123 - a1321) * v[2]) / det;
+  const double t2 = ((a2132 - a2231) * v[0] + -(a1132 - a1231) * v[1] + (a1122 - a1221) * v[2]) / det;
+  v[0] = t0; v[1] = t1; v[2] = t2;
+  return true;
+}
+
+// slv_4x4 (inv.f90:447-600): block LDU through 2 x 2 adjugates (the second
+// singularity test repeats the first, as in the library)
+__device__ bool solve4(const double A[4][4], double v[4], double eps) {
+  const double adet = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  if (!(fabs(adet) > eps)) return false;
+  const double i00 = A[1][1], i01 = -A[0][1], i10 = -A[1][0], i11 = A[0][0];
+  // L = A21 adj(A11), U = adj(A11) A12, S = A22 - L A12 / adet (mul_2x2 with
+  // scal +1, +1 and -1/adet)
+  double L00 = 0.0, L01 = 0.0, L10 = 0.0, L11 = 0.0, U00 = 0.0, U01 = 0.0, U10 = 0.0, U11 = 0.0;
+  L00 = L00 + (A[2][0] * i00 + A[2][1] * i10);
+  L10 = L10 + (A[3][0] * i00 + A[3][1] * i10);
+  L01 = L01 + (A[2][0] * i01 + A[2][1] * i11);
+  L11 = L11 + (A[3][0] * i01 + A[3][1] * i11);
+  U00 = U00 + (i00 * A[0][2] + i01 * A[1][2]);
+  U10 = U10 + (i10 * A[0][2] + i11 * A[1][2]);
+  U01 = U01 + (i00 * A[0][3] + i01 * A[1][3]);
+  U11 = U11 + (i10 * A[0][3] + i11 * A[1][3]);
+  const double sc = -1.0 / adet;
+  double S[2][2] = {{A[2][2], A[2][3]}, {A[3][2], A[3][3]}};
+  const double p00 = L00 * A[0][2] + L01 * A[1][2], p10 = L10 * A[0][2] + L11 * A[1][2];
+  const double p01 = L00 * A[0][3] + L01 * A[1][3], p11 = L10 * A[0][3] + L11 * A[1][3];
+  if (sc == 1.0) {
+    S[0][0] = S[0][0] + p00; S[1][0] = S[1][0] + p10; S[0][1] = S[0][1] + p01; S[1][1] = S[1][1] + p11;
+  } else if (sc == -1.0) {
+    S[0][0] = S[0][0] - p00; S[1][0] = S[1][0] - p10; S[0][1] = S[0][1] - p01; S[1][1] = S[1][1] - p11;
+  } else {
+    S[0][0] = S[0][0] + sc * p00; S[1][0] = S[1][0] + sc * p10;
+    S[0][1] = S[0][1] + sc * p01; S[1][1] = S[1][1] + sc * p11;
+  }
+  const double sdet = S[0][0] * S[1][1] - S[0][1] * S[1][0];
+  v[2] = v[2] - (L00 * v[0] + L01 * v[1]) / adet;
+  v[3] = v[3] - (L10 * v[0] + L11 * v[1]) / adet;
+  const double t0 = (i00 * v[0] + i01 * v[1]) / adet, t1 = (i10 * v[0] + i11 * v[1]) / adet;
+  v[0] = t0; v[1] = t1;
+  const double t2 = (S[1][1] * v[2] + -S[0][1] * v[3]) / sdet, t3 = (-S[1][0] * v[2] + S[0][0] * v[3]) / sdet;
+  v[2] = t2; v[3] = t3;
+  v[0] = v[0] - (U00 * v[2] + U01 * v[3]) / adet;
+  v[1] = v[1] - (U10 * v[2] + U11 * v[3]) / adet;
+  return true;
+}
+
+#define PX(i) zo[(i) - 1]
+#define PXN(i) zn[(i) - 1]
+#define PF(i) vo[(i) - 1]
+#define PDX(i) w.Hz[i]
+#define PE(i) w.FC[i]
+
+// loose end fit: the quadratic with the means of cells c-1, c, c+1, valued
+// at edges c-1..c+1 (low end) or c..c+2 (high end)
+__device__ void end_fit(bool low, int c, bool uni, const double* vo, double dmin, Work& w) {
+  double xhat, h0, h1, h2;
+  if (!uni) {
+    xhat = fmax(PDX(c), dmin) * 0.5;
+    h0 = PDX(c - 1); h1 = PDX(c); h2 = PDX(c + 1);
+  } else {
+    xhat = fmax(PDX(1), dmin) * 0.5;
+    h0 = h1 = h2 = PDX(1);
+  }
+  const double s[4] = {-(h0 + h1 * 0.5) / xhat, -1.0, 1.0, (h2 + h1 * 0.5) / xhat};
+  double A[3][3], v[3];
+  const double h[3] = {h0, h1, h2};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    A[r][0] = s[r + 1] / 1.0 - s[r] / 1.0;
+    A[r][1] = pw2(s[r + 1]) / 2.0 - pw2(s[r]) / 2.0;
+    A[r][2] = pw3(s[r + 1]) / 3.0 - pw3(s[r]) / 3.0;
+    v[r] = h[r] * PF(c - 1 + r) / xhat;
+  }
+  const int e0 = low ? c - 1 : c, o = low ? 0 : 1;
+  if (solve3(A, v, 1.0e-14 * dmin)) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double x = s[k + o];
+      PE(e0 + k) = 1.0 * v[0] + (x * 1.0) * v[1] + (pw2(x) * 1.0) * v[2];
+    }
+  } else if (low) {
+    PE(c - 1) = PF(c - 1) * 1.0;
+    PE(c) = PF(c - 1) * 0.5 + PF(c) * 0.5;
+    PE(c + 1) = PF(c) * 0.5 + PF(c + 1) * 0.5;
+  } else {
+    PE(c) = PF(c - 1) * 0.5 + PF(c) * 0.5;
+    PE(c + 1) = PF(c) * 0.5 + PF(c + 1) * 0.5;
+    PE(c + 2) = PF(c + 1) * 1.0;
+  }
+}
+
+__device__ __forceinline__ void parabola(double f0, double el, double er, double& c0, double& c1, double& c2) {
+  c0 = 1.5 * f0 - 0.25 * (er + el);
+  c1 = 0.5 * (er - el);
+  c2 = -(1.5 * f0) + 0.75 * (er + el);
+}
+
+__device__ void remap_ppr(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
+  const int np = N + 1;
+  // rmap1d: evenly spaced to 1e-14 keeps one spacing
+  const double same = (PX(np) - PX(1)) / (np - 1), xtol = same * 1.0e-14;
+  bool uni = true;
+  for (int i = 1; i <= np - 1; ++i) {
+    PDX(i) = PX(i + 1) - PX(i);
+    if (fabs(PDX(i) - same) > xtol) uni = false;
+  }
+  // rcon1d: dmin = 1e-14 of the mean spacing
+  double dmid = PDX(1);
+  if (!uni) {
+    for (int i = 2; i <= np - 1; ++i) dmid = dmid + PDX(i);
+    dmid = dmid / (np - 1);
+  }
+  const double dmin = 1.0e-14 * dmid;
+  // edge values
+  if (N >= 2 && np <= 4) {
+    for (int i = 2; i <= np - 1; ++i) {
+      if (uni) {
+        const double d = PDX(1) * 2.0;
+        PE(i) = (PDX(1) * PF(i - 1) + PDX(1) * PF(i)) / d;
+      } else {
+        const double hl = fmax(PDX(i - 1), dmin), h0 = fmax(PDX(i), dmin);
+        PE(i) = (h0 * PF(i - 1) + hl * PF(i)) / (hl + h0);
+      }
+    }
+    PE(1) = PF(1);
+    PE(np) = PF(np - 1);
+  } else if (N >= 2) {
+    end_fit(true, 2, uni, vo, dmin, w);
+    end_fit(false, np - 2, uni, vo, dmin, w);
+    for (int i = 3; i <= np - 2; ++i) {
+      if (uni) {
+        PE(i) = (-(1.0 * PF(i - 2)) + 7.0 * PF(i - 1) + 7.0 * PF(i) - 1.0 * PF(i + 1)) / 12.0;
+        continue;
+      }
+      const double h[4] = {PDX(i - 2), PDX(i - 1), PDX(i), PDX(i + 1)};
+      const double xhat = 0.5 * fmax(h[1], dmin) + 0.5 * fmax(h[2], dmin);
+      const double s[5] = {-(h[0] + h[1]) / xhat, -h[1] / xhat, 0.0, h[2] / xhat, (h[2] + h[3]) / xhat};
+      double iv[5][4];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        iv[k][0] = s[k] / 1.0;
+        iv[k][1] = pw2(s[k]) / 2.0;
+        iv[k][2] = pw3(s[k]) / 3.0;
+        iv[k][3] = pw4(s[k]) / 4.0;
+      }
+      double A[4][4], v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) A[r][p] = iv[r + 1][p] - iv[r][p];
+        v[r] = h[r] * PF(i - 2 + r) / xhat;
+      }
+      PE(i) = solve4(A, v, 1.0e-14 * dmin) ? v[0] : PF(i - 1) * 0.5 + PF(i) * 0.5;
+    }
+  }
+  // monotone parabolas (coefficients of 1, s, s^2 on s in [-1, 1]) in aL, aR, CF
+  for (int i = 1; i <= N; ++i) {
+    if (N == 1) {
+      w.aL[1] = PF(1); w.aR[1] = 0.0; w.CF[1] = 0.0;
+      break;
+    }
+    const int il = i - 1 > 1 ? i - 1 : 1, ir = i + 1 < N ? i + 1 : N;
+    const double f0 = PF(i), fl = PF(il), fr = PF(ir);
+    double el = PE(i), er = PE(i + 1);
+    // plsv / plsc: the limited centred slope
+    double d0 = 0.0;
+    {
+      const double dm = f0 - fl, dp = fr - f0;
+      if (dm * dp > 0.0) {
+        double sl, sr;
+        if (uni) {
+          sl = (fl + f0) * 0.5;
+          sr = (f0 + fr) * 0.5;
+        } else {
+          const double hl = PDX(il), h0 = PDX(i), hr = PDX(ir);
+          sl = (h0 * fl + hl * f0) / (hl + h0);
+          sr = (hr * f0 + h0 * fr) / (h0 + hr);
+        }
+        d0 = 0.5 * (sr - sl);
+        double scal = fmin(fabs(dm), fabs(dp)) / fmax(fabs(d0), 1.0e-14);
+        scal = fmin(scal, 1.0);
+        d0 = scal * d0;
+      }
+    }
+    double u0, u1, u2, l0, l1, l2;
+    parabola(f0, el, er, u0, u1, u2);
+    if ((fr - f0) * (f0 - fl) < 0.0) {
+      l0 = f0; l1 = 0.0; l2 = 0.0;
+    } else {
+      if ((fl - el) * (el - f0) <= 0.0) el = f0 - d0;
+      if ((fr - er) * (er - f0) <= 0.0) er = f0 + d0;
+      parabola(f0, el, er, l0, l1, l2);
+      if (fabs(l2) > fabs(l1) * 0.5) {
+        const double turn = -0.5 * l1 / l2;
+        if (turn >= -1.0 && turn <= 0.0) {
+          er = 3.0 * f0 - 2.0 * el;
+          parabola(f0, el, er, l0, l1, l2);
+        } else if (turn > 0.0 && turn <= 1.0) {
+          el = 3.0 * f0 - 2.0 * er;
+          parabola(f0, el, er, l0, l1, l2);
+        }
+      }
+    }
+    w.aL[i] = 0.0 * u0 + 1.0 * l0;
+    w.aR[i] = 0.0 * u1 + 1.0 * l1;
+    w.CF[i] = 0.0 * u2 + 1.0 * l2;
+  }
+  // imap1d: integrate the parabolas over the new cells
+  int kmin = 1, kmax = 1, p0 = 1, p1 = 1;
+  double* fnew = w.FC1;
+  for (int k = 1; k <= N; ++k) {
+    if (p1 < 1) p1 = 1;
+    for (p0 = p1; p0 <= np - 1; ++p0)
+      if (PX(p0 + 1) > PXN(k)) break;
+    for (p1 = p0; p1 <= np - 1; ++p1)
+      if (PX(p1) >= PXN(k + 1)) break;
+    p1 = p1 - 1;
+    const double khat = fmax(PXN(k + 1) - PXN(k), xtol);
+    double acc = 0.0;
+    for (int i = p0; i <= p1; ++i) {
+      const double lo = fmax(PX(i), PXN(k)), hi = fmin(PX(i + 1), PXN(k + 1));
+      const double mid = PX(i + 1) * 0.5 + PX(i) * 0.5, half = PX(i + 1) * 0.5 - PX(i) * 0.5;
+      const double sl = (lo - mid) / fmax(half, xtol), sh = (hi - mid) / fmax(half, xtol);
+      const double w0 = sh / 1.0 - sl / 1.0, w1 = pw2(sh) / 2.0 - pw2(sl) / 2.0, w2 = pw3(sh) / 3.0 - pw3(sl) / 3.0;
+      double in = w0 * w.aL[i] + w1 * w.aR[i] + w2 * w.CF[i];
+      in = in * half;
+      acc = acc + in;
+    }
+    fnew[k] = acc / khat;
+    if (fnew[k] > fnew[kmax])
+      kmax = k;
+    else if (fnew[k] < fnew[kmin])
+      kmin = k;
+  }
+  // Kahan-Babuska-Neumaier sums of the old and new column contents; the
+  // difference goes to the smallest (deficit) or largest (excess) new cell
+  double so = 0.0, eo = 0.0, sn = 0.0, en = 0.0;
+  for (int i = 1; i <= N; ++i) {
+    const double b = (PX(i + 1) - PX(i)) * PF(i), t = so + b;
+    eo = fabs(so) >= fabs(b) ? eo + ((so - t) + b) : eo + ((b - t) + so);
+    so = t;
+  }
+  so = so + eo;
+  for (int i = 1; i <= N; ++i) {
+    const double b = (PXN(i + 1) - PXN(i)) * fnew[i], t = sn + b;
+    en = fabs(sn) >= fabs(b) ? en + ((sn - t) + b) : en + ((b - t) + sn);
+    sn = t;
+  }
+  sn = sn + en;
+  const double err = so - sn;
+  if (err > 0.0)
+    fnew[kmin] = fnew[kmin] + err / (PXN(kmin + 1) - PXN(kmin));
+  else if (err < 0.0)
+    fnew[kmax] = fnew[kmax] + err / (PXN(kmax + 1) - PXN(kmax));
+  for (int k = 1; k <= N; ++k) vn[k - 1] = fnew[k];
+}
+#undef PX
+#undef PXN
+#undef PF
+#undef PDX
+#undef PE
+
 __device__ __forceinline__ void remap_col(int scheme, int N, double* vn, const double* zn, const double* vo,
                                           const double* zo, Work& w) {
   if (scheme == 0)
     remap0(N, vn, zn, vo, zo, w);
   else if (scheme == 1)
     remap1(N, vn, zn, vo, zo, w);
-  else
+  else if (scheme == 2)
     remap2ppm(N, vn, zn, vo, zo, w);
+  else
+    remap_ppr(N, vn, zn, vo, zo, w);
 }
 
 // find_coordinates_incompressible (remap.f90:235-257) of column c
@@ -221,8 +513,9 @@ extern "C" int wtg_remap(wtg_ctx* c, int slot, int scheme, const double* a_vert,
   if (!c || !a_vert || !b_vert) return WTG_E_ARG;
   if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
   if (scheme != WTG_REMAP_PIECEWISE_CONSTANT && scheme != WTG_REMAP_PIECEWISE_LINEAR &&
-      scheme != WTG_REMAP_PARABOLIC_PPM)
-    return wtg_fail(c, WTG_E_ARG, "remap scheme %d is not built (0: remap0, 1: remap1, 2: remap2PPM)", scheme);
+      scheme != WTG_REMAP_PARABOLIC_PPM && scheme != WTG_REMAP_PPR)
+    return wtg_fail(c, WTG_E_ARG, "remap scheme %d is not built (0: remap0, 1: remap1, 2: remap2PPM, 3: remapPPR)",
+                    scheme);
   const int L = c->params.nlayers, M = c->nmembers;
   if (L > RMAX) return wtg_fail(c, WTG_E_ARG, "the remap takes at most %d layers, not %d", RMAX, L);
   if (!c->topo_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_topography must be called first");
