@@ -232,16 +232,21 @@ int wtg_set_diffusion(wtg_ctx* ctx, const wtg_diffusion* d);
  * the buoyancy and the edge velocities (not at the poles) remapped with the
  * reference's remap_type "0" (WTG_REMAP_PIECEWISE_CONSTANT, remap0), "1"
  * (WTG_REMAP_PIECEWISE_LINEAR, remap1), "2PPM" (WTG_REMAP_PARABOLIC_PPM,
- * remap2PPM) or its default "PPR" (WTG_REMAP_PPR, remapPPR at
- * remap.f90:950-1010: order 2, mono limiter, loose ends), the new mass
- * rho0 dz; the slot is ghost filled before and after.  The reference's
- * "2S", "2W", "4" and its sigma_z coordinates are not built.  At most
- * WTG_REMAP_MAX_LAYERS layers; the first work slot holds the old masses
- * meanwhile. */
+ * remap2PPM), its default "PPR" (WTG_REMAP_PPR, remapPPR at
+ * remap.f90:950-1010: order 2, mono limiter, loose ends), "2S"
+ * (WTG_REMAP_PARABOLIC_SPLINE, remap2S; at least 2 layers), "2W"
+ * (WTG_REMAP_PARABOLIC_WENO, remap2W) or "4" (WTG_REMAP_QUARTIC_WENO,
+ * remap4; at least 3 layers), the new mass rho0 dz; the slot is ghost
+ * filled before and after.  The reference's sigma_z coordinates and its
+ * compressible branch are not built.  At most WTG_REMAP_MAX_LAYERS layers;
+ * the first work slot holds the old masses meanwhile. */
 #define WTG_REMAP_PIECEWISE_CONSTANT 0
 #define WTG_REMAP_PIECEWISE_LINEAR 1
 #define WTG_REMAP_PARABOLIC_PPM 2
 #define WTG_REMAP_PPR 3
+#define WTG_REMAP_PARABOLIC_SPLINE 4
+#define WTG_REMAP_PARABOLIC_WENO 5
+#define WTG_REMAP_QUARTIC_WENO 6
 #define WTG_REMAP_MAX_LAYERS 64
 int wtg_remap(wtg_ctx* ctx, int slot, int scheme, const double* a_vert, const double* b_vert);
 

@@ -13,6 +13,10 @@
  *     weighted buoyancy = rho0 dz theta                 remap.f90:147-216
  *   find_coordinates_incompressible (hybrid a/b coordinates)
  *                                                       remap.f90:235-257
+ *   remap2S (parabolic spline, parabolic continuation)  remap.f90:497-583
+ *   remap2W (parabolic WENO, limited interior)          remap.f90:585-735
+ *   remap4 (parabolic WENO with power-law reconciliation: quartic profiles)
+ *                                                       remap.f90:737-948
  *   remapPPR (Engwirda's PPR library at order 2: p3e cubic edge values with
  *           loose end fits, PPM cells with the mono limiter, integral
  *           remap with the compensated conservation fix)
@@ -169,6 +173,235 @@ static void remap2ppm(int N, double* vn, const double* zn, const double* vo, con
 #undef HZ
   free(Hz); free(aL); free(aR); free(CF); free(FC); free(FC1);
 }
+
+/* The three schemes below use the reference's (1:N) and (0:N) arrays at the
+ * same indices: V(k) is var_old(k); Hz, FC, r, ... hold N + 2 entries (zeroed:
+ * a one-layer column reads entries the reference never sets, which do not
+ * reach its result). */
+#define V(k) vo[(k) - 1]
+
+/* remap2S (remap.f90:497-583): parabolic spline with parabolic continuation
+ * at both ends; at least two layers (one layer is the flux-free update) */
+static void remap2s(int N, double* vn, const double* zn, const double* vo, const double* zo) {
+  double* w = (double*)calloc(6 * (size_t)(N + 2), sizeof(double));
+  double *Hz = w, *dL = w + (N + 2), *dR = w + 2 * (N + 2), *FC = w + 3 * (N + 2), *r = w + 4 * (N + 2);
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  if (N >= 2) {
+    double cff = Hz[1] / Hz[2];
+    FC[1] = 1.0 + cff;
+    r[0] = 2.0 * V(1) + cff * (V(1) + cff * V(2)) / FC[1];
+    for (int k = 1; k <= N - 1; ++k) {
+      cff = 1.0 / (2.0 * Hz[k] + Hz[k + 1] * (2.0 - FC[k]));
+      FC[k + 1] = cff * Hz[k];
+      r[k] = cff * (3.0 * (V(k + 1) * Hz[k] + V(k) * Hz[k + 1]) - Hz[k + 1] * r[k - 1]);
+    }
+    cff = Hz[N] / Hz[N - 1];
+    const double cff1 = 1.0 + cff;
+    r[N] = (cff * (V(N) + cff * V(N - 1)) + cff1 * (2.0 * V(N) - cff1 * r[N - 1])) / (cff1 * (1.0 - cff1 * FC[N]));
+    for (int k = N - 1; k >= 0; --k) r[k] = r[k] - FC[k + 1] * r[k + 1];
+  }
+  for (int k = 1; k <= N; ++k) {
+    dR[k] = 0.5 * (r[k] - r[k - 1]);
+    dL[k] = 0.5 * (r[k] + r[k - 1]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    double alpha, cff, cff1;
+    if (dz > 0.0) {
+      alpha = Hz[k + 1]; cff = dR[k + 1]; cff1 = dL[k + 1];
+    } else {
+      alpha = Hz[k]; cff = dR[k]; cff1 = -dL[k];
+    }
+    alpha = dz / alpha;
+    FC[k] = dz * (r[k] + alpha * (cff - cff1 * (3.0 - 2.0 * fabs(alpha))));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+  free(w);
+}
+
+/* parabolic WENO side limits shared by remap2W and remap4 (remap.f90:605-639,
+ * 760-795): monotonised parabolic aL, aR and their quadratic variations dL,
+ * dR in the inner cells; d holds the centred differences (remap4's FC * dv,
+ * remap2W's FC) */
+static void weno_sides(int N, const double* vo, const double* Hz, const double* d, double* aL, double* aR, double* dL,
+                       double* dR) {
+  for (int k = 2; k <= N - 1; ++k) {
+    double deltaR = Hz[k] * d[k], deltaL = Hz[k] * d[k - 1];
+    if (deltaR * deltaL < 0.0) {
+      deltaR = 0.0;
+      deltaL = 0.0;
+    }
+    double cff = Hz[k - 1] + 2.0 * Hz[k] + Hz[k + 1];
+    const double cffR = cff * d[k], cffL = cff * d[k - 1];
+    if (fabs(deltaR) > fabs(cffL)) deltaR = cffL;
+    if (fabs(deltaL) > fabs(cffR)) deltaL = cffR;
+    cff = (deltaR - deltaL) / (Hz[k - 1] + Hz[k] + Hz[k + 1]);
+    deltaR = deltaR - cff * Hz[k + 1];
+    deltaL = deltaL + cff * Hz[k - 1];
+    aR[k] = V(k) + deltaR;
+    aL[k] = V(k) - deltaL;
+    const double pr = 2.0 * deltaR - deltaL, pl = 2.0 * deltaL - deltaR;
+    dR[k] = pr * pr;
+    dL[k] = pl * pl;
+  }
+}
+
+/* remap2W (remap.f90:585-735) with LIMIT_INTERIOR and no NEUMANN */
+static void remap2w(int N, double* vn, const double* zn, const double* vo, const double* zo) {
+  double* w = (double*)calloc(7 * (size_t)(N + 2), sizeof(double));
+  double *Hz = w, *aL = w + (N + 2), *aR = w + 2 * (N + 2), *dL = w + 3 * (N + 2), *dR = w + 4 * (N + 2);
+  double *FC = w + 5 * (N + 2), *r = w + 6 * (N + 2);
+  const double eps = 1e-8;
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  for (int k = N - 1; k >= 1; --k) FC[k] = (V(k + 1) - V(k)) / (Hz[k + 1] + Hz[k]);
+  weno_sides(N, vo, Hz, FC, aL, aR, dL, dR);
+  aR[N] = V(N); aL[N] = V(N); dR[N] = 0.0; dL[N] = 0.0;
+  aR[1] = V(1); aL[1] = V(1); dR[1] = 0.0; dL[1] = 0.0;
+  for (int k = 1; k <= N - 1; ++k) {
+    const double deltaL = fmax(dL[k], eps), deltaR = fmax(dR[k + 1], eps);
+    r[k] = (deltaR * aR[k] + deltaL * aL[k + 1]) / (deltaR + deltaL);
+  }
+  r[N] = 2.0 * V(N) - r[N - 1];
+  r[0] = 2.0 * V(1) - r[1];
+  for (int k = 1; k <= N; ++k) {
+    double deltaR = r[k] - V(k), deltaL = V(k) - r[k - 1];
+    const double cffR = 2.0 * deltaR, cffL = 2.0 * deltaL;
+    if (deltaR * deltaL < 0.0) {
+      deltaR = 0.0;
+      deltaL = 0.0;
+    } else if (fabs(deltaR) > fabs(cffL)) {
+      deltaR = cffL;
+    } else if (fabs(deltaL) > fabs(cffR)) {
+      deltaL = cffR;
+    }
+    aR[k] = V(k) + deltaR;
+    aL[k] = V(k) - deltaL;
+    dL[k] = 0.5 * (aR[k] - aL[k]);
+    dR[k] = 0.5 * (aR[k] + aL[k]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    double alpha, cff, cffL, cffR;
+    if (dz > 0.0) {
+      alpha = Hz[k + 1]; cff = aL[k + 1]; cffL = dL[k + 1]; cffR = dR[k + 1];
+    } else {
+      alpha = -Hz[k]; cff = aR[k]; cffL = -dL[k]; cffR = dR[k];
+    }
+    alpha = dz / alpha;
+    FC[k] = dz * (cff + alpha * (cffL - cffR * (3.0 - 2.0 * alpha)));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+  free(w);
+}
+
+/* remap4 (remap.f90:737-948): parabolic WENO, then the power-law
+ * reconciliation of interface values and derivatives (quartic profiles),
+ * without NEUMANN; at least three layers (two read an unset value in the
+ * reference).  Its Pade constants 0.0763..., 0.004329... and 0.0363... are
+ * default-kind (single precision) literals in the reference. */
+static void remap4(int N, double* vn, const double* zn, const double* vo, const double* zo) {
+  double* w = (double*)calloc(10 * (size_t)(N + 2), sizeof(double));
+  double *Hz = w, *aL = w + (N + 2), *aR = w + 2 * (N + 2), *dL = w + 3 * (N + 2), *dR = w + 4 * (N + 2);
+  double *d = w + 5 * (N + 2), *FC = w + 6 * (N + 2), *r = w + 7 * (N + 2), *r1 = w + 8 * (N + 2);
+  const double eps = 1e-8;
+  const double c1 = (double)0.0763636363636363636f, c2 = (double)0.004329004329004329f;
+  const double c3 = (double)0.0363636363636363636f;
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  for (int k = N - 1; k >= 1; --k) {
+    FC[k] = 1.0 / (Hz[k + 1] + Hz[k]);
+    d[k] = FC[k] * (V(k + 1) - V(k));
+  }
+  weno_sides(N, vo, Hz, d, aL, aR, dL, dR);
+  {
+    aL[N] = aR[N - 1];
+    aR[N] = 2.0 * V(N) - aL[N];
+    const double pr = 2.0 * aR[N] + aL[N] - 3.0 * V(N), pl = 3.0 * V(N) - 2.0 * aL[N] - aR[N];
+    dR[N] = pr * pr;
+    dL[N] = pl * pl;
+  }
+  if (N >= 2) {
+    aR[1] = aL[2];
+    aL[1] = 2.0 * V(1) - aR[1];
+    const double pr = 2.0 * aR[1] + aL[1] - 3.0 * V(1), pl = 3.0 * V(1) - 2.0 * aL[1] - aR[1];
+    dR[1] = pr * pr;
+    dL[1] = pl * pl;
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double deltaL = fmax(dL[k], eps), deltaR = fmax(dR[k + 1], eps);
+    r1[k] = (deltaR * aR[k] + deltaL * aL[k + 1]) / (deltaR + deltaL);
+  }
+  r1[N] = 2.0 * V(N) - r1[N - 1];
+  r1[0] = 2.0 * V(1) - r1[1];
+  for (int k = 1; k <= N; ++k) {
+    const double deltaR = r1[k] - V(k), deltaL = V(k) - r1[k - 1];
+    double cff = deltaR * deltaL;
+    cff = cff > eps ? (deltaR + deltaL) / cff : 0.0;
+    double cffL = cff * deltaL, cffR = cff * deltaR;
+    if (cffL > 3.0) {
+      cffL = cffL * deltaL;
+      cffR = 0.0;
+    } else if (cffR > 3.0) {
+      cffL = 0.0;
+      cffR = cffR * deltaR;
+    } else {
+      cffL = 4.0 * deltaL - 2.0 * deltaR;
+      cffR = 4.0 * deltaR - 2.0 * deltaL;
+    }
+    cff = 1.0 / Hz[k];
+    dR[k] = cff * cffR;
+    dL[k] = cff * cffL;
+  }
+  for (int k = N - 1; k >= 1; --k) {
+    d[k] = FC[k] * (Hz[k + 1] * dL[k + 1] + Hz[k] * dR[k]);
+    const double cffR = 8.0 * (dR[k] + 2.0 * dL[k]), cffL = 8.0 * (dL[k + 1] + 2.0 * dR[k + 1]);
+    if (fabs(d[k]) > fabs(cffR)) d[k] = cffR;
+    if (fabs(d[k]) > fabs(cffL)) d[k] = cffL;
+    double Hdd, rr;
+    if ((dL[k + 1] - dR[k]) * (V(k + 1) - V(k)) > 0.0) {
+      Hdd = Hz[k] * (d[k] - dR[k]);
+      rr = V(k) - r1[k - 1];
+    } else {
+      Hdd = Hz[k + 1] * (dL[k + 1] - d[k]);
+      rr = r1[k + 1] - V(k + 1);
+    }
+    rr = fabs(rr);
+    double Ampl = 0.2 * Hdd * rr;
+    Hdd = fabs(Hdd);
+    const double cff = rr * rr + c1 * Hdd * (rr + c2 * Hdd);
+    Ampl = cff > eps ? Ampl * (rr + c3 * Hdd) / cff : 0.0;
+    r[k] = r1[k] + Ampl;
+  }
+  r[0] = 2.0 * V(1) - r[1];
+  r[N] = 2.0 * V(N) - r[N - 1];
+  d[0] = d[1];
+  d[N] = d[N - 1];
+  for (int k = 1; k <= N; ++k) {
+    aR[k] = r[k] + r[k - 1] - 2.0 * V(k);
+    dR[k] = Hz[k] * (d[k] + d[k - 1]) - 2.0 * (r[k] - r[k - 1]);
+    dL[k] = Hz[k] * (d[k] - d[k - 1]) - 6.0 * aR[k];
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    double alpha, cff, cffR, cffL;
+    if (dz > 0.0) {
+      alpha = Hz[k + 1]; cff = aR[k + 1]; cffR = dR[k + 1]; cffL = dL[k + 1];
+    } else {
+      alpha = -Hz[k]; cff = aR[k]; cffR = -dR[k]; cffL = dL[k];
+    }
+    alpha = dz / alpha;
+    FC[k] = dz * (r[k] + 0.5 * dz * d[k] +
+                  alpha * alpha * (cff - cffR * (0.5 - 0.25 * alpha) + cffL * (1.0 - alpha * (1.25 - 0.5 * alpha))));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+  free(w);
+}
+#undef V
 
 /* ---- remapPPR (remap.f90:950-1010): Engwirda's PPR library at order 2 ----
  * cubic edge values (p3e.f90) with loose one-sided end fits (pbc.f90 lbc/ubc),
@@ -465,8 +698,14 @@ void wto_remap_column(int scheme, int N, double* vn, const double* zn, const dou
     remap1(N, vn, zn, vo, zo);
   else if (scheme == 2)
     remap2ppm(N, vn, zn, vo, zo);
-  else
+  else if (scheme == 3)
     remap_ppr(N, vn, zn, vo, zo);
+  else if (scheme == 4)
+    remap2s(N, vn, zn, vo, zo);
+  else if (scheme == 5)
+    remap2w(N, vn, zn, vo, zo);
+  else
+    remap4(N, vn, zn, vo, zo);
 }
 
 /* find_coordinates_incompressible of column c from the saved masses */

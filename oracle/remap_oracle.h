@@ -11,7 +11,8 @@ extern "C" {
 /* one column: var_new on the cells of z_new from var_old on the cells of
  * z_old (both [0..N] interface heights, bottom up); scheme 0 = remap0,
  * 1 = remap1, 2 = remap2PPM (remap.f90:259-495), 3 = remapPPR
- * (remap.f90:950-1010, order 2 with the mono limiter and loose ends) */
+ * (remap.f90:950-1010, order 2 with the mono limiter and loose ends),
+ * 4 = remap2S, 5 = remap2W, 6 = remap4 (remap.f90:497-948) */
 void wto_remap_column(int scheme, int N, double* var_new, const double* z_new, const double* var_old,
                       const double* z_old);
 

@@ -306,7 +306,7 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
         data["remapped_velo"] = field("r0_velo", 3, False)
         k = np.arange(L + 1)
         b = (1.0 + np.cos(k / L * np.pi)) / 2.0 if remap[1] == "chebyshev" else 1.0 - k / L
-        data.update(remap_scheme={"0": 0, "1": 1, "2PPM": 2, "PPR": 3}[remap[0]], b_vert=b, a_vert=1.0 - b)
+        data.update(remap_scheme={"0": 0, "1": 1, "2PPM": 2, "PPR": 3, "2S": 4, "2W": 5, "4": 6}[remap[0]], b_vert=b, a_vert=1.0 - b)
         keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
                 "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "remapped_mass",
                 "remapped_temp", "remapped_velo", "remap_scheme", "a_vert", "b_vert")
@@ -674,7 +674,8 @@ def main():
         for c in DIFF_CASES:
             trisk_fixture(*c)
     if "remap" in what:  # remap_vertical_coordinates after one RK4 step of the J4 ocean
-        for sch, crd in (("1", "chebyshev"), ("0", "uniform"), ("2PPM", "chebyshev"), ("PPR", "chebyshev")):
+        for sch, crd in (("1", "chebyshev"), ("0", "uniform"), ("2PPM", "chebyshev"), ("PPR", "chebyshev"),
+                         ("2S", "chebyshev"), ("2W", "chebyshev"), ("4", "chebyshev")):
             trisk_fixture(f"ocean_remap{sch.lower()}_J4", 4, "ocean", 6, 1, 1200.0, remap=(sch, crd))
         # PPR from uniformly spaced old layers (the uniform-spacing branch) and with three layers (two-point
         # edge values)

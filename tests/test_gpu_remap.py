@@ -4,7 +4,7 @@ Bars: against the reference's remap_vertical_coordinates (fixtures
 trisk_ocean_remap{1,0,2ppm,ppr}_J4: remap_type "1", "2PPM" and "PPR" onto
 Chebyshev and "0" onto uniform hybrid coordinates after one RK4 step of the
 J4 ocean, 6 layers; PPR also from evenly spaced old layers and with three
-layers) within 1e-15 relative; against the oracle on a J7 ocean with 20 layers within 1e-15 (same
+layers; "2S", "2W" and "4" onto Chebyshev) within 1e-15 relative; against the oracle on a J7 ocean with 20 layers within 1e-15 (same
 evaluation order, no FMA contraction: observed bit for bit); every member of
 an ensemble equal to its own single-member run bit for bit; at J8 with 30
 layers the column mass and the column buoyancy content are conserved to
@@ -28,7 +28,8 @@ def _coords(L, kind):
 
 
 @pytest.mark.parametrize("case", ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4", "ocean_remapppr_J4",
-                                  "ocean_remapppr_uniform_J4", "ocean_remapppr_l3_J4"])
+                                  "ocean_remapppr_uniform_J4", "ocean_remapppr_l3_J4", "ocean_remap2s_J4",
+                                  "ocean_remap2w_J4", "ocean_remap4_J4"])
 def test_remap_matches_reference(case):
     print(f"remap {case}", flush=True)
     f = golden(f"trisk_{case}.npz")
@@ -52,7 +53,9 @@ def test_remap_matches_reference(case):
 # the synthetic ocean's layers are uniform (PPR's uniform-spacing branch): every scheme remaps onto Chebyshev
 # coordinates
 @pytest.mark.parametrize("scheme,kind", [(W.REMAP_PIECEWISE_LINEAR, "chebyshev"), (W.REMAP_PIECEWISE_CONSTANT, "chebyshev"),
-                                         (W.REMAP_PARABOLIC_PPM, "chebyshev"), (W.REMAP_PPR, "chebyshev")])
+                                         (W.REMAP_PARABOLIC_PPM, "chebyshev"), (W.REMAP_PPR, "chebyshev"),
+                                         (W.REMAP_PARABOLIC_SPLINE, "chebyshev"), (W.REMAP_PARABOLIC_WENO, "chebyshev"),
+                                         (W.REMAP_QUARTIC_WENO, "chebyshev")])
 def test_remap_j7_l20_equals_oracle(scheme, kind):
     print(f"remap J7 L20 scheme {scheme} onto {kind} coordinates", flush=True)
     from oracle.oracle import Oracle, geometry_of
@@ -131,10 +134,19 @@ def test_remap_refusals():
     c.upload_state(0, m, t, u)
     a, b = _coords(3, "uniform")
     with pytest.raises(W.WtgError):
-        c.remap(0, 4, a, b)  # the reference's remap_type "2S", "2W" and "4" are not built
+        c.remap(0, 7, a, b)  # no such remap_type
     with pytest.raises(ValueError):
         c.remap(0, W.REMAP_PIECEWISE_LINEAR, a[:-1], b[:-1])
     c.close()
+    # remap4 reads a third layer, remap2S a second one, that the reference leaves unset below that
+    for L, scheme in ((2, W.REMAP_QUARTIC_WENO), (1, W.REMAP_PARABOLIC_SPLINE)):
+        few = W.Context(g, nlayers=L)
+        t2, m2, tt2, u2 = cases.ocean(g, L, seed=1)
+        few.set_topography(t2)
+        few.upload_state(0, m2, tt2, u2)
+        with pytest.raises(W.WtgError):
+            few.remap(0, scheme, *_coords(L, "uniform"))
+        few.close()
     big = W.Context(g, nlayers=W.REMAP_MAX_LAYERS + 1)
     big.set_topography(topo)
     a, b = _coords(W.REMAP_MAX_LAYERS + 1, "uniform")

@@ -5,10 +5,14 @@ Fixtures tests/golden/trisk_ocean_remap*_J4.npz come from the reference's
 remap_vertical_coordinates (remap.f90:18-75, incompressible branch 147-257)
 run by the harness (oracle/ref_harness/dump_trisk.f90) after one RK4 step of
 the J4 ocean with 6 layers: remap_type "1" (piecewise linear), "2PPM"
-(Colella-Woodward piecewise parabolic) and "PPR" (Engwirda's library at
-order 2, the reference's default) onto Chebyshev hybrid coordinates, "0"
-(piecewise constant) onto uniform ones (b_vert as the test cases'
-initialize_a_b_vert, a_vert = 1 - b_vert).  Two more PPR fixtures take the
+(Colella-Woodward piecewise parabolic), "PPR" (Engwirda's library at
+order 2, the reference's default), "2S" (parabolic spline), "2W" (parabolic
+WENO) and "4" (parabolic WENO with power-law reconciliation) onto Chebyshev
+hybrid coordinates, "0" (piecewise constant) onto uniform ones (b_vert as the
+test cases' initialize_a_b_vert, a_vert = 1 - b_vert).  remap4's three Pade
+constants are single-precision literals in the reference: written as double
+literals, 15,371 buoyancy and 30,520 velocity values of the "4" fixture move
+(up to 5e-14), so the fixture pins that reading.  Two more PPR fixtures take the
 branches the first leaves: "ppr_uniform" remaps old layers that a
 piecewise-constant remap first made evenly spaced (rmap1d's uniform-spacing
 branch: plsc slopes, the closed-form cubic edge values), "ppr_l3" has three
@@ -33,7 +37,7 @@ from helpers import grid_of_fixture, oracle_of
 from oracle.oracle import remap_column
 
 CASES = ["ocean_remap1_J4", "ocean_remap0_J4", "ocean_remap2ppm_J4", "ocean_remapppr_J4", "ocean_remapppr_uniform_J4",
-         "ocean_remapppr_l3_J4"]
+         "ocean_remapppr_l3_J4", "ocean_remap2s_J4", "ocean_remap2w_J4", "ocean_remap4_J4"]
 
 
 def load(case):
@@ -83,7 +87,7 @@ def test_remap_conserves_columns_and_is_idempotent(case):
 @pytest.mark.parametrize("case", CASES)
 def test_planted_errors_are_caught(case):
     f, g, o, st, scheme = load(case)
-    other = o.remap(*st, (scheme + 1) % 4, f["a_vert"], f["b_vert"])
+    other = o.remap(*st, (scheme + 1) % 7, f["a_vert"], f["b_vert"])
     assert max(errors(f, g, other)) > 1e-9
     b = f["b_vert"].copy()
     b[1:-1] += 1e-6  # the inner interfaces moved by 1e-6 of the column depth

@@ -148,7 +148,8 @@ SOLVE_BICGSTAB, SOLVE_JACOBI, SOLVE_FMG, SOLVE_SRJ, SOLVE_VCYCLE, SOLVE_LEARNED 
 # precision of Context.rk4 / rk4_timed: FP64 (bit-exact with the reference's
 # RK4 of trend_ml) or FP32 bulk stencils with FP64 column scan and updates
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
-REMAP_PIECEWISE_CONSTANT, REMAP_PIECEWISE_LINEAR, REMAP_PARABOLIC_PPM, REMAP_PPR, REMAP_MAX_LAYERS = 0, 1, 2, 3, 64
+REMAP_PIECEWISE_CONSTANT, REMAP_PIECEWISE_LINEAR, REMAP_PARABOLIC_PPM, REMAP_PPR = 0, 1, 2, 3
+REMAP_PARABOLIC_SPLINE, REMAP_PARABOLIC_WENO, REMAP_QUARTIC_WENO, REMAP_MAX_LAYERS = 4, 5, 6, 64
 # launch shape of the FP32-bulk stage kernel (AUTO: timed on the first step)
 SHAPE_AUTO, SHAPE_LAYER_PAIRS, SHAPE_EDGE_TASKS, SHAPE_ONE_LAYER = 0, 1, 2, 3
 
@@ -366,8 +367,10 @@ class Context:
     def remap(self, slot: int, scheme: int, a_vert, b_vert):
         """remap_vertical_coordinates of slot onto z = a_vert eta + b_vert z_s
         (nlayers + 1 interfaces, bottom up): REMAP_PIECEWISE_CONSTANT,
-        REMAP_PIECEWISE_LINEAR, REMAP_PARABOLIC_PPM or REMAP_PPR (the
-        reference's remap_type "0", "1", "2PPM" and its default "PPR")"""
+        REMAP_PIECEWISE_LINEAR, REMAP_PARABOLIC_PPM, REMAP_PPR,
+        REMAP_PARABOLIC_SPLINE, REMAP_PARABOLIC_WENO or REMAP_QUARTIC_WENO
+        (the reference's remap_type "0", "1", "2PPM", its default "PPR",
+        "2S", "2W" and "4")"""
         a = np.ascontiguousarray(a_vert, dtype=np.float64)
         b = np.ascontiguousarray(b_vert, dtype=np.float64)
         L = self.params.nlayers

@@ -6,7 +6,10 @@
 // remap.f90:287-380), "2PPM" (remap2PPM, Colella-Woodward piecewise
 // parabolic with limited slopes, remap.f90:382-495) or "PPR" (remapPPR, the
 // reference's default: Engwirda's PPR library at order 2 with the mono
-// limiter and loose end conditions, remap.f90:950-1010).
+// limiter and loose end conditions, remap.f90:950-1010), "2S" (remap2S,
+// parabolic spline, remap.f90:497-583), "2W" (remap2W, parabolic WENO,
+// remap.f90:585-735) or "4" (remap4, parabolic WENO with power-law
+// reconciliation, remap.f90:737-948).
 //
 // One thread per column (owned node or pole master; grid y = ensemble
 // member): the old interface heights z_old(k) = z_s + sum of mass / rho0 and
@@ -39,9 +42,11 @@ struct RemapArgs {
   double rho0;
 };
 
-// remapPPR indexes edges 1..N+1 in these arrays (hence RMAX + 2)
+// remapPPR indexes edges 1..N+1 in these arrays (hence RMAX + 2); d, r, r1
+// serve remap2S, remap2W and remap4
 struct Work {
   double FC[RMAX + 2], aL[RMAX + 2], aR[RMAX + 2], Hz[RMAX + 2], CF[RMAX + 2], FC1[RMAX + 2];
+  double d[RMAX + 2], r[RMAX + 2], r1[RMAX + 2];
 };
 
 __device__ void remap0(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
@@ -150,6 +155,219 @@ __device__ void remap2ppm(int N, double* vn, const double* zn, const double* vo,
 #undef V
 #undef HZ
 }
+
+// remap2S, remap2W and remap4 index the reference's (1:N) and (0:N) arrays at
+// the same positions; V(k) is var_old(k).  Their arrays are cleared first: a
+// column with fewer layers than a scheme's stencil reads entries the reference
+// leaves unset, and those do not reach its result.
+#define V(k) vo[(k) - 1]
+
+__device__ void clear_work(int N, Work& w) {
+  for (int k = 0; k <= N + 1; ++k) w.FC[k] = w.aL[k] = w.aR[k] = w.Hz[k] = w.CF[k] = w.FC1[k] = w.d[k] = w.r[k] = w.r1[k] = 0.0;
+}
+
+// remap2S (remap.f90:497-583): parabolic spline with parabolic continuation
+// at both ends (dL in CF, dR in FC1)
+__device__ void remap2s(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
+  clear_work(N, w);
+  double *Hz = w.Hz, *dL = w.CF, *dR = w.FC1, *FC = w.FC, *r = w.r;
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  if (N >= 2) {
+    double cff = Hz[1] / Hz[2];
+    FC[1] = 1.0 + cff;
+    r[0] = 2.0 * V(1) + cff * (V(1) + cff * V(2)) / FC[1];
+    for (int k = 1; k <= N - 1; ++k) {
+      cff = 1.0 / (2.0 * Hz[k] + Hz[k + 1] * (2.0 - FC[k]));
+      FC[k + 1] = cff * Hz[k];
+      r[k] = cff * (3.0 * (V(k + 1) * Hz[k] + V(k) * Hz[k + 1]) - Hz[k + 1] * r[k - 1]);
+    }
+    cff = Hz[N] / Hz[N - 1];
+    const double cff1 = 1.0 + cff;
+    r[N] = (cff * (V(N) + cff * V(N - 1)) + cff1 * (2.0 * V(N) - cff1 * r[N - 1])) / (cff1 * (1.0 - cff1 * FC[N]));
+    for (int k = N - 1; k >= 0; --k) r[k] = r[k] - FC[k + 1] * r[k + 1];
+  }
+  for (int k = 1; k <= N; ++k) {
+    dR[k] = 0.5 * (r[k] - r[k - 1]);
+    dL[k] = 0.5 * (r[k] + r[k - 1]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    const bool up = dz > 0.0;
+    const double alpha = dz / (up ? Hz[k + 1] : Hz[k]);
+    const double cff = up ? dR[k + 1] : dR[k], cff1 = up ? dL[k + 1] : -dL[k];
+    FC[k] = dz * (r[k] + alpha * (cff - cff1 * (3.0 - 2.0 * fabs(alpha))));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+}
+
+// parabolic WENO side limits of the inner cells shared by remap2W and remap4
+// (remap.f90:605-639, 760-795); d is remap2W's FC or remap4's d
+__device__ void weno_sides(int N, const double* vo, const double* Hz, const double* d, double* aL, double* aR,
+                           double* dL, double* dR) {
+  for (int k = 2; k <= N - 1; ++k) {
+    double deltaR = Hz[k] * d[k], deltaL = Hz[k] * d[k - 1];
+    if (deltaR * deltaL < 0.0) {
+      deltaR = 0.0;
+      deltaL = 0.0;
+    }
+    double cff = Hz[k - 1] + 2.0 * Hz[k] + Hz[k + 1];
+    const double cffR = cff * d[k], cffL = cff * d[k - 1];
+    if (fabs(deltaR) > fabs(cffL)) deltaR = cffL;
+    if (fabs(deltaL) > fabs(cffR)) deltaL = cffR;
+    cff = (deltaR - deltaL) / (Hz[k - 1] + Hz[k] + Hz[k + 1]);
+    deltaR = deltaR - cff * Hz[k + 1];
+    deltaL = deltaL + cff * Hz[k - 1];
+    aR[k] = V(k) + deltaR;
+    aL[k] = V(k) - deltaL;
+    const double pr = 2.0 * deltaR - deltaL, pl = 2.0 * deltaL - deltaR;
+    dR[k] = pr * pr;
+    dL[k] = pl * pl;
+  }
+}
+
+// remap2W (remap.f90:585-735): LIMIT_INTERIOR, no NEUMANN (dL in CF, dR in FC1)
+__device__ void remap2w(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
+  clear_work(N, w);
+  double *Hz = w.Hz, *aL = w.aL, *aR = w.aR, *dL = w.CF, *dR = w.FC1, *FC = w.FC, *r = w.r;
+  constexpr double eps = 1e-8;
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  for (int k = N - 1; k >= 1; --k) FC[k] = (V(k + 1) - V(k)) / (Hz[k + 1] + Hz[k]);
+  weno_sides(N, vo, Hz, FC, aL, aR, dL, dR);
+  aR[N] = V(N); aL[N] = V(N); dR[N] = 0.0; dL[N] = 0.0;
+  aR[1] = V(1); aL[1] = V(1); dR[1] = 0.0; dL[1] = 0.0;
+  for (int k = 1; k <= N - 1; ++k) {
+    const double deltaL = fmax(dL[k], eps), deltaR = fmax(dR[k + 1], eps);
+    r[k] = (deltaR * aR[k] + deltaL * aL[k + 1]) / (deltaR + deltaL);
+  }
+  r[N] = 2.0 * V(N) - r[N - 1];
+  r[0] = 2.0 * V(1) - r[1];
+  for (int k = 1; k <= N; ++k) {
+    double deltaR = r[k] - V(k), deltaL = V(k) - r[k - 1];
+    const double cffR = 2.0 * deltaR, cffL = 2.0 * deltaL;
+    if (deltaR * deltaL < 0.0) {
+      deltaR = 0.0;
+      deltaL = 0.0;
+    } else if (fabs(deltaR) > fabs(cffL)) {
+      deltaR = cffL;
+    } else if (fabs(deltaL) > fabs(cffR)) {
+      deltaL = cffR;
+    }
+    aR[k] = V(k) + deltaR;
+    aL[k] = V(k) - deltaL;
+    dL[k] = 0.5 * (aR[k] - aL[k]);
+    dR[k] = 0.5 * (aR[k] + aL[k]) - V(k);
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    const bool up = dz > 0.0;
+    const double alpha = dz / (up ? Hz[k + 1] : -Hz[k]);
+    const double cff = up ? aL[k + 1] : aR[k], cffL = up ? dL[k + 1] : -dL[k], cffR = up ? dR[k + 1] : dR[k];
+    FC[k] = dz * (cff + alpha * (cffL - cffR * (3.0 - 2.0 * alpha)));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+}
+
+// remap4 (remap.f90:737-948): parabolic WENO, then the power-law
+// reconciliation of interface values and derivatives (quartic profiles), no
+// NEUMANN (dL in CF, dR in FC1).  The Pade constants are single-precision
+// literals in the reference (default real kind), widened to double.
+__device__ void remap4(int N, double* vn, const double* zn, const double* vo, const double* zo, Work& w) {
+  clear_work(N, w);
+  double *Hz = w.Hz, *aL = w.aL, *aR = w.aR, *dL = w.CF, *dR = w.FC1, *d = w.d, *FC = w.FC, *r = w.r, *r1 = w.r1;
+  constexpr double eps = 1e-8;
+  const double c1 = (double)0.0763636363636363636f, c2 = (double)0.004329004329004329f;
+  const double c3 = (double)0.0363636363636363636f;
+  for (int k = 1; k <= N; ++k) Hz[k] = zo[k] - zo[k - 1];
+  for (int k = N - 1; k >= 1; --k) {
+    FC[k] = 1.0 / (Hz[k + 1] + Hz[k]);
+    d[k] = FC[k] * (V(k + 1) - V(k));
+  }
+  weno_sides(N, vo, Hz, d, aL, aR, dL, dR);
+  {
+    aL[N] = aR[N - 1];
+    aR[N] = 2.0 * V(N) - aL[N];
+    const double pr = 2.0 * aR[N] + aL[N] - 3.0 * V(N), pl = 3.0 * V(N) - 2.0 * aL[N] - aR[N];
+    dR[N] = pr * pr;
+    dL[N] = pl * pl;
+  }
+  if (N >= 2) {
+    aR[1] = aL[2];
+    aL[1] = 2.0 * V(1) - aR[1];
+    const double pr = 2.0 * aR[1] + aL[1] - 3.0 * V(1), pl = 3.0 * V(1) - 2.0 * aL[1] - aR[1];
+    dR[1] = pr * pr;
+    dL[1] = pl * pl;
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double deltaL = fmax(dL[k], eps), deltaR = fmax(dR[k + 1], eps);
+    r1[k] = (deltaR * aR[k] + deltaL * aL[k + 1]) / (deltaR + deltaL);
+  }
+  r1[N] = 2.0 * V(N) - r1[N - 1];
+  r1[0] = 2.0 * V(1) - r1[1];
+  for (int k = 1; k <= N; ++k) {
+    const double deltaR = r1[k] - V(k), deltaL = V(k) - r1[k - 1];
+    double cff = deltaR * deltaL;
+    cff = cff > eps ? (deltaR + deltaL) / cff : 0.0;
+    double cffL = cff * deltaL, cffR = cff * deltaR;
+    if (cffL > 3.0) {
+      cffL = cffL * deltaL;
+      cffR = 0.0;
+    } else if (cffR > 3.0) {
+      cffL = 0.0;
+      cffR = cffR * deltaR;
+    } else {
+      cffL = 4.0 * deltaL - 2.0 * deltaR;
+      cffR = 4.0 * deltaR - 2.0 * deltaL;
+    }
+    cff = 1.0 / Hz[k];
+    dR[k] = cff * cffR;
+    dL[k] = cff * cffL;
+  }
+  for (int k = N - 1; k >= 1; --k) {
+    d[k] = FC[k] * (Hz[k + 1] * dL[k + 1] + Hz[k] * dR[k]);
+    const double cffR = 8.0 * (dR[k] + 2.0 * dL[k]), cffL = 8.0 * (dL[k + 1] + 2.0 * dR[k + 1]);
+    if (fabs(d[k]) > fabs(cffR)) d[k] = cffR;
+    if (fabs(d[k]) > fabs(cffL)) d[k] = cffL;
+    double Hdd, rr;
+    if ((dL[k + 1] - dR[k]) * (V(k + 1) - V(k)) > 0.0) {
+      Hdd = Hz[k] * (d[k] - dR[k]);
+      rr = V(k) - r1[k - 1];
+    } else {
+      Hdd = Hz[k + 1] * (dL[k + 1] - d[k]);
+      rr = r1[k + 1] - V(k + 1);
+    }
+    rr = fabs(rr);
+    double Ampl = 0.2 * Hdd * rr;
+    Hdd = fabs(Hdd);
+    const double cff = rr * rr + c1 * Hdd * (rr + c2 * Hdd);
+    Ampl = cff > eps ? Ampl * (rr + c3 * Hdd) / cff : 0.0;
+    r[k] = r1[k] + Ampl;
+  }
+  r[0] = 2.0 * V(1) - r[1];
+  r[N] = 2.0 * V(N) - r[N - 1];
+  d[0] = d[1];
+  d[N] = d[N - 1];
+  for (int k = 1; k <= N; ++k) {
+    aR[k] = r[k] + r[k - 1] - 2.0 * V(k);
+    dR[k] = Hz[k] * (d[k] + d[k - 1]) - 2.0 * (r[k] - r[k - 1]);
+    dL[k] = Hz[k] * (d[k] - d[k - 1]) - 6.0 * aR[k];
+  }
+  for (int k = 1; k <= N - 1; ++k) {
+    const double dz = zn[k] - zo[k];
+    const bool up = dz > 0.0;
+    const double alpha = dz / (up ? Hz[k + 1] : -Hz[k]);
+    const double cff = up ? aR[k + 1] : aR[k], cffR = up ? dR[k + 1] : -dR[k], cffL = up ? dL[k + 1] : dL[k];
+    FC[k] = dz * (r[k] + 0.5 * dz * d[k] +
+                  alpha * alpha * (cff - cffR * (0.5 - 0.25 * alpha) + cffL * (1.0 - alpha * (1.25 - 0.5 * alpha))));
+  }
+  FC[0] = 0.0;
+  FC[N] = 0.0;
+  for (int k = 1; k <= N; ++k) vn[k - 1] = (Hz[k] * V(k) + FC[k] - FC[k - 1]) / (zn[k] - zn[k - 1]);
+}
+#undef V
 
 // ---- remapPPR (remap.f90:950-1010): Engwirda's PPR library at order 2 ----
 // cubic edge values (ppr/p3e.f90; p1e.f90 for at most three cells) with loose
@@ -446,8 +664,14 @@ __device__ __forceinline__ void remap_col(int scheme, int N, double* vn, const d
     remap1(N, vn, zn, vo, zo, w);
   else if (scheme == 2)
     remap2ppm(N, vn, zn, vo, zo, w);
-  else
+  else if (scheme == 3)
     remap_ppr(N, vn, zn, vo, zo, w);
+  else if (scheme == 4)
+    remap2s(N, vn, zn, vo, zo, w);
+  else if (scheme == 5)
+    remap2w(N, vn, zn, vo, zo, w);
+  else
+    remap4(N, vn, zn, vo, zo, w);
 }
 
 // find_coordinates_incompressible (remap.f90:235-257) of column c
@@ -512,12 +736,17 @@ __global__ __launch_bounds__(RT) void remap_kernel(RemapArgs a) {
 extern "C" int wtg_remap(wtg_ctx* c, int slot, int scheme, const double* a_vert, const double* b_vert) {
   if (!c || !a_vert || !b_vert) return WTG_E_ARG;
   if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
-  if (scheme != WTG_REMAP_PIECEWISE_CONSTANT && scheme != WTG_REMAP_PIECEWISE_LINEAR &&
-      scheme != WTG_REMAP_PARABOLIC_PPM && scheme != WTG_REMAP_PPR)
-    return wtg_fail(c, WTG_E_ARG, "remap scheme %d is not built (0: remap0, 1: remap1, 2: remap2PPM, 3: remapPPR)",
+  if (scheme < WTG_REMAP_PIECEWISE_CONSTANT || scheme > WTG_REMAP_QUARTIC_WENO)
+    return wtg_fail(c, WTG_E_ARG,
+                    "remap scheme %d is not built (0: remap0, 1: remap1, 2: remap2PPM, 3: remapPPR, 4: remap2S, "
+                    "5: remap2W, 6: remap4)",
                     scheme);
   const int L = c->params.nlayers, M = c->nmembers;
   if (L > RMAX) return wtg_fail(c, WTG_E_ARG, "the remap takes at most %d layers, not %d", RMAX, L);
+  // remap2S reads a second layer and remap4 a third one that the reference leaves unset below that
+  if ((scheme == WTG_REMAP_PARABOLIC_SPLINE && L < 2) || (scheme == WTG_REMAP_QUARTIC_WENO && L < 3))
+    return wtg_fail(c, WTG_E_ARG, "remap scheme %d needs at least %d layers, not %d", scheme,
+                    scheme == WTG_REMAP_PARABOLIC_SPLINE ? 2 : 3, L);
   if (!c->topo_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_topography must be called first");
   if (!c->d_remap_ab)
     if (int rc = wtg_dalloc<double>(c, &c->d_remap_ab, 2 * (RMAX + 1))) return rc;

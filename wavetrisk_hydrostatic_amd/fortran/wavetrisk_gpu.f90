@@ -15,9 +15,10 @@ module wavetrisk_gpu
   integer(c_int), parameter :: WTG_SOLVE_BICGSTAB = 0, WTG_SOLVE_JACOBI = 1, WTG_SOLVE_FMG = 2, &
                                WTG_SOLVE_SRJ = 3, WTG_SOLVE_VCYCLE = 4, WTG_SOLVE_LEARNED = 5
   integer(c_int), parameter :: WTG_PRECISION_FP64 = 0, WTG_PRECISION_FP32_BULK = 1
-  ! remap_type "0", "1", "2PPM" and "PPR" of remap_vertical_coordinates
+  ! remap_type "0", "1", "2PPM", "PPR", "2S", "2W" and "4" of remap_vertical_coordinates
   integer(c_int), parameter :: WTG_REMAP_PIECEWISE_CONSTANT = 0, WTG_REMAP_PIECEWISE_LINEAR = 1, &
-                               WTG_REMAP_PARABOLIC_PPM = 2, WTG_REMAP_PPR = 3
+                               WTG_REMAP_PARABOLIC_PPM = 2, WTG_REMAP_PPR = 3, WTG_REMAP_PARABOLIC_SPLINE = 4, &
+                               WTG_REMAP_PARABOLIC_WENO = 5, WTG_REMAP_QUARTIC_WENO = 6
 
   type, bind(C) :: wtg_grid_desc
      integer(c_int) :: level, min_level, domain_level
