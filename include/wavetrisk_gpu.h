@@ -250,6 +250,45 @@ int wtg_set_diffusion(wtg_ctx* ctx, const wtg_diffusion* d);
 #define WTG_REMAP_MAX_LAYERS 64
 int wtg_remap(wtg_ctx* ctx, int slot, int scheme, const double* a_vert, const double* b_vert);
 
+/* vertical_diffusion (vert_diffusion.f90:66-86), the ocean's implicit
+ * vertical mixing after the dynamics of a step (main.f90:420): per column the
+ * eddy diffusivity Kt and viscosity Kv at the layer interfaces, from the
+ * analytic profiles (Kt_analytic, Kv_analytic) or the one-equation TKE closure
+ * (turbulent_diffusion: a backward-Euler TKE step through a tridiagonal
+ * solve, then update_Kv_Kt), then backward-Euler steps of the buoyancy
+ * (backwards_euler_temp) and of the three edge velocities
+ * (backwards_euler_velo), each a tridiagonal solve (LAPACK dgtsv, with its
+ * partial pivoting).  The fields of wtg_vdiff are vert_diffusion_mod's
+ * module variables (vert_diffusion.f90:27-64), shared.f90's tke_closure and
+ * c_p, and the test case's bottom_friction; wtg_vdiff_defaults fills the
+ * reference's defaults (bottom_friction 0).  mode_split 1 takes eta from the
+ * free-surface field (sol(S_MASS, zlevels+1) of the split-explicit step), 0
+ * from the column (free_surface, utils.f90:505-526). */
+typedef struct {
+  int tke_closure, patankar, enhance_diff, mode_split;
+  double C_e, C_eps, C_l, C_k, C_srf, e_0, e_min, e_min_srf, eps_s, kappa_VK, Kt_enh, Kt_max, Kt_min, Kv_max,
+      Kv_min, mixed_layer, l_0, l_min, Neps_sq, Nsq_min, Q_sr, c_p, bottom_friction;
+} wtg_vdiff;
+void wtg_vdiff_defaults(wtg_vdiff* p);
+/* enables the vertical diffusion with p (at least 2 layers, 3 with the TKE
+ * closure); allocates Kt, Kv and the TKE (zero, as init.f90:513-520 leaves
+ * them) and the forcing (zero) on first use */
+int wtg_set_vdiff(wtg_ctx* ctx, const wtg_vdiff* p);
+/* host arrays, every member one after another: Kt, Kv [nlayers + 1][ncells]
+ * (interfaces 0..nlayers, bottom up), tke [nlayers][ncells] (layers 1..nlayers) */
+int wtg_vdiff_set_coefficients(wtg_ctx* ctx, const double* Kt, const double* Kv, const double* tke);
+int wtg_vdiff_get_coefficients(wtg_ctx* ctx, double* Kt, double* Kv, double* tke);
+/* the test case's hooks evaluated per node (host arrays, members one after
+ * another): bottom_buoy_flux, top_buoy_flux, tau_mag [ncells] and wind_flux
+ * [3][ncells] (the RT, DG, UP edges of each node); NULL keeps an array */
+int wtg_vdiff_set_forcing(wtg_ctx* ctx, const double* bottom_buoy_flux, const double* top_buoy_flux,
+                          const double* tau_mag, const double* wind_flux);
+/* one call of vertical_diffusion on slot with time step dt; istep 1 (the
+ * first step of a run, main.f90:384) recomputes Kt and Kv from the TKE
+ * before its step, as init_diffuse does.  The slot is ghost filled before and
+ * after. */
+int wtg_vertical_diffusion(wtg_ctx* ctx, int slot, double dt, int istep);
+
 /* Device addresses of a slot's arrays (for zero-copy interop). */
 int wtg_state_device_ptrs(wtg_ctx* ctx, int slot, double** mass, double** temp, double** velo);
 

@@ -34,6 +34,9 @@ def load():
         _lib.wto_trend_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 6
         _lib.wto_remap.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 5
         _lib.wto_remap_column.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 4
+        _lib.wto_vertical_diffusion.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 11
+        _lib.wto_dgtsv.argtypes = [C.c_int] + [C.c_void_p] * 4
+        _lib.wto_dgtsv.restype = C.c_int
         _lib.wto_rk4_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 3
         _lib.wto_num_threads.restype = C.c_int
         _lib.wto_node_ghost.argtypes = [C.c_void_p, C.c_void_p]
@@ -85,6 +88,29 @@ class Diff(C.Structure):
     _fields_ = [("lsclr", C.c_int), ("ldivu", C.c_int), ("lrotu", C.c_int), ("visc_m", C.c_double),
                 ("visc_t", C.c_double), ("visc_divu", C.c_double), ("visc_rotu", C.c_double),
                 ("plant_owner_halo", C.c_int)]
+
+
+# vert_diffusion_mod's module variables (vert_diffusion.f90:27-64), c_p of
+# shared.f90 and the test case's bottom_friction
+VDIFF_DEFAULTS = dict(tke_closure=0, patankar=0, enhance_diff=0, C_e=1.0, C_eps=0.7, C_l=2.0e5, C_k=0.1, C_srf=67.83,
+                      e_0=1.0e-6 / np.sqrt(2.0), e_min=1.0e-6, e_min_srf=1.0e-4, eps_s=1.0e-20, kappa_VK=0.4,
+                      Kt_enh=1.0, Kt_max=1e-2, Kt_min=1.2e-5, Kv_max=1e-2, Kv_min=1.2e-4, mixed_layer=-200.0,
+                      l_0=4.0e-2, l_min=1.0e-2, Neps_sq=1.0e-20, Nsq_min=1.0e-12, Q_sr=0.0, c_p=1004.64,
+                      bottom_friction=4e-4)
+
+
+class VDiff(C.Structure):
+    """wto_vdiff: switches and constants of the vertical diffusion"""
+    _fields_ = ([(k, C.c_int) for k in ("tke_closure", "patankar", "enhance_diff")] +
+                [(k, C.c_double) for k in VDIFF_DEFAULTS if k not in ("tke_closure", "patankar", "enhance_diff")])
+
+
+def dgtsv(dl, d, du, b):
+    """LAPACK dgtsv with one right-hand side (dgtsv.f90): (x, info)"""
+    lib = load()
+    dl, d, du, b = (_f64(a).copy() for a in (dl, d, du, b))
+    info = lib.wto_dgtsv(d.size, dl.ctypes.data, d.ctypes.data, du.ctypes.data, b.ctypes.data)
+    return b, info
 
 
 class Oracle:
@@ -143,6 +169,23 @@ class Oracle:
         self.lib.wto_remap(C.byref(self.g), int(scheme), mass.shape[0], a.ctypes.data, b.ctypes.data,
                            mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
         return mass, temp, velo
+
+    def vertical_diffusion(self, mass, temp, velo, dt: float, istep: int, forcing: dict, Kt, Kv, tke, eta=None,
+                           **params):
+        """vertical_diffusion (vert_diffusion.f90:66-86) of a ghost-filled
+        state: forcing holds bot, top, tau [nc] and wind [3, nc]; Kt, Kv
+        [L + 1, nc], tke [L, nc]; params override VDIFF_DEFAULTS.  Returns the
+        new (mass, temp, velo, Kt, Kv, tke)."""
+        mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))
+        Kt, Kv, tke = (_f64(x).copy() for x in (Kt, Kv, tke))
+        fb, ft, fs, fw = (_f64(forcing[k]) for k in ("bot", "top", "tau", "wind"))
+        vd = VDiff(**{**VDIFF_DEFAULTS, **params})
+        e = None if eta is None else _f64(eta)
+        self.lib.wto_vertical_diffusion(C.byref(self.g), C.byref(vd), mass.shape[0], dt, istep, fb.ctypes.data,
+                                        ft.ctypes.data, fs.ctypes.data, fw.ctypes.data,
+                                        None if e is None else e.ctypes.data, Kt.ctypes.data, Kv.ctypes.data,
+                                        tke.ctypes.data, mass.ctypes.data, temp.ctypes.data, velo.ctypes.data)
+        return mass, temp, velo, Kt, Kv, tke
 
     def rk4(self, mass, temp, velo, dt: float, nsteps: int, diffusion=None):
         mass, temp, velo = (x.copy() for x in self._state(mass, temp, velo))

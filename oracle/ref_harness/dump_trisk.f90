@@ -19,6 +19,11 @@
 ! initialize_a_b_vert, a_vert = 1 - b_vert), dumped with the tag 'r0_'.
 ! uniform_first (logical): before that remap, a piecewise-constant remap onto
 ! uniform coordinates (so the old layers are evenly spaced), dumped as 'p0_'.
+! vdiff (1 analytic eddy coefficients, 2 TKE closure): after that, two calls of
+! the reference's vertical_diffusion (vert_diffusion.f90:66-86, istep 1 then 2,
+! dt = step) with position-dependent forcing hooks (vd_forcing, dumped as
+! 'vf_*'), each followed by update_bdry and dumps 'v<n>_' of the state and of
+! Kt, Kv (interfaces 0..zlevels) and tke ('v0_': the coefficients before).
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
@@ -32,7 +37,9 @@
 ! 2=f64), count int64, data), read by tests/golden/make_golden.py.
 module harness_hooks
   use domain_mod
+  use geom_mod
   implicit none
+  real(8), target :: vd_friction = 4d-4  ! bottom_friction of the vertical diffusion
 contains
   function no_scalar_flux (q, dom, id, idE, idNE, idN, v, zlev, type)
     real(8), dimension(1:EDGE)                           :: no_scalar_flux
@@ -119,6 +126,56 @@ contains
     cr(UP+1) = (vort(TRIAG*idW+LORT+1) - vort(TRIAG*id +UPLT+1)) / dom%pedlen%elts(EDGE*id+UP+1)
     diff_velo_source = (-1)**(Laplace_rotu-1) * (visc_divu * gd - visc_rotu * cr)
   end function diff_velo_source
+
+  ! forcing of the vertical diffusion (the test cases' bottom_buoy_flux,
+  ! top_buoy_flux, wind_flux, tau_mag and bottom_friction): smooth functions
+  ! of the node position, also evaluated by vd_forcing for the dump
+  subroutine vd_forcing (p, bot, top, tau, wind)
+    type(Coord) :: p
+    real(8)     :: bot, top, tau, wind(1:EDGE), lon, lat
+    integer     :: e
+    call cart2sph (p, lon, lat)
+    bot = 2d-7 * sin (lat) * cos (2d0*lon)
+    top = 1d-6 * cos (lon) * cos (lat) + 3d-7
+    tau = 0.1d0 * (1d0 + 0.5d0 * sin (lat) * cos (lon))
+    do e = 1, EDGE
+       wind(e) = 1d-4 * (1d0 + 0.3d0*e) * sin (2d0*lat) * cos (lon - 0.5d0*e)
+    end do
+  end subroutine vd_forcing
+
+  real(8) function vd_bottom_buoy (dom, i, j, z_lev, offs, dims)
+    type(Domain)                   :: dom
+    integer                        :: i, j, z_lev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    real(8) :: top, tau, wind(1:EDGE)
+    call vd_forcing (dom%node%elts(idx (i, j, offs, dims)+1), vd_bottom_buoy, top, tau, wind)
+  end function vd_bottom_buoy
+
+  real(8) function vd_top_buoy (dom, i, j, z_lev, offs, dims)
+    type(Domain)                   :: dom
+    integer                        :: i, j, z_lev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    real(8) :: bot, tau, wind(1:EDGE)
+    call vd_forcing (dom%node%elts(idx (i, j, offs, dims)+1), bot, vd_top_buoy, tau, wind)
+  end function vd_top_buoy
+
+  function vd_wind (dom, i, j, z_lev, offs, dims)
+    real(8), dimension(1:EDGE)     :: vd_wind
+    type(Domain)                   :: dom
+    integer                        :: i, j, z_lev
+    integer, dimension(N_BDRY+1)   :: offs
+    integer, dimension(2,N_BDRY+1) :: dims
+    real(8) :: bot, top, tau
+    call vd_forcing (dom%node%elts(idx (i, j, offs, dims)+1), bot, top, tau, vd_wind)
+  end function vd_wind
+
+  real(8) function vd_tau (p)
+    type(Coord) :: p
+    real(8) :: bot, top, wind(1:EDGE)
+    call vd_forcing (p, bot, top, vd_tau, wind)
+  end function vd_tau
 end module harness_hooks
 
 module harness_ic
@@ -222,6 +279,7 @@ end module harness_ic
 program dump_trisk
   use time_integr_mod
   use remap_mod
+  use vert_diffusion_mod
   use harness_hooks
   use harness_ic
   implicit none
@@ -232,13 +290,15 @@ program dump_trisk
   integer(8)     :: t_start, t_end, t_rate
   real(8)        :: step, vsclr(2), vdivu, vrotu
   logical        :: nanfill, uniform_first
+  integer        :: vdiff, id
+  real(8)        :: lon, lat
   character(16)  :: vremap, coords
   namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every, &
-                 vremap, coords, uniform_first
+                 vremap, coords, uniform_first, vdiff
 
   case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0; every = 1
   lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0; vremap = ''; coords = 'uniform'
-  uniform_first = .false.
+  uniform_first = .false.; vdiff = 0
   read (5, nml=run)
   ic_case = case
 
@@ -251,6 +311,8 @@ program dump_trisk
   compressible  = .false.
   mode_split    = .false.
   optimize_grid = NO_OPTIM
+  vert_diffuse  = vdiff > 0
+  tke_closure   = vdiff == 2
   Laplace_sclr  = lsclr; Laplace_divu = ldivu; Laplace_rotu = lrotu
   visc_sclr(S_MASS) = vsclr(1); visc_sclr(S_TEMP) = vsclr(2)
   visc_divu     = vdivu
@@ -272,6 +334,11 @@ program dump_trisk
      a_vert_mass(k) = 0d0; b_vert_mass(k) = b_vert(k) - b_vert(k-1)
   end do
 
+  bottom_friction  => vd_friction
+  bottom_buoy_flux => vd_bottom_buoy
+  top_buoy_flux    => vd_top_buoy
+  wind_flux        => vd_wind
+  tau_mag          => vd_tau
   physics_scalar_flux => no_scalar_flux
   physics_velo_source => no_velo_source
   if (lsclr /= 0 .or. lrotu /= 0) then
@@ -363,6 +430,31 @@ program dump_trisk
      call remap_vertical_coordinates
      call update_bdry (sol(:,1:zlevels), NONE, 1)
      call dump_state ('r0_')
+  end if
+  if (vdiff > 0) then
+     ! initial TKE (Kt and Kv are recomputed from it at istep 1) and the forcing
+     do d = 1, size(grid)
+        do id = 1, grid(d)%node%length
+           call cart2sph (grid(d)%node%elts(id), lon, lat)
+           do k = 1, zlevels
+              tke(k)%data(d)%elts(id) = 1d-5 * (1d0 + 0.5d0 * sin (3d0*lon + dble(k)) * cos (lat))
+           end do
+           do k = 0, zlevels
+              Kt(k)%data(d)%elts(id) = Kt_min
+              Kv(k)%data(d)%elts(id) = Kv_min
+           end do
+        end do
+        call dump_forcing (d)
+     end do
+     call dump_coefficients ('v0_')
+     dt = step
+     do n = 1, 2
+        istep = n
+        call vertical_diffusion
+        call update_bdry (sol(:,1:zlevels), NONE, 1)
+        call dump_state ('v'//achar(48+n)//'_')
+        call dump_coefficients ('v'//achar(48+n)//'_')
+     end do
   end if
   close (u)
   if (ntimed > 0) then
@@ -480,6 +572,36 @@ contains
        end do
     end do
   end subroutine dump_state
+
+  subroutine dump_coefficients (pre)
+    character(*) :: pre
+    integer :: dd, kk
+    do kk = 0, zlevels
+       do dd = 1, size(grid)
+          call wr_f (pre//'kt', Kt(kk)%data(dd)%elts)
+          call wr_f (pre//'kv', Kv(kk)%data(dd)%elts)
+       end do
+    end do
+    do kk = 1, zlevels
+       do dd = 1, size(grid)
+          call wr_f (pre//'tke', tke(kk)%data(dd)%elts)
+       end do
+    end do
+  end subroutine dump_coefficients
+
+  subroutine dump_forcing (dd)
+    integer :: dd, ii
+    real(8), dimension(:), allocatable :: bot, top, tau, wind
+    allocate (bot(grid(dd)%node%length), top(grid(dd)%node%length), tau(grid(dd)%node%length), &
+              wind(EDGE*grid(dd)%node%length))
+    do ii = 1, grid(dd)%node%length
+       call vd_forcing (grid(dd)%node%elts(ii), bot(ii), top(ii), tau(ii), wind(EDGE*(ii-1)+1:EDGE*ii))
+    end do
+    call wr_f ('vf_bot', bot)
+    call wr_f ('vf_top', top)
+    call wr_f ('vf_tau', tau)
+    call wr_f ('vf_wind', wind)
+  end subroutine dump_forcing
 
   subroutine dump_trend (pre)
     character(*) :: pre

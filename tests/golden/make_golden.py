@@ -229,13 +229,18 @@ def grid_fixture(J, geometry=True):
     print(f"{name}.npz written")
 
 
-def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
+def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, vdiff=0):
     """trisk_<name>.npz: the reference's initial state, trend and the state
     after each RK4 step (after every `every`-th one: mass<k> is step k*every);
     remap = (scheme, coords): also the state after remap_vertical_coordinates
-    of the last step's state (remapped_mass, ...) and the target coordinates"""
+    of the last step's state (remapped_mass, ...) and the target coordinates;
+    vdiff (1 analytic, 2 TKE closure): the forcing, the initial TKE and the
+    state, Kt, Kv and TKE after each of two vertical_diffusion calls (vd1_*,
+    vd2_*) from the last step's state"""
     param, ml, dl = PARAMS[J]
     extra = f", every={every}" if every > 1 else ""
+    if vdiff:
+        extra += f", vdiff={vdiff}"
     if remap:
         extra += f", vremap='{remap[0]}', coords='{remap[1]}'"
         if len(remap) > 2:  # a piecewise-constant remap onto uniform coordinates first
@@ -261,10 +266,11 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
     poles = [(0, n, 0), (5, 0, n)]
     cells = [(lz, i, j) for lz in range(10) for j in range(n) for i in range(n)]
 
-    def field(tag, mult, with_halo):
-        out = np.full((L, mult, NC), np.nan)
+    def field(tag, mult, with_halo, nk=None):
+        nk = L if nk is None else nk
+        out = np.full((nk, mult, NC), np.nan)
         arr = o[tag]
-        for k in range(L):
+        for k in range(nk):
             for (lz, i, j) in cells + (poles if mult == 1 else []):
                 c = cell_index(S, lz, i, j)
                 for q in range(mult):
@@ -310,6 +316,21 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None):
         keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
                 "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "remapped_mass",
                 "remapped_temp", "remapped_velo", "remap_scheme", "a_vert", "b_vert")
+        data = {k: v for k, v in data.items() if k in keep}
+    if vdiff:
+        data.update(vdiff=vdiff, vf_bot=field("vf_bot", 1, False, 1)[0], vf_top=field("vf_top", 1, False, 1)[0],
+                    vf_tau=field("vf_tau", 1, False, 1)[0], vf_wind=field("vf_wind", 3, False, 1)[0],
+                    vd0_tke=field("v0_tke", 1, False), vd0_kt=field("v0_kt", 1, False, L + 1),
+                    vd0_kv=field("v0_kv", 1, False, L + 1))
+        for c in (1, 2):
+            for v, m in (("mass", 1), ("temp", 1), ("velo", 3)):
+                data[f"vd{c}_{v}"] = field(f"v{c}_{v}", m, False)
+            data[f"vd{c}_kt"] = field(f"v{c}_kt", 1, False, L + 1)
+            data[f"vd{c}_kv"] = field(f"v{c}_kv", 1, False, L + 1)
+            data[f"vd{c}_tke"] = field(f"v{c}_tke", 1, False)
+        keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
+                "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "vdiff", "vf_bot", "vf_top",
+                "vf_tau", "vf_wind") + tuple(k for k in data if k.startswith("vd"))
         data = {k: v for k, v in data.items() if k in keep}
     if diff:
         data.update(laplace=np.array([diff["lsclr"], diff["ldivu"], diff["lrotu"]]),
@@ -681,6 +702,9 @@ def main():
         # edge values)
         trisk_fixture("ocean_remapppr_uniform_J4", 4, "ocean", 6, 1, 1200.0, remap=("PPR", "chebyshev", "uniform"))
         trisk_fixture("ocean_remapppr_l3_J4", 4, "ocean", 3, 1, 1200.0, remap=("PPR", "chebyshev"))
+    if "vdiff" in what:  # vertical_diffusion (analytic and TKE closure) after one RK4 step of the J4 ocean
+        trisk_fixture("ocean_vdiff_analytic_J4", 4, "ocean", 6, 1, 1200.0, vdiff=1)
+        trisk_fixture("ocean_vdiff_tke_J4", 4, "ocean", 6, 1, 1200.0, vdiff=2)
     if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
         trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:

@@ -1,0 +1,172 @@
+"""Vertical diffusion on the GPU (wtg_vertical_diffusion; needs a GPU).
+
+Bars: against the reference's vertical_diffusion (fixtures
+trisk_ocean_vdiff_{analytic,tke}_J4: two calls after one RK4 step of the J4
+ocean with 6 layers, position-dependent forcing) and against the oracle on a
+J7 ocean with 20 layers: with the TKE closure within 1e-15 relative (same
+evaluation order, no FMA contraction, sqrt correctly rounded: observed bit
+for bit); with the analytic profiles within 1e-14 (they call exp, whose last
+bit the device library may round otherwise than the host's).  Every member of
+an ensemble equals its own single-member run bit for bit.  At J8 with 30
+layers, without wind and bottom friction, each edge column's dz-weighted
+velocity is conserved to rounding and the coefficients stay in their clamps
+(size-independent properties of the backward-Euler step).
+"""
+import numpy as np
+import pytest
+
+import wavetrisk_hydrostatic_amd as W
+from wavetrisk_hydrostatic_amd import cases
+from conftest import golden, maxrel
+from helpers import grid_of_fixture
+
+pytestmark = pytest.mark.gpu
+
+BAR = {0: 1e-14, 1: 1e-15}
+
+
+def _fixture_ctx(f, g, tke):
+    L, ns = int(f["nlayers"]), int(f["nsteps"])
+    ctx = W.Context(g, nlayers=L, grav=float(f["grav"]), ref_density=float(f["ref_density"]))
+    ctx.set_topography(np.nan_to_num(f["topo"]))
+    ctx.upload_state(0, *(np.nan_to_num(f[f"{v}{ns}"]) for v in ("mass", "temp", "velo")))
+    ctx.set_vdiff(tke_closure=tke, bottom_friction=4e-4)
+    ctx.vdiff_set_coefficients(*(np.nan_to_num(f[k]) for k in ("vd0_kt", "vd0_kv", "vd0_tke")))
+    ctx.vdiff_set_forcing(*(np.nan_to_num(f[k]) for k in ("vf_bot", "vf_top", "vf_tau", "vf_wind")))
+    return ctx
+
+
+@pytest.mark.parametrize("case,tke", [("ocean_vdiff_analytic_J4", 0), ("ocean_vdiff_tke_J4", 1)])
+def test_vertical_diffusion_matches_reference(case, tke):
+    print(f"vertical diffusion {case}", flush=True)
+    f = golden(f"trisk_{case}.npz")
+    g = grid_of_fixture(f)
+    ctx = _fixture_ctx(f, g, tke)
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    for c in (1, 2):
+        ctx.vertical_diffusion(0, float(f["dt"]), c)
+        m, t, u = ctx.download_state(0)
+        Kt, Kv, e = ctx.vdiff_coefficients()
+        err = (maxrel(m[:, oc], f[f"vd{c}_mass"][:, oc]), maxrel(t[:, oc], f[f"vd{c}_temp"][:, oc]),
+               maxrel(u[:, :, oe], f[f"vd{c}_velo"][:, :, oe]), maxrel(Kt[:, oc], f[f"vd{c}_kt"][:, oc]),
+               maxrel(Kv[:, oc], f[f"vd{c}_kv"][:, oc]), maxrel(e[:, oc], f[f"vd{c}_tke"][:, oc]))
+        assert max(err) <= BAR[tke], (c, err)
+    ctx.close()
+
+
+def _synthetic(g, L, seed, members=1):
+    rng = np.random.default_rng(seed)
+    NC = g.layout.ncells
+    draws = [cases.ocean(g, L, seed=seed + k) for k in range(members)]
+    forcing = dict(bot=rng.uniform(-3e-7, 3e-7, members * NC), top=rng.uniform(-1e-6, 1e-6, members * NC),
+                   tau=rng.uniform(0.02, 0.2, members * NC), wind=rng.uniform(-2e-4, 2e-4, (members * 3, NC)))
+    coef = (np.full((members * (L + 1), NC), 1.2e-5), np.full((members * (L + 1), NC), 1.2e-4),
+            rng.uniform(1e-6, 3e-5, (members * L, NC)))
+    return draws, forcing, coef
+
+
+def _gpu(g, L, draws, forcing, coef, tke, dt, calls):
+    M = len(draws)
+    ctx = W.Context(g, nlayers=L, members=M)
+    ctx.set_topography(np.stack([d[0] for d in draws]) if M > 1 else draws[0][0])
+    ctx.upload_state(0, *(np.concatenate([d[v] for d in draws]) for v in (1, 2, 3)))
+    ctx.set_vdiff(tke_closure=tke, bottom_friction=4e-4)
+    ctx.vdiff_set_coefficients(*coef)
+    ctx.vdiff_set_forcing(forcing["bot"], forcing["top"], forcing["tau"], forcing["wind"])
+    for c in range(1, calls + 1):
+        ctx.vertical_diffusion(0, dt, c)
+    out = ctx.download_state(0) + ctx.vdiff_coefficients()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("tke", [0, 1])
+def test_j7_l20_equals_oracle(tke):
+    print(f"vertical diffusion J7 L20 tke_closure={tke}", flush=True)
+    from oracle.oracle import Oracle, geometry_of
+    g = W.Grid(level=7, min_level=7, domain_level=2)
+    L = 20
+    draws, forcing, coef = _synthetic(g, L, seed=70)
+    got = _gpu(g, L, draws, forcing, coef, tke, 1800.0, 2)
+    topo, m, t, u = draws[0]
+    o = Oracle(g.layout, geometry_of(g), topo)
+    st = (m.copy(), t.copy(), u.copy())
+    o.ghost_fill(*st)
+    for c in (1, 2):
+        res = o.vertical_diffusion(*st, 1800.0, c, forcing, *coef, tke_closure=tke, bottom_friction=4e-4)
+        st, coef = res[:3], res[3:]
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    err = [maxrel(got[0][:, oc], res[0][:, oc]), maxrel(got[1][:, oc], res[1][:, oc]),
+           maxrel(got[2][:, :, oe], res[2][:, :, oe])] + [maxrel(got[k][:, oc], res[k][:, oc]) for k in (3, 4, 5)]
+    assert max(err) <= BAR[tke], err
+    assert maxrel(got[1][:, oc], t[:, oc]) > 1e-9  # not a no-op
+
+
+def test_ensemble_members_equal_single_runs():
+    print("vertical diffusion ensemble of 2, J5 L6, TKE closure", flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, NC = 6, g.layout.ncells
+    draws, forcing, coef = _synthetic(g, L, seed=40, members=2)
+    ens = _gpu(g, L, draws, forcing, coef, 1, 900.0, 2)
+    for k in range(2):
+        fk = dict(bot=forcing["bot"][k * NC:(k + 1) * NC], top=forcing["top"][k * NC:(k + 1) * NC],
+                  tau=forcing["tau"][k * NC:(k + 1) * NC], wind=forcing["wind"][3 * k:3 * k + 3])
+        ck = (coef[0][k * (L + 1):(k + 1) * (L + 1)], coef[1][k * (L + 1):(k + 1) * (L + 1)], coef[2][k * L:(k + 1) * L])
+        one = _gpu(g, L, [draws[k]], fk, ck, 1, 900.0, 2)
+        sl = slice(k * L, (k + 1) * L)
+        si = slice(k * (L + 1), (k + 1) * (L + 1))
+        assert np.array_equal(ens[0][sl], one[0]) and np.array_equal(ens[1][sl], one[1])
+        assert np.array_equal(ens[2][sl], one[2])
+        assert np.array_equal(ens[3][si], one[3]) and np.array_equal(ens[4][si], one[4])
+        assert np.array_equal(ens[5][sl], one[5])
+
+
+def test_j8_l30_conserves_edge_columns():
+    print("vertical diffusion J8 L30: velocity column content without wind and friction", flush=True)
+    g = W.Grid(level=8, min_level=8, domain_level=2)
+    L = 30
+    draws, forcing, coef = _synthetic(g, L, seed=80)
+    forcing["wind"][:] = 0.0
+    topo, m, t, u = draws[0]
+    u = u * (1.0 + 0.3 * np.arange(L))[:, None, None]  # sheared layers
+    ctx = W.Context(g, nlayers=L)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.set_vdiff(tke_closure=1, bottom_friction=0.0)
+    ctx.vdiff_set_coefficients(*coef)
+    ctx.vdiff_set_forcing(forcing["bot"], forcing["top"], forcing["tau"], forcing["wind"])
+    ctx.vertical_diffusion(0, 5e7, 1)
+    gm, gt, gu = ctx.download_state(0)
+    Kt, Kv, e = ctx.vdiff_coefficients()
+    ctx.close()
+    lay = g.layout
+    S, oe, oc = lay.S, lay.owned_cells(with_poles=False), lay.owned_cells()
+    dz = gm / 1028.0  # the mass is not changed, and its halo is filled
+    for k, off in enumerate((1, S + 1, S)):
+        dze = 0.5 * (dz[:, oe] + dz[:, oe + off])
+        before, after = (dze * u[:, k, oe]).sum(0), (dze * gu[:, k, oe]).sum(0)
+        assert np.max(np.abs(after - before)) <= 1e-13 * np.max(np.abs(dze * u[:, k, oe]).sum(0))
+    assert maxrel(gu[:, :, oe], u[:, :, oe]) > 1e-4
+    assert e[:, oc].min() >= 1e-6 and 1.2e-4 <= Kv[:, oc].min() and Kv[:, oc].max() <= 1e-2
+    assert 1.2e-5 <= Kt[:, oc].min() and Kt[:, oc].max() <= 1e-2
+
+
+def test_vertical_diffusion_refusals():
+    print("vertical diffusion refusals", flush=True)
+    g = W.Grid(level=4, min_level=4, domain_level=2)
+    topo, m, t, u = cases.ocean(g, 2, seed=1)
+    c = W.Context(g, nlayers=2)
+    c.set_topography(topo)
+    c.upload_state(0, m, t, u)
+    with pytest.raises(W.WtgError):
+        c.vertical_diffusion(0, 600.0, 1)  # not enabled
+    with pytest.raises(W.WtgError):
+        c.set_vdiff(tke_closure=1)  # the TKE system needs three layers
+    with pytest.raises(ValueError):
+        c.set_vdiff(no_such_constant=1.0)
+    c.set_vdiff(tke_closure=0)
+    c.vertical_diffusion(0, 600.0, 1)
+    with pytest.raises(W.WtgError):
+        c.set_vdiff(tke_closure=0, mode_split=1)
+        c.vertical_diffusion(0, 600.0, 1)  # no free surface to take eta from
+    c.close()

@@ -1,0 +1,131 @@
+"""Vertical diffusion of buoyancy and velocity (vertical_diffusion) in the
+oracle against the reference.
+
+Fixtures tests/golden/trisk_ocean_vdiff_{analytic,tke}_J4.npz come from the
+reference's vertical_diffusion (vert_diffusion.f90:66-86) run by the harness
+(oracle/ref_harness/dump_trisk.f90, mode vdiff) after one RK4 step of the J4
+ocean with 6 layers: the analytic eddy coefficients (tke_closure false) and
+the one-equation TKE closure, each called twice (istep 1, which recomputes Kt
+and Kv from the TKE first, then istep 2), with position-dependent forcing
+hooks (bottom and top buoyancy flux, wind stress and wind flux, bottom
+friction 4e-4) that the fixture holds per node.
+
+Bars: the oracle equals the reference to 1e-15 relative in the state and in
+Kt, Kv and the TKE after each call (observed: bit for bit, poles included).
+What the mathematics fixes: the tridiagonal solver (LAPACK dgtsv) agrees with
+a dense solve, also where it pivots; with no wind and no bottom friction the
+velocity step conserves each edge column's dz-weighted velocity; the TKE at
+the surface is max(C_srf tau / rho0, e_min_srf); a zero time step changes
+neither buoyancy nor velocity; the coefficients stay within their clamps.
+Planted errors (in the oracle's inputs, on the CPU): swapped top and bottom
+buoyancy fluxes and a doubled time step leave the bar.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, maxrel
+from helpers import grid_of_fixture, oracle_of
+from oracle.oracle import VDIFF_DEFAULTS, dgtsv
+
+CASES = [("ocean_vdiff_analytic_J4", 0), ("ocean_vdiff_tke_J4", 1)]
+
+
+def load(case):
+    f = golden(f"trisk_{case}.npz")
+    g = grid_of_fixture(f)
+    o = oracle_of(g, np.nan_to_num(f["topo"]), f)
+    ns = int(f["nsteps"])
+    m, t, u = (np.nan_to_num(f[f"{v}{ns}"]).copy() for v in ("mass", "temp", "velo"))
+    o.ghost_fill(m, t, u)
+    coef = tuple(np.nan_to_num(f[k]) for k in ("vd0_kt", "vd0_kv", "vd0_tke"))
+    forcing = dict(bot=np.nan_to_num(f["vf_bot"]), top=np.nan_to_num(f["vf_top"]), tau=np.nan_to_num(f["vf_tau"]),
+                   wind=np.nan_to_num(f["vf_wind"]))
+    return f, g, o, (m, t, u), coef, forcing
+
+
+def errors(f, g, c, got):
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    m, t, u, Kt, Kv, tke = got
+    return (maxrel(m[:, oc], f[f"vd{c}_mass"][:, oc]), maxrel(t[:, oc], f[f"vd{c}_temp"][:, oc]),
+            maxrel(u[:, :, oe], f[f"vd{c}_velo"][:, :, oe]), maxrel(Kt[:, oc], f[f"vd{c}_kt"][:, oc]),
+            maxrel(Kv[:, oc], f[f"vd{c}_kv"][:, oc]), maxrel(tke[:, oc], f[f"vd{c}_tke"][:, oc]))
+
+
+@pytest.mark.parametrize("case,tke", CASES)
+def test_vertical_diffusion_matches_reference(case, tke):
+    f, g, o, st, coef, forcing = load(case)
+    dt = float(f["dt"])
+    for c in (1, 2):
+        got = o.vertical_diffusion(*st, dt, c, forcing, *coef, tke_closure=tke, bottom_friction=4e-4)
+        err = errors(f, g, c, got)
+        assert max(err) <= 1e-15, (c, err)
+        st, coef = got[:3], got[3:]
+    oc = g.layout.owned_cells()
+    assert maxrel(st[2][:, :, g.layout.owned_cells(with_poles=False)],
+                  np.nan_to_num(f[f"velo{int(f['nsteps'])}"])[:, :, g.layout.owned_cells(with_poles=False)]) > 1e-5
+    assert maxrel(st[1][:, oc], np.nan_to_num(f[f"temp{int(f['nsteps'])}"])[:, oc]) > 1e-9  # not a no-op
+
+
+@pytest.mark.parametrize("case,tke", CASES)
+def test_planted_errors_are_caught(case, tke):
+    f, g, o, st, coef, forcing = load(case)
+    dt = float(f["dt"])
+    swapped = dict(forcing, bot=forcing["top"], top=forcing["bot"])
+    assert max(errors(f, g, 1, o.vertical_diffusion(*st, dt, 1, swapped, *coef, tke_closure=tke,
+                                                       bottom_friction=4e-4))) > 1e-9
+    assert max(errors(f, g, 1, o.vertical_diffusion(*st, 2 * dt, 1, forcing, *coef, tke_closure=tke,
+                                                       bottom_friction=4e-4))) > 1e-9
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 30])
+def test_dgtsv_solves_tridiagonal_systems(n):
+    rng = np.random.default_rng(n)
+    for dominant in (True, False):
+        dl, du = rng.uniform(-1, 1, max(n - 1, 0)), rng.uniform(-1, 1, max(n - 1, 0))
+        d = rng.uniform(2.5, 3, n) if dominant else rng.uniform(-0.3, 0.3, n)  # small pivots: row interchanges
+        b = rng.uniform(-1, 1, n)
+        A = np.diag(d) + np.diag(dl, -1) + np.diag(du, 1)
+        x, info = dgtsv(dl, d, du, b)
+        assert info == 0
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    x, info = dgtsv(np.zeros(2), np.array([0.0, 1.0, 1.0]), np.zeros(2), np.ones(3))
+    assert info == 1  # the zero pivot is reported as LAPACK does
+
+
+def test_velocity_step_conserves_edge_columns_without_wind_and_friction():
+    f, g, o, st, coef, forcing = load("ocean_vdiff_tke_J4")
+    calm = dict(forcing, wind=np.zeros_like(forcing["wind"]))
+    # layers sheared (velocity scaled by 1 + 0.3 k in layer k) and a long step (5e7 s): the columns mix substantially
+    L = st[0].shape[0]
+    st = (st[0], st[1], st[2] * (1.0 + 0.3 * np.arange(L))[:, None, None])
+    m, t, u, Kt, Kv, tke = o.vertical_diffusion(*st, 5e7, 1, calm, *coef, tke_closure=1, bottom_friction=0.0)
+    lay = g.layout
+    S, oe = lay.S, lay.owned_cells(with_poles=False)
+    dz = st[0] / 1028.0
+    for e, off in enumerate((1, S + 1, S)):
+        dze = 0.5 * (dz[:, oe] + dz[:, oe + off])
+        before, after = (dze * st[2][:, e, oe]).sum(0), (dze * u[:, e, oe]).sum(0)
+        assert np.max(np.abs(after - before)) <= 1e-13 * np.max(np.abs(dze * st[2][:, e, oe]).sum(0))
+    assert maxrel(u[:, :, oe], st[2][:, :, oe]) > 1e-3  # the velocities did mix
+
+
+def test_tke_bounds_and_surface_value():
+    f, g, o, st, coef, forcing = load("ocean_vdiff_tke_J4")
+    m, t, u, Kt, Kv, tke = o.vertical_diffusion(*st, float(f["dt"]), 1, forcing, *coef, tke_closure=1,
+                                                 bottom_friction=4e-4)
+    oc = g.layout.owned_cells()
+    d = VDIFF_DEFAULTS
+    surf = np.maximum(d["C_srf"] * forcing["tau"][oc] / 1028.0, d["e_min_srf"])
+    assert np.array_equal(tke[-1, oc], np.maximum(surf, d["e_min"]))
+    assert tke[:, oc].min() >= d["e_min"]
+    assert Kv[:, oc].min() >= d["Kv_min"] and Kv[:, oc].max() <= d["Kv_max"]
+    assert Kt[:, oc].min() >= d["Kt_min"] and Kt[:, oc].max() <= d["Kt_max"]
+
+
+@pytest.mark.parametrize("tke", [0, 1])
+def test_zero_time_step_changes_nothing(tke):
+    f, g, o, st, coef, forcing = load(CASES[tke][0])
+    m, t, u, Kt, Kv, e = o.vertical_diffusion(*st, 0.0, 1, forcing, *coef, tke_closure=tke, bottom_friction=4e-4)
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    assert maxrel(t[:, oc], st[1][:, oc]) <= 1e-15
+    assert np.array_equal(u[:, :, oe], st[2][:, :, oe])

@@ -54,6 +54,14 @@ class _Params(C.Structure):
     _fields_ = [("nlayers", C.c_int), ("grav", C.c_double), ("ref_density", C.c_double), ("p_top", C.c_double)]
 
 
+class _VDiff(C.Structure):
+    _fields_ = ([(k, C.c_int) for k in ("tke_closure", "patankar", "enhance_diff", "mode_split")] +
+                [(k, C.c_double) for k in ("C_e", "C_eps", "C_l", "C_k", "C_srf", "e_0", "e_min", "e_min_srf", "eps_s",
+                                           "kappa_VK", "Kt_enh", "Kt_max", "Kt_min", "Kv_max", "Kv_min",
+                                           "mixed_layer", "l_0", "l_min", "Neps_sq", "Nsq_min", "Q_sr", "c_p",
+                                           "bottom_friction")])
+
+
 class _Diffusion(C.Structure):
     _fields_ = [("laplace_sclr", C.c_int), ("laplace_divu", C.c_int), ("laplace_rotu", C.c_int),
                 ("visc_mass", C.c_double), ("visc_temp", C.c_double), ("visc_divu", C.c_double),
@@ -86,6 +94,12 @@ SIGNATURES = {
     "wtg_set_graph": (C.c_int, [_P, C.c_int]),
     "wtg_set_diffusion": (C.c_int, [_P, C.POINTER(_Diffusion)]),
     "wtg_remap": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "wtg_vdiff_defaults": (None, [C.POINTER(_VDiff)]),
+    "wtg_set_vdiff": (C.c_int, [_P, C.POINTER(_VDiff)]),
+    "wtg_vdiff_set_coefficients": (C.c_int, [_P, _P, _P, _P]),
+    "wtg_vdiff_get_coefficients": (C.c_int, [_P, _P, _P, _P]),
+    "wtg_vdiff_set_forcing": (C.c_int, [_P, _P, _P, _P, _P]),
+    "wtg_vertical_diffusion": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "wtg_grid_shard_halo": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "wtg_ghost_fill_split_timed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D]),
     "wtg_fs_set_graph": (C.c_int, [_P, C.c_int]),
@@ -377,6 +391,45 @@ class Context:
         if a.shape != (L + 1,) or b.shape != (L + 1,):
             raise ValueError(f"a_vert and b_vert need {L + 1} interfaces")
         _check(_lib.wtg_remap(self._h, slot, scheme, _ptr(a), _ptr(b)), self._h, "wtg_remap")
+
+    def set_vdiff(self, **params):
+        """vertical_diffusion's switches and constants (wtg_vdiff: tke_closure,
+        patankar, enhance_diff, mode_split, C_e, ..., bottom_friction); those
+        not given keep the reference's defaults"""
+        p = _VDiff()
+        _lib.wtg_vdiff_defaults(C.byref(p))
+        for k, v in params.items():
+            if k not in dict(_VDiff._fields_):
+                raise ValueError(f"unknown vertical-diffusion parameter {k}")
+            setattr(p, k, v)
+        _check(_lib.wtg_set_vdiff(self._h, C.byref(p)), self._h, "wtg_set_vdiff")
+
+    def vdiff_set_coefficients(self, Kt=None, Kv=None, tke=None):
+        """Kt, Kv [members * (nlayers + 1), ncells], tke [members * nlayers, ncells]"""
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (Kt, Kv, tke)]
+        _check(_lib.wtg_vdiff_set_coefficients(self._h, *[None if x is None else _ptr(x) for x in a]), self._h,
+               "wtg_vdiff_set_coefficients")
+
+    def vdiff_coefficients(self):
+        """(Kt, Kv, tke) as set by the last call"""
+        L, M, NC = self.params.nlayers, self.members, self.layout.ncells
+        Kt, Kv, tke = np.empty((M * (L + 1), NC)), np.empty((M * (L + 1), NC)), np.empty((M * L, NC))
+        _check(_lib.wtg_vdiff_get_coefficients(self._h, _ptr(Kt), _ptr(Kv), _ptr(tke)), self._h,
+               "wtg_vdiff_get_coefficients")
+        return Kt, Kv, tke
+
+    def vdiff_set_forcing(self, bottom_buoy_flux=None, top_buoy_flux=None, tau_mag=None, wind_flux=None):
+        """the test case's hooks per node: [members * ncells] fluxes and wind
+        stress, [members * 3, ncells] wind flux of the RT, DG, UP edges"""
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+             for x in (bottom_buoy_flux, top_buoy_flux, tau_mag, wind_flux)]
+        _check(_lib.wtg_vdiff_set_forcing(self._h, *[None if x is None else _ptr(x) for x in a]), self._h,
+               "wtg_vdiff_set_forcing")
+
+    def vertical_diffusion(self, slot: int, dt: float, istep: int):
+        """vertical_diffusion (vert_diffusion.f90:66-86) of slot; istep 1
+        recomputes Kt, Kv from the TKE first"""
+        _check(_lib.wtg_vertical_diffusion(self._h, slot, float(dt), int(istep)), self._h, "wtg_vertical_diffusion")
 
     def set_diffusion(self, laplace=(0, 0, 0), visc=(0.0, 0.0, 0.0, 0.0)):
         """Laplacian diffusion (order 1) or hyperdiffusion (order 2) in the

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libwavetrisk_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("trisk.hip", "elliptic.hip", "ml.hip", "mlp.hip", "diffusion.hip", "remap.hip", "grid.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("trisk.hip", "elliptic.hip", "ml.hip", "mlp.hip", "diffusion.hip",
+                                                     "remap.hip", "vdiff.hip", "grid.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

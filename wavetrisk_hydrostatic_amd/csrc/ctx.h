@@ -142,6 +142,13 @@ struct wtg_ctx {
   double *dif_phu = nullptr;                        // [3] velocity source
   // wtg_remap: a_vert then b_vert, [WTG_REMAP_MAX_LAYERS + 1] each (first use)
   double* d_remap_ab = nullptr;
+  // vertical diffusion (wtg_set_vdiff): its constants, the eddy coefficients
+  // at the interfaces [member][nlayers + 1][ncells], the TKE
+  // [member][nlayers][ncells] and the forcing [member][6][ncells] (bottom and
+  // top buoyancy flux, wind-stress magnitude, wind flux of the three edges)
+  wtg_vdiff vd{};
+  bool vd_set = false;
+  double *vd_kt = nullptr, *vd_kv = nullptr, *vd_tke = nullptr, *vd_force = nullptr;
 };
 
 extern thread_local std::string g_create_error;

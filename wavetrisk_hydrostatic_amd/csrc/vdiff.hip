@@ -1,0 +1,475 @@
+// Vertical diffusion of an ocean column (wtg_vertical_diffusion):
+// vertical_diffusion (vert_diffusion.f90:66-86) with the analytic eddy
+// coefficients or the one-equation TKE closure (turbulent_diffusion,
+// vert_diffusion.f90:88-264, with N_sq, l_scales, Richardson, Prandtl, Kv_tke
+// and Kt_tke at 438-573), then the backward-Euler steps of the buoyancy
+// (backwards_euler_temp, 266-355) and of the edge velocities
+// (backwards_euler_velo, 357-424), each through LAPACK's dgtsv (dgtsv.f90).
+//
+// Two launches, one thread per column (grid y = ensemble member).  The first
+// computes Kt, Kv and the TKE of every owned node and pole master; it reads
+// the velocities of the node's six edges (its kinetic energy, io.f90:169-206),
+// so it must finish before any velocity changes.  The second steps the
+// buoyancy of the same columns and the velocities of each owned node's RT,
+// DG and UP edges (with the node's Kv and the layer thicknesses averaged with
+// the edge's other node).  Column arrays are private (up to
+// WTG_REMAP_MAX_LAYERS layers); FP64 in the reference's evaluation order
+// without FMA contraction.  The analytic profiles call exp, whose last bit may
+// differ from the host library's; everything else equals the oracle bit for
+// bit.  Both kernels are HBM-bound single passes over the column state.
+#include <hip/hip_runtime.h>
+
+#include "ctx.h"
+
+namespace {
+
+constexpr int H = wtg::HALO;
+constexpr int VMAX = WTG_REMAP_MAX_LAYERS;
+constexpr int VT = 64;  // threads per block: the private column arrays are large
+
+struct VdArgs {
+  int n, S, L, istep;
+  int64_t NC, spole, npole;
+  double dt, rho0, grav;
+  wtg_vdiff p;
+  const double *len, *ped, *hinv, *topo, *eta;  // topo, eta [members][NC] (eta may be null)
+  const double* force;                          // [members][6][NC]
+  double *m, *t, *u;                            // the state slot
+  double *kt, *kv, *tke;                        // [members][L + 1][NC], [members][L][NC]
+};
+
+// LAPACK dgtsv with one right-hand side on rows 1..n of 1-based arrays
+__device__ void dgtsv(int n, double* dl, double* d, double* du, double* b) {
+  for (int i = 1; i <= n - 2; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) return;
+      const double fact = dl[i] / d[i];
+      d[i + 1] = d[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+      dl[i] = 0.0;
+    } else {
+      const double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      double temp = d[i + 1];
+      d[i + 1] = du[i] - fact * temp;
+      dl[i] = du[i + 1];
+      du[i + 1] = -fact * dl[i];
+      du[i] = temp;
+      temp = b[i];
+      b[i] = b[i + 1];
+      b[i + 1] = temp - fact * b[i + 1];
+    }
+  }
+  if (n > 1) {
+    const int i = n - 1;
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) return;
+      const double fact = dl[i] / d[i];
+      d[i + 1] = d[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+    } else {
+      const double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      double temp = d[i + 1];
+      d[i + 1] = du[i] - fact * temp;
+      du[i] = temp;
+      temp = b[i];
+      b[i] = b[i + 1];
+      b[i + 1] = temp - fact * b[i + 1];
+    }
+  }
+  if (d[n] == 0.0) return;
+  b[n] = b[n] / d[n];
+  if (n > 1) b[n - 1] = (b[n - 1] - du[n - 1] * b[n]) / d[n - 1];
+  for (int i = n - 2; i >= 1; --i) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+}
+
+// the column of thread tid: owned nodes, then the south and north pole masters
+__device__ __forceinline__ int64_t column(const VdArgs& a, int64_t tid, bool& pole) {
+  const int64_t nown = (int64_t)10 * a.n * a.n;
+  pole = tid >= nown;
+  if (pole) return tid == nown ? a.spole : a.npole;
+  const int lz = (int)(tid / ((int64_t)a.n * a.n)), r = (int)(tid % ((int64_t)a.n * a.n));
+  return (int64_t)lz * a.S * a.S + (int64_t)(r / a.n + H) * a.S + (r % a.n + H);
+}
+
+__device__ __forceinline__ double prandtl(double Ri) {
+  if (Ri < 0.2) return 1.0;
+  if (Ri >= 0.2 && Ri <= 2.0) return 5.0 * Ri;
+  return 10.0;
+}
+
+__device__ __forceinline__ double kv_tke(const wtg_vdiff& p, double e, double lk) {
+  return fmin(p.Kv_max, fmax(p.C_k * lk * sqrt(e), p.Kv_min));
+}
+
+__device__ __forceinline__ double kt_tke(const wtg_vdiff& p, double Kv, double Nsq, double Ri) {
+  double kt = fmin(p.Kt_max, fmax(Kv / prandtl(Ri), p.Kt_min));
+  if (p.enhance_diff && Nsq <= p.Nsq_min) kt = p.Kt_enh;
+  return kt;
+}
+
+__device__ __forceinline__ double free_surface(const VdArgs& a, const double* m, double zs, int64_t c) {
+  double depth = 0.0;
+  for (int k = 0; k < a.L; ++k) depth = depth + (0.0 + m[(int64_t)k * a.NC + c]) / (a.rho0 * 1.0);
+  return depth + zs;
+}
+
+struct TkeWork {
+  double dz[VMAX + 2], Umag[VMAX + 2], dzl[VMAX + 2], dU2[VMAX + 2], Nsq[VMAX + 2], e[VMAX + 2];
+  double leps[VMAX + 2], lk[VMAX + 2], S1[VMAX + 2], S2[VMAX + 2], dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2];
+  double rhs[VMAX + 2], dn[VMAX + 2], up[VMAX + 2];
+};
+
+// l_scales (vert_diffusion.f90:502-536) on [0..L]
+__device__ void l_scales(const VdArgs& a, double tau, TkeWork& w) {
+  const wtg_vdiff& p = a.p;
+  const int L = a.L;
+  for (int l = 0; l <= L; ++l) {
+    w.dn[l] = sqrt(2.0 * w.e[l] / fmax(w.Nsq[l], p.Neps_sq));
+    w.up[l] = w.dn[l];
+  }
+  w.dn[0] = p.l_0;
+  for (int l = 1; l <= L; ++l) w.dn[l] = fmin(w.dn[l], w.dn[l - 1] + w.dz[l]);
+  w.up[L] = fmax(p.l_0, p.kappa_VK * p.C_l / (a.rho0 * a.grav) * tau);
+  for (int l = L - 1; l >= 0; --l) w.up[l] = fmin(w.up[l], w.up[l + 1] + w.dz[l + 1]);
+  for (int l = 0; l <= L; ++l) {
+    w.leps[l] = fmax(p.l_min, sqrt(w.up[l] * w.dn[l]));
+    w.lk[l] = fmax(p.l_min, fmin(w.up[l], w.dn[l]));
+  }
+}
+
+// turbulent_diffusion of every owned node and pole master
+__global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
+  const int64_t tid = (int64_t)blockIdx.x * VT + threadIdx.x;
+  if (tid >= (int64_t)10 * a.n * a.n + 2) return;
+  bool pole;
+  const int64_t c = column(a, tid, pole);
+  const int L = a.L;
+  const int64_t NC = a.NC, mb = blockIdx.y;
+  const double* m = a.m + mb * L * NC;
+  const double* t = a.t + mb * L * NC;
+  const double* u = a.u + mb * 3 * L * NC;
+  double* kt = a.kt + mb * (L + 1) * NC;
+  double* kv = a.kv + mb * (L + 1) * NC;
+  double* tke = a.tke + mb * L * NC;
+  const double zs = a.topo[mb * NC + c];
+  const double tau = a.force[(mb * 6 + 2) * NC + c];
+  const wtg_vdiff& p = a.p;
+  if (!p.tke_closure) {
+    const double eta = a.eta ? a.eta[mb * NC + c] : free_surface(a, m, zs, c);
+    const double e20 = -20.0;
+    double z = zs;
+    kt[c] = p.Kt_min + p.Kt_max * exp(e20 * (z - eta) / p.mixed_layer);
+    kv[c] = p.Kv_min + p.Kv_max * exp(e20 * (z - eta) / p.mixed_layer);
+    for (int l = 1; l <= L; ++l) {
+      z = z + (0.0 + m[(int64_t)(l - 1) * NC + c]) / (a.rho0 * 1.0);
+      kt[(int64_t)l * NC + c] = p.Kt_min + p.Kt_max * exp(e20 * (z - eta) / p.mixed_layer);
+      kv[(int64_t)l * NC + c] = p.Kv_min + p.Kv_max * exp(e20 * (z - eta) / p.mixed_layer);
+    }
+    return;
+  }
+  TkeWork w;
+  // init_diffuse: dz and |u| (twice the TRiSK kinetic energy) per layer
+  const int64_t W = c - 1, SW = c - a.S - 1, Sx = c - a.S;
+  for (int k = 1; k <= L; ++k) {
+    w.dz[k] = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
+    const double* uk = u + (int64_t)(k - 1) * 3 * NC;
+    const double pRT = uk[c] * a.len[c], pDG = uk[NC + c] * a.len[NC + c], pUP = uk[2 * NC + c] * a.len[2 * NC + c];
+    const double dRT = uk[c] * a.ped[c], dDG = uk[NC + c] * a.ped[NC + c], dUP = uk[2 * NC + c] * a.ped[2 * NC + c];
+    const double pUPS = uk[2 * NC + Sx] * a.len[2 * NC + Sx], pDGSW = uk[NC + SW] * a.len[NC + SW];
+    const double pRTW = uk[W] * a.len[W];
+    const double dRTW = uk[W] * a.ped[W], dDGSW = uk[NC + SW] * a.ped[NC + SW], dUPS = uk[2 * NC + Sx] * a.ped[2 * NC + Sx];
+    const double ke =
+        (pUP * dUP + pDG * dDG + pRT * dRT + pUPS * dUPS + pDGSW * dDGSW + pRTW * dRTW) * a.hinv[c] / 4.0;
+    w.Umag[k] = sqrt(2.0 * ke);
+  }
+  for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
+  for (int l = 0; l <= L; ++l) {
+    if (l == 0 || l == L) {
+      w.dU2[l] = 0.0;
+    } else {
+      const double q = (w.Umag[l + 1] - w.Umag[l]) / w.dzl[l];
+      w.dU2[l] = q * q;
+    }
+  }
+  for (int l = 0; l <= L; ++l) {
+    const int k = l == 0 ? 1 : l == L ? L - 1 : l;  // N_sq at the interface above layer k
+    const double bb = t[(int64_t)(k - 1) * NC + c] / (0.0 + m[(int64_t)(k - 1) * NC + c]);
+    const double ba = t[(int64_t)k * NC + c] / (0.0 + m[(int64_t)k * NC + c]);
+    w.Nsq[l] = a.grav * (ba - bb) / (0.5 * (w.dz[k] + w.dz[k + 1]));
+  }
+  w.e[0] = p.e_0;
+  for (int l = 1; l <= L; ++l) w.e[l] = fmax(tke[(int64_t)(l - 1) * NC + c], p.e_min);
+  l_scales(a, tau, w);
+  if (a.istep == 1)
+    for (int l = 0; l <= L; ++l) {
+      const double Ri = w.Nsq[l] / (w.dU2[l] + p.eps_s);
+      const double v = kv_tke(p, w.e[l], w.lk[l]);
+      kv[(int64_t)l * NC + c] = v;
+      kt[(int64_t)l * NC + c] = kt_tke(p, v, w.Nsq[l], Ri);
+    }
+  for (int l = 1; l <= L - 1; ++l) {
+    const double KV = kv[(int64_t)l * NC + c], KT = kt[(int64_t)l * NC + c];
+    const double turb = w.e[l] == 0.0 ? 0.0 : p.C_eps * sqrt(w.e[l]) / w.leps[l];
+    w.S1[l] = KV * w.dU2[l] - KT * w.Nsq[l];
+    if (p.patankar && w.S1[l] <= 0.0) {
+      w.S1[l] = KV * w.dU2[l];
+      w.S2[l] = -turb - KT * w.Nsq[l] / w.e[l];
+    } else {
+      w.S2[l] = -turb;
+    }
+  }
+  // the TKE rows 1..L-1 (coeff reads dzl at the row's own l)
+  const double dt = a.dt;
+#define KVI(l) kv[(int64_t)(l) * NC + c]
+#define COEFF(l, dzv, kvv) (dt * p.C_e * (kvv) / (w.dzl[l] * (dzv)))
+  {
+    int l = 1;
+    w.du[l] = -COEFF(l, w.dz[l + 1], 0.5 * (KVI(l) + KVI(l + 1)));
+    w.dg[l] = 1.0 - w.du[l] - dt * w.S2[l];
+    w.rhs[l] = w.e[l] + dt * w.S1[l];
+    for (l = 2; l <= L - 2; ++l) {
+      w.du[l] = -COEFF(l, w.dz[l + 1], 0.5 * (KVI(l) + KVI(l + 1)));
+      w.dl[l - 1] = -COEFF(l, w.dz[l], 0.5 * (KVI(l) + KVI(l - 1)));
+      w.dg[l] = 1.0 - (w.du[l] + w.dl[l - 1]) - dt * w.S2[l];
+      w.rhs[l] = w.e[l] + dt * w.S1[l];
+    }
+    l = L - 1;
+    w.dl[l - 1] = -COEFF(l, w.dz[l], 0.5 * (KVI(l) + KVI(l - 1)));
+    w.dg[l] = 1.0 - w.dl[l - 1] - dt * w.S2[l];
+    w.rhs[l] = w.e[l] + dt * w.S1[l];
+  }
+#undef COEFF
+#undef KVI
+  // rows 1..L-1 of the library are entries 1..L-1 of dg / rhs and 1..L-2 of dl / du
+  dgtsv(L - 1, w.dl, w.dg, w.du, w.rhs);
+  w.e[0] = p.e_0;
+  for (int l = 1; l <= L - 1; ++l) w.e[l] = p.patankar ? w.rhs[l] : fmax(w.rhs[l], p.e_min);
+  w.e[L] = fmax(p.C_srf * tau / a.rho0, p.e_min_srf);
+  // update_Kv_Kt
+  l_scales(a, tau, w);
+  for (int l = 0; l <= L; ++l) {
+    const double Ri = w.Nsq[l] / (w.dU2[l] + p.eps_s);
+    const double v = kv_tke(p, w.e[l], w.lk[l]);
+    kv[(int64_t)l * NC + c] = v;
+    kt[(int64_t)l * NC + c] = kt_tke(p, v, w.Nsq[l], Ri);
+  }
+  for (int l = 1; l <= L; ++l) tke[(int64_t)(l - 1) * NC + c] = fmax(w.e[l], p.e_min);
+}
+
+__device__ __forceinline__ double irradiance(const wtg_vdiff& p, double depth) {
+  return p.Q_sr * (0.58 * exp(-depth / 0.35) + (1.0 - 0.58) * exp(-depth / 23.0));
+}
+
+struct SolveWork {
+  double z[VMAX + 2], dz[VMAX + 2], dzl[VMAX + 2], dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2], r[VMAX + 2];
+};
+
+// backwards_euler_temp of every owned node and pole master, then
+// backwards_euler_velo of the owned nodes' edges
+__global__ __launch_bounds__(VT) void vd_solve_kernel(VdArgs a) {
+  const int64_t tid = (int64_t)blockIdx.x * VT + threadIdx.x;
+  if (tid >= (int64_t)10 * a.n * a.n + 2) return;
+  bool pole;
+  const int64_t c = column(a, tid, pole);
+  const int L = a.L;
+  const int64_t NC = a.NC, mb = blockIdx.y;
+  const double* m = a.m + mb * L * NC;
+  double* t = a.t + mb * L * NC;
+  double* u = a.u + mb * 3 * L * NC;
+  const double* kt = a.kt + mb * (L + 1) * NC;
+  const double* kv = a.kv + mb * (L + 1) * NC;
+  const double* f = a.force + mb * 6 * NC;
+  const wtg_vdiff& p = a.p;
+  const double dt = a.dt;
+  SolveWork w;
+  const double zs = a.topo[mb * NC + c];
+  const double eta = a.eta ? a.eta[mb * NC + c] : free_surface(a, m, zs, c);
+  w.z[0] = zs;
+  for (int k = 1; k <= L; ++k) {
+    w.dz[k] = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
+    w.z[k] = w.z[k - 1] + w.dz[k];
+  }
+  for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
+#define B(k) (t[(int64_t)((k) - 1) * NC + c] / (0.0 + m[(int64_t)((k) - 1) * NC + c]))
+#define SOLAR(k) (p.Q_sr * (irradiance(p, eta - w.z[k]) - irradiance(p, eta - w.z[(k) - 1])) / (a.rho0 * p.c_p))
+#define CT(k, kk) (dt * kt[(int64_t)(kk) * NC + c] / (w.dzl[kk] * w.dz[k]))
+  {
+    int k = 1;
+    w.du[k] = -CT(k, k);
+    w.dg[k] = 1.0 - w.du[k];
+    w.r[k] = B(k) + dt * (-f[c] + SOLAR(k)) / w.dz[k];
+    for (k = 2; k <= L - 1; ++k) {
+      w.du[k] = -CT(k, k);
+      w.dl[k - 1] = -CT(k, k - 1);
+      w.dg[k] = 1.0 - (w.du[k] + w.dl[k - 1]);
+      w.r[k] = B(k) + dt * SOLAR(k) / w.dz[k];
+    }
+    k = L;
+    w.dl[k - 1] = -CT(k, k - 1);
+    w.dg[k] = 1.0 - w.du[k - 1];  // the reference's top row takes the row below's super-diagonal entry
+    w.r[k] = B(k) + dt * (f[NC + c] + p.Q_sr / (a.rho0 * p.c_p)) / w.dz[k];
+  }
+#undef B
+#undef SOLAR
+#undef CT
+  dgtsv(L, w.dl, w.dg, w.du, w.r);
+  for (int k = 1; k <= L; ++k) {
+    const int64_t x = (int64_t)(k - 1) * NC + c;
+    t[x] = (0.0 + m[x]) * w.r[k];
+  }
+  if (pole) return;  // the poles carry no edges of their own
+  const int64_t nb[3] = {c + 1, c + a.S + 1, c + a.S};
+  for (int e = 0; e < 3; ++e) {
+    for (int k = 1; k <= L; ++k) {
+      const int64_t x = (int64_t)(k - 1) * NC;
+      w.dz[k] = 0.5 * ((0.0 + m[x + c]) / (a.rho0 * 1.0) + (0.0 + m[x + nb[e]]) / (a.rho0 * 1.0));
+    }
+    for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
+#define CV(k, kk) (dt * kv[(int64_t)(kk) * NC + c] / (w.dzl[kk] * w.dz[k]))
+#define UV(k) u[((int64_t)((k) - 1) * 3 + e) * NC + c]
+    int k = 1;
+    w.du[k] = -CV(k, k);
+    w.dg[k] = 1.0 - w.du[k] + dt * p.bottom_friction / w.dz[k];
+    w.r[k] = UV(k);
+    for (k = 2; k <= L - 1; ++k) {
+      w.du[k] = -CV(k, k);
+      w.dl[k - 1] = -CV(k, k - 1);
+      w.dg[k] = 1.0 - (w.du[k] + w.dl[k - 1]);
+      w.r[k] = UV(k);
+    }
+    k = L;
+    w.dl[k - 1] = -CV(k, k - 1);
+    w.dg[k] = 1.0 - w.dl[k - 1];
+    w.r[k] = UV(k) + dt * f[(3 + e) * NC + c] / w.dz[k];
+    dgtsv(L, w.dl, w.dg, w.du, w.r);
+    for (k = 1; k <= L; ++k) UV(k) = w.r[k];
+#undef CV
+#undef UV
+  }
+}
+
+}  // namespace
+
+extern "C" void wtg_vdiff_defaults(wtg_vdiff* p) {
+  if (!p) return;
+  *p = wtg_vdiff{};
+  p->C_e = 1.0;
+  p->C_eps = 7.0e-1;
+  p->C_l = 2.0e5;
+  p->C_k = 1.0e-1;
+  p->C_srf = 6.783e1;
+  p->e_0 = 1.0e-6 / sqrt(2.0);
+  p->e_min = 1.0e-6;
+  p->e_min_srf = 1.0e-4;
+  p->eps_s = 1.0e-20;
+  p->kappa_VK = 4.0e-1;
+  p->Kt_enh = 1.0;
+  p->Kt_max = 1e-2;
+  p->Kt_min = 1.2e-5;
+  p->Kv_max = 1e-2;
+  p->Kv_min = 1.2e-4;
+  p->mixed_layer = -200.0;
+  p->l_0 = 4.0e-2;
+  p->l_min = 1.0e-2;
+  p->Neps_sq = 1.0e-20;
+  p->Nsq_min = 1.0e-12;
+  p->Q_sr = 0.0;
+  p->c_p = 1004.64;
+}
+
+extern "C" int wtg_set_vdiff(wtg_ctx* c, const wtg_vdiff* p) {
+  if (!c || !p) return WTG_E_ARG;
+  const int L = c->params.nlayers, M = c->nmembers;
+  if (L > VMAX) return wtg_fail(c, WTG_E_ARG, "the vertical diffusion takes at most %d layers, not %d", VMAX, L);
+  // backwards_euler_temp indexes a second layer and the TKE system a third one
+  if (L < (p->tke_closure ? 3 : 2))
+    return wtg_fail(c, WTG_E_ARG, "the vertical diffusion needs at least %d layers%s, not %d", p->tke_closure ? 3 : 2,
+                    p->tke_closure ? " with the TKE closure" : "", L);
+  const int64_t NC = c->info.ncells;
+  if (!c->vd_kt) {
+    if (int rc = wtg_dalloc<double>(c, &c->vd_kt, (size_t)M * (L + 1) * NC)) return rc;
+    if (int rc = wtg_dalloc<double>(c, &c->vd_kv, (size_t)M * (L + 1) * NC)) return rc;
+    if (int rc = wtg_dalloc<double>(c, &c->vd_tke, (size_t)M * L * NC)) return rc;
+    if (int rc = wtg_dalloc<double>(c, &c->vd_force, (size_t)M * 6 * NC)) return rc;
+    WTG_HIPCHK(c, hipMemsetAsync(c->vd_kt, 0, (size_t)M * (L + 1) * NC * sizeof(double), c->stream));
+    WTG_HIPCHK(c, hipMemsetAsync(c->vd_kv, 0, (size_t)M * (L + 1) * NC * sizeof(double), c->stream));
+    WTG_HIPCHK(c, hipMemsetAsync(c->vd_tke, 0, (size_t)M * L * NC * sizeof(double), c->stream));
+    WTG_HIPCHK(c, hipMemsetAsync(c->vd_force, 0, (size_t)M * 6 * NC * sizeof(double), c->stream));
+    WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->vd = *p;
+  c->vd_set = true;
+  return 0;
+}
+
+static int vd_ready(wtg_ctx* c) {
+  if (!c) return WTG_E_ARG;
+  if (!c->vd_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_vdiff must be called first");
+  return 0;
+}
+
+extern "C" int wtg_vdiff_set_coefficients(wtg_ctx* c, const double* Kt, const double* Kv, const double* tke) {
+  if (int rc = vd_ready(c)) return rc;
+  const int L = c->params.nlayers, M = c->nmembers;
+  const size_t NC = (size_t)c->info.ncells, ni = (size_t)M * (L + 1) * NC, nl = (size_t)M * L * NC;
+  if (Kt) WTG_HIPCHK(c, hipMemcpyAsync(c->vd_kt, Kt, ni * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (Kv) WTG_HIPCHK(c, hipMemcpyAsync(c->vd_kv, Kv, ni * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (tke) WTG_HIPCHK(c, hipMemcpyAsync(c->vd_tke, tke, nl * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int wtg_vdiff_get_coefficients(wtg_ctx* c, double* Kt, double* Kv, double* tke) {
+  if (int rc = vd_ready(c)) return rc;
+  const int L = c->params.nlayers, M = c->nmembers;
+  const size_t NC = (size_t)c->info.ncells, ni = (size_t)M * (L + 1) * NC, nl = (size_t)M * L * NC;
+  if (Kt) WTG_HIPCHK(c, hipMemcpyAsync(Kt, c->vd_kt, ni * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (Kv) WTG_HIPCHK(c, hipMemcpyAsync(Kv, c->vd_kv, ni * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (tke) WTG_HIPCHK(c, hipMemcpyAsync(tke, c->vd_tke, nl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int wtg_vdiff_set_forcing(wtg_ctx* c, const double* bot, const double* top, const double* tau,
+                                     const double* wind) {
+  if (int rc = vd_ready(c)) return rc;
+  const size_t NC = (size_t)c->info.ncells;
+  const double* src[4] = {bot, top, tau, wind};
+  const size_t off[4] = {0, 1, 2, 3}, cnt[4] = {1, 1, 1, 3};
+  for (int mb = 0; mb < c->nmembers; ++mb)
+    for (int q = 0; q < 4; ++q)
+      if (src[q])
+        WTG_HIPCHK(c, hipMemcpyAsync(c->vd_force + ((size_t)mb * 6 + off[q]) * NC, src[q] + (size_t)mb * cnt[q] * NC,
+                                     cnt[q] * NC * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int wtg_vertical_diffusion(wtg_ctx* c, int slot, double dt, int istep) {
+  if (int rc = vd_ready(c)) return rc;
+  if (slot < 0 || slot >= c->nslots) return wtg_fail(c, WTG_E_ARG, "slot %d out of range [0,%d)", slot, c->nslots);
+  if (!c->topo_set) return wtg_fail(c, WTG_E_STATE, "wtg_set_topography must be called first");
+  const double* eta = nullptr;
+  if (c->vd.mode_split) {
+    if (!c->fs[FS_ETA])
+      return wtg_fail(c, WTG_E_STATE, "mode_split takes eta from the free surface, which has not been set");
+    eta = c->fs[FS_ETA];
+  }
+  const int L = c->params.nlayers, M = c->nmembers;
+  const auto& s = c->slots[slot];
+  if (int rc = wtg_ghost_fill_layers(c, L * M, s.m, s.t, s.u)) return rc;
+  VdArgs a{c->info.n, c->info.S, L, istep, c->info.ncells, c->info.south_pole, c->info.north_pole, dt,
+           c->params.ref_density, c->params.grav, c->vd, c->d_len, c->d_ped, c->d_hinv, c->d_topo, eta, c->vd_force,
+           s.m, s.t, s.u, c->vd_kt, c->vd_kv, c->vd_tke};
+  const int64_t cols = (int64_t)10 * c->info.n * c->info.n + 2;
+  const dim3 grid((unsigned)((cols + VT - 1) / VT), (unsigned)M);
+  hipLaunchKernelGGL(vd_turbulent_kernel, grid, dim3(VT), 0, c->stream, a);
+  WTG_HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(vd_solve_kernel, grid, dim3(VT), 0, c->stream, a);
+  WTG_HIPCHK(c, hipGetLastError());
+  if (int rc = wtg_ghost_fill_layers(c, L * M, s.m, s.t, s.u)) return rc;
+  WTG_HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}

@@ -44,6 +44,12 @@ module wavetrisk_gpu
      integer(c_int) :: laplace_sclr, laplace_divu, laplace_rotu
      real(c_double) :: visc_mass, visc_temp, visc_divu, visc_rotu
   end type
+  ! vert_diffusion_mod's module variables, tke_closure, c_p and the test case's bottom_friction
+  type, bind(C) :: wtg_vdiff
+     integer(c_int) :: tke_closure, patankar, enhance_diff, mode_split
+     real(c_double) :: C_e, C_eps, C_l, C_k, C_srf, e_0, e_min, e_min_srf, eps_s, kappa_VK, Kt_enh, Kt_max, Kt_min, &
+                       Kv_max, Kv_min, mixed_layer, l_0, l_min, Neps_sq, Nsq_min, Q_sr, c_p, bottom_friction
+  end type
   type, bind(C) :: wtg_mg_params
      real(c_double) :: coarse_tol, fine_tol
      integer(c_int) :: coarse_iter, max_vcycle, pre_iter, down_iter, up_iter, post_iter, max_srj_iter, &
@@ -118,6 +124,26 @@ module wavetrisk_gpu
      integer(c_int) function wtg_remap (ctx, slot, scheme, a_vert, b_vert) bind(C)
        import; type(c_ptr), value :: ctx; integer(c_int), value :: slot, scheme
        real(c_double), intent(in) :: a_vert(*), b_vert(*)
+     end function
+     ! ---- vertical_diffusion (Kt, Kv(0:zlevels), tke(1:zlevels) and the forcing per node) ----
+     subroutine wtg_vdiff_defaults (p) bind(C)
+       import; type(wtg_vdiff), intent(out) :: p
+     end subroutine
+     integer(c_int) function wtg_set_vdiff (ctx, p) bind(C)
+       import; type(c_ptr), value :: ctx; type(wtg_vdiff), intent(in) :: p
+     end function
+     integer(c_int) function wtg_vdiff_set_coefficients (ctx, Kt, Kv, tke) bind(C)
+       import; type(c_ptr), value :: ctx; real(c_double), intent(in) :: Kt(*), Kv(*), tke(*)
+     end function
+     integer(c_int) function wtg_vdiff_get_coefficients (ctx, Kt, Kv, tke) bind(C)
+       import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: Kt(*), Kv(*), tke(*)
+     end function
+     integer(c_int) function wtg_vdiff_set_forcing (ctx, bottom_buoy_flux, top_buoy_flux, tau_mag, wind_flux) bind(C)
+       import; type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: bottom_buoy_flux(*), top_buoy_flux(*), tau_mag(*), wind_flux(*)
+     end function
+     integer(c_int) function wtg_vertical_diffusion (ctx, slot, dt, istep) bind(C)
+       import; type(c_ptr), value :: ctx; integer(c_int), value :: slot, istep; real(c_double), value :: dt
      end function
      ! ---- free surface (mode_split) ----
      integer(c_int) function wtg_fs_set_params (ctx, p) bind(C)
