@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--only-mlp", action="store_true",
                     help="run only the network's one-launch inference at 64 right-hand sides of J7 (163 840 rows)")
+    ap.add_argument("--only-columns", action="store_true",
+                    help="run only the column physics at --level/--layers: the vertical remap (PPR and remap1) and "
+                         "the vertical diffusion (TKE closure and analytic), with their CPU baselines at J6")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -96,6 +99,9 @@ def main():
         return
     if args.only_mlp:
         print(json.dumps({"mlp_forward_64_rhs": mlp_forward(64 * 10 * (2 ** args.fs_level // 8) ** 2)}))
+        return
+    if args.only_columns:
+        print(json.dumps({"columns": columns(args.level, args.layers)}))
         return
     if args.only_batched:
         from wavetrisk_hydrostatic_amd import learned as LC
@@ -735,6 +741,79 @@ def mlp_forward(rows: int, params=None):
     return {"rows": rows, "ms_per_forward": ms, "tflops": flops / (ms * 1e-3) / 1e12,
             "frac_of_bf16_dense_peak": flops / (ms * 1e-3) / 2.5e15,
             "hbm_gbs": hbm / (ms * 1e-3) / 1e9, "hbm_bound_ms": hbm / 8e12 * 1e3}
+
+
+def columns(level: int, L: int, calls: int = 10):
+    """The column physics after a step: wtg_remap (remapPPR and remap1 onto
+    Chebyshev hybrid coordinates) and wtg_vertical_diffusion (TKE closure and
+    analytic coefficients) on the ocean at `level` with L layers: mean wall
+    ms per call (each call ghost-fills before and after and synchronizes) over
+    `calls` calls after one warm-up, the algorithmic HBM rate (remap: m, t, 3 u
+    read and written plus the old-mass copy, 112 B per node-layer; vertical
+    diffusion: two passes reading m, t, 3 u, Kt, Kv, TKE and the forcing and
+    writing t, 3 u, Kt, Kv, TKE, 176 B per node-layer) and the oracle's time
+    on the same physics at J6 (one thread), scaled per node-layer"""
+    import numpy as np
+
+    import wavetrisk_hydrostatic_amd as W
+    from wavetrisk_hydrostatic_amd import cases
+    from oracle.oracle import Oracle, geometry_of
+
+    def setup(lev):
+        g = W.Grid(level=lev, min_level=lev, domain_level=2)
+        topo, m, t, u = cases.ocean(g, L, seed=11)
+        rng = np.random.default_rng(3)
+        NC = g.layout.ncells
+        forcing = dict(bot=rng.uniform(-3e-7, 3e-7, NC), top=rng.uniform(-1e-6, 1e-6, NC),
+                       tau=rng.uniform(0.02, 0.2, NC), wind=rng.uniform(-2e-4, 2e-4, (3, NC)))
+        coef = (np.full((L + 1, NC), 1.2e-5), np.full((L + 1, NC), 1.2e-4), rng.uniform(1e-6, 3e-5, (L, NC)))
+        return g, topo, m, t, u, forcing, coef
+
+    k = np.arange(L + 1)
+    b = (1.0 + np.cos(k / L * np.pi)) / 2.0
+    a = 1.0 - b
+    g, topo, m, t, u, forcing, coef = setup(level)
+    nl = 10 * g.layout.n ** 2 * L  # node-layers
+    ctx = W.Context(g, nlayers=L)
+    ctx.set_topography(topo)
+    out = {"workload": f"J{level} L{L} ocean", "calls": calls}
+
+    def timed(fn):
+        ctx.upload_state(0, m, t, u)
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / calls * 1e3
+
+    for name, scheme in (("remap_ppr", W.REMAP_PPR), ("remap_linear", W.REMAP_PIECEWISE_LINEAR)):
+        ms = timed(lambda: ctx.remap(0, scheme, a, b))
+        out[name] = {"ms_per_call": ms, "alg_gbs": nl * 112 / (ms * 1e-3) / 1e9}
+    for name, tk in (("vdiff_tke", 1), ("vdiff_analytic", 0)):
+        ctx.set_vdiff(tke_closure=tk, bottom_friction=4e-4)
+        ctx.vdiff_set_coefficients(*coef)
+        ctx.vdiff_set_forcing(forcing["bot"], forcing["top"], forcing["tau"], forcing["wind"])
+        ms = timed(lambda: ctx.vertical_diffusion(0, 900.0, 2))
+        out[name] = {"ms_per_call": ms, "alg_gbs": nl * 176 / (ms * 1e-3) / 1e9}
+    ctx.close()
+    # CPU baseline: the oracle at J6 (one thread), per node-layer scaled to the GPU workload
+    g6, topo6, m6, t6, u6, f6, c6 = setup(6)
+    nl6 = 10 * g6.layout.n ** 2 * L
+    o = Oracle(g6.layout, geometry_of(g6), topo6)
+    st = (m6.copy(), t6.copy(), u6.copy())
+    o.ghost_fill(*st)
+    t0 = time.perf_counter()
+    o.remap(*st, W.REMAP_PPR, a, b)
+    ppr6 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.vertical_diffusion(*st, 900.0, 2, f6, *c6, tke_closure=1, bottom_friction=4e-4)
+    vd6 = time.perf_counter() - t0
+    out["cpu_baseline"] = {"kind": "port", "cores": 1, "sample": f"the C oracle at J6 L{L} (one thread), scaled "
+                                                                 f"by node-layers to J{level}",
+                           "remap_ppr_ms": ppr6 * nl / nl6 * 1e3, "vdiff_tke_ms": vd6 * nl / nl6 * 1e3}
+    return out
 
 
 def free_surface_cpu(prob, dt: float = 1800.0):

@@ -38,50 +38,56 @@ struct VdArgs {
   double *kt, *kv, *tke;                        // [members][L + 1][NC], [members][L][NC]
 };
 
-// LAPACK dgtsv with one right-hand side on rows 1..n of 1-based arrays
-__device__ void dgtsv(int n, double* dl, double* d, double* du, double* b) {
-  for (int i = 1; i <= n - 2; ++i) {
-    if (fabs(d[i]) >= fabs(dl[i])) {
-      if (d[i] == 0.0) return;
-      const double fact = dl[i] / d[i];
-      d[i + 1] = d[i + 1] - fact * du[i];
-      b[i + 1] = b[i + 1] - fact * b[i];
-      dl[i] = 0.0;
-    } else {
-      const double fact = d[i] / dl[i];
-      d[i] = dl[i];
-      double temp = d[i + 1];
-      d[i + 1] = du[i] - fact * temp;
-      dl[i] = du[i + 1];
-      du[i + 1] = -fact * dl[i];
-      du[i] = temp;
-      temp = b[i];
-      b[i] = b[i + 1];
-      b[i + 1] = temp - fact * b[i + 1];
+// LAPACK dgtsv with one right-hand side on rows 1..n of 1-based arrays.  The
+// current row's eliminated diagonal and right-hand side, and the last two
+// solution values of the back substitution, ride in registers (the arrays
+// live in scratch memory, and a reload of what the previous row just stored
+// would put a memory round trip on every step of the sweep); every update is
+// also stored, so the arrays hold what the library's would, an early return
+// at a zero pivot included.
+__device__ void dgtsv(int n, double* __restrict__ dl, double* __restrict__ d, double* __restrict__ du,
+                      double* __restrict__ b) {
+  double di = d[1], bi = b[1];
+  for (int i = 1; i <= n - 1; ++i) {
+    const double dli = dl[i], dui = du[i], dn = d[i + 1], bn = b[i + 1];
+    if (fabs(di) >= fabs(dli)) {
+      if (di == 0.0) return;
+      const double fact = dli / di;
+      di = dn - fact * dui;
+      bi = bn - fact * bi;
+      d[i + 1] = di;
+      b[i + 1] = bi;
+      if (i <= n - 2) dl[i] = 0.0;
+    } else {  // interchange rows i and i + 1
+      const double fact = di / dli;
+      d[i] = dli;
+      di = dui - fact * dn;
+      d[i + 1] = di;
+      if (i <= n - 2) {
+        dl[i] = du[i + 1];
+        du[i + 1] = -fact * dl[i];
+      }
+      du[i] = dn;
+      b[i] = bn;
+      bi = bi - fact * bn;
+      b[i + 1] = bi;
     }
   }
+  if (di == 0.0) return;  // d(n)
+  double x1 = bi / di, x2 = 0.0;
+  b[n] = x1;
   if (n > 1) {
-    const int i = n - 1;
-    if (fabs(d[i]) >= fabs(dl[i])) {
-      if (d[i] == 0.0) return;
-      const double fact = dl[i] / d[i];
-      d[i + 1] = d[i + 1] - fact * du[i];
-      b[i + 1] = b[i + 1] - fact * b[i];
-    } else {
-      const double fact = d[i] / dl[i];
-      d[i] = dl[i];
-      double temp = d[i + 1];
-      d[i + 1] = du[i] - fact * temp;
-      du[i] = temp;
-      temp = b[i];
-      b[i] = b[i + 1];
-      b[i + 1] = temp - fact * b[i + 1];
-    }
+    const double x = (b[n - 1] - du[n - 1] * x1) / d[n - 1];
+    b[n - 1] = x;
+    x2 = x1;
+    x1 = x;
   }
-  if (d[n] == 0.0) return;
-  b[n] = b[n] / d[n];
-  if (n > 1) b[n - 1] = (b[n - 1] - du[n - 1] * b[n]) / d[n - 1];
-  for (int i = n - 2; i >= 1; --i) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+  for (int i = n - 2; i >= 1; --i) {
+    const double x = (b[i] - du[i] * x1 - dl[i] * x2) / d[i];
+    b[i] = x;
+    x2 = x1;
+    x1 = x;
+  }
 }
 
 // the column of thread tid: owned nodes, then the south and north pole masters
@@ -129,10 +135,13 @@ __device__ void l_scales(const VdArgs& a, double tau, TkeWork& w) {
     w.dn[l] = sqrt(2.0 * w.e[l] / fmax(w.Nsq[l], p.Neps_sq));
     w.up[l] = w.dn[l];
   }
-  w.dn[0] = p.l_0;
-  for (int l = 1; l <= L; ++l) w.dn[l] = fmin(w.dn[l], w.dn[l - 1] + w.dz[l]);
-  w.up[L] = fmax(p.l_0, p.kappa_VK * p.C_l / (a.rho0 * a.grav) * tau);
-  for (int l = L - 1; l >= 0; --l) w.up[l] = fmin(w.up[l], w.up[l + 1] + w.dz[l + 1]);
+  // the running minima carry the previous interface's value in a register
+  double prev = p.l_0;
+  w.dn[0] = prev;
+  for (int l = 1; l <= L; ++l) w.dn[l] = prev = fmin(w.dn[l], prev + w.dz[l]);
+  prev = fmax(p.l_0, p.kappa_VK * p.C_l / (a.rho0 * a.grav) * tau);
+  w.up[L] = prev;
+  for (int l = L - 1; l >= 0; --l) w.up[l] = prev = fmin(w.up[l], prev + w.dz[l + 1]);
   for (int l = 0; l <= L; ++l) {
     w.leps[l] = fmax(p.l_min, sqrt(w.up[l] * w.dn[l]));
     w.lk[l] = fmax(p.l_min, fmin(w.up[l], w.dn[l]));
@@ -287,9 +296,12 @@ __global__ __launch_bounds__(VT) void vd_solve_kernel(VdArgs a) {
   const double zs = a.topo[mb * NC + c];
   const double eta = a.eta ? a.eta[mb * NC + c] : free_surface(a, m, zs, c);
   w.z[0] = zs;
+  double z = zs;  // the interface heights' running sum rides in a register
   for (int k = 1; k <= L; ++k) {
-    w.dz[k] = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
-    w.z[k] = w.z[k - 1] + w.dz[k];
+    const double dzk = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
+    w.dz[k] = dzk;
+    z = z + dzk;
+    w.z[k] = z;
   }
   for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
 #define B(k) (t[(int64_t)((k) - 1) * NC + c] / (0.0 + m[(int64_t)((k) - 1) * NC + c]))
