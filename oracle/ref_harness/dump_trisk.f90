@@ -24,6 +24,9 @@
 ! dt = step) with position-dependent forcing hooks (vd_forcing, dumped as
 ! 'vf_*'), each followed by update_bdry and dumps 'v<n>_' of the state and of
 ! Kt, Kv (interfaces 0..zlevels) and tke ('v0_': the coefficients before).
+! vdinline (logical, with vdiff): instead, vertical_diffusion after every RK4
+! step (istep = step number, as time_step orders them, main.f90:384-420); the
+! step dumps hold the state after both, 'v9_' the coefficients at the end.
 ! ntimed > 0: after the dumps, ntimed more RK4 steps timed by the wall clock
 ! (system_clock) and printed as "RK4 <ms> ms/step" (the CPU reference's cost).
 ! lsclr, ldivu, lrotu (0, 1, 2) and vsclr(2), vdivu, vrotu: Laplacian
@@ -291,14 +294,15 @@ program dump_trisk
   real(8)        :: step, vsclr(2), vdivu, vrotu
   logical        :: nanfill, uniform_first
   integer        :: vdiff, id
+  logical        :: vdinline
   real(8)        :: lon, lat
   character(16)  :: vremap, coords
   namelist /run/ case, zlev, nsteps, step, out, nanfill, ntimed, lsclr, ldivu, lrotu, vsclr, vdivu, vrotu, every, &
-                 vremap, coords, uniform_first, vdiff
+                 vremap, coords, uniform_first, vdiff, vdinline
 
   case = 'tc5'; zlev = 1; nsteps = 0; step = 0d0; out = 'dump.bin'; nanfill = .false.; ntimed = 0; every = 1
   lsclr = 0; ldivu = 0; lrotu = 0; vsclr = 0d0; vdivu = 0d0; vrotu = 0d0; vremap = ''; coords = 'uniform'
-  uniform_first = .false.; vdiff = 0
+  uniform_first = .false.; vdiff = 0; vdinline = .false.
   read (5, nml=run)
   ic_case = case
 
@@ -399,9 +403,18 @@ program dump_trisk
   call trend_ml (sol(:,1:zlevels), trend(:,1:zlevels))
   call dump_trend ('t0_')
   call dump_work
+  if (vdiff > 0 .and. vdinline) then
+     call init_vdiff
+     dt = step
+  end if
   do n = 1, nsteps
      call RK4 (sol(:,1:zlevels), wav_coeff(:,1:zlevels), trend_ml, step)
      call update_bdry (sol(:,1:zlevels), NONE, 1)
+     if (vdiff > 0 .and. vdinline) then  ! time_step's order: the dynamics, then the vertical diffusion
+        istep = n
+        call vertical_diffusion
+        call update_bdry (sol(:,1:zlevels), NONE, 1)
+     end if
      if (mod(n, every) == 0) &
           call dump_state ('s'//achar(48+(n/every)/10)//achar(48+mod(n/every,10))//'_')
   end do
@@ -431,22 +444,9 @@ program dump_trisk
      call update_bdry (sol(:,1:zlevels), NONE, 1)
      call dump_state ('r0_')
   end if
-  if (vdiff > 0) then
-     ! initial TKE (Kt and Kv are recomputed from it at istep 1) and the forcing
-     do d = 1, size(grid)
-        do id = 1, grid(d)%node%length
-           call cart2sph (grid(d)%node%elts(id), lon, lat)
-           do k = 1, zlevels
-              tke(k)%data(d)%elts(id) = 1d-5 * (1d0 + 0.5d0 * sin (3d0*lon + dble(k)) * cos (lat))
-           end do
-           do k = 0, zlevels
-              Kt(k)%data(d)%elts(id) = Kt_min
-              Kv(k)%data(d)%elts(id) = Kv_min
-           end do
-        end do
-        call dump_forcing (d)
-     end do
-     call dump_coefficients ('v0_')
+  if (vdiff > 0 .and. vdinline) call dump_coefficients ('v9_')
+  if (vdiff > 0 .and. .not. vdinline) then
+     call init_vdiff
      dt = step
      do n = 1, 2
         istep = n
@@ -572,6 +572,25 @@ contains
        end do
     end do
   end subroutine dump_state
+
+  subroutine init_vdiff
+    ! initial TKE (Kt and Kv are recomputed from it at istep 1) and the forcing
+    integer :: dd, ii, kk
+    do dd = 1, size(grid)
+       do ii = 1, grid(dd)%node%length
+          call cart2sph (grid(dd)%node%elts(ii), lon, lat)
+          do kk = 1, zlevels
+             tke(kk)%data(dd)%elts(ii) = 1d-5 * (1d0 + 0.5d0 * sin (3d0*lon + dble(kk)) * cos (lat))
+          end do
+          do kk = 0, zlevels
+             Kt(kk)%data(dd)%elts(ii) = Kt_min
+             Kv(kk)%data(dd)%elts(ii) = Kv_min
+          end do
+       end do
+       call dump_forcing (dd)
+    end do
+    call dump_coefficients ('v0_')
+  end subroutine init_vdiff
 
   subroutine dump_coefficients (pre)
     character(*) :: pre

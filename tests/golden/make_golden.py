@@ -229,7 +229,7 @@ def grid_fixture(J, geometry=True):
     print(f"{name}.npz written")
 
 
-def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, vdiff=0):
+def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, vdiff=0, vdinline=False):
     """trisk_<name>.npz: the reference's initial state, trend and the state
     after each RK4 step (after every `every`-th one: mass<k> is step k*every);
     remap = (scheme, coords): also the state after remap_vertical_coordinates
@@ -240,7 +240,7 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, 
     param, ml, dl = PARAMS[J]
     extra = f", every={every}" if every > 1 else ""
     if vdiff:
-        extra += f", vdiff={vdiff}"
+        extra += f", vdiff={vdiff}" + (", vdinline=.true." if vdinline else "")
     if remap:
         extra += f", vremap='{remap[0]}', coords='{remap[1]}'"
         if len(remap) > 2:  # a piecewise-constant remap onto uniform coordinates first
@@ -322,7 +322,10 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, 
                     vf_tau=field("vf_tau", 1, False, 1)[0], vf_wind=field("vf_wind", 3, False, 1)[0],
                     vd0_tke=field("v0_tke", 1, False), vd0_kt=field("v0_kt", 1, False, L + 1),
                     vd0_kv=field("v0_kv", 1, False, L + 1))
-        for c in (1, 2):
+        if vdinline:  # every step: RK4 then vertical_diffusion; the coefficients at the end
+            data.update(vd9_kt=field("v9_kt", 1, False, L + 1), vd9_kv=field("v9_kv", 1, False, L + 1),
+                        vd9_tke=field("v9_tke", 1, False))
+        for c in (() if vdinline else (1, 2)):
             for v, m in (("mass", 1), ("temp", 1), ("velo", 3)):
                 data[f"vd{c}_{v}"] = field(f"v{c}_{v}", m, False)
             data[f"vd{c}_kt"] = field(f"v{c}_kt", 1, False, L + 1)
@@ -331,6 +334,8 @@ def trisk_fixture(name, J, case, L, nsteps, dt, diff=None, every=1, remap=None, 
         keep = ("level", "min_level", "domain_level", "nlayers", "dt", "nsteps", "every", "radius", "omega", "grav",
                 "ref_density", "topo", f"mass{nsteps}", f"temp{nsteps}", f"velo{nsteps}", "vdiff", "vf_bot", "vf_top",
                 "vf_tau", "vf_wind") + tuple(k for k in data if k.startswith("vd"))
+        if vdinline:
+            keep += ("mass0", "temp0", "velo0")
         data = {k: v for k, v in data.items() if k in keep}
     if diff:
         data.update(laplace=np.array([diff["lsclr"], diff["ldivu"], diff["lrotu"]]),
@@ -705,6 +710,8 @@ def main():
     if "vdiff" in what:  # vertical_diffusion (analytic and TKE closure) after one RK4 step of the J4 ocean
         trisk_fixture("ocean_vdiff_analytic_J4", 4, "ocean", 6, 1, 1200.0, vdiff=1)
         trisk_fixture("ocean_vdiff_tke_J4", 4, "ocean", 6, 1, 1200.0, vdiff=2)
+    if "vdsteps" in what:  # three steps of RK4 then vertical_diffusion (TKE closure), as time_step orders them
+        trisk_fixture("ocean_vdiff_steps_J4", 4, "ocean", 6, 3, 1200.0, vdiff=2, vdinline=True)
     if "c1" in what:  # the CPU-runnable case: J5 TC5, one layer, 100 RK4 steps at CFL 0.8
         trisk_fixture("tc5_c1_J5", 5, "tc5", 1, 100, c1_dt(), every=25)
     if "elliptic" in what:

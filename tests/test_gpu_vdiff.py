@@ -170,3 +170,47 @@ def test_vertical_diffusion_refusals():
         c.set_vdiff(tke_closure=0, mode_split=1)
         c.vertical_diffusion(0, 600.0, 1)  # no free surface to take eta from
     c.close()
+
+
+def test_rk4_then_vertical_diffusion_steps():
+    """three steps of wtg_rk4 then wtg_vertical_diffusion (TKE closure) in
+    time_step's order: equal to the oracle's same sequence bit for bit
+    everywhere, and to the reference (trisk_ocean_vdiff_steps_J4) within 1e-15
+    equatorward of 65 degrees (the reference's unreconciled per-lozenge pole
+    copies part under the vertical diffusion; this build keeps one pole value)"""
+    print("RK4 then vertical diffusion, three steps, J4 L6", flush=True)
+    from helpers import oracle_of
+    from test_vdiff_oracle import polar_caps_excluded
+    f = golden("trisk_ocean_vdiff_steps_J4.npz")
+    g = grid_of_fixture(f)
+    L, dt = int(f["nlayers"]), float(f["dt"])
+    st0 = tuple(np.nan_to_num(f[f"{v}0"]) for v in ("mass", "temp", "velo"))
+    coef0 = tuple(np.nan_to_num(f[k]) for k in ("vd0_kt", "vd0_kv", "vd0_tke"))
+    forcing = dict(bot=np.nan_to_num(f["vf_bot"]), top=np.nan_to_num(f["vf_top"]), tau=np.nan_to_num(f["vf_tau"]),
+                   wind=np.nan_to_num(f["vf_wind"]))
+    ctx = W.Context(g, nlayers=L, grav=float(f["grav"]), ref_density=float(f["ref_density"]))
+    ctx.set_topography(np.nan_to_num(f["topo"]))
+    ctx.upload_state(0, *st0)
+    ctx.set_vdiff(tke_closure=1, bottom_friction=4e-4)
+    ctx.vdiff_set_coefficients(*coef0)
+    ctx.vdiff_set_forcing(forcing["bot"], forcing["top"], forcing["tau"], forcing["wind"])
+    for n in (1, 2, 3):
+        ctx.rk4(0, dt, 1)
+        ctx.vertical_diffusion(0, dt, n)
+    got = ctx.download_state(0) + ctx.vdiff_coefficients()
+    ctx.close()
+    o = oracle_of(g, np.nan_to_num(f["topo"]), f)
+    st = tuple(x.copy() for x in st0)
+    o.ghost_fill(*st)
+    coef = coef0
+    for n in (1, 2, 3):
+        st = o.rk4(*st, dt, 1)
+        res = o.vertical_diffusion(*st, dt, n, forcing, *coef, tke_closure=1, bottom_friction=4e-4)
+        st, coef = res[:3], res[3:]
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    assert np.array_equal(got[0][:, oc], res[0][:, oc]) and np.array_equal(got[1][:, oc], res[1][:, oc])
+    assert np.array_equal(got[2][:, :, oe], res[2][:, :, oe])
+    assert all(np.array_equal(got[k][:, oc], res[k][:, oc]) for k in (3, 4, 5))
+    oc6, oe6 = polar_caps_excluded(g, oc), polar_caps_excluded(g, oe)
+    assert maxrel(got[1][:, oc6], f["temp3"][:, oc6]) <= 1e-15
+    assert maxrel(got[2][:, :, oe6], f["velo3"][:, :, oe6]) <= 1e-15
