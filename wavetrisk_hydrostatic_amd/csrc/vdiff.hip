@@ -272,8 +272,133 @@ __device__ __forceinline__ double irradiance(const wtg_vdiff& p, double depth) {
 }
 
 struct SolveWork {
-  double z[VMAX + 2], dz[VMAX + 2], dzl[VMAX + 2], dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2], r[VMAX + 2];
+  double dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2], r[VMAX + 2];  // the rows, built only for the full dgtsv
+  double Dp[VMAX + 2], Up[VMAX + 2], X[VMAX + 2];                // the elimination's diagonal, super-diagonal, rhs
 };
+
+// Rows of the backward-Euler systems, generated in order k = 1..L with the
+// previous and next layer thicknesses carried in registers (each is the same
+// expression as the reference's array entry, so the rows are identical).
+// Buoyancy (backwards_euler_temp): e < 0; an edge's velocity: e = 0, 1, 2.
+struct Rows {
+  const VdArgs& a;
+  const double *m, *t, *u, *K, *f;
+  int64_t c, nb;
+  int e;
+  double eta, dt;
+  int k;
+  double dzp, dzc, dzn, z0, z1;  // dz(k-1), dz(k), dz(k+1); z(k-1), z(k)
+  __device__ double dz(int kk) const {
+    const int64_t x = (int64_t)(kk - 1) * a.NC;
+    if (e < 0) return (0.0 + m[x + c]) / (a.rho0 * 1.0);
+    return 0.5 * ((0.0 + m[x + c]) / (a.rho0 * 1.0) + (0.0 + m[x + nb]) / (a.rho0 * 1.0));
+  }
+  __device__ void reset(double zs) {
+    k = 0;
+    dzc = 0.0;
+    dzn = dz(1);
+    z1 = zs;
+  }
+  // row k + 1: sub-diagonal lo (k + 1 >= 2), diagonal dg, super-diagonal up (k + 1 <= L - 1), rhs r
+  __device__ void next(double& lo, double& dg, double& up, double& r) {
+    const int L = a.L;
+    const wtg_vdiff& p = a.p;
+    ++k;
+    dzp = dzc;
+    dzc = dzn;
+    dzn = k < L ? dz(k + 1) : 0.0;
+    z0 = z1;
+    z1 = z0 + dzc;
+    const double dzl_lo = 0.5 * (dzp + dzc), dzl_up = 0.5 * (dzc + dzn);
+    lo = k >= 2 ? -(dt * K[(int64_t)(k - 1) * a.NC + c] / (dzl_lo * dzc)) : 0.0;
+    up = k <= L - 1 ? -(dt * K[(int64_t)k * a.NC + c] / (dzl_up * dzc)) : 0.0;
+    if (e < 0) {
+      const int64_t x = (int64_t)(k - 1) * a.NC + c;
+      const double b = t[x] / (0.0 + m[x]);
+      if (k == 1) {
+        dg = 1.0 - up;
+        r = b + dt * (-f[c] + solar()) / dzc;
+      } else if (k < L) {
+        dg = 1.0 - (up + lo);
+        r = b + dt * solar() / dzc;
+      } else {
+        // the reference's top row takes the row below's super-diagonal entry
+        dg = 1.0 - (-(dt * K[(int64_t)(k - 1) * a.NC + c] / (dzl_lo * dzp)));
+        r = b + dt * (f[a.NC + c] + p.Q_sr / (a.rho0 * p.c_p)) / dzc;
+      }
+    } else {
+      r = u[((int64_t)(k - 1) * 3 + e) * a.NC + c];
+      if (k == 1)
+        dg = 1.0 - up + dt * p.bottom_friction / dzc;
+      else if (k < L)
+        dg = 1.0 - (up + lo);
+      else
+        dg = 1.0 - lo;
+      if (k == L) r = r + dt * f[(3 + e) * a.NC + c] / dzc;
+    }
+  }
+  __device__ double solar() const {
+    const wtg_vdiff& p = a.p;
+    return p.Q_sr * (irradiance(p, eta - z1) - irradiance(p, eta - z0)) / (a.rho0 * p.c_p);
+  }
+};
+
+// Solves the rows with dgtsv's arithmetic: while no row interchange or zero
+// pivot occurs, the elimination keeps only its diagonal, super-diagonal and
+// right-hand side (what dgtsv's no-interchange branch computes, the zeroed
+// sub-diagonal entering the back substitution as 0.0 * x); otherwise the
+// rows are built and dgtsv runs in full.  The solution is left in X[1..n].
+__device__ void solve_rows(int n, Rows& rows, double zs, SolveWork& w) {
+  double lo, dg, up, r;
+  rows.reset(zs);
+  rows.next(lo, dg, up, r);
+  double di = dg, xi = r, ui = up;
+  w.Dp[1] = di;
+  w.X[1] = xi;
+  w.Up[1] = ui;
+  bool fast = true;
+  for (int i = 1; i <= n - 1; ++i) {
+    rows.next(lo, dg, up, r);  // row i + 1: lo is dgtsv's dl(i)
+    if (!(fabs(di) >= fabs(lo)) || di == 0.0) {
+      fast = false;
+      break;
+    }
+    const double fact = lo / di;
+    di = dg - fact * ui;
+    xi = r - fact * xi;
+    ui = up;
+    w.Dp[i + 1] = di;
+    w.X[i + 1] = xi;
+    w.Up[i + 1] = ui;
+  }
+  if (fast && di != 0.0) {
+    double x1 = xi / di, x2 = 0.0;
+    w.X[n] = x1;
+    if (n > 1) {
+      const double x = (w.X[n - 1] - w.Up[n - 1] * x1) / w.Dp[n - 1];
+      w.X[n - 1] = x;
+      x2 = x1;
+      x1 = x;
+    }
+    for (int i = n - 2; i >= 1; --i) {
+      const double x = (w.X[i] - w.Up[i] * x1 - 0.0 * x2) / w.Dp[i];
+      w.X[i] = x;
+      x2 = x1;
+      x1 = x;
+    }
+    return;
+  }
+  rows.reset(zs);
+  for (int k = 1; k <= n; ++k) {
+    rows.next(lo, dg, up, r);
+    if (k >= 2) w.dl[k - 1] = lo;
+    w.dg[k] = dg;
+    if (k <= n - 1) w.du[k] = up;
+    w.r[k] = r;
+  }
+  dgtsv(n, w.dl, w.dg, w.du, w.r);
+  for (int k = 1; k <= n; ++k) w.X[k] = w.r[k];
+}
 
 // backwards_euler_temp of every owned node and pole master, then
 // backwards_euler_velo of the owned nodes' edges
@@ -287,78 +412,22 @@ __global__ __launch_bounds__(VT) void vd_solve_kernel(VdArgs a) {
   const double* m = a.m + mb * L * NC;
   double* t = a.t + mb * L * NC;
   double* u = a.u + mb * 3 * L * NC;
-  const double* kt = a.kt + mb * (L + 1) * NC;
-  const double* kv = a.kv + mb * (L + 1) * NC;
   const double* f = a.force + mb * 6 * NC;
-  const wtg_vdiff& p = a.p;
-  const double dt = a.dt;
   SolveWork w;
   const double zs = a.topo[mb * NC + c];
   const double eta = a.eta ? a.eta[mb * NC + c] : free_surface(a, m, zs, c);
-  w.z[0] = zs;
-  double z = zs;  // the interface heights' running sum rides in a register
-  for (int k = 1; k <= L; ++k) {
-    const double dzk = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
-    w.dz[k] = dzk;
-    z = z + dzk;
-    w.z[k] = z;
-  }
-  for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
-#define B(k) (t[(int64_t)((k) - 1) * NC + c] / (0.0 + m[(int64_t)((k) - 1) * NC + c]))
-#define SOLAR(k) (p.Q_sr * (irradiance(p, eta - w.z[k]) - irradiance(p, eta - w.z[(k) - 1])) / (a.rho0 * p.c_p))
-#define CT(k, kk) (dt * kt[(int64_t)(kk) * NC + c] / (w.dzl[kk] * w.dz[k]))
-  {
-    int k = 1;
-    w.du[k] = -CT(k, k);
-    w.dg[k] = 1.0 - w.du[k];
-    w.r[k] = B(k) + dt * (-f[c] + SOLAR(k)) / w.dz[k];
-    for (k = 2; k <= L - 1; ++k) {
-      w.du[k] = -CT(k, k);
-      w.dl[k - 1] = -CT(k, k - 1);
-      w.dg[k] = 1.0 - (w.du[k] + w.dl[k - 1]);
-      w.r[k] = B(k) + dt * SOLAR(k) / w.dz[k];
-    }
-    k = L;
-    w.dl[k - 1] = -CT(k, k - 1);
-    w.dg[k] = 1.0 - w.du[k - 1];  // the reference's top row takes the row below's super-diagonal entry
-    w.r[k] = B(k) + dt * (f[NC + c] + p.Q_sr / (a.rho0 * p.c_p)) / w.dz[k];
-  }
-#undef B
-#undef SOLAR
-#undef CT
-  dgtsv(L, w.dl, w.dg, w.du, w.r);
+  Rows rows{a, m, t, u, a.kt + mb * (L + 1) * NC, f, c, c, -1, eta, a.dt};
+  solve_rows(L, rows, zs, w);
   for (int k = 1; k <= L; ++k) {
     const int64_t x = (int64_t)(k - 1) * NC + c;
-    t[x] = (0.0 + m[x]) * w.r[k];
+    t[x] = (0.0 + m[x]) * w.X[k];
   }
   if (pole) return;  // the poles carry no edges of their own
   const int64_t nb[3] = {c + 1, c + a.S + 1, c + a.S};
   for (int e = 0; e < 3; ++e) {
-    for (int k = 1; k <= L; ++k) {
-      const int64_t x = (int64_t)(k - 1) * NC;
-      w.dz[k] = 0.5 * ((0.0 + m[x + c]) / (a.rho0 * 1.0) + (0.0 + m[x + nb[e]]) / (a.rho0 * 1.0));
-    }
-    for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
-#define CV(k, kk) (dt * kv[(int64_t)(kk) * NC + c] / (w.dzl[kk] * w.dz[k]))
-#define UV(k) u[((int64_t)((k) - 1) * 3 + e) * NC + c]
-    int k = 1;
-    w.du[k] = -CV(k, k);
-    w.dg[k] = 1.0 - w.du[k] + dt * p.bottom_friction / w.dz[k];
-    w.r[k] = UV(k);
-    for (k = 2; k <= L - 1; ++k) {
-      w.du[k] = -CV(k, k);
-      w.dl[k - 1] = -CV(k, k - 1);
-      w.dg[k] = 1.0 - (w.du[k] + w.dl[k - 1]);
-      w.r[k] = UV(k);
-    }
-    k = L;
-    w.dl[k - 1] = -CV(k, k - 1);
-    w.dg[k] = 1.0 - w.dl[k - 1];
-    w.r[k] = UV(k) + dt * f[(3 + e) * NC + c] / w.dz[k];
-    dgtsv(L, w.dl, w.dg, w.du, w.r);
-    for (k = 1; k <= L; ++k) UV(k) = w.r[k];
-#undef CV
-#undef UV
+    Rows ru{a, m, t, u, a.kv + mb * (L + 1) * NC, f, c, nb[e], e, eta, a.dt};
+    solve_rows(L, ru, zs, w);
+    for (int k = 1; k <= L; ++k) u[((int64_t)(k - 1) * 3 + e) * NC + c] = w.X[k];
   }
 }
 
