@@ -90,6 +90,70 @@ __device__ void dgtsv(int n, double* __restrict__ dl, double* __restrict__ d, do
   }
 }
 
+struct SolveWork {
+  double dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2], r[VMAX + 2];  // the rows, built only for the full dgtsv
+  double Dp[VMAX + 2], Up[VMAX + 2], X[VMAX + 2];                // the elimination's diagonal, super-diagonal, rhs
+};
+
+// Solves the rows with dgtsv's arithmetic: while no row interchange or zero
+// pivot occurs, the elimination keeps only its diagonal, super-diagonal and
+// right-hand side (what dgtsv's no-interchange branch computes, the zeroed
+// sub-diagonal entering the back substitution as 0.0 * x); otherwise the
+// rows are built and dgtsv runs in full.  The solution is left in X[1..n].
+// Rows gives reset(zs) and next(lo, dg, up, r), row after row from 1.
+template <class Rows>
+__device__ void solve_rows(int n, Rows& rows, double zs, SolveWork& w) {
+  double lo, dg, up, r;
+  rows.reset(zs);
+  rows.next(lo, dg, up, r);
+  double di = dg, xi = r, ui = up;
+  w.Dp[1] = di;
+  w.X[1] = xi;
+  w.Up[1] = ui;
+  bool fast = true;
+  for (int i = 1; i <= n - 1; ++i) {
+    rows.next(lo, dg, up, r);  // row i + 1: lo is dgtsv's dl(i)
+    if (!(fabs(di) >= fabs(lo)) || di == 0.0) {
+      fast = false;
+      break;
+    }
+    const double fact = lo / di;
+    di = dg - fact * ui;
+    xi = r - fact * xi;
+    ui = up;
+    w.Dp[i + 1] = di;
+    w.X[i + 1] = xi;
+    w.Up[i + 1] = ui;
+  }
+  if (fast && di != 0.0) {
+    double x1 = xi / di, x2 = 0.0;
+    w.X[n] = x1;
+    if (n > 1) {
+      const double x = (w.X[n - 1] - w.Up[n - 1] * x1) / w.Dp[n - 1];
+      w.X[n - 1] = x;
+      x2 = x1;
+      x1 = x;
+    }
+    for (int i = n - 2; i >= 1; --i) {
+      const double x = (w.X[i] - w.Up[i] * x1 - 0.0 * x2) / w.Dp[i];
+      w.X[i] = x;
+      x2 = x1;
+      x1 = x;
+    }
+    return;
+  }
+  rows.reset(zs);
+  for (int k = 1; k <= n; ++k) {
+    rows.next(lo, dg, up, r);
+    if (k >= 2) w.dl[k - 1] = lo;
+    w.dg[k] = dg;
+    if (k <= n - 1) w.du[k] = up;
+    w.r[k] = r;
+  }
+  dgtsv(n, w.dl, w.dg, w.du, w.r);
+  for (int k = 1; k <= n; ++k) w.X[k] = w.r[k];
+}
+
 // the column of thread tid: owned nodes, then the south and north pole masters
 __device__ __forceinline__ int64_t column(const VdArgs& a, int64_t tid, bool& pole) {
   const int64_t nown = (int64_t)10 * a.n * a.n;
@@ -122,9 +186,8 @@ __device__ __forceinline__ double free_surface(const VdArgs& a, const double* m,
 }
 
 struct TkeWork {
-  double dz[VMAX + 2], Umag[VMAX + 2], dzl[VMAX + 2], dU2[VMAX + 2], Nsq[VMAX + 2], e[VMAX + 2];
-  double leps[VMAX + 2], lk[VMAX + 2], S1[VMAX + 2], S2[VMAX + 2], dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2];
-  double rhs[VMAX + 2], dn[VMAX + 2], up[VMAX + 2];
+  double dz[VMAX + 2], dU2[VMAX + 2], Nsq[VMAX + 2], e[VMAX + 2], leps[VMAX + 2], lk[VMAX + 2];
+  double dn[VMAX + 2], up[VMAX + 2];
 };
 
 // l_scales (vert_diffusion.f90:502-536) on [0..L]
@@ -147,6 +210,44 @@ __device__ void l_scales(const VdArgs& a, double tau, TkeWork& w) {
     w.lk[l] = fmax(p.l_min, fmin(w.up[l], w.dn[l]));
   }
 }
+
+// Rows 1..L-1 of the TKE system (turbulent_diffusion), generated in order:
+// the sources S1, S2 and the coefficients (coeff reads dzl at the row's own
+// l) with the expressions of the reference's arrays
+struct TkeRows {
+  const VdArgs& a;
+  const double *kv, *kt;
+  int64_t c;
+  const TkeWork& w;
+  int l;
+  __device__ void reset(double) { l = 0; }
+  __device__ void next(double& lo, double& dg, double& up, double& r) {
+    const wtg_vdiff& p = a.p;
+    const int64_t NC = a.NC;
+    const int last = a.L - 1;
+    const double dt = a.dt;
+    ++l;
+    const double KV = kv[(int64_t)l * NC + c], KT = kt[(int64_t)l * NC + c];
+    const double turb = w.e[l] == 0.0 ? 0.0 : p.C_eps * sqrt(w.e[l]) / w.leps[l];
+    double S1 = KV * w.dU2[l] - KT * w.Nsq[l], S2;
+    if (p.patankar && S1 <= 0.0) {
+      S1 = KV * w.dU2[l];
+      S2 = -turb - KT * w.Nsq[l] / w.e[l];
+    } else {
+      S2 = -turb;
+    }
+    const double dzl = 0.5 * (w.dz[l] + w.dz[l + 1]);
+    up = l <= last - 1 ? -(dt * p.C_e * (0.5 * (KV + kv[(int64_t)(l + 1) * NC + c])) / (dzl * w.dz[l + 1])) : 0.0;
+    lo = l >= 2 ? -(dt * p.C_e * (0.5 * (KV + kv[(int64_t)(l - 1) * NC + c])) / (dzl * w.dz[l])) : 0.0;
+    if (l == 1)
+      dg = 1.0 - up - dt * S2;
+    else if (l < last)
+      dg = 1.0 - (up + lo) - dt * S2;
+    else
+      dg = 1.0 - lo - dt * S2;
+    r = w.e[l] + dt * S1;
+  }
+};
 
 // turbulent_diffusion of every owned node and pole master
 __global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
@@ -181,6 +282,7 @@ __global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
   TkeWork w;
   // init_diffuse: dz and |u| (twice the TRiSK kinetic energy) per layer
   const int64_t W = c - 1, SW = c - a.S - 1, Sx = c - a.S;
+  double umag = 0.0;  // |u| of the layer below, in a register
   for (int k = 1; k <= L; ++k) {
     w.dz[k] = (0.0 + m[(int64_t)(k - 1) * NC + c]) / (a.rho0 * 1.0);
     const double* uk = u + (int64_t)(k - 1) * 3 * NC;
@@ -191,17 +293,15 @@ __global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
     const double dRTW = uk[W] * a.ped[W], dDGSW = uk[NC + SW] * a.ped[NC + SW], dUPS = uk[2 * NC + Sx] * a.ped[2 * NC + Sx];
     const double ke =
         (pUP * dUP + pDG * dDG + pRT * dRT + pUPS * dUPS + pDGSW * dDGSW + pRTW * dRTW) * a.hinv[c] / 4.0;
-    w.Umag[k] = sqrt(2.0 * ke);
-  }
-  for (int l = 1; l <= L - 1; ++l) w.dzl[l] = 0.5 * (w.dz[l] + w.dz[l + 1]);
-  for (int l = 0; l <= L; ++l) {
-    if (l == 0 || l == L) {
-      w.dU2[l] = 0.0;
-    } else {
-      const double q = (w.Umag[l + 1] - w.Umag[l]) / w.dzl[l];
-      w.dU2[l] = q * q;
+    const double um = sqrt(2.0 * ke);
+    if (k >= 2) {  // dUdZ_sq at the interface below layer k
+      const double q = (um - umag) / (0.5 * (w.dz[k - 1] + w.dz[k]));
+      w.dU2[k - 1] = q * q;
     }
+    umag = um;
   }
+  w.dU2[0] = 0.0;
+  w.dU2[L] = 0.0;
   for (int l = 0; l <= L; ++l) {
     const int k = l == 0 ? 1 : l == L ? L - 1 : l;  // N_sq at the interface above layer k
     const double bb = t[(int64_t)(k - 1) * NC + c] / (0.0 + m[(int64_t)(k - 1) * NC + c]);
@@ -218,43 +318,12 @@ __global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
       kv[(int64_t)l * NC + c] = v;
       kt[(int64_t)l * NC + c] = kt_tke(p, v, w.Nsq[l], Ri);
     }
-  for (int l = 1; l <= L - 1; ++l) {
-    const double KV = kv[(int64_t)l * NC + c], KT = kt[(int64_t)l * NC + c];
-    const double turb = w.e[l] == 0.0 ? 0.0 : p.C_eps * sqrt(w.e[l]) / w.leps[l];
-    w.S1[l] = KV * w.dU2[l] - KT * w.Nsq[l];
-    if (p.patankar && w.S1[l] <= 0.0) {
-      w.S1[l] = KV * w.dU2[l];
-      w.S2[l] = -turb - KT * w.Nsq[l] / w.e[l];
-    } else {
-      w.S2[l] = -turb;
-    }
-  }
-  // the TKE rows 1..L-1 (coeff reads dzl at the row's own l)
-  const double dt = a.dt;
-#define KVI(l) kv[(int64_t)(l) * NC + c]
-#define COEFF(l, dzv, kvv) (dt * p.C_e * (kvv) / (w.dzl[l] * (dzv)))
-  {
-    int l = 1;
-    w.du[l] = -COEFF(l, w.dz[l + 1], 0.5 * (KVI(l) + KVI(l + 1)));
-    w.dg[l] = 1.0 - w.du[l] - dt * w.S2[l];
-    w.rhs[l] = w.e[l] + dt * w.S1[l];
-    for (l = 2; l <= L - 2; ++l) {
-      w.du[l] = -COEFF(l, w.dz[l + 1], 0.5 * (KVI(l) + KVI(l + 1)));
-      w.dl[l - 1] = -COEFF(l, w.dz[l], 0.5 * (KVI(l) + KVI(l - 1)));
-      w.dg[l] = 1.0 - (w.du[l] + w.dl[l - 1]) - dt * w.S2[l];
-      w.rhs[l] = w.e[l] + dt * w.S1[l];
-    }
-    l = L - 1;
-    w.dl[l - 1] = -COEFF(l, w.dz[l], 0.5 * (KVI(l) + KVI(l - 1)));
-    w.dg[l] = 1.0 - w.dl[l - 1] - dt * w.S2[l];
-    w.rhs[l] = w.e[l] + dt * w.S1[l];
-  }
-#undef COEFF
-#undef KVI
-  // rows 1..L-1 of the library are entries 1..L-1 of dg / rhs and 1..L-2 of dl / du
-  dgtsv(L - 1, w.dl, w.dg, w.du, w.rhs);
+  // the TKE system's backward-Euler step
+  SolveWork sw;
+  TkeRows rows{a, kv, kt, c, w, 0};
+  solve_rows(L - 1, rows, 0.0, sw);
   w.e[0] = p.e_0;
-  for (int l = 1; l <= L - 1; ++l) w.e[l] = p.patankar ? w.rhs[l] : fmax(w.rhs[l], p.e_min);
+  for (int l = 1; l <= L - 1; ++l) w.e[l] = p.patankar ? sw.X[l] : fmax(sw.X[l], p.e_min);
   w.e[L] = fmax(p.C_srf * tau / a.rho0, p.e_min_srf);
   // update_Kv_Kt
   l_scales(a, tau, w);
@@ -270,11 +339,6 @@ __global__ __launch_bounds__(VT) void vd_turbulent_kernel(VdArgs a) {
 __device__ __forceinline__ double irradiance(const wtg_vdiff& p, double depth) {
   return p.Q_sr * (0.58 * exp(-depth / 0.35) + (1.0 - 0.58) * exp(-depth / 23.0));
 }
-
-struct SolveWork {
-  double dg[VMAX + 2], dl[VMAX + 2], du[VMAX + 2], r[VMAX + 2];  // the rows, built only for the full dgtsv
-  double Dp[VMAX + 2], Up[VMAX + 2], X[VMAX + 2];                // the elimination's diagonal, super-diagonal, rhs
-};
 
 // Rows of the backward-Euler systems, generated in order k = 1..L with the
 // previous and next layer thicknesses carried in registers (each is the same
@@ -342,63 +406,6 @@ struct Rows {
     return p.Q_sr * (irradiance(p, eta - z1) - irradiance(p, eta - z0)) / (a.rho0 * p.c_p);
   }
 };
-
-// Solves the rows with dgtsv's arithmetic: while no row interchange or zero
-// pivot occurs, the elimination keeps only its diagonal, super-diagonal and
-// right-hand side (what dgtsv's no-interchange branch computes, the zeroed
-// sub-diagonal entering the back substitution as 0.0 * x); otherwise the
-// rows are built and dgtsv runs in full.  The solution is left in X[1..n].
-__device__ void solve_rows(int n, Rows& rows, double zs, SolveWork& w) {
-  double lo, dg, up, r;
-  rows.reset(zs);
-  rows.next(lo, dg, up, r);
-  double di = dg, xi = r, ui = up;
-  w.Dp[1] = di;
-  w.X[1] = xi;
-  w.Up[1] = ui;
-  bool fast = true;
-  for (int i = 1; i <= n - 1; ++i) {
-    rows.next(lo, dg, up, r);  // row i + 1: lo is dgtsv's dl(i)
-    if (!(fabs(di) >= fabs(lo)) || di == 0.0) {
-      fast = false;
-      break;
-    }
-    const double fact = lo / di;
-    di = dg - fact * ui;
-    xi = r - fact * xi;
-    ui = up;
-    w.Dp[i + 1] = di;
-    w.X[i + 1] = xi;
-    w.Up[i + 1] = ui;
-  }
-  if (fast && di != 0.0) {
-    double x1 = xi / di, x2 = 0.0;
-    w.X[n] = x1;
-    if (n > 1) {
-      const double x = (w.X[n - 1] - w.Up[n - 1] * x1) / w.Dp[n - 1];
-      w.X[n - 1] = x;
-      x2 = x1;
-      x1 = x;
-    }
-    for (int i = n - 2; i >= 1; --i) {
-      const double x = (w.X[i] - w.Up[i] * x1 - 0.0 * x2) / w.Dp[i];
-      w.X[i] = x;
-      x2 = x1;
-      x1 = x;
-    }
-    return;
-  }
-  rows.reset(zs);
-  for (int k = 1; k <= n; ++k) {
-    rows.next(lo, dg, up, r);
-    if (k >= 2) w.dl[k - 1] = lo;
-    w.dg[k] = dg;
-    if (k <= n - 1) w.du[k] = up;
-    w.r[k] = r;
-  }
-  dgtsv(n, w.dl, w.dg, w.du, w.r);
-  for (int k = 1; k <= n; ++k) w.X[k] = w.r[k];
-}
 
 // backwards_euler_temp of every owned node and pole master, then
 // backwards_euler_velo of the owned nodes' edges
