@@ -214,3 +214,40 @@ def test_rk4_then_vertical_diffusion_steps():
     oc6, oe6 = polar_caps_excluded(g, oc), polar_caps_excluded(g, oe)
     assert maxrel(got[1][:, oc6], f["temp3"][:, oc6]) <= 1e-15
     assert maxrel(got[2][:, :, oe6], f["velo3"][:, :, oe6]) <= 1e-15
+
+
+def test_mode_split_eta_and_solar_flux_equal_oracle():
+    """mode_split takes eta from the free-surface field (the analytic profiles
+    depend on z - eta) and a penetrative solar flux Q_sr != 0 heats the
+    interior layers through the irradiance profile: against the oracle with
+    the same eta within 1e-14 (exp on the device and the host)"""
+    print("vertical diffusion, mode_split eta and solar flux, J5 L8", flush=True)
+    from oracle.oracle import Oracle, geometry_of
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, NC = 8, g.layout.ncells
+    draws, forcing, coef = _synthetic(g, L, seed=55)
+    topo, m, t, u = draws[0]
+    eta = 0.4 * np.sin(np.arange(NC) * 0.37)
+    params = dict(tke_closure=0, Q_sr=80.0, c_p=3991.87, bottom_friction=4e-4)
+    ctx = W.Context(g, nlayers=L)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.fs_upload(W.FS_ETA, eta)
+    ctx.set_vdiff(mode_split=1, **params)
+    ctx.vdiff_set_coefficients(*coef)
+    ctx.vdiff_set_forcing(forcing["bot"], forcing["top"], forcing["tau"], forcing["wind"])
+    ctx.vertical_diffusion(0, 1800.0, 1)
+    got = ctx.download_state(0) + ctx.vdiff_coefficients()
+    ctx.close()
+    o = Oracle(g.layout, geometry_of(g), topo)
+    st = (m.copy(), t.copy(), u.copy())
+    o.ghost_fill(*st)
+    res = o.vertical_diffusion(*st, 1800.0, 1, forcing, *coef, eta=eta, **params)
+    free = o.vertical_diffusion(*st, 1800.0, 1, forcing, *coef, **params)  # eta from the column instead
+    oc, oe = g.layout.owned_cells(), g.layout.owned_cells(with_poles=False)
+    err = [maxrel(got[1][:, oc], res[1][:, oc]), maxrel(got[2][:, :, oe], res[2][:, :, oe]),
+           maxrel(got[3][:, oc], res[3][:, oc]), maxrel(got[4][:, oc], res[4][:, oc])]
+    assert max(err) <= 1e-14, err
+    assert maxrel(res[3][:, oc], free[3][:, oc]) > 1e-6  # eta matters
+    nosun = o.vertical_diffusion(*st, 1800.0, 1, forcing, *coef, eta=eta, **dict(params, Q_sr=0.0))
+    assert maxrel(res[1][:, oc], nosun[1][:, oc]) > 1e-9  # and so does the solar flux
