@@ -8,6 +8,11 @@
 !                                   orders 2, 2, 2, viscosities 4e16, 2e16, 6e16, 4e16)
 !   binding_check ml   <in> <out>   a hierarchy: wtg_ml_rk4_split steps with
 !                                   the FMG free surface (wtg_mg_*)
+!   binding_check vdr  <in> <out>   one level: wtg_vertical_diffusion (TKE
+!                                   closure, Q_sr 20, bottom friction 4e-4,
+!                                   uniform forcing and coefficients, istep 1,
+!                                   time step dt) then wtg_remap (PPR) onto
+!                                   b_vert(k) = 1 - (k/nlayers)^2
 !
 ! <in>: int32 level_start, level_end, min_level, domain_level, nlayers,
 ! nsteps; float64 dt, theta1, theta2; then per level coarse to fine: topo,
@@ -26,6 +31,9 @@ program binding_check
   type(wtg_fs_params) :: fsp
   type(wtg_mg_params) :: mgp
   type(wtg_diffusion) :: dif
+  type(wtg_vdiff) :: vdp
+  real(c_double), allocatable :: fb(:), ftp(:), fta(:), fw(:), kt(:), kv(:), tk(:), av(:), bv(:)
+  integer :: k
   type(c_ptr), target :: grids(16), ctxs(16)
   type(c_ptr) :: ml, mg
   type arr
@@ -81,6 +89,25 @@ program binding_check
         call wtg_check(wtg_set_diffusion(ctxs(1), dif), ctxs(1))
      end if
      call wtg_check(wtg_rk4(ctxs(1), 0, par(1), nsteps), ctxs(1))
+  else if (trim(mode) == 'vdr') then
+     call wtg_vdiff_defaults(vdp)
+     vdp%tke_closure = 1
+     vdp%Q_sr = 20d0
+     vdp%bottom_friction = 4d-4
+     call wtg_check(wtg_set_vdiff(ctxs(1), vdp), ctxs(1))
+     call wtg_check(wtg_grid_info_get(grids(1), info))
+     nc = info%ncells
+     allocate (fb(nc), ftp(nc), fta(nc), fw(3 * nc), kt((nlay + 1) * nc), kv((nlay + 1) * nc), tk(nlay * nc))
+     fb = 1d-7; ftp = 2d-7; fta = 0.1d0; fw = 1d-4; kt = 1.2d-5; kv = 1.2d-4; tk = 2d-5
+     call wtg_check(wtg_vdiff_set_coefficients(ctxs(1), kt, kv, tk), ctxs(1))
+     call wtg_check(wtg_vdiff_set_forcing(ctxs(1), fb, ftp, fta, fw), ctxs(1))
+     call wtg_check(wtg_vertical_diffusion(ctxs(1), 0, par(1), 1), ctxs(1))
+     allocate (av(0:nlay), bv(0:nlay))
+     do k = 0, nlay
+        bv(k) = 1d0 - (dble(k) / dble(nlay)) * (dble(k) / dble(nlay))
+     end do
+     av = 1d0 - bv
+     call wtg_check(wtg_remap(ctxs(1), 0, WTG_REMAP_PPR, av, bv), ctxs(1))
   else if (trim(mode) == 'ml') then
      do l = 1, nl
         call wtg_check(wtg_fs_set_params(ctxs(l), fsp), ctxs(l))

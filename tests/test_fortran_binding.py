@@ -143,3 +143,33 @@ def test_ml_split_through_fortran_equals_ctypes(tmp_path):
     finally:
         ml.close()
         h.close()
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_vertical_diffusion_and_remap_through_fortran_equals_ctypes(tmp_path):
+    """wtg_vdiff filled and changed through the Fortran derived type (a field
+    out of place in the bind(C) type would change the result)"""
+    print("Fortran binding: wtg_vertical_diffusion + wtg_remap (PPR) J5 L6", flush=True)
+    g = W.Grid(level=5, min_level=5, domain_level=2)
+    L, NC = 6, g.layout.ncells
+    topo, m, t, u = cases.ocean(g, L, seed=4)
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write(inp, [5], 5, 2, L, 1, [900.0, 1.0, 1.0], [(topo, m, t, u, np.zeros(NC))])
+    _run("vdr", inp, out)
+    res, _ = _read(out, [g], L)
+    fm, ft, fu, _ = res[0]
+    ctx = W.Context(g, nlayers=L, nslots=2)
+    ctx.set_topography(topo)
+    ctx.upload_state(0, m, t, u)
+    ctx.set_vdiff(tke_closure=1, Q_sr=20.0, bottom_friction=4e-4)
+    ctx.vdiff_set_coefficients(np.full((L + 1, NC), 1.2e-5), np.full((L + 1, NC), 1.2e-4), np.full((L, NC), 2e-5))
+    ctx.vdiff_set_forcing(np.full(NC, 1e-7), np.full(NC, 2e-7), np.full(NC, 0.1), np.full((3, NC), 1e-4))
+    ctx.vertical_diffusion(0, 900.0, 1)
+    k = np.arange(L + 1) / L
+    b = 1.0 - k * k
+    ctx.remap(0, W.REMAP_PPR, 1.0 - b, b)
+    cm, ct, cu = ctx.download_state(0)
+    ctx.close()
+    assert np.array_equal(fm, cm) and np.array_equal(ft, ct) and np.array_equal(fu, cu)
+    assert np.max(np.abs(cm - m)) > 0  # the remap moved the layers
