@@ -73,15 +73,19 @@ __device__ void remap1(int N, double* vn, const double* zn, const double* vo, co
     aR[k - 1] = vo[k - 1] + cff * Hz[k - 1];
     aL[k - 1] = vo[k - 1] - cff * Hz[k - 1];
   }
+  // each sweep's interface values FC(k) come from the previous sweep's aR, aL;
+  // one pass computes FC(k) just before cell k overwrites aR(k), with FC(k-1)
+  // in a register (the same numbers as the reference's two loops)
   for (int iter = 1; iter <= 10; ++iter) {
-    for (int k = 1; k <= N - 1; ++k) FC[k] = 0.5 * (aR[k - 1] + aL[k]);
-    FC[N] = aR[N - 1];
-    FC[0] = aL[0];
+    double fprev = aL[0];  // FC(0)
     for (int k = 1; k <= N; ++k) {
-      const double dR = FC[k] - vo[k - 1], dL = vo[k - 1] - FC[k - 1];
+      const double fk = k < N ? 0.5 * (aR[k - 1] + aL[k]) : aR[N - 1];  // FC(k)
+      const double v = vo[k - 1];
+      const double dR = fk - v, dL = v - fprev;
       const double cff = dR * dL < 0.0 ? 0.0 : (fabs(dR) > fabs(dL) ? dL : dR);
-      aR[k - 1] = vo[k - 1] + cff;
-      aL[k - 1] = vo[k - 1] - cff;
+      aR[k - 1] = v + cff;
+      aL[k - 1] = v - cff;
+      fprev = fk;
     }
   }
   for (int k = 1; k <= N - 1; ++k) {
