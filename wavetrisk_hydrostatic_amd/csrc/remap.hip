@@ -527,12 +527,18 @@ __device__ void remap_ppr(int N, double* vn, const double* zn, const double* vo,
   } else if (N >= 2) {
     end_fit(true, 2, uni, vo, dmin, w);
     end_fit(false, np - 2, uni, vo, dmin, w);
+    // the four cells' spacings and values slide along in registers
+    double h0 = PDX(1), h1 = PDX(2), h2 = PDX(3), f0 = PF(1), f1 = PF(2), f2 = PF(3);
     for (int i = 3; i <= np - 2; ++i) {
+      const double h3 = PDX(i + 1), f3 = PF(i + 1);
+      const double fw[4] = {f0, f1, f2, f3};
+      h0 = h1; h1 = h2; h2 = h3;
+      f0 = f1; f1 = f2; f2 = f3;
       if (uni) {
-        PE(i) = (-(1.0 * PF(i - 2)) + 7.0 * PF(i - 1) + 7.0 * PF(i) - 1.0 * PF(i + 1)) / 12.0;
+        PE(i) = (-(1.0 * fw[0]) + 7.0 * fw[1] + 7.0 * fw[2] - 1.0 * fw[3]) / 12.0;
         continue;
       }
-      const double h[4] = {PDX(i - 2), PDX(i - 1), PDX(i), PDX(i + 1)};
+      const double h[4] = {PDX(i - 2), h0, h1, h2};  // h0..h2 now hold PDX(i - 1 .. i + 1)
       const double xhat = 0.5 * fmax(h[1], dmin) + 0.5 * fmax(h[2], dmin);
       const double s[5] = {-(h[0] + h[1]) / xhat, -h[1] / xhat, 0.0, h[2] / xhat, (h[2] + h[3]) / xhat};
       double iv[5][4];
@@ -548,20 +554,28 @@ __device__ void remap_ppr(int N, double* vn, const double* zn, const double* vo,
       for (int r = 0; r < 4; ++r) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) A[r][p] = iv[r + 1][p] - iv[r][p];
-        v[r] = h[r] * PF(i - 2 + r) / xhat;
+        v[r] = h[r] * fw[r] / xhat;
       }
-      PE(i) = solve4(A, v, 1.0e-14 * dmin) ? v[0] : PF(i - 1) * 0.5 + PF(i) * 0.5;
+      PE(i) = solve4(A, v, 1.0e-14 * dmin) ? v[0] : fw[1] * 0.5 + fw[2] * 0.5;
     }
   }
   // monotone parabolas (coefficients of 1, s, s^2 on s in [-1, 1]) in aL, aR, CF
+  // the neighbouring values, spacings and edges slide along in registers
+  // (the end cells are their own outer neighbours)
+  double fprev = PF(1), fcur = PF(1), hprev = PDX(1), hcur = hprev, ecur = PE(1);
   for (int i = 1; i <= N; ++i) {
     if (N == 1) {
       w.aL[1] = PF(1); w.aR[1] = 0.0; w.CF[1] = 0.0;
       break;
     }
-    const int il = i - 1 > 1 ? i - 1 : 1, ir = i + 1 < N ? i + 1 : N;
-    const double f0 = PF(i), fl = PF(il), fr = PF(ir);
-    double el = PE(i), er = PE(i + 1);
+    const double fnext = i < N ? PF(i + 1) : fcur, enext = PE(i + 1);
+    const double hnext = uni ? hcur : (i < N ? PDX(i + 1) : hcur);
+    const double f0 = fcur, fl = fprev, fr = fnext;
+    const double hl_ = hprev, h0_ = hcur, hr_ = hnext;
+    double el = ecur, er = enext;
+    fprev = fcur; fcur = fnext;
+    hprev = hcur; hcur = hnext;
+    ecur = enext;
     // plsv / plsc: the limited centred slope
     double d0 = 0.0;
     {
@@ -572,7 +586,7 @@ __device__ void remap_ppr(int N, double* vn, const double* zn, const double* vo,
           sl = (fl + f0) * 0.5;
           sr = (f0 + fr) * 0.5;
         } else {
-          const double hl = PDX(il), h0 = PDX(i), hr = PDX(ir);
+          const double hl = hl_, h0 = h0_, hr = hr_;
           sl = (h0 * fl + hl * f0) / (hl + h0);
           sr = (hr * f0 + h0 * fr) / (h0 + hr);
         }
