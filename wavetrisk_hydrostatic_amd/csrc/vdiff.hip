@@ -186,28 +186,32 @@ __device__ __forceinline__ double free_surface(const VdArgs& a, const double* m,
 }
 
 struct TkeWork {
-  double dz[VMAX + 2], dU2[VMAX + 2], Nsq[VMAX + 2], e[VMAX + 2], leps[VMAX + 2], lk[VMAX + 2];
-  double dn[VMAX + 2], up[VMAX + 2];
+  double dz[VMAX + 2], dU2[VMAX + 2], Nsq[VMAX + 2], e[VMAX + 2], leps[VMAX + 2], lk[VMAX + 2], dn[VMAX + 2];
 };
 
-// l_scales (vert_diffusion.f90:502-536) on [0..L]
+// l_scales (vert_diffusion.f90:502-536) on [0..L]: the upward sweep of the
+// mixing length from the bottom, then the downward one from the surface with
+// the two length scales taken as it passes; the first-order length
+// sqrt(2 e / max(N^2, Neps_sq)) is formed where each sweep needs it, and the
+// running minima ride in registers (the same numbers as the reference's
+// array passes)
 __device__ void l_scales(const VdArgs& a, double tau, TkeWork& w) {
   const wtg_vdiff& p = a.p;
   const int L = a.L;
-  for (int l = 0; l <= L; ++l) {
-    w.dn[l] = sqrt(2.0 * w.e[l] / fmax(w.Nsq[l], p.Neps_sq));
-    w.up[l] = w.dn[l];
-  }
-  // the running minima carry the previous interface's value in a register
   double prev = p.l_0;
   w.dn[0] = prev;
-  for (int l = 1; l <= L; ++l) w.dn[l] = prev = fmin(w.dn[l], prev + w.dz[l]);
-  prev = fmax(p.l_0, p.kappa_VK * p.C_l / (a.rho0 * a.grav) * tau);
-  w.up[L] = prev;
-  for (int l = L - 1; l >= 0; --l) w.up[l] = prev = fmin(w.up[l], prev + w.dz[l + 1]);
-  for (int l = 0; l <= L; ++l) {
-    w.leps[l] = fmax(p.l_min, sqrt(w.up[l] * w.dn[l]));
-    w.lk[l] = fmax(p.l_min, fmin(w.up[l], w.dn[l]));
+  for (int l = 1; l <= L; ++l) {
+    const double first = sqrt(2.0 * w.e[l] / fmax(w.Nsq[l], p.Neps_sq));
+    w.dn[l] = prev = fmin(first, prev + w.dz[l]);
+  }
+  prev = fmax(p.l_0, p.kappa_VK * p.C_l / (a.rho0 * a.grav) * tau);  // l_up at the surface
+  w.leps[L] = fmax(p.l_min, sqrt(prev * w.dn[L]));
+  w.lk[L] = fmax(p.l_min, fmin(prev, w.dn[L]));
+  for (int l = L - 1; l >= 0; --l) {
+    const double first = sqrt(2.0 * w.e[l] / fmax(w.Nsq[l], p.Neps_sq));
+    prev = fmin(first, prev + w.dz[l + 1]);
+    w.leps[l] = fmax(p.l_min, sqrt(prev * w.dn[l]));
+    w.lk[l] = fmax(p.l_min, fmin(prev, w.dn[l]));
   }
 }
 
